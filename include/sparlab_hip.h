@@ -1,0 +1,211 @@
+/* sparlab_hip.h - C ABI of the MI355X (gfx950) SIMP linear-solve path.
+ *
+ * One opaque context owns one GPU stream and the device-resident state of one
+ * structured SIMP problem: the cached K_ff pattern and values, the SA-AMG
+ * hierarchy, the load cases and their last solutions, the density filter and
+ * the design vectors. Every entry point takes plain pointers and sizes; host
+ * arrays are caller-owned and read or written before the call returns. Each
+ * call returns a status code; the message of the last failure on a context is
+ * available from slx_last_error(). Status codes map one to one onto the
+ * reference's exception classes (include/sparlab/core/Exceptions.hpp:15-64):
+ * SLX_CONFIG_ERROR -> ConfigError, SLX_SOLVER_ERROR -> SolverError,
+ * SLX_CONVERGENCE_ERROR -> ConvergenceError, SLX_MODEL_ERROR -> ModelError.
+ *
+ * The entry points replace these seams of the reference (paths relative to the
+ * SparLab source tree):
+ *   slx_model_*          FemModel + DofManager + Assembler construction
+ *                        (src/fem/FemModel.cpp, src/fem/DofManager.cpp:58-71,
+ *                         src/fem/Assembler.cpp:41-58,85-172)
+ *   slx_assemble         Assembler::assemble_stiffness + reduce_free_free
+ *                        (src/fem/Assembler.cpp:182-258,410-438)
+ *   slx_amg_setup        LinearSolver::factorize of AmgCgSolver
+ *                        (src/fem/LinearSolver.cpp:229-239 -> src/fem/Multigrid.cpp:929-1111)
+ *   slx_factorize_csr    the same seam for a caller-assembled CSR matrix and DofLayout
+ *                        (include/sparlab/fem/LinearSolver.hpp:71,87)
+ *   slx_solve            LinearSolver::solve_from (src/fem/LinearSolver.cpp:246-281 ->
+ *                        src/fem/Multigrid.cpp:1152-1237)
+ *   slx_solve_case       StaticAnalysis::solve_load_vector with the warm start of
+ *                        ComplianceObjective (src/fem/StaticAnalysis.cpp:114-143,
+ *                        src/topopt/Sensitivity.cpp:169-176)
+ *   slx_filter_*         DensityFilter build / to_physical / pull_back
+ *                        (src/topopt/DensityFilter.cpp:115-195)
+ *   slx_evaluate         ComplianceObjective::evaluate (src/topopt/Sensitivity.cpp:110-237)
+ *   slx_oc_update        optimality_criteria_update (src/topopt/OptimalityCriteria.cpp:40-149)
+ *   slx_run_oc           TopologyOptimizer::run_oc (src/topopt/TopologyOptimizer.cpp:556-724)
+ */
+#ifndef SPARLAB_HIP_H
+#define SPARLAB_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct slx_context slx_context;
+
+typedef enum {
+  SLX_OK = 0,
+  SLX_CONFIG_ERROR = 1,
+  SLX_SOLVER_ERROR = 2,
+  SLX_CONVERGENCE_ERROR = 3,
+  SLX_MODEL_ERROR = 4,
+  SLX_DEVICE_ERROR = 5
+} slx_status;
+
+/* Multigrid options (include/sparlab/fem/Multigrid.hpp:75-103), plus the
+ * precision of the hierarchy: 0 = FP64 everywhere (the drop-in path),
+ * 1 = FP32 hierarchy and V-cycle under an FP64 outer CG. */
+typedef struct {
+  double strength_threshold;     /* 0.02 */
+  int max_levels;                /* 12 */
+  int coarse_size;               /* 1500 */
+  int smoother;                  /* 0 Chebyshev (only this one runs on the GPU) */
+  int smoother_degree;           /* 3 */
+  double chebyshev_ratio;        /* 30 */
+  double prolongator_damping;    /* 4/3 */
+  int lanczos_steps;             /* 12 */
+  double coarse_pivot_tolerance; /* 1e-13 */
+  int reuse_aggregates;          /* 1 */
+  int precision;                 /* 0 */
+} slx_amg_options;
+
+typedef struct {
+  int iterations;
+  double relative_residual; /* true ||b - A x|| / ||b|| at exit */
+  int converged;
+} slx_pcg_result;
+
+typedef struct {
+  int levels;
+  int reused_aggregates;
+  double operator_complexity;
+  double grid_complexity;
+  double coarse_pivot_ratio;
+  int near_null_space_dimension;
+  long long unknowns[16];
+  long long nonzeros[16];
+  long long aggregates[16];
+  double lambda_max[16];
+  double setup_seconds;
+} slx_amg_stats;
+
+/* Evaluation options: SIMP law (include/sparlab/topopt/SimpInterpolation.hpp:71-80)
+ * and the linear-solver settings ComplianceObjective uses. */
+typedef struct {
+  double penalty;         /* 3 */
+  double emin_ratio;      /* 1e-9 */
+  double tolerance;       /* relative residual of PCG */
+  int max_iterations;     /* 0 -> 1000 */
+  int warm_start;         /* 1 */
+} slx_eval_options;
+
+typedef struct {
+  double compliance;
+  double volume;
+  double volume_fraction;
+  int linear_iterations;
+  double assemble_seconds;
+  double solve_seconds;
+} slx_evaluation;
+
+/* include/sparlab/topopt/OptimalityCriteria.hpp:35-52 */
+typedef struct {
+  double move_limit;       /* 0.2 */
+  double damping;          /* 0.5 */
+  double lambda_lower;     /* 1e-12 */
+  double lambda_upper;     /* 1e12 */
+  int max_bisections;      /* 200 */
+  double volume_tolerance; /* 1e-10 */
+} slx_oc_options;
+
+typedef struct {
+  double lambda;
+  double achieved_volume;
+  double max_change;
+  int bisections;
+  int volume_converged;
+} slx_oc_step;
+
+/* include/sparlab/topopt/TopologyOptimizer.hpp:81-121 (OC subset) */
+typedef struct {
+  int max_iterations;
+  double change_tolerance;
+  double objective_tolerance;
+  int objective_window;
+  int continuation_steps;
+  double penalty_start;
+  int continuation_iterations;
+  slx_oc_options oc;
+  slx_eval_options eval;
+} slx_run_options;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+slx_status slx_create(int device, slx_context** out);
+void slx_destroy(slx_context* ctx);
+const char* slx_last_error(const slx_context* ctx);
+void slx_amg_default_options(slx_amg_options* o);
+void slx_eval_default_options(slx_eval_options* o);
+void slx_oc_default_options(slx_oc_options* o);
+
+/* ---- model: structured Hex8 (dim 3) or Q4 plane-stress (dim 2) grid -------
+ * Node / element numbering of src/mesh/StructuredMesh.hpp:6-27. */
+slx_status slx_model_structured(slx_context* ctx, int dim, int nx, int ny, int nz, double lx, double ly,
+                                double lz, double E, double nu, double thickness);
+/* fixed[ndofs]: 1 for a prescribed (zero) DOF, DOF = node * dim + component. */
+slx_status slx_model_set_supports(slx_context* ctx, const unsigned char* fixed);
+/* loads[ncases * ndofs] full-length load vectors; weights are normalised to sum 1. */
+slx_status slx_model_set_loads(slx_context* ctx, int ncases, const double* loads, const double* weights);
+/* Builds Ke0, the DOF map and the K_ff pattern on the device. */
+slx_status slx_model_finalize(slx_context* ctx);
+/* out[0..5] = nnodes, nelem, ndofs, nfree, kff_nnz, ncases */
+slx_status slx_model_sizes(const slx_context* ctx, long long* out);
+slx_status slx_ke0_get(const slx_context* ctx, double* ke);
+slx_status slx_free_dofs_get(const slx_context* ctx, int* free_dofs);
+
+/* ---- assembly: K_ff = reduce(sum_e scale_e Ke0) on the device -------------- */
+slx_status slx_assemble(slx_context* ctx, const double* scale /* nelem, host, or NULL for ones */);
+slx_status slx_kff_get(const slx_context* ctx, int* ptr, int* col, double* val);
+
+/* ---- linear solver ---------------------------------------------------------- */
+slx_status slx_amg_setup(slx_context* ctx, const slx_amg_options* opt);
+slx_status slx_factorize_csr(slx_context* ctx, int n, const int* ptr, const int* col, const double* val, int dim,
+                             int ndpn, int nnodes, const double* coords, const int* unknowns,
+                             const slx_amg_options* opt);
+slx_status slx_solve(slx_context* ctx, const double* b, double* x, double tol, int max_iterations,
+                     slx_pcg_result* res);
+slx_status slx_solve_case(slx_context* ctx, int load_case, int warm_start, double tol, int max_iterations,
+                          slx_pcg_result* res);
+slx_status slx_amg_apply(slx_context* ctx, const double* r, double* z);
+slx_status slx_amg_stats_get(const slx_context* ctx, slx_amg_stats* out);
+/* Test access to the hierarchy. which: 0 A_l, 1 P, 2 P-hat, 3 R. info = rows, cols, nnz. */
+slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long long* info);
+slx_status slx_amg_level_get(const slx_context* ctx, int level, int which, int* ptr, int* col, double* val);
+slx_status slx_amg_aggregates_get(const slx_context* ctx, int level, int* aggregates);
+/* u[nfree] of the last solve of a load case */
+slx_status slx_solution_get(const slx_context* ctx, int load_case, double* u);
+
+/* ---- topology optimisation ------------------------------------------------- */
+slx_status slx_design_set(slx_context* ctx, double volume_fraction, const double* lower, const double* upper,
+                          const unsigned char* passive);
+slx_status slx_filter_build(slx_context* ctx, double radius);
+slx_status slx_filter_apply(slx_context* ctx, int adjoint, const double* in, double* out);
+slx_status slx_evaluate(slx_context* ctx, const double* x, const slx_eval_options* opt, slx_evaluation* out,
+                        double* dc_dx, double* dv_dx, double* energy);
+slx_status slx_oc_update(slx_context* ctx, const double* x, const double* dc_dx, const double* dv_dx,
+                         const slx_oc_options* opt, double* x_new, slx_oc_step* step);
+/* history[max_iterations * 8]: iteration, penalty, compliance, volume fraction,
+ * max change, lambda, linear iterations, grey level. */
+slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history, int* performed,
+                      double* final_compliance);
+
+/* ---- measurement ------------------------------------------------------------
+ * With timing on, every launch of the named kernels is bracketed by HIP events
+ * on the context's stream and the elapsed times are summed. Names: "spmv"
+ * (finest-level operator products of PCG and the V-cycle), "assemble". */
+slx_status slx_timing_enable(slx_context* ctx, int on);
+slx_status slx_timing_get(slx_context* ctx, const char* kernel, double* total_ms, long long* launches);
+slx_status slx_synchronize(slx_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPARLAB_HIP_H */

@@ -1,0 +1,758 @@
+// SA-AMG hierarchy of the sparlab_hip library (src/fem/Multigrid.cpp:383-1111).
+//
+// Split between host and device:
+//  * once per sparsity structure, on the host: node blocks and the rigid-body
+//    near-null space, strength of connection and the three greedy aggregation
+//    passes (serial by definition: the aggregates must equal the reference's bit
+//    for bit), the tentative prolongators (one small column-pivoted QR per
+//    aggregate) and the symbolic patterns of P, R = P^T, A P, R (A P) and the
+//    symmetrised coarse operator;
+//  * at every setup, on the device: diagonal, Lanczos estimate of
+//    lambda_max(D^-1 A), prolongator smoothing, the two Galerkin products and
+//    the symmetrisation, level by level; the coarsest operator (at most a few
+//    thousand unknowns) comes back to the host once for its factorisation and
+//    returns as a dense inverse, which the V-cycle applies as one GEMV.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <sstream>
+
+#include "slx_kernels.hpp"
+#include "slx_solver.hpp"
+
+namespace slx {
+
+void DCsr::upload_pattern(const HCsr& h, hipStream_t s) {
+  rows = h.rows;
+  cols = h.cols;
+  nnz = h.nnz();
+  max_row = 0;
+  for (int i = 0; i < h.rows; ++i) max_row = std::max(max_row, h.ptr[(size_t)i + 1] - h.ptr[(size_t)i]);
+  ptr.upload(h.ptr, s);
+  col.upload(h.col, s);
+  val.alloc((size_t)nnz);
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// ---------------------------------------------------------------------------
+// Symbolic sparse algebra (patterns only; rows sorted by column)
+// ---------------------------------------------------------------------------
+HCsr pattern_product(const HCsr& a, const HCsr& b) {
+  HCsr c;
+  c.rows = a.rows;
+  c.cols = b.cols;
+  c.ptr.assign((size_t)a.rows + 1, 0);
+  const int nt = omp_get_max_threads();
+  std::vector<std::vector<int>> rows_cols((size_t)a.rows);
+#pragma omp parallel num_threads(nt)
+  {
+    std::vector<int> marker((size_t)b.cols, -1), cols;
+#pragma omp for schedule(dynamic, 512)
+    for (int i = 0; i < a.rows; ++i) {
+      cols.clear();
+      for (int k = a.ptr[(size_t)i]; k < a.ptr[(size_t)i + 1]; ++k) {
+        const int j = a.col[(size_t)k];
+        for (int kk = b.ptr[(size_t)j]; kk < b.ptr[(size_t)j + 1]; ++kk) {
+          const int cc = b.col[(size_t)kk];
+          if (marker[(size_t)cc] != i) {
+            marker[(size_t)cc] = i;
+            cols.push_back(cc);
+          }
+        }
+      }
+      std::sort(cols.begin(), cols.end());
+      rows_cols[(size_t)i] = cols;
+      c.ptr[(size_t)i + 1] = (int)cols.size();
+    }
+  }
+  for (int i = 0; i < a.rows; ++i) {
+    const long long next = (long long)c.ptr[(size_t)i + 1] + c.ptr[(size_t)i];
+    if (next > std::numeric_limits<int>::max()) fail(SLX_SOLVER_ERROR, "multigrid: a Galerkin product exceeds 2^31 entries");
+    c.ptr[(size_t)i + 1] = (int)next;
+  }
+  c.col.resize((size_t)c.nnz());
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < a.rows; ++i) {
+    std::copy(rows_cols[(size_t)i].begin(), rows_cols[(size_t)i].end(), c.col.begin() + c.ptr[(size_t)i]);
+    std::vector<int>().swap(rows_cols[(size_t)i]);
+  }
+  return c;
+}
+
+// R = P^T with src[k] = index in P of R's entry k; columns of R ascend.
+void transpose_pattern(const HCsr& p, HCsr& r, std::vector<int>& src) {
+  r.rows = p.cols;
+  r.cols = p.rows;
+  r.ptr.assign((size_t)p.cols + 1, 0);
+  for (long long k = 0; k < p.nnz(); ++k) ++r.ptr[(size_t)p.col[(size_t)k] + 1];
+  for (int j = 0; j < p.cols; ++j) r.ptr[(size_t)j + 1] += r.ptr[(size_t)j];
+  r.col.resize((size_t)p.nnz());
+  src.resize((size_t)p.nnz());
+  std::vector<int> next(r.ptr.begin(), r.ptr.end() - 1);
+  for (int i = 0; i < p.rows; ++i)
+    for (int k = p.ptr[(size_t)i]; k < p.ptr[(size_t)i + 1]; ++k) {
+      const int q = next[(size_t)p.col[(size_t)k]]++;
+      r.col[(size_t)q] = i;
+      src[(size_t)q] = k;
+    }
+}
+
+// S = (C + C^T)/2 pattern; ia / ib: index in C of (i, j) / (j, i) or -1.
+void symmetrize_pattern(const HCsr& c, HCsr& s, std::vector<int>& ia, std::vector<int>& ib) {
+  HCsr t;
+  std::vector<int> tsrc;
+  transpose_pattern(c, t, tsrc);
+  s.rows = c.rows;
+  s.cols = c.cols;
+  s.ptr.assign((size_t)c.rows + 1, 0);
+  s.col.clear();
+  ia.clear();
+  ib.clear();
+  const int big = std::numeric_limits<int>::max();
+  for (int i = 0; i < c.rows; ++i) {
+    int p = c.ptr[(size_t)i], q = t.ptr[(size_t)i];
+    const int pe = c.ptr[(size_t)i + 1], qe = t.ptr[(size_t)i + 1];
+    while (p < pe || q < qe) {
+      const int cp = p < pe ? c.col[(size_t)p] : big;
+      const int cq = q < qe ? t.col[(size_t)q] : big;
+      if (cp == cq) {
+        s.col.push_back(cp);
+        ia.push_back(p++);
+        ib.push_back(tsrc[(size_t)q++]);
+      } else if (cp < cq) {
+        s.col.push_back(cp);
+        ia.push_back(p++);
+        ib.push_back(-1);
+      } else {
+        s.col.push_back(cq);
+        ia.push_back(-1);
+        ib.push_back(tsrc[(size_t)q++]);
+      }
+    }
+    s.ptr[(size_t)i + 1] = (int)s.col.size();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Aggregation (Multigrid.cpp:457-565): block Frobenius strength, theta test,
+// greedy passes in node order.
+// ---------------------------------------------------------------------------
+void aggregate_nodes(const HCsr& a, const std::vector<int>& node_ptr, double theta, std::vector<int>& agg) {
+  const int nn = (int)node_ptr.size() - 1;
+  std::vector<int> node_of((size_t)a.rows);
+  for (int nd = 0; nd < nn; ++nd)
+    for (int i = node_ptr[(size_t)nd]; i < node_ptr[(size_t)nd + 1]; ++i) node_of[(size_t)i] = nd;
+  std::vector<double> bdiag((size_t)nn);
+#pragma omp parallel for schedule(static)
+  for (int nd = 0; nd < nn; ++nd) {
+    double s = 0.0;
+    for (int i = node_ptr[(size_t)nd]; i < node_ptr[(size_t)nd + 1]; ++i)
+      for (int k = a.ptr[(size_t)i]; k < a.ptr[(size_t)i + 1]; ++k)
+        if (node_of[(size_t)a.col[(size_t)k]] == nd) s += a.val[(size_t)k] * a.val[(size_t)k];
+    bdiag[(size_t)nd] = std::sqrt(s);
+  }
+  // strong neighbours per node, ascending, with their relative strength
+  std::vector<std::vector<std::pair<int, double>>> strong((size_t)nn);
+#pragma omp parallel
+  {
+    std::vector<int> marker((size_t)nn, -1), touched;
+    std::vector<double> acc((size_t)nn, 0.0);
+#pragma omp for schedule(dynamic, 1024)
+    for (int nd = 0; nd < nn; ++nd) {
+      touched.clear();
+      for (int i = node_ptr[(size_t)nd]; i < node_ptr[(size_t)nd + 1]; ++i)
+        for (int k = a.ptr[(size_t)i]; k < a.ptr[(size_t)i + 1]; ++k) {
+          const int o = node_of[(size_t)a.col[(size_t)k]];
+          if (o == nd) continue;
+          if (marker[(size_t)o] != nd) {
+            marker[(size_t)o] = nd;
+            acc[(size_t)o] = 0.0;
+            touched.push_back(o);
+          }
+          acc[(size_t)o] += a.val[(size_t)k] * a.val[(size_t)k];
+        }
+      std::sort(touched.begin(), touched.end());
+      const double di = bdiag[(size_t)nd];
+      auto& out = strong[(size_t)nd];
+      for (int o : touched) {
+        const double s = std::sqrt(acc[(size_t)o]);
+        const double ref = std::sqrt(di * bdiag[(size_t)o]);
+        if (s > 0.0 && s >= theta * ref) out.emplace_back(o, ref > 0.0 ? s / ref : 0.0);
+      }
+    }
+  }
+  agg.assign((size_t)nn, -1);
+  int count = 0;
+  for (int nd = 0; nd < nn; ++nd) {  // pass 1: whole free neighbourhoods
+    if (agg[(size_t)nd] >= 0 || strong[(size_t)nd].empty()) continue;
+    bool free_nb = true;
+    for (auto& pr : strong[(size_t)nd])
+      if (agg[(size_t)pr.first] >= 0) { free_nb = false; break; }
+    if (!free_nb) continue;
+    agg[(size_t)nd] = count;
+    for (auto& pr : strong[(size_t)nd]) agg[(size_t)pr.first] = count;
+    ++count;
+  }
+  const std::vector<int> first = agg;
+  for (int nd = 0; nd < nn; ++nd) {  // pass 2: join the strongest pass-1 aggregate
+    if (first[(size_t)nd] >= 0) continue;
+    int best = -1;
+    double best_v = -1.0;
+    for (auto& pr : strong[(size_t)nd]) {
+      const int target = first[(size_t)pr.first];
+      if (target >= 0 && pr.second > best_v) { best = target; best_v = pr.second; }
+    }
+    if (best >= 0) agg[(size_t)nd] = best;
+  }
+  for (int nd = 0; nd < nn; ++nd) {  // pass 3: new aggregates of what is left
+    if (agg[(size_t)nd] >= 0) continue;
+    agg[(size_t)nd] = count;
+    for (auto& pr : strong[(size_t)nd])
+      if (agg[(size_t)pr.first] < 0) agg[(size_t)pr.first] = count;
+    ++count;
+  }
+}
+
+// Householder QR with column pivoting (Businger-Golub; LAPACK xGEQP3 norm
+// downdating; reflector beta = -sign(x0)|x|). a: rows x m row-major, in place.
+// Returns the rank at 1e-10 |R00| (Multigrid.cpp:606-617), Q's first rank
+// columns (rows x rank) and R (rank x m, columns in the original order).
+int qr_colpiv(std::vector<double>& a, int rows, int m, std::vector<double>& q, std::vector<double>& r) {
+  const int size = std::min(rows, m);
+  std::vector<int> perm((size_t)m);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::vector<double> tau((size_t)size, 0.0), nu((size_t)m), nd((size_t)m);
+  auto at = [&](int i, int j) -> double& { return a[(size_t)i * m + j]; };
+  auto tail_norm = [&](int j, int from) {
+    double s = 0.0;
+    for (int i = from; i < rows; ++i) s += at(i, j) * at(i, j);
+    return std::sqrt(s);
+  };
+  for (int j = 0; j < m; ++j) nu[(size_t)j] = nd[(size_t)j] = tail_norm(j, 0);
+  const double thr = std::sqrt(std::numeric_limits<double>::epsilon());
+  for (int k = 0; k < size; ++k) {
+    int piv = k;
+    for (int j = k + 1; j < m; ++j)
+      if (nu[(size_t)j] > nu[(size_t)piv]) piv = j;
+    if (piv != k) {
+      for (int i = 0; i < rows; ++i) std::swap(at(i, k), at(i, piv));
+      std::swap(nu[(size_t)k], nu[(size_t)piv]);
+      std::swap(nd[(size_t)k], nd[(size_t)piv]);
+      std::swap(perm[(size_t)k], perm[(size_t)piv]);
+    }
+    const double x0 = at(k, k);
+    double tail = 0.0;
+    for (int i = k + 1; i < rows; ++i) tail += at(i, k) * at(i, k);
+    double beta, t;
+    if (tail <= std::numeric_limits<double>::min()) {
+      t = 0.0;
+      beta = x0;
+      for (int i = k + 1; i < rows; ++i) at(i, k) = 0.0;
+    } else {
+      beta = std::sqrt(x0 * x0 + tail);
+      if (x0 >= 0.0) beta = -beta;
+      const double den = x0 - beta;
+      for (int i = k + 1; i < rows; ++i) at(i, k) /= den;
+      t = (beta - x0) / beta;
+    }
+    tau[(size_t)k] = t;
+    at(k, k) = beta;
+    for (int j = k + 1; j < m; ++j) {
+      double w = at(k, j);
+      for (int i = k + 1; i < rows; ++i) w += at(i, k) * at(i, j);
+      w *= t;
+      at(k, j) -= w;
+      for (int i = k + 1; i < rows; ++i) at(i, j) -= w * at(i, k);
+    }
+    for (int j = k + 1; j < m; ++j) {
+      if (nu[(size_t)j] == 0.0) continue;
+      double g = std::abs(at(k, j)) / nu[(size_t)j];
+      g = (1.0 + g) * (1.0 - g);
+      if (g < 0.0) g = 0.0;
+      const double ratio = nu[(size_t)j] / nd[(size_t)j];
+      if (g * ratio * ratio <= thr) {
+        nd[(size_t)j] = tail_norm(j, k + 1);
+        nu[(size_t)j] = nd[(size_t)j];
+      } else {
+        nu[(size_t)j] *= std::sqrt(g);
+      }
+    }
+  }
+  const double lead = size > 0 ? std::abs(at(0, 0)) : 0.0;
+  int rank = 0;
+  for (int i = 0; i < size; ++i)
+    if (std::abs(at(i, i)) > 1.0e-10 * lead) ++rank;
+  rank = std::max(rank, 1);
+  q.assign((size_t)rows * rank, 0.0);
+  std::vector<double> e((size_t)rows);
+  for (int c = 0; c < rank; ++c) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[(size_t)c] = 1.0;
+    for (int k = std::min(c, size - 1); k >= 0; --k) {
+      double w = e[(size_t)k];
+      for (int i = k + 1; i < rows; ++i) w += at(i, k) * e[(size_t)i];
+      w *= tau[(size_t)k];
+      e[(size_t)k] -= w;
+      for (int i = k + 1; i < rows; ++i) e[(size_t)i] -= w * at(i, k);
+    }
+    for (int i = 0; i < rows; ++i) q[(size_t)i * rank + c] = e[(size_t)i];
+  }
+  r.assign((size_t)rank * m, 0.0);
+  for (int i = 0; i < rank; ++i)
+    for (int j = i; j < m; ++j) r[(size_t)i * m + perm[(size_t)j]] = at(i, j);
+  return rank;
+}
+
+// Tentative prolongator and the next level's node blocks / near-null space
+// (Multigrid.cpp:567-669).
+void tentative(Level& lvl, Level& next, int m, HCsr& pt) {
+  const int n = lvl.n, nn = lvl.num_nodes;
+  const std::vector<int>& agg = lvl.aggregate;
+  const int na = agg.empty() ? 0 : *std::max_element(agg.begin(), agg.end()) + 1;
+  std::vector<int> aptr((size_t)na + 1, 0), anodes((size_t)nn);
+  for (int nd = 0; nd < nn; ++nd) ++aptr[(size_t)agg[(size_t)nd] + 1];
+  for (int a = 0; a < na; ++a) aptr[(size_t)a + 1] += aptr[(size_t)a];
+  {
+    std::vector<int> fill(aptr.begin(), aptr.end() - 1);
+    for (int nd = 0; nd < nn; ++nd) anodes[(size_t)fill[(size_t)agg[(size_t)nd]]++] = nd;
+  }
+  std::vector<std::vector<double>> qs((size_t)na), rs((size_t)na);
+  std::vector<int> rank((size_t)na);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int a = 0; a < na; ++a) {
+    int rows = 0;
+    for (int k = aptr[(size_t)a]; k < aptr[(size_t)a + 1]; ++k) {
+      const int nd = anodes[(size_t)k];
+      rows += lvl.node_ptr[(size_t)nd + 1] - lvl.node_ptr[(size_t)nd];
+    }
+    std::vector<double> block((size_t)rows * m);
+    int row = 0;
+    for (int k = aptr[(size_t)a]; k < aptr[(size_t)a + 1]; ++k) {
+      const int nd = anodes[(size_t)k];
+      for (int i = lvl.node_ptr[(size_t)nd]; i < lvl.node_ptr[(size_t)nd + 1]; ++i, ++row)
+        std::memcpy(&block[(size_t)row * m], &lvl.nullspace[(size_t)i * m], sizeof(double) * m);
+    }
+    rank[(size_t)a] = qr_colpiv(block, rows, m, qs[(size_t)a], rs[(size_t)a]);
+  }
+  next.node_ptr.assign((size_t)na + 1, 0);
+  for (int a = 0; a < na; ++a) next.node_ptr[(size_t)a + 1] = next.node_ptr[(size_t)a] + rank[(size_t)a];
+  next.num_nodes = na;
+  next.n = next.node_ptr.back();
+  next.nullspace.assign((size_t)next.n * m, 0.0);
+  for (int a = 0; a < na; ++a)
+    std::copy(rs[(size_t)a].begin(), rs[(size_t)a].end(), next.nullspace.begin() + (size_t)next.node_ptr[(size_t)a] * m);
+  std::vector<int> agg_of_row((size_t)n), local((size_t)n);
+  for (int a = 0; a < na; ++a) {
+    int loc = 0;
+    for (int k = aptr[(size_t)a]; k < aptr[(size_t)a + 1]; ++k) {
+      const int nd = anodes[(size_t)k];
+      for (int i = lvl.node_ptr[(size_t)nd]; i < lvl.node_ptr[(size_t)nd + 1]; ++i) {
+        agg_of_row[(size_t)i] = a;
+        local[(size_t)i] = loc++;
+      }
+    }
+  }
+  pt.rows = n;
+  pt.cols = next.n;
+  pt.ptr.assign((size_t)n + 1, 0);
+  for (int i = 0; i < n; ++i) pt.ptr[(size_t)i + 1] = pt.ptr[(size_t)i] + rank[(size_t)agg_of_row[(size_t)i]];
+  pt.col.resize((size_t)pt.nnz());
+  pt.val.resize((size_t)pt.nnz());
+  for (int i = 0; i < n; ++i) {
+    const int a = agg_of_row[(size_t)i], ra = rank[(size_t)a];
+    for (int j = 0; j < ra; ++j) {
+      pt.col[(size_t)pt.ptr[(size_t)i] + j] = next.node_ptr[(size_t)a] + j;
+      pt.val[(size_t)pt.ptr[(size_t)i] + j] = qs[(size_t)a][(size_t)local[(size_t)i] * ra + j];
+    }
+  }
+}
+
+double tridiag_max(const std::vector<double>& d, const std::vector<double>& e) {
+  // largest eigenvalue of a symmetric tridiagonal matrix by Sturm bisection
+  const size_t k = d.size();
+  if (k == 1) return d[0];
+  double lo = std::numeric_limits<double>::max(), hi = -lo;
+  for (size_t i = 0; i < k; ++i) {
+    const double rad = (i ? std::abs(e[i - 1]) : 0.0) + (i + 1 < k ? std::abs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - rad);
+    hi = std::max(hi, d[i] + rad);
+  }
+  const double tiny = std::numeric_limits<double>::epsilon() * (std::abs(lo) + std::abs(hi) + 1e-300);
+  auto below = [&](double x) {
+    int c = 0;
+    double piv = d[0] - x;
+    if (piv < 0) ++c;
+    for (size_t i = 1; i < k; ++i) {
+      if (piv == 0.0) piv = tiny;
+      piv = d[i] - x - e[i - 1] * e[i - 1] / piv;
+      if (piv < 0) ++c;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (below(mid) >= (int)k) hi = mid; else lo = mid;
+  }
+  return hi;
+}
+
+// Dense LDL^T of the coarsest operator, its pivot ratio (Multigrid.cpp:766-781)
+// and its inverse, row-major.
+void coarse_inverse(int n, const std::vector<double>& dense, std::vector<double>& inv, double& ratio) {
+  std::vector<double> L(dense);
+  std::vector<double> D((size_t)n);
+  for (int k = 0; k < n; ++k) {
+    double dk = L[(size_t)k * n + k];
+    for (int j = 0; j < k; ++j) dk -= L[(size_t)k * n + j] * L[(size_t)k * n + j] * D[(size_t)j];
+    D[(size_t)k] = dk;
+    if (dk == 0.0) continue;
+#pragma omp parallel for schedule(static) if (n - k > 256)
+    for (int i = k + 1; i < n; ++i) {
+      double s = L[(size_t)i * n + k];
+      for (int j = 0; j < k; ++j) s -= L[(size_t)i * n + j] * L[(size_t)k * n + j] * D[(size_t)j];
+      L[(size_t)i * n + k] = s / dk;
+    }
+  }
+  double mx = 0.0, mn = std::numeric_limits<double>::max();
+  for (double v : D) {
+    mx = std::max(mx, std::abs(v));
+    mn = std::min(mn, v);
+  }
+  ratio = mx > 0.0 ? mn / mx : 0.0;
+  inv.assign((size_t)n * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int c = 0; c < n; ++c) {
+    std::vector<double> y((size_t)n, 0.0);
+    y[(size_t)c] = 1.0;
+    for (int i = c + 1; i < n; ++i) {  // L y = e_c
+      double s = 0.0;
+      for (int j = c; j < i; ++j) s += L[(size_t)i * n + j] * y[(size_t)j];
+      y[(size_t)i] = -s;
+    }
+    for (int i = 0; i < n; ++i) y[(size_t)i] /= D[(size_t)i];
+    for (int i = n - 1; i >= 0; --i) {  // L^T z = y
+      double s = y[(size_t)i];
+      for (int j = i + 1; j < n; ++j) s -= L[(size_t)j * n + i] * y[(size_t)j];
+      y[(size_t)i] = s;
+    }
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = y[(size_t)i];
+  }
+}
+
+void build_finest_nodes(Context& c, Level& lvl) {
+  const int n = lvl.n, dim = c.layout_dim, ndpn = c.layout_ndpn;
+  const std::vector<int>& u = c.layout_unknowns;
+  lvl.node_ptr.assign(1, 0);
+  int prev = -1;
+  for (int i = 0; i < n; ++i) {
+    const int g = u[(size_t)i], node = g / ndpn;
+    if (g < 0 || node >= c.layout_nnodes) fail(SLX_SOLVER_ERROR, "multigrid: an unknown maps outside the model's nodes");
+    if (i > 0 && g <= u[(size_t)i - 1]) fail(SLX_SOLVER_ERROR, "multigrid: the DOF layout must list unknowns in ascending order");
+    if (node != prev) {
+      if (i > 0) lvl.node_ptr.push_back(i);
+      prev = node;
+    }
+  }
+  lvl.node_ptr.push_back(n);
+  lvl.num_nodes = (int)lvl.node_ptr.size() - 1;
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = std::numeric_limits<double>::max();
+    hi[d] = -std::numeric_limits<double>::max();
+  }
+  for (int nd = 0; nd < c.layout_nnodes; ++nd)
+    for (int d = 0; d < 3; ++d) {
+      const double x = d < dim ? c.layout_coords[(size_t)nd * dim + d] : 0.0;
+      lo[d] = std::min(lo[d], x);
+      hi[d] = std::max(hi[d], x);
+    }
+  double ctr[3], scale = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    ctr[d] = 0.5 * (lo[d] + hi[d]);
+    scale = std::max(scale, hi[d] - lo[d]);
+  }
+  scale = std::max(scale, 1.0e-300);
+  const int m = dim == 2 ? 3 : 6;
+  c.nullspace_dim = m;
+  lvl.nullspace.assign((size_t)n * m, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const int g = u[(size_t)i], node = g / ndpn, k = g % ndpn;
+    double r[3];
+    for (int d = 0; d < 3; ++d) r[d] = ((d < dim ? c.layout_coords[(size_t)node * dim + d] : 0.0) - ctr[d]) / scale;
+    double* row = &lvl.nullspace[(size_t)i * m];
+    if (k >= 3) {
+      row[k] = 1.0 / scale;
+      continue;
+    }
+    row[k] = 1.0;
+    if (dim == 2) {
+      row[2] = k == 0 ? -r[1] : r[0];
+    } else {
+      const double rx[3] = {0.0, -r[2], r[1]}, ry[3] = {r[2], 0.0, -r[0]}, rz[3] = {-r[1], r[0], 0.0};
+      row[3] = rx[k];
+      row[4] = ry[k];
+      row[5] = rz[k];
+    }
+  }
+}
+
+DCsr& level_op(Context& c, size_t l) { return l == 0 ? c.K : c.levels[l].A; }
+
+HCsr download_csr(const DCsr& a, hipStream_t s) {
+  HCsr h;
+  h.rows = a.rows;
+  h.cols = a.cols;
+  h.ptr.resize((size_t)a.rows + 1);
+  h.col.resize((size_t)a.nnz);
+  h.val.resize((size_t)a.nnz);
+  a.ptr.download(h.ptr.data(), h.ptr.size(), s);
+  a.col.download(h.col.data(), h.col.size(), s);
+  a.val.download(h.val.data(), h.val.size(), s);
+  SLX_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+int read_flag(Context& c, DBuf<int>& flag) {
+  int v = 0;
+  flag.download(&v, 1, c.stream);
+  SLX_HIP(hipStreamSynchronize(c.stream));
+  return v;
+}
+
+// Lanczos estimate of lambda_max(D^-1 A) on the device (Multigrid.cpp:288-327)
+double lanczos(Context& c, size_t l) {
+  Level& lv = c.levels[l];
+  DCsr& A = level_op(c, l);
+  const int n = lv.n;
+  hipStream_t s = c.stream;
+  if (lv.lz_start.n != (size_t)n) {
+    std::vector<double> v0((size_t)n);
+    for (int i = 0; i < n; ++i) v0[(size_t)i] = 1.0 + 0.5 * std::sin(1.7 * double(i) + 0.3);
+    lv.lz_start.upload(v0, s);
+  }
+  lv.sq.alloc((size_t)n);
+  lv.lz_v.alloc((size_t)n);
+  lv.lz_vprev.alloc((size_t)n);
+  lv.lz_w.alloc((size_t)n);
+  k::vsqrt(n, lv.dinv.p, lv.sq.p, s);
+  SLX_HIP(hipMemcpyAsync(lv.lz_v.p, lv.lz_start.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  SLX_HIP(hipMemsetAsync(lv.lz_vprev.p, 0, sizeof(double) * n, s));
+  double nrm = std::sqrt(device_dot(c, n, lv.lz_v.p, lv.lz_v.p));
+  k::lanczos_shift(n, nrm, lv.lz_v.p, lv.lz_w.p, lv.lz_vprev.p, s);  // w = v / |v| (vprev gets v, reset below)
+  SLX_HIP(hipMemcpyAsync(lv.lz_v.p, lv.lz_w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  SLX_HIP(hipMemsetAsync(lv.lz_vprev.p, 0, sizeof(double) * n, s));
+  std::vector<double> alpha, beta;
+  const int m = std::max(1, std::min(c.amg_opt.lanczos_steps, n));
+  for (int j = 0; j < m; ++j) {
+    k::lanczos_apply(A, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, j > 0 ? beta.back() : 0.0, lv.lz_w.p, s);
+    const double aj = device_dot(c, n, lv.lz_w.p, lv.lz_v.p);
+    alpha.push_back(aj);
+    k::axmy(n, aj, lv.lz_v.p, lv.lz_w.p, s);
+    const double bj = std::sqrt(device_dot(c, n, lv.lz_w.p, lv.lz_w.p));
+    if (j + 1 == m || !(bj > 1.0e-12 * std::abs(aj))) break;
+    beta.push_back(bj);
+    k::lanczos_shift(n, bj, lv.lz_w.p, lv.lz_v.p, lv.lz_vprev.p, s);
+  }
+  std::vector<double> sub(beta.begin(), beta.begin() + (long)(alpha.size() - 1));
+  return tridiag_max(alpha, sub);
+}
+
+void numeric_level(Context& c, size_t l) {
+  Level& lv = c.levels[l];
+  DCsr& A = level_op(c, l);
+  lv.dinv.alloc((size_t)lv.n);
+  DBuf<int> bad;
+  bad.alloc(1);
+  const int init = std::numeric_limits<int>::max();
+  bad.upload(&init, 1, c.stream);
+  k::inv_diag(A, lv.dinv.p, bad.p, c.stream);
+  const int b = read_flag(c, bad);
+  if (b != init) {
+    std::ostringstream os;
+    os << "multigrid level " << l << ": diagonal entry " << b
+       << " is not positive; a stiffness matrix must have a positive diagonal. The model has an unconnected degree "
+          "of freedom or a non-positive stiffness";
+    fail(SLX_SOLVER_ERROR, os.str());
+  }
+  lv.lambda_max = lanczos(c, l);
+  if (!(lv.lambda_max > 0.0) || !std::isfinite(lv.lambda_max)) {
+    std::ostringstream os;
+    os << "multigrid level " << l << ": the spectral estimate of D^-1 A is " << lv.lambda_max
+       << "; the matrix is not positive definite";
+    fail(SLX_SOLVER_ERROR, os.str());
+  }
+}
+
+void check_flag(Context& c, DBuf<int>& flag, const char* what) {
+  const int v = read_flag(c, flag);
+  if (v != 0) fail(SLX_SOLVER_ERROR, std::string("multigrid: internal error in ") + what + " (pattern mismatch)");
+}
+
+}  // namespace
+
+double device_dot(Context& c, long long n, const double* a, const double* b) {
+  k::dot(n, a, b, c.partials.p, c.scal.p + 15, c.stream);
+  double v = 0.0;
+  SLX_HIP(hipMemcpyAsync(&v, c.scal.p + 15, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SLX_HIP(hipStreamSynchronize(c.stream));
+  return v;
+}
+
+void amg_setup(Context& c) {
+  const auto t0 = Clock::now();
+  hipStream_t s = c.stream;
+  const int n = c.K.rows;
+  if (n == 0) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix must be square and non-empty");
+  if (c.layout_dim != 2 && c.layout_dim != 3) fail(SLX_SOLVER_ERROR, "multigrid: the DOF layout must be two- or three-dimensional");
+  if (c.layout_ndpn != c.layout_dim && !(c.layout_dim == 3 && c.layout_ndpn == 6))
+    fail(SLX_SOLVER_ERROR, "multigrid: unsupported DOFs per node (translations only, or 3 + 3 in 3-D)");
+  if ((int)c.layout_unknowns.size() != n) fail(SLX_SOLVER_ERROR, "multigrid: the DOF layout does not match the matrix");
+  const slx_amg_options& o = c.amg_opt;
+  if (o.smoother != 0) fail(SLX_CONFIG_ERROR, "multigrid: only the Chebyshev smoother runs on the GPU");
+  ensure_workspace(c);
+  const bool same = c.amg_ready && c.fp_rows == n && c.fp_nnz == c.K.nnz && c.fp_unknowns == c.layout_unknowns;
+  const bool reuse = o.reuse_aggregates && same;
+  // finiteness of every entry, each call (Multigrid.cpp:992-996)
+  if (!device_all_finite(c, c.K.nnz, c.K.val.p)) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix has a non-finite entry");
+  HCsr host_a;  // values of the level being aggregated (first setup only)
+  if (!reuse) {
+    host_a = download_csr(c.K, s);
+    // symmetry check once per structure (Multigrid.cpp:968-991)
+    double max_abs = 0.0;
+    for (double v : host_a.val) max_abs = std::max(max_abs, std::abs(v));
+    int bad_i = -1, bad_j = -1;
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int i = 0; i < n; ++i)
+      for (int k = host_a.ptr[(size_t)i]; k < host_a.ptr[(size_t)i + 1]; ++k) {
+        const int j = host_a.col[(size_t)k];
+        const int* b0 = host_a.col.data() + host_a.ptr[(size_t)j];
+        const int* b1 = host_a.col.data() + host_a.ptr[(size_t)j + 1];
+        const int* hit = std::lower_bound(b0, b1, i);
+        const double other = (hit != b1 && *hit == i) ? host_a.val[(size_t)(hit - host_a.col.data())] : 0.0;
+        if (std::abs(other - host_a.val[(size_t)k]) > 1.0e-10 * max_abs) {
+#pragma omp critical
+          {
+            bad_i = i;
+            bad_j = j;
+          }
+        }
+      }
+    if (bad_i >= 0) {
+      std::ostringstream os;
+      os << "multigrid: the system matrix is not symmetric (entry (" << bad_i << ", " << bad_j
+         << ")); conjugate gradients needs a symmetric positive-definite matrix";
+      fail(SLX_SOLVER_ERROR, os.str());
+    }
+    c.levels.clear();
+    c.levels.reserve((size_t)o.max_levels + 1);
+    c.levels.emplace_back();
+    c.levels[0].n = n;
+    build_finest_nodes(c, c.levels[0]);
+    c.amg_ready = false;
+  }
+  DBuf<int> flag;
+  flag.alloc(1);
+  const int zero = 0;
+  size_t l = 0;
+  const size_t reused_levels = c.levels.size();
+  while (true) {
+    numeric_level(c, l);
+    Level& lv = c.levels[l];
+    const bool stop = reuse ? (l + 1 == reused_levels)
+                            : (lv.n <= o.coarse_size || (int)l + 1 >= o.max_levels);
+    if (stop) break;
+    if (!reuse) {
+      if (l > 0) host_a = download_csr(lv.A, s);
+      aggregate_nodes(host_a, lv.node_ptr, o.strength_threshold, lv.aggregate);
+      Level next;
+      HCsr pt;
+      tentative(lv, next, c.nullspace_dim, pt);
+      if (next.n >= (int)(0.85 * double(lv.n)) || next.n == 0) {
+        lv.aggregate.clear();
+        break;
+      }
+      // symbolic patterns of P, R, A P, R (A P) and the coarse operator
+      HCsr ppat = pattern_product(host_a, pt);
+      HCsr rpat;
+      std::vector<int> rsrc;
+      transpose_pattern(ppat, rpat, rsrc);
+      HCsr appat = pattern_product(host_a, ppat);
+      HCsr rawpat = pattern_product(rpat, appat);
+      HCsr spat;
+      std::vector<int> ia, ib;
+      symmetrize_pattern(rawpat, spat, ia, ib);
+      lv.Ptent.upload(pt, s);
+      lv.P.upload_pattern(ppat, s);
+      lv.R.upload_pattern(rpat, s);
+      lv.r_src.upload(rsrc, s);
+      lv.AP.upload_pattern(appat, s);
+      lv.RAW.upload_pattern(rawpat, s);
+      lv.sym_a.upload(ia, s);
+      lv.sym_b.upload(ib, s);
+      next.A.upload_pattern(spat, s);
+      lv.has_next = true;
+      SLX_HIP(hipStreamSynchronize(s));  // host pattern arrays go out of scope
+      c.levels.push_back(std::move(next));
+    }
+    Level& cur = c.levels[l];
+    Level& coarse = c.levels[l + 1];
+    DCsr& A = level_op(c, l);
+    flag.upload(&zero, 1, s);
+    k::smooth_prolongator(A, cur.Ptent, cur.P, cur.dinv.p, o.prolongator_damping / cur.lambda_max, flag.p, s);
+    k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
+    k::spgemm_numeric(A, cur.P, cur.AP, flag.p, s);
+    k::spgemm_numeric(cur.R, cur.AP, cur.RAW, flag.p, s);
+    k::symmetrize_values(coarse.A.nnz, cur.sym_a.p, cur.sym_b.p, cur.RAW.val.p, coarse.A.val.p, s);
+    check_flag(c, flag, "the Galerkin product");
+    ++l;
+  }
+  c.levels.resize(l + 1);
+  c.levels.back().has_next = false;
+  // coarsest level: dense inverse
+  {
+    Level& cl = c.levels.back();
+    const int nc = cl.n;
+    if (nc > 8000) fail(SLX_SOLVER_ERROR, "multigrid: the coarsest level exceeds the dense solver's 8000 unknowns");
+    HCsr ac = download_csr(level_op(c, l), s);
+    std::vector<double> dense((size_t)nc * nc, 0.0), inv;
+    for (int i = 0; i < nc; ++i)
+      for (int k2 = ac.ptr[(size_t)i]; k2 < ac.ptr[(size_t)i + 1]; ++k2) dense[(size_t)i * nc + ac.col[(size_t)k2]] = ac.val[(size_t)k2];
+    double ratio = 0.0;
+    coarse_inverse(nc, dense, inv, ratio);
+    c.coarse_pivot_ratio = ratio;
+    if (!(ratio > o.coarse_pivot_tolerance)) {
+      std::ostringstream os;
+      os << "multigrid: the coarsest operator (" << nc << " unknowns) is singular or indefinite (smallest / largest pivot "
+         << ratio << "). A rigid-body mode survived to the coarse level, so the model is under-constrained: too few "
+                     "supports, or a part connected to nothing";
+      fail(SLX_SOLVER_ERROR, os.str());
+    }
+    c.coarse_inv.upload(inv, s);
+  }
+  for (auto& lv : c.levels) {
+    lv.x.alloc((size_t)lv.n);
+    lv.b.alloc((size_t)lv.n);
+    lv.res.alloc((size_t)lv.n);
+    lv.d.alloc((size_t)lv.n);
+    lv.d2.alloc((size_t)lv.n);
+  }
+  if (o.precision == 1) prepare_mixed(c);
+  SLX_HIP(hipStreamSynchronize(s));
+  c.fp_rows = n;
+  c.fp_nnz = c.K.nnz;
+  c.fp_unknowns = c.layout_unknowns;
+  c.amg_ready = true;
+  c.reused = reuse;
+  c.setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+}  // namespace slx

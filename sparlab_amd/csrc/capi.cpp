@@ -1,0 +1,772 @@
+// C ABI of the sparlab_hip library (include/sparlab_hip.h): context, model,
+// assembly, linear solver seam, density filter, compliance objective, OC
+// update and the OC loop, all on one HIP stream.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <sstream>
+
+#include "slx_kernels.hpp"
+#include "slx_model.hpp"
+#include "slx_solver.hpp"
+
+namespace slx {
+
+void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    std::ostringstream os;
+    os << "HIP error " << hipGetErrorString(e) << " in " << what;
+    throw Error(SLX_DEVICE_ERROR, os.str());
+  }
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+template <class F>
+slx_status guarded(slx_context* ctx, F&& f) {
+  Context* c = reinterpret_cast<Context*>(ctx);
+  if (!c) return SLX_CONFIG_ERROR;
+  try {
+    if (c->device >= 0) SLX_HIP(hipSetDevice(c->device));
+    f(*c);
+    c->last_error.clear();
+    return SLX_OK;
+  } catch (const Error& e) {
+    c->last_error = e.msg;
+    return (slx_status)e.code;
+  } catch (const std::exception& e) {
+    c->last_error = e.what();
+    return SLX_DEVICE_ERROR;
+  }
+}
+
+void require_model(const Context& c) {
+  if (!c.finalized) fail(SLX_MODEL_ERROR, "the model is not finalised; call slx_model_finalize first");
+}
+
+void sync(Context& c) { SLX_HIP(hipStreamSynchronize(c.stream)); }
+
+double read_scalar(Context& c, const double* dev) {
+  double v = 0.0;
+  SLX_HIP(hipMemcpyAsync(&v, dev, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  sync(c);
+  return v;
+}
+
+void set_layout_from_model(Context& c) {
+  c.layout_dim = c.dim;
+  c.layout_ndpn = c.dim;
+  c.layout_nnodes = (int)c.nnodes;
+  c.layout_coords = c.coords;
+  c.layout_unknowns = c.free_dofs;
+}
+
+void upload_filter(Context& c) {
+  c.dH.upload(c.H, c.stream);
+  c.dHt.upload(c.Ht, c.stream);
+}
+
+void ensure_topopt_buffers(Context& c) {
+  const size_t ne = (size_t)c.nelem;
+  for (DBuf<double>* b : {&c.d_x, &c.d_phys, &c.d_fac, &c.d_dfac, &c.d_dc, &c.d_energy, &c.d_tmp, &c.d_tmp2,
+                          &c.d_dcx, &c.d_dvx})
+    b->alloc(ne);
+  if (c.d_vol.n != ne) c.d_vol.upload(c.volumes, c.stream);
+  if (c.lower.size() != ne) {
+    c.lower.assign(ne, 0.0);
+    c.upper.assign(ne, 1.0);
+    c.passive.assign(ne, 0);
+  }
+  c.d_lower.upload(c.lower, c.stream);
+  c.d_upper.upload(c.upper, c.stream);
+  c.d_passive.upload(c.passive, c.stream);
+  if (c.H.rows != (int)ne) {
+    identity_filter(c);
+    upload_filter(c);
+  }
+  if (c.partials.n < 4096) c.partials.alloc(4096);
+  if (!c.scal.p) c.scal.alloc(16);
+}
+
+void assemble(Context& c, const double* d_scale) {
+  Timed t(c, "assemble");
+  k::assemble_structured(c.dim, c.K, c.d_free.p, d_scale, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream);
+}
+
+// one load case from its last solution (or zero) to the tolerance
+slx_pcg_result solve_case(Context& c, int l, bool warm, double tol, int maxit) {
+  if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "AMG-CG: solve called before factorize");
+  const size_t nf = c.free_dofs.size();
+  double* x = c.d_u.p + (size_t)l * nf;
+  if (!warm || !c.have_u[(size_t)l]) SLX_HIP(hipMemsetAsync(x, 0, sizeof(double) * nf, c.stream));
+  slx_pcg_result r = pcg(c, c.d_loads_free.p + (size_t)l * nf, x, tol, maxit > 0 ? maxit : 1000);
+  c.have_u[(size_t)l] = 1;
+  return r;
+}
+
+void require_converged(Context& c, const slx_pcg_result& r, double tol) {
+  if (r.converged) return;
+  std::ostringstream os;
+  os << "AMG-CG(chebyshev) did not converge: " << r.iterations << " iterations left a relative residual of "
+     << r.relative_residual << " against a tolerance of " << tol << " (" << c.levels.size()
+     << " levels). Raise solver.linear.max_iterations, lower solver.linear.amg.strength_threshold, raise "
+        "amg.smoother_degree, or use \"simplicial_ldlt\"";
+  fail(SLX_CONVERGENCE_ERROR, os.str());
+}
+
+// ComplianceObjective::evaluate on the device-resident design d_x
+// (src/topopt/Sensitivity.cpp:110-237); fills d_phys, d_fac, d_dfac, d_energy,
+// d_dc (dc / d physical), d_dcx and d_dvx (pulled back through the filter).
+slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_gradients) {
+  require_model(c);
+  if (c.ncases == 0) fail(SLX_MODEL_ERROR, "the model has no load case");
+  const long long ne = c.nelem;
+  const size_t nf = c.free_dofs.size();
+  hipStream_t s = c.stream;
+  slx_evaluation out{};
+  const auto t0 = Clock::now();
+  k::spmv_serial(c.dH, c.d_x.p, c.d_tmp.p, s);
+  k::simp((int)ne, c.d_tmp.p, o.penalty, o.emin_ratio, c.d_phys.p, c.d_fac.p, c.d_dfac.p, s);
+  assemble(c, c.d_fac.p);
+  amg_setup(c);
+  out.assemble_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+  const auto t1 = Clock::now();
+  std::vector<double> comp((size_t)c.ncases);
+  for (int l = 0; l < c.ncases; ++l) {
+    const slx_pcg_result r = solve_case(c, l, o.warm_start != 0, o.tolerance, o.max_iterations);
+    require_converged(c, r, o.tolerance);
+    out.linear_iterations += r.iterations;
+    comp[(size_t)l] = device_dot(c, (long long)nf, c.d_loads_free.p + (size_t)l * nf, c.d_u.p + (size_t)l * nf);
+  }
+  for (int l = 0; l < c.ncases; ++l) out.compliance += c.weights[(size_t)l] * comp[(size_t)l];
+  if (c.d_weights.n != c.weights.size()) c.d_weights.upload(c.weights, s);
+  k::sensitivity(c.dim, (int)ne, c.nx, c.ny, c.d_ke0.p, c.d_reduced.p, c.d_u.p, (long long)nf, c.ncases, c.d_weights.p, c.d_fac.p,
+                 c.d_dfac.p, c.d_energy.p, c.d_dc.p, s);
+  out.volume = device_dot(c, ne, c.d_phys.p, c.d_vol.p);
+  double dv = 0.0;
+  for (double v : c.volumes) dv += v;
+  out.volume_fraction = out.volume / dv;
+  if (need_gradients) {
+    k::spmv_serial(c.dHt, c.d_dc.p, c.d_dcx.p, s);
+    k::spmv_serial(c.dHt, c.d_vol.p, c.d_dvx.p, s);
+  }
+  sync(c);
+  out.solve_seconds = std::chrono::duration<double>(Clock::now() - t1).count();
+  if (!std::isfinite(out.compliance)) fail(SLX_SOLVER_ERROR, "the compliance objective evaluated to a non-finite value");
+  return out;
+}
+
+// optimality_criteria_update on device vectors d_x, d_dcx, d_dvx; the new
+// design lands in d_tmp2 (src/topopt/OptimalityCriteria.cpp:40-149).
+slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
+  const int ne = (int)c.nelem;
+  hipStream_t s = c.stream;
+  if (!(o.move_limit > 0.0 && o.move_limit <= 1.0)) fail(SLX_CONFIG_ERROR, "optimizer.move_limit must lie in (0, 1]");
+  if (!(o.damping > 0.0 && o.damping <= 1.0)) fail(SLX_CONFIG_ERROR, "optimizer.damping must lie in (0, 1]");
+  if (!device_all_finite(c, ne, c.d_dcx.p) || !device_all_finite(c, ne, c.d_dvx.p))
+    fail(SLX_SOLVER_ERROR, "optimality criteria received a non-finite gradient");
+  double dv = 0.0;
+  for (double v : c.volumes) dv += v;
+  const double target = c.volume_fraction * dv;
+  auto volume_at = [&](double lambda) {
+    k::oc_candidate(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, lambda,
+                    o.move_limit, o.damping, c.d_tmp2.p, s);
+    k::filter_volume(c.dH, c.d_tmp2.p, c.d_vol.p, c.partials.p, c.scal.p + 14, s);
+    return read_scalar(c, c.scal.p + 14);
+  };
+  double lo = o.lambda_lower, hi = o.lambda_upper;
+  double vlo = volume_at(lo), vhi = volume_at(hi);
+  int widen = 0;
+  while (vlo < target && widen < 60) {
+    lo *= 1.0e-3;
+    vlo = volume_at(lo);
+    ++widen;
+  }
+  while (vhi > target && widen < 120) {
+    hi *= 1.0e3;
+    vhi = volume_at(hi);
+    ++widen;
+  }
+  if (vlo < target || vhi > target) {
+    std::ostringstream os;
+    os << "the optimality-criteria bisection cannot bracket the volume target " << target
+       << " m^3: the move-limited box yields volumes in [" << vhi << ", " << vlo << "] m^3";
+    fail(SLX_CONVERGENCE_ERROR, os.str());
+  }
+  slx_oc_step step{};
+  double volume = volume_at(lo);
+  for (int it = 0; it < o.max_bisections; ++it) {
+    const double mid = std::sqrt(lo * hi);
+    volume = volume_at(mid);
+    step.bisections = it + 1;
+    step.lambda = mid;
+    if (std::abs(volume - target) <= o.volume_tolerance * target) {
+      step.volume_converged = 1;
+      break;
+    }
+    if (volume > target) lo = mid; else hi = mid;
+    if (hi / lo - 1.0 < 1.0e-15) {
+      step.volume_converged = std::abs(volume - target) <= 1.0e-6 * std::max(target, 1.0e-30);
+      break;
+    }
+  }
+  step.achieved_volume = volume;
+  k::max_abs_diff(ne, c.d_tmp2.p, c.d_x.p, c.partials.p, c.scal.p + 13, s);
+  step.max_change = read_scalar(c, c.scal.p + 13);
+  return step;
+}
+
+}  // namespace
+}  // namespace slx
+
+using namespace slx;
+
+extern "C" {
+
+slx_status slx_create(int device, slx_context** out) {
+  if (!out) return SLX_CONFIG_ERROR;
+  *out = nullptr;
+  Context* c = new Context();
+  c->device = device;
+  try {
+    SLX_HIP(hipSetDevice(device));
+    SLX_HIP(hipStreamCreate(&c->stream));
+    c->scal.alloc(16);
+    c->partials.alloc(4096);
+  } catch (const Error&) {
+    delete c;
+    return SLX_DEVICE_ERROR;
+  }
+  *out = reinterpret_cast<slx_context*>(c);
+  return SLX_OK;
+}
+
+void slx_destroy(slx_context* ctx) {
+  Context* c = reinterpret_cast<Context*>(ctx);
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& pe : c->pending) {
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+  }
+  for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  hipStream_t s = c->stream;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+const char* slx_last_error(const slx_context* ctx) {
+  const Context* c = reinterpret_cast<const Context*>(ctx);
+  return c ? c->last_error.c_str() : "null context";
+}
+
+void slx_amg_default_options(slx_amg_options* o) {
+  o->strength_threshold = 0.02;
+  o->max_levels = 12;
+  o->coarse_size = 1500;
+  o->smoother = 0;
+  o->smoother_degree = 3;
+  o->chebyshev_ratio = 30.0;
+  o->prolongator_damping = 4.0 / 3.0;
+  o->lanczos_steps = 12;
+  o->coarse_pivot_tolerance = 1.0e-13;
+  o->reuse_aggregates = 1;
+  o->precision = 0;
+}
+
+void slx_eval_default_options(slx_eval_options* o) {
+  o->penalty = 3.0;
+  o->emin_ratio = 1.0e-9;
+  o->tolerance = 1.0e-8;
+  o->max_iterations = 1000;
+  o->warm_start = 1;
+}
+
+void slx_oc_default_options(slx_oc_options* o) {
+  o->move_limit = 0.2;
+  o->damping = 0.5;
+  o->lambda_lower = 1.0e-12;
+  o->lambda_upper = 1.0e12;
+  o->max_bisections = 200;
+  o->volume_tolerance = 1.0e-10;
+}
+
+slx_status slx_model_structured(slx_context* ctx, int dim, int nx, int ny, int nz, double lx, double ly, double lz,
+                                double E, double nu, double thickness) {
+  return guarded(ctx, [&](Context& c) {
+    if (dim != 2 && dim != 3) fail(SLX_CONFIG_ERROR, "dim must be 2 (Q4) or 3 (Hex8)");
+    if (nx < 1 || ny < 1 || (dim == 3 && nz < 1)) fail(SLX_CONFIG_ERROR, "structured mesh needs positive element counts");
+    if (!(lx > 0) || !(ly > 0) || (dim == 3 && !(lz > 0))) fail(SLX_CONFIG_ERROR, "structured mesh needs positive lengths");
+    if (dim == 3 && thickness != 1.0) fail(SLX_CONFIG_ERROR, "a solid (Hex8) element has no thickness; leave it at 1");
+    const long long nn = (long long)(nx + 1) * (ny + 1) * (dim == 3 ? nz + 1 : 1);
+    if (nn * dim > 2147483647LL / 27) fail(SLX_CONFIG_ERROR, "the grid exceeds the int32 index range of K_ff");
+    c.dim = dim;
+    c.nx = nx;
+    c.ny = ny;
+    c.nz = dim == 3 ? nz : 0;
+    c.lx = lx;
+    c.ly = ly;
+    c.lz = lz;
+    c.E = E;
+    c.nu = nu;
+    c.thickness = thickness;
+    build_structured_grid(c);
+    c.fixed.assign((size_t)c.ndofs, 0);
+    c.finalized = false;
+    c.amg_ready = false;
+    c.ncases = 0;
+    c.loads.clear();
+    c.H = HCsr();
+    c.Ht = HCsr();
+  });
+}
+
+slx_status slx_model_set_supports(slx_context* ctx, const unsigned char* fixed) {
+  return guarded(ctx, [&](Context& c) {
+    if (c.ndofs == 0) fail(SLX_MODEL_ERROR, "define the mesh first");
+    for (long long d = 0; d < c.ndofs; ++d) c.fixed[(size_t)d] = fixed[d] ? 1 : 0;
+    c.finalized = false;
+    c.amg_ready = false;
+  });
+}
+
+slx_status slx_model_set_loads(slx_context* ctx, int ncases, const double* loads, const double* weights) {
+  return guarded(ctx, [&](Context& c) {
+    if (ncases < 1) fail(SLX_CONFIG_ERROR, "at least one load case is needed");
+    double sum = 0.0;
+    for (int l = 0; l < ncases; ++l) sum += weights[l];
+    if (!(sum > 0.0)) fail(SLX_CONFIG_ERROR, "load-case weights must sum to a positive number");
+    c.ncases = ncases;
+    c.loads.assign(loads, loads + (size_t)ncases * c.ndofs);
+    c.weights.resize((size_t)ncases);
+    for (int l = 0; l < ncases; ++l) c.weights[(size_t)l] = weights[l] / sum;
+    c.d_weights.upload(c.weights, c.stream);
+    if (c.finalized) {
+      const size_t nf = c.free_dofs.size();
+      std::vector<double> lf((size_t)ncases * nf);
+      for (int l = 0; l < ncases; ++l)
+        for (size_t i = 0; i < nf; ++i) lf[(size_t)l * nf + i] = c.loads[(size_t)l * c.ndofs + c.free_dofs[i]];
+      c.d_loads_free.upload(lf, c.stream);
+      c.d_u.alloc((size_t)ncases * nf);
+      c.have_u.assign((size_t)ncases, 0);
+      sync(c);
+    }
+  });
+}
+
+slx_status slx_model_finalize(slx_context* ctx) {
+  return guarded(ctx, [&](Context& c) {
+    if (c.ndofs == 0) fail(SLX_MODEL_ERROR, "define the mesh first");
+    // DofManager::rebuild (src/fem/DofManager.cpp:58-71)
+    c.free_dofs.clear();
+    c.reduced.assign((size_t)c.ndofs, -1);
+    for (long long d = 0; d < c.ndofs; ++d) {
+      if (c.fixed[(size_t)d]) continue;
+      c.reduced[(size_t)d] = (int)c.free_dofs.size();
+      c.free_dofs.push_back((int)d);
+    }
+    if (c.free_dofs.empty()) fail(SLX_MODEL_ERROR, "every DOF is fixed");
+    const std::vector<double> xe = element0_coords(c);
+    c.ke0 = c.dim == 3 ? hex8_ke0(xe.data(), c.E, c.nu) : quad4_ke0(xe.data(), c.E, c.nu, c.thickness);
+    c.volumes.assign((size_t)c.nelem, element_volume(c));
+    const HCsr pat = structured_kff_pattern(c);
+    c.K.upload_pattern(pat, c.stream);
+    c.d_free.upload(c.free_dofs, c.stream);
+    c.d_reduced.upload(c.reduced, c.stream);
+    c.d_ke0.upload(c.ke0, c.stream);
+    c.d_vol.upload(c.volumes, c.stream);
+    set_layout_from_model(c);
+    ensure_workspace(c);
+    c.finalized = true;
+    c.amg_ready = false;
+    if (c.ncases > 0) {
+      const size_t nf = c.free_dofs.size();
+      std::vector<double> lf((size_t)c.ncases * nf);
+      for (int l = 0; l < c.ncases; ++l)
+        for (size_t i = 0; i < nf; ++i) lf[(size_t)l * nf + i] = c.loads[(size_t)l * c.ndofs + c.free_dofs[i]];
+      c.d_loads_free.upload(lf, c.stream);
+      c.d_u.alloc((size_t)c.ncases * nf);
+      c.have_u.assign((size_t)c.ncases, 0);
+    }
+    sync(c);
+  });
+}
+
+slx_status slx_model_sizes(const slx_context* ctx, long long* out) {
+  const Context* c = reinterpret_cast<const Context*>(ctx);
+  if (!c) return SLX_CONFIG_ERROR;
+  out[0] = c->nnodes;
+  out[1] = c->nelem;
+  out[2] = c->ndofs;
+  out[3] = (long long)c->free_dofs.size();
+  out[4] = c->K.nnz;
+  out[5] = c->ncases;
+  return SLX_OK;
+}
+
+slx_status slx_ke0_get(const slx_context* ctx, double* ke) {
+  const Context* c = reinterpret_cast<const Context*>(ctx);
+  if (!c || c->ke0.empty()) return SLX_MODEL_ERROR;
+  std::memcpy(ke, c->ke0.data(), c->ke0.size() * sizeof(double));
+  return SLX_OK;
+}
+
+slx_status slx_free_dofs_get(const slx_context* ctx, int* free_dofs) {
+  const Context* c = reinterpret_cast<const Context*>(ctx);
+  if (!c || !c->finalized) return SLX_MODEL_ERROR;
+  std::memcpy(free_dofs, c->free_dofs.data(), c->free_dofs.size() * sizeof(int));
+  return SLX_OK;
+}
+
+slx_status slx_assemble(slx_context* ctx, const double* scale) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    const size_t ne = (size_t)c.nelem;
+    std::vector<double> ones;
+    const double* h = scale;
+    if (!h) {
+      ones.assign(ne, 1.0);
+      h = ones.data();
+    }
+    double mn = std::numeric_limits<double>::max();
+    for (size_t e = 0; e < ne; ++e) {
+      if (!std::isfinite(h[e])) fail(SLX_MODEL_ERROR, "stiffness scale vector contains NaN or Inf");
+      mn = std::min(mn, h[e]);
+    }
+    if (mn < 0.0) fail(SLX_MODEL_ERROR, "stiffness scale vector has a negative entry");
+    c.d_scale.upload(h, ne, c.stream);
+    assemble(c, c.d_scale.p);
+    c.flush_timers();
+    sync(c);
+  });
+}
+
+slx_status slx_kff_get(const slx_context* ctx, int* ptr, int* col, double* val) {
+  return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
+    require_model(c);
+    if (ptr) c.K.ptr.download(ptr, (size_t)c.K.rows + 1, c.stream);
+    if (col) c.K.col.download(col, (size_t)c.K.nnz, c.stream);
+    if (val) c.K.val.download(val, (size_t)c.K.nnz, c.stream);
+    sync(c);
+  });
+}
+
+slx_status slx_amg_setup(slx_context* ctx, const slx_amg_options* opt) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    slx_amg_options o;
+    slx_amg_default_options(&o);
+    if (opt) o = *opt;
+    if (!(o.strength_threshold >= 0.0 && o.strength_threshold < 1.0)) fail(SLX_CONFIG_ERROR, "multigrid strength_threshold must lie in [0, 1)");
+    if (o.max_levels < 1 || o.max_levels > 16) fail(SLX_CONFIG_ERROR, "multigrid max_levels must lie in [1, 16]");
+    if (o.coarse_size < 1) fail(SLX_CONFIG_ERROR, "multigrid coarse_size must be positive");
+    if (o.smoother_degree < 1) fail(SLX_CONFIG_ERROR, "multigrid smoother_degree must be at least 1");
+    if (!(o.chebyshev_ratio > 1.0)) fail(SLX_CONFIG_ERROR, "multigrid chebyshev_ratio must exceed 1");
+    if (!(o.prolongator_damping > 0.0 && o.prolongator_damping < 2.0)) fail(SLX_CONFIG_ERROR, "multigrid prolongator_damping must lie in (0, 2)");
+    if (o.lanczos_steps < 2) fail(SLX_CONFIG_ERROR, "multigrid lanczos_steps must be at least 2");
+    if (o.precision != 0 && o.precision != 1) fail(SLX_CONFIG_ERROR, "multigrid precision must be 0 (FP64) or 1 (FP32 cycle)");
+    if (c.amg_ready && (o.precision != c.amg_opt.precision || o.max_levels != c.amg_opt.max_levels ||
+                        o.coarse_size != c.amg_opt.coarse_size || o.strength_threshold != c.amg_opt.strength_threshold))
+      c.amg_ready = false;
+    c.amg_opt = o;
+    amg_setup(c);
+  });
+}
+
+slx_status slx_factorize_csr(slx_context* ctx, int n, const int* ptr, const int* col, const double* val, int dim,
+                             int ndpn, int nnodes, const double* coords, const int* unknowns,
+                             const slx_amg_options* opt) {
+  return guarded(ctx, [&](Context& c) {
+    if (n <= 0) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix must be square and non-empty");
+    HCsr h;
+    h.rows = h.cols = n;
+    h.ptr.assign(ptr, ptr + n + 1);
+    h.col.assign(col, col + ptr[n]);
+    h.val.assign(val, val + ptr[n]);
+    for (int i = 0; i < n; ++i)
+      for (int k2 = h.ptr[(size_t)i] + 1; k2 < h.ptr[(size_t)i + 1]; ++k2)
+        if (h.col[(size_t)k2] <= h.col[(size_t)k2 - 1]) fail(SLX_SOLVER_ERROR, "CSR rows must be sorted by column without duplicates");
+    const bool same_pattern = c.K.rows == n && c.K.nnz == h.nnz() && c.layout_unknowns.size() == (size_t)n &&
+                              std::equal(unknowns, unknowns + n, c.layout_unknowns.begin());
+    if (same_pattern) c.K.val.upload(h.val, c.stream); else c.K.upload(h, c.stream);
+    c.layout_dim = dim;
+    c.layout_ndpn = ndpn > 0 ? ndpn : dim;
+    c.layout_nnodes = nnodes;
+    c.layout_coords.assign(coords, coords + (size_t)nnodes * dim);
+    c.layout_unknowns.assign(unknowns, unknowns + n);
+    slx_amg_options o;
+    slx_amg_default_options(&o);
+    if (opt) o = *opt;
+    c.amg_opt = o;
+    amg_setup(c);
+  });
+}
+
+slx_status slx_solve(slx_context* ctx, const double* b, double* x, double tol, int max_iterations,
+                     slx_pcg_result* res) {
+  return guarded(ctx, [&](Context& c) {
+    if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "AMG-CG: solve called before factorize");
+    const size_t n = (size_t)c.K.rows;
+    c.bw.upload(b, n, c.stream);
+    DBuf<double> xd;
+    xd.upload(x, n, c.stream);
+    slx_pcg_result r = pcg(c, c.bw.p, xd.p, tol, max_iterations > 0 ? max_iterations : 1000);
+    xd.download(x, n, c.stream);
+    sync(c);
+    if (res) *res = r;
+  });
+}
+
+slx_status slx_solve_case(slx_context* ctx, int load_case, int warm_start, double tol, int max_iterations,
+                          slx_pcg_result* res) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    if (load_case < 0 || load_case >= c.ncases) fail(SLX_CONFIG_ERROR, "load case index out of range");
+    slx_pcg_result r = solve_case(c, load_case, warm_start != 0, tol, max_iterations);
+    if (res) *res = r;
+  });
+}
+
+slx_status slx_amg_apply(slx_context* ctx, const double* r, double* z) {
+  return guarded(ctx, [&](Context& c) {
+    const size_t n = (size_t)c.K.rows;
+    c.bw.upload(r, n, c.stream);
+    vcycle(c, c.bw.p, c.xw.p);
+    c.xw.download(z, n, c.stream);
+    sync(c);
+  });
+}
+
+slx_status slx_amg_stats_get(const slx_context* ctx, slx_amg_stats* out) {
+  const Context* c = reinterpret_cast<const Context*>(ctx);
+  if (!c || !c->amg_ready) return SLX_SOLVER_ERROR;
+  std::memset(out, 0, sizeof(*out));
+  out->levels = (int)c->levels.size();
+  out->reused_aggregates = c->reused ? 1 : 0;
+  out->coarse_pivot_ratio = c->coarse_pivot_ratio;
+  out->near_null_space_dimension = c->nullspace_dim;
+  out->setup_seconds = c->setup_seconds;
+  double nnz_sum = 0, rows_sum = 0;
+  for (size_t l = 0; l < c->levels.size() && l < 16; ++l) {
+    const Level& lv = c->levels[l];
+    out->unknowns[l] = lv.n;
+    out->nonzeros[l] = l == 0 ? c->K.nnz : lv.A.nnz;
+    out->aggregates[l] = l + 1 < c->levels.size() ? c->levels[l + 1].num_nodes : 0;
+    out->lambda_max[l] = lv.lambda_max;
+    nnz_sum += (double)out->nonzeros[l];
+    rows_sum += (double)out->unknowns[l];
+  }
+  out->operator_complexity = nnz_sum / (double)out->nonzeros[0];
+  out->grid_complexity = rows_sum / (double)out->unknowns[0];
+  return SLX_OK;
+}
+
+slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long long* info) {
+  return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
+    if (!c.amg_ready || level < 0 || level >= (int)c.levels.size()) fail(SLX_SOLVER_ERROR, "no such multigrid level");
+    Level& lv = c.levels[(size_t)level];
+    const DCsr* m = which == 0 ? (level == 0 ? &c.K : &lv.A) : which == 1 ? &lv.P : which == 2 ? &lv.Ptent : &lv.R;
+    info[0] = m->rows;
+    info[1] = m->cols;
+    info[2] = m->nnz;
+  });
+}
+
+slx_status slx_amg_level_get(const slx_context* ctx, int level, int which, int* ptr, int* col, double* val) {
+  return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
+    if (!c.amg_ready || level < 0 || level >= (int)c.levels.size()) fail(SLX_SOLVER_ERROR, "no such multigrid level");
+    Level& lv = c.levels[(size_t)level];
+    const DCsr* m = which == 0 ? (level == 0 ? &c.K : &lv.A) : which == 1 ? &lv.P : which == 2 ? &lv.Ptent : &lv.R;
+    m->ptr.download(ptr, (size_t)m->rows + 1, c.stream);
+    m->col.download(col, (size_t)m->nnz, c.stream);
+    m->val.download(val, (size_t)m->nnz, c.stream);
+    sync(c);
+  });
+}
+
+slx_status slx_amg_aggregates_get(const slx_context* ctx, int level, int* aggregates) {
+  return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
+    if (!c.amg_ready || level < 0 || level >= (int)c.levels.size()) fail(SLX_SOLVER_ERROR, "no such multigrid level");
+    const std::vector<int>& a = c.levels[(size_t)level].aggregate;
+    std::memcpy(aggregates, a.data(), a.size() * sizeof(int));
+  });
+}
+
+slx_status slx_solution_get(const slx_context* ctx, int load_case, double* u) {
+  return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
+    if (load_case < 0 || load_case >= c.ncases) fail(SLX_CONFIG_ERROR, "load case index out of range");
+    const size_t nf = c.free_dofs.size();
+    SLX_HIP(hipMemcpyAsync(u, c.d_u.p + (size_t)load_case * nf, sizeof(double) * nf, hipMemcpyDeviceToHost, c.stream));
+    sync(c);
+  });
+}
+
+slx_status slx_design_set(slx_context* ctx, double volume_fraction, const double* lower, const double* upper,
+                          const unsigned char* passive) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    if (!(volume_fraction > 0.0 && volume_fraction <= 1.0)) fail(SLX_CONFIG_ERROR, "volume fraction must lie in (0, 1]");
+    const size_t ne = (size_t)c.nelem;
+    c.volume_fraction = volume_fraction;
+    c.lower.assign(lower, lower + ne);
+    c.upper.assign(upper, upper + ne);
+    c.passive.assign(passive, passive + ne);
+    ensure_topopt_buffers(c);
+    sync(c);
+  });
+}
+
+slx_status slx_filter_build(slx_context* ctx, double radius) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    if (radius > 0.0) build_filter(c, radius); else identity_filter(c);
+    upload_filter(c);
+    ensure_topopt_buffers(c);
+    sync(c);
+  });
+}
+
+slx_status slx_filter_apply(slx_context* ctx, int adjoint, const double* in, double* out) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    ensure_topopt_buffers(c);
+    const size_t ne = (size_t)c.nelem;
+    c.d_tmp.upload(in, ne, c.stream);
+    k::spmv_serial(adjoint ? c.dHt : c.dH, c.d_tmp.p, c.d_tmp2.p, c.stream);
+    c.d_tmp2.download(out, ne, c.stream);
+    sync(c);
+  });
+}
+
+slx_status slx_evaluate(slx_context* ctx, const double* x, const slx_eval_options* opt, slx_evaluation* out,
+                        double* dc_dx, double* dv_dx, double* energy) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    ensure_topopt_buffers(c);
+    slx_eval_options o;
+    slx_eval_default_options(&o);
+    if (opt) o = *opt;
+    const size_t ne = (size_t)c.nelem;
+    for (size_t e = 0; e < ne; ++e)
+      if (!std::isfinite(x[e])) fail(SLX_CONFIG_ERROR, "design vector contains NaN or Inf");
+    c.d_x.upload(x, ne, c.stream);
+    const slx_evaluation ev = evaluate_device(c, o, dc_dx != nullptr || dv_dx != nullptr);
+    if (dc_dx) c.d_dcx.download(dc_dx, ne, c.stream);
+    if (dv_dx) c.d_dvx.download(dv_dx, ne, c.stream);
+    if (energy) c.d_energy.download(energy, ne, c.stream);
+    sync(c);
+    if (out) *out = ev;
+  });
+}
+
+slx_status slx_oc_update(slx_context* ctx, const double* x, const double* dc_dx, const double* dv_dx,
+                         const slx_oc_options* opt, double* x_new, slx_oc_step* step) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    ensure_topopt_buffers(c);
+    slx_oc_options o;
+    slx_oc_default_options(&o);
+    if (opt) o = *opt;
+    const size_t ne = (size_t)c.nelem;
+    c.d_x.upload(x, ne, c.stream);
+    c.d_dcx.upload(dc_dx, ne, c.stream);
+    c.d_dvx.upload(dv_dx, ne, c.stream);
+    const slx_oc_step st = oc_device(c, o);
+    c.d_tmp2.download(x_new, ne, c.stream);
+    sync(c);
+    if (step) *step = st;
+  });
+}
+
+slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history, int* performed,
+                      double* final_compliance) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    ensure_topopt_buffers(c);
+    if (!opt) fail(SLX_CONFIG_ERROR, "run options are required");
+    const slx_run_options& o = *opt;
+    const int ne = (int)c.nelem;
+    c.d_x.upload(x, (size_t)ne, c.stream);
+    k::clamp(ne, c.d_x.p, c.d_lower.p, c.d_upper.p, c.stream);
+    std::vector<double> chist;
+    std::vector<double> phys((size_t)ne);
+    int iteration;
+    for (iteration = 1; iteration <= o.max_iterations; ++iteration) {
+      // penalty continuation (src/topopt/TopologyOptimizer.cpp:245-253)
+      slx_eval_options eo = o.eval;
+      if (o.continuation_steps > 1) {
+        const int stage = std::min((iteration - 1) / std::max(o.continuation_iterations, 1), o.continuation_steps - 1);
+        const double t = double(stage) / double(o.continuation_steps - 1);
+        eo.penalty = o.penalty_start + t * (o.eval.penalty - o.penalty_start);
+      }
+      const slx_evaluation ev = evaluate_device(c, eo, true);
+      const slx_oc_step st = oc_device(c, o.oc);
+      c.d_phys.download(phys.data(), (size_t)ne, c.stream);
+      sync(c);
+      double gray = 0.0;
+      for (double r : phys) gray += r * (1.0 - r);
+      gray = 4.0 * gray / double(ne);
+      if (history) {
+        double* row = history + (size_t)(iteration - 1) * 8;
+        row[0] = iteration;
+        row[1] = eo.penalty;
+        row[2] = ev.compliance;
+        row[3] = ev.volume_fraction;
+        row[4] = st.max_change;
+        row[5] = st.lambda;
+        row[6] = ev.linear_iterations;
+        row[7] = gray;
+      }
+      chist.push_back(ev.compliance);
+      std::swap(c.d_x, c.d_tmp2);
+      k::clamp(ne, c.d_x.p, c.d_lower.p, c.d_upper.p, c.stream);
+      const bool at_final = o.continuation_steps <= 1 || iteration >= o.continuation_iterations * (o.continuation_steps - 1);
+      const bool change_ok = st.max_change <= o.change_tolerance;
+      double spread = std::numeric_limits<double>::infinity();
+      const size_t w = (size_t)std::max(o.objective_window, 1);
+      if (chist.size() > w) {
+        auto mm = std::minmax_element(chist.end() - (long)(w + 1), chist.end());
+        spread = (*mm.second - *mm.first) / std::max(std::abs(chist.back()), 1.0e-30);
+      }
+      const bool objective_ok = o.objective_tolerance > 0.0 && spread <= o.objective_tolerance;
+      if (at_final && (change_ok || objective_ok)) break;
+    }
+    const int done = std::min(iteration, o.max_iterations);
+    slx_eval_options eo = o.eval;
+    const slx_evaluation fin = evaluate_device(c, eo, true);
+    c.d_x.download(x, (size_t)ne, c.stream);
+    sync(c);
+    if (performed) *performed = done;
+    if (final_compliance) *final_compliance = fin.compliance;
+  });
+}
+
+slx_status slx_timing_enable(slx_context* ctx, int on) {
+  return guarded(ctx, [&](Context& c) {
+    c.flush_timers();
+    c.timing = on != 0;
+    c.timers.clear();
+  });
+}
+
+slx_status slx_timing_get(slx_context* ctx, const char* kernel, double* total_ms, long long* launches) {
+  return guarded(ctx, [&](Context& c) {
+    c.flush_timers();
+    auto it = c.timers.find(kernel);
+    *total_ms = it == c.timers.end() ? 0.0 : it->second.total_ms;
+    *launches = it == c.timers.end() ? 0 : it->second.launches;
+  });
+}
+
+slx_status slx_synchronize(slx_context* ctx) {
+  return guarded(ctx, [&](Context& c) { sync(c); });
+}
+
+}  // extern "C"

@@ -1,0 +1,614 @@
+// HIP kernels of the SIMP solve path for gfx950 (MI355X): 64-wide wavefronts,
+// operator products in CSR with a group of lanes per row, deterministic
+// two-stage reductions (fixed block -> partial map, partials summed in a fixed
+// tree), structured-grid gather assembly without atomics, element kernels with
+// the element matrix staged in LDS.
+//
+// Floating-point contraction is off for this file (build flag), so every
+// expression rounds as written; that keeps assembly, filter and SIMP arithmetic
+// bit-identical to the CPU restatement where the tests ask for it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "slx_kernels.hpp"
+
+namespace slx {
+namespace k {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kReduceBlocks = 1024;  // fixed partial count of every reduction
+
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+  // Workgroups are dealt round-robin to the 8 XCDs; map them so each XCD sweeps
+  // one contiguous range of rows (its own L2 then holds that range's x). The
+  // grid is a multiple of 8 (grid8), which makes the map a permutation.
+  return (b % 8) * (nblocks / 8) + b / 8;
+}
+
+template <int LANES>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, LANES);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  // wave reduce then cross-wave through LDS; fixed order
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) t += sh[i];
+  }
+  return t;  // valid on thread 0
+}
+
+// MODE 0: y = A x; 1: y = b - A x; 2: y += A x; 3: y = A x and partial x.y
+template <int LANES, int MODE, typename VT>
+__global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                  const VT* __restrict__ val, const VT* __restrict__ x,
+                                                  const VT* __restrict__ b, VT* __restrict__ y,
+                                                  double* __restrict__ partials) {
+  __shared__ double sh[kBlock / 64];
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int rows_per_block = kBlock / LANES;
+  const int row = blk * rows_per_block + threadIdx.x / LANES;
+  const int lane = threadIdx.x % LANES;
+  double prod = 0.0;
+  if (row < n) {
+    const int s = ptr[row], e = ptr[row + 1];
+    double acc = 0.0;
+    for (int k = s + lane; k < e; k += LANES) acc += (double)val[k] * (double)x[col[k]];
+    acc = group_sum<LANES>(acc);
+    if (lane == 0) {
+      if (MODE == 0) y[row] = (VT)acc;
+      if (MODE == 1) y[row] = (VT)((double)b[row] - acc);
+      if (MODE == 2) y[row] = (VT)((double)y[row] + acc);
+      if (MODE == 3) {
+        y[row] = (VT)acc;
+        prod = (double)x[row] * acc;
+      }
+    }
+  }
+  if (MODE == 3) {
+    const double t = block_sum(prod, sh);
+    if (threadIdx.x == 0) partials[blk] = t;
+  }
+}
+
+// One thread per row, products summed in column order: the CPU restatement's
+// arithmetic exactly (used where results must match bit for bit).
+__global__ void __launch_bounds__(kBlock) k_spmv_serial(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                        const double* __restrict__ val, const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  double s = 0.0;
+  for (int k = ptr[row]; k < ptr[row + 1]; ++k) s += val[k] * x[col[k]];
+  y[row] = s;
+}
+
+// Chebyshev step (src/fem/Multigrid.cpp:833-853), fused:
+//   r_i <- r_i - (A d_in)_i;  d_out_i = c1 d_in_i + c2 dinv_i r_i;  x_i += d_out_i
+template <int LANES, typename VT>
+__global__ void __launch_bounds__(kBlock) k_cheb_step(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                       const VT* __restrict__ val, const VT* __restrict__ dinv,
+                                                       const VT* __restrict__ r_in, VT* __restrict__ r_out,
+                                                       const VT* __restrict__ d_in, VT* __restrict__ d_out,
+                                                       VT* __restrict__ x, double c1, double c2) {
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int row = blk * (kBlock / LANES) + threadIdx.x / LANES;
+  const int lane = threadIdx.x % LANES;
+  if (row >= n) return;
+  const int s = ptr[row], e = ptr[row + 1];
+  double acc = 0.0;
+  for (int k = s + lane; k < e; k += LANES) acc += (double)val[k] * (double)d_in[col[k]];
+  acc = group_sum<LANES>(acc);
+  if (lane == 0) {
+    const double r = (double)r_in[row] - acc;
+    const double dn = c1 * (double)d_in[row] + c2 * (double)dinv[row] * r;
+    r_out[row] = (VT)r;
+    d_out[row] = (VT)dn;
+    x[row] = (VT)((double)x[row] + dn);
+  }
+}
+
+// Zero-guess start of the smoother: d = dinv b / theta, x = d.
+template <typename VT>
+__global__ void k_cheb_first_zero(int n, const VT* __restrict__ dinv, const VT* __restrict__ b, double theta,
+                                  VT* __restrict__ x, VT* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double di = (double)dinv[i] * (double)b[i] / theta;
+  d[i] = (VT)di;
+  x[i] = (VT)(0.0 + di);
+}
+
+// Non-zero-guess start: d = dinv r / theta, x += d (r = b - A x computed before).
+template <typename VT>
+__global__ void k_cheb_first(int n, const VT* __restrict__ dinv, const VT* __restrict__ r, double theta,
+                             VT* __restrict__ x, VT* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double di = (double)dinv[i] * (double)r[i] / theta;
+  d[i] = (VT)di;
+  x[i] = (VT)((double)x[i] + di);
+}
+
+// y = M b for a dense row-major n x n M: one wave per row.
+template <typename VT>
+__global__ void __launch_bounds__(kBlock) k_dense_gemv(int n, const VT* __restrict__ M, const VT* __restrict__ b,
+                                                        VT* __restrict__ y) {
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double acc = 0.0;
+  const VT* m = M + (size_t)row * n;
+  for (int j = lane; j < n; j += 64) acc += (double)m[j] * (double)b[j];
+  acc = group_sum<64>(acc);
+  if (lane == 0) y[row] = (VT)acc;
+}
+
+__global__ void __launch_bounds__(kBlock) k_dot_partials(int n, const double* __restrict__ a,
+                                                          const double* __restrict__ b, double* __restrict__ partials) {
+  __shared__ double sh[kBlock / 64];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += a[i] * b[i];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// Sums m partials in a fixed order into out[0]; optional sqrt.
+__global__ void __launch_bounds__(1024) k_reduce_partials(int m, const double* __restrict__ partials,
+                                                          double* __restrict__ out, int take_sqrt) {
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) s += partials[i];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(t) : t;
+}
+
+// PCG update (src/fem/Multigrid.cpp:1194-1203): alpha = rz / pq;
+// x += alpha p; r -= alpha q; partial sums of r.r
+__global__ void __launch_bounds__(kBlock) k_pcg_update(int n, double* __restrict__ x, double* __restrict__ r,
+                                                        const double* __restrict__ p, const double* __restrict__ q,
+                                                        const double* __restrict__ scal, int i_rz, int i_pq,
+                                                        double* __restrict__ partials) {
+  __shared__ double sh[kBlock / 64];
+  const double alpha = scal[i_rz] / scal[i_pq];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// p = z + beta p with beta = rz_new / rz_old (src/fem/Multigrid.cpp:1209-1215)
+__global__ void k_pcg_direction(int n, double* __restrict__ p, const double* __restrict__ z,
+                                const double* __restrict__ scal, int i_new, int i_old) {
+  const double beta = i_old >= 0 ? scal[i_new] / scal[i_old] : scal[i_new];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+}
+
+__global__ void k_copy_scalar(double* scal, int from, int to) { scal[to] = scal[from]; }
+
+// Polak-Ribiere beta of flexible CG: (r.z_new - r.z_old) / rz_old
+__global__ void k_flexible_beta(double* scal, int i_new, int i_cross, int i_old, int i_out) {
+  scal[i_out] = (scal[i_new] - scal[i_cross]) / scal[i_old];
+}
+
+template <typename A, typename B>
+__global__ void k_convert(int n, const A* __restrict__ in, B* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (B)in[i];
+}
+
+__global__ void k_axpy(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+
+// ---------------------------------------------------------------------------
+// Structured-grid gather assembly: every K_ff entry sums s_e Ke0 over the
+// cells shared by its two nodes in ascending element order, which is the order
+// the reference's element-loop scatter adds them (src/fem/Assembler.cpp:182-214).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int hex_corner(int lx, int ly, int lz) {
+  // VTK hexahedron corner of the local offsets (StructuredMesh.cpp:224-234)
+  return lz * 4 + (ly ? (lx ? 2 : 3) : (lx ? 1 : 0));
+}
+
+template <int LANES>
+__global__ void __launch_bounds__(kBlock) k_assemble_hex8(int nrows, const int* __restrict__ ptr,
+                                                           const int* __restrict__ col, double* __restrict__ val,
+                                                           const int* __restrict__ free_dofs,
+                                                           const double* __restrict__ scale,
+                                                           const double* __restrict__ ke0, int nx, int ny, int nz) {
+  __shared__ double ke[576];
+  for (int t = threadIdx.x; t < 576; t += blockDim.x) ke[t] = ke0[t];
+  __syncthreads();
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int row = blk * (kBlock / LANES) + threadIdx.x / LANES;
+  const int lane = threadIdx.x % LANES;
+  if (row >= nrows) return;
+  const int nnx = nx + 1, nny = ny + 1;
+  const int g = free_dofs[row];
+  const int I = g / 3, ci = g % 3;
+  const int ix = I % nnx, iy = (I / nnx) % nny, iz = I / (nnx * nny);
+  for (int k = ptr[row] + lane; k < ptr[row + 1]; k += LANES) {
+    const int gj = free_dofs[col[k]];
+    const int J = gj / 3, cj = gj % 3;
+    const int jx = J % nnx, jy = (J / nnx) % nny, jz = J / (nnx * nny);
+    const int x0 = max(max(ix, jx) - 1, 0), x1 = min(min(ix, jx), nx - 1);
+    const int y0 = max(max(iy, jy) - 1, 0), y1 = min(min(iy, jy), ny - 1);
+    const int z0 = max(max(iz, jz) - 1, 0), z1 = min(min(iz, jz), nz - 1);
+    double v = 0.0;
+    for (int cz = z0; cz <= z1; ++cz)
+      for (int cy = y0; cy <= y1; ++cy)
+        for (int cx = x0; cx <= x1; ++cx) {
+          const int e = (cz * ny + cy) * nx + cx;
+          const int a = hex_corner(jx - cx, jy - cy, jz - cz);
+          const int b = hex_corner(ix - cx, iy - cy, iz - cz);
+          v += scale[e] * ke[(a * 3 + cj) * 24 + b * 3 + ci];
+        }
+    val[k] = v;
+  }
+}
+
+template <int LANES>
+__global__ void __launch_bounds__(kBlock) k_assemble_quad4(int nrows, const int* __restrict__ ptr,
+                                                            const int* __restrict__ col, double* __restrict__ val,
+                                                            const int* __restrict__ free_dofs,
+                                                            const double* __restrict__ scale,
+                                                            const double* __restrict__ ke0, int nx, int ny) {
+  __shared__ double ke[64];
+  for (int t = threadIdx.x; t < 64; t += blockDim.x) ke[t] = ke0[t];
+  __syncthreads();
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int row = blk * (kBlock / LANES) + threadIdx.x / LANES;
+  const int lane = threadIdx.x % LANES;
+  if (row >= nrows) return;
+  const int nnx = nx + 1;
+  const int g = free_dofs[row];
+  const int I = g / 2, ci = g % 2;
+  const int ix = I % nnx, iy = I / nnx;
+  for (int k = ptr[row] + lane; k < ptr[row + 1]; k += LANES) {
+    const int gj = free_dofs[col[k]];
+    const int J = gj / 2, cj = gj % 2;
+    const int jx = J % nnx, jy = J / nnx;
+    const int x0 = max(max(ix, jx) - 1, 0), x1 = min(min(ix, jx), nx - 1);
+    const int y0 = max(max(iy, jy) - 1, 0), y1 = min(min(iy, jy), ny - 1);
+    double v = 0.0;
+    for (int cy = y0; cy <= y1; ++cy)
+      for (int cx = x0; cx <= x1; ++cx) {
+        const int e = cy * nx + cx;
+        const int a = hex_corner(jx - cx, jy - cy, 0);
+        const int b = hex_corner(ix - cx, iy - cy, 0);
+        v += scale[e] * ke[(a * 2 + cj) * 8 + b * 2 + ci];
+      }
+    val[k] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element loop of the compliance sensitivity (src/topopt/Sensitivity.cpp:182-189):
+// q_e = u_e^T Ke0 u_e per load case; energy_e += w 0.5 s_e q_e; dc_e -= w ds_e q_e.
+// One thread per element, Ke0 in LDS (every lane reads the same entry: broadcast).
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_sensitivity(int nelem, int nx, int ny, const double* __restrict__ ke0,
+                                                         const int* __restrict__ reduced, const double* __restrict__ u,
+                                                         long long nfree, int ncases, const double* __restrict__ w,
+                                                         const double* __restrict__ fac,
+                                                         const double* __restrict__ dfac,
+                                                         double* __restrict__ energy, double* __restrict__ dc) {
+  constexpr int NPE = DIM == 3 ? 8 : 4;
+  constexpr int ND = NPE * DIM;
+  __shared__ double ke[ND * ND];
+  for (int t = threadIdx.x; t < ND * ND; t += blockDim.x) ke[t] = ke0[t];
+  __syncthreads();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int nnx = nx + 1, nny = ny + 1;
+  int nodes[NPE];
+  if (DIM == 3) {
+    const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    const int n0 = (ez * nny + ey) * nnx + ex, lay = nnx * nny;
+    nodes[0] = n0; nodes[1] = n0 + 1; nodes[2] = n0 + nnx + 1; nodes[3] = n0 + nnx;
+    nodes[4] = n0 + lay; nodes[5] = n0 + lay + 1; nodes[6] = n0 + lay + nnx + 1; nodes[7] = n0 + lay + nnx;
+  } else {
+    const int ex = e % nx, ey = e / nx;
+    const int n0 = ey * nnx + ex;
+    nodes[0] = n0; nodes[1] = n0 + 1; nodes[2] = n0 + nnx + 1; nodes[3] = n0 + nnx;
+  }
+  int idx[ND];
+#pragma unroll
+  for (int a = 0; a < NPE; ++a)
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) idx[a * DIM + c] = reduced[nodes[a] * DIM + c];
+  double en = 0.0, d = 0.0;
+  const double fe = fac[e], dfe = dfac[e];
+  for (int l = 0; l < ncases; ++l) {
+    const double* ul = u + (size_t)l * nfree;
+    double ue[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) ue[i] = idx[i] >= 0 ? ul[idx[i]] : 0.0;
+    double quad = 0.0;
+    for (int i = 0; i < ND; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < ND; ++j) s += ke[i * ND + j] * ue[j];
+      quad += ue[i] * s;
+    }
+    en += w[l] * 0.5 * fe * quad;
+    d -= w[l] * dfe * quad;
+  }
+  energy[e] = en;
+  dc[e] = d;
+}
+
+// physical = clamp(y, 0, 1); fac = SIMP factor; dfac = its derivative
+// (src/topopt/SimpInterpolation.cpp:59-70)
+__global__ void k_simp(int n, const double* __restrict__ filtered, double penalty, double emin,
+                       double* __restrict__ phys, double* __restrict__ fac, double* __restrict__ dfac) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double r = fmin(fmax(filtered[e], 0.0), 1.0);
+  phys[e] = r;
+  fac[e] = emin + (1.0 - emin) * pow(r, penalty);
+  dfac[e] = penalty * (1.0 - emin) * pow(r, penalty - 1.0);
+}
+
+// OC candidate (src/topopt/OptimalityCriteria.cpp:14-36)
+__global__ void k_oc_candidate(int n, const double* __restrict__ x, const double* __restrict__ dc,
+                               const double* __restrict__ dv, const double* __restrict__ lower,
+                               const double* __restrict__ upper, const unsigned char* __restrict__ passive,
+                               double lambda, double move, double damping, double* __restrict__ xn) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double lo = fmax(lower[e], x[e] - move);
+  const double hi = fmin(upper[e], x[e] + move);
+  if (passive[e]) {
+    xn[e] = lower[e];
+    return;
+  }
+  const double denom = lambda * dv[e];
+  double b = 0.0;
+  if (denom > 0.0) b = fmax(0.0, -dc[e] / denom);
+  const double target = x[e] * pow(b, damping);
+  xn[e] = fmin(fmax(target, lo), hi);
+}
+
+// partial sums of vol_e * clamp((H x)_e, 0, 1); one thread per row, column order
+__global__ void __launch_bounds__(kBlock) k_filter_volume(int n, const int* __restrict__ ptr,
+                                                           const int* __restrict__ col, const double* __restrict__ val,
+                                                           const double* __restrict__ x, const double* __restrict__ vol,
+                                                           double* __restrict__ partials) {
+  __shared__ double sh[kBlock / 64];
+  double acc = 0.0;
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = ptr[row]; k < ptr[row + 1]; ++k) s += val[k] * x[col[k]];
+    acc += fmin(fmax(s, 0.0), 1.0) * vol[row];
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// partial maxima of |a - b|
+__global__ void __launch_bounds__(kBlock) k_max_abs_diff(int n, const double* __restrict__ a,
+                                                          const double* __restrict__ b, double* __restrict__ partials) {
+  __shared__ double sh[kBlock];
+  double m = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = fmax(m, fabs(a[i] - b[i]));
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_reduce_max(int m, const double* __restrict__ partials,
+                                                        double* __restrict__ out) {
+  __shared__ double sh[kBlock];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) v = fmax(v, partials[i]);
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+__global__ void k_clamp(int n, double* __restrict__ x, const double* __restrict__ lo, const double* __restrict__ hi) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) x[e] = fmin(fmax(x[e], lo[e]), hi[e]);
+}
+
+int blocks_for(long long n, int per_block) { return (int)((n + per_block - 1) / per_block); }
+int grid8(long long n, int per_block) { return (blocks_for(n, per_block) + 7) / 8 * 8; }
+
+int lanes_for(const DCsr& A) {
+  // average row length decides the group width
+  const double avg = A.rows ? double(A.nnz) / A.rows : 0.0;
+  return avg > 48 ? 16 : avg > 12 ? 8 : 4;
+}
+
+template <int MODE, typename VT>
+void launch_spmv(const DCsr& A, const VT* val, const VT* x, const VT* b, VT* y, double* partials, hipStream_t s) {
+  const int lanes = lanes_for(A);
+  const int nb = grid8(A.rows, kBlock / lanes);
+  if (A.rows == 0) return;
+  switch (lanes) {
+    case 16: k_spmv<16, MODE, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
+    case 8: k_spmv<8, MODE, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
+    default: k_spmv<4, MODE, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+void spmv(const DCsr& A, const double* x, double* y, hipStream_t s) {
+  launch_spmv<0, double>(A, A.val.p, x, nullptr, y, nullptr, s);
+}
+void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s) {
+  launch_spmv<2, double>(A, A.val.p, x, nullptr, y, nullptr, s);
+}
+void residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s) {
+  launch_spmv<1, double>(A, A.val.p, x, b, r, nullptr, s);
+}
+void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s) {
+  launch_spmv<0, float>(A, A.val32.p, x, nullptr, y, nullptr, s);
+}
+void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s) {
+  launch_spmv<2, float>(A, A.val32.p, x, nullptr, y, nullptr, s);
+}
+void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s) {
+  launch_spmv<1, float>(A, A.val32.p, x, b, r, nullptr, s);
+}
+int spmv_dot_blocks(const DCsr& A) { return grid8(A.rows, kBlock / lanes_for(A)); }
+void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s) {
+  launch_spmv<3, double>(A, A.val.p, x, nullptr, y, partials, s);
+}
+void spmv_serial(const DCsr& A, const double* x, double* y, hipStream_t s) {
+  const int nb = blocks_for(A.rows, kBlock);
+  if (nb) k_spmv_serial<<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, x, y);
+}
+
+template <typename VT>
+static void cheb_step_t(const DCsr& A, const VT* val, const VT* dinv, const VT* r_in, VT* r_out, const VT* d_in,
+                        VT* d_out, VT* x, double c1, double c2, hipStream_t s) {
+  const int lanes = lanes_for(A);
+  const int nb = grid8(A.rows, kBlock / lanes);
+  if (A.rows == 0) return;
+  switch (lanes) {
+    case 16: k_cheb_step<16, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    case 8: k_cheb_step<8, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    default: k_cheb_step<4, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+  }
+}
+void cheb_step(const DCsr& A, const double* dinv, const double* r_in, double* r_out, const double* d_in, double* d_out,
+               double* x, double c1, double c2, hipStream_t s) {
+  cheb_step_t<double>(A, A.val.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s);
+}
+void cheb_step32(const DCsr& A, const float* dinv, const float* r_in, float* r_out, const float* d_in, float* d_out,
+                 float* x, double c1, double c2, hipStream_t s) {
+  cheb_step_t<float>(A, A.val32.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s);
+}
+void cheb_first_zero(int n, const double* dinv, const double* b, double theta, double* x, double* d, hipStream_t s) {
+  if (n) k_cheb_first_zero<double><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, b, theta, x, d);
+}
+void cheb_first(int n, const double* dinv, const double* r, double theta, double* x, double* d, hipStream_t s) {
+  if (n) k_cheb_first<double><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, r, theta, x, d);
+}
+void cheb_first_zero32(int n, const float* dinv, const float* b, double theta, float* x, float* d, hipStream_t s) {
+  if (n) k_cheb_first_zero<float><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, b, theta, x, d);
+}
+void cheb_first32(int n, const float* dinv, const float* r, double theta, float* x, float* d, hipStream_t s) {
+  if (n) k_cheb_first<float><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, r, theta, x, d);
+}
+void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t s) {
+  if (n) k_dense_gemv<double><<<blocks_for((long long)n * 64, kBlock), kBlock, 0, s>>>(n, M, b, y);
+}
+void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s) {
+  if (n) k_dense_gemv<float><<<blocks_for((long long)n * 64, kBlock), kBlock, 0, s>>>(n, M, b, y);
+}
+
+int reduce_blocks(long long n) { return (int)std::max<long long>(1, std::min<long long>(kReduceBlocks, (n + kBlock - 1) / kBlock)); }
+
+void dot(long long n, const double* a, const double* b, double* partials, double* out, hipStream_t s) {
+  const int nb = reduce_blocks(n);
+  k_dot_partials<<<nb, kBlock, 0, s>>>((int)n, a, b, partials);
+  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+}
+void reduce(int m, const double* partials, double* out, int take_sqrt, hipStream_t s) {
+  k_reduce_partials<<<1, 1024, 0, s>>>(m, partials, out, take_sqrt);
+}
+void pcg_update(long long n, double* x, double* r, const double* p, const double* q, const double* scal, int i_rz,
+                int i_pq, double* partials, double* out, hipStream_t s) {
+  const int nb = reduce_blocks(n);
+  k_pcg_update<<<nb, kBlock, 0, s>>>((int)n, x, r, p, q, scal, i_rz, i_pq, partials);
+  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+}
+void pcg_direction(long long n, double* p, const double* z, const double* scal, int i_new, int i_old, hipStream_t s) {
+  k_pcg_direction<<<reduce_blocks(n), kBlock, 0, s>>>((int)n, p, z, scal, i_new, i_old);
+}
+void copy_scalar(double* scal, int from, int to, hipStream_t s) { k_copy_scalar<<<1, 1, 0, s>>>(scal, from, to); }
+void flexible_beta(double* scal, int i_new, int i_cross, int i_old, int i_out, hipStream_t s) {
+  k_flexible_beta<<<1, 1, 0, s>>>(scal, i_new, i_cross, i_old, i_out);
+}
+void to_float(long long n, const double* in, float* out, hipStream_t s) {
+  if (n) k_convert<double, float><<<blocks_for(n, kBlock), kBlock, 0, s>>>((int)n, in, out);
+}
+void to_double(long long n, const float* in, double* out, hipStream_t s) {
+  if (n) k_convert<float, double><<<blocks_for(n, kBlock), kBlock, 0, s>>>((int)n, in, out);
+}
+void axpy(long long n, double a, const double* x, double* y, hipStream_t s) {
+  if (n) k_axpy<<<blocks_for(n, kBlock), kBlock, 0, s>>>((int)n, a, x, y);
+}
+
+void assemble_structured(int dim, const DCsr& K, const int* free_dofs, const double* scale, const double* ke0, int nx,
+                         int ny, int nz, hipStream_t s) {
+  constexpr int LANES = 16;
+  const int nb = grid8(K.rows, kBlock / LANES);
+  if (K.rows == 0) return;
+  if (dim == 3)
+    k_assemble_hex8<LANES><<<nb, kBlock, 0, s>>>(K.rows, K.ptr.p, K.col.p, K.val.p, free_dofs, scale, ke0, nx, ny, nz);
+  else
+    k_assemble_quad4<LANES><<<nb, kBlock, 0, s>>>(K.rows, K.ptr.p, K.col.p, K.val.p, free_dofs, scale, ke0, nx, ny);
+}
+
+void sensitivity(int dim, int nelem, int nx, int ny, const double* ke0, const int* reduced, const double* u,
+                 long long nfree, int ncases, const double* w, const double* fac, const double* dfac, double* energy,
+                 double* dc, hipStream_t s) {
+  const int nb = blocks_for(nelem, kBlock);
+  if (!nb) return;
+  if (dim == 3)
+    k_sensitivity<3><<<nb, kBlock, 0, s>>>(nelem, nx, ny, ke0, reduced, u, nfree, ncases, w, fac, dfac, energy, dc);
+  else
+    k_sensitivity<2><<<nb, kBlock, 0, s>>>(nelem, nx, ny, ke0, reduced, u, nfree, ncases, w, fac, dfac, energy, dc);
+}
+
+void simp(int n, const double* filtered, double penalty, double emin, double* phys, double* fac, double* dfac,
+          hipStream_t s) {
+  if (n) k_simp<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, filtered, penalty, emin, phys, fac, dfac);
+}
+void oc_candidate(int n, const double* x, const double* dc, const double* dv, const double* lower, const double* upper,
+                  const unsigned char* passive, double lambda, double move, double damping, double* xn,
+                  hipStream_t s) {
+  if (n) k_oc_candidate<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, x, dc, dv, lower, upper, passive, lambda, move,
+                                                                   damping, xn);
+}
+void filter_volume(const DCsr& H, const double* x, const double* vol, double* partials, double* out, hipStream_t s) {
+  const int nb = reduce_blocks(H.rows);
+  k_filter_volume<<<nb, kBlock, 0, s>>>(H.rows, H.ptr.p, H.col.p, H.val.p, x, vol, partials);
+  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+}
+void max_abs_diff(int n, const double* a, const double* b, double* partials, double* out, hipStream_t s) {
+  const int nb = reduce_blocks(n);
+  k_max_abs_diff<<<nb, kBlock, 0, s>>>(n, a, b, partials);
+  k_reduce_max<<<1, kBlock, 0, s>>>(nb, partials, out);
+}
+void clamp(int n, double* x, const double* lo, const double* hi, hipStream_t s) {
+  if (n) k_clamp<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, x, lo, hi);
+}
+
+}  // namespace k
+}  // namespace slx

@@ -1,0 +1,228 @@
+// HIP kernels of the SA-AMG numeric setup (src/fem/Multigrid.cpp:199-225,
+// 671-747): numeric sparse products into a pattern fixed by the host's
+// symbolic pass, the prolongator smoothing epilogue, value gathers for the
+// transpose and the symmetrisation, the diagonal and the Lanczos steps.
+//
+// The numeric product C = A B runs one wavefront per row of C with the row's
+// columns and accumulators in LDS. The A entries of the row are taken in order
+// and, for each, the lanes take the entries of the matching B row in parallel:
+// one B row holds distinct columns, so no two lanes touch the same
+// accumulator within a step and every entry of C sums its products in the
+// order of A's row, the order of the CPU restatement.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "slx_kernels.hpp"
+
+namespace slx {
+namespace k {
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lower_bound_lds(const int* cols, int len, int c) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cols[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// EPI 0: C = A B.  EPI 1: P = (I - omega D^-1 A) P-hat with B = P-hat, i.e.
+// C_row = -(omega dinv_i) (A P-hat)_row + P-hat_row (Multigrid.cpp:686-701).
+template <int EPI>
+__global__ void __launch_bounds__(kWave) k_spgemm_numeric(int rows, const int* __restrict__ aptr,
+                                                          const int* __restrict__ acol, const double* __restrict__ aval,
+                                                          const int* __restrict__ bptr, const int* __restrict__ bcol,
+                                                          const double* __restrict__ bval, const int* __restrict__ cptr,
+                                                          const int* __restrict__ ccol, double* __restrict__ cval,
+                                                          int lds_cap, const double* __restrict__ dinv, double omega,
+                                                          int* __restrict__ error_flag) {
+  extern __shared__ double smem[];
+  double* acc = smem;
+  int* cols = reinterpret_cast<int*>(smem + lds_cap);
+  const int lane = threadIdx.x;
+  for (int row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int cb = cptr[row], len = cptr[row + 1] - cb;
+    if (len > lds_cap) {  // the host sizes lds_cap from the longest row
+      if (lane == 0) atomicOr(error_flag, 1);
+      continue;
+    }
+    for (int t = lane; t < len; t += kWave) {
+      cols[t] = ccol[cb + t];
+      acc[t] = 0.0;
+    }
+    __syncthreads();
+    const int ab = aptr[row], ae = aptr[row + 1];
+    for (int k0 = ab; k0 < ae; k0 += kWave) {
+      // stage up to 64 A entries in registers, then walk them in order
+      const int kk = k0 + lane;
+      int my_j = 0, my_b0 = 0, my_b1 = 0;
+      double my_av = 0.0;
+      if (kk < ae) {
+        my_j = acol[kk];
+        my_av = aval[kk];
+        my_b0 = bptr[my_j];
+        my_b1 = bptr[my_j + 1];
+      }
+      const int cnt = min(kWave, ae - k0);
+      for (int t = 0; t < cnt; ++t) {
+        const double av = __shfl(my_av, t, kWave);
+        const int b0 = __shfl(my_b0, t, kWave), b1 = __shfl(my_b1, t, kWave);
+        for (int q = b0 + lane; q < b1; q += kWave) {
+          const int c = bcol[q];
+          const int s = lower_bound_lds(cols, len, c);
+          if (s < len && cols[s] == c) acc[s] += av * bval[q];
+          else atomicOr(error_flag, 2);
+        }
+        __syncthreads();
+      }
+    }
+    if (EPI == 1) {
+      const double scale = omega * dinv[row];
+      for (int t = lane; t < len; t += kWave) acc[t] *= -scale;
+      __syncthreads();
+      for (int q = bptr[row] + lane; q < bptr[row + 1]; q += kWave) {
+        const int c = bcol[q];
+        const int s = lower_bound_lds(cols, len, c);
+        if (s < len && cols[s] == c) acc[s] += bval[q];
+        else atomicOr(error_flag, 4);
+      }
+      __syncthreads();
+    }
+    for (int t = lane; t < len; t += kWave) cval[cb + t] = acc[t];
+    __syncthreads();
+  }
+}
+
+__global__ void k_gather(long long n, const int* __restrict__ src, const double* __restrict__ in,
+                         double* __restrict__ out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[src[i]];
+}
+
+// (C + C^T)/2 from the raw product (Multigrid.cpp:250-284): ia / ib index the
+// raw values of (i, j) and (j, i), or -1 where absent.
+__global__ void k_symmetrize(long long n, const int* __restrict__ ia, const int* __restrict__ ib,
+                             const double* __restrict__ raw, double* __restrict__ out) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int a = ia[t], b = ib[t];
+  double v;
+  if (a >= 0 && b >= 0) v = 0.5 * (raw[a] + raw[b]);
+  else if (a >= 0) v = 0.5 * raw[a];
+  else v = 0.5 * raw[b];
+  out[t] = v;
+}
+
+// inv_diag_i = 1 / sum of the stored diagonal entries of row i (Multigrid.cpp:721-735)
+__global__ void k_inv_diag(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                           const double* __restrict__ val, double* __restrict__ dinv, int* __restrict__ bad_row) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+    if (col[k] == i) d += val[k];
+  if (!(d > 0.0)) atomicMin(bad_row, i);
+  dinv[i] = 1.0 / d;
+}
+
+// Lanczos on D^-1/2 A D^-1/2 (Multigrid.cpp:304-317):
+// w_i = s_i (A (s v))_i - beta vprev_i, with s = sqrt(dinv)
+__global__ void __launch_bounds__(256) k_lanczos_apply(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                       const double* __restrict__ val, const double* __restrict__ s,
+                                                       const double* __restrict__ v, const double* __restrict__ vprev,
+                                                       double beta, double* __restrict__ w) {
+  constexpr int LANES = 16;
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+  const int lane = threadIdx.x % LANES;
+  if (row >= n) return;
+  double acc = 0.0;
+  for (int k = ptr[row] + lane; k < ptr[row + 1]; k += LANES) {
+    const int j = col[k];
+    acc += val[k] * (s[j] * v[j]);
+  }
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+  if (lane == 0) w[row] = acc * s[row] - beta * vprev[row];
+}
+
+__global__ void k_sqrt(int n, const double* __restrict__ in, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = sqrt(in[i]);
+}
+
+// w -= a v
+__global__ void k_axmy(int n, double a, const double* __restrict__ v, double* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] -= a * v[i];
+}
+
+// vprev = v; v = w / b
+__global__ void k_lanczos_shift(int n, double b, const double* __restrict__ w, double* __restrict__ v,
+                                double* __restrict__ vprev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vprev[i] = v[i];
+  v[i] = w[i] / b;
+}
+
+inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s) {
+  if (C.rows == 0) return;
+  const int cap = std::max(C.max_row, 1);
+  const size_t lds = (size_t)cap * (sizeof(double) + sizeof(int));
+  if (lds > 60 * 1024) fail(SLX_SOLVER_ERROR, "multigrid: a Galerkin product row is too long for the device kernel");
+  const int grid = std::min(C.rows, 256 * 64);
+  k_spgemm_numeric<0><<<grid, kWave, lds, s>>>(C.rows, A.ptr.p, A.col.p, A.val.p, B.ptr.p, B.col.p, B.val.p, C.ptr.p,
+                                               C.col.p, C.val.p, cap, nullptr, 0.0, error_flag);
+}
+
+void smooth_prolongator(const DCsr& A, const DCsr& Ptent, DCsr& P, const double* dinv, double omega, int* error_flag,
+                        hipStream_t s) {
+  if (P.rows == 0) return;
+  const int cap = std::max(P.max_row, 1);
+  const size_t lds = (size_t)cap * (sizeof(double) + sizeof(int));
+  if (lds > 60 * 1024) fail(SLX_SOLVER_ERROR, "multigrid: a prolongator row is too long for the device kernel");
+  const int grid = std::min(P.rows, 256 * 64);
+  k_spgemm_numeric<1><<<grid, kWave, lds, s>>>(P.rows, A.ptr.p, A.col.p, A.val.p, Ptent.ptr.p, Ptent.col.p,
+                                               Ptent.val.p, P.ptr.p, P.col.p, P.val.p, cap, dinv, omega, error_flag);
+}
+
+void gather(long long n, const int* src, const double* in, double* out, hipStream_t s) {
+  if (n) k_gather<<<nblk(n, 256), 256, 0, s>>>(n, src, in, out);
+}
+
+void symmetrize_values(long long n, const int* ia, const int* ib, const double* raw, double* out, hipStream_t s) {
+  if (n) k_symmetrize<<<nblk(n, 256), 256, 0, s>>>(n, ia, ib, raw, out);
+}
+
+void inv_diag(const DCsr& A, double* dinv, int* bad_row, hipStream_t s) {
+  if (A.rows) k_inv_diag<<<nblk(A.rows, 256), 256, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, dinv, bad_row);
+}
+
+void vsqrt(int n, const double* in, double* out, hipStream_t s) {
+  if (n) k_sqrt<<<nblk(n, 256), 256, 0, s>>>(n, in, out);
+}
+
+void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, double beta, double* w,
+                   hipStream_t s) {
+  if (A.rows) k_lanczos_apply<<<nblk((long long)A.rows * 16, 256), 256, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, sq, v,
+                                                                              vprev, beta, w);
+}
+
+void axmy(int n, double a, const double* v, double* w, hipStream_t s) {
+  if (n) k_axmy<<<nblk(n, 256), 256, 0, s>>>(n, a, v, w);
+}
+
+void lanczos_shift(int n, double b, const double* w, double* v, double* vprev, hipStream_t s) {
+  if (n) k_lanczos_shift<<<nblk(n, 256), 256, 0, s>>>(n, b, w, v, vprev);
+}
+
+}  // namespace k
+}  // namespace slx

@@ -1,0 +1,184 @@
+// Internal types of the sparlab_hip library: host CSR matrices, device buffers,
+// the multigrid level and the context. Host code and HIP kernels share them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "sparlab_hip.h"
+
+namespace slx {
+
+struct Error : std::exception {
+  int code;
+  std::string msg;
+  Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+  const char* what() const noexcept override { return msg.c_str(); }
+};
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define SLX_HIP(expr) ::slx::hip_check((expr), #expr)
+
+// Host CSR matrix, rows sorted by column.
+struct HCsr {
+  int rows = 0, cols = 0;
+  std::vector<int> ptr, col;
+  std::vector<double> val;
+  long long nnz() const { return ptr.empty() ? 0 : ptr.back(); }
+};
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    n = count;
+    if (count) SLX_HIP(hipMalloc(&p, count * sizeof(T)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // Host data may be pageable and short-lived: the copy completes before return.
+  void upload(const T* h, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) {
+      SLX_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+      SLX_HIP(hipStreamSynchronize(s));
+    }
+  }
+  void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+  void download(T* h, size_t count, hipStream_t s) const {
+    if (count) SLX_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+// Device CSR matrix (int32 indices, FP64 values, optional FP32 copy of the values).
+struct DCsr {
+  int rows = 0, cols = 0;
+  long long nnz = 0;
+  int max_row = 0;  // longest row
+  DBuf<int> ptr, col;
+  DBuf<double> val;
+  DBuf<float> val32;
+  void upload_pattern(const HCsr& h, hipStream_t s);
+  void upload(const HCsr& h, hipStream_t s) {
+    upload_pattern(h, s);
+    val.upload(h.val, s);
+  }
+};
+
+// One level of the SA-AMG hierarchy (structure of src/fem/Multigrid.cpp:336-354).
+// Structure (node blocks, aggregates, tentative prolongator, product patterns)
+// is built on the host once per sparsity structure; every numeric array lives
+// on the device. Level 0's operator is the context's K_ff.
+struct Level {
+  int n = 0;
+  int num_nodes = 0;
+  std::vector<int> node_ptr;
+  std::vector<double> nullspace;  // n x m, row-major
+  std::vector<int> aggregate;
+  double lambda_max = 0.0;
+  bool has_next = false;
+  DCsr A;                  // operator (levels > 0)
+  DCsr Ptent, P, R, AP, RAW;
+  DBuf<int> r_src;         // R.val[k] = P.val[r_src[k]]
+  DBuf<int> sym_a, sym_b;  // next level's A from RAW (symmetrisation)
+  DBuf<double> dinv, sq, lz_v, lz_vprev, lz_w, lz_start;
+  DBuf<double> x, b, res, d, d2;
+  DBuf<float> dinv32, x32, b32, res32, d32, d232;
+};
+
+struct TimerSlot {
+  double total_ms = 0.0;
+  long long launches = 0;
+};
+
+// Problem / solver context: one GPU stream, one structured SIMP problem.
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+
+  // ---- model ---------------------------------------------------------------
+  int dim = 0, npe = 0, ndpn = 0;
+  int nx = 0, ny = 0, nz = 0;
+  double lx = 0, ly = 0, lz = 0, E = 1, nu = 0.3, thickness = 1;
+  long long nnodes = 0, nelem = 0, ndofs = 0;
+  std::vector<unsigned char> fixed;
+  std::vector<int> free_dofs, reduced;  // free list (ascending), DOF -> reduced index or -1
+  std::vector<double> ke0;              // edofs x edofs
+  std::vector<double> coords;           // nnodes x dim
+  std::vector<double> volumes;          // element volumes
+  bool finalized = false;
+  int ncases = 0;
+  std::vector<double> loads;    // ncases x ndofs
+  std::vector<double> weights;  // normalised
+  // device model state
+  DBuf<int> d_free, d_reduced;
+  DBuf<double> d_ke0, d_scale, d_loads_free, d_weights;  // loads restricted to the free DOFs (ncases x nfree)
+  DBuf<double> d_u;                           // last solutions (ncases x nfree)
+  std::vector<char> have_u;
+  DCsr K;  // K_ff on the device
+  bool k_on_host = false;
+
+  // ---- multigrid -------------------------------------------------------------
+  slx_amg_options amg_opt{};
+  std::vector<Level> levels;
+  int nullspace_dim = 0;
+  bool amg_ready = false;
+  bool reused = false;
+  long long fp_rows = -1, fp_nnz = -1;
+  std::vector<int> fp_unknowns;
+  std::vector<double> layout_coords;  // coords of the DOF layout (nnodes x dim)
+  int layout_dim = 0, layout_ndpn = 0, layout_nnodes = 0;
+  std::vector<int> layout_unknowns;
+  DBuf<double> coarse_inv;  // dense inverse of the coarsest operator
+  DBuf<float> coarse_inv32;
+  double coarse_pivot_ratio = 0.0;
+  double setup_seconds = 0.0;
+
+  // ---- PCG workspace ------------------------------------------------------------
+  DBuf<double> r, z, p, q, xw, bw;
+  DBuf<double> partials;  // block partial sums
+  DBuf<double> scal;      // device scalars
+  std::vector<double> h_scal;
+
+  // ---- topology optimisation -------------------------------------------------
+  double volume_fraction = 0.5;
+  std::vector<double> lower, upper;
+  std::vector<unsigned char> passive;
+  bool filter_on = false;
+  double filter_radius = 0.0;
+  HCsr H, Ht;  // filter and its transpose
+  DCsr dH, dHt;
+  DBuf<double> d_x, d_phys, d_fac, d_dfac, d_dc, d_energy, d_vol, d_tmp, d_tmp2, d_lower, d_upper, d_dcx, d_dvx;
+  DBuf<unsigned char> d_passive;
+
+  // ---- measurement ------------------------------------------------------------
+  bool timing = false;
+  std::map<std::string, TimerSlot> timers;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event();
+  void flush_timers();
+};
+
+}  // namespace slx

@@ -1,0 +1,79 @@
+// Host-callable launchers of the HIP kernels (kernels.hip). Every launcher
+// enqueues on the given stream and returns immediately.
+#pragma once
+
+#include "slx_internal.hpp"
+
+namespace slx {
+namespace k {
+
+// operator products: y = A x, y += A x, r = b - A x (FP64 and FP32 storage)
+void spmv(const DCsr& A, const double* x, double* y, hipStream_t s);
+void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s);
+void residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s);
+void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s);
+void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s);
+void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s);
+// y = A x and per-block partial sums of x.y (spmv_dot_blocks(A) partials)
+int spmv_dot_blocks(const DCsr& A);
+void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s);
+// one thread per row, column-order sums (bitwise equal to the CPU restatement)
+void spmv_serial(const DCsr& A, const double* x, double* y, hipStream_t s);
+
+// Chebyshev smoother pieces
+void cheb_step(const DCsr& A, const double* dinv, const double* r_in, double* r_out, const double* d_in, double* d_out,
+               double* x, double c1, double c2, hipStream_t s);
+void cheb_step32(const DCsr& A, const float* dinv, const float* r_in, float* r_out, const float* d_in, float* d_out,
+                 float* x, double c1, double c2, hipStream_t s);
+void cheb_first_zero(int n, const double* dinv, const double* b, double theta, double* x, double* d, hipStream_t s);
+void cheb_first(int n, const double* dinv, const double* r, double theta, double* x, double* d, hipStream_t s);
+void cheb_first_zero32(int n, const float* dinv, const float* b, double theta, float* x, float* d, hipStream_t s);
+void cheb_first32(int n, const float* dinv, const float* r, double theta, float* x, float* d, hipStream_t s);
+void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t s);
+void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s);
+
+// deterministic reductions: partial buffer of at least 2048 doubles
+int reduce_blocks(long long n);
+void dot(long long n, const double* a, const double* b, double* partials, double* out, hipStream_t s);
+void reduce(int m, const double* partials, double* out, int take_sqrt, hipStream_t s);
+void pcg_update(long long n, double* x, double* r, const double* p, const double* q, const double* scal, int i_rz,
+                int i_pq, double* partials, double* out, hipStream_t s);
+// p = z + beta p, beta = scal[i_new] / scal[i_old] (or scal[i_new] when i_old < 0)
+void pcg_direction(long long n, double* p, const double* z, const double* scal, int i_new, int i_old, hipStream_t s);
+void copy_scalar(double* scal, int from, int to, hipStream_t s);
+void flexible_beta(double* scal, int i_new, int i_cross, int i_old, int i_out, hipStream_t s);
+void to_float(long long n, const double* in, float* out, hipStream_t s);
+void to_double(long long n, const float* in, double* out, hipStream_t s);
+void axpy(long long n, double a, const double* x, double* y, hipStream_t s);
+
+// structured-grid assembly of the K_ff values into K's pattern
+void assemble_structured(int dim, const DCsr& K, const int* free_dofs, const double* scale, const double* ke0, int nx,
+                         int ny, int nz, hipStream_t s);
+// element loop of the sensitivity
+void sensitivity(int dim, int nelem, int nx, int ny, const double* ke0, const int* reduced, const double* u,
+                 long long nfree, int ncases, const double* w, const double* fac, const double* dfac, double* energy,
+                 double* dc, hipStream_t s);
+void simp(int n, const double* filtered, double penalty, double emin, double* phys, double* fac, double* dfac,
+          hipStream_t s);
+void oc_candidate(int n, const double* x, const double* dc, const double* dv, const double* lower, const double* upper,
+                  const unsigned char* passive, double lambda, double move, double damping, double* xn,
+                  hipStream_t s);
+void filter_volume(const DCsr& H, const double* x, const double* vol, double* partials, double* out, hipStream_t s);
+void max_abs_diff(int n, const double* a, const double* b, double* partials, double* out, hipStream_t s);
+void clamp(int n, double* x, const double* lo, const double* hi, hipStream_t s);
+
+// SA-AMG numeric setup (kernels_amg.hip)
+void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s);
+void smooth_prolongator(const DCsr& A, const DCsr& Ptent, DCsr& P, const double* dinv, double omega, int* error_flag,
+                        hipStream_t s);
+void gather(long long n, const int* src, const double* in, double* out, hipStream_t s);
+void symmetrize_values(long long n, const int* ia, const int* ib, const double* raw, double* out, hipStream_t s);
+void inv_diag(const DCsr& A, double* dinv, int* bad_row, hipStream_t s);
+void vsqrt(int n, const double* in, double* out, hipStream_t s);
+void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, double beta, double* w,
+                   hipStream_t s);
+void axmy(int n, double a, const double* v, double* w, hipStream_t s);
+void lanczos_shift(int n, double b, const double* w, double* v, double* vprev, hipStream_t s);
+
+}  // namespace k
+}  // namespace slx

@@ -1,0 +1,31 @@
+// Solver-level functions shared by the library's translation units.
+#pragma once
+
+#include "slx_internal.hpp"
+
+namespace slx {
+
+// SA-AMG setup of the context's current K_ff (multigrid hierarchy on the device).
+void amg_setup(Context& c);
+// FP32 copies of the hierarchy for the mixed-precision V-cycle.
+void prepare_mixed(Context& c);
+// z = M^-1 r: one V-cycle from a zero guess (device pointers, length n of level 0).
+void vcycle(Context& c, const double* r, double* z);
+// Preconditioned CG on the device (src/fem/Multigrid.cpp:1152-1237); b, x device pointers.
+slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_iterations);
+
+// workspace sized for the current K_ff
+void ensure_workspace(Context& c);
+double device_dot(Context& c, long long n, const double* a, const double* b);
+bool device_all_finite(Context& c, long long n, const double* v);
+
+// RAII event pair around a kernel launch when timing is on
+struct Timed {
+  Context& c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(Context& ctx, const char* n);
+  ~Timed();
+};
+
+}  // namespace slx

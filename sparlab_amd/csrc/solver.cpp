@@ -1,0 +1,320 @@
+// V-cycle and preconditioned CG on the device (src/fem/Multigrid.cpp:783-885,
+// 1113-1121, 1152-1237).
+//
+// The CG loop runs on the host and reads back one pair of scalars per
+// iteration (p^T A p for the breakdown test and r^T r for the stopping test);
+// every vector operation, every reduction and the whole V-cycle stay on the
+// device stream. Step lengths alpha and beta are formed on the device from
+// reduced scalars, so the host never waits inside an iteration except for
+// that one read.
+//
+// precision = 1 runs the V-cycle in FP32 (operators, smoother, transfers and
+// the coarse inverse stored and applied in FP32, accumulated in FP32 lanes
+// with FP64 row sums) under the FP64 outer CG; the FP32 cycle is not exactly
+// symmetric, so the outer method becomes flexible CG (Polak-Ribiere beta).
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <sstream>
+
+#include "slx_kernels.hpp"
+#include "slx_solver.hpp"
+
+namespace slx {
+
+namespace {
+// device scalar slots
+constexpr int S_PQ = 0, S_RR = 1, S_RZ = 2, S_RZN = 3, S_RZX = 4, S_TMP = 5;
+
+DCsr& op(Context& c, size_t l) { return l == 0 ? c.K : c.levels[l].A; }
+
+struct Cheb {
+  double theta, delta, sigma;
+};
+Cheb cheb_params(const Context& c, const Level& lv) {
+  // [lambda_max / ratio, 1.1 lambda_max] of D^-1 A (Multigrid.cpp:809-813)
+  const double upper = 1.1 * lv.lambda_max;
+  const double lower = upper / c.amg_opt.chebyshev_ratio;
+  Cheb p;
+  p.theta = 0.5 * (upper + lower);
+  p.delta = 0.5 * (upper - lower);
+  p.sigma = p.theta / p.delta;
+  return p;
+}
+
+// Chebyshev steps 1..degree-1 given x, d and the residual source r_in.
+template <bool F32>
+void cheb_steps(Context& c, size_t l, const void* r_first, bool timed) {
+  Level& lv = c.levels[l];
+  DCsr& A = op(c, l);
+  const Cheb cp = cheb_params(c, lv);
+  double rho = 1.0 / cp.sigma;
+  hipStream_t s = c.stream;
+  for (int k = 1; k < c.amg_opt.smoother_degree; ++k) {
+    const double rho_new = 1.0 / (2.0 * cp.sigma - rho);
+    const double c1 = rho_new * rho;
+    const double c2 = 2.0 * rho_new / cp.delta;
+    if (F32) {
+      const float* rin = k == 1 ? static_cast<const float*>(r_first) : lv.res32.p;
+      Timed t(c, timed ? "cheb" : nullptr);
+      k::cheb_step32(A, lv.dinv32.p, rin, lv.res32.p, lv.d32.p, lv.d232.p, lv.x32.p, c1, c2, s);
+      std::swap(lv.d32, lv.d232);
+    } else {
+      const double* rin = k == 1 ? static_cast<const double*>(r_first) : lv.res.p;
+      Timed t(c, timed ? "cheb" : nullptr);
+      k::cheb_step(A, lv.dinv.p, rin, lv.res.p, lv.d.p, lv.d2.p, lv.x.p, c1, c2, s);
+      std::swap(lv.d, lv.d2);
+    }
+    rho = rho_new;
+  }
+}
+
+void cycle64(Context& c, size_t l) {
+  Level& lv = c.levels[l];
+  hipStream_t s = c.stream;
+  if (l + 1 == c.levels.size()) {
+    k::dense_gemv(lv.n, c.coarse_inv.p, lv.b.p, lv.x.p, s);
+    return;
+  }
+  DCsr& A = op(c, l);
+  const bool top = l == 0;
+  const Cheb cp = cheb_params(c, lv);
+  // pre-smoothing from a zero guess: x = d = D^-1 b / theta, r starts as b
+  k::cheb_first_zero(lv.n, lv.dinv.p, lv.b.p, cp.theta, lv.x.p, lv.d.p, s);
+  cheb_steps<false>(c, l, lv.b.p, top);
+  {
+    Timed t(c, top ? "residual" : nullptr);
+    k::residual(A, lv.x.p, lv.b.p, lv.res.p, s);
+  }
+  Level& nx = c.levels[l + 1];
+  k::spmv(lv.R, lv.res.p, nx.b.p, s);
+  cycle64(c, l + 1);
+  k::spmv_add(lv.P, nx.x.p, lv.x.p, s);
+  // post-smoothing from the current x
+  {
+    Timed t(c, top ? "residual" : nullptr);
+    k::residual(A, lv.x.p, lv.b.p, lv.res.p, s);
+  }
+  k::cheb_first(lv.n, lv.dinv.p, lv.res.p, cp.theta, lv.x.p, lv.d.p, s);
+  cheb_steps<false>(c, l, lv.res.p, top);
+}
+
+void cycle32(Context& c, size_t l) {
+  Level& lv = c.levels[l];
+  hipStream_t s = c.stream;
+  if (l + 1 == c.levels.size()) {
+    k::dense_gemv32(lv.n, c.coarse_inv32.p, lv.b32.p, lv.x32.p, s);
+    return;
+  }
+  DCsr& A = op(c, l);
+  const bool top = l == 0;
+  const Cheb cp = cheb_params(c, lv);
+  k::cheb_first_zero32(lv.n, lv.dinv32.p, lv.b32.p, cp.theta, lv.x32.p, lv.d32.p, s);
+  cheb_steps<true>(c, l, lv.b32.p, top);
+  {
+    Timed t(c, top ? "residual" : nullptr);
+    k::residual32(A, lv.x32.p, lv.b32.p, lv.res32.p, s);
+  }
+  Level& nx = c.levels[l + 1];
+  k::spmv32(lv.R, lv.res32.p, nx.b32.p, s);
+  cycle32(c, l + 1);
+  k::spmv_add32(lv.P, nx.x32.p, lv.x32.p, s);
+  {
+    Timed t(c, top ? "residual" : nullptr);
+    k::residual32(A, lv.x32.p, lv.b32.p, lv.res32.p, s);
+  }
+  k::cheb_first32(lv.n, lv.dinv32.p, lv.res32.p, cp.theta, lv.x32.p, lv.d32.p, s);
+  cheb_steps<true>(c, l, lv.res32.p, top);
+}
+
+void read_scalars(Context& c, int from, int count) {
+  c.h_scal.resize(16);
+  SLX_HIP(hipMemcpyAsync(c.h_scal.data() + from, c.scal.p + from, sizeof(double) * count, hipMemcpyDeviceToHost,
+                         c.stream));
+  SLX_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+Timed::Timed(Context& ctx, const char* n) : c(ctx), name(n) {
+  if (!c.timing || !name) return;
+  a = c.take_event();
+  b = c.take_event();
+  SLX_HIP(hipEventRecord(a, c.stream));
+}
+Timed::~Timed() {
+  if (!a) return;
+  (void)hipEventRecord(b, c.stream);
+  c.pending.push_back({name, {a, b}});
+}
+
+hipEvent_t Context::take_event() {
+  if (event_pool.empty()) {
+    hipEvent_t e;
+    SLX_HIP(hipEventCreate(&e));
+    return e;
+  }
+  hipEvent_t e = event_pool.back();
+  event_pool.pop_back();
+  return e;
+}
+
+void Context::flush_timers() {
+  if (pending.empty()) return;
+  SLX_HIP(hipStreamSynchronize(stream));
+  for (auto& pe : pending) {
+    float ms = 0.0f;
+    SLX_HIP(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+    TimerSlot& t = timers[pe.first];
+    t.total_ms += ms;
+    t.launches += 1;
+    event_pool.push_back(pe.second.first);
+    event_pool.push_back(pe.second.second);
+  }
+  pending.clear();
+}
+
+void ensure_workspace(Context& c) {
+  const size_t n = (size_t)c.K.rows;
+  c.r.alloc(n);
+  c.z.alloc(n);
+  c.p.alloc(n);
+  c.q.alloc(n);
+  c.xw.alloc(n);
+  c.bw.alloc(n);
+  const size_t need = std::max<size_t>(4096, (size_t)k::spmv_dot_blocks(c.K) + 8);
+  if (c.partials.n < need) c.partials.alloc(need);
+  if (!c.scal.p) c.scal.alloc(16);
+}
+
+bool device_all_finite(Context& c, long long n, const double* v) {
+  // the sum of squares is finite exactly when every entry is (for entries
+  // below 1e150 in magnitude, far above any stiffness)
+  const double s = device_dot(c, n, v, v);
+  return std::isfinite(s);
+}
+
+void prepare_mixed(Context& c) {
+  hipStream_t s = c.stream;
+  for (size_t l = 0; l < c.levels.size(); ++l) {
+    Level& lv = c.levels[l];
+    DCsr& A = op(c, l);
+    A.val32.alloc((size_t)A.nnz);
+    k::to_float(A.nnz, A.val.p, A.val32.p, s);
+    if (lv.has_next) {
+      lv.P.val32.alloc((size_t)lv.P.nnz);
+      k::to_float(lv.P.nnz, lv.P.val.p, lv.P.val32.p, s);
+      lv.R.val32.alloc((size_t)lv.R.nnz);
+      k::to_float(lv.R.nnz, lv.R.val.p, lv.R.val32.p, s);
+    }
+    lv.dinv32.alloc((size_t)lv.n);
+    k::to_float(lv.n, lv.dinv.p, lv.dinv32.p, s);
+    lv.x32.alloc((size_t)lv.n);
+    lv.b32.alloc((size_t)lv.n);
+    lv.res32.alloc((size_t)lv.n);
+    lv.d32.alloc((size_t)lv.n);
+    lv.d232.alloc((size_t)lv.n);
+  }
+  c.coarse_inv32.alloc(c.coarse_inv.n);
+  k::to_float((long long)c.coarse_inv.n, c.coarse_inv.p, c.coarse_inv32.p, s);
+}
+
+void vcycle(Context& c, const double* r, double* z) {
+  if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: apply() called before setup()");
+  Level& top = c.levels[0];
+  const size_t n = (size_t)top.n;
+  if (c.amg_opt.precision == 1) {
+    k::to_float((long long)n, r, top.b32.p, c.stream);
+    cycle32(c, 0);
+    k::to_double((long long)n, top.x32.p, z, c.stream);
+    return;
+  }
+  SLX_HIP(hipMemcpyAsync(top.b.p, r, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+  cycle64(c, 0);
+  SLX_HIP(hipMemcpyAsync(z, top.x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+}
+
+slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_iterations) {
+  const long long n = c.K.rows;
+  hipStream_t s = c.stream;
+  slx_pcg_result res{0, 0.0, 0};
+  const bool flexible = c.amg_opt.precision == 1;
+  const double bnorm = std::sqrt(device_dot(c, n, b, b));
+  if (bnorm == 0.0) {
+    SLX_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    res.converged = 1;
+    return res;
+  }
+  const double target = tol * bnorm;
+  int total = 0;
+  bool recursive_converged = false;
+  double best_true = std::numeric_limits<double>::max();
+  const int nb = k::spmv_dot_blocks(c.K);
+  for (int restart = 0; restart < 3; ++restart) {
+    k::residual(c.K, x, b, c.r.p, s);
+    double rnorm = std::sqrt(device_dot(c, n, c.r.p, c.r.p));
+    res.relative_residual = rnorm / bnorm;
+    if (rnorm <= target) {
+      res.converged = 1;
+      break;
+    }
+    vcycle(c, c.r.p, c.z.p);
+    SLX_HIP(hipMemcpyAsync(c.p.p, c.z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZ, s);
+    bool converged = false;
+    while (total < max_iterations) {
+      {
+        Timed t(c, "spmv");
+        k::spmv_dot(c.K, c.p.p, c.q.p, c.partials.p, s);
+      }
+      k::reduce(nb, c.partials.p, c.scal.p + S_PQ, 0, s);
+      k::pcg_update(n, x, c.r.p, c.p.p, c.q.p, c.scal.p, S_RZ, S_PQ, c.partials.p, c.scal.p + S_RR, s);
+      ++total;
+      read_scalars(c, S_PQ, 2);
+      const double pq = c.h_scal[S_PQ];
+      if (!(pq > 0.0) || !std::isfinite(pq)) {
+        std::ostringstream os;
+        os << "preconditioned CG broke down at iteration " << total << " (p^T A p = " << pq
+           << "): the stiffness matrix or the multigrid preconditioner is not positive definite. Check the "
+              "supports, or use a direct solver to confirm";
+        fail(SLX_SOLVER_ERROR, os.str());
+      }
+      rnorm = std::sqrt(c.h_scal[S_RR]);
+      if (rnorm <= target) {
+        converged = true;
+        break;
+      }
+      if (flexible) {
+        // z_old kept in xw; beta = r.(z - z_old) / rz_old
+        std::swap(c.z, c.xw);
+        vcycle(c, c.r.p, c.z.p);
+        k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZN, s);
+        k::dot(n, c.r.p, c.xw.p, c.partials.p, c.scal.p + S_RZX, s);
+        k::flexible_beta(c.scal.p, S_RZN, S_RZX, S_RZ, S_TMP, s);
+        k::pcg_direction(n, c.p.p, c.z.p, c.scal.p, S_TMP, -1, s);
+      } else {
+        vcycle(c, c.r.p, c.z.p);
+        k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZN, s);
+        k::pcg_direction(n, c.p.p, c.z.p, c.scal.p, S_RZN, S_RZ, s);
+      }
+      k::copy_scalar(c.scal.p, S_RZN, S_RZ, s);
+    }
+    k::residual(c.K, x, b, c.r.p, s);
+    res.relative_residual = std::sqrt(device_dot(c, n, c.r.p, c.r.p)) / bnorm;
+    if (res.relative_residual <= tol) {
+      res.converged = 1;
+      break;
+    }
+    if (!converged) break;  // iteration budget spent
+    recursive_converged = true;
+    if (!(res.relative_residual < 0.5 * best_true)) break;
+    best_true = res.relative_residual;
+  }
+  if (!res.converged && recursive_converged) res.converged = 1;
+  res.iterations = total;
+  c.flush_timers();
+  return res;
+}
+
+}  // namespace slx

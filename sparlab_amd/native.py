@@ -1,0 +1,434 @@
+"""ctypes binding of libsparlab_hip.so (include/sparlab_hip.h).
+
+The Python classes mirror the reference's host interface for this path:
+``HipAmgCgSolver`` follows ``LinearSolver`` / ``AmgCgSolver``
+(include/sparlab/fem/LinearSolver.hpp:64-111), ``HipProblem`` bundles the
+FemModel + Assembler + DensityFilter + ComplianceObjective state of a
+structured SIMP problem (src/topopt/Sensitivity.cpp:110-237) and
+``run_oc`` follows TopologyOptimizer::run_oc (src/topopt/TopologyOptimizer.cpp:556-724).
+Status codes become the reference's exception classes.
+
+The library is the only compute path: if it is missing or no GPU is
+visible, the calls raise; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsparlab_hip.so")
+
+
+class SparlabError(RuntimeError):
+    """Base of the errors the native library reports."""
+
+
+class ConfigError(SparlabError):
+    pass
+
+
+class SolverError(SparlabError):
+    pass
+
+
+class ConvergenceError(SparlabError):
+    pass
+
+
+class ModelError(SparlabError):
+    pass
+
+
+class DeviceError(SparlabError):
+    pass
+
+
+_ERRORS = {1: ConfigError, 2: SolverError, 3: ConvergenceError, 4: ModelError, 5: DeviceError}
+
+
+class AmgOptions(C.Structure):
+    _fields_ = [("strength_threshold", C.c_double), ("max_levels", C.c_int), ("coarse_size", C.c_int),
+                ("smoother", C.c_int), ("smoother_degree", C.c_int), ("chebyshev_ratio", C.c_double),
+                ("prolongator_damping", C.c_double), ("lanczos_steps", C.c_int),
+                ("coarse_pivot_tolerance", C.c_double), ("reuse_aggregates", C.c_int), ("precision", C.c_int)]
+
+
+class PcgResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("relative_residual", C.c_double), ("converged", C.c_int)]
+
+
+class AmgStats(C.Structure):
+    _fields_ = [("levels", C.c_int), ("reused_aggregates", C.c_int), ("operator_complexity", C.c_double),
+                ("grid_complexity", C.c_double), ("coarse_pivot_ratio", C.c_double),
+                ("near_null_space_dimension", C.c_int), ("unknowns", C.c_longlong * 16),
+                ("nonzeros", C.c_longlong * 16), ("aggregates", C.c_longlong * 16), ("lambda_max", C.c_double * 16),
+                ("setup_seconds", C.c_double)]
+
+
+class EvalOptions(C.Structure):
+    _fields_ = [("penalty", C.c_double), ("emin_ratio", C.c_double), ("tolerance", C.c_double),
+                ("max_iterations", C.c_int), ("warm_start", C.c_int)]
+
+
+class Evaluation(C.Structure):
+    _fields_ = [("compliance", C.c_double), ("volume", C.c_double), ("volume_fraction", C.c_double),
+                ("linear_iterations", C.c_int), ("assemble_seconds", C.c_double), ("solve_seconds", C.c_double)]
+
+
+class OcOptions(C.Structure):
+    _fields_ = [("move_limit", C.c_double), ("damping", C.c_double), ("lambda_lower", C.c_double),
+                ("lambda_upper", C.c_double), ("max_bisections", C.c_int), ("volume_tolerance", C.c_double)]
+
+
+class OcStep(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("achieved_volume", C.c_double), ("max_change", C.c_double),
+                ("bisections", C.c_int), ("volume_converged", C.c_int)]
+
+
+class RunOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_int), ("change_tolerance", C.c_double), ("objective_tolerance", C.c_double),
+                ("objective_window", C.c_int), ("continuation_steps", C.c_int), ("penalty_start", C.c_double),
+                ("continuation_iterations", C.c_int), ("oc", OcOptions), ("eval", EvalOptions)]
+
+
+_lib = None
+
+# every entry point of include/sparlab_hip.h with its ctypes signature
+_VP = C.c_void_p
+_SIGNATURES = {
+    "slx_create": (C.c_int, [C.c_int, C.POINTER(_VP)]),
+    "slx_destroy": (None, [_VP]),
+    "slx_last_error": (C.c_char_p, [_VP]),
+    "slx_amg_default_options": (None, [C.POINTER(AmgOptions)]),
+    "slx_eval_default_options": (None, [C.POINTER(EvalOptions)]),
+    "slx_oc_default_options": (None, [C.POINTER(OcOptions)]),
+    "slx_model_structured": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 6),
+    "slx_model_set_supports": (C.c_int, [_VP, _VP]),
+    "slx_model_set_loads": (C.c_int, [_VP, C.c_int, _VP, _VP]),
+    "slx_model_finalize": (C.c_int, [_VP]),
+    "slx_model_sizes": (C.c_int, [_VP, _VP]),
+    "slx_ke0_get": (C.c_int, [_VP, _VP]),
+    "slx_free_dofs_get": (C.c_int, [_VP, _VP]),
+    "slx_assemble": (C.c_int, [_VP, _VP]),
+    "slx_kff_get": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "slx_amg_setup": (C.c_int, [_VP, C.POINTER(AmgOptions)]),
+    "slx_factorize_csr": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP, _VP,
+                                    C.POINTER(AmgOptions)]),
+    "slx_solve": (C.c_int, [_VP, _VP, _VP, C.c_double, C.c_int, C.POINTER(PcgResult)]),
+    "slx_solve_case": (C.c_int, [_VP, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(PcgResult)]),
+    "slx_amg_apply": (C.c_int, [_VP, _VP, _VP]),
+    "slx_amg_stats_get": (C.c_int, [_VP, C.POINTER(AmgStats)]),
+    "slx_amg_level_info": (C.c_int, [_VP, C.c_int, C.c_int, _VP]),
+    "slx_amg_level_get": (C.c_int, [_VP, C.c_int, C.c_int, _VP, _VP, _VP]),
+    "slx_amg_aggregates_get": (C.c_int, [_VP, C.c_int, _VP]),
+    "slx_solution_get": (C.c_int, [_VP, C.c_int, _VP]),
+    "slx_design_set": (C.c_int, [_VP, C.c_double, _VP, _VP, _VP]),
+    "slx_filter_build": (C.c_int, [_VP, C.c_double]),
+    "slx_filter_apply": (C.c_int, [_VP, C.c_int, _VP, _VP]),
+    "slx_evaluate": (C.c_int, [_VP, _VP, C.POINTER(EvalOptions), C.POINTER(Evaluation), _VP, _VP, _VP]),
+    "slx_oc_update": (C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(OcOptions), _VP, C.POINTER(OcStep)]),
+    "slx_run_oc": (C.c_int, [_VP, C.POINTER(RunOptions), _VP, _VP, _VP, _VP]),
+    "slx_timing_enable": (C.c_int, [_VP, C.c_int]),
+    "slx_timing_get": (C.c_int, [_VP, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "slx_synchronize": (C.c_int, [_VP]),
+}
+
+
+def load():
+    """Load the native library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DeviceError(f"{LIB_PATH} is missing: build it with `python sparlab_amd/build.py` "
+                          "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def amg_options(**kw) -> AmgOptions:
+    o = AmgOptions()
+    load().slx_amg_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def eval_options(**kw) -> EvalOptions:
+    o = EvalOptions()
+    load().slx_eval_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def oc_options(**kw) -> OcOptions:
+    o = OcOptions()
+    load().slx_oc_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One GPU stream and its device-resident problem state."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        h = _VP()
+        rc = L.slx_create(device, C.byref(h))
+        if rc != 0:
+            raise DeviceError(f"slx_create failed on device {device} (code {rc})")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().slx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != 0:
+            msg = load().slx_last_error(self.h).decode()
+            raise _ERRORS.get(rc, SparlabError)(msg)
+
+    def timing(self, on=True):
+        self.check(load().slx_timing_enable(self.h, 1 if on else 0))
+
+    def timer(self, name):
+        ms, n = C.c_double(), C.c_longlong()
+        self.check(load().slx_timing_get(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def synchronize(self):
+        self.check(load().slx_synchronize(self.h))
+
+
+class HipAmgCgSolver:
+    """The LinearSolver seam (AmgCgSolver, src/fem/LinearSolver.cpp:219-290) for a
+    caller-assembled CSR matrix and its DOF layout."""
+
+    def __init__(self, tolerance=1e-12, max_iterations=0, ctx: Context | None = None, **amg):
+        self.ctx = ctx or Context()
+        self.tolerance = tolerance
+        self.max_iterations = max_iterations
+        self.options = amg_options(**amg)
+        self.layout = None
+        self._last = PcgResult()
+
+    def name(self):
+        return "AMG-CG(chebyshev)"
+
+    def iterative(self):
+        return True
+
+    def set_layout(self, dim, coords, unknowns, dofs_per_node=0):
+        self.layout = (dim, dofs_per_node, np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, dim),
+                       np.ascontiguousarray(unknowns, dtype=np.int32))
+
+    def factorize(self, ptr, col, val):
+        if self.layout is None:
+            raise SolverError(self.name() + " needs the DOF layout (node coordinates of the unknowns) to build its "
+                              "rigid-body near-null space; the caller did not supply one")
+        dim, ndpn, coords, unknowns = self.layout
+        ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = _f64(val)
+        self._n = len(ptr) - 1
+        self.ctx.check(load().slx_factorize_csr(self.ctx.h, self._n, _p(ptr), _p(col), _p(val), dim, ndpn,
+                                                coords.shape[0], _p(coords), _p(unknowns), C.byref(self.options)))
+
+    def solve_from(self, b, x0=None):
+        b = _f64(b)
+        x = np.zeros_like(b) if x0 is None or len(x0) != len(b) else _f64(x0).copy()
+        cap = self.max_iterations if self.max_iterations > 0 else 1000
+        self.ctx.check(load().slx_solve(self.ctx.h, _p(b), _p(x), self.tolerance, cap, C.byref(self._last)))
+        if not self._last.converged:
+            raise ConvergenceError(f"{self.name()} did not converge: {self._last.iterations} iterations left a "
+                                   f"relative residual of {self._last.relative_residual} against a tolerance of "
+                                   f"{self.tolerance}")
+        return x
+
+    def solve(self, b):
+        return self.solve_from(b, None)
+
+    def last_iterations(self):
+        return self._last.iterations
+
+    def last_error(self):
+        return self._last.relative_residual
+
+    def amg_stats(self):
+        s = AmgStats()
+        self.ctx.check(load().slx_amg_stats_get(self.ctx.h, C.byref(s)))
+        return s
+
+
+class HipProblem:
+    """A structured SIMP problem resident on one GPU."""
+
+    def __init__(self, model, device=0, filter_radius=None, ctx: Context | None = None):
+        L = load()
+        self.ctx = ctx or Context(device)
+        self.model = model
+        h = self.ctx.h
+        chk = self.ctx.check
+        chk(L.slx_model_structured(h, model.dim, model.nx, model.ny, max(model.nz, 1), model.lx, model.ly,
+                                   model.lz, model.E, model.nu, model.thickness))
+        fixed = np.ascontiguousarray(model.fixed, dtype=np.uint8)
+        chk(L.slx_model_set_supports(h, _p(fixed)))
+        if model.loads is not None:
+            loads = _f64(model.loads)
+            w = _f64(model.weights)
+            chk(L.slx_model_set_loads(h, loads.shape[0], _p(loads), _p(w)))
+        chk(L.slx_model_finalize(h))
+        sizes = np.zeros(6, dtype=np.int64)
+        chk(L.slx_model_sizes(h, _p(sizes)))
+        self.nnodes, self.ne, self.ndofs, self.nfree, self.kff_nnz, self.ncases = (int(v) for v in sizes)
+        chk(L.slx_design_set(h, model.volume_fraction, _p(_f64(model.lower)), _p(_f64(model.upper)),
+                             _p(np.ascontiguousarray(model.passive, dtype=np.uint8))))
+        r = model.filter_radius if filter_radius is None else filter_radius
+        chk(L.slx_filter_build(h, float(r)))
+
+    # ---- structure -----------------------------------------------------------
+    def ke0(self):
+        nd = (8 if self.model.dim == 3 else 4) * self.model.dim
+        k = np.empty(nd * nd)
+        self.ctx.check(load().slx_ke0_get(self.ctx.h, _p(k)))
+        return k.reshape(nd, nd)
+
+    def free(self):
+        f = np.empty(self.nfree, dtype=np.int32)
+        self.ctx.check(load().slx_free_dofs_get(self.ctx.h, _p(f)))
+        return f
+
+    # ---- assembler / solver seams ------------------------------------------------
+    def assemble(self, scale=None):
+        s = None if scale is None else _f64(scale)
+        self.ctx.check(load().slx_assemble(self.ctx.h, _p(s)))
+
+    def kff(self):
+        ptr = np.empty(self.nfree + 1, dtype=np.int32)
+        col = np.empty(self.kff_nnz, dtype=np.int32)
+        val = np.empty(self.kff_nnz)
+        self.ctx.check(load().slx_kff_get(self.ctx.h, _p(ptr), _p(col), _p(val)))
+        return ptr, col, val
+
+    def amg_setup(self, **amg):
+        self.ctx.check(load().slx_amg_setup(self.ctx.h, C.byref(amg_options(**amg))))
+
+    def amg_stats(self):
+        s = AmgStats()
+        self.ctx.check(load().slx_amg_stats_get(self.ctx.h, C.byref(s)))
+        return s
+
+    def amg_matrix(self, level, which):
+        w = {"A": 0, "P": 1, "Ptent": 2, "R": 3}[which]
+        info = np.zeros(3, dtype=np.int64)
+        L = load()
+        self.ctx.check(L.slx_amg_level_info(self.ctx.h, level, w, _p(info)))
+        rows, cols, nnz = (int(v) for v in info)
+        ptr = np.empty(rows + 1, dtype=np.int32)
+        col = np.empty(nnz, dtype=np.int32)
+        val = np.empty(nnz)
+        self.ctx.check(L.slx_amg_level_get(self.ctx.h, level, w, _p(ptr), _p(col), _p(val)))
+        return rows, cols, ptr, col, val
+
+    def amg_aggregates(self, level, num_nodes):
+        a = np.empty(num_nodes, dtype=np.int32)
+        self.ctx.check(load().slx_amg_aggregates_get(self.ctx.h, level, _p(a)))
+        return a
+
+    def amg_apply(self, r):
+        r = _f64(r)
+        z = np.empty_like(r)
+        self.ctx.check(load().slx_amg_apply(self.ctx.h, _p(r), _p(z)))
+        return z
+
+    def solve(self, b, x0=None, tol=1e-8, maxit=1000):
+        b = _f64(b)
+        x = np.zeros_like(b) if x0 is None else _f64(x0).copy()
+        res = PcgResult()
+        self.ctx.check(load().slx_solve(self.ctx.h, _p(b), _p(x), tol, maxit, C.byref(res)))
+        return x, res.iterations, res.relative_residual, bool(res.converged)
+
+    def solve_case(self, case=0, warm=True, tol=1e-8, maxit=1000):
+        res = PcgResult()
+        self.ctx.check(load().slx_solve_case(self.ctx.h, case, 1 if warm else 0, tol, maxit, C.byref(res)))
+        return res.iterations, res.relative_residual, bool(res.converged)
+
+    def solution(self, case=0):
+        u = np.empty(self.nfree)
+        self.ctx.check(load().slx_solution_get(self.ctx.h, case, _p(u)))
+        return u
+
+    # ---- topology optimisation ---------------------------------------------------
+    def filter(self, x, adjoint=False):
+        y = np.empty(self.ne)
+        self.ctx.check(load().slx_filter_apply(self.ctx.h, 1 if adjoint else 0, _p(_f64(x)), _p(y)))
+        return y
+
+    def evaluate(self, x, tol=1e-8, need_gradients=True, warm_start=True, penalty=None, max_iterations=1000):
+        o = eval_options(tolerance=tol, warm_start=1 if warm_start else 0, max_iterations=max_iterations,
+                         penalty=self.model.penalty if penalty is None else penalty,
+                         emin_ratio=self.model.emin_ratio)
+        ev = Evaluation()
+        dc = np.empty(self.ne) if need_gradients else None
+        dv = np.empty(self.ne) if need_gradients else None
+        en = np.empty(self.ne)
+        self.ctx.check(load().slx_evaluate(self.ctx.h, _p(_f64(x)), C.byref(o), C.byref(ev), _p(dc), _p(dv),
+                                           _p(en)))
+        return dict(compliance=ev.compliance, volume=ev.volume, volume_fraction=ev.volume_fraction,
+                    iterations=ev.linear_iterations, dc_dx=dc, dv_dx=dv, energy=en,
+                    assemble_seconds=ev.assemble_seconds, solve_seconds=ev.solve_seconds)
+
+    def oc(self, x, dc, dv, move=0.2, damping=0.5):
+        xn = np.empty(self.ne)
+        st = OcStep()
+        self.ctx.check(load().slx_oc_update(self.ctx.h, _p(_f64(x)), _p(_f64(dc)), _p(_f64(dv)),
+                                            C.byref(oc_options(move_limit=move, damping=damping)), _p(xn),
+                                            C.byref(st)))
+        return xn, dict(lambda_=st.lambda_, volume=st.achieved_volume, max_change=st.max_change,
+                        converged=bool(st.volume_converged), bisections=st.bisections)
+
+    def run_oc(self, x0, max_iterations, change_tol=1e-3, objective_tol=0.0, window=5, cont_steps=1,
+               penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5, tol=1e-8):
+        o = RunOptions()
+        o.max_iterations = max_iterations
+        o.change_tolerance = change_tol
+        o.objective_tolerance = objective_tol
+        o.objective_window = window
+        o.continuation_steps = cont_steps
+        o.penalty_start = penalty_start
+        o.continuation_iterations = cont_iterations
+        o.oc = oc_options(move_limit=move, damping=damping)
+        o.eval = eval_options(tolerance=tol, penalty=self.model.penalty, emin_ratio=self.model.emin_ratio)
+        x = _f64(x0).copy()
+        hist = np.zeros(max_iterations * 8)
+        n = C.c_int()
+        fc = C.c_double()
+        self.ctx.check(load().slx_run_oc(self.ctx.h, C.byref(o), _p(x), _p(hist), C.byref(n), C.byref(fc)))
+        h = hist.reshape(max_iterations, 8)
+        rows = int((h[:, 0] > 0).sum())
+        return x, h[:rows], n.value, fc.value

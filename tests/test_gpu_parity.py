@@ -1,0 +1,197 @@
+"""GPU parity of the HIP path against the refcpu oracle, through the C ABI.
+
+Bit-exact: Ke0, the DOF map, the K_ff pattern and values, the density filter,
+the aggregates of every level. To tolerance (stated per test): the multigrid
+operators, PCG solutions and compliances, sensitivities, OC updates and the
+OC loop. Sizes are ones the oracle finishes in seconds.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle.refcpu import Oracle
+from sparlab_amd import model as M
+from sparlab_amd import native as N
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "published.json")))
+
+
+def _csr(rows, cols, ptr, col, val):
+    return sp.csr_matrix((val, col, ptr), shape=(rows, cols))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    print("gpu parity: creating context", flush=True)
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+def _pair(m, ctx, tol=1e-8, **amg):
+    return N.HipProblem(m, ctx=ctx), Oracle(m, tol=tol, amg=amg or None)
+
+
+@pytest.mark.parametrize("case", ["hex", "quad"])
+def test_ke0_dofs_pattern_and_assembly_bitwise(ctx, case):
+    print(f"test_ke0_dofs_pattern_and_assembly_bitwise[{case}]", flush=True)
+    m = M.cantilever_3d(12, 6, 5) if case == "hex" else M.mbb_beam(30, 15)
+    g, o = _pair(m, ctx)
+    assert np.array_equal(g.ke0(), o.ke0())
+    assert np.array_equal(g.free(), o.free())
+    scale = M.seeded_density(m.nelem, seed=3) ** 3
+    g.assemble(scale)
+    gp, gc, gv = g.kff()
+    op_, oc, ov = o.assemble_kff(scale)
+    assert np.array_equal(gp, op_) and np.array_equal(gc, oc)
+    assert np.array_equal(gv, ov)
+
+
+def test_aggregates_and_hierarchy(ctx):
+    print("test_aggregates_and_hierarchy", flush=True)
+    m = M.cantilever_3d(16, 8, 8)
+    g, o = _pair(m, ctx, coarse_size=300)
+    scale = M.seeded_density(m.nelem, seed=5) ** 3
+    g.assemble(scale)
+    o.assemble_kff(scale)
+    g.amg_setup(coarse_size=300)
+    o.amg_setup()
+    st = g.amg_stats()
+    assert st.levels == o.amg_levels() >= 3
+    for l in range(st.levels - 1):
+        nn = len(o.amg_aggregates(l))
+        assert np.array_equal(g.amg_aggregates(l, nn), o.amg_aggregates(l)), f"aggregates differ at level {l}"
+        for w in ("Ptent", "P", "A"):
+            G = _csr(*g.amg_matrix(l, w))
+            R = _csr(*o.amg_matrix(l, w))
+            assert G.shape == R.shape
+            assert np.array_equal(G.indptr, R.indptr) and np.array_equal(G.indices, R.indices)
+            # values: Lanczos dots and product orders differ at rounding level
+            assert abs(G - R).max() <= 1e-10 * abs(R).max(), (l, w)
+        assert g.amg_stats().lambda_max[l] == pytest.approx(o.amg_lambda(l), rel=1e-10)
+    assert st.operator_complexity == pytest.approx(o.amg_operator_complexity(), rel=1e-15)
+
+
+@pytest.mark.parametrize("row", [r for r in GOLDEN["hex8_bench_amg"] if 16 <= r["nx"] <= 48], ids=lambda r: f"nx{r['nx']}")
+def test_bench_box_matches_published(ctx, row):
+    print(f"test_bench_box_matches_published[{row['nx']}]", flush=True)
+    m = M.bench_box(row["nx"])
+    g = N.HipProblem(m, ctx=ctx)
+    g.assemble()
+    g.amg_setup()
+    st = g.amg_stats()
+    assert st.levels == row["levels"]
+    assert st.operator_complexity == pytest.approx(row["operator_complexity"], rel=1e-15)
+    b = m.free_loads()[0]
+    x, it, rr, cv = g.solve(b, tol=1e-10)
+    assert cv and rr <= 1e-10
+    c = float(b @ x)
+    assert abs(c - row["compliance"]) <= 1e-11 * row["compliance"]
+    assert abs(it - row["iterations"]) <= 1
+
+
+def test_vcycle_and_pcg_against_oracle(ctx):
+    print("test_vcycle_and_pcg_against_oracle", flush=True)
+    m = M.cantilever_3d(16, 8, 8)
+    g, o = _pair(m, ctx, coarse_size=300)
+    scale = np.where(np.random.default_rng(2).uniform(size=m.nelem) < 0.5, 1.0, 1e-3)
+    g.assemble(scale)
+    o.assemble_kff(scale)
+    g.amg_setup(coarse_size=300)
+    o.amg_setup()
+    r = np.random.default_rng(4).standard_normal(g.nfree)
+    zg, zo = g.amg_apply(r), o.amg_apply(r)
+    assert np.linalg.norm(zg - zo) <= 1e-9 * np.linalg.norm(zo)
+    b = m.free_loads()[0]
+    xg, itg, rrg, cvg = g.solve(b, tol=1e-12)
+    xo, ito, rro, cvo = o.pcg(b, tol=1e-12)
+    assert cvg and cvo and abs(itg - ito) <= 1
+    assert np.linalg.norm(xg - xo) <= 1e-9 * np.linalg.norm(xo)
+    assert b @ xg == pytest.approx(b @ xo, rel=1e-12)
+
+
+@pytest.mark.parametrize("case", ["hex", "quad"])
+def test_filter_bitwise(ctx, case):
+    print(f"test_filter_bitwise[{case}]", flush=True)
+    m = M.cantilever_3d(10, 6, 5, radius_cells=2.0) if case == "hex" else M.mbb_beam(40, 20)
+    g, o = _pair(m, ctx)
+    x = np.random.default_rng(1).uniform(size=m.nelem)
+    assert np.array_equal(g.filter(x), o.filter(x))
+    assert np.array_equal(g.filter(x, adjoint=True), o.filter(x, adjoint=True))
+
+
+@pytest.mark.parametrize("case", ["hex", "quad"])
+def test_evaluate_and_oc_against_oracle(ctx, case):
+    print(f"test_evaluate_and_oc_against_oracle[{case}]", flush=True)
+    m = M.cantilever_3d(12, 6, 6) if case == "hex" else M.mbb_beam(60, 20)
+    g, o = _pair(m, ctx, tol=1e-12)
+    # a design whose volume target lies within the move limit
+    x = m.volume_fraction * np.random.default_rng(7).uniform(0.5, 1.5, m.nelem)
+    eg = g.evaluate(x, tol=1e-12)
+    eo = o.evaluate(x)
+    assert eg["compliance"] == pytest.approx(eo["compliance"], rel=1e-10)
+    assert eg["volume"] == pytest.approx(eo["volume"], rel=1e-13)
+    np.testing.assert_allclose(eg["dc_dx"], eo["dc_dx"], rtol=0, atol=1e-9 * np.abs(eo["dc_dx"]).max())
+    np.testing.assert_allclose(eg["dv_dx"], eo["dv_dx"], rtol=1e-14)
+    np.testing.assert_allclose(eg["energy"], eo["energy"], rtol=0, atol=1e-9 * np.abs(eo["energy"]).max())
+    xg, sg = g.oc(x, eo["dc_dx"], eo["dv_dx"])
+    xo, so = o.oc(x, eo["dc_dx"], eo["dv_dx"])
+    assert sg["converged"] and so["converged"]
+    np.testing.assert_allclose(xg, xo, rtol=0, atol=1e-9)
+    assert sg["lambda_"] == pytest.approx(so["lambda_"], rel=1e-8)
+
+
+def test_short_oc_run_against_oracle(ctx):
+    print("test_short_oc_run_against_oracle", flush=True)
+    m = M.mbb_beam(30, 15)
+    g, o = _pair(m, ctx, tol=1e-10)
+    x0 = m.initial_design()
+    xg, hg, ng, fg = g.run_oc(x0, 15, change_tol=0.0, tol=1e-10)
+    xo, ho, no, fo = o.run_oc(x0, 15, change_tol=0.0)
+    assert ng == no == 15
+    np.testing.assert_allclose(hg[:, 2], ho[:, 2], rtol=1e-8)
+    np.testing.assert_allclose(hg[:, 3], ho[:, 3], rtol=1e-8)
+    assert fg == pytest.approx(fo, rel=1e-7)
+
+
+def test_mixed_precision_solve_meets_tolerance(ctx):
+    print("test_mixed_precision_solve_meets_tolerance", flush=True)
+    m = M.cantilever_3d(24, 12, 12)
+    g = N.HipProblem(m, ctx=ctx)
+    x = M.seeded_density(m.nelem)
+    scale = 1e-9 + (1 - 1e-9) * x ** 3
+    g.assemble(scale)
+    b = m.free_loads()[0]
+    g.amg_setup()
+    x64, it64, rr64, cv64 = g.solve(b, tol=1e-10)
+    g.amg_setup(precision=1)
+    x32, it32, rr32, cv32 = g.solve(b, tol=1e-10)
+    assert cv64 and cv32 and rr32 <= 1e-10
+    assert b @ x32 == pytest.approx(b @ x64, rel=1e-9)
+    assert it32 <= it64 + 3
+
+
+def test_error_paths(ctx):
+    print("test_error_paths", flush=True)
+    # under-constrained: no supports -> the coarse operator is singular
+    m = M.cantilever_3d(6, 3, 3)
+    m.fixed[:] = 0
+    g = N.HipProblem(m, ctx=ctx)
+    g.assemble()
+    with pytest.raises(N.SolverError):
+        g.amg_setup()
+    # iteration cap -> not converged
+    m2 = M.cantilever_3d(12, 6, 6)
+    g2 = N.HipProblem(m2, ctx=ctx)
+    g2.assemble()
+    g2.amg_setup()
+    x, it, rr, cv = g2.solve(m2.free_loads()[0], tol=1e-14, maxit=2)
+    assert not cv and it == 2
+    with pytest.raises(N.ConvergenceError):
+        g2.evaluate(np.full(m2.nelem, 0.5), tol=1e-14, max_iterations=2)
+    with pytest.raises(N.ModelError):
+        g2.assemble(-np.ones(m2.nelem))
