@@ -72,7 +72,19 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--size", type=str, default="128,64,64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference reading for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the path on a single MI355X and has no "
               f"multi-GPU mode (asked for --gpus {args.gpus}).", file=sys.stderr)
