@@ -117,6 +117,13 @@ def main():
         ev = step(w)
         print(f"bench: warmup {w}: c = {ev['compliance']:.6e}, {ev['iterations']} CG its", file=sys.stderr,
               flush=True)
+    st = g.amg_stats()
+    phases = {k: round(1e3 * getattr(st, k), 2) for k in ("setup_seconds", "check_seconds", "spectral_seconds",
+                                                          "aggregation_seconds", "smoothing_seconds",
+                                                          "galerkin_seconds", "coarse_seconds")}
+    print(f"bench: AMG setup phases (ms) {phases}, levels {st.levels}, unknowns "
+          f"{[st.unknowns[i] for i in range(st.levels)]}, node-product levels {st.node_product_levels}",
+          file=sys.stderr, flush=True)
     g.ctx.timing(True)
     g.ctx.synchronize()
     its = []

@@ -86,6 +86,14 @@ typedef struct {
   long long aggregates[16];
   double lambda_max[16];
   double setup_seconds;
+  /* setup phases (include/sparlab/fem/Multigrid.hpp:140-149) */
+  double check_seconds;
+  double spectral_seconds;
+  double aggregation_seconds;
+  double smoothing_seconds;
+  double galerkin_seconds;
+  double coarse_seconds;
+  int node_product_levels; /* levels whose products ran on the node-row kernels */
 } slx_amg_stats;
 
 /* Evaluation options: SIMP law (include/sparlab/topopt/SimpInterpolation.hpp:71-80)

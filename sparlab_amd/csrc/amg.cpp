@@ -505,6 +505,200 @@ void build_finest_nodes(Context& c, Level& lvl) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Node-row product metadata
+// ---------------------------------------------------------------------------
+std::vector<int> node_of_rows(const std::vector<int>& node_ptr) {
+  std::vector<int> of((size_t)node_ptr.back());
+  for (size_t nd = 0; nd + 1 < node_ptr.size(); ++nd)
+    for (int i = node_ptr[nd]; i < node_ptr[nd + 1]; ++i) of[(size_t)i] = (int)nd;
+  return of;
+}
+
+// all rows of each node share one column pattern
+bool node_uniform(const HCsr& m, const std::vector<int>& node_ptr) {
+  bool ok = true;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+  for (int nd = 0; nd < (int)node_ptr.size() - 1; ++nd) {
+    const int r0 = node_ptr[(size_t)nd];
+    const int l0 = m.ptr[(size_t)r0 + 1] - m.ptr[(size_t)r0];
+    for (int r = r0 + 1; r < node_ptr[(size_t)nd + 1] && ok; ++r) {
+      if (m.ptr[(size_t)r + 1] - m.ptr[(size_t)r] != l0) { ok = false; break; }
+      if (!std::equal(m.col.begin() + m.ptr[(size_t)r], m.col.begin() + m.ptr[(size_t)r + 1], m.col.begin() + m.ptr[(size_t)r0]))
+        ok = false;
+    }
+  }
+  return ok;
+}
+
+// the columns of every row come in complete runs of the column nodes
+bool runs_complete(const HCsr& m, const std::vector<int>& col_node_ptr, const std::vector<int>& col_node_of) {
+  bool ok = true;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+  for (int i = 0; i < m.rows; ++i) {
+    int k = m.ptr[(size_t)i];
+    while (k < m.ptr[(size_t)i + 1] && ok) {
+      const int j = m.col[(size_t)k];
+      const int J = col_node_of[(size_t)j];
+      const int j0 = col_node_ptr[(size_t)J], dj = col_node_ptr[(size_t)J + 1] - j0;
+      if (j != j0 || k + dj > m.ptr[(size_t)i + 1]) { ok = false; break; }
+      for (int q = 0; q < dj; ++q)
+        if (m.col[(size_t)k + q] != j0 + q) { ok = false; break; }
+      k += dj;
+    }
+  }
+  return ok;
+}
+
+// per row node: the aggregate blocks of its (shared) row pattern, with the
+// position of each block in the row; false if a block is not a complete
+// aggregate column range
+bool block_lists(const HCsr& m, const std::vector<int>& node_ptr, const std::vector<int>& agg_of_col,
+                 const std::vector<int>& agg_off, std::vector<int>& bptr, std::vector<int>& bagg,
+                 std::vector<int>& bpos) {
+  const int nn = (int)node_ptr.size() - 1;
+  bptr.assign((size_t)nn + 1, 0);
+  bagg.clear();
+  bpos.clear();
+  for (int nd = 0; nd < nn; ++nd) {
+    const int r0 = node_ptr[(size_t)nd];
+    int k = m.ptr[(size_t)r0];
+    const int k1 = m.ptr[(size_t)r0 + 1];
+    while (k < k1) {
+      const int c = m.col[(size_t)k];
+      const int a = agg_of_col[(size_t)c];
+      const int w = agg_off[(size_t)a + 1] - agg_off[(size_t)a];
+      if (c != agg_off[(size_t)a] || k + w > k1) return false;
+      for (int q = 0; q < w; ++q)
+        if (m.col[(size_t)k + q] != c + q) return false;
+      bagg.push_back(a);
+      bpos.push_back(k - m.ptr[(size_t)r0]);
+      k += w;
+    }
+    bptr[(size_t)nd + 1] = (int)bagg.size();
+  }
+  return true;
+}
+
+// lanes x slots needed by the widest node-row of c
+// runs of each node-row of m (its first row stands for all): per run the
+// offset in the row, the first B row, the number of B rows and the B node
+bool run_table(const HCsr& m, const std::vector<int>& node_ptr, const std::vector<int>& col_node_ptr,
+               const std::vector<int>& col_node_of, std::vector<int>& run_ptr, std::vector<int>& runs) {
+  const int nn = (int)node_ptr.size() - 1;
+  run_ptr.assign((size_t)nn + 1, 0);
+  runs.clear();
+  for (int nd = 0; nd < nn; ++nd) {
+    const int r0 = node_ptr[(size_t)nd];
+    const int k0 = m.ptr[(size_t)r0], k1 = m.ptr[(size_t)r0 + 1];
+    int k = k0;
+    while (k < k1) {
+      const int j = m.col[(size_t)k];
+      const int J = col_node_of[(size_t)j];
+      const int dj = col_node_ptr[(size_t)J + 1] - col_node_ptr[(size_t)J];
+      if (dj > 6) return false;
+      runs.insert(runs.end(), {k - k0, j, dj, J});
+      k += dj;
+    }
+    run_ptr[(size_t)nd + 1] = (int)(runs.size() / 4);
+  }
+  return true;
+}
+
+int max_blocks(const std::vector<int>& bptr) {
+  int mx = 0;
+  for (size_t i = 0; i + 1 < bptr.size(); ++i) mx = std::max(mx, bptr[i + 1] - bptr[i]);
+  return mx;
+}
+
+int slots_for(const HCsr& c, const std::vector<int>& node_ptr) {
+  long long mx = 0;
+  for (size_t nd = 0; nd + 1 < node_ptr.size(); ++nd) {
+    const int r0 = node_ptr[nd];
+    mx = std::max<long long>(mx, (long long)(node_ptr[nd + 1] - r0) * (c.ptr[(size_t)r0 + 1] - c.ptr[(size_t)r0]));
+  }
+  const long long ns = (mx + 63) / 64;
+  return ns <= 1 ? 1 : ns <= 2 ? 2 : ns <= 4 ? 4 : ns <= 8 ? 8 : 16;  // wider rows take several passes
+}
+
+void prepare_node_products(Level& lv, const Level& next, const HCsr& a, const HCsr& pt, const HCsr& ppat,
+                           const HCsr& rpat, const HCsr& appat, const HCsr& rawpat, hipStream_t s) {
+  lv.node_ok = false;
+  const std::vector<int> f_of = node_of_rows(lv.node_ptr);
+  const std::vector<int> c_of = node_of_rows(next.node_ptr);
+  const std::vector<int>& off = next.node_ptr;
+  if (!node_uniform(a, lv.node_ptr) || !node_uniform(ppat, lv.node_ptr) || !node_uniform(appat, lv.node_ptr) ||
+      !node_uniform(rpat, next.node_ptr) || !node_uniform(rawpat, next.node_ptr))
+    return;
+  if (!runs_complete(a, lv.node_ptr, f_of) || !runs_complete(rpat, lv.node_ptr, f_of)) return;
+  std::vector<int> tp, ta, tq, pp, pa, pq, qp, qa, qq;
+  if (!block_lists(pt, lv.node_ptr, c_of, off, tp, ta, tq) || !block_lists(ppat, lv.node_ptr, c_of, off, pp, pa, pq) ||
+      !block_lists(appat, lv.node_ptr, c_of, off, qp, qa, qq))
+    return;
+  std::vector<int> arp, ar, rrp, rr;
+  if (!run_table(a, lv.node_ptr, lv.node_ptr, f_of, arp, ar) || !run_table(rpat, next.node_ptr, lv.node_ptr, f_of, rrp, rr))
+    return;
+  lv.maxblk_p = max_blocks(pp);
+  lv.maxblk_ap = max_blocks(qp);
+  if (lv.maxblk_p > 1024 || lv.maxblk_ap > 1024 || max_blocks(tp) != 1) return;
+  lv.a_run_ptr.upload(arp, s);
+  lv.a_runs.upload(ar, s);
+  lv.r_run_ptr.upload(rrp, s);
+  lv.r_runs.upload(rr, s);
+  lv.ns_p = slots_for(ppat, lv.node_ptr);
+  lv.ns_ap = slots_for(appat, lv.node_ptr);
+  lv.ns_raw = slots_for(rawpat, next.node_ptr);
+  if (!lv.ns_p || !lv.ns_ap || !lv.ns_raw) return;
+  std::vector<int> row_agg((size_t)lv.n);
+  for (int i = 0; i < lv.n; ++i) row_agg[(size_t)i] = c_of[(size_t)pt.col[(size_t)pt.ptr[(size_t)i]]];
+  lv.f_node_ptr.upload(lv.node_ptr, s);
+  lv.f_node_of_row.upload(f_of, s);
+  lv.c_node_ptr.upload(next.node_ptr, s);
+  lv.c_node_of_row.upload(c_of, s);
+  lv.row_agg.upload(row_agg, s);
+  lv.pt_blk_ptr.upload(tp, s);
+  lv.pt_blk_agg.upload(ta, s);
+  lv.pt_blk_pos.upload(tq, s);
+  lv.p_blk_ptr.upload(pp, s);
+  lv.p_blk_agg.upload(pa, s);
+  lv.p_blk_pos.upload(pq, s);
+  lv.ap_blk_ptr.upload(qp, s);
+  lv.ap_blk_agg.upload(qa, s);
+  lv.ap_blk_pos.upload(qq, s);
+  lv.node_ok = true;
+}
+
+k::NodeProductArgs node_args(const Level& lv, const DCsr& A, bool a_is_r, const int* bblk_ptr, const int* bblk_agg,
+                             const int* bblk_pos, const DCsr& B, DCsr& C, int* flag) {
+  k::NodeProductArgs g{};
+  g.nodes = a_is_r ? (int)lv.c_node_ptr.n - 1 : (int)lv.f_node_ptr.n - 1;
+  g.out_node_ptr = a_is_r ? lv.c_node_ptr.p : lv.f_node_ptr.p;
+  g.aptr = A.ptr.p;
+  g.acol = A.col.p;
+  g.aval = A.val.p;
+  g.bnode_of_row = lv.f_node_of_row.p;
+  g.bnode_ptr = lv.f_node_ptr.p;
+  g.bblk_ptr = bblk_ptr;
+  g.bblk_agg = bblk_agg;
+  g.bblk_pos = bblk_pos;
+  g.bptr = B.ptr.p;
+  g.bval = B.val.p;
+  g.cptr = C.ptr.p;
+  g.ccol = C.col.p;
+  g.cval = C.val.p;
+  g.agg_of_col = lv.c_node_of_row.p;
+  g.agg_off = lv.c_node_ptr.p;
+  g.error_flag = flag;
+  g.run_ptr = a_is_r ? lv.r_run_ptr.p : lv.a_run_ptr.p;
+  g.runs = reinterpret_cast<const int4*>(a_is_r ? lv.r_runs.p : lv.a_runs.p);
+  return g;
+}
+
+void set_chunk(k::NodeProductArgs& g, int maxblk) {
+  g.chunk = std::max(1, std::min(64, 1024 / std::max(maxblk, 1)));
+  g.blk_cap = g.chunk * std::max(maxblk, 1);
+}
+
 DCsr& level_op(Context& c, size_t l) { return l == 0 ? c.K : c.levels[l].A; }
 
 HCsr download_csr(const DCsr& a, hipStream_t s) {
@@ -611,6 +805,14 @@ void amg_setup(Context& c) {
   const auto t0 = Clock::now();
   hipStream_t s = c.stream;
   const int n = c.K.rows;
+  for (double& ph : c.phase_seconds) ph = 0.0;
+  auto tick = Clock::now();
+  auto lap = [&](int phase) {
+    SLX_HIP(hipStreamSynchronize(s));
+    const auto now = Clock::now();
+    c.phase_seconds[phase] += std::chrono::duration<double>(now - tick).count();
+    tick = now;
+  };
   if (n == 0) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix must be square and non-empty");
   if (c.layout_dim != 2 && c.layout_dim != 3) fail(SLX_SOLVER_ERROR, "multigrid: the DOF layout must be two- or three-dimensional");
   if (c.layout_ndpn != c.layout_dim && !(c.layout_dim == 3 && c.layout_ndpn == 6))
@@ -659,6 +861,7 @@ void amg_setup(Context& c) {
     build_finest_nodes(c, c.levels[0]);
     c.amg_ready = false;
   }
+  lap(0);
   DBuf<int> flag;
   flag.alloc(1);
   const int zero = 0;
@@ -666,6 +869,7 @@ void amg_setup(Context& c) {
   const size_t reused_levels = c.levels.size();
   while (true) {
     numeric_level(c, l);
+    lap(1);
     Level& lv = c.levels[l];
     const bool stop = reuse ? (l + 1 == reused_levels)
                             : (lv.n <= o.coarse_size || (int)l + 1 >= o.max_levels);
@@ -699,6 +903,8 @@ void amg_setup(Context& c) {
       lv.sym_a.upload(ia, s);
       lv.sym_b.upload(ib, s);
       next.A.upload_pattern(spat, s);
+      prepare_node_products(lv, next, host_a, pt, ppat, rpat, appat, rawpat, s);
+      lap(2);
       lv.has_next = true;
       SLX_HIP(hipStreamSynchronize(s));  // host pattern arrays go out of scope
       c.levels.push_back(std::move(next));
@@ -707,12 +913,36 @@ void amg_setup(Context& c) {
     Level& coarse = c.levels[l + 1];
     DCsr& A = level_op(c, l);
     flag.upload(&zero, 1, s);
-    k::smooth_prolongator(A, cur.Ptent, cur.P, cur.dinv.p, o.prolongator_damping / cur.lambda_max, flag.p, s);
-    k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
-    k::spgemm_numeric(A, cur.P, cur.AP, flag.p, s);
-    k::spgemm_numeric(cur.R, cur.AP, cur.RAW, flag.p, s);
+    const double omega = o.prolongator_damping / cur.lambda_max;
+    if (cur.node_ok) {
+      k::NodeProductArgs g = node_args(cur, A, false, cur.pt_blk_ptr.p, cur.pt_blk_agg.p, cur.pt_blk_pos.p, cur.Ptent,
+                                       cur.P, flag.p);
+      g.dinv = cur.dinv.p;
+      g.omega = omega;
+      g.tptr = cur.Ptent.ptr.p;
+      g.tval = cur.Ptent.val.p;
+      g.row_agg = cur.row_agg.p;
+      set_chunk(g, 1);
+      k::node_product(g, cur.ns_p, true, s);
+      k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
+      lap(3);
+      k::NodeProductArgs g2 = node_args(cur, A, false, cur.p_blk_ptr.p, cur.p_blk_agg.p, cur.p_blk_pos.p, cur.P, cur.AP,
+                                        flag.p);
+      set_chunk(g2, cur.maxblk_p);
+      k::node_product(g2, cur.ns_ap, false, s);
+      k::NodeProductArgs g3 = node_args(cur, cur.R, true, cur.ap_blk_ptr.p, cur.ap_blk_agg.p, cur.ap_blk_pos.p, cur.AP,
+                                        cur.RAW, flag.p);
+      set_chunk(g3, cur.maxblk_ap);
+      k::node_product(g3, cur.ns_raw, false, s);
+    } else {
+      k::smooth_prolongator(A, cur.Ptent, cur.P, cur.dinv.p, omega, flag.p, s);
+      k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
+      k::spgemm_numeric(A, cur.P, cur.AP, flag.p, s);
+      k::spgemm_numeric(cur.R, cur.AP, cur.RAW, flag.p, s);
+    }
     k::symmetrize_values(coarse.A.nnz, cur.sym_a.p, cur.sym_b.p, cur.RAW.val.p, coarse.A.val.p, s);
     check_flag(c, flag, "the Galerkin product");
+    lap(4);
     ++l;
   }
   c.levels.resize(l + 1);
@@ -738,6 +968,7 @@ void amg_setup(Context& c) {
     }
     c.coarse_inv.upload(inv, s);
   }
+  lap(5);
   for (auto& lv : c.levels) {
     lv.x.alloc((size_t)lv.n);
     lv.b.alloc((size_t)lv.n);

@@ -90,7 +90,8 @@ void ensure_topopt_buffers(Context& c) {
     identity_filter(c);
     upload_filter(c);
   }
-  if (c.partials.n < 4096) c.partials.alloc(4096);
+  const size_t need = std::max<size_t>(4096, (size_t)k::filter_volume_blocks(c.dH) + 8);
+  if (c.partials.n < need) c.partials.alloc(need);
   if (!c.scal.p) c.scal.alloc(16);
 }
 
@@ -553,6 +554,13 @@ slx_status slx_amg_stats_get(const slx_context* ctx, slx_amg_stats* out) {
   out->coarse_pivot_ratio = c->coarse_pivot_ratio;
   out->near_null_space_dimension = c->nullspace_dim;
   out->setup_seconds = c->setup_seconds;
+  out->check_seconds = c->phase_seconds[0];
+  out->spectral_seconds = c->phase_seconds[1];
+  out->aggregation_seconds = c->phase_seconds[2];
+  out->smoothing_seconds = c->phase_seconds[3];
+  out->galerkin_seconds = c->phase_seconds[4];
+  out->coarse_seconds = c->phase_seconds[5];
+  for (const Level& lv : c->levels) out->node_product_levels += lv.node_ok ? 1 : 0;
   double nnz_sum = 0, rows_sum = 0;
   for (size_t l = 0; l < c->levels.size() && l < 16; ++l) {
     const Level& lv = c->levels[l];

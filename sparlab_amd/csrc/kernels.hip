@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "slx_kernels.hpp"
 
@@ -49,6 +50,27 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return t;  // valid on thread 0
 }
 
+// One lane's share of a row's products: the lane takes every LANES-th entry;
+// four entries are loaded before any is used so that each lane keeps eight
+// independent loads (then four gathers) in flight.
+template <int LANES, typename VT>
+__device__ __forceinline__ double row_dot(int s, int e, int lane, const int* __restrict__ col,
+                                          const VT* __restrict__ val, const VT* __restrict__ x) {
+  double acc = 0.0;
+  int k = s + lane;
+  for (; k + 3 * LANES < e; k += 4 * LANES) {
+    const int c0 = col[k], c1 = col[k + LANES], c2 = col[k + 2 * LANES], c3 = col[k + 3 * LANES];
+    const VT v0 = val[k], v1 = val[k + LANES], v2 = val[k + 2 * LANES], v3 = val[k + 3 * LANES];
+    const VT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    acc += (double)v0 * (double)x0;
+    acc += (double)v1 * (double)x1;
+    acc += (double)v2 * (double)x2;
+    acc += (double)v3 * (double)x3;
+  }
+  for (; k < e; k += LANES) acc += (double)val[k] * (double)x[col[k]];
+  return acc;
+}
+
 // MODE 0: y = A x; 1: y = b - A x; 2: y += A x; 3: y = A x and partial x.y
 template <int LANES, int MODE, typename VT>
 __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ ptr, const int* __restrict__ col,
@@ -62,10 +84,8 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ 
   const int lane = threadIdx.x % LANES;
   double prod = 0.0;
   if (row < n) {
-    const int s = ptr[row], e = ptr[row + 1];
-    double acc = 0.0;
-    for (int k = s + lane; k < e; k += LANES) acc += (double)val[k] * (double)x[col[k]];
-    acc = group_sum<LANES>(acc);
+    const double acc0 = row_dot<LANES>(ptr[row], ptr[row + 1], lane, col, val, x);
+    const double acc = group_sum<LANES>(acc0);
     if (lane == 0) {
       if (MODE == 0) y[row] = (VT)acc;
       if (MODE == 1) y[row] = (VT)((double)b[row] - acc);
@@ -94,6 +114,37 @@ __global__ void __launch_bounds__(kBlock) k_spmv_serial(int n, const int* __rest
   y[row] = s;
 }
 
+// Products staged through LDS, rows summed in column order by one thread each:
+// the column-order arithmetic of k_spmv_serial with coalesced loads. A block
+// handles SR rows whose entries fit in the stage (the host checks). MODE 0:
+// y = A x; MODE 1: partial sums of vol_e * clamp(y_e, 0, 1) (OC volume).
+constexpr int kStage = 4096;
+template <int SR, int MODE>
+__global__ void __launch_bounds__(kBlock) k_spmv_staged(int n, const int* __restrict__ ptr,
+                                                         const int* __restrict__ col, const double* __restrict__ val,
+                                                         const double* __restrict__ x, double* __restrict__ y,
+                                                         const double* __restrict__ vol, double* __restrict__ partials) {
+  __shared__ double prod[kStage];
+  __shared__ double sh[kBlock / 64];
+  const int r0 = blockIdx.x * SR;
+  const int r1 = min(n, r0 + SR);
+  const int k0 = ptr[r0], k1 = ptr[r1];
+  for (int k = k0 + (int)threadIdx.x; k < k1; k += kBlock) prod[k - k0] = val[k] * x[col[k]];
+  __syncthreads();
+  double part = 0.0;
+  if ((int)threadIdx.x < r1 - r0) {
+    const int row = r0 + threadIdx.x;
+    double acc = 0.0;
+    for (int k = ptr[row]; k < ptr[row + 1]; ++k) acc += prod[k - k0];
+    if (MODE == 0) y[row] = acc;
+    if (MODE == 1) part = fmin(fmax(acc, 0.0), 1.0) * vol[row];
+  }
+  if (MODE == 1) {
+    const double t = block_sum(part, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  }
+}
+
 // Chebyshev step (src/fem/Multigrid.cpp:833-853), fused:
 //   r_i <- r_i - (A d_in)_i;  d_out_i = c1 d_in_i + c2 dinv_i r_i;  x_i += d_out_i
 template <int LANES, typename VT>
@@ -106,10 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, const int* __restri
   const int row = blk * (kBlock / LANES) + threadIdx.x / LANES;
   const int lane = threadIdx.x % LANES;
   if (row >= n) return;
-  const int s = ptr[row], e = ptr[row + 1];
-  double acc = 0.0;
-  for (int k = s + lane; k < e; k += LANES) acc += (double)val[k] * (double)d_in[col[k]];
-  acc = group_sum<LANES>(acc);
+  const double acc = group_sum<LANES>(row_dot<LANES>(ptr[row], ptr[row + 1], lane, col, val, d_in));
   if (lane == 0) {
     const double r = (double)r_in[row] - acc;
     const double dn = c1 * (double)d_in[row] + c2 * (double)dinv[row] * r;
@@ -445,6 +493,11 @@ int grid8(long long n, int per_block) { return (blocks_for(n, per_block) + 7) / 
 int lanes_for(const DCsr& A) {
   // average row length decides the group width
   const double avg = A.rows ? double(A.nnz) / A.rows : 0.0;
+  static const int forced = [] {
+    const char* e = std::getenv("SPARLAB_SPMV_LANES");
+    return e ? std::atoi(e) : 0;
+  }();
+  if (forced == 4 || forced == 8 || forced == 16) return forced;
   return avg > 48 ? 16 : avg > 12 ? 8 : 4;
 }
 
@@ -487,9 +540,18 @@ int spmv_dot_blocks(const DCsr& A) { return grid8(A.rows, kBlock / lanes_for(A))
 void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s) {
   launch_spmv<3, double>(A, A.val.p, x, nullptr, y, partials, s);
 }
+int staged_rows(const DCsr& A) {
+  // rows per block such that every block's entries fit the stage
+  return A.max_row * 256 <= kStage ? 256 : A.max_row * 128 <= kStage ? 128 : A.max_row * 64 <= kStage ? 64 : 0;
+}
 void spmv_serial(const DCsr& A, const double* x, double* y, hipStream_t s) {
-  const int nb = blocks_for(A.rows, kBlock);
-  if (nb) k_spmv_serial<<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, x, y);
+  if (!A.rows) return;
+  switch (staged_rows(A)) {
+    case 256: k_spmv_staged<256, 0><<<blocks_for(A.rows, 256), kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, x, y, nullptr, nullptr); break;
+    case 128: k_spmv_staged<128, 0><<<blocks_for(A.rows, 128), kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, x, y, nullptr, nullptr); break;
+    case 64: k_spmv_staged<64, 0><<<blocks_for(A.rows, 64), kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, x, y, nullptr, nullptr); break;
+    default: k_spmv_serial<<<blocks_for(A.rows, kBlock), kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, x, y); break;
+  }
 }
 
 template <typename VT>
@@ -597,10 +659,19 @@ void oc_candidate(int n, const double* x, const double* dc, const double* dv, co
                                                                    damping, xn);
 }
 void filter_volume(const DCsr& H, const double* x, const double* vol, double* partials, double* out, hipStream_t s) {
+  const int sr = staged_rows(H);
+  if (sr == 128 || sr == 256) {
+    // 128 rows per block keeps the partial count within the reduction buffer for any n up to 2^19 blocks
+    const int nb = blocks_for(H.rows, 128);
+    k_spmv_staged<128, 1><<<nb, kBlock, 0, s>>>(H.rows, H.ptr.p, H.col.p, H.val.p, x, nullptr, vol, partials);
+    k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+    return;
+  }
   const int nb = reduce_blocks(H.rows);
   k_filter_volume<<<nb, kBlock, 0, s>>>(H.rows, H.ptr.p, H.col.p, H.val.p, x, vol, partials);
   k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
 }
+int filter_volume_blocks(const DCsr& H) { return blocks_for(H.rows, 128); }
 void max_abs_diff(int n, const double* a, const double* b, double* partials, double* out, hipStream_t s) {
   const int nb = reduce_blocks(n);
   k_max_abs_diff<<<nb, kBlock, 0, s>>>(n, a, b, partials);

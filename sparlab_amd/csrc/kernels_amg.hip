@@ -117,16 +117,137 @@ __global__ void k_symmetrize(long long n, const int* __restrict__ ia, const int*
   out[t] = v;
 }
 
-// inv_diag_i = 1 / sum of the stored diagonal entries of row i (Multigrid.cpp:721-735)
+// inv_diag_i = 1 / the diagonal entry of row i (Multigrid.cpp:721-735); rows are
+// sorted and hold each column once, so the entry is found by bisection.
 __global__ void k_inv_diag(int n, const int* __restrict__ ptr, const int* __restrict__ col,
                            const double* __restrict__ val, double* __restrict__ dinv, int* __restrict__ bad_row) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double d = 0.0;
-  for (int k = ptr[i]; k < ptr[i + 1]; ++k)
-    if (col[k] == i) d += val[k];
+  int lo = ptr[i], hi = ptr[i + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (col[mid] < i) lo = mid + 1; else hi = mid;
+  }
+  const double d = (lo < ptr[i + 1] && col[lo] == i) ? 0.0 + val[lo] : 0.0;
   if (!(d > 0.0)) atomicMin(bad_row, i);
   dinv[i] = 1.0 / d;
+}
+
+// ---------------------------------------------------------------------------
+// Node-row products. Every operator of the hierarchy has rows grouped in
+// nodes (the free DOFs of a mesh node, the coarse DOFs of an aggregate) whose
+// rows share one column pattern, and the columns of the products fall in
+// aggregate blocks (contiguous coarse columns). One wavefront computes one
+// node-row of C = A B: each lane owns up to NS entries of the node's rows;
+// the A row is walked one B-node at a time (a run of that node's rows), the
+// B-node's block list tells each lane where its aggregate sits in those B
+// rows, and the products are added in the order of A's row, the order of the
+// CPU restatement.
+// EPI 1 forms P = (I - omega D^-1 A) P-hat with B = P-hat (Multigrid.cpp:686-701).
+// ---------------------------------------------------------------------------
+template <int EPI, int NS>
+__global__ void __launch_bounds__(kWave) k_node_product(NodeProductArgs g) {
+  extern __shared__ int smem_i[];
+  // stage layout: per run (koff, rows, first block, blocks), 6 B-row starts, then the blocks
+  int* s_koff = smem_i;
+  int* s_dj = s_koff + g.chunk;
+  int* s_b0 = s_dj + g.chunk;
+  int* s_nb = s_b0 + g.chunk;
+  int* s_brow = s_nb + g.chunk;
+  int2* s_blk = reinterpret_cast<int2*>(s_brow + 6 * g.chunk + ((6 * g.chunk) & 1));
+  const int lane = threadIdx.x;
+  for (int I = blockIdx.x; I < g.nodes; I += gridDim.x) {
+    const int r0 = g.out_node_ptr[I];
+    const int d = g.out_node_ptr[I + 1] - r0;
+    const int L = g.cptr[r0 + 1] - g.cptr[r0];
+    const int total = d * L;
+    const int ka0 = g.aptr[r0];
+    const int q0 = g.run_ptr[I], q1 = g.run_ptr[I + 1];
+    for (int base = 0; base < total; base += kWave * NS) {
+      double acc[NS];
+      int my_a[NS], my_t[NS], bpos[NS], arow[NS], cidx[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int f = base + lane + kWave * s;
+        acc[s] = 0.0;
+        my_a[s] = -1;
+        my_t[s] = 0;
+        arow[s] = 0;
+        cidx[s] = -1;
+        if (f < total) {
+          const int r = f / L, pp = f - r * L;
+          const int row = r0 + r;
+          const int ci = g.cptr[row] + pp;
+          const int c = g.ccol[ci];
+          my_a[s] = g.agg_of_col[c];
+          my_t[s] = c - g.agg_off[my_a[s]];
+          arow[s] = g.aptr[row] - ka0;
+          cidx[s] = ci;
+        }
+      }
+      for (int qc = q0; qc < q1; qc += g.chunk) {
+        const int cnt = min(g.chunk, q1 - qc);
+        // stage the chunk's run metadata, B-row starts and block lists (all lanes load in parallel)
+        for (int t = lane; t < cnt; t += kWave) {
+          const int4 rn = g.runs[qc + t];
+          s_koff[t] = rn.x;
+          s_dj[t] = rn.z;
+          const int b0 = g.bblk_ptr[rn.w];
+          s_b0[t] = b0;
+          s_nb[t] = g.bblk_ptr[rn.w + 1] - b0;
+          for (int m = 0; m < rn.z; ++m) s_brow[t * 6 + m] = g.bptr[rn.y + m];
+        }
+        __syncthreads();
+        // block lists, packed in run order
+        if (lane == 0) {
+          int off = 0;
+          for (int t = 0; t < cnt; ++t) {
+            const int nb = s_nb[t];
+            s_nb[t] = off;  // now the start in the stage
+            off += nb;
+            s_dj[t] = s_dj[t] | (nb << 8);
+          }
+          if (off > g.blk_cap) atomicOr(g.error_flag, 16);
+        }
+        __syncthreads();
+        for (int t = 0; t < cnt; ++t) {
+          const int nb = s_dj[t] >> 8, at = s_nb[t], gb = s_b0[t];
+          for (int u = lane; u < nb; u += kWave)
+            if (at + u < g.blk_cap) s_blk[at + u] = make_int2(g.bblk_agg[gb + u], g.bblk_pos[gb + u]);
+        }
+        __syncthreads();
+        for (int t = 0; t < cnt; ++t) {
+          const int dj = s_dj[t] & 255, nb = s_dj[t] >> 8, at = s_nb[t];
+#pragma unroll
+          for (int s = 0; s < NS; ++s) bpos[s] = -1;
+          for (int u = 0; u < nb; ++u) {
+            const int2 bl = s_blk[min(at + u, g.blk_cap - 1)];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) bpos[s] = my_a[s] == bl.x ? bl.y : bpos[s];
+          }
+          const int kbase = ka0 + s_koff[t];
+          for (int m = 0; m < dj; ++m) {
+            const int brow = s_brow[t * 6 + m];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+              if (bpos[s] >= 0) acc[s] += g.aval[kbase + m + arow[s]] * g.bval[brow + bpos[s] + my_t[s]];
+          }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        if (cidx[s] < 0) continue;
+        double v = acc[s];
+        if (EPI == 1) {
+          const int row = r0 + (base + lane + kWave * s) / L;
+          v *= -(g.omega * g.dinv[row]);
+          if (my_a[s] == g.row_agg[row]) v += g.tval[g.tptr[row] + my_t[s]];
+        }
+        g.cval[cidx[s]] = v;
+      }
+    }
+  }
 }
 
 // Lanczos on D^-1/2 A D^-1/2 (Multigrid.cpp:304-317):
@@ -171,7 +292,25 @@ __global__ void k_lanczos_shift(int n, double b, const double* __restrict__ w, d
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
+template <int EPI>
+void launch_node_product(const NodeProductArgs& a, int ns, hipStream_t s) {
+  const int grid = std::min(a.nodes, 256 * 128);
+  if (a.nodes == 0) return;
+  const size_t lds = sizeof(int) * (size_t)(10 * a.chunk + 2) + sizeof(int2) * (size_t)a.blk_cap;
+  switch (ns) {
+    case 1: k_node_product<EPI, 1><<<grid, kWave, lds, s>>>(a); break;
+    case 2: k_node_product<EPI, 2><<<grid, kWave, lds, s>>>(a); break;
+    case 4: k_node_product<EPI, 4><<<grid, kWave, lds, s>>>(a); break;
+    case 8: k_node_product<EPI, 8><<<grid, kWave, lds, s>>>(a); break;
+    default: k_node_product<EPI, 16><<<grid, kWave, lds, s>>>(a); break;
+  }
+}
+
 }  // namespace
+
+void node_product(const NodeProductArgs& a, int ns, bool smooth, hipStream_t s) {
+  if (smooth) launch_node_product<1>(a, ns, s); else launch_node_product<0>(a, ns, s);
+}
 
 void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s) {
   if (C.rows == 0) return;

@@ -101,6 +101,14 @@ struct Level {
   DCsr Ptent, P, R, AP, RAW;
   DBuf<int> r_src;         // R.val[k] = P.val[r_src[k]]
   DBuf<int> sym_a, sym_b;  // next level's A from RAW (symmetrisation)
+  // node-row product metadata (see kernels_amg.hip); node_ok when every
+  // operator of this level has node-uniform rows in complete blocks
+  bool node_ok = false;
+  int ns_p = 0, ns_ap = 0, ns_raw = 0;
+  int maxblk_p = 0, maxblk_ap = 0;
+  DBuf<int> a_run_ptr, a_runs, r_run_ptr, r_runs;  // run tables (runs hold 4 ints each)
+  DBuf<int> f_node_ptr, f_node_of_row, c_node_ptr, c_node_of_row, row_agg;
+  DBuf<int> pt_blk_ptr, pt_blk_agg, pt_blk_pos, p_blk_ptr, p_blk_agg, p_blk_pos, ap_blk_ptr, ap_blk_agg, ap_blk_pos;
   DBuf<double> dinv, sq, lz_v, lz_vprev, lz_w, lz_start;
   DBuf<double> x, b, res, d, d2;
   DBuf<float> dinv32, x32, b32, res32, d32, d232;
@@ -154,6 +162,7 @@ struct Context {
   DBuf<float> coarse_inv32;
   double coarse_pivot_ratio = 0.0;
   double setup_seconds = 0.0;
+  double phase_seconds[6] = {0, 0, 0, 0, 0, 0};  // check, spectral, aggregation, smoothing, Galerkin, coarse
 
   // ---- PCG workspace ------------------------------------------------------------
   DBuf<double> r, z, p, q, xw, bw;

@@ -58,11 +58,44 @@ void simp(int n, const double* filtered, double penalty, double emin, double* ph
 void oc_candidate(int n, const double* x, const double* dc, const double* dv, const double* lower, const double* upper,
                   const unsigned char* passive, double lambda, double move, double damping, double* xn,
                   hipStream_t s);
+int filter_volume_blocks(const DCsr& H);
 void filter_volume(const DCsr& H, const double* x, const double* vol, double* partials, double* out, hipStream_t s);
 void max_abs_diff(int n, const double* a, const double* b, double* partials, double* out, hipStream_t s);
 void clamp(int n, double* x, const double* lo, const double* hi, hipStream_t s);
 
 // SA-AMG numeric setup (kernels_amg.hip)
+struct NodeProductArgs {
+  int nodes;                  // node-rows of A and C
+  const int* out_node_ptr;    // rows of node I: [out_node_ptr[I], out_node_ptr[I+1])
+  const int* aptr;
+  const int* acol;
+  const double* aval;
+  const int* bnode_of_row;    // B row -> B node
+  const int* bnode_ptr;       // B node -> its first row
+  const int* bblk_ptr;        // B node -> range of its aggregate blocks
+  const int* bblk_agg;        // aggregate of each block
+  const int* bblk_pos;        // position of the block within each row of the node
+  const int* bptr;
+  const double* bval;
+  const int* cptr;
+  const int* ccol;
+  double* cval;
+  const int* agg_of_col;      // coarse column -> aggregate
+  const int* agg_off;         // aggregate -> first coarse column
+  const double* dinv;         // prolongator smoothing only
+  double omega;
+  const int* tptr;
+  const double* tval;
+  const int* row_agg;         // aggregate of each fine row
+  int* error_flag;
+  // run table of the A-role matrix: per node-row the runs of its (shared)
+  // row pattern, one per B-node: (offset in the row, first B row, rows, B node)
+  const int* run_ptr;
+  const int4* runs;
+  int chunk;                  // runs staged per pass
+  int blk_cap;                // block entries the stage holds
+};
+void node_product(const NodeProductArgs& a, int ns, bool smooth, hipStream_t s);
 void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s);
 void smooth_prolongator(const DCsr& A, const DCsr& Ptent, DCsr& P, const double* dinv, double omega, int* error_flag,
                         hipStream_t s);

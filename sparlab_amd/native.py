@@ -65,7 +65,9 @@ class AmgStats(C.Structure):
                 ("grid_complexity", C.c_double), ("coarse_pivot_ratio", C.c_double),
                 ("near_null_space_dimension", C.c_int), ("unknowns", C.c_longlong * 16),
                 ("nonzeros", C.c_longlong * 16), ("aggregates", C.c_longlong * 16), ("lambda_max", C.c_double * 16),
-                ("setup_seconds", C.c_double)]
+                ("setup_seconds", C.c_double), ("check_seconds", C.c_double), ("spectral_seconds", C.c_double),
+                ("aggregation_seconds", C.c_double), ("smoothing_seconds", C.c_double),
+                ("galerkin_seconds", C.c_double), ("coarse_seconds", C.c_double), ("node_product_levels", C.c_int)]
 
 
 class EvalOptions(C.Structure):
