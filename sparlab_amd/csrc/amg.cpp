@@ -722,7 +722,11 @@ int read_flag(Context& c, DBuf<int>& flag) {
   return v;
 }
 
-// Lanczos estimate of lambda_max(D^-1 A) on the device (Multigrid.cpp:288-327)
+// Lanczos estimate of lambda_max(D^-1 A) on the device (Multigrid.cpp:288-327).
+// All steps are queued with their scalars in device memory; one read at the
+// end brings alpha and beta back, and the recurrence is cut on the host where
+// the reference breaks (beta <= 1e-12 |alpha|). Steps queued after such a
+// breakdown compute values nobody reads.
 double lanczos(Context& c, size_t l) {
   Level& lv = c.levels[l];
   DCsr& A = level_op(c, l);
@@ -733,31 +737,41 @@ double lanczos(Context& c, size_t l) {
     for (int i = 0; i < n; ++i) v0[(size_t)i] = 1.0 + 0.5 * std::sin(1.7 * double(i) + 0.3);
     lv.lz_start.upload(v0, s);
   }
+  const int m = std::max(1, std::min(c.amg_opt.lanczos_steps, n));
   lv.sq.alloc((size_t)n);
   lv.lz_v.alloc((size_t)n);
   lv.lz_vprev.alloc((size_t)n);
   lv.lz_w.alloc((size_t)n);
+  DBuf<double> sc;  // [0] norm, [1..m] alpha, [m+1..2m] beta
+  sc.alloc((size_t)(2 * m + 1));
   k::vsqrt(n, lv.dinv.p, lv.sq.p, s);
-  SLX_HIP(hipMemcpyAsync(lv.lz_v.p, lv.lz_start.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  SLX_HIP(hipMemsetAsync(lv.lz_vprev.p, 0, sizeof(double) * n, s));
-  double nrm = std::sqrt(device_dot(c, n, lv.lz_v.p, lv.lz_v.p));
-  k::lanczos_shift(n, nrm, lv.lz_v.p, lv.lz_w.p, lv.lz_vprev.p, s);  // w = v / |v| (vprev gets v, reset below)
-  SLX_HIP(hipMemcpyAsync(lv.lz_v.p, lv.lz_w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  SLX_HIP(hipMemsetAsync(lv.lz_vprev.p, 0, sizeof(double) * n, s));
-  std::vector<double> alpha, beta;
-  const int m = std::max(1, std::min(c.amg_opt.lanczos_steps, n));
+  k::dot(n, lv.lz_start.p, lv.lz_start.p, c.partials.p, sc.p, s);
+  k::sqrt_scalar(sc.p, s);
+  k::copy(n, lv.lz_start.p, lv.lz_w.p, s);
+  k::lanczos_shift(n, sc.p, lv.lz_w.p, lv.lz_v.p, lv.lz_vprev.p, s);  // v = v0 / |v0|
+  k::fill(n, 0.0, lv.lz_vprev.p, s);
+  double* alpha = sc.p + 1;
+  double* beta = sc.p + 1 + m;
   for (int j = 0; j < m; ++j) {
-    k::lanczos_apply(A, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, j > 0 ? beta.back() : 0.0, lv.lz_w.p, s);
-    const double aj = device_dot(c, n, lv.lz_w.p, lv.lz_v.p);
-    alpha.push_back(aj);
-    k::axmy(n, aj, lv.lz_v.p, lv.lz_w.p, s);
-    const double bj = std::sqrt(device_dot(c, n, lv.lz_w.p, lv.lz_w.p));
-    if (j + 1 == m || !(bj > 1.0e-12 * std::abs(aj))) break;
-    beta.push_back(bj);
-    k::lanczos_shift(n, bj, lv.lz_w.p, lv.lz_v.p, lv.lz_vprev.p, s);
+    k::lanczos_apply(A, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, j > 0 ? beta + j - 1 : nullptr, lv.lz_w.p, s);
+    k::dot(n, lv.lz_w.p, lv.lz_v.p, c.partials.p, alpha + j, s);
+    k::axmy(n, alpha + j, lv.lz_v.p, lv.lz_w.p, s);
+    k::dot(n, lv.lz_w.p, lv.lz_w.p, c.partials.p, beta + j, s);
+    k::sqrt_scalar(beta + j, s);
+    if (j + 1 < m) k::lanczos_shift(n, beta + j, lv.lz_w.p, lv.lz_v.p, lv.lz_vprev.p, s);
   }
-  std::vector<double> sub(beta.begin(), beta.begin() + (long)(alpha.size() - 1));
-  return tridiag_max(alpha, sub);
+  std::vector<double> h((size_t)(2 * m + 1));
+  sc.download(h.data(), h.size(), s);
+  SLX_HIP(hipStreamSynchronize(s));
+  std::vector<double> a, b;
+  for (int j = 0; j < m; ++j) {
+    const double aj = h[(size_t)(1 + j)], bj = h[(size_t)(1 + m + j)];
+    a.push_back(aj);
+    if (j + 1 == m || !(bj > 1.0e-12 * std::abs(aj))) break;
+    b.push_back(bj);
+  }
+  std::vector<double> sub(b.begin(), b.begin() + (long)(a.size() - 1));
+  return tridiag_max(a, sub);
 }
 
 void numeric_level(Context& c, size_t l) {
@@ -805,6 +819,7 @@ void amg_setup(Context& c) {
   const auto t0 = Clock::now();
   hipStream_t s = c.stream;
   const int n = c.K.rows;
+  invalidate_graphs(c);
   for (double& ph : c.phase_seconds) ph = 0.0;
   auto tick = Clock::now();
   auto lap = [&](int phase) {
@@ -823,6 +838,7 @@ void amg_setup(Context& c) {
   ensure_workspace(c);
   const bool same = c.amg_ready && c.fp_rows == n && c.fp_nnz == c.K.nnz && c.fp_unknowns == c.layout_unknowns;
   const bool reuse = o.reuse_aggregates && same;
+  c.amg_ready = false;  // until this setup completes
   // finiteness of every entry, each call (Multigrid.cpp:992-996)
   if (!device_all_finite(c, c.K.nnz, c.K.val.p)) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix has a non-finite entry");
   HCsr host_a;  // values of the level being aggregated (first setup only)

@@ -105,7 +105,7 @@ slx_pcg_result solve_case(Context& c, int l, bool warm, double tol, int maxit) {
   if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "AMG-CG: solve called before factorize");
   const size_t nf = c.free_dofs.size();
   double* x = c.d_u.p + (size_t)l * nf;
-  if (!warm || !c.have_u[(size_t)l]) SLX_HIP(hipMemsetAsync(x, 0, sizeof(double) * nf, c.stream));
+  if (!warm || !c.have_u[(size_t)l]) k::fill((long long)nf, 0.0, x, c.stream);
   slx_pcg_result r = pcg(c, c.d_loads_free.p + (size_t)l * nf, x, tol, maxit > 0 ? maxit : 1000);
   c.have_u[(size_t)l] = 1;
   return r;
@@ -258,6 +258,7 @@ void slx_destroy(slx_context* ctx) {
     (void)hipEventDestroy(pe.second.second);
   }
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  invalidate_graphs(*c);
   hipStream_t s = c->stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);

@@ -250,16 +250,18 @@ __global__ void __launch_bounds__(kWave) k_node_product(NodeProductArgs g) {
   }
 }
 
-// Lanczos on D^-1/2 A D^-1/2 (Multigrid.cpp:304-317):
-// w_i = s_i (A (s v))_i - beta vprev_i, with s = sqrt(dinv)
+// Lanczos on D^-1/2 A D^-1/2 (Multigrid.cpp:304-317). The step's scalars stay
+// in device memory, so a level's steps run without host round trips:
+// w_i = s_i (A (s v))_i - beta vprev_i, with s = sqrt(dinv), beta = *beta_ptr (0 if null)
 __global__ void __launch_bounds__(256) k_lanczos_apply(int n, const int* __restrict__ ptr, const int* __restrict__ col,
                                                        const double* __restrict__ val, const double* __restrict__ s,
                                                        const double* __restrict__ v, const double* __restrict__ vprev,
-                                                       double beta, double* __restrict__ w) {
+                                                       const double* __restrict__ beta_ptr, double* __restrict__ w) {
   constexpr int LANES = 16;
   const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
   const int lane = threadIdx.x % LANES;
   if (row >= n) return;
+  const double beta = beta_ptr ? *beta_ptr : 0.0;
   double acc = 0.0;
   for (int k = ptr[row] + lane; k < ptr[row + 1]; k += LANES) {
     const int j = col[k];
@@ -275,19 +277,31 @@ __global__ void k_sqrt(int n, const double* __restrict__ in, double* __restrict_
   if (i < n) out[i] = sqrt(in[i]);
 }
 
-// w -= a v
-__global__ void k_axmy(int n, double a, const double* __restrict__ v, double* __restrict__ w) {
+// w -= a v with a = *a_ptr
+__global__ void k_axmy(int n, const double* __restrict__ a_ptr, const double* __restrict__ v, double* __restrict__ w) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) w[i] -= a * v[i];
+  if (i < n) w[i] -= *a_ptr * v[i];
 }
 
-// vprev = v; v = w / b
-__global__ void k_lanczos_shift(int n, double b, const double* __restrict__ w, double* __restrict__ v,
-                                double* __restrict__ vprev) {
+// vprev = v; v = w / b with b = *b_ptr
+__global__ void k_lanczos_shift(int n, const double* __restrict__ b_ptr, const double* __restrict__ w,
+                                double* __restrict__ v, double* __restrict__ vprev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   vprev[i] = v[i];
-  v[i] = w[i] / b;
+  v[i] = w[i] / *b_ptr;
+}
+
+__global__ void k_sqrt_scalar(double* p) { p[0] = sqrt(p[0]); }
+
+__global__ void k_fill(long long n, double v, double* __restrict__ out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+
+__global__ void k_copy(long long n, const double* __restrict__ in, double* __restrict__ out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
 }
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
@@ -349,18 +363,28 @@ void vsqrt(int n, const double* in, double* out, hipStream_t s) {
   if (n) k_sqrt<<<nblk(n, 256), 256, 0, s>>>(n, in, out);
 }
 
-void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, double beta, double* w,
-                   hipStream_t s) {
+void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, const double* beta,
+                   double* w, hipStream_t s) {
   if (A.rows) k_lanczos_apply<<<nblk((long long)A.rows * 16, 256), 256, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, sq, v,
                                                                               vprev, beta, w);
 }
 
-void axmy(int n, double a, const double* v, double* w, hipStream_t s) {
+void axmy(int n, const double* a, const double* v, double* w, hipStream_t s) {
   if (n) k_axmy<<<nblk(n, 256), 256, 0, s>>>(n, a, v, w);
 }
 
-void lanczos_shift(int n, double b, const double* w, double* v, double* vprev, hipStream_t s) {
+void lanczos_shift(int n, const double* b, const double* w, double* v, double* vprev, hipStream_t s) {
   if (n) k_lanczos_shift<<<nblk(n, 256), 256, 0, s>>>(n, b, w, v, vprev);
+}
+
+void sqrt_scalar(double* p, hipStream_t s) { k_sqrt_scalar<<<1, 1, 0, s>>>(p); }
+
+void fill(long long n, double v, double* out, hipStream_t s) {
+  if (n) k_fill<<<nblk(n, 256), 256, 0, s>>>(n, v, out);
+}
+
+void copy(long long n, const double* in, double* out, hipStream_t s) {
+  if (n) k_copy<<<nblk(n, 256), 256, 0, s>>>(n, in, out);
 }
 
 }  // namespace k

@@ -165,7 +165,11 @@ struct Context {
   double phase_seconds[6] = {0, 0, 0, 0, 0, 0};  // check, spectral, aggregation, smoothing, Galerkin, coarse
 
   // ---- PCG workspace ------------------------------------------------------------
-  DBuf<double> r, z, p, q, xw, bw;
+  DBuf<double> r, z, p, q, xw, bw, xpcg, zold;
+  // the CG iteration as three captured graphs: q = K p (timed alone), the
+  // alpha / x / r update, and the preconditioner with the direction update
+  hipGraphExec_t g_spmv = nullptr, g_update = nullptr, g_precond = nullptr;
+  bool capturing = false;
   DBuf<double> partials;  // block partial sums
   DBuf<double> scal;      // device scalars
   std::vector<double> h_scal;

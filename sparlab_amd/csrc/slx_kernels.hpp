@@ -103,10 +103,13 @@ void gather(long long n, const int* src, const double* in, double* out, hipStrea
 void symmetrize_values(long long n, const int* ia, const int* ib, const double* raw, double* out, hipStream_t s);
 void inv_diag(const DCsr& A, double* dinv, int* bad_row, hipStream_t s);
 void vsqrt(int n, const double* in, double* out, hipStream_t s);
-void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, double beta, double* w,
-                   hipStream_t s);
-void axmy(int n, double a, const double* v, double* w, hipStream_t s);
-void lanczos_shift(int n, double b, const double* w, double* v, double* vprev, hipStream_t s);
+void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, const double* beta,
+                   double* w, hipStream_t s);
+void axmy(int n, const double* a, const double* v, double* w, hipStream_t s);
+void lanczos_shift(int n, const double* b, const double* w, double* v, double* vprev, hipStream_t s);
+void sqrt_scalar(double* p, hipStream_t s);
+void fill(long long n, double v, double* out, hipStream_t s);
+void copy(long long n, const double* in, double* out, hipStream_t s);
 
 }  // namespace k
 }  // namespace slx

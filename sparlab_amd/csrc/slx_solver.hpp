@@ -14,6 +14,8 @@ void vcycle(Context& c, const double* r, double* z);
 // Preconditioned CG on the device (src/fem/Multigrid.cpp:1152-1237); b, x device pointers.
 slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_iterations);
 
+// drop the captured CG graphs (after a setup or a reallocation)
+void invalidate_graphs(Context& c);
 // workspace sized for the current K_ff
 void ensure_workspace(Context& c);
 double device_dot(Context& c, long long n, const double* a, const double* b);

@@ -13,6 +13,7 @@
 // with FP64 row sums) under the FP64 outer CG; the FP32 cycle is not exactly
 // symmetric, so the outer method becomes flexible CG (Polak-Ribiere beta).
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <limits>
 #include <sstream>
@@ -42,9 +43,9 @@ Cheb cheb_params(const Context& c, const Level& lv) {
   return p;
 }
 
-// Chebyshev steps 1..degree-1 given x, d and the residual source r_in.
+// Chebyshev steps 1..degree-1 given x, d and the residual source r_first.
 template <bool F32>
-void cheb_steps(Context& c, size_t l, const void* r_first, bool timed) {
+void cheb_steps(Context& c, size_t l, const void* r_first, void* xv, bool timed) {
   Level& lv = c.levels[l];
   DCsr& A = op(c, l);
   const Cheb cp = cheb_params(c, lv);
@@ -57,74 +58,75 @@ void cheb_steps(Context& c, size_t l, const void* r_first, bool timed) {
     if (F32) {
       const float* rin = k == 1 ? static_cast<const float*>(r_first) : lv.res32.p;
       Timed t(c, timed ? "cheb" : nullptr);
-      k::cheb_step32(A, lv.dinv32.p, rin, lv.res32.p, lv.d32.p, lv.d232.p, lv.x32.p, c1, c2, s);
+      k::cheb_step32(A, lv.dinv32.p, rin, lv.res32.p, lv.d32.p, lv.d232.p, static_cast<float*>(xv), c1, c2, s);
       std::swap(lv.d32, lv.d232);
     } else {
       const double* rin = k == 1 ? static_cast<const double*>(r_first) : lv.res.p;
       Timed t(c, timed ? "cheb" : nullptr);
-      k::cheb_step(A, lv.dinv.p, rin, lv.res.p, lv.d.p, lv.d2.p, lv.x.p, c1, c2, s);
+      k::cheb_step(A, lv.dinv.p, rin, lv.res.p, lv.d.p, lv.d2.p, static_cast<double*>(xv), c1, c2, s);
       std::swap(lv.d, lv.d2);
     }
     rho = rho_new;
   }
 }
 
-void cycle64(Context& c, size_t l) {
+// V-cycle from level l for right-hand side b into x (Multigrid.cpp:857-885)
+void cycle64(Context& c, size_t l, const double* b, double* x) {
   Level& lv = c.levels[l];
   hipStream_t s = c.stream;
   if (l + 1 == c.levels.size()) {
-    k::dense_gemv(lv.n, c.coarse_inv.p, lv.b.p, lv.x.p, s);
+    k::dense_gemv(lv.n, c.coarse_inv.p, b, x, s);
     return;
   }
   DCsr& A = op(c, l);
   const bool top = l == 0;
   const Cheb cp = cheb_params(c, lv);
   // pre-smoothing from a zero guess: x = d = D^-1 b / theta, r starts as b
-  k::cheb_first_zero(lv.n, lv.dinv.p, lv.b.p, cp.theta, lv.x.p, lv.d.p, s);
-  cheb_steps<false>(c, l, lv.b.p, top);
+  k::cheb_first_zero(lv.n, lv.dinv.p, b, cp.theta, x, lv.d.p, s);
+  cheb_steps<false>(c, l, b, x, top);
   {
     Timed t(c, top ? "residual" : nullptr);
-    k::residual(A, lv.x.p, lv.b.p, lv.res.p, s);
+    k::residual(A, x, b, lv.res.p, s);
   }
   Level& nx = c.levels[l + 1];
   k::spmv(lv.R, lv.res.p, nx.b.p, s);
-  cycle64(c, l + 1);
-  k::spmv_add(lv.P, nx.x.p, lv.x.p, s);
+  cycle64(c, l + 1, nx.b.p, nx.x.p);
+  k::spmv_add(lv.P, nx.x.p, x, s);
   // post-smoothing from the current x
   {
     Timed t(c, top ? "residual" : nullptr);
-    k::residual(A, lv.x.p, lv.b.p, lv.res.p, s);
+    k::residual(A, x, b, lv.res.p, s);
   }
-  k::cheb_first(lv.n, lv.dinv.p, lv.res.p, cp.theta, lv.x.p, lv.d.p, s);
-  cheb_steps<false>(c, l, lv.res.p, top);
+  k::cheb_first(lv.n, lv.dinv.p, lv.res.p, cp.theta, x, lv.d.p, s);
+  cheb_steps<false>(c, l, lv.res.p, x, top);
 }
 
-void cycle32(Context& c, size_t l) {
+void cycle32(Context& c, size_t l, const float* b, float* x) {
   Level& lv = c.levels[l];
   hipStream_t s = c.stream;
   if (l + 1 == c.levels.size()) {
-    k::dense_gemv32(lv.n, c.coarse_inv32.p, lv.b32.p, lv.x32.p, s);
+    k::dense_gemv32(lv.n, c.coarse_inv32.p, b, x, s);
     return;
   }
   DCsr& A = op(c, l);
   const bool top = l == 0;
   const Cheb cp = cheb_params(c, lv);
-  k::cheb_first_zero32(lv.n, lv.dinv32.p, lv.b32.p, cp.theta, lv.x32.p, lv.d32.p, s);
-  cheb_steps<true>(c, l, lv.b32.p, top);
+  k::cheb_first_zero32(lv.n, lv.dinv32.p, b, cp.theta, x, lv.d32.p, s);
+  cheb_steps<true>(c, l, b, x, top);
   {
     Timed t(c, top ? "residual" : nullptr);
-    k::residual32(A, lv.x32.p, lv.b32.p, lv.res32.p, s);
+    k::residual32(A, x, b, lv.res32.p, s);
   }
   Level& nx = c.levels[l + 1];
   k::spmv32(lv.R, lv.res32.p, nx.b32.p, s);
-  cycle32(c, l + 1);
-  k::spmv_add32(lv.P, nx.x32.p, lv.x32.p, s);
+  cycle32(c, l + 1, nx.b32.p, nx.x32.p);
+  k::spmv_add32(lv.P, nx.x32.p, x, s);
   {
     Timed t(c, top ? "residual" : nullptr);
-    k::residual32(A, lv.x32.p, lv.b32.p, lv.res32.p, s);
+    k::residual32(A, x, b, lv.res32.p, s);
   }
-  k::cheb_first32(lv.n, lv.dinv32.p, lv.res32.p, cp.theta, lv.x32.p, lv.d32.p, s);
-  cheb_steps<true>(c, l, lv.res32.p, top);
+  k::cheb_first32(lv.n, lv.dinv32.p, lv.res32.p, cp.theta, x, lv.d32.p, s);
+  cheb_steps<true>(c, l, lv.res32.p, x, top);
 }
 
 void read_scalars(Context& c, int from, int count) {
@@ -138,7 +140,7 @@ void read_scalars(Context& c, int from, int count) {
 
 // ---------------------------------------------------------------------------
 Timed::Timed(Context& ctx, const char* n) : c(ctx), name(n) {
-  if (!c.timing || !name) return;
+  if (!c.timing || !name || c.capturing) return;
   a = c.take_event();
   b = c.take_event();
   SLX_HIP(hipEventRecord(a, c.stream));
@@ -175,8 +177,18 @@ void Context::flush_timers() {
   pending.clear();
 }
 
+void invalidate_graphs(Context& c) {
+  for (hipGraphExec_t* g : {&c.g_spmv, &c.g_update, &c.g_precond}) {
+    if (*g) (void)hipGraphExecDestroy(*g);
+    *g = nullptr;
+  }
+}
+
 void ensure_workspace(Context& c) {
   const size_t n = (size_t)c.K.rows;
+  if (c.r.n != n) invalidate_graphs(c);
+  c.xpcg.alloc(n);
+  c.zold.alloc(n);
   c.r.alloc(n);
   c.z.alloc(n);
   c.p.alloc(n);
@@ -184,7 +196,10 @@ void ensure_workspace(Context& c) {
   c.xw.alloc(n);
   c.bw.alloc(n);
   const size_t need = std::max<size_t>(4096, (size_t)k::spmv_dot_blocks(c.K) + 8);
-  if (c.partials.n < need) c.partials.alloc(need);
+  if (c.partials.n < need) {
+    invalidate_graphs(c);
+    c.partials.alloc(need);
+  }
   if (!c.scal.p) c.scal.alloc(16);
 }
 
@@ -226,23 +241,51 @@ void vcycle(Context& c, const double* r, double* z) {
   const size_t n = (size_t)top.n;
   if (c.amg_opt.precision == 1) {
     k::to_float((long long)n, r, top.b32.p, c.stream);
-    cycle32(c, 0);
+    cycle32(c, 0, top.b32.p, top.x32.p);
     k::to_double((long long)n, top.x32.p, z, c.stream);
     return;
   }
-  SLX_HIP(hipMemcpyAsync(top.b.p, r, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
-  cycle64(c, 0);
-  SLX_HIP(hipMemcpyAsync(z, top.x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+  cycle64(c, 0, r, z);
 }
 
-slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_iterations) {
+namespace {
+
+bool graphs_enabled() {
+  const char* e = std::getenv("SPARLAB_NO_GRAPHS");
+  return !(e && e[0] == '1');
+}
+
+template <class F>
+hipGraphExec_t capture(Context& c, F&& body) {
+  hipGraph_t graph = nullptr;
+  c.capturing = true;
+  SLX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(c.stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    c.capturing = false;
+    throw;
+  }
+  SLX_HIP(hipStreamEndCapture(c.stream, &graph));
+  c.capturing = false;
+  hipGraphExec_t exec = nullptr;
+  SLX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  SLX_HIP(hipGraphDestroy(graph));
+  return exec;
+}
+
+}  // namespace
+
+slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int max_iterations) {
   const long long n = c.K.rows;
   hipStream_t s = c.stream;
   slx_pcg_result res{0, 0.0, 0};
   const bool flexible = c.amg_opt.precision == 1;
   const double bnorm = std::sqrt(device_dot(c, n, b, b));
   if (bnorm == 0.0) {
-    SLX_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    k::fill(n, 0.0, x_out, s);
     res.converged = 1;
     return res;
   }
@@ -251,6 +294,36 @@ slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_i
   bool recursive_converged = false;
   double best_true = std::numeric_limits<double>::max();
   const int nb = k::spmv_dot_blocks(c.K);
+  double* x = c.xpcg.p;
+  k::copy(n, x_out, x, s);
+  // the three parts of one iteration (Multigrid.cpp:1184-1216)
+  auto body_spmv = [&] { k::spmv_dot(c.K, c.p.p, c.q.p, c.partials.p, s); };
+  auto body_update = [&] {
+    k::reduce(nb, c.partials.p, c.scal.p + S_PQ, 0, s);
+    k::pcg_update(n, x, c.r.p, c.p.p, c.q.p, c.scal.p, S_RZ, S_PQ, c.partials.p, c.scal.p + S_RR, s);
+  };
+  auto body_precond = [&] {
+    if (flexible) {
+      // beta = r.(z - z_old) / rz_old (Polak-Ribiere)
+      k::copy(n, c.z.p, c.zold.p, s);
+      vcycle(c, c.r.p, c.z.p);
+      k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZN, s);
+      k::dot(n, c.r.p, c.zold.p, c.partials.p, c.scal.p + S_RZX, s);
+      k::flexible_beta(c.scal.p, S_RZN, S_RZX, S_RZ, S_TMP, s);
+      k::pcg_direction(n, c.p.p, c.z.p, c.scal.p, S_TMP, -1, s);
+    } else {
+      vcycle(c, c.r.p, c.z.p);
+      k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZN, s);
+      k::pcg_direction(n, c.p.p, c.z.p, c.scal.p, S_RZN, S_RZ, s);
+    }
+    k::copy_scalar(c.scal.p, S_RZN, S_RZ, s);
+  };
+  const bool graphs = graphs_enabled();
+  if (graphs && !c.g_spmv) {
+    c.g_spmv = capture(c, body_spmv);
+    c.g_update = capture(c, body_update);
+    c.g_precond = capture(c, body_precond);
+  }
   for (int restart = 0; restart < 3; ++restart) {
     k::residual(c.K, x, b, c.r.p, s);
     double rnorm = std::sqrt(device_dot(c, n, c.r.p, c.r.p));
@@ -260,16 +333,15 @@ slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_i
       break;
     }
     vcycle(c, c.r.p, c.z.p);
-    SLX_HIP(hipMemcpyAsync(c.p.p, c.z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    k::copy(n, c.z.p, c.p.p, s);
     k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZ, s);
     bool converged = false;
     while (total < max_iterations) {
       {
         Timed t(c, "spmv");
-        k::spmv_dot(c.K, c.p.p, c.q.p, c.partials.p, s);
+        if (graphs) SLX_HIP(hipGraphLaunch(c.g_spmv, s)); else body_spmv();
       }
-      k::reduce(nb, c.partials.p, c.scal.p + S_PQ, 0, s);
-      k::pcg_update(n, x, c.r.p, c.p.p, c.q.p, c.scal.p, S_RZ, S_PQ, c.partials.p, c.scal.p + S_RR, s);
+      if (graphs) SLX_HIP(hipGraphLaunch(c.g_update, s)); else body_update();
       ++total;
       read_scalars(c, S_PQ, 2);
       const double pq = c.h_scal[S_PQ];
@@ -285,20 +357,7 @@ slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_i
         converged = true;
         break;
       }
-      if (flexible) {
-        // z_old kept in xw; beta = r.(z - z_old) / rz_old
-        std::swap(c.z, c.xw);
-        vcycle(c, c.r.p, c.z.p);
-        k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZN, s);
-        k::dot(n, c.r.p, c.xw.p, c.partials.p, c.scal.p + S_RZX, s);
-        k::flexible_beta(c.scal.p, S_RZN, S_RZX, S_RZ, S_TMP, s);
-        k::pcg_direction(n, c.p.p, c.z.p, c.scal.p, S_TMP, -1, s);
-      } else {
-        vcycle(c, c.r.p, c.z.p);
-        k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZN, s);
-        k::pcg_direction(n, c.p.p, c.z.p, c.scal.p, S_RZN, S_RZ, s);
-      }
-      k::copy_scalar(c.scal.p, S_RZN, S_RZ, s);
+      if (graphs) SLX_HIP(hipGraphLaunch(c.g_precond, s)); else body_precond();
     }
     k::residual(c.K, x, b, c.r.p, s);
     res.relative_residual = std::sqrt(device_dot(c, n, c.r.p, c.r.p)) / bnorm;
@@ -313,6 +372,7 @@ slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_i
   }
   if (!res.converged && recursive_converged) res.converged = 1;
   res.iterations = total;
+  k::copy(n, x, x_out, s);
   c.flush_timers();
   return res;
 }

@@ -175,6 +175,29 @@ def test_mixed_precision_solve_meets_tolerance(ctx):
     assert it32 <= it64 + 3
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_captured_cg_graphs_match_eager_launches(ctx, precision, monkeypatch):
+    """The CG iteration replayed from HIP graphs gives the eager launches' bits,
+    on every replay, with eager work (a V-cycle, a fresh setup) in between."""
+    print(f"test_captured_cg_graphs_match_eager_launches[{precision}]", flush=True)
+    m = M.cantilever_3d(16, 8, 8)
+    g = N.HipProblem(m, ctx=ctx)
+    g.assemble(1e-9 + (1 - 1e-9) * M.seeded_density(m.nelem, seed=11) ** 3)
+    b = m.free_loads()[0]
+    r = np.random.default_rng(6).standard_normal(g.nfree)
+    monkeypatch.setenv("SPARLAB_NO_GRAPHS", "1")
+    g.amg_setup(coarse_size=300, precision=precision)
+    x_eager, it_eager, rr_eager, _ = g.solve(b, tol=1e-10)
+    monkeypatch.delenv("SPARLAB_NO_GRAPHS")
+    for replay in range(3):
+        if replay == 2:
+            g.amg_setup(coarse_size=300, precision=precision)
+        x, it, rr, cv = g.solve(b, tol=1e-10)
+        assert cv and it == it_eager and rr == rr_eager
+        assert np.array_equal(x, x_eager), f"replay {replay}"
+        g.amg_apply(r)
+
+
 def test_error_paths(ctx):
     print("test_error_paths", flush=True)
     # under-constrained: no supports -> the coarse operator is singular
