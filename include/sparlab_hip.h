@@ -32,6 +32,14 @@
  *   slx_evaluate         ComplianceObjective::evaluate (src/topopt/Sensitivity.cpp:110-237)
  *   slx_oc_update        optimality_criteria_update (src/topopt/OptimalityCriteria.cpp:40-149)
  *   slx_run_oc           TopologyOptimizer::run_oc (src/topopt/TopologyOptimizer.cpp:556-724)
+ *
+ * Additions with no counterpart in the reference:
+ *   slx_warmstart_*      a learned initial guess for the compliance solves, which
+ *                        the reference warm-starts from the previous solution
+ *                        (src/topopt/Sensitivity.cpp:169-176); CG still runs to
+ *                        the same tolerance
+ *   slx_net_*            direct access to that network (parameters, forward pass,
+ *                        gradients, training) for tests against a CPU restatement
  */
 #ifndef SPARLAB_HIP_H
 #define SPARLAB_HIP_H
@@ -146,6 +154,44 @@ typedef struct {
   slx_eval_options eval;
 } slx_run_options;
 
+/* Learned warm start: a 3-level 3-D convolutional network on the node grid maps
+ * per-node (density, density change, previous displacement, residual of the
+ * previous displacement, previous displacement change s) to the direction d of
+ * the part of the coming displacement change that s does not explain. It trains
+ * on the first train_solves solves of the evaluation sequence (loss weighted by
+ * the nodes' stiffness factors) and then proposes d for every solve; CG starts
+ * from x0 = u_prev + a d + b s with (a, b) from the Galerkin condition on
+ * span(d, s) -- the point of that plane closest to the solution in the K-norm,
+ * so x0 is never further from it than u_prev. With train_solves = 0 and
+ * min_samples beyond reach only s is used. CG then runs to the unchanged
+ * tolerance. */
+typedef struct {
+  int enable;             /* 0: plain warm start */
+  int train_solves;       /* solves whose results train the network (40) */
+  int min_samples;        /* training samples before the first prediction (4) */
+  int steps_per_solve;    /* Adam steps after each training solve (16) */
+  int buffer;             /* replay samples kept on the device (8) */
+  double learning_rate;   /* 1e-3 */
+  unsigned long long seed;
+} slx_warmstart_options;
+
+typedef struct {
+  int samples;                /* training samples collected */
+  int train_steps;            /* Adam steps taken */
+  double first_loss;          /* loss of the first training step */
+  double last_loss;           /* loss of the last training step */
+  int predictions;            /* solves started from x0 */
+  int rejected;               /* proposals with no usable direction (CG started from u_prev) */
+  double step_sum;            /* sum of the network coefficients a */
+  double energy_ratio_sum;    /* sum of ||u - x0||_K / ||u - u_prev||_K, u the converged solution */
+  double last_energy_ratio;
+  double net_ratio_sum;       /* the same ratio for the network's direction alone */
+  double secant_ratio_sum;    /* and for the previous change alone */
+  double residual_ratio_sum;  /* sum of ||b - K x0|| / ||b - K u_prev|| */
+  double train_seconds;
+  double predict_seconds;
+} slx_warmstart_stats;
+
 /* ---- lifecycle ------------------------------------------------------------ */
 slx_status slx_create(int device, slx_context** out);
 void slx_destroy(slx_context* ctx);
@@ -172,6 +218,10 @@ slx_status slx_free_dofs_get(const slx_context* ctx, int* free_dofs);
 /* ---- assembly: K_ff = reduce(sum_e scale_e Ke0) on the device -------------- */
 slx_status slx_assemble(slx_context* ctx, const double* scale /* nelem, host, or NULL for ones */);
 slx_status slx_kff_get(const slx_context* ctx, int* ptr, int* col, double* val);
+/* y = K_ff x with the operator the solver uses (host arrays of nfree); matrix_free
+ * (optional) reports whether it ran matrix-free (structured Hex8 after slx_assemble,
+ * unless SPARLAB_NO_MATRIX_FREE=1 was set at assembly) or on the assembled CSR. */
+slx_status slx_kff_apply(slx_context* ctx, const double* x, double* y, int* matrix_free);
 
 /* ---- linear solver ---------------------------------------------------------- */
 slx_status slx_amg_setup(slx_context* ctx, const slx_amg_options* opt);
@@ -204,6 +254,30 @@ slx_status slx_oc_update(slx_context* ctx, const double* x, const double* dc_dx,
  * max change, lambda, linear iterations, grey level. */
 slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history, int* performed,
                       double* final_compliance);
+
+/* ---- learned warm start ------------------------------------------------------ */
+void slx_warmstart_default_options(slx_warmstart_options* o);
+/* Structured models only; resets the network and its training state. */
+slx_status slx_warmstart_configure(slx_context* ctx, const slx_warmstart_options* opt);
+slx_status slx_warmstart_stats_get(const slx_context* ctx, slx_warmstart_stats* out);
+
+/* The network on an arbitrary X x Y x Z grid (tests). Parameters are FP32,
+ * per layer: weights [cout][27][cin] then biases [cout], layers in order
+ * (cin, cout) = (16,16) (16,32) (32,64) (64,32) (32,16) (16,16); tap =
+ * (dz+1)*9 + (dy+1)*3 + (dx+1). x is [V][nin], y and t are [V][nout], voxel
+ * v = (z*Y + y)*X + x. */
+slx_status slx_net_configure(slx_context* ctx, int X, int Y, int Z, int nin, int nout, unsigned long long seed);
+slx_status slx_net_parameter_count(const slx_context* ctx, long long* count);
+slx_status slx_net_parameters_get(slx_context* ctx, float* params);
+slx_status slx_net_parameters_set(slx_context* ctx, const float* params);
+slx_status slx_net_forward(slx_context* ctx, const float* x, float* y);
+/* loss = sum_v w_v sum_k (y - t)^2 / (nout sum_v w_v) (w = 1 when NULL) and its
+ * gradient with respect to every parameter */
+slx_status slx_net_gradients(slx_context* ctx, const float* x, const float* t, const float* w, double* loss,
+                             float* grad);
+/* steps Adam steps on one (x, t, w) triple; losses before the first and after the last step */
+slx_status slx_net_train(slx_context* ctx, const float* x, const float* t, const float* w, int steps,
+                         double learning_rate, double* loss_first, double* loss_last);
 
 /* ---- measurement ------------------------------------------------------------
  * With timing on, every launch of the named kernels is bracketed by HIP events

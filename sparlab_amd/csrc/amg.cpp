@@ -753,7 +753,10 @@ double lanczos(Context& c, size_t l) {
   double* alpha = sc.p + 1;
   double* beta = sc.p + 1 + m;
   for (int j = 0; j < m; ++j) {
-    k::lanczos_apply(A, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, j > 0 ? beta + j - 1 : nullptr, lv.lz_w.p, s);
+    if (&A == &c.K)
+      op_lanczos(c, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, j > 0 ? beta + j - 1 : nullptr, lv.lz_w.p);
+    else
+      k::lanczos_apply(A, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, j > 0 ? beta + j - 1 : nullptr, lv.lz_w.p, s);
     k::dot(n, lv.lz_w.p, lv.lz_v.p, c.partials.p, alpha + j, s);
     k::axmy(n, alpha + j, lv.lz_v.p, lv.lz_w.p, s);
     k::dot(n, lv.lz_w.p, lv.lz_w.p, c.partials.p, beta + j, s);

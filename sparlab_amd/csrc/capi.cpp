@@ -10,6 +10,7 @@
 
 #include "slx_kernels.hpp"
 #include "slx_model.hpp"
+#include "slx_net.hpp"
 #include "slx_solver.hpp"
 
 namespace slx {
@@ -96,8 +97,22 @@ void ensure_topopt_buffers(Context& c) {
 }
 
 void assemble(Context& c, const double* d_scale) {
-  Timed t(c, "assemble");
-  k::assemble_structured(c.dim, c.K, c.d_free.p, d_scale, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream);
+  {
+    Timed t(c, "assemble");
+    k::assemble_structured(c.dim, c.K, c.d_free.p, d_scale, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream);
+  }
+  // the same factors drive the matrix-free products of Hex8 models
+  c.mf = c.dim == 3 && matrix_free_enabled();
+  if (!c.mf) return;
+  const long long ne = c.nelem;
+  c.mf_scale.alloc((size_t)ne);
+  c.mf_scale32.alloc((size_t)ne);
+  k::copy(ne, d_scale, c.mf_scale.p, c.stream);
+  k::to_float(ne, d_scale, c.mf_scale32.p, c.stream);
+  if (c.mf_ke32.n != c.d_ke0.n) {
+    c.mf_ke32.alloc(c.d_ke0.n);
+    k::to_float((long long)c.d_ke0.n, c.d_ke0.p, c.mf_ke32.p, c.stream);
+  }
 }
 
 // one load case from its last solution (or zero) to the tolerance
@@ -140,8 +155,10 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   const auto t1 = Clock::now();
   std::vector<double> comp((size_t)c.ncases);
   for (int l = 0; l < c.ncases; ++l) {
+    if (o.warm_start) warm_before_solve(c, l);
     const slx_pcg_result r = solve_case(c, l, o.warm_start != 0, o.tolerance, o.max_iterations);
     require_converged(c, r, o.tolerance);
+    if (o.warm_start) warm_after_solve(c, l);
     out.linear_iterations += r.iterations;
     comp[(size_t)l] = device_dot(c, (long long)nf, c.d_loads_free.p + (size_t)l * nf, c.d_u.p + (size_t)l * nf);
   }
@@ -157,6 +174,7 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
     k::spmv_serial(c.dHt, c.d_dc.p, c.d_dcx.p, s);
     k::spmv_serial(c.dHt, c.d_vol.p, c.d_dvx.p, s);
   }
+  warm_after_evaluate(c);
   sync(c);
   out.solve_seconds = std::chrono::duration<double>(Clock::now() - t1).count();
   if (!std::isfinite(out.compliance)) fail(SLX_SOLVER_ERROR, "the compliance objective evaluated to a non-finite value");
@@ -238,6 +256,7 @@ slx_status slx_create(int device, slx_context** out) {
   try {
     SLX_HIP(hipSetDevice(device));
     SLX_HIP(hipStreamCreate(&c->stream));
+    slx_amg_default_options(&c->amg_opt);  // evaluate and run_oc use these until slx_amg_setup
     c->scal.alloc(16);
     c->partials.alloc(4096);
   } catch (const Error&) {
@@ -366,6 +385,7 @@ slx_status slx_model_set_loads(slx_context* ctx, int ncases, const double* loads
 slx_status slx_model_finalize(slx_context* ctx) {
   return guarded(ctx, [&](Context& c) {
     if (c.ndofs == 0) fail(SLX_MODEL_ERROR, "define the mesh first");
+    c.mf = false;
     // DofManager::rebuild (src/fem/DofManager.cpp:58-71)
     c.free_dofs.clear();
     c.reduced.assign((size_t)c.ndofs, -1);
@@ -450,6 +470,20 @@ slx_status slx_assemble(slx_context* ctx, const double* scale) {
   });
 }
 
+slx_status slx_kff_apply(slx_context* ctx, const double* x, double* y, int* matrix_free) {
+  return guarded(ctx, [&](Context& c) {
+    if (c.K.rows == 0) fail(SLX_SOLVER_ERROR, "K_ff is not assembled");
+    const size_t n = (size_t)c.K.rows;
+    DBuf<double> dx, dy;
+    dx.upload(x, n, c.stream);
+    dy.alloc(n);
+    op_apply(c, dx.p, dy.p);
+    dy.download(y, n, c.stream);
+    sync(c);
+    if (matrix_free) *matrix_free = c.mf ? 1 : 0;
+  });
+}
+
 slx_status slx_kff_get(const slx_context* ctx, int* ptr, int* col, double* val) {
   return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
     require_model(c);
@@ -498,6 +532,7 @@ slx_status slx_factorize_csr(slx_context* ctx, int n, const int* ptr, const int*
     const bool same_pattern = c.K.rows == n && c.K.nnz == h.nnz() && c.layout_unknowns.size() == (size_t)n &&
                               std::equal(unknowns, unknowns + n, c.layout_unknowns.begin());
     if (same_pattern) c.K.val.upload(h.val, c.stream); else c.K.upload(h, c.stream);
+    c.mf = false;  // a caller's matrix: products use it as given
     c.layout_dim = dim;
     c.layout_ndpn = ndpn > 0 ? ndpn : dim;
     c.layout_nnodes = nnodes;

@@ -1,0 +1,185 @@
+// Matrix-free K_ff products of a structured Hex8 model:
+//   (K x)_i = sum over the elements e around node i of s_e (Ke0 x_e) restricted to i's rows,
+// the same operator the assembled K_ff holds (K_ff = sum_e s_e Ke0 scattered,
+// src/fem/Assembler.cpp:182-214), applied without reading it. One thread per
+// node; a block stages the input of an 8 x 8 x 4 node tile plus a one-node halo
+// and the factors of the tile's elements in LDS. For each of the 8 elements
+// around a node the node's local corner is fixed by the element's offset, so
+// the Ke0 coefficients a wave needs are uniform across it.
+//
+// Epilogues fuse what the callers do with the product: y = Kx with the partial
+// sums of x.y (CG), r = b - Kx (residuals), the Chebyshev step, and the Lanczos
+// step on D^-1/2 K D^-1/2. Vectors are in the free-DOF layout; prescribed DOFs
+// read as zero and are never written.
+#include <hip/hip_runtime.h>
+
+#include "slx_mf.hpp"
+
+namespace slx {
+namespace kmf {
+namespace {
+
+constexpr int TX = 8, TY = 8, TZ = 4, kThreads = TX * TY * TZ;
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;  // node halo
+constexpr int EX = TX + 1, EY = TY + 1, EZ = TZ + 1;  // elements touching the tile
+
+// VTK hexahedron corner of local offsets (StructuredMesh.cpp:224-234) and its inverse
+__device__ __forceinline__ constexpr int corner(int lx, int ly, int lz) {
+  return lz * 4 + (ly ? (lx ? 2 : 3) : (lx ? 1 : 0));
+}
+__device__ __forceinline__ constexpr int corner_x(int b) { return (b & 3) == 1 || (b & 3) == 2; }
+__device__ __forceinline__ constexpr int corner_y(int b) { return (b & 3) >= 2; }
+__device__ __forceinline__ constexpr int corner_z(int b) { return b >= 4; }
+
+// bijective block -> tile remap: consecutive tiles (which share halos) on one XCD
+__device__ __forceinline__ int xcd_tile(int b, int nb) { return (b % 8) * (nb / 8) + b / 8; }
+
+template <typename T>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kThreads / 64; ++w) s += sh[w];
+  return s;
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
+  __shared__ T s_u[HZ * HY * HX * 3];
+  __shared__ T s_s[EZ * EY * EX];
+  __shared__ double s_red[kThreads / 64];
+  const int ntx = (a.X + TX - 1) / TX, nty = (a.Y + TY - 1) / TY, ntz = (a.Z + TZ - 1) / TZ;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= ntx * nty * ntz) {
+    if (MODE == 1 && threadIdx.x == 0) a.partials[blockIdx.x] = 0.0;
+    return;
+  }
+  const int x0 = (tile % ntx) * TX, y0 = ((tile / ntx) % nty) * TY, z0 = (tile / (ntx * nty)) * TZ;
+  // stage the input (scaled for the Lanczos mode) over the tile and its halo
+  for (int h = threadIdx.x; h < HZ * HY * HX; h += kThreads) {
+    const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+    T u0 = 0, u1 = 0, u2 = 0;
+    if (gx >= 0 && gx < a.X && gy >= 0 && gy < a.Y && gz >= 0 && gz < a.Z) {
+      const int* f = a.reduced + 3 * (((long long)gz * a.Y + gy) * a.X + gx);
+      const int f0 = f[0], f1 = f[1], f2 = f[2];
+      if (MODE == 4) {
+        if (f0 >= 0) u0 = a.x[f0] * a.in_scale[f0];
+        if (f1 >= 0) u1 = a.x[f1] * a.in_scale[f1];
+        if (f2 >= 0) u2 = a.x[f2] * a.in_scale[f2];
+      } else {
+        if (f0 >= 0) u0 = a.x[f0];
+        if (f1 >= 0) u1 = a.x[f1];
+        if (f2 >= 0) u2 = a.x[f2];
+      }
+    }
+    s_u[3 * h] = u0;
+    s_u[3 * h + 1] = u1;
+    s_u[3 * h + 2] = u2;
+  }
+  const int nex = a.X - 1, ney = a.Y - 1, nez = a.Z - 1;  // elements per direction
+  for (int h = threadIdx.x; h < EZ * EY * EX; h += kThreads) {
+    const int ex = h % EX, ey = (h / EX) % EY, ez = h / (EX * EY);
+    const int gx = x0 - 1 + ex, gy = y0 - 1 + ey, gz = z0 - 1 + ez;
+    T sv = 0;
+    if (gx >= 0 && gx < nex && gy >= 0 && gy < ney && gz >= 0 && gz < nez)
+      sv = a.scale[((long long)gz * ney + gy) * nex + gx];
+    s_s[h] = sv;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
+  const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+  const bool inside = gx < a.X && gy < a.Y && gz < a.Z;
+  T acc0 = 0, acc1 = 0, acc2 = 0;
+  if (inside) {
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int ox = (o & 1) - 1, oy = ((o >> 1) & 1) - 1, oz = (o >> 2) - 1;
+      // element at (node + offset); tile-local element and halo coordinates coincide
+      const int ex = tx + ox + 1, ey = ty + oy + 1, ez = tz + oz + 1;
+      const T se = s_s[(ez * EY + ey) * EX + ex];
+      if (se == T(0)) continue;  // outside the mesh
+      const int ca = corner(-ox, -oy, -oz);
+      const T* kr = a.ke + (3 * ca) * 24;
+      T t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const int h = ((ez + corner_z(b)) * HY + (ey + corner_y(b))) * HX + (ex + corner_x(b));
+        const T v0 = s_u[3 * h], v1 = s_u[3 * h + 1], v2 = s_u[3 * h + 2];
+        const int cb = 3 * b;
+        t0 += kr[cb] * v0 + kr[cb + 1] * v1 + kr[cb + 2] * v2;
+        t1 += kr[24 + cb] * v0 + kr[25 + cb] * v1 + kr[26 + cb] * v2;
+        t2 += kr[48 + cb] * v0 + kr[49 + cb] * v1 + kr[50 + cb] * v2;
+      }
+      acc0 += se * t0;
+      acc1 += se * t1;
+      acc2 += se * t2;
+    }
+  }
+  double dot = 0.0;
+  if (inside) {
+    const int* f = a.reduced + 3 * (((long long)gz * a.Y + gy) * a.X + gx);
+    const T acc[3] = {acc0, acc1, acc2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int fi = f[c];
+      if (fi < 0) continue;
+      const T y = acc[c];
+      if (MODE == 0) {
+        a.y[fi] = y;
+      } else if (MODE == 1) {
+        a.y[fi] = y;
+        dot += (double)a.x[fi] * (double)y;
+      } else if (MODE == 2) {
+        a.y[fi] = a.b[fi] - y;
+      } else if (MODE == 3) {
+        // r_out = r_in - K d_in, d_out = c1 d_in + c2 D^-1 r_out, x += d_out (x input = d_in)
+        const double r = (double)a.b[fi] - (double)y;
+        const double dn = a.c1 * (double)a.x[fi] + a.c2 * (double)a.dinv[fi] * r;
+        a.y[fi] = (T)r;
+        a.d_out[fi] = (T)dn;
+        a.xs[fi] = (T)((double)a.xs[fi] + dn);
+      } else {
+        // Lanczos: w = s (K (s v)) - beta vprev
+        const double beta = a.beta ? *a.beta : 0.0;
+        a.y[fi] = (T)((double)y * (double)a.in_scale[fi] - beta * (double)a.vprev[fi]);
+      }
+    }
+  }
+  if (MODE == 1) {
+    const double s = block_sum<T>(dot, s_red);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+  }
+}
+
+template <typename T, int MODE>
+void launch(const MfArgs<T>& a, hipStream_t s) {
+  k_mf_hex8<T, MODE><<<blocks(a.X, a.Y, a.Z), kThreads, 0, s>>>(a);
+}
+
+}  // namespace
+
+int blocks(int X, int Y, int Z) {
+  const int n = ((X + TX - 1) / TX) * ((Y + TY - 1) / TY) * ((Z + TZ - 1) / TZ);
+  return (n + 7) / 8 * 8;
+}
+
+template <typename T>
+void apply(const MfArgs<T>& a, int mode, hipStream_t s) {
+  switch (mode) {
+    case 0: launch<T, 0>(a, s); break;
+    case 1: launch<T, 1>(a, s); break;
+    case 2: launch<T, 2>(a, s); break;
+    case 3: launch<T, 3>(a, s); break;
+    default: launch<T, 4>(a, s); break;
+  }
+}
+
+template void apply<double>(const MfArgs<double>&, int, hipStream_t);
+template void apply<float>(const MfArgs<float>&, int, hipStream_t);
+
+}  // namespace kmf
+}  // namespace slx

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -119,6 +120,8 @@ struct TimerSlot {
   long long launches = 0;
 };
 
+struct WarmStart;  // learned warm start (slx_net.hpp)
+
 // Problem / solver context: one GPU stream, one structured SIMP problem.
 struct Context {
   int device = 0;
@@ -146,6 +149,11 @@ struct Context {
   std::vector<char> have_u;
   DCsr K;  // K_ff on the device
   bool k_on_host = false;
+  // matrix-free K_ff of a structured Hex8 model: the element factors of the last
+  // assembly (FP64 and FP32) and Ke0 in FP32; used for level-0 products when mf
+  bool mf = false;
+  DBuf<double> mf_scale;
+  DBuf<float> mf_scale32, mf_ke32;
 
   // ---- multigrid -------------------------------------------------------------
   slx_amg_options amg_opt{};
@@ -184,6 +192,9 @@ struct Context {
   DCsr dH, dHt;
   DBuf<double> d_x, d_phys, d_fac, d_dfac, d_dc, d_energy, d_vol, d_tmp, d_tmp2, d_lower, d_upper, d_dcx, d_dvx;
   DBuf<unsigned char> d_passive;
+
+  // ---- learned warm start -------------------------------------------------------
+  std::shared_ptr<WarmStart> ws;
 
   // ---- measurement ------------------------------------------------------------
   bool timing = false;

@@ -1,0 +1,43 @@
+// Matrix-free K_ff products of structured Hex8 models (kernels_mf.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace slx {
+namespace kmf {
+
+// One launch computes y = K_ff x in one of these modes:
+//   0  y = K x
+//   1  y = K x and partials[block] = sum of x.y over the block (blocks() partials)
+//   2  y = b - K x
+//   3  Chebyshev step with d_in = x, r_in = b: y = r_out = r_in - K d_in,
+//      d_out = c1 d_in + c2 dinv r_out, xs += d_out
+//   4  Lanczos step: y = in_scale (K (in_scale x)) - (*beta) vprev (beta null: 0)
+// X, Y, Z are the node counts per direction; reduced maps DOF = node * 3 + k to
+// the free index or -1; scale holds the element factors s_e and ke the 24 x 24
+// element matrix (node-major DOF order, VTK corner numbering).
+template <typename T>
+struct MfArgs {
+  int X, Y, Z;
+  const int* reduced;
+  const T* scale;
+  const T* ke;
+  const T* x;
+  T* y;
+  const T* b = nullptr;
+  const T* dinv = nullptr;
+  T* d_out = nullptr;
+  T* xs = nullptr;
+  double c1 = 0.0, c2 = 0.0;
+  const T* in_scale = nullptr;
+  const T* vprev = nullptr;
+  const double* beta = nullptr;
+  double* partials = nullptr;
+};
+
+int blocks(int X, int Y, int Z);
+template <typename T>
+void apply(const MfArgs<T>& a, int mode, hipStream_t s);
+
+}  // namespace kmf
+}  // namespace slx

@@ -14,6 +14,17 @@ void vcycle(Context& c, const double* r, double* z);
 // Preconditioned CG on the device (src/fem/Multigrid.cpp:1152-1237); b, x device pointers.
 slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_iterations);
 
+// K_ff products on the device: matrix-free for structured Hex8 models
+// (after slx_assemble), the assembled CSR otherwise
+void op_apply(Context& c, const double* x, double* y);
+void op_residual(Context& c, const double* x, const double* b, double* r);
+int op_dot_blocks(const Context& c);
+void op_apply_dot(Context& c, const double* x, double* y, double* partials);
+// Lanczos step at level 0: w = sq (K (sq v)) - (*beta) vprev
+void op_lanczos(Context& c, const double* sq, const double* v, const double* vprev, const double* beta, double* w);
+// whether level-0 products run matrix-free (SPARLAB_NO_MATRIX_FREE=1 turns it off)
+bool matrix_free_enabled();
+
 // drop the captured CG graphs (after a setup or a reallocation)
 void invalidate_graphs(Context& c);
 // workspace sized for the current K_ff

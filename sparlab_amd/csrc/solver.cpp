@@ -19,6 +19,7 @@
 #include <sstream>
 
 #include "slx_kernels.hpp"
+#include "slx_mf.hpp"
 #include "slx_solver.hpp"
 
 namespace slx {
@@ -28,6 +29,45 @@ namespace {
 constexpr int S_PQ = 0, S_RR = 1, S_RZ = 2, S_RZN = 3, S_RZX = 4, S_TMP = 5;
 
 DCsr& op(Context& c, size_t l) { return l == 0 ? c.K : c.levels[l].A; }
+
+bool use_mf(const Context& c, size_t l) { return l == 0 && c.mf; }
+
+template <typename T>
+kmf::MfArgs<T> mf_base(const Context& c);
+template <>
+kmf::MfArgs<double> mf_base<double>(const Context& c) {
+  kmf::MfArgs<double> a{};
+  a.X = c.nx + 1, a.Y = c.ny + 1, a.Z = c.nz + 1;
+  a.reduced = c.d_reduced.p;
+  a.scale = c.mf_scale.p;
+  a.ke = c.d_ke0.p;
+  return a;
+}
+template <>
+kmf::MfArgs<float> mf_base<float>(const Context& c) {
+  kmf::MfArgs<float> a{};
+  a.X = c.nx + 1, a.Y = c.ny + 1, a.Z = c.nz + 1;
+  a.reduced = c.d_reduced.p;
+  a.scale = c.mf_scale32.p;
+  a.ke = c.mf_ke32.p;
+  return a;
+}
+
+// r = b - A_l x at level l
+template <typename T>
+void level_residual(Context& c, size_t l, const T* x, const T* b, T* r) {
+  if (use_mf(c, l)) {
+    kmf::MfArgs<T> a = mf_base<T>(c);
+    a.x = x;
+    a.b = b;
+    a.y = r;
+    kmf::apply(a, 2, c.stream);
+  } else if constexpr (sizeof(T) == 8) {
+    k::residual(op(c, l), x, b, r, c.stream);
+  } else {
+    k::residual32(op(c, l), x, b, r, c.stream);
+  }
+}
 
 struct Cheb {
   double theta, delta, sigma;
@@ -58,12 +98,26 @@ void cheb_steps(Context& c, size_t l, const void* r_first, void* xv, bool timed)
     if (F32) {
       const float* rin = k == 1 ? static_cast<const float*>(r_first) : lv.res32.p;
       Timed t(c, timed ? "cheb" : nullptr);
-      k::cheb_step32(A, lv.dinv32.p, rin, lv.res32.p, lv.d32.p, lv.d232.p, static_cast<float*>(xv), c1, c2, s);
+      if (use_mf(c, l)) {
+        kmf::MfArgs<float> a = mf_base<float>(c);
+        a.x = lv.d32.p, a.b = rin, a.y = lv.res32.p, a.dinv = lv.dinv32.p, a.d_out = lv.d232.p;
+        a.xs = static_cast<float*>(xv), a.c1 = c1, a.c2 = c2;
+        kmf::apply(a, 3, s);
+      } else {
+        k::cheb_step32(A, lv.dinv32.p, rin, lv.res32.p, lv.d32.p, lv.d232.p, static_cast<float*>(xv), c1, c2, s);
+      }
       std::swap(lv.d32, lv.d232);
     } else {
       const double* rin = k == 1 ? static_cast<const double*>(r_first) : lv.res.p;
       Timed t(c, timed ? "cheb" : nullptr);
-      k::cheb_step(A, lv.dinv.p, rin, lv.res.p, lv.d.p, lv.d2.p, static_cast<double*>(xv), c1, c2, s);
+      if (use_mf(c, l)) {
+        kmf::MfArgs<double> a = mf_base<double>(c);
+        a.x = lv.d.p, a.b = rin, a.y = lv.res.p, a.dinv = lv.dinv.p, a.d_out = lv.d2.p;
+        a.xs = static_cast<double*>(xv), a.c1 = c1, a.c2 = c2;
+        kmf::apply(a, 3, s);
+      } else {
+        k::cheb_step(A, lv.dinv.p, rin, lv.res.p, lv.d.p, lv.d2.p, static_cast<double*>(xv), c1, c2, s);
+      }
       std::swap(lv.d, lv.d2);
     }
     rho = rho_new;
@@ -78,7 +132,6 @@ void cycle64(Context& c, size_t l, const double* b, double* x) {
     k::dense_gemv(lv.n, c.coarse_inv.p, b, x, s);
     return;
   }
-  DCsr& A = op(c, l);
   const bool top = l == 0;
   const Cheb cp = cheb_params(c, lv);
   // pre-smoothing from a zero guess: x = d = D^-1 b / theta, r starts as b
@@ -86,7 +139,7 @@ void cycle64(Context& c, size_t l, const double* b, double* x) {
   cheb_steps<false>(c, l, b, x, top);
   {
     Timed t(c, top ? "residual" : nullptr);
-    k::residual(A, x, b, lv.res.p, s);
+    level_residual<double>(c, l, x, b, lv.res.p);
   }
   Level& nx = c.levels[l + 1];
   k::spmv(lv.R, lv.res.p, nx.b.p, s);
@@ -95,7 +148,7 @@ void cycle64(Context& c, size_t l, const double* b, double* x) {
   // post-smoothing from the current x
   {
     Timed t(c, top ? "residual" : nullptr);
-    k::residual(A, x, b, lv.res.p, s);
+    level_residual<double>(c, l, x, b, lv.res.p);
   }
   k::cheb_first(lv.n, lv.dinv.p, lv.res.p, cp.theta, x, lv.d.p, s);
   cheb_steps<false>(c, l, lv.res.p, x, top);
@@ -108,14 +161,13 @@ void cycle32(Context& c, size_t l, const float* b, float* x) {
     k::dense_gemv32(lv.n, c.coarse_inv32.p, b, x, s);
     return;
   }
-  DCsr& A = op(c, l);
   const bool top = l == 0;
   const Cheb cp = cheb_params(c, lv);
   k::cheb_first_zero32(lv.n, lv.dinv32.p, b, cp.theta, x, lv.d32.p, s);
   cheb_steps<true>(c, l, b, x, top);
   {
     Timed t(c, top ? "residual" : nullptr);
-    k::residual32(A, x, b, lv.res32.p, s);
+    level_residual<float>(c, l, x, b, lv.res32.p);
   }
   Level& nx = c.levels[l + 1];
   k::spmv32(lv.R, lv.res32.p, nx.b32.p, s);
@@ -123,7 +175,7 @@ void cycle32(Context& c, size_t l, const float* b, float* x) {
   k::spmv_add32(lv.P, nx.x32.p, x, s);
   {
     Timed t(c, top ? "residual" : nullptr);
-    k::residual32(A, x, b, lv.res32.p, s);
+    level_residual<float>(c, l, x, b, lv.res32.p);
   }
   k::cheb_first32(lv.n, lv.dinv32.p, lv.res32.p, cp.theta, x, lv.d32.p, s);
   cheb_steps<true>(c, l, lv.res32.p, x, top);
@@ -177,6 +229,54 @@ void Context::flush_timers() {
   pending.clear();
 }
 
+bool matrix_free_enabled() {
+  const char* e = std::getenv("SPARLAB_NO_MATRIX_FREE");
+  return !(e && e[0] == '1');
+}
+
+void op_apply(Context& c, const double* x, double* y) {
+  if (c.mf) {
+    kmf::MfArgs<double> a = mf_base<double>(c);
+    a.x = x;
+    a.y = y;
+    kmf::apply(a, 0, c.stream);
+  } else {
+    k::spmv(c.K, x, y, c.stream);
+  }
+}
+
+void op_residual(Context& c, const double* x, const double* b, double* r) { level_residual<double>(c, 0, x, b, r); }
+
+int op_dot_blocks(const Context& c) {
+  return c.mf ? kmf::blocks(c.nx + 1, c.ny + 1, c.nz + 1) : k::spmv_dot_blocks(c.K);
+}
+
+void op_apply_dot(Context& c, const double* x, double* y, double* partials) {
+  if (c.mf) {
+    kmf::MfArgs<double> a = mf_base<double>(c);
+    a.x = x;
+    a.y = y;
+    a.partials = partials;
+    kmf::apply(a, 1, c.stream);
+  } else {
+    k::spmv_dot(c.K, x, y, partials, c.stream);
+  }
+}
+
+void op_lanczos(Context& c, const double* sq, const double* v, const double* vprev, const double* beta, double* w) {
+  if (c.mf) {
+    kmf::MfArgs<double> a = mf_base<double>(c);
+    a.x = v;
+    a.y = w;
+    a.in_scale = sq;
+    a.vprev = vprev;
+    a.beta = beta;
+    kmf::apply(a, 4, c.stream);
+  } else {
+    k::lanczos_apply(c.K, sq, v, vprev, beta, w, c.stream);
+  }
+}
+
 void invalidate_graphs(Context& c) {
   for (hipGraphExec_t* g : {&c.g_spmv, &c.g_update, &c.g_precond}) {
     if (*g) (void)hipGraphExecDestroy(*g);
@@ -195,7 +295,7 @@ void ensure_workspace(Context& c) {
   c.q.alloc(n);
   c.xw.alloc(n);
   c.bw.alloc(n);
-  const size_t need = std::max<size_t>(4096, (size_t)k::spmv_dot_blocks(c.K) + 8);
+  const size_t need = std::max<size_t>(4096, (size_t)std::max(k::spmv_dot_blocks(c.K), op_dot_blocks(c)) + 8);
   if (c.partials.n < need) {
     invalidate_graphs(c);
     c.partials.alloc(need);
@@ -215,8 +315,10 @@ void prepare_mixed(Context& c) {
   for (size_t l = 0; l < c.levels.size(); ++l) {
     Level& lv = c.levels[l];
     DCsr& A = op(c, l);
-    A.val32.alloc((size_t)A.nnz);
-    k::to_float(A.nnz, A.val.p, A.val32.p, s);
+    if (!use_mf(c, l)) {
+      A.val32.alloc((size_t)A.nnz);
+      k::to_float(A.nnz, A.val.p, A.val32.p, s);
+    }
     if (lv.has_next) {
       lv.P.val32.alloc((size_t)lv.P.nnz);
       k::to_float(lv.P.nnz, lv.P.val.p, lv.P.val32.p, s);
@@ -293,11 +395,11 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
   int total = 0;
   bool recursive_converged = false;
   double best_true = std::numeric_limits<double>::max();
-  const int nb = k::spmv_dot_blocks(c.K);
+  const int nb = op_dot_blocks(c);
   double* x = c.xpcg.p;
   k::copy(n, x_out, x, s);
   // the three parts of one iteration (Multigrid.cpp:1184-1216)
-  auto body_spmv = [&] { k::spmv_dot(c.K, c.p.p, c.q.p, c.partials.p, s); };
+  auto body_spmv = [&] { op_apply_dot(c, c.p.p, c.q.p, c.partials.p); };
   auto body_update = [&] {
     k::reduce(nb, c.partials.p, c.scal.p + S_PQ, 0, s);
     k::pcg_update(n, x, c.r.p, c.p.p, c.q.p, c.scal.p, S_RZ, S_PQ, c.partials.p, c.scal.p + S_RR, s);
@@ -325,7 +427,7 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
     c.g_precond = capture(c, body_precond);
   }
   for (int restart = 0; restart < 3; ++restart) {
-    k::residual(c.K, x, b, c.r.p, s);
+    op_residual(c, x, b, c.r.p);
     double rnorm = std::sqrt(device_dot(c, n, c.r.p, c.r.p));
     res.relative_residual = rnorm / bnorm;
     if (rnorm <= target) {
@@ -359,7 +461,7 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
       }
       if (graphs) SLX_HIP(hipGraphLaunch(c.g_precond, s)); else body_precond();
     }
-    k::residual(c.K, x, b, c.r.p, s);
+    op_residual(c, x, b, c.r.p);
     res.relative_residual = std::sqrt(device_dot(c, n, c.r.p, c.r.p)) / bnorm;
     if (res.relative_residual <= tol) {
       res.converged = 1;

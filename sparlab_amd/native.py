@@ -96,6 +96,24 @@ class RunOptions(C.Structure):
                 ("continuation_iterations", C.c_int), ("oc", OcOptions), ("eval", EvalOptions)]
 
 
+class WarmStartOptions(C.Structure):
+    _fields_ = [("enable", C.c_int), ("train_solves", C.c_int), ("min_samples", C.c_int),
+                ("steps_per_solve", C.c_int), ("buffer", C.c_int), ("learning_rate", C.c_double),
+                ("seed", C.c_ulonglong)]
+
+
+class WarmStartStats(C.Structure):
+    _fields_ = [("samples", C.c_int), ("train_steps", C.c_int), ("first_loss", C.c_double),
+                ("last_loss", C.c_double), ("predictions", C.c_int), ("rejected", C.c_int),
+                ("step_sum", C.c_double), ("energy_ratio_sum", C.c_double), ("last_energy_ratio", C.c_double),
+                ("net_ratio_sum", C.c_double), ("secant_ratio_sum", C.c_double),
+                ("residual_ratio_sum", C.c_double), ("train_seconds", C.c_double),
+                ("predict_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 # every entry point of include/sparlab_hip.h with its ctypes signature
@@ -116,6 +134,7 @@ _SIGNATURES = {
     "slx_free_dofs_get": (C.c_int, [_VP, _VP]),
     "slx_assemble": (C.c_int, [_VP, _VP]),
     "slx_kff_get": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "slx_kff_apply": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_int)]),
     "slx_amg_setup": (C.c_int, [_VP, C.POINTER(AmgOptions)]),
     "slx_factorize_csr": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP, _VP,
                                     C.POINTER(AmgOptions)]),
@@ -133,6 +152,17 @@ _SIGNATURES = {
     "slx_evaluate": (C.c_int, [_VP, _VP, C.POINTER(EvalOptions), C.POINTER(Evaluation), _VP, _VP, _VP]),
     "slx_oc_update": (C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(OcOptions), _VP, C.POINTER(OcStep)]),
     "slx_run_oc": (C.c_int, [_VP, C.POINTER(RunOptions), _VP, _VP, _VP, _VP]),
+    "slx_warmstart_default_options": (None, [C.POINTER(WarmStartOptions)]),
+    "slx_warmstart_configure": (C.c_int, [_VP, C.POINTER(WarmStartOptions)]),
+    "slx_warmstart_stats_get": (C.c_int, [_VP, C.POINTER(WarmStartStats)]),
+    "slx_net_configure": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong]),
+    "slx_net_parameter_count": (C.c_int, [_VP, C.POINTER(C.c_longlong)]),
+    "slx_net_parameters_get": (C.c_int, [_VP, _VP]),
+    "slx_net_parameters_set": (C.c_int, [_VP, _VP]),
+    "slx_net_forward": (C.c_int, [_VP, _VP, _VP]),
+    "slx_net_gradients": (C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(C.c_double), _VP]),
+    "slx_net_train": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_double, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)]),
     "slx_timing_enable": (C.c_int, [_VP, C.c_int]),
     "slx_timing_get": (C.c_int, [_VP, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "slx_synchronize": (C.c_int, [_VP]),
@@ -225,6 +255,69 @@ class Context:
 
     def synchronize(self):
         self.check(load().slx_synchronize(self.h))
+
+
+def warmstart_options(**kw) -> WarmStartOptions:
+    o = WarmStartOptions()
+    load().slx_warmstart_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class WarmNet:
+    """The warm-start network on an X x Y x Z node grid (the slx_net_* entry points).
+
+    Parameters are FP32, per layer weights [cout][27][cin] then biases [cout];
+    inputs [V][nin], outputs and targets [V][nout], voxel = (z*Y + y)*X + x.
+    """
+
+    LAYERS = ((16, 16), (16, 32), (32, 64), (64, 32), (32, 16), (16, 16))
+
+    def __init__(self, X, Y, Z, nin, nout, seed=1, ctx: Context | None = None):
+        self.ctx = ctx or Context(0)
+        self.shape = (X, Y, Z)
+        self.V = X * Y * Z
+        self.nin, self.nout = nin, nout
+        self.ctx.check(load().slx_net_configure(self.ctx.h, X, Y, Z, nin, nout, seed))
+        n = C.c_longlong()
+        self.ctx.check(load().slx_net_parameter_count(self.ctx.h, C.byref(n)))
+        self.nparams = n.value
+
+    def parameters(self):
+        p = np.empty(self.nparams, dtype=np.float32)
+        self.ctx.check(load().slx_net_parameters_get(self.ctx.h, _p(p)))
+        return p
+
+    def set_parameters(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        self.ctx.check(load().slx_net_parameters_set(self.ctx.h, _p(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.V, self.nin)
+        y = np.empty((self.V, self.nout), dtype=np.float32)
+        self.ctx.check(load().slx_net_forward(self.ctx.h, _p(x), _p(y)))
+        return y
+
+    def _xtw(self, x, t, w):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.V, self.nin)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(self.V, self.nout)
+        w = None if w is None else np.ascontiguousarray(w, dtype=np.float32).reshape(self.V)
+        return x, t, w
+
+    def gradients(self, x, t, w=None):
+        x, t, w = self._xtw(x, t, w)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = C.c_double()
+        self.ctx.check(load().slx_net_gradients(self.ctx.h, _p(x), _p(t), _p(w), C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train(self, x, t, steps, lr=1e-3, w=None):
+        x, t, w = self._xtw(x, t, w)
+        a, b = C.c_double(), C.c_double()
+        self.ctx.check(load().slx_net_train(self.ctx.h, _p(x), _p(t), _p(w), steps, lr, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
 
 class HipAmgCgSolver:
@@ -337,6 +430,13 @@ class HipProblem:
         self.ctx.check(load().slx_kff_get(self.ctx.h, _p(ptr), _p(col), _p(val)))
         return ptr, col, val
 
+    def kff_apply(self, x):
+        """y = K_ff x with the solver's operator; returns (y, matrix_free)."""
+        y = np.empty(self.nfree)
+        mf = C.c_int()
+        self.ctx.check(load().slx_kff_apply(self.ctx.h, _p(_f64(x)), _p(y), C.byref(mf)))
+        return y, bool(mf.value)
+
     def amg_setup(self, **amg):
         self.ctx.check(load().slx_amg_setup(self.ctx.h, C.byref(amg_options(**amg))))
 
@@ -390,6 +490,16 @@ class HipProblem:
         y = np.empty(self.ne)
         self.ctx.check(load().slx_filter_apply(self.ctx.h, 1 if adjoint else 0, _p(_f64(x)), _p(y)))
         return y
+
+    def warm_start(self, enable=True, **kw):
+        """Turn the learned warm start on (or off) and reset its network."""
+        o = warmstart_options(enable=1 if enable else 0, **kw)
+        self.ctx.check(load().slx_warmstart_configure(self.ctx.h, C.byref(o)))
+
+    def warm_start_stats(self):
+        st = WarmStartStats()
+        self.ctx.check(load().slx_warmstart_stats_get(self.ctx.h, C.byref(st)))
+        return st.as_dict()
 
     def evaluate(self, x, tol=1e-8, need_gradients=True, warm_start=True, penalty=None, max_iterations=1000):
         o = eval_options(tolerance=tol, warm_start=1 if warm_start else 0, max_iterations=max_iterations,

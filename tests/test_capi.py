@@ -44,7 +44,8 @@ def test_binding_struct_layout_matches_header(tmp_path):
     structs = {"slx_amg_options": native.AmgOptions, "slx_pcg_result": native.PcgResult,
                "slx_amg_stats": native.AmgStats, "slx_eval_options": native.EvalOptions,
                "slx_evaluation": native.Evaluation, "slx_oc_options": native.OcOptions,
-               "slx_oc_step": native.OcStep, "slx_run_options": native.RunOptions}
+               "slx_oc_step": native.OcStep, "slx_run_options": native.RunOptions,
+               "slx_warmstart_options": native.WarmStartOptions, "slx_warmstart_stats": native.WarmStartStats}
     src = tmp_path / "layout.c"
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "sparlab_hip.h"', "int main(void) {"]
     for cname, cls in structs.items():

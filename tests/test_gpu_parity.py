@@ -175,6 +175,42 @@ def test_mixed_precision_solve_meets_tolerance(ctx):
     assert it32 <= it64 + 3
 
 
+def test_matrix_free_operator_matches_assembled_kff(ctx):
+    """The matrix-free Hex8 product equals the assembled K_ff's (partial tiles:
+    node counts that are not multiples of the 8 x 8 x 4 tile)."""
+    print("test_matrix_free_operator_matches_assembled_kff", flush=True)
+    m = M.cantilever_3d(20, 9, 6)
+    g = N.HipProblem(m, ctx=ctx)
+    g.assemble(1e-9 + (1 - 1e-9) * M.seeded_density(m.nelem, seed=21) ** 3)
+    K = _csr(g.nfree, g.nfree, *g.kff())
+    rng = np.random.default_rng(8)
+    for _ in range(2):
+        x = rng.standard_normal(g.nfree)
+        y, mf = g.kff_apply(x)
+        assert mf
+        ref = K @ x
+        np.testing.assert_allclose(y, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+def test_matrix_free_and_assembled_solves_agree(ctx, monkeypatch):
+    print("test_matrix_free_and_assembled_solves_agree", flush=True)
+    m = M.cantilever_3d(24, 12, 12)
+    scale = 1e-9 + (1 - 1e-9) * M.seeded_density(m.nelem, seed=22) ** 3
+    b = m.free_loads()[0]
+    out = {}
+    for mode in ("mf", "csr"):
+        if mode == "csr":
+            monkeypatch.setenv("SPARLAB_NO_MATRIX_FREE", "1")
+        g = N.HipProblem(m, ctx=ctx)
+        g.assemble(scale)
+        assert g.kff_apply(np.ones(g.nfree))[1] == (mode == "mf")
+        g.amg_setup()
+        out[mode] = g.solve(b, tol=1e-10)
+    (xm, im, rm, cm), (xc, ic, rc, cc) = out["mf"], out["csr"]
+    assert cm and cc and abs(im - ic) <= 1
+    assert np.linalg.norm(xm - xc) <= 1e-9 * np.linalg.norm(xc)
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_captured_cg_graphs_match_eager_launches(ctx, precision, monkeypatch):
     """The CG iteration replayed from HIP graphs gives the eager launches' bits,
