@@ -15,6 +15,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -605,20 +606,74 @@ bool run_table(const HCsr& m, const std::vector<int>& node_ptr, const std::vecto
   return true;
 }
 
-int max_blocks(const std::vector<int>& bptr) {
-  int mx = 0;
-  for (size_t i = 0; i + 1 < bptr.size(); ++i) mx = std::max(mx, bptr[i + 1] - bptr[i]);
-  return mx;
-}
-
-int slots_for(const HCsr& c, const std::vector<int>& node_ptr) {
-  long long mx = 0;
-  for (size_t nd = 0; nd + 1 < node_ptr.size(); ++nd) {
-    const int r0 = node_ptr[nd];
-    mx = std::max<long long>(mx, (long long)(node_ptr[nd + 1] - r0) * (c.ptr[(size_t)r0 + 1] - c.ptr[(size_t)r0]));
+// Contribution lists of C = A B: the items are C's blocks (block lists cbp /
+// cba / cbq over the C nodes); item q of node I lists, in the order of A's row,
+// every run t of I's A row whose B node J holds q's aggregate, with that
+// block's position in J's rows. False if a list entry does not fit 16 + 16 bits
+// or a product block is missing from C's pattern.
+bool build_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& cbp, const std::vector<int>& cba,
+                const std::vector<int>& cbq, const std::vector<int>& run_ptr, const std::vector<int>& runs,
+                const std::vector<int>& bbp, const std::vector<int>& bba, const std::vector<int>& bbq,
+                const std::vector<int>& bptr, const std::vector<int>& agg_off, ProductPlan& plan, hipStream_t s) {
+  const int nn = (int)cnode_ptr.size() - 1;
+  const int naggs = (int)agg_off.size() - 1;
+  const long long nitems = cbp[(size_t)nn];
+  std::vector<long long> begin((size_t)nitems + 1, 0);
+  std::atomic<bool> ok{true};
+  auto walk = [&](bool fill, std::vector<uint2>* list, std::vector<long long>* pos) {
+#pragma omp parallel
+    {
+      std::vector<int> item_of((size_t)naggs, -1);
+#pragma omp for schedule(dynamic, 256)
+      for (int I = 0; I < nn; ++I) {
+        for (int q = cbp[(size_t)I]; q < cbp[(size_t)I + 1]; ++q) item_of[(size_t)cba[(size_t)q]] = q;
+        for (int t = run_ptr[(size_t)I]; t < run_ptr[(size_t)I + 1]; ++t) {
+          const int koff = runs[(size_t)4 * t], brow = runs[(size_t)4 * t + 1], dj = runs[(size_t)4 * t + 2];
+          const int J = runs[(size_t)4 * t + 3];
+          const int lb = bptr[(size_t)brow + 1] - bptr[(size_t)brow];
+          for (int u = bbp[(size_t)J]; u < bbp[(size_t)J + 1]; ++u) {
+            const int q = item_of[(size_t)bba[(size_t)u]];
+            if (q < 0 || koff > 0xffff || lb > 0x1fff || dj > 7) {
+              ok.store(false, std::memory_order_relaxed);
+              continue;
+            }
+            if (fill)
+              (*list)[(size_t)(*pos)[(size_t)q]++] =
+                  make_uint2((unsigned)(bptr[(size_t)brow] + bbq[(size_t)u]),
+                             ((unsigned)koff << 16) | ((unsigned)lb << 3) | (unsigned)dj);
+            else
+              ++begin[(size_t)q + 1];
+          }
+        }
+        for (int q = cbp[(size_t)I]; q < cbp[(size_t)I + 1]; ++q) item_of[(size_t)cba[(size_t)q]] = -1;
+      }
+    }
+  };
+  walk(false, nullptr, nullptr);
+  if (!ok.load()) return false;
+  for (long long q = 0; q < nitems; ++q) begin[(size_t)q + 1] += begin[(size_t)q];
+  std::vector<uint2> list((size_t)begin[(size_t)nitems]);
+  std::vector<long long> pos(begin.begin(), begin.end() - 1);
+  walk(true, &list, &pos);
+  std::vector<int> node((size_t)nitems);
+  int dmax = 1, wmax = 1;
+  for (int I = 0; I < nn; ++I) {
+    dmax = std::max(dmax, cnode_ptr[(size_t)I + 1] - cnode_ptr[(size_t)I]);
+    for (int q = cbp[(size_t)I]; q < cbp[(size_t)I + 1]; ++q) {
+      node[(size_t)q] = I;
+      const int g = cba[(size_t)q];
+      wmax = std::max(wmax, agg_off[(size_t)g + 1] - agg_off[(size_t)g]);
+    }
   }
-  const long long ns = (mx + 63) / 64;
-  return ns <= 1 ? 1 : ns <= 2 ? 2 : ns <= 4 ? 4 : ns <= 8 ? 8 : 16;  // wider rows take several passes
+  if (dmax * wmax > 64) return false;
+  plan.nitems = nitems;
+  plan.gsize = dmax * wmax;
+  plan.item_node.upload(node, s);
+  plan.item_agg.upload(cba, s);
+  plan.item_pos.upload(cbq, s);
+  plan.item_begin.upload(begin, s);
+  plan.list.upload(list, s);
+  return true;
 }
 
 void prepare_node_products(Level& lv, const Level& next, const HCsr& a, const HCsr& pt, const HCsr& ppat,
@@ -631,72 +686,46 @@ void prepare_node_products(Level& lv, const Level& next, const HCsr& a, const HC
       !node_uniform(rpat, next.node_ptr) || !node_uniform(rawpat, next.node_ptr))
     return;
   if (!runs_complete(a, lv.node_ptr, f_of) || !runs_complete(rpat, lv.node_ptr, f_of)) return;
-  std::vector<int> tp, ta, tq, pp, pa, pq, qp, qa, qq;
+  std::vector<int> tp, ta, tq, pp, pa, pq, qp, qa, qq, wp, wa, wq;
   if (!block_lists(pt, lv.node_ptr, c_of, off, tp, ta, tq) || !block_lists(ppat, lv.node_ptr, c_of, off, pp, pa, pq) ||
-      !block_lists(appat, lv.node_ptr, c_of, off, qp, qa, qq))
+      !block_lists(appat, lv.node_ptr, c_of, off, qp, qa, qq) ||
+      !block_lists(rawpat, next.node_ptr, c_of, off, wp, wa, wq))
     return;
   std::vector<int> arp, ar, rrp, rr;
   if (!run_table(a, lv.node_ptr, lv.node_ptr, f_of, arp, ar) || !run_table(rpat, next.node_ptr, lv.node_ptr, f_of, rrp, rr))
     return;
-  lv.maxblk_p = max_blocks(pp);
-  lv.maxblk_ap = max_blocks(qp);
-  if (lv.maxblk_p > 1024 || lv.maxblk_ap > 1024 || max_blocks(tp) != 1) return;
-  lv.a_run_ptr.upload(arp, s);
-  lv.a_runs.upload(ar, s);
-  lv.r_run_ptr.upload(rrp, s);
-  lv.r_runs.upload(rr, s);
-  lv.ns_p = slots_for(ppat, lv.node_ptr);
-  lv.ns_ap = slots_for(appat, lv.node_ptr);
-  lv.ns_raw = slots_for(rawpat, next.node_ptr);
-  if (!lv.ns_p || !lv.ns_ap || !lv.ns_raw) return;
+  for (size_t nd = 0; nd + 1 < tp.size(); ++nd)
+    if (tp[nd + 1] - tp[nd] != 1) return;  // P-hat: one block per node
+  if (!build_plan(lv.node_ptr, pp, pa, pq, arp, ar, tp, ta, tq, pt.ptr, off, lv.plan_p, s) ||
+      !build_plan(lv.node_ptr, qp, qa, qq, arp, ar, pp, pa, pq, ppat.ptr, off, lv.plan_ap, s) ||
+      !build_plan(next.node_ptr, wp, wa, wq, rrp, rr, qp, qa, qq, appat.ptr, off, lv.plan_raw, s))
+    return;
   std::vector<int> row_agg((size_t)lv.n);
   for (int i = 0; i < lv.n; ++i) row_agg[(size_t)i] = c_of[(size_t)pt.col[(size_t)pt.ptr[(size_t)i]]];
   lv.f_node_ptr.upload(lv.node_ptr, s);
-  lv.f_node_of_row.upload(f_of, s);
   lv.c_node_ptr.upload(next.node_ptr, s);
-  lv.c_node_of_row.upload(c_of, s);
   lv.row_agg.upload(row_agg, s);
-  lv.pt_blk_ptr.upload(tp, s);
-  lv.pt_blk_agg.upload(ta, s);
-  lv.pt_blk_pos.upload(tq, s);
-  lv.p_blk_ptr.upload(pp, s);
-  lv.p_blk_agg.upload(pa, s);
-  lv.p_blk_pos.upload(pq, s);
-  lv.ap_blk_ptr.upload(qp, s);
-  lv.ap_blk_agg.upload(qa, s);
-  lv.ap_blk_pos.upload(qq, s);
   lv.node_ok = true;
 }
 
-k::NodeProductArgs node_args(const Level& lv, const DCsr& A, bool a_is_r, const int* bblk_ptr, const int* bblk_agg,
-                             const int* bblk_pos, const DCsr& B, DCsr& C, int* flag) {
-  k::NodeProductArgs g{};
-  g.nodes = a_is_r ? (int)lv.c_node_ptr.n - 1 : (int)lv.f_node_ptr.n - 1;
+k::BlockProductArgs plan_args(const Level& lv, const ProductPlan& plan, bool a_is_r, const DCsr& A, const DCsr& B,
+                              DCsr& C) {
+  k::BlockProductArgs g{};
+  g.nitems = plan.nitems;
+  g.gsize = plan.gsize;
+  g.item_node = plan.item_node.p;
+  g.item_agg = plan.item_agg.p;
+  g.item_pos = plan.item_pos.p;
+  g.item_begin = plan.item_begin.p;
+  g.list = plan.list.p;
   g.out_node_ptr = a_is_r ? lv.c_node_ptr.p : lv.f_node_ptr.p;
   g.aptr = A.ptr.p;
-  g.acol = A.col.p;
   g.aval = A.val.p;
-  g.bnode_of_row = lv.f_node_of_row.p;
-  g.bnode_ptr = lv.f_node_ptr.p;
-  g.bblk_ptr = bblk_ptr;
-  g.bblk_agg = bblk_agg;
-  g.bblk_pos = bblk_pos;
-  g.bptr = B.ptr.p;
   g.bval = B.val.p;
   g.cptr = C.ptr.p;
-  g.ccol = C.col.p;
   g.cval = C.val.p;
-  g.agg_of_col = lv.c_node_of_row.p;
   g.agg_off = lv.c_node_ptr.p;
-  g.error_flag = flag;
-  g.run_ptr = a_is_r ? lv.r_run_ptr.p : lv.a_run_ptr.p;
-  g.runs = reinterpret_cast<const int4*>(a_is_r ? lv.r_runs.p : lv.a_runs.p);
   return g;
-}
-
-void set_chunk(k::NodeProductArgs& g, int maxblk) {
-  g.chunk = std::max(1, std::min(64, 1024 / std::max(maxblk, 1)));
-  g.blk_cap = g.chunk * std::max(maxblk, 1);
 }
 
 DCsr& level_op(Context& c, size_t l) { return l == 0 ? c.K : c.levels[l].A; }
@@ -934,25 +963,17 @@ void amg_setup(Context& c) {
     flag.upload(&zero, 1, s);
     const double omega = o.prolongator_damping / cur.lambda_max;
     if (cur.node_ok) {
-      k::NodeProductArgs g = node_args(cur, A, false, cur.pt_blk_ptr.p, cur.pt_blk_agg.p, cur.pt_blk_pos.p, cur.Ptent,
-                                       cur.P, flag.p);
+      k::BlockProductArgs g = plan_args(cur, cur.plan_p, false, A, cur.Ptent, cur.P);
       g.dinv = cur.dinv.p;
       g.omega = omega;
       g.tptr = cur.Ptent.ptr.p;
       g.tval = cur.Ptent.val.p;
       g.row_agg = cur.row_agg.p;
-      set_chunk(g, 1);
-      k::node_product(g, cur.ns_p, true, s);
+      k::block_product(g, true, s);
       k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
       lap(3);
-      k::NodeProductArgs g2 = node_args(cur, A, false, cur.p_blk_ptr.p, cur.p_blk_agg.p, cur.p_blk_pos.p, cur.P, cur.AP,
-                                        flag.p);
-      set_chunk(g2, cur.maxblk_p);
-      k::node_product(g2, cur.ns_ap, false, s);
-      k::NodeProductArgs g3 = node_args(cur, cur.R, true, cur.ap_blk_ptr.p, cur.ap_blk_agg.p, cur.ap_blk_pos.p, cur.AP,
-                                        cur.RAW, flag.p);
-      set_chunk(g3, cur.maxblk_ap);
-      k::node_product(g3, cur.ns_raw, false, s);
+      k::block_product(plan_args(cur, cur.plan_ap, false, A, cur.P, cur.AP), false, s);
+      k::block_product(plan_args(cur, cur.plan_raw, true, cur.R, cur.AP, cur.RAW), false, s);
     } else {
       k::smooth_prolongator(A, cur.Ptent, cur.P, cur.dinv.p, omega, flag.p, s);
       k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);

@@ -134,120 +134,58 @@ __global__ void k_inv_diag(int n, const int* __restrict__ ptr, const int* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Node-row products. Every operator of the hierarchy has rows grouped in
+// Block products C = A B. Every operator of the hierarchy has rows grouped in
 // nodes (the free DOFs of a mesh node, the coarse DOFs of an aggregate) whose
 // rows share one column pattern, and the columns of the products fall in
-// aggregate blocks (contiguous coarse columns). One wavefront computes one
-// node-row of C = A B: each lane owns up to NS entries of the node's rows;
-// the A row is walked one B-node at a time (a run of that node's rows), the
-// B-node's block list tells each lane where its aggregate sits in those B
-// rows, and the products are added in the order of A's row, the order of the
-// CPU restatement.
+// aggregate blocks (contiguous coarse columns). An item is one block of one
+// C node-row; its contribution list, built once per sparsity structure, holds
+// one entry per (A run, B block) pair that feeds it, in the order of A's row. A
+// run is one B-node's stretch of A's row; the entry carries where it starts in
+// A's row, how many B rows it spans, the offset of the item's aggregate block
+// in the first of those B rows and the B row length (node-uniform rows), so a
+// lane needs no further index lookups. One group of rows x width lanes computes
+// an item: lane (r, t) sums A(row r, run) . B(run rows, block column t) over
+// the list, in the order of A's row, the order of the CPU restatement.
 // EPI 1 forms P = (I - omega D^-1 A) P-hat with B = P-hat (Multigrid.cpp:686-701).
 // ---------------------------------------------------------------------------
-template <int EPI, int NS>
-__global__ void __launch_bounds__(kWave) k_node_product(NodeProductArgs g) {
-  extern __shared__ int smem_i[];
-  // stage layout: per run (koff, rows, first block, blocks), 6 B-row starts, then the blocks
-  int* s_koff = smem_i;
-  int* s_dj = s_koff + g.chunk;
-  int* s_b0 = s_dj + g.chunk;
-  int* s_nb = s_b0 + g.chunk;
-  int* s_brow = s_nb + g.chunk;
-  int2* s_blk = reinterpret_cast<int2*>(s_brow + 6 * g.chunk + ((6 * g.chunk) & 1));
-  const int lane = threadIdx.x;
-  for (int I = blockIdx.x; I < g.nodes; I += gridDim.x) {
-    const int r0 = g.out_node_ptr[I];
-    const int d = g.out_node_ptr[I + 1] - r0;
-    const int L = g.cptr[r0 + 1] - g.cptr[r0];
-    const int total = d * L;
-    const int ka0 = g.aptr[r0];
-    const int q0 = g.run_ptr[I], q1 = g.run_ptr[I + 1];
-    for (int base = 0; base < total; base += kWave * NS) {
-      double acc[NS];
-      int my_a[NS], my_t[NS], bpos[NS], arow[NS], cidx[NS];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const int f = base + lane + kWave * s;
-        acc[s] = 0.0;
-        my_a[s] = -1;
-        my_t[s] = 0;
-        arow[s] = 0;
-        cidx[s] = -1;
-        if (f < total) {
-          const int r = f / L, pp = f - r * L;
-          const int row = r0 + r;
-          const int ci = g.cptr[row] + pp;
-          const int c = g.ccol[ci];
-          my_a[s] = g.agg_of_col[c];
-          my_t[s] = c - g.agg_off[my_a[s]];
-          arow[s] = g.aptr[row] - ka0;
-          cidx[s] = ci;
-        }
-      }
-      for (int qc = q0; qc < q1; qc += g.chunk) {
-        const int cnt = min(g.chunk, q1 - qc);
-        // stage the chunk's run metadata, B-row starts and block lists (all lanes load in parallel)
-        for (int t = lane; t < cnt; t += kWave) {
-          const int4 rn = g.runs[qc + t];
-          s_koff[t] = rn.x;
-          s_dj[t] = rn.z;
-          const int b0 = g.bblk_ptr[rn.w];
-          s_b0[t] = b0;
-          s_nb[t] = g.bblk_ptr[rn.w + 1] - b0;
-          for (int m = 0; m < rn.z; ++m) s_brow[t * 6 + m] = g.bptr[rn.y + m];
-        }
-        __syncthreads();
-        // block lists, packed in run order
-        if (lane == 0) {
-          int off = 0;
-          for (int t = 0; t < cnt; ++t) {
-            const int nb = s_nb[t];
-            s_nb[t] = off;  // now the start in the stage
-            off += nb;
-            s_dj[t] = s_dj[t] | (nb << 8);
-          }
-          if (off > g.blk_cap) atomicOr(g.error_flag, 16);
-        }
-        __syncthreads();
-        for (int t = 0; t < cnt; ++t) {
-          const int nb = s_dj[t] >> 8, at = s_nb[t], gb = s_b0[t];
-          for (int u = lane; u < nb; u += kWave)
-            if (at + u < g.blk_cap) s_blk[at + u] = make_int2(g.bblk_agg[gb + u], g.bblk_pos[gb + u]);
-        }
-        __syncthreads();
-        for (int t = 0; t < cnt; ++t) {
-          const int dj = s_dj[t] & 255, nb = s_dj[t] >> 8, at = s_nb[t];
-#pragma unroll
-          for (int s = 0; s < NS; ++s) bpos[s] = -1;
-          for (int u = 0; u < nb; ++u) {
-            const int2 bl = s_blk[min(at + u, g.blk_cap - 1)];
-#pragma unroll
-            for (int s = 0; s < NS; ++s) bpos[s] = my_a[s] == bl.x ? bl.y : bpos[s];
-          }
-          const int kbase = ka0 + s_koff[t];
-          for (int m = 0; m < dj; ++m) {
-            const int brow = s_brow[t * 6 + m];
-#pragma unroll
-            for (int s = 0; s < NS; ++s)
-              if (bpos[s] >= 0) acc[s] += g.aval[kbase + m + arow[s]] * g.bval[brow + bpos[s] + my_t[s]];
-          }
-        }
-        __syncthreads();
-      }
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        if (cidx[s] < 0) continue;
-        double v = acc[s];
-        if (EPI == 1) {
-          const int row = r0 + (base + lane + kWave * s) / L;
-          v *= -(g.omega * g.dinv[row]);
-          if (my_a[s] == g.row_agg[row]) v += g.tval[g.tptr[row] + my_t[s]];
-        }
-        g.cval[cidx[s]] = v;
-      }
+template <int EPI>
+__global__ void __launch_bounds__(256) k_block_product(BlockProductArgs g) {
+  const int lane = threadIdx.x & 63;
+  const int gpw = kWave / g.gsize;  // groups per wavefront
+  const int grp = lane / g.gsize, gl = lane - grp * g.gsize;
+  if (grp >= gpw) return;
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long item = wave * gpw + grp;
+  if (item >= g.nitems) return;
+  const int node = g.item_node[item];
+  const int agg = g.item_agg[item];
+  const int w = g.agg_off[agg + 1] - g.agg_off[agg];
+  const int r0 = g.out_node_ptr[node];
+  const int r = gl / w, t = gl - r * w;
+  if (r >= g.out_node_ptr[node + 1] - r0) return;
+  const int row = r0 + r;
+  const double* arow = g.aval + g.aptr[row];
+  double acc = 0.0;
+  const long long e1 = g.item_begin[item + 1];
+  for (long long e = g.item_begin[item]; e < e1; ++e) {
+    const uint2 en = g.list[e];
+    const double* a = arow + (en.y >> 16);
+    const double* b = g.bval + en.x + t;
+    const int lb = (int)((en.y >> 3) & 0x1fffu), dj = (int)(en.y & 7u);
+    if (dj == 3) {
+      acc += a[0] * b[0];
+      acc += a[1] * b[lb];
+      acc += a[2] * b[2 * lb];
+    } else {
+      for (int m = 0; m < dj; ++m) acc += a[m] * b[m * lb];
     }
   }
+  double v = acc;
+  if (EPI == 1) {
+    v *= -(g.omega * g.dinv[row]);
+    if (agg == g.row_agg[row]) v += g.tval[g.tptr[row] + t];
+  }
+  g.cval[g.cptr[row] + g.item_pos[item] + t] = v;
 }
 
 // Lanczos on D^-1/2 A D^-1/2 (Multigrid.cpp:304-317). The step's scalars stay
@@ -306,24 +244,14 @@ __global__ void k_copy(long long n, const double* __restrict__ in, double* __res
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
-template <int EPI>
-void launch_node_product(const NodeProductArgs& a, int ns, hipStream_t s) {
-  const int grid = std::min(a.nodes, 256 * 128);
-  if (a.nodes == 0) return;
-  const size_t lds = sizeof(int) * (size_t)(10 * a.chunk + 2) + sizeof(int2) * (size_t)a.blk_cap;
-  switch (ns) {
-    case 1: k_node_product<EPI, 1><<<grid, kWave, lds, s>>>(a); break;
-    case 2: k_node_product<EPI, 2><<<grid, kWave, lds, s>>>(a); break;
-    case 4: k_node_product<EPI, 4><<<grid, kWave, lds, s>>>(a); break;
-    case 8: k_node_product<EPI, 8><<<grid, kWave, lds, s>>>(a); break;
-    default: k_node_product<EPI, 16><<<grid, kWave, lds, s>>>(a); break;
-  }
-}
-
 }  // namespace
 
-void node_product(const NodeProductArgs& a, int ns, bool smooth, hipStream_t s) {
-  if (smooth) launch_node_product<1>(a, ns, s); else launch_node_product<0>(a, ns, s);
+void block_product(const BlockProductArgs& a, bool smooth, hipStream_t s) {
+  if (a.nitems == 0) return;
+  if (a.gsize < 1 || a.gsize > kWave) fail(SLX_SOLVER_ERROR, "multigrid: block product group wider than a wavefront");
+  const long long waves = (a.nitems + kWave / a.gsize - 1) / (kWave / a.gsize);
+  const int grid = (int)((waves + 3) / 4);
+  if (smooth) k_block_product<1><<<grid, 256, 0, s>>>(a); else k_block_product<0><<<grid, 256, 0, s>>>(a);
 }
 
 void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s) {

@@ -222,12 +222,19 @@ __global__ void k_colsum_partials(int V, int C, const float* __restrict__ g, dou
   }
 }
 
+// out[c] = sum_b part[b][c]: one block of 256 per channel, fixed summation order
 __global__ void k_colsum_final(int blocks, int C, const double* __restrict__ part, float* __restrict__ out) {
-  const int c = threadIdx.x;
-  if (c >= C) return;
-  double t = 0.0;
-  for (int b = 0; b < blocks; ++b) t += part[(size_t)b * C + c];
-  out[c] = (float)t;
+  __shared__ double s[256];
+  const int c = blockIdx.x;
+  double a = 0.0;
+  for (int b = threadIdx.x; b < blocks; b += 256) a += part[(size_t)b * C + c];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = (float)s[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -342,13 +349,19 @@ __global__ void k_mse(int V, int C, int nout, const float* __restrict__ y, const
   if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-// *out = scale * sum(part) / (den ? *den : 1)
+// *out = scale * sum(part) / (den ? *den : 1); one block of 256, fixed summation order
 __global__ void k_sum_double(int n, const double* __restrict__ part, double scale, const double* __restrict__ den,
                              double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s = 0.0;
-  for (int i = 0; i < n; ++i) s += part[i];
-  *out = s * scale / (den ? *den : 1.0);
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) a += part[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0] * scale / (den ? *den : 1.0);
 }
 
 // per-block partial sums of a float vector (256 per block)
@@ -506,7 +519,7 @@ void colsum(int V, int C, const float* g, double* part, float* out, hipStream_t 
   if (C > 256 || 256 % C) fail(SLX_SOLVER_ERROR, "net: unsupported channel count for a bias gradient");
   const int nb = colsum_blocks(V);
   k_colsum_partials<<<nb, 256, 0, s>>>(V, C, g, part);
-  k_colsum_final<<<1, 64, 0, s>>>(nb, C, part, out);
+  k_colsum_final<<<C, 256, 0, s>>>(nb, C, part, out);
 }
 
 void pool(int X, int Y, int Z, int C, const uint16_t* in, uint16_t* out, hipStream_t s) {
@@ -539,13 +552,13 @@ void mse(int V, int C, int nout, const float* y, const float* t, const float* w,
          uint16_t* gy16, double* part, double* loss, hipStream_t s) {
   const int nb = mse_blocks(V, C);
   k_mse<<<nb, 256, 0, s>>>(V, C, nout, y, t, w, wsum, gy, gy16, part);
-  k_sum_double<<<1, 64, 0, s>>>(nb, part, w ? 1.0 / nout : 1.0 / ((double)V * nout), w ? wsum : nullptr, loss);
+  k_sum_double<<<1, 256, 0, s>>>(nb, part, w ? 1.0 / nout : 1.0 / ((double)V * nout), w ? wsum : nullptr, loss);
 }
 
 void sum_float(int n, const float* a, double* part, double* out, hipStream_t s) {
   const int nb = blocks_for(n, 256);
   k_sum_float_partials<<<nb, 256, 0, s>>>(n, a, part);
-  k_sum_double<<<1, 64, 0, s>>>(nb, part, 1.0, nullptr, out);
+  k_sum_double<<<1, 256, 0, s>>>(nb, part, 1.0, nullptr, out);
 }
 
 void adam(long long n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2, float eps, int step,

@@ -86,6 +86,16 @@ struct DCsr {
   }
 };
 
+// Contribution lists of one block product C = A B (kernels_amg.hip): one item
+// per aggregate block of each C node-row, each with its (A run, B block) list.
+struct ProductPlan {
+  long long nitems = 0;
+  int gsize = 0;  // lanes per item
+  DBuf<int> item_node, item_agg, item_pos;
+  DBuf<long long> item_begin;
+  DBuf<uint2> list;  // two words per entry (kernels_amg.hip)
+};
+
 // One level of the SA-AMG hierarchy (structure of src/fem/Multigrid.cpp:336-354).
 // Structure (node blocks, aggregates, tentative prolongator, product patterns)
 // is built on the host once per sparsity structure; every numeric array lives
@@ -102,14 +112,11 @@ struct Level {
   DCsr Ptent, P, R, AP, RAW;
   DBuf<int> r_src;         // R.val[k] = P.val[r_src[k]]
   DBuf<int> sym_a, sym_b;  // next level's A from RAW (symmetrisation)
-  // node-row product metadata (see kernels_amg.hip); node_ok when every
-  // operator of this level has node-uniform rows in complete blocks
+  // block-product plans (see kernels_amg.hip); node_ok when every operator of
+  // this level has node-uniform rows in complete aggregate blocks
   bool node_ok = false;
-  int ns_p = 0, ns_ap = 0, ns_raw = 0;
-  int maxblk_p = 0, maxblk_ap = 0;
-  DBuf<int> a_run_ptr, a_runs, r_run_ptr, r_runs;  // run tables (runs hold 4 ints each)
-  DBuf<int> f_node_ptr, f_node_of_row, c_node_ptr, c_node_of_row, row_agg;
-  DBuf<int> pt_blk_ptr, pt_blk_agg, pt_blk_pos, p_blk_ptr, p_blk_agg, p_blk_pos, ap_blk_ptr, ap_blk_agg, ap_blk_pos;
+  DBuf<int> f_node_ptr, c_node_ptr, row_agg;
+  ProductPlan plan_p, plan_ap, plan_raw;  // P = (I - w D^-1 A) P-hat, A P, R (A P)
   DBuf<double> dinv, sq, lz_v, lz_vprev, lz_w, lz_start;
   DBuf<double> x, b, res, d, d2;
   DBuf<float> dinv32, x32, b32, res32, d32, d232;

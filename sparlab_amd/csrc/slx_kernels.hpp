@@ -64,38 +64,30 @@ void max_abs_diff(int n, const double* a, const double* b, double* partials, dou
 void clamp(int n, double* x, const double* lo, const double* hi, hipStream_t s);
 
 // SA-AMG numeric setup (kernels_amg.hip)
-struct NodeProductArgs {
-  int nodes;                  // node-rows of A and C
-  const int* out_node_ptr;    // rows of node I: [out_node_ptr[I], out_node_ptr[I+1])
+struct BlockProductArgs {
+  long long nitems;           // C blocks (one per aggregate block of each C node-row)
+  int gsize;                  // lanes per item: max rows per node x max block width
+  const int* item_node;       // C node of the item
+  const int* item_agg;        // its aggregate
+  const int* item_pos;        // position of the block within each row of the node
+  const long long* item_begin;  // contribution list of item i: [item_begin[i], item_begin[i+1])
+  // entry: x = index in B's values of the block in the run's first B row,
+  // y = (offset of the run in A's row << 16) | (B row length << 3) | B rows
+  const uint2* list;
+  const int* out_node_ptr;    // rows of C node I: [out_node_ptr[I], out_node_ptr[I+1])
   const int* aptr;
-  const int* acol;
   const double* aval;
-  const int* bnode_of_row;    // B row -> B node
-  const int* bnode_ptr;       // B node -> its first row
-  const int* bblk_ptr;        // B node -> range of its aggregate blocks
-  const int* bblk_agg;        // aggregate of each block
-  const int* bblk_pos;        // position of the block within each row of the node
-  const int* bptr;
   const double* bval;
   const int* cptr;
-  const int* ccol;
   double* cval;
-  const int* agg_of_col;      // coarse column -> aggregate
-  const int* agg_off;         // aggregate -> first coarse column
+  const int* agg_off;         // aggregate -> first coarse column (width = next - this)
   const double* dinv;         // prolongator smoothing only
   double omega;
   const int* tptr;
   const double* tval;
   const int* row_agg;         // aggregate of each fine row
-  int* error_flag;
-  // run table of the A-role matrix: per node-row the runs of its (shared)
-  // row pattern, one per B-node: (offset in the row, first B row, rows, B node)
-  const int* run_ptr;
-  const int4* runs;
-  int chunk;                  // runs staged per pass
-  int blk_cap;                // block entries the stage holds
 };
-void node_product(const NodeProductArgs& a, int ns, bool smooth, hipStream_t s);
+void block_product(const BlockProductArgs& a, bool smooth, hipStream_t s);
 void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s);
 void smooth_prolongator(const DCsr& A, const DCsr& Ptent, DCsr& P, const double* dinv, double omega, int* error_flag,
                         hipStream_t s);
