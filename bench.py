@@ -19,10 +19,12 @@ iterations' solves) and the plain FP64 GPU path (the drop-in: FP64 everywhere,
 warm start from the previous solution). `value` is the headline run's solves
 per second; the FP64 run's figure and the ratio sit beside it.
 
-The JSON line also carries the roofline of the dominant kernel (the FP64 CSR
-operator product q = K_ff p of PCG), timed with HIP events on the library's
-stream over the timed region, and the CPU baseline: the refcpu restatement of
-the reference algorithm timed on the host on one solve of the same workload.
+The JSON line also carries the roofline of the dominant kernel (the one with
+the largest total time among those timed with HIP events on the library's
+stream over the timed region: the level-0 Galerkin products and the
+matrix-free CG operator product), the others beside it, and the CPU baseline:
+the refcpu restatement of the reference algorithm timed on the host on a
+bounded sample of one solve of the same workload.
 """
 from __future__ import annotations
 
@@ -39,6 +41,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Hex8 1.6M-DOF AMG-PCG solves/s @rtol 1e-8: mixed+learned vs FP64 GPU"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64, vector and matrix
 
 
 def cpu_baseline(model, x_first, x_timed, tol, gpu_iterations, sample_iterations=6):
@@ -103,8 +106,7 @@ def simp_run(N, m, x0, warmup, steps, precision, learned, tol, label):
     x2, h2, done, c2 = g.run_oc(x1, steps, change_tol=0.0, tol=tol)
     g.ctx.synchronize()
     elapsed = time.perf_counter() - t0
-    spmv_ms, spmv_n = g.ctx.timer("spmv")
-    asm_ms, asm_n = g.ctx.timer("assemble")
+    timers = {name: g.ctx.timer(name) for name in ("spmv", "galerkin_ap", "galerkin_raw", "assemble")}
     g.ctx.timing(False)
     its = h2[:, 6].astype(int).tolist()
     print(f"bench[{label}]: {done} timed iterations in {elapsed:.3f} s, CG its {its}, c = {c2:.8e}",
@@ -116,8 +118,137 @@ def simp_run(N, m, x0, warmup, steps, precision, learned, tol, label):
     print(f"bench[{label}]: last AMG setup (ms) {phases}, unknowns {[st.unknowns[i] for i in range(st.levels)]}",
           file=sys.stderr, flush=True)
     return dict(problem=g, x_first=x1, x=x2, steps=done, elapsed=elapsed, iterations=its, compliance=c2,
-                spmv_ms=spmv_ms, spmv_n=spmv_n, assemble_ms=asm_ms, phases=phases,
-                warm=g.warm_start_stats() if learned else None)
+                timers=timers, phases=phases, warm=g.warm_start_stats() if learned else None)
+
+
+def level_info(g, level, which):
+    import ctypes as C
+
+    from sparlab_amd import native as N
+
+    info = (C.c_longlong * 3)()
+    g.ctx.check(N.load().slx_amg_level_info(g.ctx.h, level, which, info))
+    return list(info)
+
+
+def rooflines(g, m, timers):
+    """Roofline of each kernel timed with HIP events over the timed region, largest total first.
+
+    galerkin_ap / galerkin_raw: the level-0 block products A P and R (A P), HBM-bound;
+    algorithmic bytes = 8 (nnz A + nnz B + nnz C) values + 8 per contribution-list entry
+    + 20 per output block (item metadata) + 8 per output row (row pointers).
+    spmv: the matrix-free K_ff product of CG (with x.y partials), FP64 compute-bound;
+    1200 flop per element (Ke0 u_e: 576 FMA, the factor: 24 more).
+    """
+    out = []
+    n, nnz = g.nfree, g.kff_nnz
+    sizes = {"P": level_info(g, 0, 1)[2], "R": level_info(g, 0, 3)[2], "AP": level_info(g, 0, 4)[2],
+             "RAW": level_info(g, 0, 5), "plan_ap": level_info(g, 0, 6), "plan_raw": level_info(g, 0, 7)}
+    for name, (ms, launches) in timers.items():
+        if not launches or name == "assemble":
+            continue
+        avg = ms / launches
+        if name == "spmv":
+            flops = 1200.0 * m.nelem
+            ach = flops / (avg * 1e-3) / 1e12
+            out.append({"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                        "kernel": "k_mf_hex8<double,1> (CG q = K_ff p matrix-free, FP64 vector FMA)",
+                        "flops_per_launch": flops, "avg_ms": avg, "launches": launches, "total_ms": ms})
+            continue
+        if name == "galerkin_ap":
+            a_nnz, b_nnz, c_nnz, rows = nnz, sizes["P"], sizes["AP"], n
+            items, entries = sizes["plan_ap"][0], sizes["plan_ap"][1]
+            label = "k_block_product<0> (level-0 A P, FP64)"
+        else:
+            a_nnz, b_nnz, c_nnz, rows = sizes["R"], sizes["AP"], sizes["RAW"][2], sizes["RAW"][0]
+            items, entries = sizes["plan_raw"][0], sizes["plan_raw"][1]
+            label = "k_block_product<0> (level-0 R (A P), FP64)"
+        nbytes = 8 * (a_nnz + b_nnz + c_nnz) + 8 * entries + 20 * items + 8 * rows
+        ach = nbytes / (avg * 1e-3) / 1e9
+        out.append({"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                    "traffic": None, "kernel": label, "bytes_per_launch": nbytes, "avg_ms": avg,
+                    "launches": launches, "total_ms": ms,
+                    "sizes": {"nnz_a": a_nnz, "nnz_b": b_nnz, "nnz_c": c_nnz, "blocks": items, "entries": entries}})
+    out.sort(key=lambda r: -r["total_ms"])
+    return out
+
+
+def batch_main(args):
+    """The batched workload: a 240 x 160 two-bolt bracket's three load cases and 32
+    MBB designs of seeded sizes, one block-diagonal system, segmented CG."""
+    from sparlab_amd import batch as B
+    from sparlab_amd import native as N
+
+    tol = 1e-8
+    prec = 1 if args.precision == "mixed" else 0
+    ctx = N.Context(0)
+    t0 = time.perf_counter()
+    bt = B.assemble(B.c4_parts(), 2, ctx=ctx)
+    nseg = len(bt.seg_ptr) - 1
+    print(f"bench[batch]: {nseg} segments, {len(bt.b)} unknowns, {len(bt.val)} nonzeros, assembled in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    s = B.solver_for(bt, tol=tol, ctx=ctx, precision=prec)  # first setup: aggregation, untimed
+    for _ in range(args.warmup):
+        s.solve_segmented(bt.seg_ptr, bt.b)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.factorize(bt.ptr, bt.col, bt.val)  # values upload + numeric setup (aggregates reused)
+    ctx.synchronize()
+    t_setup = (time.perf_counter() - t0) / args.steps
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x, res = s.solve_segmented(bt.seg_ptr, bt.b)
+    ctx.synchronize()
+    t_pcg = (time.perf_counter() - t0) / args.steps
+    spmv_ms, spmv_n = ctx.timer("spmv")
+    ctx.timing(False)
+    its = [r[0] for r in res]
+    assert all(r[2] for r in res), "a segment did not converge"
+    print(f"bench[batch]: setup {1e3 * t_setup:.1f} ms, PCG {1e3 * t_pcg:.1f} ms per batch; iterations {its}",
+          file=sys.stderr, flush=True)
+    # the same segments one at a time (each block factorized alone, untimed; solves timed)
+    import scipy.sparse as sp
+
+    K = sp.csr_matrix((bt.val, bt.col, bt.ptr))
+    singles = []
+    for g in range(nseg):
+        a, b = int(bt.seg_ptr[g]), int(bt.seg_ptr[g + 1])
+        Kg = K[a:b, a:b].tocsr()
+        nodes = bt.unknowns[a:b] // 2
+        n0 = int(nodes.min())
+        one = N.HipAmgCgSolver(tolerance=tol, ctx=N.Context(0), precision=prec)  # a context per matrix
+        one.set_layout(2, bt.coords[n0:int(nodes.max()) + 1], bt.unknowns[a:b] - 2 * n0, dofs_per_node=2)
+        one.factorize(Kg.indptr, Kg.indices, Kg.data)
+        singles.append((one, bt.b[a:b]))
+    for one, bg in singles:
+        one.solve(bg)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for one, bg in singles:
+            one.solve(bg)
+    ctx.synchronize()
+    t_single = (time.perf_counter() - t0) / args.steps
+    print(f"bench[batch]: one at a time {1e3 * t_single:.1f} ms per batch", file=sys.stderr, flush=True)
+    out = {
+        "metric": "batched Q4 AMG-PCG segment solves/s @rtol 1e-8 (bracket 3 load cases + 32 MBB designs)",
+        "value": nseg / t_pcg, "unit": "segment solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_pcg, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32" if prec else "f64", "data": "synthetic",
+        "config": {"workload": f"block-diagonal batch: bracket 240x160 x 3 load cases + 32 MBB designs "
+                               f"(numpy PCG64(4242) sizes, nx 60..240), seeded densities, one segmented CG",
+                   "segments": nseg, "unknowns": int(len(bt.b)), "nonzeros": int(len(bt.val)),
+                   "pcg_ms_per_batch": 1e3 * t_pcg, "setup_ms_per_batch": 1e3 * t_setup,
+                   "with_setup_segment_solves_per_s": nseg / (t_pcg + t_setup),
+                   "one_at_a_time_ms_per_batch": 1e3 * t_single,
+                   "speedup_vs_one_at_a_time": t_single / t_pcg,
+                   "iterations": its,
+                   "spmv_avg_ms": spmv_ms / max(spmv_n, 1)},
+    }
+    print(json.dumps(out))
 
 
 def main():
@@ -131,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="headline run without the learned warm start")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--workload", choices=["simp", "batch"], default="simp",
+                    help="simp: the headline SIMP loop; batch: the batched load-case and design solves")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference reading for the HBM byte counters)")
     args = ap.parse_args()
@@ -148,6 +281,10 @@ def main():
         print(f"bench.py measures one GPU: this build runs the path on a single MI355X and has no "
               f"multi-GPU mode (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        batch_main(args)
+        return
 
     from sparlab_amd import model as M
     from sparlab_amd import native as N
@@ -176,9 +313,7 @@ def main():
     n, nnz = g.nfree, g.kff_nnz
     solves = head["steps"] + 1  # run_oc re-evaluates the final design
     value = solves / head["elapsed"]
-    spmv_bytes = 12 * nnz + 4 * (n + 1) + 16 * n
-    avg_ms = head["spmv_ms"] / max(head["spmv_n"], 1)
-    achieved = spmv_bytes / (avg_ms * 1e-3) / 1e9 if head["spmv_n"] else 0.0
+    roofs = rooflines(g, m, head["timers"])
     config = {"workload": f"cantilever Hex8 {nx}x{ny}x{nz} SIMP/OC from x0 = 0.2, {n} free DOFs, rtol {tol}; "
                           f"{'FP32 V-cycle under FP64 flexible CG' if args.precision == 'mixed' else 'FP64'}"
                           f"{' + learned warm start' if not args.no_learned else ''}",
@@ -218,10 +353,8 @@ def main():
         "dtype": "f64+f32" if args.precision == "mixed" else "f64",
         "data": "synthetic",
         "config": config,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv<16,3,double> (PCG q = K_ff p, FP64 CSR)",
-                     "bytes_per_launch": spmv_bytes, "avg_ms": avg_ms, "launches": head["spmv_n"]},
+        "roofline": roofs[0] if roofs else None,
+        "rooflines_other": roofs[1:],
     }
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)

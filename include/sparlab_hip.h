@@ -232,9 +232,22 @@ slx_status slx_solve(slx_context* ctx, const double* b, double* x, double tol, i
                      slx_pcg_result* res);
 slx_status slx_solve_case(slx_context* ctx, int load_case, int warm_start, double tol, int max_iterations,
                           slx_pcg_result* res);
+/* Batched solve (no counterpart in the reference, which solves designs and load
+ * cases one at a time): the factorized matrix holds independent systems on its
+ * diagonal blocks -- designs of different sizes, or copies of one design's K_ff
+ * for its load cases -- with rows [seg_ptr[s], seg_ptr[s+1]) in segment s
+ * (seg_ptr[0] = 0, seg_ptr[nseg] = n). Each segment runs PCG with its own step
+ * lengths, residual, convergence test and restarts (the semantics of slx_solve);
+ * every kernel launch covers all segments. A matrix that couples two segments
+ * is rejected (SLX_CONFIG_ERROR). results[nseg]. */
+slx_status slx_solve_segmented(slx_context* ctx, int nseg, const int* seg_ptr, const double* b, double* x, double tol,
+                               int max_iterations, slx_pcg_result* results);
 slx_status slx_amg_apply(slx_context* ctx, const double* r, double* z);
 slx_status slx_amg_stats_get(const slx_context* ctx, slx_amg_stats* out);
-/* Test access to the hierarchy. which: 0 A_l, 1 P, 2 P-hat, 3 R. info = rows, cols, nnz. */
+/* Test access to the hierarchy. which: 0 A_l, 1 P, 2 P-hat, 3 R, 4 A P, 5 R (A P):
+ * info = rows, cols, nnz (slx_amg_level_get: 0..3 only); 6, 7: the device plans of
+ * the products A P and R (A P): info = output blocks, contribution-list entries,
+ * lanes per block (0 when the level uses the generic product). */
 slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long long* info);
 slx_status slx_amg_level_get(const slx_context* ctx, int level, int which, int* ptr, int* col, double* val);
 slx_status slx_amg_aggregates_get(const slx_context* ctx, int level, int* aggregates);

@@ -18,6 +18,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -837,6 +838,63 @@ void check_flag(Context& c, DBuf<int>& flag, const char* what) {
   if (v != 0) fail(SLX_SOLVER_ERROR, std::string("multigrid: internal error in ") + what + " (pattern mismatch)");
 }
 
+// The CSR kernels' lanes per row, timed on each operator of the hierarchy (the
+// precision the V-cycle reads) and kept at the fastest; once per structure.
+void autotune(Context& c) {
+  const char* e = std::getenv("SPARLAB_NO_AUTOTUNE");
+  if (e && e[0] == '1') return;
+  hipStream_t s = c.stream;
+  const bool f32 = c.amg_opt.precision == 1;
+  std::vector<DCsr*> mats;
+  for (size_t l = 0; l + 1 < c.levels.size(); ++l) {
+    if (l > 0 || !c.mf) mats.push_back(&level_op(c, l));
+    mats.push_back(&c.levels[l].P);
+    mats.push_back(&c.levels[l].R);
+  }
+  size_t rows = 1, cols = 1;
+  for (DCsr* m : mats) {
+    rows = std::max(rows, (size_t)m->rows);
+    cols = std::max(cols, (size_t)m->cols);
+  }
+  DBuf<double> x, y;
+  DBuf<float> x32, y32;
+  x.alloc(cols);
+  k::fill((long long)cols, 1.0, x.p, s);
+  if (f32) {
+    x32.alloc(cols);
+    y32.alloc(rows);
+    k::to_float((long long)cols, x.p, x32.p, s);
+  } else {
+    y.alloc(rows);
+  }
+  hipEvent_t a = c.take_event(), b = c.take_event();
+  for (DCsr* m : mats) {
+    if (m->rows == 0 || (f32 && !m->val32.p)) continue;
+    float best = std::numeric_limits<float>::max();
+    int pick = 0;
+    for (int lanes : {4, 8, 16}) {
+      m->lanes = lanes;
+      auto run = [&] {
+        if (f32) k::spmv32(*m, x32.p, y32.p, s); else k::spmv(*m, x.p, y.p, s);
+      };
+      run();  // warm
+      SLX_HIP(hipEventRecord(a, s));
+      for (int r = 0; r < 4; ++r) run();
+      SLX_HIP(hipEventRecord(b, s));
+      SLX_HIP(hipEventSynchronize(b));
+      float ms = 0.0f;
+      SLX_HIP(hipEventElapsedTime(&ms, a, b));
+      if (ms < best) {
+        best = ms;
+        pick = lanes;
+      }
+    }
+    m->lanes = pick;
+  }
+  c.event_pool.push_back(a);
+  c.event_pool.push_back(b);
+}
+
 }  // namespace
 
 double device_dot(Context& c, long long n, const double* a, const double* b) {
@@ -972,8 +1030,14 @@ void amg_setup(Context& c) {
       k::block_product(g, true, s);
       k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
       lap(3);
-      k::block_product(plan_args(cur, cur.plan_ap, false, A, cur.P, cur.AP), false, s);
-      k::block_product(plan_args(cur, cur.plan_raw, true, cur.R, cur.AP, cur.RAW), false, s);
+      {
+        Timed t(c, l == 0 ? "galerkin_ap" : nullptr);
+        k::block_product(plan_args(cur, cur.plan_ap, false, A, cur.P, cur.AP), false, s);
+      }
+      {
+        Timed t(c, l == 0 ? "galerkin_raw" : nullptr);
+        k::block_product(plan_args(cur, cur.plan_raw, true, cur.R, cur.AP, cur.RAW), false, s);
+      }
     } else {
       k::smooth_prolongator(A, cur.Ptent, cur.P, cur.dinv.p, omega, flag.p, s);
       k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
@@ -1017,7 +1081,12 @@ void amg_setup(Context& c) {
     lv.d2.alloc((size_t)lv.n);
   }
   if (o.precision == 1) prepare_mixed(c);
+  if (!reuse || c.tuned_precision != o.precision) {
+    autotune(c);
+    c.tuned_precision = o.precision;
+  }
   SLX_HIP(hipStreamSynchronize(s));
+  ++c.setup_id;
   c.fp_rows = n;
   c.fp_nnz = c.K.nnz;
   c.fp_unknowns = c.layout_unknowns;

@@ -561,6 +561,27 @@ slx_status slx_solve(slx_context* ctx, const double* b, double* x, double tol, i
   });
 }
 
+slx_status slx_solve_segmented(slx_context* ctx, int nseg, const int* seg_ptr, const double* b, double* x, double tol,
+                               int max_iterations, slx_pcg_result* results) {
+  return guarded(ctx, [&](Context& c) {
+    if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "AMG-CG: solve called before factorize");
+    const int n = c.K.rows;
+    if (nseg < 1 || !seg_ptr || seg_ptr[0] != 0 || seg_ptr[nseg] != n)
+      fail(SLX_CONFIG_ERROR, "segmented solve: seg_ptr must run from 0 to the matrix size");
+    for (int g = 0; g < nseg; ++g)
+      if (seg_ptr[g + 1] <= seg_ptr[g]) fail(SLX_CONFIG_ERROR, "segmented solve: segments must be non-empty");
+    const std::vector<int> sp(seg_ptr, seg_ptr + nseg + 1);
+    c.bw.upload(b, (size_t)n, c.stream);
+    DBuf<double> xd;
+    xd.upload(x, (size_t)n, c.stream);
+    std::vector<slx_pcg_result> r;
+    seg_pcg(c, sp, c.bw.p, xd.p, tol, max_iterations > 0 ? max_iterations : 1000, r);
+    xd.download(x, (size_t)n, c.stream);
+    sync(c);
+    if (results) std::copy(r.begin(), r.end(), results);
+  });
+}
+
 slx_status slx_solve_case(slx_context* ctx, int load_case, int warm_start, double tol, int max_iterations,
                           slx_pcg_result* res) {
   return guarded(ctx, [&](Context& c) {
@@ -616,7 +637,24 @@ slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long
   return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
     if (!c.amg_ready || level < 0 || level >= (int)c.levels.size()) fail(SLX_SOLVER_ERROR, "no such multigrid level");
     Level& lv = c.levels[(size_t)level];
-    const DCsr* m = which == 0 ? (level == 0 ? &c.K : &lv.A) : which == 1 ? &lv.P : which == 2 ? &lv.Ptent : &lv.R;
+    if (which == 6 || which == 7) {  // block-product plans of A P and R (A P): items, list entries, lanes per item
+      const ProductPlan& pl = which == 6 ? lv.plan_ap : lv.plan_raw;
+      long long entries = 0;
+      if (pl.nitems)
+        SLX_HIP(hipMemcpyAsync(&entries, pl.item_begin.p + pl.nitems, sizeof(long long), hipMemcpyDeviceToHost,
+                               c.stream));
+      sync(c);
+      info[0] = pl.nitems;
+      info[1] = entries;
+      info[2] = pl.gsize;
+      return;
+    }
+    const DCsr* m = which == 0   ? (level == 0 ? &c.K : &lv.A)
+                    : which == 1 ? &lv.P
+                    : which == 2 ? &lv.Ptent
+                    : which == 3 ? &lv.R
+                    : which == 4 ? &lv.AP
+                                 : &lv.RAW;
     info[0] = m->rows;
     info[1] = m->cols;
     info[2] = m->nnz;

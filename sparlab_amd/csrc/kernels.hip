@@ -498,6 +498,7 @@ int lanes_for(const DCsr& A) {
     return e ? std::atoi(e) : 0;
   }();
   if (forced == 4 || forced == 8 || forced == 16) return forced;
+  if (A.lanes == 4 || A.lanes == 8 || A.lanes == 16) return A.lanes;
   return avg > 48 ? 16 : avg > 12 ? 8 : 4;
 }
 

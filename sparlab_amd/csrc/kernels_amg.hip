@@ -154,7 +154,11 @@ __global__ void __launch_bounds__(256) k_block_product(BlockProductArgs g) {
   const int gpw = kWave / g.gsize;  // groups per wavefront
   const int grp = lane / g.gsize, gl = lane - grp * g.gsize;
   if (grp >= gpw) return;
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // consecutive items (one node-row's blocks, then its neighbours', which share A
+  // and B rows) stay on one XCD and its L2: block b runs logical block
+  // (b % 8) * (grid / 8) + b / 8 (the grid is a multiple of 8)
+  const int blk = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  const long long wave = ((long long)blk * blockDim.x + threadIdx.x) >> 6;
   const long long item = wave * gpw + grp;
   if (item >= g.nitems) return;
   const int node = g.item_node[item];
@@ -167,18 +171,36 @@ __global__ void __launch_bounds__(256) k_block_product(BlockProductArgs g) {
   const double* arow = g.aval + g.aptr[row];
   double acc = 0.0;
   const long long e1 = g.item_begin[item + 1];
-  for (long long e = g.item_begin[item]; e < e1; ++e) {
-    const uint2 en = g.list[e];
-    const double* a = arow + (en.y >> 16);
-    const double* b = g.bval + en.x + t;
-    const int lb = (int)((en.y >> 3) & 0x1fffu), dj = (int)(en.y & 7u);
-    if (dj == 3) {
-      acc += a[0] * b[0];
-      acc += a[1] * b[lb];
-      acc += a[2] * b[2 * lb];
+  long long e = g.item_begin[item];
+  // two entries per step with all their loads issued before the (in-order) sums
+  for (; e + 1 < e1; e += 2) {
+    const uint2 p = g.list[e], q = g.list[e + 1];
+    const double* a0 = arow + (p.y >> 16);
+    const double* b0 = g.bval + p.x + t;
+    const double* a1 = arow + (q.y >> 16);
+    const double* b1 = g.bval + q.x + t;
+    const int l0 = (int)((p.y >> 3) & 0x1fffu), d0 = (int)(p.y & 7u);
+    const int l1 = (int)((q.y >> 3) & 0x1fffu), d1 = (int)(q.y & 7u);
+    if (d0 == 3 && d1 == 3) {
+      const double x0 = a0[0], x1 = a0[1], x2 = a0[2], y0 = b0[0], y1 = b0[l0], y2 = b0[2 * l0];
+      const double u0 = a1[0], u1 = a1[1], u2 = a1[2], v0 = b1[0], v1 = b1[l1], v2 = b1[2 * l1];
+      acc += x0 * y0;
+      acc += x1 * y1;
+      acc += x2 * y2;
+      acc += u0 * v0;
+      acc += u1 * v1;
+      acc += u2 * v2;
     } else {
-      for (int m = 0; m < dj; ++m) acc += a[m] * b[m * lb];
+      for (int m = 0; m < d0; ++m) acc += a0[m] * b0[m * l0];
+      for (int m = 0; m < d1; ++m) acc += a1[m] * b1[m * l1];
     }
+  }
+  if (e < e1) {
+    const uint2 p = g.list[e];
+    const double* a0 = arow + (p.y >> 16);
+    const double* b0 = g.bval + p.x + t;
+    const int l0 = (int)((p.y >> 3) & 0x1fffu), d0 = (int)(p.y & 7u);
+    for (int m = 0; m < d0; ++m) acc += a0[m] * b0[m * l0];
   }
   double v = acc;
   if (EPI == 1) {
@@ -250,7 +272,7 @@ void block_product(const BlockProductArgs& a, bool smooth, hipStream_t s) {
   if (a.nitems == 0) return;
   if (a.gsize < 1 || a.gsize > kWave) fail(SLX_SOLVER_ERROR, "multigrid: block product group wider than a wavefront");
   const long long waves = (a.nitems + kWave / a.gsize - 1) / (kWave / a.gsize);
-  const int grid = (int)((waves + 3) / 4);
+  const int grid = (int)((waves + 3) / 4 + 7) / 8 * 8;
   if (smooth) k_block_product<1><<<grid, 256, 0, s>>>(a); else k_block_product<0><<<grid, 256, 0, s>>>(a);
 }
 

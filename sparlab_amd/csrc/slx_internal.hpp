@@ -76,6 +76,7 @@ struct DCsr {
   int rows = 0, cols = 0;
   long long nnz = 0;
   int max_row = 0;  // longest row
+  int lanes = 0;    // lanes per row of the CSR kernels, set by the autotuner (0: by row length)
   DBuf<int> ptr, col;
   DBuf<double> val;
   DBuf<float> val32;
@@ -128,6 +129,7 @@ struct TimerSlot {
 };
 
 struct WarmStart;  // learned warm start (slx_net.hpp)
+struct SegState;   // segmented CG of a block-diagonal batch (seg.cpp)
 
 // Problem / solver context: one GPU stream, one structured SIMP problem.
 struct Context {
@@ -177,6 +179,7 @@ struct Context {
   DBuf<float> coarse_inv32;
   double coarse_pivot_ratio = 0.0;
   double setup_seconds = 0.0;
+  int tuned_precision = -1;  // precision the CSR kernels were last autotuned for
   double phase_seconds[6] = {0, 0, 0, 0, 0, 0};  // check, spectral, aggregation, smoothing, Galerkin, coarse
 
   // ---- PCG workspace ------------------------------------------------------------
@@ -202,6 +205,9 @@ struct Context {
 
   // ---- learned warm start -------------------------------------------------------
   std::shared_ptr<WarmStart> ws;
+  // ---- batched (segmented) CG ------------------------------------------------------
+  std::shared_ptr<SegState> seg;
+  long long setup_id = 0;  // counts completed multigrid setups
 
   // ---- measurement ------------------------------------------------------------
   bool timing = false;

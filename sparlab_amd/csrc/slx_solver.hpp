@@ -1,6 +1,8 @@
 // Solver-level functions shared by the library's translation units.
 #pragma once
 
+#include <functional>
+
 #include "slx_internal.hpp"
 
 namespace slx {
@@ -14,6 +16,11 @@ void vcycle(Context& c, const double* r, double* z);
 // Preconditioned CG on the device (src/fem/Multigrid.cpp:1152-1237); b, x device pointers.
 slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_iterations);
 
+// Independent PCG solves on the diagonal blocks [seg_ptr[s], seg_ptr[s+1]) of the
+// factorized matrix, all segments per kernel launch; b, x device pointers.
+void seg_pcg(Context& c, const std::vector<int>& seg_ptr, const double* b, double* x, double tol, int max_iterations,
+             std::vector<slx_pcg_result>& results);
+
 // K_ff products on the device: matrix-free for structured Hex8 models
 // (after slx_assemble), the assembled CSR otherwise
 void op_apply(Context& c, const double* x, double* y);
@@ -24,6 +31,10 @@ void op_apply_dot(Context& c, const double* x, double* y, double* partials);
 void op_lanczos(Context& c, const double* sq, const double* v, const double* vprev, const double* beta, double* w);
 // whether level-0 products run matrix-free (SPARLAB_NO_MATRIX_FREE=1 turns it off)
 bool matrix_free_enabled();
+
+// launches recorded into an executable graph (SPARLAB_NO_GRAPHS=1: callers launch eagerly)
+bool graphs_enabled();
+hipGraphExec_t capture_graph(Context& c, const std::function<void()>& body);
 
 // drop the captured CG graphs (after a setup or a reallocation)
 void invalidate_graphs(Context& c);

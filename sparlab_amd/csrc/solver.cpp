@@ -14,6 +14,7 @@
 // symmetric, so the outer method becomes flexible CG (Polak-Ribiere beta).
 #include <algorithm>
 #include <cstdlib>
+#include <functional>
 #include <cmath>
 #include <limits>
 #include <sstream>
@@ -350,15 +351,12 @@ void vcycle(Context& c, const double* r, double* z) {
   cycle64(c, 0, r, z);
 }
 
-namespace {
-
 bool graphs_enabled() {
   const char* e = std::getenv("SPARLAB_NO_GRAPHS");
   return !(e && e[0] == '1');
 }
 
-template <class F>
-hipGraphExec_t capture(Context& c, F&& body) {
+hipGraphExec_t capture_graph(Context& c, const std::function<void()>& body) {
   hipGraph_t graph = nullptr;
   c.capturing = true;
   SLX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
@@ -377,8 +375,6 @@ hipGraphExec_t capture(Context& c, F&& body) {
   SLX_HIP(hipGraphDestroy(graph));
   return exec;
 }
-
-}  // namespace
 
 slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int max_iterations) {
   const long long n = c.K.rows;
@@ -422,9 +418,9 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
   };
   const bool graphs = graphs_enabled();
   if (graphs && !c.g_spmv) {
-    c.g_spmv = capture(c, body_spmv);
-    c.g_update = capture(c, body_update);
-    c.g_precond = capture(c, body_precond);
+    c.g_spmv = capture_graph(c, body_spmv);
+    c.g_update = capture_graph(c, body_update);
+    c.g_precond = capture_graph(c, body_precond);
   }
   for (int restart = 0; restart < 3; ++restart) {
     op_residual(c, x, b, c.r.p);

@@ -140,6 +140,7 @@ _SIGNATURES = {
                                     C.POINTER(AmgOptions)]),
     "slx_solve": (C.c_int, [_VP, _VP, _VP, C.c_double, C.c_int, C.POINTER(PcgResult)]),
     "slx_solve_case": (C.c_int, [_VP, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(PcgResult)]),
+    "slx_solve_segmented": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, C.c_double, C.c_int, _VP]),
     "slx_amg_apply": (C.c_int, [_VP, _VP, _VP]),
     "slx_amg_stats_get": (C.c_int, [_VP, C.POINTER(AmgStats)]),
     "slx_amg_level_info": (C.c_int, [_VP, C.c_int, C.c_int, _VP]),
@@ -353,9 +354,18 @@ class HipAmgCgSolver:
         self._n = len(ptr) - 1
         self.ctx.check(load().slx_factorize_csr(self.ctx.h, self._n, _p(ptr), _p(col), _p(val), dim, ndpn,
                                                 coords.shape[0], _p(coords), _p(unknowns), C.byref(self.options)))
+        self.ctx.owner = self  # the context now holds this solver's matrix
+
+    def _require_matrix(self, n):
+        if getattr(self, "_n", None) is None or getattr(self.ctx, "owner", None) is not self:
+            raise SolverError(f"{self.name()}: factorize() this solver's matrix first (another solver sharing the "
+                              "context replaced it)")
+        if n != self._n:
+            raise SolverError(f"{self.name()}: right-hand side of length {n} for a matrix of size {self._n}")
 
     def solve_from(self, b, x0=None):
         b = _f64(b)
+        self._require_matrix(len(b))
         x = np.zeros_like(b) if x0 is None or len(x0) != len(b) else _f64(x0).copy()
         cap = self.max_iterations if self.max_iterations > 0 else 1000
         self.ctx.check(load().slx_solve(self.ctx.h, _p(b), _p(x), self.tolerance, cap, C.byref(self._last)))
@@ -367,6 +377,19 @@ class HipAmgCgSolver:
 
     def solve(self, b):
         return self.solve_from(b, None)
+
+    def solve_segmented(self, seg_ptr, b, x0=None):
+        """Independent solves on the diagonal blocks [seg_ptr[s], seg_ptr[s+1]);
+        returns x and per segment (iterations, relative residual, converged)."""
+        seg_ptr = np.ascontiguousarray(seg_ptr, dtype=np.int32)
+        b = _f64(b)
+        self._require_matrix(len(b))
+        x = np.zeros_like(b) if x0 is None else _f64(x0).copy()
+        res = (PcgResult * (len(seg_ptr) - 1))()
+        cap = self.max_iterations if self.max_iterations > 0 else 1000
+        self.ctx.check(load().slx_solve_segmented(self.ctx.h, len(seg_ptr) - 1, _p(seg_ptr), _p(b), _p(x),
+                                                  self.tolerance, cap, res))
+        return x, [(r.iterations, r.relative_residual, bool(r.converged)) for r in res]
 
     def last_iterations(self):
         return self._last.iterations
@@ -405,25 +428,35 @@ class HipProblem:
                              _p(np.ascontiguousarray(model.passive, dtype=np.uint8))))
         r = model.filter_radius if filter_radius is None else filter_radius
         chk(L.slx_filter_build(h, float(r)))
+        self.ctx.owner = self  # the context now holds this problem
+
+    def _own(self):
+        if getattr(self.ctx, "owner", None) is not self:
+            raise ModelError("this problem's context now holds another problem or matrix; give each problem its "
+                             "own Context to use them side by side")
 
     # ---- structure -----------------------------------------------------------
     def ke0(self):
+        self._own()
         nd = (8 if self.model.dim == 3 else 4) * self.model.dim
         k = np.empty(nd * nd)
         self.ctx.check(load().slx_ke0_get(self.ctx.h, _p(k)))
         return k.reshape(nd, nd)
 
     def free(self):
+        self._own()
         f = np.empty(self.nfree, dtype=np.int32)
         self.ctx.check(load().slx_free_dofs_get(self.ctx.h, _p(f)))
         return f
 
     # ---- assembler / solver seams ------------------------------------------------
     def assemble(self, scale=None):
+        self._own()
         s = None if scale is None else _f64(scale)
         self.ctx.check(load().slx_assemble(self.ctx.h, _p(s)))
 
     def kff(self):
+        self._own()
         ptr = np.empty(self.nfree + 1, dtype=np.int32)
         col = np.empty(self.kff_nnz, dtype=np.int32)
         val = np.empty(self.kff_nnz)
@@ -432,20 +465,24 @@ class HipProblem:
 
     def kff_apply(self, x):
         """y = K_ff x with the solver's operator; returns (y, matrix_free)."""
+        self._own()
         y = np.empty(self.nfree)
         mf = C.c_int()
         self.ctx.check(load().slx_kff_apply(self.ctx.h, _p(_f64(x)), _p(y), C.byref(mf)))
         return y, bool(mf.value)
 
     def amg_setup(self, **amg):
+        self._own()
         self.ctx.check(load().slx_amg_setup(self.ctx.h, C.byref(amg_options(**amg))))
 
     def amg_stats(self):
+        self._own()
         s = AmgStats()
         self.ctx.check(load().slx_amg_stats_get(self.ctx.h, C.byref(s)))
         return s
 
     def amg_matrix(self, level, which):
+        self._own()
         w = {"A": 0, "P": 1, "Ptent": 2, "R": 3}[which]
         info = np.zeros(3, dtype=np.int64)
         L = load()
@@ -458,17 +495,20 @@ class HipProblem:
         return rows, cols, ptr, col, val
 
     def amg_aggregates(self, level, num_nodes):
+        self._own()
         a = np.empty(num_nodes, dtype=np.int32)
         self.ctx.check(load().slx_amg_aggregates_get(self.ctx.h, level, _p(a)))
         return a
 
     def amg_apply(self, r):
+        self._own()
         r = _f64(r)
         z = np.empty_like(r)
         self.ctx.check(load().slx_amg_apply(self.ctx.h, _p(r), _p(z)))
         return z
 
     def solve(self, b, x0=None, tol=1e-8, maxit=1000):
+        self._own()
         b = _f64(b)
         x = np.zeros_like(b) if x0 is None else _f64(x0).copy()
         res = PcgResult()
@@ -476,32 +516,38 @@ class HipProblem:
         return x, res.iterations, res.relative_residual, bool(res.converged)
 
     def solve_case(self, case=0, warm=True, tol=1e-8, maxit=1000):
+        self._own()
         res = PcgResult()
         self.ctx.check(load().slx_solve_case(self.ctx.h, case, 1 if warm else 0, tol, maxit, C.byref(res)))
         return res.iterations, res.relative_residual, bool(res.converged)
 
     def solution(self, case=0):
+        self._own()
         u = np.empty(self.nfree)
         self.ctx.check(load().slx_solution_get(self.ctx.h, case, _p(u)))
         return u
 
     # ---- topology optimisation ---------------------------------------------------
     def filter(self, x, adjoint=False):
+        self._own()
         y = np.empty(self.ne)
         self.ctx.check(load().slx_filter_apply(self.ctx.h, 1 if adjoint else 0, _p(_f64(x)), _p(y)))
         return y
 
     def warm_start(self, enable=True, **kw):
         """Turn the learned warm start on (or off) and reset its network."""
+        self._own()
         o = warmstart_options(enable=1 if enable else 0, **kw)
         self.ctx.check(load().slx_warmstart_configure(self.ctx.h, C.byref(o)))
 
     def warm_start_stats(self):
+        self._own()
         st = WarmStartStats()
         self.ctx.check(load().slx_warmstart_stats_get(self.ctx.h, C.byref(st)))
         return st.as_dict()
 
     def evaluate(self, x, tol=1e-8, need_gradients=True, warm_start=True, penalty=None, max_iterations=1000):
+        self._own()
         o = eval_options(tolerance=tol, warm_start=1 if warm_start else 0, max_iterations=max_iterations,
                          penalty=self.model.penalty if penalty is None else penalty,
                          emin_ratio=self.model.emin_ratio)
@@ -516,6 +562,7 @@ class HipProblem:
                     assemble_seconds=ev.assemble_seconds, solve_seconds=ev.solve_seconds)
 
     def oc(self, x, dc, dv, move=0.2, damping=0.5):
+        self._own()
         xn = np.empty(self.ne)
         st = OcStep()
         self.ctx.check(load().slx_oc_update(self.ctx.h, _p(_f64(x)), _p(_f64(dc)), _p(_f64(dv)),
@@ -526,6 +573,7 @@ class HipProblem:
 
     def run_oc(self, x0, max_iterations, change_tol=1e-3, objective_tol=0.0, window=5, cont_steps=1,
                penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5, tol=1e-8):
+        self._own()
         o = RunOptions()
         o.max_iterations = max_iterations
         o.change_tolerance = change_tol
