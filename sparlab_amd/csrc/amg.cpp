@@ -615,7 +615,8 @@ bool run_table(const HCsr& m, const std::vector<int>& node_ptr, const std::vecto
 bool build_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& cbp, const std::vector<int>& cba,
                 const std::vector<int>& cbq, const std::vector<int>& run_ptr, const std::vector<int>& runs,
                 const std::vector<int>& bbp, const std::vector<int>& bba, const std::vector<int>& bbq,
-                const std::vector<int>& bptr, const std::vector<int>& agg_off, ProductPlan& plan, hipStream_t s) {
+                const std::vector<int>& bptr, const std::vector<int>& agg_off, const std::vector<int>* order,
+                ProductPlan& plan, hipStream_t s) {
   const int nn = (int)cnode_ptr.size() - 1;
   const int naggs = (int)agg_off.size() - 1;
   const long long nitems = cbp[(size_t)nn];
@@ -666,9 +667,35 @@ bool build_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& cbp, 
       wmax = std::max(wmax, agg_off[(size_t)g + 1] - agg_off[(size_t)g]);
     }
   }
-  if (dmax * wmax > 64) return false;
+  if (dmax > 6 || wmax > 64) return false;
+  if (order) {
+    // emit the items node by node in the given order (the lists move with them)
+    std::vector<int> n2((size_t)nitems), a2((size_t)nitems), p2((size_t)nitems);
+    std::vector<long long> b2((size_t)nitems + 1, 0);
+    std::vector<uint2> l2(list.size());
+    long long q2 = 0, e2 = 0;
+    for (int I : *order)
+      for (int q = cbp[(size_t)I]; q < cbp[(size_t)I + 1]; ++q, ++q2) {
+        n2[(size_t)q2] = I;
+        a2[(size_t)q2] = cba[(size_t)q];
+        p2[(size_t)q2] = cbq[(size_t)q];
+        for (long long e = begin[(size_t)q]; e < begin[(size_t)q + 1]; ++e) l2[(size_t)e2++] = list[(size_t)e];
+        b2[(size_t)q2 + 1] = e2;
+      }
+    if (q2 != nitems) return false;
+    plan.nitems = nitems;
+    plan.dmax = dmax;
+    plan.wmax = wmax;
+    plan.item_node.upload(n2, s);
+    plan.item_agg.upload(a2, s);
+    plan.item_pos.upload(p2, s);
+    plan.item_begin.upload(b2, s);
+    plan.list.upload(l2, s);
+    return true;
+  }
   plan.nitems = nitems;
-  plan.gsize = dmax * wmax;
+  plan.dmax = dmax;
+  plan.wmax = wmax;
   plan.item_node.upload(node, s);
   plan.item_agg.upload(cba, s);
   plan.item_pos.upload(cbq, s);
@@ -677,8 +704,44 @@ bool build_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& cbp, 
   return true;
 }
 
+// Nodes sorted along a Morton (Z-order) curve of their coordinates (n x 3):
+// products visit nodes that share neighbours close together in time.
+std::vector<int> morton_order(const std::vector<double>& xyz) {
+  const size_t n = xyz.size() / 3;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (size_t i = 0; i < n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], xyz[3 * i + (size_t)d]);
+      hi[d] = std::max(hi[d], xyz[3 * i + (size_t)d]);
+    }
+  auto spread = [](unsigned long long v) {  // 21 bits -> every third of 63
+    v &= 0x1fffffULL;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+  };
+  std::vector<std::pair<unsigned long long, int>> key(n);
+  for (size_t i = 0; i < n; ++i) {
+    unsigned long long code = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double span = hi[d] - lo[d];
+      const double t = span > 0.0 ? (xyz[3 * i + (size_t)d] - lo[d]) / span : 0.0;
+      code |= spread((unsigned long long)(t * 2097151.0)) << d;
+    }
+    key[i] = {code, (int)i};
+  }
+  std::stable_sort(key.begin(), key.end());
+  std::vector<int> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = key[i].second;
+  return order;
+}
+
 void prepare_node_products(Level& lv, const Level& next, const HCsr& a, const HCsr& pt, const HCsr& ppat,
-                           const HCsr& rpat, const HCsr& appat, const HCsr& rawpat, hipStream_t s) {
+                           const HCsr& rpat, const HCsr& appat, const HCsr& rawpat, const std::vector<double>* xyz,
+                           hipStream_t s) {
   lv.node_ok = false;
   const std::vector<int> f_of = node_of_rows(lv.node_ptr);
   const std::vector<int> c_of = node_of_rows(next.node_ptr);
@@ -697,9 +760,26 @@ void prepare_node_products(Level& lv, const Level& next, const HCsr& a, const HC
     return;
   for (size_t nd = 0; nd + 1 < tp.size(); ++nd)
     if (tp[nd + 1] - tp[nd] != 1) return;  // P-hat: one block per node
-  if (!build_plan(lv.node_ptr, pp, pa, pq, arp, ar, tp, ta, tq, pt.ptr, off, lv.plan_p, s) ||
-      !build_plan(lv.node_ptr, qp, qa, qq, arp, ar, pp, pa, pq, ppat.ptr, off, lv.plan_ap, s) ||
-      !build_plan(next.node_ptr, wp, wa, wq, rrp, rr, qp, qa, qq, appat.ptr, off, lv.plan_raw, s))
+  // with node coordinates, fine nodes and aggregates (at their first node) in Morton order
+  std::vector<int> f_order, c_order;
+  if (xyz) {
+    f_order = morton_order(*xyz);
+    const int nagg = (int)next.node_ptr.size() - 1;
+    std::vector<double> axyz((size_t)nagg * 3, 0.0);
+    std::vector<char> seen((size_t)nagg, 0);
+    for (int nd = 0; nd + 1 < (int)lv.node_ptr.size(); ++nd) {
+      const int g = lv.aggregate[(size_t)nd];
+      if (g < 0 || seen[(size_t)g]) continue;
+      seen[(size_t)g] = 1;
+      for (int d = 0; d < 3; ++d) axyz[(size_t)g * 3 + d] = (*xyz)[(size_t)nd * 3 + d];
+    }
+    c_order = morton_order(axyz);
+  }
+  const std::vector<int>* fo = xyz ? &f_order : nullptr;
+  const std::vector<int>* co = xyz ? &c_order : nullptr;
+  if (!build_plan(lv.node_ptr, pp, pa, pq, arp, ar, tp, ta, tq, pt.ptr, off, fo, lv.plan_p, s) ||
+      !build_plan(lv.node_ptr, qp, qa, qq, arp, ar, pp, pa, pq, ppat.ptr, off, fo, lv.plan_ap, s) ||
+      !build_plan(next.node_ptr, wp, wa, wq, rrp, rr, qp, qa, qq, appat.ptr, off, co, lv.plan_raw, s))
     return;
   std::vector<int> row_agg((size_t)lv.n);
   for (int i = 0; i < lv.n; ++i) row_agg[(size_t)i] = c_of[(size_t)pt.col[(size_t)pt.ptr[(size_t)i]]];
@@ -713,7 +793,8 @@ k::BlockProductArgs plan_args(const Level& lv, const ProductPlan& plan, bool a_i
                               DCsr& C) {
   k::BlockProductArgs g{};
   g.nitems = plan.nitems;
-  g.gsize = plan.gsize;
+  g.dmax = plan.dmax;
+  g.wmax = plan.wmax;
   g.item_node = plan.item_node.p;
   g.item_agg = plan.item_agg.p;
   g.item_pos = plan.item_pos.p;
@@ -1009,7 +1090,17 @@ void amg_setup(Context& c) {
       lv.sym_a.upload(ia, s);
       lv.sym_b.upload(ib, s);
       next.A.upload_pattern(spat, s);
-      prepare_node_products(lv, next, host_a, pt, ppat, rpat, appat, rawpat, s);
+      std::vector<double> xyz;  // level-0 node coordinates (mesh node of each node block's first unknown)
+      if (l == 0 && !c.layout_coords.empty() && c.layout_ndpn > 0) {
+        const int nn = (int)lv.node_ptr.size() - 1;
+        xyz.assign((size_t)nn * 3, 0.0);
+        for (int nd = 0; nd < nn; ++nd) {
+          const int mesh = c.layout_unknowns[(size_t)lv.node_ptr[(size_t)nd]] / c.layout_ndpn;
+          for (int d = 0; d < c.layout_dim; ++d)
+            xyz[(size_t)nd * 3 + d] = c.layout_coords[(size_t)mesh * c.layout_dim + d];
+        }
+      }
+      prepare_node_products(lv, next, host_a, pt, ppat, rpat, appat, rawpat, xyz.empty() ? nullptr : &xyz, s);
       lap(2);
       lv.has_next = true;
       SLX_HIP(hipStreamSynchronize(s));  // host pattern arrays go out of scope

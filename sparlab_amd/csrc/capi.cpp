@@ -193,7 +193,23 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
   double dv = 0.0;
   for (double v : c.volumes) dv += v;
   const double target = c.volume_fraction * dv;
+  // designs within [0, 1] filter into [0, 1] (rows of H are convex weights), so the
+  // clamp of the filtered volume is inactive and V(x) = (H^T vol).x: one filter
+  // product per update instead of one per bisection trial
+  bool bounded = true;
+  for (size_t e = 0; e < c.lower.size() && bounded; ++e) bounded = c.lower[e] >= 0.0 && c.upper[e] <= 1.0;
+  if (bounded) {
+    c.d_ocw.alloc((size_t)ne);
+    c.oc_state.alloc(8);
+    k::spmv_serial(c.dHt, c.d_vol.p, c.d_ocw.p, s);
+  }
   auto volume_at = [&](double lambda) {
+    if (bounded) {
+      k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit,
+                  o.damping, c.d_ocw.p, c.oc_state.p, 1, lambda, target, o.volume_tolerance, o.max_bisections,
+                  c.d_tmp2.p, c.partials.p, s);
+      return read_scalar(c, c.oc_state.p + 2);
+    }
     k::oc_candidate(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, lambda,
                     o.move_limit, o.damping, c.d_tmp2.p, s);
     k::filter_volume(c.dH, c.d_tmp2.p, c.d_vol.p, c.partials.p, c.scal.p + 14, s);
@@ -219,6 +235,35 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
     fail(SLX_CONVERGENCE_ERROR, os.str());
   }
   slx_oc_step step{};
+  if (bounded) {
+    // the bisection on the device, checked from the host every few trials
+    const double init[8] = {lo, hi, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    c.oc_state.upload(init, 8, s);
+    k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit, o.damping,
+                c.d_ocw.p, c.oc_state.p, 1, lo, target, o.volume_tolerance, o.max_bisections, c.d_tmp2.p,
+                c.partials.p, s);
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (o.max_bisections > 0) {
+      do {
+        for (int k = 0; k < 8; ++k)
+          k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit,
+                      o.damping, c.d_ocw.p, c.oc_state.p, 0, 0.0, target, o.volume_tolerance, o.max_bisections,
+                      c.d_tmp2.p, c.partials.p, s);
+        c.oc_state.download(st, 8, s);
+        sync(c);
+      } while (st[5] == 0.0);
+    } else {
+      c.oc_state.download(st, 8, s);
+      sync(c);
+    }
+    step.achieved_volume = st[2];
+    step.lambda = st[3];
+    step.bisections = (int)st[4];
+    step.volume_converged = st[6] != 0.0;
+    k::max_abs_diff(ne, c.d_tmp2.p, c.d_x.p, c.partials.p, c.scal.p + 13, s);
+    step.max_change = read_scalar(c, c.scal.p + 13);
+    return step;
+  }
   double volume = volume_at(lo);
   for (int it = 0; it < o.max_bisections; ++it) {
     const double mid = std::sqrt(lo * hi);
@@ -646,7 +691,7 @@ slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long
       sync(c);
       info[0] = pl.nitems;
       info[1] = entries;
-      info[2] = pl.gsize;
+      info[2] = pl.wmax;
       return;
     }
     const DCsr* m = which == 0   ? (level == 0 ? &c.K : &lv.A)

@@ -437,6 +437,75 @@ __global__ void k_oc_candidate(int n, const double* __restrict__ x, const double
   xn[e] = fmin(fmax(target, lo), hi);
 }
 
+// One trial of the OC bisection on the device. lambda = sqrt(lo hi) from the
+// state (or lam when explicit); writes the candidate and the per-block partial
+// sums of w.x_new, w = H^T vol (the filtered volume of a design within [0, 1]).
+// A finished bisection (state[5] != 0) makes further trials no-ops.
+// state: lo, hi, volume, lambda, trials, done, converged
+__global__ void __launch_bounds__(kBlock) k_oc_trial(int n, const double* __restrict__ x, const double* __restrict__ dc,
+                                                      const double* __restrict__ dv, const double* __restrict__ lower,
+                                                      const double* __restrict__ upper,
+                                                      const unsigned char* __restrict__ passive, double move,
+                                                      double damping, const double* __restrict__ w,
+                                                      const double* __restrict__ state, int explicit_lambda, double lam,
+                                                      double* __restrict__ xn, double* __restrict__ partials) {
+  if (!explicit_lambda && state[5] != 0.0) return;
+  __shared__ double sh[kBlock / 64];
+  const double lambda = explicit_lambda ? lam : sqrt(state[0] * state[1]);
+  double acc = 0.0;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    double v;
+    if (passive[e]) {
+      v = lower[e];
+    } else {
+      const double lo = fmax(lower[e], x[e] - move);
+      const double hi = fmin(upper[e], x[e] + move);
+      const double denom = lambda * dv[e];
+      double b = 0.0;
+      if (denom > 0.0) b = fmax(0.0, -dc[e] / denom);
+      v = fmin(fmax(x[e] * pow(b, damping), lo), hi);
+    }
+    xn[e] = v;
+    acc += w[e] * v;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// Sums a trial's partials into its volume and applies the bisection rule of
+// optimality_criteria_update (src/topopt/OptimalityCriteria.cpp:96-139).
+__global__ void __launch_bounds__(1024) k_oc_step(int m, const double* __restrict__ partials, double* __restrict__ state,
+                                                  int explicit_lambda, double target, double vol_tol,
+                                                  int max_bisections) {
+  if (!explicit_lambda && state[5] != 0.0) return;
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) s += partials[i];
+  const double V = block_sum(s, sh);
+  if (threadIdx.x != 0) return;
+  state[2] = V;
+  if (explicit_lambda) return;
+  double lo = state[0], hi = state[1];
+  const double mid = sqrt(lo * hi);
+  const double it = state[4] + 1.0;
+  state[4] = it;
+  state[3] = mid;
+  if (fabs(V - target) <= vol_tol * target) {
+    state[5] = 1.0;
+    state[6] = 1.0;
+    return;
+  }
+  if (V > target) lo = mid; else hi = mid;
+  state[0] = lo;
+  state[1] = hi;
+  if (hi / lo - 1.0 < 1.0e-15) {
+    state[5] = 1.0;
+    state[6] = fabs(V - target) <= 1.0e-6 * fmax(target, 1.0e-30) ? 1.0 : 0.0;
+    return;
+  }
+  if (it >= (double)max_bisections) state[5] = 1.0;
+}
+
 // partial sums of vol_e * clamp((H x)_e, 0, 1); one thread per row, column order
 __global__ void __launch_bounds__(kBlock) k_filter_volume(int n, const int* __restrict__ ptr,
                                                            const int* __restrict__ col, const double* __restrict__ val,
@@ -658,6 +727,17 @@ void oc_candidate(int n, const double* x, const double* dc, const double* dv, co
                   hipStream_t s) {
   if (n) k_oc_candidate<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, x, dc, dv, lower, upper, passive, lambda, move,
                                                                    damping, xn);
+}
+int oc_trial_blocks(int n) { return reduce_blocks(n); }
+void oc_trial(int n, const double* x, const double* dc, const double* dv, const double* lower, const double* upper,
+              const unsigned char* passive, double move, double damping, const double* w, double* state,
+              int explicit_lambda, double lam, double target, double vol_tol, int max_bisections, double* xn,
+              double* partials, hipStream_t s) {
+  if (!n) return;
+  const int nb = reduce_blocks(n);
+  k_oc_trial<<<nb, kBlock, 0, s>>>(n, x, dc, dv, lower, upper, passive, move, damping, w, state, explicit_lambda, lam,
+                                   xn, partials);
+  k_oc_step<<<1, 1024, 0, s>>>(nb, partials, state, explicit_lambda, target, vol_tol, max_bisections);
 }
 void filter_volume(const DCsr& H, const double* x, const double* vol, double* partials, double* out, hipStream_t s) {
   const int sr = staged_rows(H);

@@ -143,16 +143,17 @@ __global__ void k_inv_diag(int n, const int* __restrict__ ptr, const int* __rest
 // run is one B-node's stretch of A's row; the entry carries where it starts in
 // A's row, how many B rows it spans, the offset of the item's aggregate block
 // in the first of those B rows and the B row length (node-uniform rows), so a
-// lane needs no further index lookups. One group of rows x width lanes computes
-// an item: lane (r, t) sums A(row r, run) . B(run rows, block column t) over
-// the list, in the order of A's row, the order of the CPU restatement.
+// lane needs no further index lookups. One lane per block column t computes an
+// item's column for all of the node's rows r: it sums A(row r, run) . B(run
+// rows, t) over the list, in the order of A's row, the order of the CPU
+// restatement (A's values are the same for the item's lanes: broadcast loads).
 // EPI 1 forms P = (I - omega D^-1 A) P-hat with B = P-hat (Multigrid.cpp:686-701).
 // ---------------------------------------------------------------------------
-template <int EPI>
+template <int EPI, int DMAX>
 __global__ void __launch_bounds__(256) k_block_product(BlockProductArgs g) {
   const int lane = threadIdx.x & 63;
-  const int gpw = kWave / g.gsize;  // groups per wavefront
-  const int grp = lane / g.gsize, gl = lane - grp * g.gsize;
+  const int gpw = kWave / g.wmax;  // items per wavefront: one lane per block column
+  const int grp = lane / g.wmax, t = lane - grp * g.wmax;
   if (grp >= gpw) return;
   // consecutive items (one node-row's blocks, then its neighbours', which share A
   // and B rows) stay on one XCD and its L2: block b runs logical block
@@ -164,50 +165,51 @@ __global__ void __launch_bounds__(256) k_block_product(BlockProductArgs g) {
   const int node = g.item_node[item];
   const int agg = g.item_agg[item];
   const int w = g.agg_off[agg + 1] - g.agg_off[agg];
+  if (t >= w) return;
   const int r0 = g.out_node_ptr[node];
-  const int r = gl / w, t = gl - r * w;
-  if (r >= g.out_node_ptr[node + 1] - r0) return;
-  const int row = r0 + r;
-  const double* arow = g.aval + g.aptr[row];
-  double acc = 0.0;
+  const int d = min(g.out_node_ptr[node + 1] - r0, DMAX);
+  const double* arow[DMAX];
+  double acc[DMAX];
+#pragma unroll
+  for (int r = 0; r < DMAX; ++r) {
+    arow[r] = g.aval + g.aptr[r0 + (r < d ? r : 0)];
+    acc[r] = 0.0;
+  }
   const long long e1 = g.item_begin[item + 1];
-  long long e = g.item_begin[item];
-  // two entries per step with all their loads issued before the (in-order) sums
-  for (; e + 1 < e1; e += 2) {
-    const uint2 p = g.list[e], q = g.list[e + 1];
-    const double* a0 = arow + (p.y >> 16);
-    const double* b0 = g.bval + p.x + t;
-    const double* a1 = arow + (q.y >> 16);
-    const double* b1 = g.bval + q.x + t;
-    const int l0 = (int)((p.y >> 3) & 0x1fffu), d0 = (int)(p.y & 7u);
-    const int l1 = (int)((q.y >> 3) & 0x1fffu), d1 = (int)(q.y & 7u);
-    if (d0 == 3 && d1 == 3) {
-      const double x0 = a0[0], x1 = a0[1], x2 = a0[2], y0 = b0[0], y1 = b0[l0], y2 = b0[2 * l0];
-      const double u0 = a1[0], u1 = a1[1], u2 = a1[2], v0 = b1[0], v1 = b1[l1], v2 = b1[2 * l1];
-      acc += x0 * y0;
-      acc += x1 * y1;
-      acc += x2 * y2;
-      acc += u0 * v0;
-      acc += u1 * v1;
-      acc += u2 * v2;
+  for (long long e = g.item_begin[item]; e < e1; ++e) {
+    const uint2 en = g.list[e];
+    const int koff = (int)(en.y >> 16), lb = (int)((en.y >> 3) & 0x1fffu), dj = (int)(en.y & 7u);
+    const double* b = g.bval + en.x + t;
+    if (dj == 3) {
+      const double b0 = b[0], b1 = b[lb], b2 = b[2 * lb];
+#pragma unroll
+      for (int r = 0; r < DMAX; ++r) {
+        if (r >= d) break;
+        const double* a = arow[r] + koff;
+        acc[r] += a[0] * b0;
+        acc[r] += a[1] * b1;
+        acc[r] += a[2] * b2;
+      }
     } else {
-      for (int m = 0; m < d0; ++m) acc += a0[m] * b0[m * l0];
-      for (int m = 0; m < d1; ++m) acc += a1[m] * b1[m * l1];
+      for (int m = 0; m < dj; ++m) {
+        const double bm = b[m * lb];
+#pragma unroll
+        for (int r = 0; r < DMAX; ++r)
+          if (r < d) acc[r] += arow[r][koff + m] * bm;
+      }
     }
   }
-  if (e < e1) {
-    const uint2 p = g.list[e];
-    const double* a0 = arow + (p.y >> 16);
-    const double* b0 = g.bval + p.x + t;
-    const int l0 = (int)((p.y >> 3) & 0x1fffu), d0 = (int)(p.y & 7u);
-    for (int m = 0; m < d0; ++m) acc += a0[m] * b0[m * l0];
+#pragma unroll
+  for (int r = 0; r < DMAX; ++r) {
+    if (r >= d) break;
+    const int row = r0 + r;
+    double v = acc[r];
+    if (EPI == 1) {
+      v *= -(g.omega * g.dinv[row]);
+      if (agg == g.row_agg[row]) v += g.tval[g.tptr[row] + t];
+    }
+    g.cval[g.cptr[row] + g.item_pos[item] + t] = v;
   }
-  double v = acc;
-  if (EPI == 1) {
-    v *= -(g.omega * g.dinv[row]);
-    if (agg == g.row_agg[row]) v += g.tval[g.tptr[row] + t];
-  }
-  g.cval[g.cptr[row] + g.item_pos[item] + t] = v;
 }
 
 // Lanczos on D^-1/2 A D^-1/2 (Multigrid.cpp:304-317). The step's scalars stay
@@ -270,10 +272,15 @@ inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
 void block_product(const BlockProductArgs& a, bool smooth, hipStream_t s) {
   if (a.nitems == 0) return;
-  if (a.gsize < 1 || a.gsize > kWave) fail(SLX_SOLVER_ERROR, "multigrid: block product group wider than a wavefront");
-  const long long waves = (a.nitems + kWave / a.gsize - 1) / (kWave / a.gsize);
+  if (a.wmax < 1 || a.wmax > kWave || a.dmax < 1 || a.dmax > 6)
+    fail(SLX_SOLVER_ERROR, "multigrid: block product shape outside the device kernel's range");
+  const long long waves = (a.nitems + kWave / a.wmax - 1) / (kWave / a.wmax);
   const int grid = (int)((waves + 3) / 4 + 7) / 8 * 8;
-  if (smooth) k_block_product<1><<<grid, 256, 0, s>>>(a); else k_block_product<0><<<grid, 256, 0, s>>>(a);
+  if (a.dmax <= 3) {
+    if (smooth) k_block_product<1, 3><<<grid, 256, 0, s>>>(a); else k_block_product<0, 3><<<grid, 256, 0, s>>>(a);
+  } else {
+    if (smooth) k_block_product<1, 6><<<grid, 256, 0, s>>>(a); else k_block_product<0, 6><<<grid, 256, 0, s>>>(a);
+  }
 }
 
 void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s) {

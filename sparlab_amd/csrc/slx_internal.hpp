@@ -91,7 +91,7 @@ struct DCsr {
 // per aggregate block of each C node-row, each with its (A run, B block) list.
 struct ProductPlan {
   long long nitems = 0;
-  int gsize = 0;  // lanes per item
+  int dmax = 0, wmax = 0;  // most rows per C node, widest block (lanes per item)
   DBuf<int> item_node, item_agg, item_pos;
   DBuf<long long> item_begin;
   DBuf<uint2> list;  // two words per entry (kernels_amg.hip)
@@ -201,6 +201,7 @@ struct Context {
   HCsr H, Ht;  // filter and its transpose
   DCsr dH, dHt;
   DBuf<double> d_x, d_phys, d_fac, d_dfac, d_dc, d_energy, d_vol, d_tmp, d_tmp2, d_lower, d_upper, d_dcx, d_dvx;
+  DBuf<double> d_ocw, oc_state;  // H^T vol and the device bisection state
   DBuf<unsigned char> d_passive;
 
   // ---- learned warm start -------------------------------------------------------

@@ -58,6 +58,13 @@ void simp(int n, const double* filtered, double penalty, double emin, double* ph
 void oc_candidate(int n, const double* x, const double* dc, const double* dv, const double* lower, const double* upper,
                   const unsigned char* passive, double lambda, double move, double damping, double* xn,
                   hipStream_t s);
+// one OC bisection trial on the device (state: lo, hi, volume, lambda, trials, done, converged);
+// explicit_lambda: evaluate lam and only record the volume
+void oc_trial(int n, const double* x, const double* dc, const double* dv, const double* lower, const double* upper,
+              const unsigned char* passive, double move, double damping, const double* w, double* state,
+              int explicit_lambda, double lam, double target, double vol_tol, int max_bisections, double* xn,
+              double* partials, hipStream_t s);
+int oc_trial_blocks(int n);
 int filter_volume_blocks(const DCsr& H);
 void filter_volume(const DCsr& H, const double* x, const double* vol, double* partials, double* out, hipStream_t s);
 void max_abs_diff(int n, const double* a, const double* b, double* partials, double* out, hipStream_t s);
@@ -66,7 +73,8 @@ void clamp(int n, double* x, const double* lo, const double* hi, hipStream_t s);
 // SA-AMG numeric setup (kernels_amg.hip)
 struct BlockProductArgs {
   long long nitems;           // C blocks (one per aggregate block of each C node-row)
-  int gsize;                  // lanes per item: max rows per node x max block width
+  int dmax;                   // most rows in a C node (at most 6)
+  int wmax;                   // widest aggregate block: lanes per item
   const int* item_node;       // C node of the item
   const int* item_agg;        // its aggregate
   const int* item_pos;        // position of the block within each row of the node
