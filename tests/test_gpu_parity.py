@@ -158,6 +158,38 @@ def test_short_oc_run_against_oracle(ctx):
     assert fg == pytest.approx(fo, rel=1e-7)
 
 
+def test_c1_mbb_120x40_hundred_oc_iterations(ctx):
+    """The MBB half-beam at 120 x 40 (r 1.5, vf 0.5, p 3, rtol 1e-8), exactly 100
+    OC iterations with no stopping rule: GPU loop against the oracle's loop."""
+    print("test_c1_mbb_120x40_hundred_oc_iterations", flush=True)
+    m = M.mbb_beam(120, 40)
+    g, o = _pair(m, ctx, tol=1e-8)
+    x0 = m.initial_design()
+    xg, hg, ng, fg = g.run_oc(x0, 100, change_tol=0.0, tol=1e-8)
+    xo, ho, no, fo = o.run_oc(x0, 100, change_tol=0.0)
+    assert ng == no == 100
+    np.testing.assert_allclose(hg[:, 2], ho[:, 2], rtol=1e-6)
+    np.testing.assert_allclose(hg[:, 3], ho[:, 3], rtol=1e-8)
+    assert fg == pytest.approx(fo, rel=1e-6)
+    assert np.abs(xg - xo).max() < 1e-4
+
+
+def test_c2_hex8_64x32x32_seeded_fp64_against_oracle(ctx):
+    """Hex8 64 x 32 x 32 cantilever, densities seeded in [0.001, 1], SIMP p 3:
+    FP64 assembly + AMG-PCG at rtol 1e-12 against the oracle's solve."""
+    print("test_c2_hex8_64x32x32_seeded_fp64_against_oracle", flush=True)
+    m = M.cantilever_3d(64, 32, 32, volume_fraction=0.3, radius_cells=2.0)
+    g, o = _pair(m, ctx, tol=1e-12)
+    x = M.seeded_density(m.nelem)
+    eg = g.evaluate(x, tol=1e-12)
+    eo = o.evaluate(x, want_u=True)
+    assert eg["compliance"] == pytest.approx(eo["compliance"], rel=1e-9)
+    ug = g.solution(0)
+    uo = eo["u"][0][g.free()]
+    assert np.linalg.norm(ug - uo) <= 1e-9 * np.linalg.norm(uo)
+    np.testing.assert_allclose(eg["dc_dx"], eo["dc_dx"], rtol=0, atol=1e-8 * np.abs(eo["dc_dx"]).max())
+
+
 def test_mixed_precision_solve_meets_tolerance(ctx):
     print("test_mixed_precision_solve_meets_tolerance", flush=True)
     m = M.cantilever_3d(24, 12, 12)
