@@ -262,6 +262,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="headline run without the learned warm start")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the bfloat16 coarse-operator variant run")
     ap.add_argument("--workload", choices=["simp", "batch"], default="simp",
                     help="simp: the headline SIMP loop; batch: the batched load-case and design solves")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -296,6 +297,9 @@ def main():
     head = simp_run(N, m, x0, args.warmup, args.steps, 1 if args.precision == "mixed" else 0,
                     not args.no_learned, tol, "headline")
     base = simp_run(N, m, x0, args.warmup, args.steps, 0, False, tol, "fp64") if not args.no_fp64 else None
+    # the bfloat16 coarse-operator storage variant of the headline run
+    b16 = (simp_run(N, m, x0, args.warmup, args.steps, 2, not args.no_learned, tol, "bf16-coarse")
+           if not args.no_bf16 and args.precision == "mixed" else None)
 
     # agreement with the FP64 path at the headline run's final design (per field)
     agree = None
@@ -330,6 +334,12 @@ def main():
                                 "knorm_error_ratio": w["energy_ratio_sum"] / k,
                                 "knorm_error_ratio_net_alone": w["net_ratio_sum"] / k,
                                 "knorm_error_ratio_previous_change_alone": w["secant_ratio_sum"] / k}
+    if b16 is not None:
+        config["bf16_coarse_variant"] = {"solves_per_s": (b16["steps"] + 1) / b16["elapsed"],
+                                         "pcg_iterations_per_solve": float(np.mean(b16["iterations"])),
+                                         "compliance_rel_to_headline": abs(b16["compliance"] - head["compliance"])
+                                         / abs(head["compliance"])}
+        b16["problem"].ctx.close()
     if base is not None:
         bsolves = base["steps"] + 1
         config["fp64_gpu"] = {"solves_per_s": bsolves / base["elapsed"],

@@ -61,7 +61,8 @@ typedef enum {
 
 /* Multigrid options (include/sparlab/fem/Multigrid.hpp:75-103), plus the
  * precision of the hierarchy: 0 = FP64 everywhere (the drop-in path),
- * 1 = FP32 hierarchy and V-cycle under an FP64 outer CG. */
+ * 1 = FP32 hierarchy and V-cycle under an FP64 outer (flexible) CG,
+ * 2 = as 1 with the coarse-level operators stored in bfloat16 (FP64 row sums). */
 typedef struct {
   double strength_threshold;     /* 0.02 */
   int max_levels;                /* 12 */

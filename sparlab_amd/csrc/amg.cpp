@@ -925,7 +925,7 @@ void autotune(Context& c) {
   const char* e = std::getenv("SPARLAB_NO_AUTOTUNE");
   if (e && e[0] == '1') return;
   hipStream_t s = c.stream;
-  const bool f32 = c.amg_opt.precision == 1;
+  const bool f32 = c.amg_opt.precision >= 1;
   std::vector<DCsr*> mats;
   for (size_t l = 0; l + 1 < c.levels.size(); ++l) {
     if (l > 0 || !c.mf) mats.push_back(&level_op(c, l));
@@ -980,10 +980,10 @@ void autotune(Context& c) {
 
 double device_dot(Context& c, long long n, const double* a, const double* b) {
   k::dot(n, a, b, c.partials.p, c.scal.p + 15, c.stream);
-  double v = 0.0;
-  SLX_HIP(hipMemcpyAsync(&v, c.scal.p + 15, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  double* hb = c.host_buffer(16);
+  SLX_HIP(hipMemcpyAsync(hb + 15, c.scal.p + 15, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   SLX_HIP(hipStreamSynchronize(c.stream));
-  return v;
+  return hb[15];
 }
 
 void amg_setup(Context& c) {
@@ -1171,7 +1171,7 @@ void amg_setup(Context& c) {
     lv.d.alloc((size_t)lv.n);
     lv.d2.alloc((size_t)lv.n);
   }
-  if (o.precision == 1) prepare_mixed(c);
+  if (o.precision >= 1) prepare_mixed(c);
   if (!reuse || c.tuned_precision != o.precision) {
     autotune(c);
     c.tuned_precision = o.precision;

@@ -54,10 +54,10 @@ void require_model(const Context& c) {
 void sync(Context& c) { SLX_HIP(hipStreamSynchronize(c.stream)); }
 
 double read_scalar(Context& c, const double* dev) {
-  double v = 0.0;
-  SLX_HIP(hipMemcpyAsync(&v, dev, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  double* hb = c.host_buffer(16);
+  SLX_HIP(hipMemcpyAsync(hb + 14, dev, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   sync(c);
-  return v;
+  return hb[14];
 }
 
 void set_layout_from_model(Context& c) {
@@ -249,8 +249,10 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
           k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit,
                       o.damping, c.d_ocw.p, c.oc_state.p, 0, 0.0, target, o.volume_tolerance, o.max_bisections,
                       c.d_tmp2.p, c.partials.p, s);
-        c.oc_state.download(st, 8, s);
+        double* hb = c.host_buffer(16);
+        SLX_HIP(hipMemcpyAsync(hb, c.oc_state.p, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
         sync(c);
+        std::copy(hb, hb + 8, st);
       } while (st[5] == 0.0);
     } else {
       c.oc_state.download(st, 8, s);
@@ -322,6 +324,8 @@ void slx_destroy(slx_context* ctx) {
     (void)hipEventDestroy(pe.second.second);
   }
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  c->pinned = nullptr;
   invalidate_graphs(*c);
   hipStream_t s = c->stream;
   delete c;
@@ -552,7 +556,8 @@ slx_status slx_amg_setup(slx_context* ctx, const slx_amg_options* opt) {
     if (!(o.chebyshev_ratio > 1.0)) fail(SLX_CONFIG_ERROR, "multigrid chebyshev_ratio must exceed 1");
     if (!(o.prolongator_damping > 0.0 && o.prolongator_damping < 2.0)) fail(SLX_CONFIG_ERROR, "multigrid prolongator_damping must lie in (0, 2)");
     if (o.lanczos_steps < 2) fail(SLX_CONFIG_ERROR, "multigrid lanczos_steps must be at least 2");
-    if (o.precision != 0 && o.precision != 1) fail(SLX_CONFIG_ERROR, "multigrid precision must be 0 (FP64) or 1 (FP32 cycle)");
+    if (o.precision < 0 || o.precision > 2)
+      fail(SLX_CONFIG_ERROR, "multigrid precision must be 0 (FP64), 1 (FP32 cycle) or 2 (FP32 cycle, bfloat16 coarse operators)");
     if (c.amg_ready && (o.precision != c.amg_opt.precision || o.max_levels != c.amg_opt.max_levels ||
                         o.coarse_size != c.amg_opt.coarse_size || o.strength_threshold != c.amg_opt.strength_threshold))
       c.amg_ready = false;

@@ -50,31 +50,39 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return t;  // valid on thread 0
 }
 
+// matrix values: FP64, FP32 or bfloat16 bits (widened exactly)
+__device__ __forceinline__ double vload(const double* v, int k) { return v[k]; }
+__device__ __forceinline__ double vload(const float* v, int k) { return (double)v[k]; }
+__device__ __forceinline__ double vload(const uint16_t* v, int k) {
+  return (double)__uint_as_float((uint32_t)v[k] << 16);
+}
+
 // One lane's share of a row's products: the lane takes every LANES-th entry;
 // four entries are loaded before any is used so that each lane keeps eight
 // independent loads (then four gathers) in flight.
-template <int LANES, typename VT>
+template <int LANES, typename VAL, typename VT>
 __device__ __forceinline__ double row_dot(int s, int e, int lane, const int* __restrict__ col,
-                                          const VT* __restrict__ val, const VT* __restrict__ x) {
+                                          const VAL* __restrict__ val, const VT* __restrict__ x) {
   double acc = 0.0;
   int k = s + lane;
   for (; k + 3 * LANES < e; k += 4 * LANES) {
     const int c0 = col[k], c1 = col[k + LANES], c2 = col[k + 2 * LANES], c3 = col[k + 3 * LANES];
-    const VT v0 = val[k], v1 = val[k + LANES], v2 = val[k + 2 * LANES], v3 = val[k + 3 * LANES];
+    const double v0 = vload(val, k), v1 = vload(val, k + LANES), v2 = vload(val, k + 2 * LANES),
+                 v3 = vload(val, k + 3 * LANES);
     const VT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-    acc += (double)v0 * (double)x0;
-    acc += (double)v1 * (double)x1;
-    acc += (double)v2 * (double)x2;
-    acc += (double)v3 * (double)x3;
+    acc += v0 * (double)x0;
+    acc += v1 * (double)x1;
+    acc += v2 * (double)x2;
+    acc += v3 * (double)x3;
   }
-  for (; k < e; k += LANES) acc += (double)val[k] * (double)x[col[k]];
+  for (; k < e; k += LANES) acc += vload(val, k) * (double)x[col[k]];
   return acc;
 }
 
 // MODE 0: y = A x; 1: y = b - A x; 2: y += A x; 3: y = A x and partial x.y
-template <int LANES, int MODE, typename VT>
+template <int LANES, int MODE, typename VT, typename VAL = VT>
 __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ ptr, const int* __restrict__ col,
-                                                  const VT* __restrict__ val, const VT* __restrict__ x,
+                                                  const VAL* __restrict__ val, const VT* __restrict__ x,
                                                   const VT* __restrict__ b, VT* __restrict__ y,
                                                   double* __restrict__ partials) {
   __shared__ double sh[kBlock / 64];
@@ -84,7 +92,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ 
   const int lane = threadIdx.x % LANES;
   double prod = 0.0;
   if (row < n) {
-    const double acc0 = row_dot<LANES>(ptr[row], ptr[row + 1], lane, col, val, x);
+    const double acc0 = row_dot<LANES, VAL, VT>(ptr[row], ptr[row + 1], lane, col, val, x);
     const double acc = group_sum<LANES>(acc0);
     if (lane == 0) {
       if (MODE == 0) y[row] = (VT)acc;
@@ -147,9 +155,9 @@ __global__ void __launch_bounds__(kBlock) k_spmv_staged(int n, const int* __rest
 
 // Chebyshev step (src/fem/Multigrid.cpp:833-853), fused:
 //   r_i <- r_i - (A d_in)_i;  d_out_i = c1 d_in_i + c2 dinv_i r_i;  x_i += d_out_i
-template <int LANES, typename VT>
+template <int LANES, typename VT, typename VAL = VT>
 __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, const int* __restrict__ ptr, const int* __restrict__ col,
-                                                       const VT* __restrict__ val, const VT* __restrict__ dinv,
+                                                       const VAL* __restrict__ val, const VT* __restrict__ dinv,
                                                        const VT* __restrict__ r_in, VT* __restrict__ r_out,
                                                        const VT* __restrict__ d_in, VT* __restrict__ d_out,
                                                        VT* __restrict__ x, double c1, double c2) {
@@ -157,7 +165,7 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, const int* __restri
   const int row = blk * (kBlock / LANES) + threadIdx.x / LANES;
   const int lane = threadIdx.x % LANES;
   if (row >= n) return;
-  const double acc = group_sum<LANES>(row_dot<LANES>(ptr[row], ptr[row + 1], lane, col, val, d_in));
+  const double acc = group_sum<LANES>(row_dot<LANES, VAL, VT>(ptr[row], ptr[row + 1], lane, col, val, d_in));
   if (lane == 0) {
     const double r = (double)r_in[row] - acc;
     const double dn = c1 * (double)d_in[row] + c2 * (double)dinv[row] * r;
@@ -253,6 +261,19 @@ __global__ void k_copy_scalar(double* scal, int from, int to) { scal[to] = scal[
 // Polak-Ribiere beta of flexible CG: (r.z_new - r.z_old) / rz_old
 __global__ void k_flexible_beta(double* scal, int i_new, int i_cross, int i_old, int i_out) {
   scal[i_out] = (scal[i_new] - scal[i_cross]) / scal[i_old];
+}
+
+// double -> bfloat16 bits via FP32, round to nearest even
+__global__ void k_to_bf16(long long n, const double* __restrict__ in, uint16_t* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t u = __float_as_uint((float)in[i]);
+  if ((u & 0x7fffffffu) > 0x7f800000u) {
+    out[i] = (uint16_t)((u >> 16) | 0x40u);
+    return;
+  }
+  u += 0x7fffu + ((u >> 16) & 1u);
+  out[i] = (uint16_t)(u >> 16);
 }
 
 template <typename A, typename B>
@@ -571,16 +592,23 @@ int lanes_for(const DCsr& A) {
   return avg > 48 ? 16 : avg > 12 ? 8 : 4;
 }
 
-template <int MODE, typename VT>
-void launch_spmv(const DCsr& A, const VT* val, const VT* x, const VT* b, VT* y, double* partials, hipStream_t s) {
+template <int MODE, typename VT, typename VAL = VT>
+void launch_spmv(const DCsr& A, const VAL* val, const VT* x, const VT* b, VT* y, double* partials, hipStream_t s) {
   const int lanes = lanes_for(A);
   const int nb = grid8(A.rows, kBlock / lanes);
   if (A.rows == 0) return;
   switch (lanes) {
-    case 16: k_spmv<16, MODE, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
-    case 8: k_spmv<8, MODE, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
-    default: k_spmv<4, MODE, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
+    case 16: k_spmv<16, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
+    case 8: k_spmv<8, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
+    default: k_spmv<4, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
   }
+}
+
+// FP32-vector products read the bfloat16 copy of the values when there is one
+template <int MODE>
+void launch_spmv32(const DCsr& A, const float* x, const float* b, float* y, hipStream_t s) {
+  if (A.val16.p) launch_spmv<MODE, float, uint16_t>(A, A.val16.p, x, b, y, nullptr, s);
+  else launch_spmv<MODE, float>(A, A.val32.p, x, b, y, nullptr, s);
 }
 
 }  // namespace
@@ -597,14 +625,10 @@ void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s) {
 void residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s) {
   launch_spmv<1, double>(A, A.val.p, x, b, r, nullptr, s);
 }
-void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s) {
-  launch_spmv<0, float>(A, A.val32.p, x, nullptr, y, nullptr, s);
-}
-void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s) {
-  launch_spmv<2, float>(A, A.val32.p, x, nullptr, y, nullptr, s);
-}
+void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s) { launch_spmv32<0>(A, x, nullptr, y, s); }
+void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s) { launch_spmv32<2>(A, x, nullptr, y, s); }
 void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s) {
-  launch_spmv<1, float>(A, A.val32.p, x, b, r, nullptr, s);
+  launch_spmv32<1>(A, x, b, r, s);
 }
 int spmv_dot_blocks(const DCsr& A) { return grid8(A.rows, kBlock / lanes_for(A)); }
 void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s) {
@@ -624,16 +648,16 @@ void spmv_serial(const DCsr& A, const double* x, double* y, hipStream_t s) {
   }
 }
 
-template <typename VT>
-static void cheb_step_t(const DCsr& A, const VT* val, const VT* dinv, const VT* r_in, VT* r_out, const VT* d_in,
+template <typename VT, typename VAL = VT>
+static void cheb_step_t(const DCsr& A, const VAL* val, const VT* dinv, const VT* r_in, VT* r_out, const VT* d_in,
                         VT* d_out, VT* x, double c1, double c2, hipStream_t s) {
   const int lanes = lanes_for(A);
   const int nb = grid8(A.rows, kBlock / lanes);
   if (A.rows == 0) return;
   switch (lanes) {
-    case 16: k_cheb_step<16, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
-    case 8: k_cheb_step<8, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
-    default: k_cheb_step<4, VT><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    case 16: k_cheb_step<16, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    case 8: k_cheb_step<8, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    default: k_cheb_step<4, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
   }
 }
 void cheb_step(const DCsr& A, const double* dinv, const double* r_in, double* r_out, const double* d_in, double* d_out,
@@ -642,7 +666,10 @@ void cheb_step(const DCsr& A, const double* dinv, const double* r_in, double* r_
 }
 void cheb_step32(const DCsr& A, const float* dinv, const float* r_in, float* r_out, const float* d_in, float* d_out,
                  float* x, double c1, double c2, hipStream_t s) {
-  cheb_step_t<float>(A, A.val32.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s);
+  if (A.val16.p)
+    cheb_step_t<float, uint16_t>(A, A.val16.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s);
+  else
+    cheb_step_t<float>(A, A.val32.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s);
 }
 void cheb_first_zero(int n, const double* dinv, const double* b, double theta, double* x, double* d, hipStream_t s) {
   if (n) k_cheb_first_zero<double><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, b, theta, x, d);
@@ -688,6 +715,9 @@ void flexible_beta(double* scal, int i_new, int i_cross, int i_old, int i_out, h
 }
 void to_float(long long n, const double* in, float* out, hipStream_t s) {
   if (n) k_convert<double, float><<<blocks_for(n, kBlock), kBlock, 0, s>>>((int)n, in, out);
+}
+void to_bf16(long long n, const double* in, uint16_t* out, hipStream_t s) {
+  if (n) k_to_bf16<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, in, out);
 }
 void to_double(long long n, const float* in, double* out, hipStream_t s) {
   if (n) k_convert<float, double><<<blocks_for(n, kBlock), kBlock, 0, s>>>((int)n, in, out);

@@ -96,10 +96,10 @@ SegState& prepare(Context& c, const std::vector<int>& seg_ptr) {
 }
 
 std::vector<double> download(Context& c, const DBuf<double>& d, int n) {
-  std::vector<double> h((size_t)n);
-  d.download(h.data(), (size_t)n, c.stream);
+  double* hb = c.host_buffer((size_t)n + 16);
+  SLX_HIP(hipMemcpyAsync(hb + 16, d.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c.stream));
   SLX_HIP(hipStreamSynchronize(c.stream));
-  return h;
+  return std::vector<double>(hb + 16, hb + 16 + n);
 }
 
 }  // namespace
@@ -111,7 +111,7 @@ void seg_pcg(Context& c, const std::vector<int>& seg_ptr, const double* b, doubl
   const long long n = c.K.rows;
   hipStream_t s = c.stream;
   const kseg::Chunks& ch = st.ch;
-  const bool flexible = c.amg_opt.precision == 1;
+  const bool flexible = c.amg_opt.precision >= 1;
   if (flexible != st.flexible) st.drop_graphs();
   st.flexible = flexible;
   res.assign((size_t)ns, slx_pcg_result{0, 0.0, 0});

@@ -71,7 +71,7 @@ struct DBuf {
   }
 };
 
-// Device CSR matrix (int32 indices, FP64 values, optional FP32 copy of the values).
+// Device CSR matrix (int32 indices, FP64 values, optional FP32 and bfloat16 copies of the values).
 struct DCsr {
   int rows = 0, cols = 0;
   long long nnz = 0;
@@ -80,6 +80,7 @@ struct DCsr {
   DBuf<int> ptr, col;
   DBuf<double> val;
   DBuf<float> val32;
+  DBuf<uint16_t> val16;  // bfloat16 copy (coarse operators of precision 2); FP32 products read it when present
   void upload_pattern(const HCsr& h, hipStream_t s);
   void upload(const HCsr& h, hipStream_t s) {
     upload_pattern(h, s);
@@ -191,6 +192,10 @@ struct Context {
   DBuf<double> partials;  // block partial sums
   DBuf<double> scal;      // device scalars
   std::vector<double> h_scal;
+  // pinned host memory for small device -> host reads (scalars, per-segment sums)
+  double* pinned = nullptr;
+  size_t pinned_n = 0;
+  double* host_buffer(size_t n);
 
   // ---- topology optimisation -------------------------------------------------
   double volume_fraction = 0.5;

@@ -44,6 +44,7 @@ void copy_scalar(double* scal, int from, int to, hipStream_t s);
 void flexible_beta(double* scal, int i_new, int i_cross, int i_old, int i_out, hipStream_t s);
 void to_float(long long n, const double* in, float* out, hipStream_t s);
 void to_double(long long n, const float* in, double* out, hipStream_t s);
+void to_bf16(long long n, const double* in, uint16_t* out, hipStream_t s);
 void axpy(long long n, double a, const double* x, double* y, hipStream_t s);
 
 // structured-grid assembly of the K_ff values into K's pattern

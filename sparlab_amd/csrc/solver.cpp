@@ -184,9 +184,10 @@ void cycle32(Context& c, size_t l, const float* b, float* x) {
 
 void read_scalars(Context& c, int from, int count) {
   c.h_scal.resize(16);
-  SLX_HIP(hipMemcpyAsync(c.h_scal.data() + from, c.scal.p + from, sizeof(double) * count, hipMemcpyDeviceToHost,
-                         c.stream));
+  double* hb = c.host_buffer(16);
+  SLX_HIP(hipMemcpyAsync(hb + from, c.scal.p + from, sizeof(double) * count, hipMemcpyDeviceToHost, c.stream));
   SLX_HIP(hipStreamSynchronize(c.stream));
+  for (int q = from; q < from + count; ++q) c.h_scal[(size_t)q] = hb[q];
 }
 
 }  // namespace
@@ -202,6 +203,17 @@ Timed::~Timed() {
   if (!a) return;
   (void)hipEventRecord(b, c.stream);
   c.pending.push_back({name, {a, b}});
+}
+
+double* Context::host_buffer(size_t n) {
+  if (n > pinned_n) {
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    pinned_n = 0;
+    SLX_HIP(hipHostMalloc((void**)&pinned, std::max<size_t>(n, 512) * sizeof(double), hipHostMallocDefault));
+    pinned_n = std::max<size_t>(n, 512);
+  }
+  return pinned;
 }
 
 hipEvent_t Context::take_event() {
@@ -320,6 +332,12 @@ void prepare_mixed(Context& c) {
       A.val32.alloc((size_t)A.nnz);
       k::to_float(A.nnz, A.val.p, A.val32.p, s);
     }
+    if (c.amg_opt.precision == 2 && l > 0) {  // coarse operators stored in bfloat16
+      A.val16.alloc((size_t)A.nnz);
+      k::to_bf16(A.nnz, A.val.p, A.val16.p, s);
+    } else {
+      A.val16.release();
+    }
     if (lv.has_next) {
       lv.P.val32.alloc((size_t)lv.P.nnz);
       k::to_float(lv.P.nnz, lv.P.val.p, lv.P.val32.p, s);
@@ -342,7 +360,7 @@ void vcycle(Context& c, const double* r, double* z) {
   if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: apply() called before setup()");
   Level& top = c.levels[0];
   const size_t n = (size_t)top.n;
-  if (c.amg_opt.precision == 1) {
+  if (c.amg_opt.precision >= 1) {
     k::to_float((long long)n, r, top.b32.p, c.stream);
     cycle32(c, 0, top.b32.p, top.x32.p);
     k::to_double((long long)n, top.x32.p, z, c.stream);
@@ -380,7 +398,7 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
   const long long n = c.K.rows;
   hipStream_t s = c.stream;
   slx_pcg_result res{0, 0.0, 0};
-  const bool flexible = c.amg_opt.precision == 1;
+  const bool flexible = c.amg_opt.precision >= 1;
   const double bnorm = std::sqrt(device_dot(c, n, b, b));
   if (bnorm == 0.0) {
     k::fill(n, 0.0, x_out, s);

@@ -54,10 +54,10 @@ void ensure_model_net(Context& c, WarmStart& w) {
 }
 
 double read_loss(Context& c, const WarmNet& n) {
-  double v = 0.0;
-  SLX_HIP(hipMemcpyAsync(&v, n.loss_device(), sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  double* hb = c.host_buffer(16);
+  SLX_HIP(hipMemcpyAsync(hb + 13, n.loss_device(), sizeof(double), hipMemcpyDeviceToHost, c.stream));
   SLX_HIP(hipStreamSynchronize(c.stream));
-  return v;
+  return hb[13];
 }
 
 void train(Context& c, WarmStart& w) {

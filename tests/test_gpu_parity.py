@@ -205,6 +205,13 @@ def test_mixed_precision_solve_meets_tolerance(ctx):
     assert cv64 and cv32 and rr32 <= 1e-10
     assert b @ x32 == pytest.approx(b @ x64, rel=1e-9)
     assert it32 <= it64 + 3
+    # coarse operators in bfloat16: the same tolerance, the same answer
+    g.amg_setup(precision=2)
+    x16, it16, rr16, cv16 = g.solve(b, tol=1e-10)
+    print(f"iterations fp64 {it64}, fp32 cycle {it32}, bf16 coarse {it16}", flush=True)
+    assert cv16 and rr16 <= 1e-10
+    assert b @ x16 == pytest.approx(b @ x64, rel=1e-9)
+    assert it16 <= it64 + 8
 
 
 def test_matrix_free_operator_matches_assembled_kff(ctx):
