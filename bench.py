@@ -134,9 +134,10 @@ def level_info(g, level, which):
 def rooflines(g, m, timers):
     """Roofline of each kernel timed with HIP events over the timed region, largest total first.
 
-    galerkin_ap / galerkin_raw: the level-0 block products A P and R (A P), HBM-bound;
-    algorithmic bytes = 8 (nnz A + nnz B + nnz C) values + 8 per contribution-list entry
-    + 20 per output block (item metadata) + 8 per output row (row pointers).
+    galerkin_ap / galerkin_raw: the level-0 row products A P and R (A P), HBM-bound;
+    algorithmic bytes = 8 (nnz A + nnz B + nnz C) values + 12 per run (descriptor and
+    entry offset) + 4 per entry (block positions) + 12 per output node (order, run
+    pointer, row pointer) + 4 per output row (A row pointer).
     spmv: the matrix-free K_ff product of CG (with x.y partials), FP64 compute-bound;
     1200 flop per element (Ke0 u_e: 576 FMA, the factor: 24 more).
     """
@@ -158,18 +159,19 @@ def rooflines(g, m, timers):
             continue
         if name == "galerkin_ap":
             a_nnz, b_nnz, c_nnz, rows = nnz, sizes["P"], sizes["AP"], n
-            items, entries = sizes["plan_ap"][0], sizes["plan_ap"][1]
-            label = "k_block_product<0> (level-0 A P, FP64)"
+            nodes, runs, entries = sizes["plan_ap"]
+            label = "k_row_product<3,3> (level-0 A P, FP64)"
         else:
             a_nnz, b_nnz, c_nnz, rows = sizes["R"], sizes["AP"], sizes["RAW"][2], sizes["RAW"][0]
-            items, entries = sizes["plan_raw"][0], sizes["plan_raw"][1]
-            label = "k_block_product<0> (level-0 R (A P), FP64)"
-        nbytes = 8 * (a_nnz + b_nnz + c_nnz) + 8 * entries + 20 * items + 8 * rows
+            nodes, runs, entries = sizes["plan_raw"]
+            label = "k_row_product<6,3> (level-0 R (A P), FP64)"
+        nbytes = 8 * (a_nnz + b_nnz + c_nnz) + 12 * runs + 4 * entries + 12 * nodes + 4 * rows
         ach = nbytes / (avg * 1e-3) / 1e9
         out.append({"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                     "traffic": None, "kernel": label, "bytes_per_launch": nbytes, "avg_ms": avg,
                     "launches": launches, "total_ms": ms,
-                    "sizes": {"nnz_a": a_nnz, "nnz_b": b_nnz, "nnz_c": c_nnz, "blocks": items, "entries": entries}})
+                    "sizes": {"nnz_a": a_nnz, "nnz_b": b_nnz, "nnz_c": c_nnz, "nodes": nodes, "runs": runs,
+                              "entries": entries}})
     out.sort(key=lambda r: -r["total_ms"])
     return out
 

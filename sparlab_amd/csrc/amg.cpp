@@ -704,6 +704,110 @@ bool build_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& cbp, 
   return true;
 }
 
+// Row plan of C = A B (one wavefront per C node): the runs of each C node's A
+// row (run table: offset in A's row, first B row, B rows, B node) and, per run,
+// one entry for each aggregate block of the B node's rows: its position in B's
+// row, its position in C's row (the C block of the same aggregate) and its
+// width. False if a field does not fit its bits, a block is missing from C's
+// pattern or a row is too long for the kernel's LDS.
+bool build_row_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& cptr, const std::vector<int>& cbp,
+                    const std::vector<int>& cba, const std::vector<int>& cbq, const std::vector<int>& run_ptr,
+                    const std::vector<int>& runs, const std::vector<int>& bbp, const std::vector<int>& bba,
+                    const std::vector<int>& bbq, const std::vector<int>& bptr, const std::vector<int>& agg_off,
+                    const std::vector<int>* order, RowPlan& plan, hipStream_t s) {
+  const int nn = (int)cnode_ptr.size() - 1;
+  const int naggs = (int)agg_off.size() - 1;
+  const long long nruns = run_ptr[(size_t)nn];
+  std::vector<int> run_entry((size_t)nruns + 1, 0);
+  long long total = 0;
+  int dmax = 1, jmax = 1, wmax = 1, lenc_max = 1, emax = 0;
+  for (int I = 0; I < nn; ++I) {
+    const int r0 = cnode_ptr[(size_t)I];
+    dmax = std::max(dmax, cnode_ptr[(size_t)I + 1] - r0);
+    lenc_max = std::max(lenc_max, cptr[(size_t)r0 + 1] - cptr[(size_t)r0]);
+    const long long e0 = total;
+    for (int t = run_ptr[(size_t)I]; t < run_ptr[(size_t)I + 1]; ++t) {
+      const int J = runs[(size_t)4 * t + 3];
+      const int nb = bbp[(size_t)J + 1] - bbp[(size_t)J];
+      if (nb < 1) return false;
+      jmax = std::max(jmax, runs[(size_t)4 * t + 2]);
+      run_entry[(size_t)t] = (int)total;
+      total += nb;
+      if (total > 0x7fffffffLL) return false;
+    }
+    emax = std::max(emax, (int)(total - e0));
+  }
+  run_entry[(size_t)nruns] = (int)total;
+  for (int g = 0; g < naggs; ++g) wmax = std::max(wmax, agg_off[(size_t)g + 1] - agg_off[(size_t)g]);
+  if (dmax > 6 || jmax > 6 || wmax > 7 || lenc_max > 0xffff) return false;
+  std::vector<uint2> run((size_t)nruns);
+  std::vector<unsigned> entry((size_t)total);
+  std::atomic<bool> ok{true};
+#pragma omp parallel
+  {
+    std::vector<int> pos_of((size_t)naggs, -1);
+#pragma omp for schedule(dynamic, 256)
+    for (int I = 0; I < nn; ++I) {
+      for (int q = cbp[(size_t)I]; q < cbp[(size_t)I + 1]; ++q) pos_of[(size_t)cba[(size_t)q]] = cbq[(size_t)q];
+      for (int t = run_ptr[(size_t)I]; t < run_ptr[(size_t)I + 1]; ++t) {
+        const int koff = runs[(size_t)4 * t], brow = runs[(size_t)4 * t + 1], dj = runs[(size_t)4 * t + 2];
+        const int J = runs[(size_t)4 * t + 3];
+        const int lb = bptr[(size_t)brow + 1] - bptr[(size_t)brow];
+        if (koff > 0xffff || lb > 0x1fff || dj > 6 || dj < 1) {
+          ok.store(false, std::memory_order_relaxed);
+          continue;
+        }
+        run[(size_t)t] = make_uint2((unsigned)bptr[(size_t)brow], ((unsigned)koff << 16) | ((unsigned)lb << 3) | (unsigned)dj);
+        int e = run_entry[(size_t)t];
+        for (int u = bbp[(size_t)J]; u < bbp[(size_t)J + 1]; ++u, ++e) {
+          const int g = bba[(size_t)u];
+          const int cpos = pos_of[(size_t)g], bpos = bbq[(size_t)u];
+          const int w = agg_off[(size_t)g + 1] - agg_off[(size_t)g];
+          if (cpos < 0 || bpos > 0x1fff || cpos > 0xffff) {
+            ok.store(false, std::memory_order_relaxed);
+            continue;
+          }
+          entry[(size_t)e] = ((unsigned)cpos << 16) | ((unsigned)bpos << 3) | (unsigned)w;
+        }
+      }
+      for (int q = cbp[(size_t)I]; q < cbp[(size_t)I + 1]; ++q) pos_of[(size_t)cba[(size_t)q]] = -1;
+    }
+  }
+  if (!ok.load()) return false;
+  // LDS per wavefront: accumulators (the kernel's row bound: 3 or 6) and, when
+  // small enough, the node's entries
+  const int dk = dmax <= 3 && jmax <= 3 ? 3 : 6;
+  const size_t budget = 16 * 1024;
+  const bool pre = k::row_product_lds(dk, lenc_max, emax) <= budget;
+  const size_t wave_lds = k::row_product_lds(dk, lenc_max, pre ? emax : 0);
+  int wpb = 4;
+  while (wpb > 1 && wave_lds * (size_t)wpb > 64 * 1024) wpb /= 2;
+  if (wave_lds * (size_t)wpb > 64 * 1024) return false;
+  std::vector<int> ord((size_t)nn);
+  if (order) {
+    if ((int)order->size() != nn) return false;
+    ord = *order;
+  } else {
+    for (int I = 0; I < nn; ++I) ord[(size_t)I] = I;
+  }
+  plan.nnodes = nn;
+  plan.dmax = dmax;
+  plan.jmax = jmax;
+  plan.wmax = wmax;
+  plan.lenc_max = lenc_max;
+  plan.emax = emax;
+  plan.wpb = wpb;
+  plan.pre = pre;
+  plan.runs = nruns;
+  plan.entries = total;
+  plan.order.upload(ord, s);
+  plan.run_ptr.upload(run_ptr, s);
+  plan.run_entry.upload(run_entry, s);
+  plan.run.upload(run, s);
+  plan.entry.upload(entry, s);
+  return true;
+}
+
 // Nodes sorted along a Morton (Z-order) curve of their coordinates (n x 3):
 // products visit nodes that share neighbours close together in time.
 std::vector<int> morton_order(const std::vector<double>& xyz) {
@@ -778,8 +882,8 @@ void prepare_node_products(Level& lv, const Level& next, const HCsr& a, const HC
   const std::vector<int>* fo = xyz ? &f_order : nullptr;
   const std::vector<int>* co = xyz ? &c_order : nullptr;
   if (!build_plan(lv.node_ptr, pp, pa, pq, arp, ar, tp, ta, tq, pt.ptr, off, fo, lv.plan_p, s) ||
-      !build_plan(lv.node_ptr, qp, qa, qq, arp, ar, pp, pa, pq, ppat.ptr, off, fo, lv.plan_ap, s) ||
-      !build_plan(next.node_ptr, wp, wa, wq, rrp, rr, qp, qa, qq, appat.ptr, off, co, lv.plan_raw, s))
+      !build_row_plan(lv.node_ptr, appat.ptr, qp, qa, qq, arp, ar, pp, pa, pq, ppat.ptr, off, fo, lv.plan_ap, s) ||
+      !build_row_plan(next.node_ptr, rawpat.ptr, wp, wa, wq, rrp, rr, qp, qa, qq, appat.ptr, off, co, lv.plan_raw, s))
     return;
   std::vector<int> row_agg((size_t)lv.n);
   for (int i = 0; i < lv.n; ++i) row_agg[(size_t)i] = c_of[(size_t)pt.col[(size_t)pt.ptr[(size_t)i]]];
@@ -807,6 +911,30 @@ k::BlockProductArgs plan_args(const Level& lv, const ProductPlan& plan, bool a_i
   g.cptr = C.ptr.p;
   g.cval = C.val.p;
   g.agg_off = lv.c_node_ptr.p;
+  return g;
+}
+
+k::RowProductArgs row_args(const Level& lv, const RowPlan& plan, bool a_is_r, const DCsr& A, const DCsr& B, DCsr& C) {
+  k::RowProductArgs g{};
+  g.nnodes = plan.nnodes;
+  g.dmax = plan.dmax;
+  g.jmax = plan.jmax;
+  g.wmax = plan.wmax;
+  g.lenc_max = plan.lenc_max;
+  g.emax = plan.emax;
+  g.wpb = plan.wpb;
+  g.pre = plan.pre;
+  g.order = plan.order.p;
+  g.node_ptr = a_is_r ? lv.c_node_ptr.p : lv.f_node_ptr.p;
+  g.aptr = A.ptr.p;
+  g.aval = A.val.p;
+  g.run_ptr = plan.run_ptr.p;
+  g.run = plan.run.p;
+  g.run_entry = plan.run_entry.p;
+  g.entry = plan.entry.p;
+  g.bval = B.val.p;
+  g.cptr = C.ptr.p;
+  g.cval = C.val.p;
   return g;
 }
 
@@ -1123,11 +1251,11 @@ void amg_setup(Context& c) {
       lap(3);
       {
         Timed t(c, l == 0 ? "galerkin_ap" : nullptr);
-        k::block_product(plan_args(cur, cur.plan_ap, false, A, cur.P, cur.AP), false, s);
+        k::row_product(row_args(cur, cur.plan_ap, false, A, cur.P, cur.AP), s);
       }
       {
         Timed t(c, l == 0 ? "galerkin_raw" : nullptr);
-        k::block_product(plan_args(cur, cur.plan_raw, true, cur.R, cur.AP, cur.RAW), false, s);
+        k::row_product(row_args(cur, cur.plan_raw, true, cur.R, cur.AP, cur.RAW), s);
       }
     } else {
       k::smooth_prolongator(A, cur.Ptent, cur.P, cur.dinv.p, omega, flag.p, s);

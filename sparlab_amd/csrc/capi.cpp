@@ -687,16 +687,11 @@ slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long
   return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
     if (!c.amg_ready || level < 0 || level >= (int)c.levels.size()) fail(SLX_SOLVER_ERROR, "no such multigrid level");
     Level& lv = c.levels[(size_t)level];
-    if (which == 6 || which == 7) {  // block-product plans of A P and R (A P): items, list entries, lanes per item
-      const ProductPlan& pl = which == 6 ? lv.plan_ap : lv.plan_raw;
-      long long entries = 0;
-      if (pl.nitems)
-        SLX_HIP(hipMemcpyAsync(&entries, pl.item_begin.p + pl.nitems, sizeof(long long), hipMemcpyDeviceToHost,
-                               c.stream));
-      sync(c);
-      info[0] = pl.nitems;
-      info[1] = entries;
-      info[2] = pl.wmax;
+    if (which == 6 || which == 7) {  // row plans of A P and R (A P): C nodes, runs, entries
+      const RowPlan& pl = which == 6 ? lv.plan_ap : lv.plan_raw;
+      info[0] = pl.nnodes;
+      info[1] = pl.runs;
+      info[2] = pl.entries;
       return;
     }
     const DCsr* m = which == 0   ? (level == 0 ? &c.K : &lv.A)

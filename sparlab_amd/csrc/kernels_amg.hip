@@ -212,6 +212,131 @@ __global__ void __launch_bounds__(256) k_block_product(BlockProductArgs g) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Row products C = A B with one wavefront per C node (the C and A rows of the
+// node share one pattern). The node's A row is walked run by run; a run is the
+// stretch of A's row that meets one B node, and its values are the same for
+// every lane (scalar loads). For each run the lanes take the B node's blocks,
+// one lane per block column, read the run's B rows with coalesced loads and add
+// A(run) . B(run rows, column) into the node's accumulators in LDS at the
+// block's position in C's row. Within one run no two lanes share an
+// accumulator, so every entry of C sums its products in the order of A's row,
+// the order of the CPU restatement (Multigrid.cpp:199-225). The next step's B
+// values are loaded while the current step is added.
+// With PRE the node's entries are staged in LDS first.
+// ---------------------------------------------------------------------------
+struct Step {
+  int t, eb, e1;  // run, first entry of the pass, end of the run's entries
+  uint2 rd;       // the run's descriptor
+};
+
+template <int DMAX, int JMAX, bool PRE>
+__global__ void __launch_bounds__(256) k_row_product(
+    int nnodes, int wpb, int lenc_max, int emax, int wmax, const int* __restrict__ order,
+    const int* __restrict__ node_ptr, const int* __restrict__ aptr, const double* __restrict__ aval,
+    const int* __restrict__ run_ptr, const uint2* __restrict__ run, const int* __restrict__ run_entry,
+    const unsigned* __restrict__ entry, const double* __restrict__ bval, const int* __restrict__ cptr,
+    double* __restrict__ cval) {
+  extern __shared__ double lds[];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // consecutive nodes (Morton order) stay on one XCD and its L2: block b runs
+  // logical block (b % 8) * (grid / 8) + b / 8 (the grid is a multiple of 8)
+  const int blk = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  const int slot = blk * wpb + wid;
+  if (slot >= nnodes) return;
+  double* acc = lds + (size_t)wid * ((size_t)DMAX * lenc_max + (PRE ? (size_t)(emax + 1) / 2 : 0));
+  unsigned* ent = reinterpret_cast<unsigned*>(acc + (size_t)DMAX * lenc_max);
+  const int I = __builtin_amdgcn_readfirstlane(order[slot]);
+  const int r0 = node_ptr[I];
+  const int d = min(node_ptr[I + 1] - r0, DMAX);
+  const int c0 = cptr[r0], lenc = cptr[r0 + 1] - c0;
+  for (int p = lane; p < d * lenc; p += 64) acc[p] = 0.0;
+  int abase[DMAX];
+#pragma unroll
+  for (int r = 0; r < DMAX; ++r) abase[r] = aptr[r0 + (r < d ? r : d - 1)];
+  const int t0 = run_ptr[I], t1 = run_ptr[I + 1];
+  if (t0 < t1) {
+    const int E0 = run_entry[t0];
+    if (PRE) {
+      const int E1 = run_entry[t1];
+      for (int e = E0 + lane; e < E1; e += 64) ent[e - E0] = entry[e];
+    }
+    const int per = 64 / wmax;
+    const int ul = lane / wmax, tt = lane - ul * wmax;
+    // one step = one pass of the lanes over (up to per) blocks of run t from
+    // entry eb; fetch() gives the lane's C position (or -1) and its B values
+    auto fetch = [&](uint2 rd, int eb, int e1, double(&bv)[JMAX]) -> int {
+      const int lb = (int)((rd.y >> 3) & 0x1fffu), dj = (int)(rd.y & 7u);
+      const int e = min(eb + ul, e1 - 1);
+      const unsigned en = PRE ? ent[e - E0] : entry[e];
+      const int w = (int)(en & 7u), bpos = (int)((en >> 3) & 0x1fffu), cpos = (int)(en >> 16);
+      const bool live = ul < per && eb + ul < e1 && tt < w;
+      const unsigned idx = rd.x + (unsigned)(bpos + min(tt, w - 1));
+#pragma unroll
+      for (int m = 0; m < JMAX; ++m) bv[m] = bval[idx + (unsigned)(min(m, dj - 1) * lb)];
+      return live ? cpos + tt : -1;
+    };
+    auto load_a = [&](uint2 rd, double(&av)[DMAX][JMAX]) {
+      const int koff = (int)(rd.y >> 16), dj = (int)(rd.y & 7u);
+#pragma unroll
+      for (int r = 0; r < DMAX; ++r)
+#pragma unroll
+        for (int m = 0; m < JMAX; ++m) av[r][m] = aval[abase[r] + koff + min(m, dj - 1)];
+    };
+    // the step after st (st.t == t1: past the last step, valid addresses, no lane live)
+    auto next = [&](const Step& st) -> Step {
+      Step n = st;
+      n.eb = st.eb + per;
+      if (n.eb >= st.e1) {
+        n.t = st.t + 1;
+        n.eb = st.e1;
+        if (n.t < t1) {
+          n.e1 = run_entry[n.t + 1];
+          n.rd = run[n.t];
+        } else {
+          n.eb = st.e1 - 1;
+        }
+      }
+      return n;
+    };
+    auto add = [&](const Step& st, int pos, const double(&bv)[JMAX], const double(&av)[DMAX][JMAX]) {
+      const int dj = (int)(st.rd.y & 7u);
+      if (pos < 0) return;
+#pragma unroll
+      for (int r = 0; r < DMAX; ++r) {
+        if (r >= d) break;
+        double v = acc[r * lenc + pos];
+#pragma unroll
+        for (int m = 0; m < JMAX; ++m)
+          if (m < dj) v += av[r][m] * bv[m];
+        acc[r * lenc + pos] = v;
+      }
+    };
+    // two register sets in turn: the B values of the next step load while
+    // the current step is added
+    Step c{t0, E0, run_entry[t0 + 1], run[t0]};
+    double b0[JMAX], b1[JMAX], a[DMAX][JMAX];
+    int p0 = fetch(c.rd, c.eb, c.e1, b0);
+    load_a(c.rd, a);
+    while (true) {
+      const Step n = next(c);
+      int p1 = fetch(n.rd, n.eb, n.e1, b1);
+      if (n.t >= t1) p1 = -1;
+      add(c, p0, b0, a);
+      if (n.t >= t1) break;
+      if (n.t != c.t) load_a(n.rd, a);
+      const Step m = next(n);
+      p0 = fetch(m.rd, m.eb, m.e1, b0);
+      if (m.t >= t1) p0 = -1;
+      add(n, p1, b1, a);
+      if (m.t >= t1) break;
+      if (m.t != n.t) load_a(m.rd, a);
+      c = m;
+    }
+  }
+  for (int p = lane; p < d * lenc; p += 64) cval[c0 + p] = acc[p];
+}
+
 // Lanczos on D^-1/2 A D^-1/2 (Multigrid.cpp:304-317). The step's scalars stay
 // in device memory, so a level's steps run without host round trips:
 // w_i = s_i (A (s v))_i - beta vprev_i, with s = sqrt(dinv), beta = *beta_ptr (0 if null)
@@ -281,6 +406,29 @@ void block_product(const BlockProductArgs& a, bool smooth, hipStream_t s) {
   } else {
     if (smooth) k_block_product<1, 6><<<grid, 256, 0, s>>>(a); else k_block_product<0, 6><<<grid, 256, 0, s>>>(a);
   }
+}
+
+void row_product(const RowProductArgs& a, hipStream_t s) {
+  if (a.nnodes == 0) return;
+  if (a.wmax < 1 || a.wmax > kWave || a.dmax < 1 || a.dmax > 6 || a.jmax < 1 || a.jmax > 6 || a.wpb < 1 || a.wpb > 4)
+    fail(SLX_SOLVER_ERROR, "multigrid: row product shape outside the device kernel's range");
+  const int dm = a.dmax <= 3 && a.jmax <= 3 ? 3 : 6;
+  const int jm = a.jmax <= 3 ? 3 : 6;
+  const size_t lds = row_product_lds(dm, a.lenc_max, a.pre ? a.emax : 0) * (size_t)a.wpb;
+  if (lds > 64 * 1024) fail(SLX_SOLVER_ERROR, "multigrid: a row product row is too long for the device kernel");
+  const int grid = (int)(((long long)a.nnodes + a.wpb - 1) / a.wpb + 7) / 8 * 8;
+#define SLX_ROW_PRODUCT(D, J, P)                                                                                  \
+  k_row_product<D, J, P><<<grid, 64 * a.wpb, lds, s>>>(a.nnodes, a.wpb, a.lenc_max, a.emax, a.wmax, a.order,     \
+                                                       a.node_ptr, a.aptr, a.aval, a.run_ptr, a.run, a.run_entry, \
+                                                       a.entry, a.bval, a.cptr, a.cval)
+  if (dm == 3) {
+    if (a.pre) SLX_ROW_PRODUCT(3, 3, true); else SLX_ROW_PRODUCT(3, 3, false);
+  } else if (jm == 3) {
+    if (a.pre) SLX_ROW_PRODUCT(6, 3, true); else SLX_ROW_PRODUCT(6, 3, false);
+  } else {
+    if (a.pre) SLX_ROW_PRODUCT(6, 6, true); else SLX_ROW_PRODUCT(6, 6, false);
+  }
+#undef SLX_ROW_PRODUCT
 }
 
 void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s) {

@@ -98,6 +98,19 @@ struct ProductPlan {
   DBuf<uint2> list;  // two words per entry (kernels_amg.hip)
 };
 
+// Row plan of C = A B (one wavefront per C node): per C node its runs of A's
+// row (one per B node met), per run the B node's aggregate blocks with their
+// positions in B's and C's rows (kernels_amg.hip k_row_product).
+struct RowPlan {
+  int nnodes = 0;
+  int dmax = 0, jmax = 0, wmax = 0, lenc_max = 0, emax = 0, wpb = 0;
+  bool pre = false;
+  long long runs = 0, entries = 0;
+  DBuf<int> order, run_ptr, run_entry;
+  DBuf<uint2> run;
+  DBuf<unsigned> entry;
+};
+
 // One level of the SA-AMG hierarchy (structure of src/fem/Multigrid.cpp:336-354).
 // Structure (node blocks, aggregates, tentative prolongator, product patterns)
 // is built on the host once per sparsity structure; every numeric array lives
@@ -118,7 +131,8 @@ struct Level {
   // this level has node-uniform rows in complete aggregate blocks
   bool node_ok = false;
   DBuf<int> f_node_ptr, c_node_ptr, row_agg;
-  ProductPlan plan_p, plan_ap, plan_raw;  // P = (I - w D^-1 A) P-hat, A P, R (A P)
+  ProductPlan plan_p;          // P = (I - w D^-1 A) P-hat
+  RowPlan plan_ap, plan_raw;   // A P, R (A P)
   DBuf<double> dinv, sq, lz_v, lz_vprev, lz_w, lz_start;
   DBuf<double> x, b, res, d, d2;
   DBuf<float> dinv32, x32, b32, res32, d32, d232;

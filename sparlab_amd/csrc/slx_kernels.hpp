@@ -97,6 +97,34 @@ struct BlockProductArgs {
   const int* row_agg;         // aggregate of each fine row
 };
 void block_product(const BlockProductArgs& a, bool smooth, hipStream_t s);
+// C = A B, one wavefront per C node (kernels_amg.hip k_row_product)
+struct RowProductArgs {
+  int nnodes;                 // C nodes
+  int dmax, jmax, wmax;       // most rows per C node, most B rows per run, widest aggregate block
+  int lenc_max, emax;         // longest C row, most entries of one C node
+  int wpb;                    // wavefronts (C nodes) per block
+  bool pre;                   // stage each node's entries in LDS
+  const int* order;           // C nodes in processing order
+  const int* node_ptr;        // rows of C node I: [node_ptr[I], node_ptr[I+1])
+  const int* aptr;
+  const double* aval;
+  const int* run_ptr;         // runs of C node I: [run_ptr[I], run_ptr[I+1])
+  // run: x = index in B's values of the run's first B row,
+  // y = (offset of the run in A's row << 16) | (B row length << 3) | B rows
+  const uint2* run;
+  const int* run_entry;       // entries of run t: [run_entry[t], run_entry[t+1])
+  // entry: one B block of the run's B node: (position of its aggregate block
+  // in C's row << 16) | (position in B's row << 3) | width
+  const unsigned* entry;
+  const double* bval;
+  const int* cptr;
+  double* cval;
+};
+// LDS bytes of one wavefront of k_row_product
+inline size_t row_product_lds(int dmax, int lenc_max, int emax) {
+  return sizeof(double) * ((size_t)dmax * (size_t)lenc_max + (size_t)(emax + 1) / 2);
+}
+void row_product(const RowProductArgs& a, hipStream_t s);
 void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s);
 void smooth_prolongator(const DCsr& A, const DCsr& Ptent, DCsr& P, const double* dinv, double omega, int* error_flag,
                         hipStream_t s);
