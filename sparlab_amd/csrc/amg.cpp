@@ -1056,9 +1056,14 @@ void autotune(Context& c) {
   const bool f32 = c.amg_opt.precision >= 1;
   std::vector<DCsr*> mats;
   for (size_t l = 0; l + 1 < c.levels.size(); ++l) {
-    if (l > 0 || !c.mf) mats.push_back(&level_op(c, l));
-    mats.push_back(&c.levels[l].P);
-    mats.push_back(&c.levels[l].R);
+    if (l > 0 || !c.mf) {
+      mats.push_back(&level_op(c, l));
+      mats.push_back(&c.levels[l].P);
+      mats.push_back(&c.levels[l].R);
+    } else {
+      mats.push_back(&c.levels[l].Ptent);
+      mats.push_back(&c.levels[l].Rt);
+    }
   }
   size_t rows = 1, cols = 1;
   for (DCsr* m : mats) {
@@ -1210,6 +1215,14 @@ void amg_setup(Context& c) {
       std::vector<int> ia, ib;
       symmetrize_pattern(rawpat, spat, ia, ib);
       lv.Ptent.upload(pt, s);
+      {
+        HCsr rt;
+        std::vector<int> rtsrc;
+        transpose_pattern(pt, rt, rtsrc);
+        rt.val.resize(rtsrc.size());
+        for (size_t q = 0; q < rtsrc.size(); ++q) rt.val[q] = pt.val[(size_t)rtsrc[q]];
+        lv.Rt.upload(rt, s);
+      }
       lv.P.upload_pattern(ppat, s);
       lv.R.upload_pattern(rpat, s);
       lv.r_src.upload(rsrc, s);
@@ -1239,6 +1252,7 @@ void amg_setup(Context& c) {
     DCsr& A = level_op(c, l);
     flag.upload(&zero, 1, s);
     const double omega = o.prolongator_damping / cur.lambda_max;
+    cur.omega = omega;
     if (cur.node_ok) {
       k::BlockProductArgs g = plan_args(cur, cur.plan_p, false, A, cur.Ptent, cur.P);
       g.dinv = cur.dinv.p;

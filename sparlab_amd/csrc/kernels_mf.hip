@@ -8,8 +8,9 @@
 // the Ke0 coefficients a wave needs are uniform across it.
 //
 // Epilogues fuse what the callers do with the product: y = Kx with the partial
-// sums of x.y (CG), r = b - Kx (residuals), the Chebyshev step, and the Lanczos
-// step on D^-1/2 K D^-1/2. Vectors are in the free-DOF layout; prescribed DOFs
+// sums of x.y (CG), r = b - Kx (residuals), the Chebyshev step, the Lanczos
+// step on D^-1/2 K D^-1/2, and the smoothing halves of the level-0 transfers
+// P = (I - omega D^-1 K) P-hat and R = P^T = P-hat^T (I - omega K D^-1). Vectors are in the free-DOF layout; prescribed DOFs
 // read as zero and are never written.
 #include <hip/hip_runtime.h>
 
@@ -46,9 +47,23 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return s;
 }
 
+// four components per node in LDS (the fourth is padding) so that one vector
+// read fetches a node's three
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
-  __shared__ T s_u[HZ * HY * HX * 3];
+  using V4 = typename Vec4<T>::type;
+  __shared__ V4 s_u[HZ * HY * HX];
   __shared__ T s_s[EZ * EY * EX];
   __shared__ double s_red[kThreads / 64];
   const int ntx = (a.X + TX - 1) / TX, nty = (a.Y + TY - 1) / TY, ntz = (a.Z + TZ - 1) / TZ;
@@ -66,7 +81,7 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
     if (gx >= 0 && gx < a.X && gy >= 0 && gy < a.Y && gz >= 0 && gz < a.Z) {
       const int* f = a.reduced + 3 * (((long long)gz * a.Y + gy) * a.X + gx);
       const int f0 = f[0], f1 = f[1], f2 = f[2];
-      if (MODE == 4) {
+      if (MODE == 4 || MODE == 5) {
         if (f0 >= 0) u0 = a.x[f0] * a.in_scale[f0];
         if (f1 >= 0) u1 = a.x[f1] * a.in_scale[f1];
         if (f2 >= 0) u2 = a.x[f2] * a.in_scale[f2];
@@ -76,9 +91,12 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
         if (f2 >= 0) u2 = a.x[f2];
       }
     }
-    s_u[3 * h] = u0;
-    s_u[3 * h + 1] = u1;
-    s_u[3 * h + 2] = u2;
+    V4 v;
+    v.x = u0;
+    v.y = u1;
+    v.z = u2;
+    v.w = 0;
+    s_u[h] = v;
   }
   const int nex = a.X - 1, ney = a.Y - 1, nez = a.Z - 1;  // elements per direction
   for (int h = threadIdx.x; h < EZ * EY * EX; h += kThreads) {
@@ -107,12 +125,21 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
       T t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
+        // fused multiply-adds: the element product differs from the assembled
+        // operator's sum in rounding only
+#pragma clang fp contract(fast)
         const int h = ((ez + corner_z(b)) * HY + (ey + corner_y(b))) * HX + (ex + corner_x(b));
-        const T v0 = s_u[3 * h], v1 = s_u[3 * h + 1], v2 = s_u[3 * h + 2];
+        const V4 u = s_u[h];
         const int cb = 3 * b;
-        t0 += kr[cb] * v0 + kr[cb + 1] * v1 + kr[cb + 2] * v2;
-        t1 += kr[24 + cb] * v0 + kr[25 + cb] * v1 + kr[26 + cb] * v2;
-        t2 += kr[48 + cb] * v0 + kr[49 + cb] * v1 + kr[50 + cb] * v2;
+        t0 += kr[cb] * u.x;
+        t0 += kr[cb + 1] * u.y;
+        t0 += kr[cb + 2] * u.z;
+        t1 += kr[24 + cb] * u.x;
+        t1 += kr[25 + cb] * u.y;
+        t1 += kr[26 + cb] * u.z;
+        t2 += kr[48 + cb] * u.x;
+        t2 += kr[49 + cb] * u.y;
+        t2 += kr[50 + cb] * u.z;
       }
       acc0 += se * t0;
       acc1 += se * t1;
@@ -142,10 +169,16 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
         a.y[fi] = (T)r;
         a.d_out[fi] = (T)dn;
         a.xs[fi] = (T)((double)a.xs[fi] + dn);
-      } else {
+      } else if (MODE == 4) {
         // Lanczos: w = s (K (s v)) - beta vprev
         const double beta = a.beta ? *a.beta : 0.0;
         a.y[fi] = (T)((double)y * (double)a.in_scale[fi] - beta * (double)a.vprev[fi]);
+      } else if (MODE == 5) {
+        // smoothed restriction, first half: y = (I - omega K D^-1) x
+        a.y[fi] = (T)((double)a.x[fi] - a.c1 * (double)y);
+      } else {
+        // smoothed prolongation, second half: xs += (I - omega D^-1 K) x
+        a.xs[fi] = (T)((double)a.xs[fi] + (double)a.x[fi] - a.c1 * (double)a.dinv[fi] * (double)y);
       }
     }
   }
@@ -174,7 +207,9 @@ void apply(const MfArgs<T>& a, int mode, hipStream_t s) {
     case 1: launch<T, 1>(a, s); break;
     case 2: launch<T, 2>(a, s); break;
     case 3: launch<T, 3>(a, s); break;
-    default: launch<T, 4>(a, s); break;
+    case 4: launch<T, 4>(a, s); break;
+    case 5: launch<T, 5>(a, s); break;
+    default: launch<T, 6>(a, s); break;
   }
 }
 

@@ -125,6 +125,9 @@ struct Level {
   bool has_next = false;
   DCsr A;                  // operator (levels > 0)
   DCsr Ptent, P, R, AP, RAW;
+  DCsr Rt;                 // P-hat^T: with the matrix-free operator, the level-0 transfers
+                           // apply P = (I - omega D^-1 K) P-hat and R = P-hat^T (I - omega K D^-1)
+  double omega = 0.0;      // prolongator damping / lambda_max
   DBuf<int> r_src;         // R.val[k] = P.val[r_src[k]]
   DBuf<int> sym_a, sym_b;  // next level's A from RAW (symmetrisation)
   // block-product plans (see kernels_amg.hip); node_ok when every operator of

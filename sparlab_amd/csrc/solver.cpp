@@ -70,6 +70,60 @@ void level_residual(Context& c, size_t l, const T* x, const T* b, T* r) {
   }
 }
 
+// Level-0 transfers with the matrix-free operator: the smoothed prolongator is
+// P = (I - omega D^-1 K) P-hat (Multigrid.cpp:686-701) and R = P^T =
+// P-hat^T (I - omega K D^-1), so both apply as one operator pass and one
+// product with the tentative prolongator (one block per node) instead of
+// reading the assembled P and R. scratch is a free level-0 vector.
+template <typename T>
+void restrict_mf(Context& c, const T* r, T* scratch, T* bc);
+template <typename T>
+void prolong_mf(Context& c, const T* xc, T* scratch, T* x);
+template <>
+void restrict_mf<double>(Context& c, const double* r, double* scratch, double* bc) {
+  Level& lv = c.levels[0];
+  kmf::MfArgs<double> a = mf_base<double>(c);
+  a.x = r;
+  a.in_scale = lv.dinv.p;
+  a.c1 = lv.omega;
+  a.y = scratch;
+  kmf::apply(a, 5, c.stream);
+  k::spmv(lv.Rt, scratch, bc, c.stream);
+}
+template <>
+void restrict_mf<float>(Context& c, const float* r, float* scratch, float* bc) {
+  Level& lv = c.levels[0];
+  kmf::MfArgs<float> a = mf_base<float>(c);
+  a.x = r;
+  a.in_scale = lv.dinv32.p;
+  a.c1 = lv.omega;
+  a.y = scratch;
+  kmf::apply(a, 5, c.stream);
+  k::spmv32(lv.Rt, scratch, bc, c.stream);
+}
+template <>
+void prolong_mf<double>(Context& c, const double* xc, double* scratch, double* x) {
+  Level& lv = c.levels[0];
+  k::spmv(lv.Ptent, xc, scratch, c.stream);
+  kmf::MfArgs<double> a = mf_base<double>(c);
+  a.x = scratch;
+  a.dinv = lv.dinv.p;
+  a.c1 = lv.omega;
+  a.xs = x;
+  kmf::apply(a, 6, c.stream);
+}
+template <>
+void prolong_mf<float>(Context& c, const float* xc, float* scratch, float* x) {
+  Level& lv = c.levels[0];
+  k::spmv32(lv.Ptent, xc, scratch, c.stream);
+  kmf::MfArgs<float> a = mf_base<float>(c);
+  a.x = scratch;
+  a.dinv = lv.dinv32.p;
+  a.c1 = lv.omega;
+  a.xs = x;
+  kmf::apply(a, 6, c.stream);
+}
+
 struct Cheb {
   double theta, delta, sigma;
 };
@@ -143,9 +197,9 @@ void cycle64(Context& c, size_t l, const double* b, double* x) {
     level_residual<double>(c, l, x, b, lv.res.p);
   }
   Level& nx = c.levels[l + 1];
-  k::spmv(lv.R, lv.res.p, nx.b.p, s);
+  if (use_mf(c, l)) restrict_mf<double>(c, lv.res.p, lv.d.p, nx.b.p); else k::spmv(lv.R, lv.res.p, nx.b.p, s);
   cycle64(c, l + 1, nx.b.p, nx.x.p);
-  k::spmv_add(lv.P, nx.x.p, x, s);
+  if (use_mf(c, l)) prolong_mf<double>(c, nx.x.p, lv.d.p, x); else k::spmv_add(lv.P, nx.x.p, x, s);
   // post-smoothing from the current x
   {
     Timed t(c, top ? "residual" : nullptr);
@@ -171,9 +225,9 @@ void cycle32(Context& c, size_t l, const float* b, float* x) {
     level_residual<float>(c, l, x, b, lv.res32.p);
   }
   Level& nx = c.levels[l + 1];
-  k::spmv32(lv.R, lv.res32.p, nx.b32.p, s);
+  if (use_mf(c, l)) restrict_mf<float>(c, lv.res32.p, lv.d32.p, nx.b32.p); else k::spmv32(lv.R, lv.res32.p, nx.b32.p, s);
   cycle32(c, l + 1, nx.b32.p, nx.x32.p);
-  k::spmv_add32(lv.P, nx.x32.p, x, s);
+  if (use_mf(c, l)) prolong_mf<float>(c, nx.x32.p, lv.d32.p, x); else k::spmv_add32(lv.P, nx.x32.p, x, s);
   {
     Timed t(c, top ? "residual" : nullptr);
     level_residual<float>(c, l, x, b, lv.res32.p);
@@ -338,7 +392,12 @@ void prepare_mixed(Context& c) {
     } else {
       A.val16.release();
     }
-    if (lv.has_next) {
+    if (lv.has_next && use_mf(c, l)) {
+      lv.Ptent.val32.alloc((size_t)lv.Ptent.nnz);
+      k::to_float(lv.Ptent.nnz, lv.Ptent.val.p, lv.Ptent.val32.p, s);
+      lv.Rt.val32.alloc((size_t)lv.Rt.nnz);
+      k::to_float(lv.Rt.nnz, lv.Rt.val.p, lv.Rt.val32.p, s);
+    } else if (lv.has_next) {
       lv.P.val32.alloc((size_t)lv.P.nnz);
       k::to_float(lv.P.nnz, lv.P.val.p, lv.P.val32.p, s);
       lv.R.val32.alloc((size_t)lv.R.nnz);

@@ -236,7 +236,8 @@ def test_matrix_free_and_assembled_solves_agree(ctx, monkeypatch):
     m = M.cantilever_3d(24, 12, 12)
     scale = 1e-9 + (1 - 1e-9) * M.seeded_density(m.nelem, seed=22) ** 3
     b = m.free_loads()[0]
-    out = {}
+    r = np.random.default_rng(23).standard_normal(len(b))
+    out, cyc = {}, {}
     for mode in ("mf", "csr"):
         if mode == "csr":
             monkeypatch.setenv("SPARLAB_NO_MATRIX_FREE", "1")
@@ -244,7 +245,11 @@ def test_matrix_free_and_assembled_solves_agree(ctx, monkeypatch):
         g.assemble(scale)
         assert g.kff_apply(np.ones(g.nfree))[1] == (mode == "mf")
         g.amg_setup()
+        # one V-cycle: level-0 smoothing and the transfers (matrix-free: P = (I - w D^-1 K) P-hat
+        # and R = P^T applied through the operator; assembled: the stored P and R)
+        cyc[mode] = g.amg_apply(r)
         out[mode] = g.solve(b, tol=1e-10)
+    np.testing.assert_allclose(cyc["mf"], cyc["csr"], rtol=0, atol=1e-11 * np.abs(cyc["csr"]).max())
     (xm, im, rm, cm), (xc, ic, rc, cc) = out["mf"], out["csr"]
     assert cm and cc and abs(im - ic) <= 1
     assert np.linalg.norm(xm - xc) <= 1e-9 * np.linalg.norm(xc)
