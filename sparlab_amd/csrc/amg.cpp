@@ -38,6 +38,8 @@ void DCsr::upload_pattern(const HCsr& h, hipStream_t s) {
   ptr.upload(h.ptr, s);
   col.upload(h.col, s);
   val.alloc((size_t)nnz);
+  cbase.release();
+  ncol.release();
 }
 
 namespace {
@@ -531,6 +533,26 @@ bool node_uniform(const HCsr& m, const std::vector<int>& node_ptr) {
     }
   }
   return ok;
+}
+
+// One column list per node for the V-cycle's products when the rows of every
+// node share their pattern (node_ptr covers all rows); otherwise none.
+void share_columns(DCsr& a, const HCsr& m, const std::vector<int>& node_ptr, hipStream_t s) {
+  a.cbase.release();
+  a.ncol.release();
+  if (node_ptr.size() < 2 || node_ptr.back() != m.rows || (int)node_ptr.size() - 1 == m.rows ||
+      !node_uniform(m, node_ptr))
+    return;
+  std::vector<int> cbase((size_t)m.rows), ncol;
+  ncol.reserve((size_t)m.nnz() / 2 + 1);
+  for (size_t nd = 0; nd + 1 < node_ptr.size(); ++nd) {
+    const int r0 = node_ptr[nd];
+    const int base = (int)ncol.size();
+    ncol.insert(ncol.end(), m.col.begin() + m.ptr[(size_t)r0], m.col.begin() + m.ptr[(size_t)r0 + 1]);
+    for (int r = r0; r < node_ptr[nd + 1]; ++r) cbase[(size_t)r] = base - m.ptr[(size_t)r];
+  }
+  a.cbase.upload(cbase, s);
+  a.ncol.upload(ncol, s);
 }
 
 // the columns of every row come in complete runs of the column nodes
@@ -1231,6 +1253,10 @@ void amg_setup(Context& c) {
       lv.sym_a.upload(ia, s);
       lv.sym_b.upload(ib, s);
       next.A.upload_pattern(spat, s);
+      share_columns(next.A, spat, next.node_ptr, s);
+      share_columns(lv.P, ppat, lv.node_ptr, s);
+      share_columns(lv.R, rpat, next.node_ptr, s);
+      if (l == 0 && !c.mf) share_columns(c.K, host_a, lv.node_ptr, s);
       std::vector<double> xyz;  // level-0 node coordinates (mesh node of each node block's first unknown)
       if (l == 0 && !c.layout_coords.empty() && c.layout_ndpn > 0) {
         const int nn = (int)lv.node_ptr.size() - 1;

@@ -80,8 +80,11 @@ __device__ __forceinline__ double row_dot(int s, int e, int lane, const int* __r
 }
 
 // MODE 0: y = A x; 1: y = b - A x; 2: y += A x; 3: y = A x and partial x.y
+// cbase (may be null): rows of one node share one column list, entry k of row
+// i reads its column at col[cbase[i] + k]
 template <int LANES, int MODE, typename VT, typename VAL = VT>
 __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                  const int* __restrict__ cbase,
                                                   const VAL* __restrict__ val, const VT* __restrict__ x,
                                                   const VT* __restrict__ b, VT* __restrict__ y,
                                                   double* __restrict__ partials) {
@@ -92,7 +95,8 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ 
   const int lane = threadIdx.x % LANES;
   double prod = 0.0;
   if (row < n) {
-    const double acc0 = row_dot<LANES, VAL, VT>(ptr[row], ptr[row + 1], lane, col, val, x);
+    const int* colr = cbase ? col + cbase[row] : col;
+    const double acc0 = row_dot<LANES, VAL, VT>(ptr[row], ptr[row + 1], lane, colr, val, x);
     const double acc = group_sum<LANES>(acc0);
     if (lane == 0) {
       if (MODE == 0) y[row] = (VT)acc;
@@ -157,6 +161,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv_staged(int n, const int* __rest
 //   r_i <- r_i - (A d_in)_i;  d_out_i = c1 d_in_i + c2 dinv_i r_i;  x_i += d_out_i
 template <int LANES, typename VT, typename VAL = VT>
 __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                       const int* __restrict__ cbase,
                                                        const VAL* __restrict__ val, const VT* __restrict__ dinv,
                                                        const VT* __restrict__ r_in, VT* __restrict__ r_out,
                                                        const VT* __restrict__ d_in, VT* __restrict__ d_out,
@@ -165,7 +170,8 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, const int* __restri
   const int row = blk * (kBlock / LANES) + threadIdx.x / LANES;
   const int lane = threadIdx.x % LANES;
   if (row >= n) return;
-  const double acc = group_sum<LANES>(row_dot<LANES, VAL, VT>(ptr[row], ptr[row + 1], lane, col, val, d_in));
+  const int* colr = cbase ? col + cbase[row] : col;
+  const double acc = group_sum<LANES>(row_dot<LANES, VAL, VT>(ptr[row], ptr[row + 1], lane, colr, val, d_in));
   if (lane == 0) {
     const double r = (double)r_in[row] - acc;
     const double dn = c1 * (double)d_in[row] + c2 * (double)dinv[row] * r;
@@ -597,10 +603,11 @@ void launch_spmv(const DCsr& A, const VAL* val, const VT* x, const VT* b, VT* y,
   const int lanes = lanes_for(A);
   const int nb = grid8(A.rows, kBlock / lanes);
   if (A.rows == 0) return;
+  const int* cols = A.cbase.p ? A.ncol.p : A.col.p;
   switch (lanes) {
-    case 16: k_spmv<16, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
-    case 8: k_spmv<8, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
-    default: k_spmv<4, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, x, b, y, partials); break;
+    case 16: k_spmv<16, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
+    case 8: k_spmv<8, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
+    default: k_spmv<4, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
   }
 }
 
@@ -654,10 +661,11 @@ static void cheb_step_t(const DCsr& A, const VAL* val, const VT* dinv, const VT*
   const int lanes = lanes_for(A);
   const int nb = grid8(A.rows, kBlock / lanes);
   if (A.rows == 0) return;
+  const int* cols = A.cbase.p ? A.ncol.p : A.col.p;
   switch (lanes) {
-    case 16: k_cheb_step<16, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
-    case 8: k_cheb_step<8, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
-    default: k_cheb_step<4, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, A.col.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    case 16: k_cheb_step<16, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    case 8: k_cheb_step<8, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    default: k_cheb_step<4, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
   }
 }
 void cheb_step(const DCsr& A, const double* dinv, const double* r_in, double* r_out, const double* d_in, double* d_out,

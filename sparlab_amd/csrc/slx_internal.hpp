@@ -81,6 +81,9 @@ struct DCsr {
   DBuf<double> val;
   DBuf<float> val32;
   DBuf<uint16_t> val16;  // bfloat16 copy (coarse operators of precision 2); FP32 products read it when present
+  // shared column lists (rows of one node have one pattern): the V-cycle's
+  // products read entry k of row i's column at ncol[cbase[i] + k]
+  DBuf<int> cbase, ncol;
   void upload_pattern(const HCsr& h, hipStream_t s);
   void upload(const HCsr& h, hipStream_t s) {
     upload_pattern(h, s);
