@@ -1244,6 +1244,7 @@ void amg_setup(Context& c) {
         rt.val.resize(rtsrc.size());
         for (size_t q = 0; q < rtsrc.size(); ++q) rt.val[q] = pt.val[(size_t)rtsrc[q]];
         lv.Rt.upload(rt, s);
+        share_columns(lv.Rt, rt, next.node_ptr, s);
       }
       lv.P.upload_pattern(ppat, s);
       lv.R.upload_pattern(rpat, s);
@@ -1256,6 +1257,7 @@ void amg_setup(Context& c) {
       share_columns(next.A, spat, next.node_ptr, s);
       share_columns(lv.P, ppat, lv.node_ptr, s);
       share_columns(lv.R, rpat, next.node_ptr, s);
+      share_columns(lv.Ptent, pt, lv.node_ptr, s);
       if (l == 0 && !c.mf) share_columns(c.K, host_a, lv.node_ptr, s);
       std::vector<double> xyz;  // level-0 node coordinates (mesh node of each node block's first unknown)
       if (l == 0 && !c.layout_coords.empty() && c.layout_ndpn > 0) {

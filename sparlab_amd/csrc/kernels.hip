@@ -255,6 +255,68 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, double* __restrict
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// End of the preconditioner step (src/fem/Multigrid.cpp:1205-1215), first
+// pass: z = the cycle's result (zc, widened; null: z already holds it) and the
+// block partials of r.z and, for flexible CG, of r.z_old with the previous z.
+template <typename VT>
+__global__ void __launch_bounds__(kBlock) k_precond_partials(int n, const VT* __restrict__ zc,
+                                                              const double* __restrict__ r, double* __restrict__ z,
+                                                              int flexible, double* __restrict__ partials) {
+  __shared__ double sh[kBlock / 64];
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double zo = z[i];
+    const double zn = zc ? (double)zc[i] : zo;
+    if (zc) z[i] = zn;
+    const double ri = r[i];
+    s1 += ri * zn;
+    s2 += ri * zo;
+  }
+  const double t1 = block_sum(s1, sh);
+  __syncthreads();
+  const double t2 = flexible ? block_sum(s2, sh) : 0.0;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = t1;
+    partials[gridDim.x + blockIdx.x] = t2;
+  }
+}
+
+// Second pass (one block): rz_new and r.z_old summed in a fixed order, then
+// beta = rz_new / rz_old (flexible: (rz_new - r.z_old) / rz_old) into
+// scal[i_beta], and scal[i_rzn] = scal[i_rz] = rz_new.
+__global__ void __launch_bounds__(1024) k_precond_finish(int m, const double* __restrict__ partials, double* scal,
+                                                         int i_rz, int i_rzn, int i_beta, int flexible) {
+  __shared__ double sh[16];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) v += partials[i];
+  const double rzn = block_sum(v, sh);
+  __syncthreads();
+  double w = 0.0;
+  if (flexible)
+    for (int i = threadIdx.x; i < m; i += blockDim.x) w += partials[m + i];
+  const double rzx = flexible ? block_sum(w, sh) : 0.0;
+  if (threadIdx.x == 0) {
+    const double rz = scal[i_rz];
+    scal[i_beta] = flexible ? (rzn - rzx) / rz : rzn / rz;
+    scal[i_rzn] = rzn;
+    scal[i_rz] = rzn;
+  }
+}
+
+// Start of the FP32 cycle: b = (float) r, then the zero-guess smoother start
+// x = d = dinv b / theta.
+__global__ void k_cheb_first_zero_from64(int n, const float* __restrict__ dinv, const double* __restrict__ r,
+                                         double theta, float* __restrict__ b, float* __restrict__ x,
+                                         float* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float bi = (float)r[i];
+  b[i] = bi;
+  const double di = (double)dinv[i] * (double)bi / theta;
+  d[i] = (float)di;
+  x[i] = (float)(0.0 + di);
+}
+
 // p = z + beta p with beta = rz_new / rz_old (src/fem/Multigrid.cpp:1209-1215)
 __global__ void k_pcg_direction(int n, double* __restrict__ p, const double* __restrict__ z,
                                 const double* __restrict__ scal, int i_new, int i_old) {
@@ -713,6 +775,16 @@ void pcg_update(long long n, double* x, double* r, const double* p, const double
   const int nb = reduce_blocks(n);
   k_pcg_update<<<nb, kBlock, 0, s>>>((int)n, x, r, p, q, scal, i_rz, i_pq, partials);
   k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+}
+void precond_tail(long long n, const float* zc, const double* r, double* z, double* scal, int i_rz, int i_rzn,
+                  int i_beta, bool flexible, double* partials, hipStream_t s) {
+  const int nb = reduce_blocks(n);
+  k_precond_partials<float><<<nb, kBlock, 0, s>>>((int)n, zc, r, z, flexible ? 1 : 0, partials);
+  k_precond_finish<<<1, 1024, 0, s>>>(nb, partials, scal, i_rz, i_rzn, i_beta, flexible ? 1 : 0);
+}
+void cheb_first_zero_from64(int n, const float* dinv, const double* r, double theta, float* b, float* x, float* d,
+                            hipStream_t s) {
+  if (n) k_cheb_first_zero_from64<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, r, theta, b, x, d);
 }
 void pcg_direction(long long n, double* p, const double* z, const double* scal, int i_new, int i_old, hipStream_t s) {
   k_pcg_direction<<<reduce_blocks(n), kBlock, 0, s>>>((int)n, p, z, scal, i_new, i_old);

@@ -40,6 +40,13 @@ void pcg_update(long long n, double* x, double* r, const double* p, const double
                 int i_pq, double* partials, double* out, hipStream_t s);
 // p = z + beta p, beta = scal[i_new] / scal[i_old] (or scal[i_new] when i_old < 0)
 void pcg_direction(long long n, double* p, const double* z, const double* scal, int i_new, int i_old, hipStream_t s);
+// z = zc (null: z is already the cycle's result), r.z (and r.z_old when
+// flexible), beta into scal[i_beta], scal[i_rzn] = scal[i_rz] = r.z;
+// partials: 2 per block (reduce_blocks(n))
+void precond_tail(long long n, const float* zc, const double* r, double* z, double* scal, int i_rz, int i_rzn,
+                  int i_beta, bool flexible, double* partials, hipStream_t s);
+void cheb_first_zero_from64(int n, const float* dinv, const double* r, double theta, float* b, float* x, float* d,
+                            hipStream_t s);
 void copy_scalar(double* scal, int from, int to, hipStream_t s);
 void flexible_beta(double* scal, int i_new, int i_cross, int i_old, int i_out, hipStream_t s);
 void to_float(long long n, const double* in, float* out, hipStream_t s);

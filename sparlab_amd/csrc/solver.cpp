@@ -209,7 +209,8 @@ void cycle64(Context& c, size_t l, const double* b, double* x) {
   cheb_steps<false>(c, l, lv.res.p, x, top);
 }
 
-void cycle32(Context& c, size_t l, const float* b, float* x) {
+// started: x and d already hold the zero-guess smoother start
+void cycle32(Context& c, size_t l, const float* b, float* x, bool started = false) {
   Level& lv = c.levels[l];
   hipStream_t s = c.stream;
   if (l + 1 == c.levels.size()) {
@@ -218,7 +219,7 @@ void cycle32(Context& c, size_t l, const float* b, float* x) {
   }
   const bool top = l == 0;
   const Cheb cp = cheb_params(c, lv);
-  k::cheb_first_zero32(lv.n, lv.dinv32.p, b, cp.theta, x, lv.d32.p, s);
+  if (!started) k::cheb_first_zero32(lv.n, lv.dinv32.p, b, cp.theta, x, lv.d32.p, s);
   cheb_steps<true>(c, l, b, x, top);
   {
     Timed t(c, top ? "residual" : nullptr);
@@ -236,13 +237,6 @@ void cycle32(Context& c, size_t l, const float* b, float* x) {
   cheb_steps<true>(c, l, lv.res32.p, x, top);
 }
 
-void read_scalars(Context& c, int from, int count) {
-  c.h_scal.resize(16);
-  double* hb = c.host_buffer(16);
-  SLX_HIP(hipMemcpyAsync(hb + from, c.scal.p + from, sizeof(double) * count, hipMemcpyDeviceToHost, c.stream));
-  SLX_HIP(hipStreamSynchronize(c.stream));
-  for (int q = from; q < from + count; ++q) c.h_scal[(size_t)q] = hb[q];
-}
 
 }  // namespace
 
@@ -415,17 +409,31 @@ void prepare_mixed(Context& c) {
   k::to_float((long long)c.coarse_inv.n, c.coarse_inv.p, c.coarse_inv32.p, s);
 }
 
-void vcycle(Context& c, const double* r, double* z) {
+// One V-cycle on r. FP64 cycle: the result goes to z, returns null. FP32
+// cycle: the result stays in level 0's FP32 iterate, which is returned.
+const float* vcycle_raw(Context& c, const double* r, double* z) {
   if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: apply() called before setup()");
   Level& top = c.levels[0];
-  const size_t n = (size_t)top.n;
+  const int n = top.n;
   if (c.amg_opt.precision >= 1) {
-    k::to_float((long long)n, r, top.b32.p, c.stream);
-    cycle32(c, 0, top.b32.p, top.x32.p);
-    k::to_double((long long)n, top.x32.p, z, c.stream);
-    return;
+    if (c.levels.size() > 1) {
+      // b = (float) r and the smoother's zero-guess start in one pass
+      k::cheb_first_zero_from64(n, top.dinv32.p, r, cheb_params(c, top).theta, top.b32.p, top.x32.p, top.d32.p,
+                                c.stream);
+      cycle32(c, 0, top.b32.p, top.x32.p, true);
+    } else {
+      k::to_float((long long)n, r, top.b32.p, c.stream);
+      cycle32(c, 0, top.b32.p, top.x32.p);
+    }
+    return top.x32.p;
   }
   cycle64(c, 0, r, z);
+  return nullptr;
+}
+
+void vcycle(Context& c, const double* r, double* z) {
+  const float* zc = vcycle_raw(c, r, z);
+  if (zc) k::to_double((long long)c.levels[0].n, zc, z, c.stream);
 }
 
 bool graphs_enabled() {
@@ -468,9 +476,9 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
   int total = 0;
   bool recursive_converged = false;
   double best_true = std::numeric_limits<double>::max();
-  const int nb = op_dot_blocks(c);
   double* x = c.xpcg.p;
   k::copy(n, x_out, x, s);
+  const int nb = op_dot_blocks(c);
   // the three parts of one iteration (Multigrid.cpp:1184-1216)
   auto body_spmv = [&] { op_apply_dot(c, c.p.p, c.q.p, c.partials.p); };
   auto body_update = [&] {
@@ -478,22 +486,20 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
     k::pcg_update(n, x, c.r.p, c.p.p, c.q.p, c.scal.p, S_RZ, S_PQ, c.partials.p, c.scal.p + S_RR, s);
   };
   auto body_precond = [&] {
-    if (flexible) {
-      // beta = r.(z - z_old) / rz_old (Polak-Ribiere)
-      k::copy(n, c.z.p, c.zold.p, s);
-      vcycle(c, c.r.p, c.z.p);
-      k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZN, s);
-      k::dot(n, c.r.p, c.zold.p, c.partials.p, c.scal.p + S_RZX, s);
-      k::flexible_beta(c.scal.p, S_RZN, S_RZX, S_RZ, S_TMP, s);
-      k::pcg_direction(n, c.p.p, c.z.p, c.scal.p, S_TMP, -1, s);
-    } else {
-      vcycle(c, c.r.p, c.z.p);
-      k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZN, s);
-      k::pcg_direction(n, c.p.p, c.z.p, c.scal.p, S_RZN, S_RZ, s);
-    }
-    k::copy_scalar(c.scal.p, S_RZN, S_RZ, s);
+    // z = M r; rz_new = r.z; beta = rz_new / rz_old, or for flexible CG
+    // (Polak-Ribiere) r.(z - z_old) / rz_old; p = z + beta p
+    const float* zc = vcycle_raw(c, c.r.p, c.z.p);
+    k::precond_tail(n, zc, c.r.p, c.z.p, c.scal.p, S_RZ, S_RZN, S_TMP, flexible, c.partials.p, s);
+    k::pcg_direction(n, c.p.p, c.z.p, c.scal.p, S_TMP, -1, s);
   };
   const bool graphs = graphs_enabled();
+  c.h_scal.resize(16);
+  hipEvent_t ev_scal = c.take_event();
+  struct EventBack {
+    Context& c;
+    hipEvent_t e;
+    ~EventBack() { c.event_pool.push_back(e); }
+  } event_back{c, ev_scal};
   if (graphs && !c.g_spmv) {
     c.g_spmv = capture_graph(c, body_spmv);
     c.g_update = capture_graph(c, body_update);
@@ -511,6 +517,7 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
     k::copy(n, c.z.p, c.p.p, s);
     k::dot(n, c.r.p, c.z.p, c.partials.p, c.scal.p + S_RZ, s);
     bool converged = false;
+    double* hb = c.host_buffer(16);
     while (total < max_iterations) {
       {
         Timed t(c, "spmv");
@@ -518,7 +525,22 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
       }
       if (graphs) SLX_HIP(hipGraphLaunch(c.g_update, s)); else body_update();
       ++total;
-      read_scalars(c, S_PQ, 2);
+      // p.q and r.r to the host; the next preconditioner step is queued before
+      // the host waits, so the device does not idle during the test (after
+      // the last iteration it only touches z, p and the step scalars, which
+      // every restart recomputes)
+      SLX_HIP(hipMemcpyAsync(hb + S_PQ, c.scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+      SLX_HIP(hipEventRecord(ev_scal, s));
+      if (graphs) SLX_HIP(hipGraphLaunch(c.g_precond, s)); else body_precond();
+      {
+        // poll: a blocking wait returns later than the copy completes
+        hipError_t q;
+        while ((q = hipEventQuery(ev_scal)) == hipErrorNotReady) {
+        }
+        SLX_HIP(q);
+      }
+      c.h_scal[S_PQ] = hb[S_PQ];
+      c.h_scal[S_RR] = hb[S_RR];
       const double pq = c.h_scal[S_PQ];
       if (!(pq > 0.0) || !std::isfinite(pq)) {
         std::ostringstream os;
@@ -532,7 +554,6 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
         converged = true;
         break;
       }
-      if (graphs) SLX_HIP(hipGraphLaunch(c.g_precond, s)); else body_precond();
     }
     op_residual(c, x, b, c.r.p);
     res.relative_residual = std::sqrt(device_dot(c, n, c.r.p, c.r.p)) / bnorm;
