@@ -135,8 +135,8 @@ def rooflines(g, m, timers):
     """Roofline of each kernel timed with HIP events over the timed region, largest total first.
 
     galerkin_ap / galerkin_raw: the level-0 row products A P and R (A P), HBM-bound;
-    algorithmic bytes = 8 (nnz A + nnz B + nnz C) values + 12 per run (descriptor and
-    entry offset) + 4 per entry (block positions) + 12 per output node (order, run
+    algorithmic bytes = 8 (nnz A + nnz B + nnz C) values + 16 per run (descriptor and
+    entry range) + 4 per entry (block positions) + 12 per output node (order, run
     pointer, row pointer) + 4 per output row (A row pointer).
     spmv: the matrix-free K_ff product of CG (with x.y partials), FP64 compute-bound;
     1200 flop per element (Ke0 u_e: 576 FMA, the factor: 24 more).
@@ -165,7 +165,7 @@ def rooflines(g, m, timers):
             a_nnz, b_nnz, c_nnz, rows = sizes["R"], sizes["AP"], sizes["RAW"][2], sizes["RAW"][0]
             nodes, runs, entries = sizes["plan_raw"]
             label = "k_row_product<6,3> (level-0 R (A P), FP64)"
-        nbytes = 8 * (a_nnz + b_nnz + c_nnz) + 12 * runs + 4 * entries + 12 * nodes + 4 * rows
+        nbytes = 8 * (a_nnz + b_nnz + c_nnz) + 16 * runs + 4 * entries + 12 * nodes + 4 * rows
         ach = nbytes / (avg * 1e-3) / 1e9
         out.append({"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                     "traffic": None, "kernel": label, "bytes_per_launch": nbytes, "avg_ms": avg,

@@ -37,7 +37,7 @@ void DCsr::upload_pattern(const HCsr& h, hipStream_t s) {
   for (int i = 0; i < h.rows; ++i) max_row = std::max(max_row, h.ptr[(size_t)i + 1] - h.ptr[(size_t)i]);
   ptr.upload(h.ptr, s);
   col.upload(h.col, s);
-  val.alloc((size_t)nnz);
+  val.alloc((size_t)nnz + 8);  // padding: the row products read up to 8 values from a run's start
   cbase.release();
   ncol.release();
 }
@@ -762,7 +762,7 @@ bool build_row_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& c
   run_entry[(size_t)nruns] = (int)total;
   for (int g = 0; g < naggs; ++g) wmax = std::max(wmax, agg_off[(size_t)g + 1] - agg_off[(size_t)g]);
   if (dmax > 6 || jmax > 6 || wmax > 7 || lenc_max > 0xffff) return false;
-  std::vector<uint2> run((size_t)nruns);
+  std::vector<uint4> run((size_t)nruns);
   std::vector<unsigned> entry((size_t)total);
   std::atomic<bool> ok{true};
 #pragma omp parallel
@@ -779,7 +779,8 @@ bool build_row_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& c
           ok.store(false, std::memory_order_relaxed);
           continue;
         }
-        run[(size_t)t] = make_uint2((unsigned)bptr[(size_t)brow], ((unsigned)koff << 16) | ((unsigned)lb << 3) | (unsigned)dj);
+        run[(size_t)t] = make_uint4((unsigned)bptr[(size_t)brow], ((unsigned)koff << 16) | ((unsigned)lb << 3) | (unsigned)dj,
+                                    (unsigned)run_entry[(size_t)t], (unsigned)run_entry[(size_t)t + 1]);
         int e = run_entry[(size_t)t];
         for (int u = bbp[(size_t)J]; u < bbp[(size_t)J + 1]; ++u, ++e) {
           const int g = bba[(size_t)u];
@@ -824,7 +825,6 @@ bool build_row_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& c
   plan.entries = total;
   plan.order.upload(ord, s);
   plan.run_ptr.upload(run_ptr, s);
-  plan.run_entry.upload(run_entry, s);
   plan.run.upload(run, s);
   plan.entry.upload(entry, s);
   return true;
@@ -952,7 +952,6 @@ k::RowProductArgs row_args(const Level& lv, const RowPlan& plan, bool a_is_r, co
   g.aval = A.val.p;
   g.run_ptr = plan.run_ptr.p;
   g.run = plan.run.p;
-  g.run_entry = plan.run_entry.p;
   g.entry = plan.entry.p;
   g.bval = B.val.p;
   g.cptr = C.ptr.p;
@@ -1108,7 +1107,7 @@ void autotune(Context& c) {
     if (m->rows == 0 || (f32 && !m->val32.p)) continue;
     float best = std::numeric_limits<float>::max();
     int pick = 0;
-    for (int lanes : {4, 8, 16}) {
+    for (int lanes : {1, 4, 8, 16, 64}) {
       m->lanes = lanes;
       auto run = [&] {
         if (f32) k::spmv32(*m, x32.p, y32.p, s); else k::spmv(*m, x.p, y.p, s);

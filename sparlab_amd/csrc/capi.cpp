@@ -581,7 +581,7 @@ slx_status slx_factorize_csr(slx_context* ctx, int n, const int* ptr, const int*
         if (h.col[(size_t)k2] <= h.col[(size_t)k2 - 1]) fail(SLX_SOLVER_ERROR, "CSR rows must be sorted by column without duplicates");
     const bool same_pattern = c.K.rows == n && c.K.nnz == h.nnz() && c.layout_unknowns.size() == (size_t)n &&
                               std::equal(unknowns, unknowns + n, c.layout_unknowns.begin());
-    if (same_pattern) c.K.val.upload(h.val, c.stream); else c.K.upload(h, c.stream);
+    if (same_pattern) c.K.upload_values(h.val.data(), c.stream); else c.K.upload(h, c.stream);
     c.mf = false;  // a caller's matrix: products use it as given
     c.layout_dim = dim;
     c.layout_ndpn = ndpn > 0 ? ndpn : dim;

@@ -655,9 +655,10 @@ int lanes_for(const DCsr& A) {
     const char* e = std::getenv("SPARLAB_SPMV_LANES");
     return e ? std::atoi(e) : 0;
   }();
-  if (forced == 4 || forced == 8 || forced == 16) return forced;
-  if (A.lanes == 4 || A.lanes == 8 || A.lanes == 16) return A.lanes;
-  return avg > 48 ? 16 : avg > 12 ? 8 : 4;
+  auto valid = [](int l) { return l == 1 || l == 4 || l == 8 || l == 16 || l == 64; };
+  if (valid(forced)) return forced;
+  if (valid(A.lanes)) return A.lanes;
+  return avg > 256 ? 64 : avg > 48 ? 16 : avg > 12 ? 8 : 4;
 }
 
 template <int MODE, typename VT, typename VAL = VT>
@@ -667,8 +668,10 @@ void launch_spmv(const DCsr& A, const VAL* val, const VT* x, const VT* b, VT* y,
   if (A.rows == 0) return;
   const int* cols = A.cbase.p ? A.ncol.p : A.col.p;
   switch (lanes) {
+    case 64: k_spmv<64, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
     case 16: k_spmv<16, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
     case 8: k_spmv<8, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
+    case 1: k_spmv<1, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
     default: k_spmv<4, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
   }
 }
@@ -725,8 +728,10 @@ static void cheb_step_t(const DCsr& A, const VAL* val, const VT* dinv, const VT*
   if (A.rows == 0) return;
   const int* cols = A.cbase.p ? A.ncol.p : A.col.p;
   switch (lanes) {
+    case 64: k_cheb_step<64, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
     case 16: k_cheb_step<16, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
     case 8: k_cheb_step<8, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    case 1: k_cheb_step<1, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
     default: k_cheb_step<4, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
   }
 }

@@ -226,15 +226,15 @@ __global__ void __launch_bounds__(256) k_block_product(BlockProductArgs g) {
 // With PRE the node's entries are staged in LDS first.
 // ---------------------------------------------------------------------------
 struct Step {
-  int t, eb, e1;  // run, first entry of the pass, end of the run's entries
-  uint2 rd;       // the run's descriptor
+  int t, eb;  // run, first entry of the pass
+  uint4 rd;   // the run's descriptor (entries [rd.z, rd.w))
 };
 
 template <int DMAX, int JMAX, bool PRE>
 __global__ void __launch_bounds__(256) k_row_product(
     int nnodes, int wpb, int lenc_max, int emax, int wmax, const int* __restrict__ order,
     const int* __restrict__ node_ptr, const int* __restrict__ aptr, const double* __restrict__ aval,
-    const int* __restrict__ run_ptr, const uint2* __restrict__ run, const int* __restrict__ run_entry,
+    const int* __restrict__ run_ptr, const uint4* __restrict__ run,
     const unsigned* __restrict__ entry, const double* __restrict__ bval, const int* __restrict__ cptr,
     double* __restrict__ cval) {
   extern __shared__ double lds[];
@@ -256,17 +256,18 @@ __global__ void __launch_bounds__(256) k_row_product(
   for (int r = 0; r < DMAX; ++r) abase[r] = aptr[r0 + (r < d ? r : d - 1)];
   const int t0 = run_ptr[I], t1 = run_ptr[I + 1];
   if (t0 < t1) {
-    const int E0 = run_entry[t0];
+    const int E0 = (int)run[t0].z;
     if (PRE) {
-      const int E1 = run_entry[t1];
+      const int E1 = (int)run[t1 - 1].w;
       for (int e = E0 + lane; e < E1; e += 64) ent[e - E0] = entry[e];
     }
     const int per = 64 / wmax;
     const int ul = lane / wmax, tt = lane - ul * wmax;
     // one step = one pass of the lanes over (up to per) blocks of run t from
     // entry eb; fetch() gives the lane's C position (or -1) and its B values
-    auto fetch = [&](uint2 rd, int eb, int e1, double(&bv)[JMAX]) -> int {
+    auto fetch = [&](const uint4& rd, int eb, double(&bv)[JMAX]) -> int {
       const int lb = (int)((rd.y >> 3) & 0x1fffu), dj = (int)(rd.y & 7u);
+      const int e1 = (int)rd.w;
       const int e = min(eb + ul, e1 - 1);
       const unsigned en = PRE ? ent[e - E0] : entry[e];
       const int w = (int)(en & 7u), bpos = (int)((en >> 3) & 0x1fffu), cpos = (int)(en >> 16);
@@ -276,26 +277,29 @@ __global__ void __launch_bounds__(256) k_row_product(
       for (int m = 0; m < JMAX; ++m) bv[m] = bval[idx + (unsigned)(min(m, dj - 1) * lb)];
       return live ? cpos + tt : -1;
     };
-    auto load_a = [&](uint2 rd, double(&av)[DMAX][JMAX]) {
-      const int koff = (int)(rd.y >> 16), dj = (int)(rd.y & 7u);
+    // one wide scalar load per row: JMAX values from the run's start (the
+    // value arrays carry 8 doubles of padding past the last row)
+    auto load_a = [&](const uint4& rd, double(&av)[DMAX][JMAX]) {
+      const int koff = (int)(rd.y >> 16);
+      struct Run {
+        double v[JMAX == 3 ? 4 : 8];
+      };
 #pragma unroll
-      for (int r = 0; r < DMAX; ++r)
+      for (int r = 0; r < DMAX; ++r) {
+        Run t;
+        __builtin_memcpy(&t, aval + abase[r] + koff, sizeof(Run));
 #pragma unroll
-        for (int m = 0; m < JMAX; ++m) av[r][m] = aval[abase[r] + koff + min(m, dj - 1)];
+        for (int m = 0; m < JMAX; ++m) av[r][m] = t.v[m];
+      }
     };
     // the step after st (st.t == t1: past the last step, valid addresses, no lane live)
     auto next = [&](const Step& st) -> Step {
       Step n = st;
       n.eb = st.eb + per;
-      if (n.eb >= st.e1) {
+      if (n.eb >= (int)st.rd.w) {
         n.t = st.t + 1;
-        n.eb = st.e1;
-        if (n.t < t1) {
-          n.e1 = run_entry[n.t + 1];
-          n.rd = run[n.t];
-        } else {
-          n.eb = st.e1 - 1;
-        }
+        n.eb = (int)st.rd.w;
+        if (n.t < t1) n.rd = run[n.t]; else n.eb = (int)st.rd.w - 1;
       }
       return n;
     };
@@ -314,19 +318,19 @@ __global__ void __launch_bounds__(256) k_row_product(
     };
     // two register sets in turn: the B values of the next step load while
     // the current step is added
-    Step c{t0, E0, run_entry[t0 + 1], run[t0]};
+    Step c{t0, E0, run[t0]};
     double b0[JMAX], b1[JMAX], a[DMAX][JMAX];
-    int p0 = fetch(c.rd, c.eb, c.e1, b0);
+    int p0 = fetch(c.rd, c.eb, b0);
     load_a(c.rd, a);
     while (true) {
       const Step n = next(c);
-      int p1 = fetch(n.rd, n.eb, n.e1, b1);
+      int p1 = fetch(n.rd, n.eb, b1);
       if (n.t >= t1) p1 = -1;
       add(c, p0, b0, a);
       if (n.t >= t1) break;
       if (n.t != c.t) load_a(n.rd, a);
       const Step m = next(n);
-      p0 = fetch(m.rd, m.eb, m.e1, b0);
+      p0 = fetch(m.rd, m.eb, b0);
       if (m.t >= t1) p0 = -1;
       add(n, p1, b1, a);
       if (m.t >= t1) break;
@@ -419,7 +423,7 @@ void row_product(const RowProductArgs& a, hipStream_t s) {
   const int grid = (int)(((long long)a.nnodes + a.wpb - 1) / a.wpb + 7) / 8 * 8;
 #define SLX_ROW_PRODUCT(D, J, P)                                                                                  \
   k_row_product<D, J, P><<<grid, 64 * a.wpb, lds, s>>>(a.nnodes, a.wpb, a.lenc_max, a.emax, a.wmax, a.order,     \
-                                                       a.node_ptr, a.aptr, a.aval, a.run_ptr, a.run, a.run_entry, \
+                                                       a.node_ptr, a.aptr, a.aval, a.run_ptr, a.run,              \
                                                        a.entry, a.bval, a.cptr, a.cval)
   if (dm == 3) {
     if (a.pre) SLX_ROW_PRODUCT(3, 3, true); else SLX_ROW_PRODUCT(3, 3, false);

@@ -85,9 +85,16 @@ struct DCsr {
   // products read entry k of row i's column at ncol[cbase[i] + k]
   DBuf<int> cbase, ncol;
   void upload_pattern(const HCsr& h, hipStream_t s);
+  // values keep 8 doubles of padding past the last row (the row products read
+  // up to 8 values from the start of a run)
+  void upload_values(const double* h, hipStream_t s) {
+    if (val.n < (size_t)nnz + 8) val.alloc((size_t)nnz + 8);
+    if (nnz) SLX_HIP(hipMemcpyAsync(val.p, h, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, s));
+    SLX_HIP(hipStreamSynchronize(s));
+  }
   void upload(const HCsr& h, hipStream_t s) {
     upload_pattern(h, s);
-    val.upload(h.val, s);
+    upload_values(h.val.data(), s);
   }
 };
 
@@ -109,8 +116,8 @@ struct RowPlan {
   int dmax = 0, jmax = 0, wmax = 0, lenc_max = 0, emax = 0, wpb = 0;
   bool pre = false;
   long long runs = 0, entries = 0;
-  DBuf<int> order, run_ptr, run_entry;
-  DBuf<uint2> run;
+  DBuf<int> order, run_ptr;
+  DBuf<uint4> run;
   DBuf<unsigned> entry;
 };
 

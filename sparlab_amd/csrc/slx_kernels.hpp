@@ -117,9 +117,9 @@ struct RowProductArgs {
   const double* aval;
   const int* run_ptr;         // runs of C node I: [run_ptr[I], run_ptr[I+1])
   // run: x = index in B's values of the run's first B row,
-  // y = (offset of the run in A's row << 16) | (B row length << 3) | B rows
-  const uint2* run;
-  const int* run_entry;       // entries of run t: [run_entry[t], run_entry[t+1])
+  // y = (offset of the run in A's row << 16) | (B row length << 3) | B rows,
+  // entries [z, w)
+  const uint4* run;
   // entry: one B block of the run's B node: (position of its aggregate block
   // in C's row << 16) | (position in B's row << 3) | width
   const unsigned* entry;
