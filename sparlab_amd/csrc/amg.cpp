@@ -742,7 +742,7 @@ bool build_row_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& c
   const long long nruns = run_ptr[(size_t)nn];
   std::vector<int> run_entry((size_t)nruns + 1, 0);
   long long total = 0;
-  int dmax = 1, jmax = 1, wmax = 1, lenc_max = 1, emax = 0;
+  int dmax = 1, jmax = 1, wmax = 1, lenc_max = 1, emax = 0, lena_max = 1;
   for (int I = 0; I < nn; ++I) {
     const int r0 = cnode_ptr[(size_t)I];
     dmax = std::max(dmax, cnode_ptr[(size_t)I + 1] - r0);
@@ -752,6 +752,7 @@ bool build_row_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& c
       const int J = runs[(size_t)4 * t + 3];
       const int nb = bbp[(size_t)J + 1] - bbp[(size_t)J];
       if (nb < 1) return false;
+      lena_max = std::max(lena_max, runs[(size_t)4 * t] + runs[(size_t)4 * t + 2]);
       jmax = std::max(jmax, runs[(size_t)4 * t + 2]);
       run_entry[(size_t)t] = (int)total;
       total += nb;
@@ -801,8 +802,9 @@ bool build_row_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& c
   // small enough, the node's entries
   const int dk = dmax <= 3 && jmax <= 3 ? 3 : 6;
   const size_t budget = 16 * 1024;
-  const bool pre = k::row_product_lds(dk, lenc_max, emax) <= budget;
-  const size_t wave_lds = k::row_product_lds(dk, lenc_max, pre ? emax : 0);
+  const bool sa = dk == 3 && k::row_product_lds(dk, lenc_max, 0, lena_max) <= budget;
+  const bool pre = k::row_product_lds(dk, lenc_max, emax, sa ? lena_max : 0) <= budget;
+  const size_t wave_lds = k::row_product_lds(dk, lenc_max, pre ? emax : 0, sa ? lena_max : 0);
   int wpb = 4;
   while (wpb > 1 && wave_lds * (size_t)wpb > 64 * 1024) wpb /= 2;
   if (wave_lds * (size_t)wpb > 64 * 1024) return false;
@@ -821,6 +823,8 @@ bool build_row_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& c
   plan.emax = emax;
   plan.wpb = wpb;
   plan.pre = pre;
+  plan.sa = sa;
+  plan.lena_max = lena_max;
   plan.runs = nruns;
   plan.entries = total;
   plan.order.upload(ord, s);
@@ -946,6 +950,8 @@ k::RowProductArgs row_args(const Level& lv, const RowPlan& plan, bool a_is_r, co
   g.emax = plan.emax;
   g.wpb = plan.wpb;
   g.pre = plan.pre;
+  g.sa = plan.sa;
+  g.lena_max = plan.lena_max;
   g.order = plan.order.p;
   g.node_ptr = a_is_r ? lv.c_node_ptr.p : lv.f_node_ptr.p;
   g.aptr = A.ptr.p;

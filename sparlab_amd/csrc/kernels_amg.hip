@@ -223,16 +223,17 @@ __global__ void __launch_bounds__(256) k_block_product(BlockProductArgs g) {
 // accumulator, so every entry of C sums its products in the order of A's row,
 // the order of the CPU restatement (Multigrid.cpp:199-225). The next step's B
 // values are loaded while the current step is added.
-// With PRE the node's entries are staged in LDS first.
+// With PRE the node's entries, with SA the node's A rows, are staged in LDS
+// first (SA: A's values come from LDS instead of scalar loads).
 // ---------------------------------------------------------------------------
 struct Step {
   int t, eb;  // run, first entry of the pass
   uint4 rd;   // the run's descriptor (entries [rd.z, rd.w))
 };
 
-template <int DMAX, int JMAX, bool PRE>
+template <int DMAX, int JMAX, bool PRE, bool SA>
 __global__ void __launch_bounds__(256) k_row_product(
-    int nnodes, int wpb, int lenc_max, int emax, int wmax, const int* __restrict__ order,
+    int nnodes, int wpb, int lenc_max, int emax, int lena_max, int wmax, const int* __restrict__ order,
     const int* __restrict__ node_ptr, const int* __restrict__ aptr, const double* __restrict__ aval,
     const int* __restrict__ run_ptr, const uint4* __restrict__ run,
     const unsigned* __restrict__ entry, const double* __restrict__ bval, const int* __restrict__ cptr,
@@ -244,8 +245,10 @@ __global__ void __launch_bounds__(256) k_row_product(
   const int blk = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
   const int slot = blk * wpb + wid;
   if (slot >= nnodes) return;
-  double* acc = lds + (size_t)wid * ((size_t)DMAX * lenc_max + (PRE ? (size_t)(emax + 1) / 2 : 0));
-  unsigned* ent = reinterpret_cast<unsigned*>(acc + (size_t)DMAX * lenc_max);
+  double* acc = lds + (size_t)wid * ((size_t)DMAX * lenc_max + (SA ? (size_t)DMAX * lena_max : 0) +
+                                     (PRE ? (size_t)(emax + 1) / 2 : 0));
+  double* sa = acc + (size_t)DMAX * lenc_max;
+  unsigned* ent = reinterpret_cast<unsigned*>(sa + (SA ? (size_t)DMAX * lena_max : 0));
   const int I = __builtin_amdgcn_readfirstlane(order[slot]);
   const int r0 = node_ptr[I];
   const int d = min(node_ptr[I + 1] - r0, DMAX);
@@ -257,6 +260,9 @@ __global__ void __launch_bounds__(256) k_row_product(
   const int t0 = run_ptr[I], t1 = run_ptr[I + 1];
   if (t0 < t1) {
     const int E0 = (int)run[t0].z;
+    const int lena = aptr[r0 + 1] - aptr[r0];
+    if (SA)  // the node's rows are consecutive in A's values
+      for (int p = lane; p < d * lena; p += 64) sa[p] = aval[abase[0] + p];
     if (PRE) {
       const int E1 = (int)run[t1 - 1].w;
       for (int e = E0 + lane; e < E1; e += 64) ent[e - E0] = entry[e];
@@ -281,6 +287,14 @@ __global__ void __launch_bounds__(256) k_row_product(
     // value arrays carry 8 doubles of padding past the last row)
     auto load_a = [&](const uint4& rd, double(&av)[DMAX][JMAX]) {
       const int koff = (int)(rd.y >> 16);
+      if (SA) {
+        const int dj = (int)(rd.y & 7u);
+#pragma unroll
+        for (int r = 0; r < DMAX; ++r)
+#pragma unroll
+          for (int m = 0; m < JMAX; ++m) av[r][m] = sa[min(r, d - 1) * lena + koff + min(m, dj - 1)];
+        return;
+      }
       struct Run {
         double v[JMAX == 3 ? 4 : 8];
       };
@@ -418,19 +432,25 @@ void row_product(const RowProductArgs& a, hipStream_t s) {
     fail(SLX_SOLVER_ERROR, "multigrid: row product shape outside the device kernel's range");
   const int dm = a.dmax <= 3 && a.jmax <= 3 ? 3 : 6;
   const int jm = a.jmax <= 3 ? 3 : 6;
-  const size_t lds = row_product_lds(dm, a.lenc_max, a.pre ? a.emax : 0) * (size_t)a.wpb;
+  if (a.sa && dm != 3) fail(SLX_SOLVER_ERROR, "multigrid: row product A staging needs at most 3 rows per node");
+  const size_t lds = row_product_lds(dm, a.lenc_max, a.pre ? a.emax : 0, a.sa ? a.lena_max : 0) * (size_t)a.wpb;
   if (lds > 64 * 1024) fail(SLX_SOLVER_ERROR, "multigrid: a row product row is too long for the device kernel");
   const int grid = (int)(((long long)a.nnodes + a.wpb - 1) / a.wpb + 7) / 8 * 8;
-#define SLX_ROW_PRODUCT(D, J, P)                                                                                  \
-  k_row_product<D, J, P><<<grid, 64 * a.wpb, lds, s>>>(a.nnodes, a.wpb, a.lenc_max, a.emax, a.wmax, a.order,     \
+#define SLX_ROW_PRODUCT(D, J, P, S)                                                                               \
+  k_row_product<D, J, P, S><<<grid, 64 * a.wpb, lds, s>>>(a.nnodes, a.wpb, a.lenc_max, a.emax, a.lena_max, a.wmax, \
+                                                          a.order,                                                 \
                                                        a.node_ptr, a.aptr, a.aval, a.run_ptr, a.run,              \
                                                        a.entry, a.bval, a.cptr, a.cval)
   if (dm == 3) {
-    if (a.pre) SLX_ROW_PRODUCT(3, 3, true); else SLX_ROW_PRODUCT(3, 3, false);
+    if (a.sa) {
+      if (a.pre) SLX_ROW_PRODUCT(3, 3, true, true); else SLX_ROW_PRODUCT(3, 3, false, true);
+    } else {
+      if (a.pre) SLX_ROW_PRODUCT(3, 3, true, false); else SLX_ROW_PRODUCT(3, 3, false, false);
+    }
   } else if (jm == 3) {
-    if (a.pre) SLX_ROW_PRODUCT(6, 3, true); else SLX_ROW_PRODUCT(6, 3, false);
+    if (a.pre) SLX_ROW_PRODUCT(6, 3, true, false); else SLX_ROW_PRODUCT(6, 3, false, false);
   } else {
-    if (a.pre) SLX_ROW_PRODUCT(6, 6, true); else SLX_ROW_PRODUCT(6, 6, false);
+    if (a.pre) SLX_ROW_PRODUCT(6, 6, true, false); else SLX_ROW_PRODUCT(6, 6, false, false);
   }
 #undef SLX_ROW_PRODUCT
 }

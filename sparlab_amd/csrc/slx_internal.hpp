@@ -113,8 +113,8 @@ struct ProductPlan {
 // positions in B's and C's rows (kernels_amg.hip k_row_product).
 struct RowPlan {
   int nnodes = 0;
-  int dmax = 0, jmax = 0, wmax = 0, lenc_max = 0, emax = 0, wpb = 0;
-  bool pre = false;
+  int dmax = 0, jmax = 0, wmax = 0, lenc_max = 0, emax = 0, lena_max = 0, wpb = 0;
+  bool pre = false, sa = false;
   long long runs = 0, entries = 0;
   DBuf<int> order, run_ptr;
   DBuf<uint4> run;

@@ -109,8 +109,10 @@ struct RowProductArgs {
   int nnodes;                 // C nodes
   int dmax, jmax, wmax;       // most rows per C node, most B rows per run, widest aggregate block
   int lenc_max, emax;         // longest C row, most entries of one C node
+  int lena_max;               // longest A row
   int wpb;                    // wavefronts (C nodes) per block
   bool pre;                   // stage each node's entries in LDS
+  bool sa;                    // stage each node's A rows in LDS (at most 3 rows per node)
   const int* order;           // C nodes in processing order
   const int* node_ptr;        // rows of C node I: [node_ptr[I], node_ptr[I+1])
   const int* aptr;
@@ -128,8 +130,8 @@ struct RowProductArgs {
   double* cval;
 };
 // LDS bytes of one wavefront of k_row_product
-inline size_t row_product_lds(int dmax, int lenc_max, int emax) {
-  return sizeof(double) * ((size_t)dmax * (size_t)lenc_max + (size_t)(emax + 1) / 2);
+inline size_t row_product_lds(int dmax, int lenc_max, int emax, int lena) {
+  return sizeof(double) * ((size_t)dmax * ((size_t)lenc_max + (size_t)lena) + (size_t)(emax + 1) / 2);
 }
 void row_product(const RowProductArgs& a, hipStream_t s);
 void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipStream_t s);
