@@ -131,6 +131,24 @@ def level_info(g, level, which):
     return list(info)
 
 
+def measured_traffic(name, sizes):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, when its sizes match the run."""
+    path = os.path.join(ROOT, "profiles", "traffic_row_products_cantilever_128x64x64.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if not parts or parts[0].startswith("#") or parts[0] != name or len(parts) != 8:
+                continue
+            na, nb, nc = (int(v) for v in parts[1:4])
+            if (na, nb, nc) != (sizes["nnz_a"], sizes["nnz_b"], sizes["nnz_c"]):
+                return None
+            fk, wk, fs, ws = float(parts[4]), float(parts[5]), float(parts[6]), float(parts[7])
+            return fk * 1024 * fs + wk * 1024 * ws
+    return None
+
+
 def rooflines(g, m, timers):
     """Roofline of each kernel timed with HIP events over the timed region, largest total first.
 
@@ -167,11 +185,10 @@ def rooflines(g, m, timers):
             label = "k_row_product<6,3> (level-0 R (A P), FP64)"
         nbytes = 8 * (a_nnz + b_nnz + c_nnz) + 16 * runs + 4 * entries + 12 * nodes + 4 * rows
         ach = nbytes / (avg * 1e-3) / 1e9
+        sz = {"nnz_a": a_nnz, "nnz_b": b_nnz, "nnz_c": c_nnz, "nodes": nodes, "runs": runs, "entries": entries}
         out.append({"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                    "traffic": None, "kernel": label, "bytes_per_launch": nbytes, "avg_ms": avg,
-                    "launches": launches, "total_ms": ms,
-                    "sizes": {"nnz_a": a_nnz, "nnz_b": b_nnz, "nnz_c": c_nnz, "nodes": nodes, "runs": runs,
-                              "entries": entries}})
+                    "traffic": measured_traffic(name, sz), "kernel": label, "bytes_per_launch": nbytes,
+                    "avg_ms": avg, "launches": launches, "total_ms": ms, "sizes": sz})
     out.sort(key=lambda r: -r["total_ms"])
     return out
 

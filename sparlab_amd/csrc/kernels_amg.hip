@@ -419,11 +419,8 @@ void block_product(const BlockProductArgs& a, bool smooth, hipStream_t s) {
     fail(SLX_SOLVER_ERROR, "multigrid: block product shape outside the device kernel's range");
   const long long waves = (a.nitems + kWave / a.wmax - 1) / (kWave / a.wmax);
   const int grid = (int)((waves + 3) / 4 + 7) / 8 * 8;
-  if (a.dmax <= 3) {
-    if (smooth) k_block_product<1, 3><<<grid, 256, 0, s>>>(a); else k_block_product<0, 3><<<grid, 256, 0, s>>>(a);
-  } else {
-    if (smooth) k_block_product<1, 6><<<grid, 256, 0, s>>>(a); else k_block_product<0, 6><<<grid, 256, 0, s>>>(a);
-  }
+  if (!smooth) fail(SLX_SOLVER_ERROR, "multigrid: plain block products run as row products");
+  if (a.dmax <= 3) k_block_product<1, 3><<<grid, 256, 0, s>>>(a); else k_block_product<1, 6><<<grid, 256, 0, s>>>(a);
 }
 
 void row_product(const RowProductArgs& a, hipStream_t s) {

@@ -214,7 +214,7 @@ struct Context {
   DBuf<double> r, z, p, q, xw, bw, xpcg, zold;
   // the CG iteration as three captured graphs: q = K p (timed alone), the
   // alpha / x / r update, and the preconditioner with the direction update
-  hipGraphExec_t g_spmv = nullptr, g_update = nullptr, g_precond = nullptr;
+  hipGraphExec_t g_update = nullptr, g_precond = nullptr;  // captured CG iteration parts
   bool capturing = false;
   DBuf<double> partials;  // block partial sums
   DBuf<double> scal;      // device scalars
