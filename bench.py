@@ -149,6 +149,37 @@ def measured_traffic(name, sizes):
     return None
 
 
+def smoother_roofline(g, fp32):
+    """The level-1 Chebyshev step (CSR with shared column lists), timed with HIP events over
+    eager V-cycles on the same hierarchy; priced in HBM bytes: 4 (FP32) or 8 (FP64) per value,
+    4 per shared column index, 8 per row (row pointer and column base) and 7 vector words per
+    row (r in and out, d in and out, x read and written, D^-1)."""
+    rows, _, nnz = level_info(g, 1, 0)
+    nodes, ncol, lanes = level_info(g, 1, 8)
+    word = 4 if fp32 else 8
+    r = np.random.default_rng(5).standard_normal(g.nfree)
+    ms0, n0 = g.ctx.timer("cheb_l1")
+    g.ctx.timing(True)
+    for _ in range(10):
+        g.amg_apply(r)
+    g.ctx.synchronize()
+    ms1, n1 = g.ctx.timer("cheb_l1")
+    g.ctx.timing(False)
+    launches = n1 - n0
+    if launches <= 0:
+        return None
+    avg = (ms1 - ms0) / launches
+    nbytes = word * nnz + 4 * (ncol if ncol else nnz) + 8 * rows + 7 * word * rows
+    ach = nbytes / (avg * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": (f"k_node_spmv<{-lanes}> (level-1 Chebyshev step, {-lanes} lanes per node" if lanes < 0 else
+                       f"k_cheb_step<{lanes}> (level-1 Chebyshev step, {lanes} lanes per row")
+                      + f", {'FP32' if fp32 else 'FP64'} CSR with shared column lists)",
+            "bytes_per_launch": nbytes, "avg_ms": avg, "launches": launches, "total_ms": ms1 - ms0,
+            "sizes": {"rows": rows, "nnz": nnz, "nodes": nodes, "shared_columns": ncol}}
+
+
 def rooflines(g, m, timers):
     """Roofline of each kernel timed with HIP events over the timed region, largest total first.
 
@@ -337,6 +368,9 @@ def main():
     solves = head["steps"] + 1  # run_oc re-evaluates the final design
     value = solves / head["elapsed"]
     roofs = rooflines(g, m, head["timers"])
+    sm = smoother_roofline(g, args.precision == "mixed")
+    if sm is not None:
+        roofs.append(sm)
     config = {"workload": f"cantilever Hex8 {nx}x{ny}x{nz} SIMP/OC from x0 = 0.2, {n} free DOFs, rtol {tol}; "
                           f"{'FP32 V-cycle under FP64 flexible CG' if args.precision == 'mixed' else 'FP64'}"
                           f"{' + learned warm start' if not args.no_learned else ''}",

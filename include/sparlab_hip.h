@@ -248,7 +248,8 @@ slx_status slx_amg_stats_get(const slx_context* ctx, slx_amg_stats* out);
 /* Test access to the hierarchy. which: 0 A_l, 1 P, 2 P-hat, 3 R, 4 A P, 5 R (A P):
  * info = rows, cols, nnz (slx_amg_level_get: 0..3 only); 6, 7: the device plans of
  * the products A P and R (A P): info = output nodes, runs, entries (all 0 when the
- * level uses the generic product). */
+ * level uses the generic product); 8: the level's operator: nodes, length of the
+ * shared column lists (0: none), lanes per row of its CSR products. */
 slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long long* info);
 slx_status slx_amg_level_get(const slx_context* ctx, int level, int which, int* ptr, int* col, double* val);
 slx_status slx_amg_aggregates_get(const slx_context* ctx, int level, int* aggregates);

@@ -687,6 +687,13 @@ slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long
   return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
     if (!c.amg_ready || level < 0 || level >= (int)c.levels.size()) fail(SLX_SOLVER_ERROR, "no such multigrid level");
     Level& lv = c.levels[(size_t)level];
+    if (which == 8) {  // the level's operator: nodes, shared column-list length (0: none), lanes per row
+      const DCsr& a = level == 0 ? c.K : lv.A;
+      info[0] = (long long)lv.node_ptr.size() - 1;
+      info[1] = a.cbase.p ? (long long)a.ncol.n : 0;
+      info[2] = a.lanes;
+      return;
+    }
     if (which == 6 || which == 7) {  // row plans of A P and R (A P): C nodes, runs, entries
       const RowPlan& pl = which == 6 ? lv.plan_ap : lv.plan_raw;
       info[0] = pl.nnodes;

@@ -152,7 +152,7 @@ void cheb_steps(Context& c, size_t l, const void* r_first, void* xv, bool timed)
     const double c2 = 2.0 * rho_new / cp.delta;
     if (F32) {
       const float* rin = k == 1 ? static_cast<const float*>(r_first) : lv.res32.p;
-      Timed t(c, timed ? "cheb" : nullptr);
+      Timed t(c, timed ? "cheb" : l == 1 ? "cheb_l1" : nullptr);
       if (use_mf(c, l)) {
         kmf::MfArgs<float> a = mf_base<float>(c);
         a.x = lv.d32.p, a.b = rin, a.y = lv.res32.p, a.dinv = lv.dinv32.p, a.d_out = lv.d232.p;
@@ -164,7 +164,7 @@ void cheb_steps(Context& c, size_t l, const void* r_first, void* xv, bool timed)
       std::swap(lv.d32, lv.d232);
     } else {
       const double* rin = k == 1 ? static_cast<const double*>(r_first) : lv.res.p;
-      Timed t(c, timed ? "cheb" : nullptr);
+      Timed t(c, timed ? "cheb" : l == 1 ? "cheb_l1" : nullptr);
       if (use_mf(c, l)) {
         kmf::MfArgs<double> a = mf_base<double>(c);
         a.x = lv.d.p, a.b = rin, a.y = lv.res.p, a.dinv = lv.dinv.p, a.d_out = lv.d2.p;
