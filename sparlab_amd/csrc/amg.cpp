@@ -22,6 +22,7 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <future>
 #include <sstream>
 
 #include "slx_kernels.hpp"
@@ -1146,10 +1147,43 @@ double device_dot(Context& c, long long n, const double* a, const double* b) {
   return hb[15];
 }
 
-void amg_setup(Context& c) {
+struct Context::CoarseJob {
+  int n = 0;
+  std::vector<double> dense, inv;
+  double ratio = 0.0;
+  std::future<void> done;
+};
+
+void finish_coarse(Context& c) {
+  if (!c.coarse_job) return;
+  if (c.capturing) fail(SLX_SOLVER_ERROR, "multigrid: internal error (coarse inverse pending during graph capture)");
+  std::shared_ptr<Context::CoarseJob> job = std::move(c.coarse_job);
+  c.coarse_job.reset();
+  if (job->done.valid()) job->done.get();
+  c.coarse_pivot_ratio = job->ratio;
+  if (!(job->ratio > c.amg_opt.coarse_pivot_tolerance)) {
+    c.amg_ready = false;
+    std::ostringstream os;
+    os << "multigrid: the coarsest operator (" << job->n << " unknowns) is singular or indefinite (smallest / largest "
+       << "pivot " << job->ratio << "). A rigid-body mode survived to the coarse level, so the model is "
+       << "under-constrained: too few supports, or a part connected to nothing";
+    fail(SLX_SOLVER_ERROR, os.str());
+  }
+  c.coarse_inv.upload(job->inv, c.stream);
+  if (c.amg_opt.precision >= 1) {
+    c.coarse_inv32.alloc(c.coarse_inv.n);
+    k::to_float((long long)c.coarse_inv.n, c.coarse_inv.p, c.coarse_inv32.p, c.stream);
+  }
+}
+
+void amg_setup(Context& c, bool defer_coarse) {
   const auto t0 = Clock::now();
   hipStream_t s = c.stream;
   const int n = c.K.rows;
+  if (c.coarse_job) {  // a previous setup's inverse nobody used
+    if (c.coarse_job->done.valid()) c.coarse_job->done.wait();
+    c.coarse_job.reset();
+  }
   invalidate_graphs(c);
   for (double& ph : c.phase_seconds) ph = 0.0;
   auto tick = Clock::now();
@@ -1323,20 +1357,18 @@ void amg_setup(Context& c) {
     const int nc = cl.n;
     if (nc > 8000) fail(SLX_SOLVER_ERROR, "multigrid: the coarsest level exceeds the dense solver's 8000 unknowns");
     HCsr ac = download_csr(level_op(c, l), s);
-    std::vector<double> dense((size_t)nc * nc, 0.0), inv;
+    auto job = std::make_shared<Context::CoarseJob>();
+    job->n = nc;
+    job->dense.assign((size_t)nc * nc, 0.0);
     for (int i = 0; i < nc; ++i)
-      for (int k2 = ac.ptr[(size_t)i]; k2 < ac.ptr[(size_t)i + 1]; ++k2) dense[(size_t)i * nc + ac.col[(size_t)k2]] = ac.val[(size_t)k2];
-    double ratio = 0.0;
-    coarse_inverse(nc, dense, inv, ratio);
-    c.coarse_pivot_ratio = ratio;
-    if (!(ratio > o.coarse_pivot_tolerance)) {
-      std::ostringstream os;
-      os << "multigrid: the coarsest operator (" << nc << " unknowns) is singular or indefinite (smallest / largest pivot "
-         << ratio << "). A rigid-body mode survived to the coarse level, so the model is under-constrained: too few "
-                     "supports, or a part connected to nothing";
-      fail(SLX_SOLVER_ERROR, os.str());
-    }
-    c.coarse_inv.upload(inv, s);
+      for (int k2 = ac.ptr[(size_t)i]; k2 < ac.ptr[(size_t)i + 1]; ++k2)
+        job->dense[(size_t)i * nc + ac.col[(size_t)k2]] = ac.val[(size_t)k2];
+    Context::CoarseJob* jp = job.get();
+    if (defer_coarse)
+      job->done = std::async(std::launch::async, [jp] { coarse_inverse(jp->n, jp->dense, jp->inv, jp->ratio); });
+    else
+      coarse_inverse(nc, job->dense, job->inv, job->ratio);
+    c.coarse_job = std::move(job);
   }
   lap(5);
   for (auto& lv : c.levels) {
@@ -1358,6 +1390,7 @@ void amg_setup(Context& c) {
   c.fp_unknowns = c.layout_unknowns;
   c.amg_ready = true;
   c.reused = reuse;
+  if (!defer_coarse) finish_coarse(c);
   c.setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
 }
 

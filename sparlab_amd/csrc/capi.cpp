@@ -150,7 +150,7 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   k::spmv_serial(c.dH, c.d_x.p, c.d_tmp.p, s);
   k::simp((int)ne, c.d_tmp.p, o.penalty, o.emin_ratio, c.d_phys.p, c.d_fac.p, c.d_dfac.p, s);
   assemble(c, c.d_fac.p);
-  amg_setup(c);
+  amg_setup(c, true);
   out.assemble_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
   const auto t1 = Clock::now();
   std::vector<double> comp((size_t)c.ncases);

@@ -106,6 +106,7 @@ std::vector<double> download(Context& c, const DBuf<double>& d, int n) {
 
 void seg_pcg(Context& c, const std::vector<int>& seg_ptr, const double* b, double* x_out, double tol,
              int max_iterations, std::vector<slx_pcg_result>& res) {
+  finish_coarse(c);
   SegState& st = prepare(c, seg_ptr);
   const int ns = st.nseg;
   const long long n = c.K.rows;

@@ -8,7 +8,11 @@
 namespace slx {
 
 // SA-AMG setup of the context's current K_ff (multigrid hierarchy on the device).
-void amg_setup(Context& c);
+// defer_coarse: the coarsest level's dense inverse is computed on a host
+// thread, overlapping the device work that follows; finish_coarse() (called by
+// every V-cycle entry) waits for it, checks its pivots and uploads it
+void amg_setup(Context& c, bool defer_coarse = false);
+void finish_coarse(Context& c);
 // FP32 copies of the hierarchy for the mixed-precision V-cycle.
 void prepare_mixed(Context& c);
 // z = M^-1 r: one V-cycle from a zero guess (device pointers, length n of level 0).

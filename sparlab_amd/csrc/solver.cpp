@@ -405,14 +405,13 @@ void prepare_mixed(Context& c) {
     lv.d32.alloc((size_t)lv.n);
     lv.d232.alloc((size_t)lv.n);
   }
-  c.coarse_inv32.alloc(c.coarse_inv.n);
-  k::to_float((long long)c.coarse_inv.n, c.coarse_inv.p, c.coarse_inv32.p, s);
 }
 
 // One V-cycle on r. FP64 cycle: the result goes to z, returns null. FP32
 // cycle: the result stays in level 0's FP32 iterate, which is returned.
 const float* vcycle_raw(Context& c, const double* r, double* z) {
   if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: apply() called before setup()");
+  finish_coarse(c);
   Level& top = c.levels[0];
   const int n = top.n;
   if (c.amg_opt.precision >= 1) {
@@ -462,6 +461,7 @@ hipGraphExec_t capture_graph(Context& c, const std::function<void()>& body) {
 }
 
 slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int max_iterations) {
+  finish_coarse(c);  // before any capture: the graphs read the installed inverse
   const long long n = c.K.rows;
   hipStream_t s = c.stream;
   slx_pcg_result res{0, 0.0, 0};
