@@ -150,25 +150,16 @@ def measured_traffic(name, sizes):
 
 
 def smoother_roofline(g, fp32):
-    """The level-1 Chebyshev step (CSR with shared column lists), timed with HIP events over
-    eager V-cycles on the same hierarchy; priced in HBM bytes: 4 (FP32) or 8 (FP64) per value,
+    """The level-1 Chebyshev step (CSR with shared column lists) of the same hierarchy, timed with
+    HIP events around 50 back-to-back launches; priced in HBM bytes: 4 (FP32) or 8 (FP64) per value,
     4 per shared column index, 8 per row (row pointer and column base) and 7 vector words per
     row (r in and out, d in and out, x read and written, D^-1)."""
     rows, _, nnz = level_info(g, 1, 0)
     nodes, ncol, lanes = level_info(g, 1, 8)
     word = 4 if fp32 else 8
-    r = np.random.default_rng(5).standard_normal(g.nfree)
-    ms0, n0 = g.ctx.timer("cheb_l1")
-    g.ctx.timing(True)
-    for _ in range(10):
-        g.amg_apply(r)
-    g.ctx.synchronize()
-    ms1, n1 = g.ctx.timer("cheb_l1")
-    g.ctx.timing(False)
-    launches = n1 - n0
-    if launches <= 0:
-        return None
-    avg = (ms1 - ms0) / launches
+    launches = 50
+    avg = g.time_smoother(1, launches)
+    g.amg_apply(np.ones(g.nfree))  # the level's vectors back to a V-cycle's state
     nbytes = word * nnz + 4 * (ncol if ncol else nnz) + 8 * rows + 7 * word * rows
     ach = nbytes / (avg * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
@@ -176,7 +167,7 @@ def smoother_roofline(g, fp32):
             "kernel": (f"k_node_spmv<{-lanes}> (level-1 Chebyshev step, {-lanes} lanes per node" if lanes < 0 else
                        f"k_cheb_step<{lanes}> (level-1 Chebyshev step, {lanes} lanes per row")
                       + f", {'FP32' if fp32 else 'FP64'} CSR with shared column lists)",
-            "bytes_per_launch": nbytes, "avg_ms": avg, "launches": launches, "total_ms": ms1 - ms0,
+            "bytes_per_launch": nbytes, "avg_ms": avg, "launches": launches, "total_ms": avg * launches,
             "sizes": {"rows": rows, "nnz": nnz, "nodes": nodes, "shared_columns": ncol}}
 
 

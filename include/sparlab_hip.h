@@ -244,6 +244,11 @@ slx_status slx_solve_case(slx_context* ctx, int load_case, int warm_start, doubl
 slx_status slx_solve_segmented(slx_context* ctx, int nseg, const int* seg_ptr, const double* b, double* x, double tol,
                                int max_iterations, slx_pcg_result* results);
 slx_status slx_amg_apply(slx_context* ctx, const double* r, double* z);
+/* Measurement (no counterpart in the reference): the average time of one
+ * Chebyshev step of a level strictly between the finest and the coarsest, over
+ * reps back-to-back launches between two device events. It overwrites that
+ * level's work vectors (the next V-cycle sets them again). */
+slx_status slx_amg_time_smoother(slx_context* ctx, int level, int reps, double* ms_per_step);
 slx_status slx_amg_stats_get(const slx_context* ctx, slx_amg_stats* out);
 /* Test access to the hierarchy. which: 0 A_l, 1 P, 2 P-hat, 3 R, 4 A P, 5 R (A P):
  * info = rows, cols, nnz (slx_amg_level_get: 0..3 only); 6, 7: the device plans of

@@ -642,6 +642,13 @@ slx_status slx_solve_case(slx_context* ctx, int load_case, int warm_start, doubl
   });
 }
 
+slx_status slx_amg_time_smoother(slx_context* ctx, int level, int reps, double* ms_per_step) {
+  return guarded(ctx, [&](Context& c) {
+    finish_coarse(c);
+    *ms_per_step = time_smoother(c, level, reps);
+  });
+}
+
 slx_status slx_amg_apply(slx_context* ctx, const double* r, double* z) {
   return guarded(ctx, [&](Context& c) {
     const size_t n = (size_t)c.K.rows;

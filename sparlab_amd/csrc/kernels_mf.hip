@@ -73,6 +73,18 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
     return;
   }
   const int x0 = (tile % ntx) * TX, y0 = ((tile / ntx) % nty) * TY, z0 = (tile / (ntx * nty)) * TZ;
+  const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
+  const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+  const bool inside = gx < a.X && gy < a.Y && gz < a.Z;
+  // the node's free indices first: the epilogue's operands load before the
+  // element products, which hide their latency
+  int fi[3] = {-1, -1, -1};
+  if (inside) {
+    const int* f = a.reduced + 3 * (((long long)gz * a.Y + gy) * a.X + gx);
+    fi[0] = f[0];
+    fi[1] = f[1];
+    fi[2] = f[2];
+  }
   // stage the input (scaled for the Lanczos mode) over the tile and its halo
   for (int h = threadIdx.x; h < HZ * HY * HX; h += kThreads) {
     const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
@@ -107,10 +119,23 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
       sv = a.scale[((long long)gz * ney + gy) * nex + gx];
     s_s[h] = sv;
   }
+  // (prescribed DOFs load entry 0 and discard it: no branches around the loads)
+  T px[3] = {0, 0, 0}, pb[3] = {0, 0, 0}, pd[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int g = fi[c] < 0 ? 0 : fi[c];
+    if (MODE == 1 || MODE == 3 || MODE == 5 || MODE == 6) px[c] = a.x[g];
+    if (MODE == 2 || MODE == 3) pb[c] = a.b[g];
+    if (MODE == 3 || MODE == 6) {
+      pd[c] = a.dinv[g];
+      ps[c] = a.xs[g];
+    }
+    if (MODE == 4) {
+      pd[c] = a.in_scale[g];
+      ps[c] = a.vprev[g];
+    }
+  }
   __syncthreads();
-  const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
-  const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
-  const bool inside = gx < a.X && gy < a.Y && gz < a.Z;
   T acc0 = 0, acc1 = 0, acc2 = 0;
   if (inside) {
 #pragma unroll
@@ -147,38 +172,37 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
     }
   }
   double dot = 0.0;
-  if (inside) {
-    const int* f = a.reduced + 3 * (((long long)gz * a.Y + gy) * a.X + gx);
+  {
     const T acc[3] = {acc0, acc1, acc2};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const int fi = f[c];
-      if (fi < 0) continue;
+      const int f = fi[c];
+      if (f < 0) continue;
       const T y = acc[c];
       if (MODE == 0) {
-        a.y[fi] = y;
+        a.y[f] = y;
       } else if (MODE == 1) {
-        a.y[fi] = y;
-        dot += (double)a.x[fi] * (double)y;
+        a.y[f] = y;
+        dot += (double)px[c] * (double)y;
       } else if (MODE == 2) {
-        a.y[fi] = a.b[fi] - y;
+        a.y[f] = pb[c] - y;
       } else if (MODE == 3) {
         // r_out = r_in - K d_in, d_out = c1 d_in + c2 D^-1 r_out, x += d_out (x input = d_in)
-        const double r = (double)a.b[fi] - (double)y;
-        const double dn = a.c1 * (double)a.x[fi] + a.c2 * (double)a.dinv[fi] * r;
-        a.y[fi] = (T)r;
-        a.d_out[fi] = (T)dn;
-        a.xs[fi] = (T)((double)a.xs[fi] + dn);
+        const double r = (double)pb[c] - (double)y;
+        const double dn = a.c1 * (double)px[c] + a.c2 * (double)pd[c] * r;
+        a.y[f] = (T)r;
+        a.d_out[f] = (T)dn;
+        a.xs[f] = (T)((double)ps[c] + dn);
       } else if (MODE == 4) {
         // Lanczos: w = s (K (s v)) - beta vprev
         const double beta = a.beta ? *a.beta : 0.0;
-        a.y[fi] = (T)((double)y * (double)a.in_scale[fi] - beta * (double)a.vprev[fi]);
+        a.y[f] = (T)((double)y * (double)pd[c] - beta * (double)ps[c]);
       } else if (MODE == 5) {
         // smoothed restriction, first half: y = (I - omega K D^-1) x
-        a.y[fi] = (T)((double)a.x[fi] - a.c1 * (double)y);
+        a.y[f] = (T)((double)px[c] - a.c1 * (double)y);
       } else {
         // smoothed prolongation, second half: xs += (I - omega D^-1 K) x
-        a.xs[fi] = (T)((double)a.xs[fi] + (double)a.x[fi] - a.c1 * (double)a.dinv[fi] * (double)y);
+        a.xs[f] = (T)((double)ps[c] + (double)px[c] - a.c1 * (double)pd[c] * (double)y);
       }
     }
   }

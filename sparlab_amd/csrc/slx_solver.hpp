@@ -13,6 +13,9 @@ namespace slx {
 // every V-cycle entry) waits for it, checks its pivots and uploads it
 void amg_setup(Context& c, bool defer_coarse = false);
 void finish_coarse(Context& c);
+// average milliseconds of one Chebyshev step at a middle level, over reps
+// back-to-back launches between two events (measurement only)
+double time_smoother(Context& c, int level, int reps);
 // FP32 copies of the hierarchy for the mixed-precision V-cycle.
 void prepare_mixed(Context& c);
 // z = M^-1 r: one V-cycle from a zero guess (device pointers, length n of level 0).

@@ -435,6 +435,41 @@ void vcycle(Context& c, const double* r, double* z) {
   if (zc) k::to_double((long long)c.levels[0].n, zc, z, c.stream);
 }
 
+double time_smoother(Context& c, int level, int reps) {
+  if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: no hierarchy to time");
+  if (level < 1 || level + 1 >= (int)c.levels.size() || reps < 1)
+    fail(SLX_CONFIG_ERROR, "smoother timing: the level must lie strictly between the finest and the coarsest");
+  Level& lv = c.levels[(size_t)level];
+  DCsr& A = op(c, (size_t)level);
+  const bool f32 = c.amg_opt.precision >= 1;
+  hipStream_t s = c.stream;
+  // finite inputs; c1 = c2 = 0 keeps d and x unchanged, every load and store still runs
+  if (f32) {
+    k::to_float(lv.n, lv.dinv.p, lv.d32.p, s);
+    k::to_float(lv.n, lv.dinv.p, lv.res32.p, s);
+  } else {
+    k::copy(lv.n, lv.dinv.p, lv.d.p, s);
+    k::copy(lv.n, lv.dinv.p, lv.res.p, s);
+  }
+  auto step = [&] {
+    if (f32)
+      k::cheb_step32(A, lv.dinv32.p, lv.res32.p, lv.b32.p, lv.d32.p, lv.d232.p, lv.x32.p, 0.0, 0.0, s);
+    else
+      k::cheb_step(A, lv.dinv.p, lv.res.p, lv.b.p, lv.d.p, lv.d2.p, lv.x.p, 0.0, 0.0, s);
+  };
+  step();  // warm
+  hipEvent_t a = c.take_event(), b = c.take_event();
+  SLX_HIP(hipEventRecord(a, s));
+  for (int r = 0; r < reps; ++r) step();
+  SLX_HIP(hipEventRecord(b, s));
+  SLX_HIP(hipEventSynchronize(b));
+  float ms = 0.0f;
+  SLX_HIP(hipEventElapsedTime(&ms, a, b));
+  c.event_pool.push_back(a);
+  c.event_pool.push_back(b);
+  return ms / reps;
+}
+
 bool graphs_enabled() {
   const char* e = std::getenv("SPARLAB_NO_GRAPHS");
   return !(e && e[0] == '1');

@@ -142,6 +142,7 @@ _SIGNATURES = {
     "slx_solve_case": (C.c_int, [_VP, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(PcgResult)]),
     "slx_solve_segmented": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, C.c_double, C.c_int, _VP]),
     "slx_amg_apply": (C.c_int, [_VP, _VP, _VP]),
+    "slx_amg_time_smoother": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "slx_amg_stats_get": (C.c_int, [_VP, C.POINTER(AmgStats)]),
     "slx_amg_level_info": (C.c_int, [_VP, C.c_int, C.c_int, _VP]),
     "slx_amg_level_get": (C.c_int, [_VP, C.c_int, C.c_int, _VP, _VP, _VP]),
@@ -499,6 +500,13 @@ class HipProblem:
         a = np.empty(num_nodes, dtype=np.int32)
         self.ctx.check(load().slx_amg_aggregates_get(self.ctx.h, level, _p(a)))
         return a
+
+    def time_smoother(self, level=1, reps=50):
+        """Milliseconds per Chebyshev step of a middle level (back-to-back launches)."""
+        self._own()
+        ms = C.c_double()
+        self.ctx.check(load().slx_amg_time_smoother(self.ctx.h, level, reps, C.byref(ms)))
+        return ms.value
 
     def amg_apply(self, r):
         self._own()
