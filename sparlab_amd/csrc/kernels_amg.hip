@@ -239,7 +239,8 @@ __global__ void __launch_bounds__(256) k_row_product(
     const unsigned* __restrict__ entry, const double* __restrict__ bval, const int* __restrict__ cptr,
     double* __restrict__ cval) {
   extern __shared__ double lds[];
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wavefront's index is uniform: its LDS bases stay in scalar registers
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   // consecutive nodes (Morton order) stay on one XCD and its L2: block b runs
   // logical block (b % 8) * (grid / 8) + b / 8 (the grid is a multiple of 8)
   const int blk = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
@@ -317,9 +318,23 @@ __global__ void __launch_bounds__(256) k_row_product(
       }
       return n;
     };
+    // v += a b in A's order with fused multiply-adds; a run spanning JMAX B rows
+    // (the common case) takes a branch-free path
     auto add = [&](const Step& st, int pos, const double(&bv)[JMAX], const double(&av)[DMAX][JMAX]) {
+#pragma clang fp contract(fast)
       const int dj = (int)(st.rd.y & 7u);
       if (pos < 0) return;
+      if (dj == JMAX) {
+#pragma unroll
+        for (int r = 0; r < DMAX; ++r) {
+          if (r >= d) break;
+          double v = acc[r * lenc + pos];
+#pragma unroll
+          for (int m = 0; m < JMAX; ++m) v += av[r][m] * bv[m];
+          acc[r * lenc + pos] = v;
+        }
+        return;
+      }
 #pragma unroll
       for (int r = 0; r < DMAX; ++r) {
         if (r >= d) break;
