@@ -221,8 +221,9 @@ __global__ void __launch_bounds__(256) k_block_product(BlockProductArgs g) {
 // A(run) . B(run rows, column) into the node's accumulators in LDS at the
 // block's position in C's row. Within one run no two lanes share an
 // accumulator, so every entry of C sums its products in the order of A's row,
-// the order of the CPU restatement (Multigrid.cpp:199-225). The next step's B
-// values are loaded while the current step is added.
+// the order of the CPU restatement (Multigrid.cpp:199-225), with fused
+// multiply-adds (rounding differs from the restatement's separate products and
+// sums). The next step's B values are loaded while the current step is added.
 // With PRE the node's entries, with SA the node's A rows, are staged in LDS
 // first (SA: A's values come from LDS instead of scalar loads).
 // ---------------------------------------------------------------------------

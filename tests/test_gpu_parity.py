@@ -298,3 +298,27 @@ def test_error_paths(ctx):
         g2.evaluate(np.full(m2.nelem, 0.5), tol=1e-14, max_iterations=2)
     with pytest.raises(N.ModelError):
         g2.assemble(-np.ones(m2.nelem))
+
+
+def test_smoother_timing_leaves_the_solver_usable(ctx):
+    """The smoother timing hook overwrites a middle level's work vectors; the next
+    V-cycle and solve must be unaffected."""
+    print("test_smoother_timing_leaves_the_solver_usable", flush=True)
+    m = M.cantilever_3d(24, 12, 12)
+    g = N.HipProblem(m, ctx=ctx)
+    g.assemble(1e-9 + (1 - 1e-9) * M.seeded_density(m.nelem, seed=31) ** 3)
+    g.amg_setup(precision=1, coarse_size=300)
+    assert g.amg_stats().levels >= 3
+    b = m.free_loads()[0]
+    r = np.random.default_rng(32).standard_normal(g.nfree)
+    z0 = g.amg_apply(r)
+    x0, it0, _, cv0 = g.solve(b, tol=1e-10)
+    ms = g.time_smoother(1, 5)
+    assert ms > 0.0
+    z1 = g.amg_apply(r)
+    x1, it1, _, cv1 = g.solve(b, tol=1e-10)
+    np.testing.assert_array_equal(z0, z1)
+    assert cv0 and cv1 and it0 == it1
+    np.testing.assert_array_equal(x0, x1)
+    with pytest.raises(N.ConfigError):
+        g.time_smoother(0, 5)
