@@ -1009,7 +1009,7 @@ double lanczos(Context& c, size_t l) {
   lv.lz_v.alloc((size_t)n);
   lv.lz_vprev.alloc((size_t)n);
   lv.lz_w.alloc((size_t)n);
-  DBuf<double> sc;  // [0] norm, [1..m] alpha, [m+1..2m] beta
+  DBuf<double>& sc = lv.lz_scal;  // [0] norm, [1..m] alpha, [m+1..2m] beta
   sc.alloc((size_t)(2 * m + 1));
   k::vsqrt(n, lv.dinv.p, lv.sq.p, s);
   k::dot(n, lv.lz_start.p, lv.lz_start.p, c.partials.p, sc.p, s);
@@ -1020,14 +1020,16 @@ double lanczos(Context& c, size_t l) {
   double* alpha = sc.p + 1;
   double* beta = sc.p + 1 + m;
   for (int j = 0; j < m; ++j) {
-    if (&A == &c.K)
-      op_lanczos(c, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, j > 0 ? beta + j - 1 : nullptr, lv.lz_w.p);
-    else
-      k::lanczos_apply(A, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, j > 0 ? beta + j - 1 : nullptr, lv.lz_w.p, s);
-    k::dot(n, lv.lz_w.p, lv.lz_v.p, c.partials.p, alpha + j, s);
-    k::axmy(n, alpha + j, lv.lz_v.p, lv.lz_w.p, s);
-    k::dot(n, lv.lz_w.p, lv.lz_w.p, c.partials.p, beta + j, s);
-    k::sqrt_scalar(beta + j, s);
+    // w = s A (s v) - beta_{j-1} vprev with the partials of w.v; alpha_j = w.v;
+    // w -= alpha_j v with the partials of w.w; beta_j = |w|; v, vprev advance
+    const double* bprev = j > 0 ? beta + j - 1 : nullptr;
+    const int nb = &A == &c.K ? op_lanczos(c, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, bprev, lv.lz_w.p, c.partials.p)
+                              : (k::lanczos_apply(A, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, bprev, lv.lz_w.p,
+                                                  c.partials.p, s),
+                                 k::lanczos_blocks(n));
+    k::reduce(nb, c.partials.p, alpha + j, 0, s);
+    k::axmy_dot(n, alpha + j, lv.lz_v.p, lv.lz_w.p, c.partials.p, s);
+    k::reduce(k::axmy_blocks(n), c.partials.p, beta + j, 1, s);
     if (j + 1 < m) k::lanczos_shift(n, beta + j, lv.lz_w.p, lv.lz_v.p, lv.lz_vprev.p, s);
   }
   std::vector<double> h((size_t)(2 * m + 1));

@@ -374,34 +374,57 @@ __global__ void __launch_bounds__(256) k_row_product(
 // Lanczos on D^-1/2 A D^-1/2 (Multigrid.cpp:304-317). The step's scalars stay
 // in device memory, so a level's steps run without host round trips:
 // w_i = s_i (A (s v))_i - beta vprev_i, with s = sqrt(dinv), beta = *beta_ptr (0 if null)
+// ... with the block partials of w.v (rows in a grid-stride loop; partials[block])
 __global__ void __launch_bounds__(256) k_lanczos_apply(int n, const int* __restrict__ ptr, const int* __restrict__ col,
                                                        const double* __restrict__ val, const double* __restrict__ s,
                                                        const double* __restrict__ v, const double* __restrict__ vprev,
-                                                       const double* __restrict__ beta_ptr, double* __restrict__ w) {
+                                                       const double* __restrict__ beta_ptr, double* __restrict__ w,
+                                                       double* __restrict__ partials) {
   constexpr int LANES = 16;
-  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+  __shared__ double sh[4];
   const int lane = threadIdx.x % LANES;
-  if (row >= n) return;
   const double beta = beta_ptr ? *beta_ptr : 0.0;
-  double acc = 0.0;
-  for (int k = ptr[row] + lane; k < ptr[row + 1]; k += LANES) {
-    const int j = col[k];
-    acc += val[k] * (s[j] * v[j]);
-  }
+  double dot = 0.0;
+  for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / LANES; row < n; row += gridDim.x * (blockDim.x / LANES)) {
+    double acc = 0.0;
+    for (int k = ptr[row] + lane; k < ptr[row + 1]; k += LANES) {
+      const int j = col[k];
+      acc += val[k] * (s[j] * v[j]);
+    }
 #pragma unroll
-  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
-  if (lane == 0) w[row] = acc * s[row] - beta * vprev[row];
+    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    if (lane == 0) {
+      const double wi = acc * s[row] - beta * vprev[row];
+      w[row] = wi;
+      dot += wi * v[row];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = dot;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// w -= a v with a = *a_ptr, and the block partials of w.w (grid-stride)
+__global__ void __launch_bounds__(256) k_axmy_dot(int n, const double* __restrict__ a_ptr, const double* __restrict__ v,
+                                                  double* __restrict__ w, double* __restrict__ partials) {
+  __shared__ double sh[4];
+  const double a = *a_ptr;
+  double dot = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double wi = w[i] - a * v[i];
+    w[i] = wi;
+    dot += wi * wi;
+  }
+  for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = dot;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 __global__ void k_sqrt(int n, const double* __restrict__ in, double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = sqrt(in[i]);
-}
-
-// w -= a v with a = *a_ptr
-__global__ void k_axmy(int n, const double* __restrict__ a_ptr, const double* __restrict__ v, double* __restrict__ w) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) w[i] -= *a_ptr * v[i];
 }
 
 // vprev = v; v = w / b with b = *b_ptr
@@ -505,15 +528,21 @@ void vsqrt(int n, const double* in, double* out, hipStream_t s) {
   if (n) k_sqrt<<<nblk(n, 256), 256, 0, s>>>(n, in, out);
 }
 
+int lanczos_blocks(int n) { return std::max(1, std::min(1024, nblk((long long)n * 16, 256))); }
+
 void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, const double* beta,
-                   double* w, hipStream_t s) {
-  if (A.rows) k_lanczos_apply<<<nblk((long long)A.rows * 16, 256), 256, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, sq, v,
-                                                                              vprev, beta, w);
+                   double* w, double* partials, hipStream_t s) {
+  if (A.rows)
+    k_lanczos_apply<<<lanczos_blocks(A.rows), 256, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, sq, v, vprev, beta, w,
+                                                           partials);
 }
 
-void axmy(int n, const double* a, const double* v, double* w, hipStream_t s) {
-  if (n) k_axmy<<<nblk(n, 256), 256, 0, s>>>(n, a, v, w);
+int axmy_blocks(int n) { return std::max(1, std::min(1024, nblk(n, 256))); }
+
+void axmy_dot(int n, const double* a, const double* v, double* w, double* partials, hipStream_t s) {
+  if (n) k_axmy_dot<<<axmy_blocks(n), 256, 0, s>>>(n, a, v, w, partials);
 }
+
 
 void lanczos_shift(int n, const double* b, const double* w, double* v, double* vprev, hipStream_t s) {
   if (n) k_lanczos_shift<<<nblk(n, 256), 256, 0, s>>>(n, b, w, v, vprev);

@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
   const int ntx = (a.X + TX - 1) / TX, nty = (a.Y + TY - 1) / TY, ntz = (a.Z + TZ - 1) / TZ;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   if (tile >= ntx * nty * ntz) {
-    if (MODE == 1 && threadIdx.x == 0) a.partials[blockIdx.x] = 0.0;
+    if ((MODE == 1 || (MODE == 4 && a.partials)) && threadIdx.x == 0) a.partials[blockIdx.x] = 0.0;
     return;
   }
   const int x0 = (tile % ntx) * TX, y0 = ((tile / ntx) % nty) * TY, z0 = (tile / (ntx * nty)) * TZ;
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int g = fi[c] < 0 ? 0 : fi[c];
-    if (MODE == 1 || MODE == 3 || MODE == 5 || MODE == 6) px[c] = a.x[g];
+    if (MODE == 1 || MODE == 3 || MODE == 4 || MODE == 5 || MODE == 6) px[c] = a.x[g];
     if (MODE == 2 || MODE == 3) pb[c] = a.b[g];
     if (MODE == 3 || MODE == 6) {
       pd[c] = a.dinv[g];
@@ -194,9 +194,11 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
         a.d_out[f] = (T)dn;
         a.xs[f] = (T)((double)ps[c] + dn);
       } else if (MODE == 4) {
-        // Lanczos: w = s (K (s v)) - beta vprev
+        // Lanczos: w = s (K (s v)) - beta vprev, with the partials of w.v
         const double beta = a.beta ? *a.beta : 0.0;
-        a.y[f] = (T)((double)y * (double)pd[c] - beta * (double)ps[c]);
+        const double wv = (double)y * (double)pd[c] - beta * (double)ps[c];
+        a.y[f] = (T)wv;
+        dot += wv * (double)px[c];
       } else if (MODE == 5) {
         // smoothed restriction, first half: y = (I - omega K D^-1) x
         a.y[f] = (T)((double)px[c] - a.c1 * (double)y);
@@ -206,7 +208,7 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
       }
     }
   }
-  if (MODE == 1) {
+  if (MODE == 1 || (MODE == 4 && a.partials)) {
     const double s = block_sum<T>(dot, s_red);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
   }

@@ -141,9 +141,13 @@ void gather(long long n, const int* src, const double* in, double* out, hipStrea
 void symmetrize_values(long long n, const int* ia, const int* ib, const double* raw, double* out, hipStream_t s);
 void inv_diag(const DCsr& A, double* dinv, int* bad_row, hipStream_t s);
 void vsqrt(int n, const double* in, double* out, hipStream_t s);
+// w = s (A (s v)) - beta vprev with the partials of w.v: lanczos_blocks(n) of them
+int lanczos_blocks(int n);
 void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, const double* beta,
-                   double* w, hipStream_t s);
-void axmy(int n, const double* a, const double* v, double* w, hipStream_t s);
+                   double* w, double* partials, hipStream_t s);
+// w -= a v with the partials of w.w: axmy_blocks(n) of them
+int axmy_blocks(int n);
+void axmy_dot(int n, const double* a, const double* v, double* w, double* partials, hipStream_t s);
 void lanczos_shift(int n, const double* b, const double* w, double* v, double* vprev, hipStream_t s);
 void sqrt_scalar(double* p, hipStream_t s);
 void fill(long long n, double v, double* out, hipStream_t s);

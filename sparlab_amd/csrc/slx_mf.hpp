@@ -12,7 +12,8 @@ namespace kmf {
 //   2  y = b - K x
 //   3  Chebyshev step with d_in = x, r_in = b: y = r_out = r_in - K d_in,
 //      d_out = c1 d_in + c2 dinv r_out, xs += d_out
-//   4  Lanczos step: y = in_scale (K (in_scale x)) - (*beta) vprev (beta null: 0)
+//   4  Lanczos step: y = in_scale (K (in_scale x)) - (*beta) vprev (beta null: 0),
+//      and when partials is set partials[block] = sum of y.x over the block
 //   5  restriction smoothing: y = x - c1 K (in_scale x)   (in_scale = D^-1, c1 = omega)
 //   6  prolongation smoothing: xs += x - c1 dinv (K x)
 // X, Y, Z are the node counts per direction; reduced maps DOF = node * 3 + k to

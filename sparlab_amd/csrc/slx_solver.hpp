@@ -35,7 +35,10 @@ void op_residual(Context& c, const double* x, const double* b, double* r);
 int op_dot_blocks(const Context& c);
 void op_apply_dot(Context& c, const double* x, double* y, double* partials);
 // Lanczos step at level 0: w = sq (K (sq v)) - (*beta) vprev
-void op_lanczos(Context& c, const double* sq, const double* v, const double* vprev, const double* beta, double* w);
+// one Lanczos step's product w = s (K (s v)) - (*beta) vprev with the block
+// partials of w.v; returns the number of partials
+int op_lanczos(Context& c, const double* sq, const double* v, const double* vprev, const double* beta, double* w,
+               double* partials);
 // whether level-0 products run matrix-free (SPARLAB_NO_MATRIX_FREE=1 turns it off)
 bool matrix_free_enabled();
 

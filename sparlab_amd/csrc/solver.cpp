@@ -324,7 +324,8 @@ void op_apply_dot(Context& c, const double* x, double* y, double* partials) {
   }
 }
 
-void op_lanczos(Context& c, const double* sq, const double* v, const double* vprev, const double* beta, double* w) {
+int op_lanczos(Context& c, const double* sq, const double* v, const double* vprev, const double* beta, double* w,
+               double* partials) {
   if (c.mf) {
     kmf::MfArgs<double> a = mf_base<double>(c);
     a.x = v;
@@ -332,10 +333,12 @@ void op_lanczos(Context& c, const double* sq, const double* v, const double* vpr
     a.in_scale = sq;
     a.vprev = vprev;
     a.beta = beta;
+    a.partials = partials;
     kmf::apply(a, 4, c.stream);
-  } else {
-    k::lanczos_apply(c.K, sq, v, vprev, beta, w, c.stream);
+    return op_dot_blocks(c);
   }
+  k::lanczos_apply(c.K, sq, v, vprev, beta, w, partials, c.stream);
+  return k::lanczos_blocks(c.K.rows);
 }
 
 void invalidate_graphs(Context& c) {
