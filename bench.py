@@ -98,7 +98,7 @@ def simp_run(N, m, x0, warmup, steps, precision, learned, tol, label):
     # one more OC update so the timed loop starts at a design no solve has seen
     ev = g.evaluate(x1, tol=tol)
     x1, _ = g.oc(x1, ev["dc_dx"], ev["dv_dx"])
-    print(f"bench[{label}]: warmup {warmup} iterations, CG its {h1[:, 6].astype(int).tolist()}", file=sys.stderr,
+    print(f"bench[{label}]: warmup {warmup} iterations, CG its {h1[:, 11].astype(int).tolist()}", file=sys.stderr,
           flush=True)
     g.ctx.timing(True)
     g.ctx.synchronize()
@@ -108,7 +108,7 @@ def simp_run(N, m, x0, warmup, steps, precision, learned, tol, label):
     elapsed = time.perf_counter() - t0
     timers = {name: g.ctx.timer(name) for name in ("spmv", "galerkin_ap", "galerkin_raw", "assemble")}
     g.ctx.timing(False)
-    its = h2[:, 6].astype(int).tolist()
+    its = h2[:, 11].astype(int).tolist()
     print(f"bench[{label}]: {done} timed iterations in {elapsed:.3f} s, CG its {its}, c = {c2:.8e}",
           file=sys.stderr, flush=True)
     st = g.amg_stats()

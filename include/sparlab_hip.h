@@ -155,6 +155,35 @@ typedef struct {
   slx_eval_options eval;
 } slx_run_options;
 
+/* TopologyOptimizationResult (include/sparlab/topopt/TopologyOptimizer.hpp:193-258,
+ * the OC fields): the figures of the final evaluation at the returned design
+ * (src/topopt/TopologyOptimizer.cpp:263-283) and how the loop ended
+ * (src/topopt/TopologyOptimizer.cpp:672-701). */
+typedef enum {
+  SLX_STOP_ITERATION_CAP = 0,   /* "iteration_cap" */
+  SLX_STOP_DESIGN_CHANGE = 1,   /* "design_change" */
+  SLX_STOP_OBJECTIVE_STALL = 2  /* "objective_stall" */
+} slx_stop_reason;
+
+typedef struct {
+  int iterations;                /* performed */
+  int converged;
+  int stop_reason;               /* slx_stop_reason */
+  double compliance;             /* J, weighted over the load cases */
+  double volume;                 /* m^3 of the physical density */
+  double volume_fraction;
+  double grey_level;             /* 4/n sum rho (1 - rho) of the physical density */
+  double final_design_change;
+  double final_objective_change; /* relative compliance spread over the window (inf before it fills) */
+  long long linear_solves;       /* one per load case and evaluation, the final one included */
+  long long linear_iterations;
+  double total_seconds;
+} slx_run_result;
+
+/* Columns of one optimisation history row, in the order of the reference's
+ * history.csv (src/io/ResultWriter.cpp:1428-1470, OC without projection). */
+#define SLX_HISTORY_COLUMNS 12
+
 /* Learned warm start: a 3-level 3-D convolutional network on the node grid maps
  * per-node (density, density change, previous displacement, residual of the
  * previous displacement, previous displacement change s) to the direction d of
@@ -270,10 +299,16 @@ slx_status slx_evaluate(slx_context* ctx, const double* x, const slx_eval_option
                         double* dc_dx, double* dv_dx, double* energy);
 slx_status slx_oc_update(slx_context* ctx, const double* x, const double* dc_dx, const double* dv_dx,
                          const slx_oc_options* opt, double* x_new, slx_oc_step* step);
-/* history[max_iterations * 8]: iteration, penalty, compliance, volume fraction,
- * max change, lambda, linear iterations, grey level. */
-slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history, int* performed,
-                      double* final_compliance);
+/* x[nelem] in: the initial design, out: the final one. history[max_iterations *
+ * SLX_HISTORY_COLUMNS] (may be NULL), one row per performed iteration:
+ * iteration, penalty, compliance [J], volume [m^3], volume fraction, max design
+ * change, Lagrange multiplier, grey level, OC bisections, volume converged (0/1),
+ * seconds, linear iterations. */
+slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history,
+                      slx_run_result* result);
+/* compliance[ncases]: f_l . u_l of each load case at the last evaluation
+ * (ObjectiveEvaluation::load_case_compliance, include/sparlab/topopt/Sensitivity.hpp:78) */
+slx_status slx_load_case_compliance_get(const slx_context* ctx, double* compliance);
 
 /* ---- learned warm start ------------------------------------------------------ */
 void slx_warmstart_default_options(slx_warmstart_options* o);

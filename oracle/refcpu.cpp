@@ -1721,8 +1721,8 @@ RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0)
     double gray = 0.0;
     for (double r : ev.physical) gray += r * (1.0 - r);
     gray = 4.0 * gray / double(ev.physical.size());
-    res.history.push_back({iteration, penalty, ev.compliance, ev.volume_fraction, step.max_change, step.lambda, gray,
-                           ev.linear_iterations, step.bisections});
+    res.history.push_back({iteration, penalty, ev.compliance, ev.volume, ev.volume_fraction, step.max_change,
+                           step.lambda, gray, step.bisections, step.volume_converged, ev.linear_iterations});
     history.push_back(ev.compliance);
     x = step.x;
     for (size_t e = 0; e < x.size(); ++e) x[e] = std::clamp(x[e], p.lower[e], p.upper[e]);
@@ -1737,12 +1737,19 @@ RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0)
     const bool objective_ok = o.objective_tolerance > 0.0 && spread <= o.objective_tolerance;
     if (at_final && (change_ok || objective_ok)) {
       res.converged = true;
+      res.stop_reason = change_ok ? 1 : 2;  // TopologyOptimizer.cpp:692
       break;
     }
   }
   res.iterations = std::min(iteration, o.max_iterations);
-  Evaluation fin = evaluate(p, x, true);
+  Evaluation fin = evaluate(p, x, true);  // TopologyOptimizer.cpp:263-283
   res.compliance = fin.compliance;
+  res.volume = fin.volume;
+  res.volume_fraction = fin.volume_fraction;
+  res.load_case_compliance = fin.load_case_compliance;
+  double g = 0.0;
+  for (double r : fin.physical) g += r * (1.0 - r);
+  res.gray = 4.0 * g / double(std::max<size_t>(fin.physical.size(), 1));
   res.x = x;
   return res;
 }

@@ -284,17 +284,21 @@ struct RunOptions {
   int continuation_iterations = 25;
   OcOptions oc;
 };
-struct IterationRecord {
+struct IterationRecord {  // one history.csv row (ResultWriter.cpp:1428-1470)
   int iteration;
-  double penalty, compliance, volume_fraction, max_change, lambda, gray;
-  int linear_iterations, bisections;
+  double penalty, compliance, volume, volume_fraction, max_change, lambda, gray;
+  int bisections;
+  bool volume_converged;
+  int linear_iterations;
 };
 struct RunResult {
   std::vector<IterationRecord> history;
   std::vector<double> x;
-  double compliance = 0;
+  double compliance = 0, volume = 0, volume_fraction = 0, gray = 0;
+  std::vector<double> load_case_compliance;
   int iterations = 0;
   bool converged = false;
+  int stop_reason = 0;  // 0 iteration_cap, 1 design_change, 2 objective_stall
 };
 RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0);  // TopologyOptimizer.cpp:556-724
 

@@ -84,7 +84,8 @@ def lib():
                                    C.c_void_p, C.c_void_p]),
         "ref_oc": (C.c_int, [v, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_int)]),
         "ref_run_oc": (C.c_int, [v, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
-                                 C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+                                 C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int)]),
         "ref_chunked_dot": (C.c_double, [_dp, _dp, C.c_int]),
         "ref_cpqr": (C.c_int, [_dp, C.c_int, C.c_int, _dp, _dp]),
         "ref_tridiag_max": (C.c_double, [_dp, _dp, C.c_int]),
@@ -301,14 +302,14 @@ class Oracle:
     def run_oc(self, x0, max_iterations, change_tol=1e-3, objective_tol=0.0, window=5, cont_steps=1,
                penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5):
         x = np.array(x0, dtype=np.float64)
-        hist = np.zeros(max_iterations * 8)
+        hist = np.zeros(max(max_iterations, 1) * 12)
         n = C.c_int()
         fc = C.c_double()
+        stop = C.c_int()
         _check(lib().ref_run_oc(self.h, max_iterations, change_tol, objective_tol, window, cont_steps, penalty_start,
-                                cont_iterations, move, damping, x, hist, C.byref(n), C.byref(fc)))
-        h = hist.reshape(max_iterations, 8)
-        rows = int((h[:, 0] > 0).sum())
-        return x, h[:rows], n.value, fc.value
+                                cont_iterations, move, damping, x, hist, C.byref(n), C.byref(fc), C.byref(stop)))
+        self.last_stop_reason = ("iteration_cap", "design_change", "objective_stall")[stop.value]
+        return x, hist.reshape(-1, 12)[: n.value], n.value, fc.value
 
 
 def chunked_dot(a, b):

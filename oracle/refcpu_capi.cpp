@@ -271,10 +271,11 @@ int ref_oc(void* h, const double* x, const double* dc, const double* dv, double 
   });
 }
 
-// history: max_iterations x 8 (iteration, penalty, compliance, vf, change, lambda, linear its, gray)
+// history: max_iterations x 12, the columns of history.csv (src/io/ResultWriter.cpp:1428-1470);
+// seconds are not timed here
 int ref_run_oc(void* h, int max_iterations, double change_tol, double objective_tol, int window, int cont_steps,
                double penalty_start, int cont_iterations, double move, double damping, double* x_inout,
-               double* history, int* performed, double* final_compliance) {
+               double* history, int* performed, double* final_compliance, int* stop_reason) {
   return guarded([&] {
     Problem* p = P(h);
     RunOptions o;
@@ -291,18 +292,23 @@ int ref_run_oc(void* h, int max_iterations, double change_tol, double objective_
     std::memcpy(x_inout, r.x.data(), r.x.size() * sizeof(double));
     for (size_t i = 0; i < r.history.size(); ++i) {
       const IterationRecord& it = r.history[i];
-      double* row = history + i * 8;
+      double* row = history + i * 12;  // the columns of history.csv
       row[0] = it.iteration;
       row[1] = it.penalty;
       row[2] = it.compliance;
-      row[3] = it.volume_fraction;
-      row[4] = it.max_change;
-      row[5] = it.lambda;
-      row[6] = it.linear_iterations;
+      row[3] = it.volume;
+      row[4] = it.volume_fraction;
+      row[5] = it.max_change;
+      row[6] = it.lambda;
       row[7] = it.gray;
+      row[8] = it.bisections;
+      row[9] = it.volume_converged ? 1.0 : 0.0;
+      row[10] = 0.0;
+      row[11] = it.linear_iterations;
     }
     *performed = r.iterations;
     *final_compliance = r.compliance;
+    *stop_reason = r.stop_reason;
   });
 }
 

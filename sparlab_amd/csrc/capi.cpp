@@ -163,6 +163,7 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
     comp[(size_t)l] = device_dot(c, (long long)nf, c.d_loads_free.p + (size_t)l * nf, c.d_u.p + (size_t)l * nf);
   }
   for (int l = 0; l < c.ncases; ++l) out.compliance += c.weights[(size_t)l] * comp[(size_t)l];
+  c.case_compliance = comp;
   if (c.d_weights.n != c.weights.size()) c.d_weights.upload(c.weights, s);
   k::sensitivity(c.dim, (int)ne, c.nx, c.ny, c.d_ke0.p, c.d_reduced.p, c.d_u.p, (long long)nf, c.ncases, c.d_weights.p, c.d_fac.p,
                  c.d_dfac.p, c.d_energy.p, c.d_dc.p, s);
@@ -826,20 +827,33 @@ slx_status slx_oc_update(slx_context* ctx, const double* x, const double* dc_dx,
   });
 }
 
-slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history, int* performed,
-                      double* final_compliance) {
+slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history,
+                      slx_run_result* result) {
   return guarded(ctx, [&](Context& c) {
     require_model(c);
     ensure_topopt_buffers(c);
     if (!opt) fail(SLX_CONFIG_ERROR, "run options are required");
     const slx_run_options& o = *opt;
+    if (o.max_iterations < 0) fail(SLX_CONFIG_ERROR, "max_iterations must not be negative");
+    const auto t_total = Clock::now();
     const int ne = (int)c.nelem;
     c.d_x.upload(x, (size_t)ne, c.stream);
     k::clamp(ne, c.d_x.p, c.d_lower.p, c.d_upper.p, c.stream);
     std::vector<double> chist;
     std::vector<double> phys((size_t)ne);
+    auto grey = [&]() {
+      c.d_phys.download(phys.data(), (size_t)ne, c.stream);
+      sync(c);
+      double g = 0.0;
+      for (double r : phys) g += r * (1.0 - r);
+      return 4.0 * g / double(std::max(ne, 1));
+    };
+    slx_run_result res{};
+    res.stop_reason = SLX_STOP_ITERATION_CAP;
+    res.final_objective_change = std::numeric_limits<double>::infinity();
     int iteration;
     for (iteration = 1; iteration <= o.max_iterations; ++iteration) {
+      const auto t_iter = Clock::now();
       // penalty continuation (src/topopt/TopologyOptimizer.cpp:245-253)
       slx_eval_options eo = o.eval;
       if (o.continuation_steps > 1) {
@@ -848,26 +862,30 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
         eo.penalty = o.penalty_start + t * (o.eval.penalty - o.penalty_start);
       }
       const slx_evaluation ev = evaluate_device(c, eo, true);
+      res.linear_solves += c.ncases;
+      res.linear_iterations += ev.linear_iterations;
       const slx_oc_step st = oc_device(c, o.oc);
-      c.d_phys.download(phys.data(), (size_t)ne, c.stream);
-      sync(c);
-      double gray = 0.0;
-      for (double r : phys) gray += r * (1.0 - r);
-      gray = 4.0 * gray / double(ne);
+      const double gray = grey();
       if (history) {
-        double* row = history + (size_t)(iteration - 1) * 8;
+        double* row = history + (size_t)(iteration - 1) * SLX_HISTORY_COLUMNS;
         row[0] = iteration;
         row[1] = eo.penalty;
         row[2] = ev.compliance;
-        row[3] = ev.volume_fraction;
-        row[4] = st.max_change;
-        row[5] = st.lambda;
-        row[6] = ev.linear_iterations;
+        row[3] = ev.volume;
+        row[4] = ev.volume_fraction;
+        row[5] = st.max_change;
+        row[6] = st.lambda;
         row[7] = gray;
+        row[8] = st.bisections;
+        row[9] = st.volume_converged ? 1.0 : 0.0;
+        row[10] = std::chrono::duration<double>(Clock::now() - t_iter).count();
+        row[11] = ev.linear_iterations;
       }
       chist.push_back(ev.compliance);
       std::swap(c.d_x, c.d_tmp2);
       k::clamp(ne, c.d_x.p, c.d_lower.p, c.d_upper.p, c.stream);
+      // convergence (src/topopt/TopologyOptimizer.cpp:672-701): only the final
+      // penalty stage may stop the loop
       const bool at_final = o.continuation_steps <= 1 || iteration >= o.continuation_iterations * (o.continuation_steps - 1);
       const bool change_ok = st.max_change <= o.change_tolerance;
       double spread = std::numeric_limits<double>::infinity();
@@ -877,15 +895,35 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
         spread = (*mm.second - *mm.first) / std::max(std::abs(chist.back()), 1.0e-30);
       }
       const bool objective_ok = o.objective_tolerance > 0.0 && spread <= o.objective_tolerance;
-      if (at_final && (change_ok || objective_ok)) break;
+      res.final_design_change = st.max_change;
+      res.final_objective_change = spread;
+      if (at_final && (change_ok || objective_ok)) {
+        res.converged = 1;
+        res.stop_reason = change_ok ? SLX_STOP_DESIGN_CHANGE : SLX_STOP_OBJECTIVE_STALL;
+        break;
+      }
     }
-    const int done = std::min(iteration, o.max_iterations);
+    res.iterations = std::min(iteration, o.max_iterations);
+    // the final evaluation at the returned design (src/topopt/TopologyOptimizer.cpp:263-283)
     slx_eval_options eo = o.eval;
     const slx_evaluation fin = evaluate_device(c, eo, true);
+    res.linear_solves += c.ncases;
+    res.linear_iterations += fin.linear_iterations;
+    res.compliance = fin.compliance;
+    res.volume = fin.volume;
+    res.volume_fraction = fin.volume_fraction;
+    res.grey_level = grey();
     c.d_x.download(x, (size_t)ne, c.stream);
     sync(c);
-    if (performed) *performed = done;
-    if (final_compliance) *final_compliance = fin.compliance;
+    res.total_seconds = std::chrono::duration<double>(Clock::now() - t_total).count();
+    if (result) *result = res;
+  });
+}
+
+slx_status slx_load_case_compliance_get(const slx_context* ctx, double* compliance) {
+  return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
+    if (c.case_compliance.empty()) fail(SLX_CONFIG_ERROR, "no evaluation has run yet");
+    std::copy(c.case_compliance.begin(), c.case_compliance.end(), compliance);
   });
 }
 

@@ -179,6 +179,7 @@ struct Context {
   int ncases = 0;
   std::vector<double> loads;    // ncases x ndofs
   std::vector<double> weights;  // normalised
+  std::vector<double> case_compliance;  // f_l . u_l of the last evaluation
   // device model state
   DBuf<int> d_free, d_reduced;
   DBuf<double> d_ke0, d_scale, d_loads_free, d_weights;  // loads restricted to the free DOFs (ncases x nfree)

@@ -1,0 +1,502 @@
+"""SparLab JSON decks for the structured SIMP path.
+
+Reads the decks ``sparlab_topopt`` takes (src/io/Config.cpp:538-1730) and turns
+the part this library runs -- a structured Q4 / Hex8 mesh, supports, point
+loads, load-case weights, the SIMP law, the density filter, passive regions and
+the optimality-criteria loop with its penalty continuation -- into a
+:class:`~sparlab_amd.model.StructuredModel` and the ``run_oc`` options. Settings
+outside that path (other meshes, tractions, body loads, projection, MMA,
+stress or buckling constraints, ...) raise :class:`DeckError` naming the key,
+rather than being ignored.
+
+The deck syntax follows the reference's JSON reader (src/io/Json.cpp:50-80):
+standard JSON plus ``//`` line comments and ``/* */`` block comments.
+
+Region semantics follow src/fem/Selector.cpp:
+* tolerance: the selector's own ``tolerance`` if positive, else 1e-9 times the
+  mesh bounding-box diagonal (Selector.cpp:12-18,48);
+* node regions test node coordinates, element regions element centroids
+  (Selector.cpp:139-141,189-191);
+* circle / annulus distances are measured in the plane normal to ``axis``
+  (default z; Selector.cpp:38-43, Config.cpp:32-39);
+* ``nearest_node`` keeps the first node at the smallest squared distance
+  (Selector.cpp:117-130);
+* ``any_of`` ORs its members, ``invert`` complements the union
+  (Selector.cpp:143-148, Config.cpp:170-191).
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import math
+import os
+
+import numpy as np
+
+from .model import StructuredModel
+
+
+class DeckError(ValueError):
+    """A deck that is malformed, or asks for something outside this path."""
+
+
+# ---------------------------------------------------------------------------
+# JSON with comments
+# ---------------------------------------------------------------------------
+def strip_comments(text: str) -> str:
+    """Remove // and /* */ comments outside string literals (src/io/Json.cpp:50-80)."""
+    out = []
+    i, n = 0, len(text)
+    while i < n:
+        c = text[i]
+        if c == '"':
+            j = i + 1
+            while j < n and text[j] != '"':
+                j += 2 if text[j] == "\\" else 1
+            out.append(text[i:j + 1])
+            i = j + 1
+        elif c == "/" and i + 1 < n and text[i + 1] == "/":
+            while i < n and text[i] != "\n":
+                i += 1
+        elif c == "/" and i + 1 < n and text[i + 1] == "*":
+            end = text.find("*/", i + 2)
+            if end < 0:
+                raise DeckError("unterminated block comment")
+            # keep the line count so that JSON errors point at the right line
+            out.append("\n" * text.count("\n", i, end))
+            i = end + 2
+        else:
+            out.append(c)
+            i += 1
+    return "".join(out)
+
+
+def read_deck(path: str) -> dict:
+    with open(path, "r", encoding="utf-8") as f:
+        text = f.read()
+    try:
+        doc = json.loads(strip_comments(text))
+    except json.JSONDecodeError as e:
+        raise DeckError(f"{path}: {e}") from None
+    if not isinstance(doc, dict):
+        raise DeckError(f"{path}: the top-level configuration must be a JSON object")
+    return doc
+
+
+# ---------------------------------------------------------------------------
+# Regions
+# ---------------------------------------------------------------------------
+_PRIMITIVES = ("all", "box", "circle", "annulus", "sphere", "node_ids", "element_ids", "nearest_node", "group")
+_AXES = {"x": 0, "y": 1, "z": 2}
+
+
+def _vec3(v, dim, what):
+    if not isinstance(v, (list, tuple)) or len(v) != dim or not all(isinstance(a, (int, float)) for a in v):
+        raise DeckError(f"'{what}' must be a list of {dim} numbers")
+    p = np.zeros(3)
+    p[:dim] = v
+    return p
+
+
+def _primitive(node: dict, dim: int, where: str) -> dict:
+    if not isinstance(node, dict):
+        raise DeckError(f"'{where}' must be an object")
+    names = [k for k in _PRIMITIVES if k in node]
+    if not names:
+        raise DeckError(f"'{where}' does not name a region primitive; expected one of "
+                        + ", ".join(_PRIMITIVES) + " (or an 'any_of' list)")
+    if len(names) > 1:
+        raise DeckError(f"'{where}' names more than one region primitive; use 'any_of' to combine several regions")
+    kind = names[0]
+    sel = {"kind": kind, "tolerance": float(node.get("tolerance", 0.0))}
+    body = node[kind]
+    inf = math.inf
+    if kind == "box":
+        b = body
+        if dim == 2 and ("zmin" in b or "zmax" in b):
+            raise DeckError(f"'{where}.box' sets zmin/zmax but the mesh is two-dimensional")
+        sel["lo"] = np.array([b.get("xmin", -inf), b.get("ymin", -inf), b.get("zmin", -inf)], dtype=float)
+        sel["hi"] = np.array([b.get("xmax", inf), b.get("ymax", inf), b.get("zmax", inf)], dtype=float)
+        if np.any(sel["lo"] > sel["hi"]):
+            raise DeckError(f"'{where}.box' has an empty interval")
+    elif kind in ("circle", "annulus", "sphere"):
+        sel["center"] = _vec3(body.get("center"), dim, f"{where}.{kind}.center")
+        sel["radius"] = float(body.get("radius", 0.0))
+        if not sel["radius"] > 0.0:
+            raise DeckError(f"'{where}.{kind}.radius' must be positive")
+        if kind != "sphere":
+            axis = body.get("axis", "z")
+            if axis not in _AXES:
+                raise DeckError(f"'{where}.{kind}.axis' must be \"x\", \"y\" or \"z\", got \"{axis}\"")
+            sel["axis"] = _AXES[axis]
+            if dim == 2 and sel["axis"] != 2:
+                raise DeckError(f"'{where}.{kind}.axis' must be \"z\" on a two-dimensional mesh")
+        if kind == "annulus":
+            sel["inner"] = float(body.get("inner_radius", 0.0))
+            if sel["inner"] >= sel["radius"]:
+                raise DeckError(f"'{where}.annulus' needs inner_radius strictly below radius")
+    elif kind in ("node_ids", "element_ids"):
+        ids = np.asarray(body, dtype=np.int64)
+        if ids.size == 0:
+            raise DeckError(f"'{where}.{kind}' is empty")
+        sel["ids"] = ids
+    elif kind == "nearest_node":
+        sel["point"] = _vec3(body, dim, f"{where}.nearest_node")
+    elif kind == "group":
+        raise DeckError(f"'{where}.group' names a mesh group; structured meshes carry none")
+    return sel
+
+
+def parse_region(node: dict, dim: int, where: str) -> dict:
+    """Config.cpp:170-191: a list of primitives (``any_of`` or one) and ``invert``."""
+    if not isinstance(node, dict):
+        raise DeckError(f"required region '{where}' is missing")
+    members = node.get("any_of")
+    if members:
+        prims = [_primitive(m, dim, f"{where}.any_of[{i}]") for i, m in enumerate(members)]
+    else:
+        prims = [_primitive(node, dim, where)]
+    return {"name": node.get("name", ""), "invert": bool(node.get("invert", False)), "members": prims}
+
+
+def _contains(sel: dict, x: np.ndarray, tol: float) -> np.ndarray:
+    """Selector::contains (Selector.cpp:46-71) over points x[:, 3]."""
+    t = sel["tolerance"] if sel["tolerance"] > 0.0 else tol
+    kind = sel["kind"]
+    if kind == "all":
+        return np.ones(len(x), dtype=bool)
+    if kind == "box":
+        return np.all((x >= sel["lo"] - t) & (x <= sel["hi"] + t), axis=1)
+    d = x - sel["center"]
+    if kind == "sphere":
+        return np.sqrt((d * d).sum(axis=1)) <= sel["radius"] + t
+    d[:, sel["axis"]] = 0.0
+    r = np.sqrt((d * d).sum(axis=1))
+    if kind == "circle":
+        return r <= sel["radius"] + t
+    return (r >= sel["inner"] - t) & (r <= sel["radius"] + t)
+
+
+class Geometry:
+    """Node coordinates and element centroids of a structured mesh with its origin."""
+
+    def __init__(self, model: StructuredModel, origin=(0.0, 0.0, 0.0)):
+        self.dim = model.dim
+        o = np.asarray(origin, dtype=float)
+        nodes = model.node_coords()
+        cents = model.element_centroids()
+        self.nodes = np.zeros((len(nodes), 3))
+        self.nodes[:, : self.dim] = nodes + o[: self.dim]
+        self.cents = np.zeros((len(cents), 3))
+        self.cents[:, : self.dim] = cents + o[: self.dim]
+        ext = [model.lx, model.ly] + ([model.lz] if model.dim == 3 else [])
+        self.tol = 1.0e-9 * max(math.sqrt(sum(e * e for e in ext)), 1.0e-30)
+
+    def nodes_of(self, region: dict, where: str) -> np.ndarray:
+        """SelectorGroup::select_nodes (Selector.cpp:73-148)."""
+        hit = np.zeros(len(self.nodes), dtype=bool)
+        for sel in region["members"]:
+            k = sel["kind"]
+            if k == "element_ids":
+                raise DeckError(f"region '{where}' uses element_ids where a node region is expected")
+            if k == "node_ids":
+                if sel["ids"].min() < 0 or sel["ids"].max() >= len(self.nodes):
+                    raise DeckError(f"region '{where}' lists a node id outside [0, {len(self.nodes) - 1}]")
+                hit[sel["ids"]] = True
+            elif k == "nearest_node":
+                d2 = ((self.nodes - sel["point"]) ** 2).sum(axis=1)
+                hit[int(np.argmin(d2))] = True  # the first minimum
+            else:
+                hit |= _contains(sel, self.nodes, self.tol)
+        return np.nonzero(hit != region["invert"])[0]
+
+    def elements_of(self, region: dict, where: str) -> np.ndarray:
+        """SelectorGroup::select_elements (Selector.cpp:150-197): centroids."""
+        hit = np.zeros(len(self.cents), dtype=bool)
+        for sel in region["members"]:
+            k = sel["kind"]
+            if k in ("node_ids", "nearest_node"):
+                raise DeckError(f"region '{where}' uses a node-only selector where an element region is expected")
+            if k == "element_ids":
+                if sel["ids"].min() < 0 or sel["ids"].max() >= len(self.cents):
+                    raise DeckError(f"region '{where}' lists an element id outside [0, {len(self.cents) - 1}]")
+                hit[sel["ids"]] = True
+            else:
+                hit |= _contains(sel, self.cents, self.tol)
+        return np.nonzero(hit != region["invert"])[0]
+
+
+# ---------------------------------------------------------------------------
+# The deck
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class Deck:
+    name: str
+    description: str
+    source: str
+    model: StructuredModel
+    origin: tuple
+    case_names: list
+    case_weights: list          # as written in the deck
+    tolerance: float            # relative residual of the linear solves
+    filter_type: str
+    filter_radius: float        # metres, 0 when the filter is off
+    initial_density: float
+    passive_solid: int
+    passive_void: int
+    run: dict                   # keyword arguments of HipProblem.run_oc
+    history_stride: int = 1
+
+    @property
+    def domain_volume(self) -> float:
+        t = self.model.thickness if self.model.dim == 2 else 1.0
+        return self.model.lx * self.model.ly * (self.model.lz if self.model.dim == 3 else t)
+
+    def initial_design(self) -> np.ndarray:
+        """DesignDomain's initial vector (DesignDomain.cpp:29-67): the start density, passive values."""
+        x = np.full(self.model.nelem, self.initial_density)
+        p = self.model.passive.astype(bool)
+        x[p] = self.model.lower[p]
+        return x
+
+
+def _unsupported(path: str, what: str):
+    raise DeckError(f"'{path}' asks for {what}, which this structured SIMP path does not run")
+
+
+def _number(d: dict, key: str, default, where: str):
+    v = d.get(key, default)
+    if isinstance(v, bool) or not isinstance(v, (int, float)):
+        raise DeckError(f"'{where}.{key}' must be a number")
+    return v
+
+
+def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> Deck:
+    """Configuration for sparlab_topopt (src/io/Config.cpp:538-1730, apps/sparlab_topopt.cpp:195-262).
+
+    ``overrides`` takes the command-line overrides of sparlab_topopt:
+    volume_fraction, penalty, filter_radius, filter_radius_elements, filter_type,
+    max_iterations, nx, ny, nz, youngs_modulus, load_weights.
+    """
+    ov = dict(overrides or {})
+    name = doc.get("name", "case")
+    mesh = doc.get("mesh")
+    if not isinstance(mesh, dict):
+        raise DeckError("'mesh' is missing")
+    mtype = mesh.get("type", "structured_quad")
+    if mtype not in ("structured_quad", "structured_hex"):
+        _unsupported("mesh.type", f"a {mtype} mesh")
+    dim = 2 if mtype == "structured_quad" else 3
+    nx = int(ov.get("nx") or mesh.get("nx", 0))
+    ny = int(ov.get("ny") or mesh.get("ny", 0))
+    nz = int(ov.get("nz") or mesh.get("nz", 0)) if dim == 3 else 0
+    if dim == 2 and ov.get("nz"):
+        raise DeckError("--nz applies to a solid (structured_hex) mesh only")
+    if nx <= 0 or ny <= 0 or (dim == 3 and nz <= 0):
+        raise DeckError("'mesh' needs positive nx, ny" + (", nz" if dim == 3 else ""))
+    lx = float(mesh.get("lx", 0.0))
+    ly = float(mesh.get("ly", 0.0))
+    lz = float(mesh.get("lz", 0.0)) if dim == 3 else 1.0
+    if not (lx > 0 and ly > 0 and lz > 0):
+        raise DeckError("'mesh' needs positive lx, ly" + (", lz" if dim == 3 else ""))
+    origin = (float(mesh.get("x0", 0.0)), float(mesh.get("y0", 0.0)), float(mesh.get("z0", 0.0)))
+
+    mat = doc.get("material")
+    if not isinstance(mat, dict):
+        raise DeckError("'material' is missing")
+    if "material_regions" in doc:
+        _unsupported("material_regions", "a second material")
+    E = float(ov.get("youngs_modulus") or _number(mat, "youngs_modulus", 0.0, "material"))
+    nu = float(_number(mat, "poisson_ratio", 0.3, "material"))
+    if not E > 0:
+        raise DeckError("'material.youngs_modulus' must be positive")
+
+    model_node = doc.get("model", {})
+    thickness = float(model_node.get("thickness", 1.0))
+    if dim == 3 and thickness != 1.0:
+        raise DeckError("'model.thickness' does not apply to a solid mesh; remove the key")
+    state = model_node.get("stress_state", "plane_stress" if dim == 2 else "three_dimensional")
+    if dim == 2 and state != "plane_stress":
+        _unsupported("model.stress_state", f"the stress state {state}")
+    if dim == 3 and state not in ("three_dimensional", "3d"):
+        raise DeckError(f"'model.stress_state' = \"{state}\" does not fit a structured_hex mesh")
+    points = model_node.get("integration", {}).get("stiffness_points", 2)
+    if points != 2:
+        _unsupported("model.integration.stiffness_points", f"{points}-point stiffness integration")
+
+    topo = doc.get("topology", {})
+    if not topo.get("enabled", False):
+        raise DeckError("'topology.enabled' is not true; this driver runs topology optimisation decks")
+    vf = float(ov.get("volume_fraction") or topo.get("volume_fraction", 0.4))
+    if not (0.0 < vf <= 1.0):
+        raise DeckError(f"volume fraction must lie in (0, 1] (got {vf})")
+    simp = topo.get("simp", {})
+    penalty = float(ov.get("penalty") or simp.get("penalty", 3.0))
+    emin = float(simp.get("emin_ratio", 1e-9))
+
+    m = StructuredModel(dim=dim, nx=nx, ny=ny, nz=nz, lx=lx, ly=ly, lz=lz, E=E, nu=nu, thickness=thickness,
+                        volume_fraction=vf, penalty=penalty, emin_ratio=emin)
+    geo = Geometry(m, origin)
+
+    # supports (Config.cpp:879-941)
+    bcs = doc.get("boundary_conditions") or []
+    if not bcs:
+        raise DeckError("'boundary_conditions' is missing or empty; an unconstrained model has a singular "
+                        "stiffness matrix")
+    for i, bc in enumerate(bcs):
+        where = f"boundary_conditions[{i}]"
+        fix = bc.get("fix") or []
+        if not fix:
+            raise DeckError(f"'{where}.fix' must list the components to constrain")
+        comps = []
+        for c in fix:
+            if c == "z" and dim == 2:
+                raise DeckError(f"'{where}.fix' fixes \"z\" but the mesh is two-dimensional")
+            if c not in ("x", "y", "z"):
+                raise DeckError(f"'{where}.fix' must hold \"x\", \"y\"" + (", \"z\"" if dim == 3 else "")
+                                + f", got \"{c}\"")
+            comps.append(_AXES[c])
+        value = bc.get("value")
+        if value is not None and any(float(v) != 0.0 for v in value):
+            _unsupported(f"{where}.value", "a prescribed non-zero displacement")
+        nodes = geo.nodes_of(parse_region(bc.get("region"), dim, f"{where}.region"), bc.get("name", f"bc{i}"))
+        if len(nodes) == 0:
+            raise DeckError(f"boundary condition '{bc.get('name', f'bc{i}')}' selected no nodes")
+        m.fix(nodes, comps)
+
+    # load cases (Config.cpp:943-1070)
+    cases = doc.get("load_cases") or []
+    if not cases:
+        raise DeckError("'load_cases' is missing or empty; define at least one")
+    names, weights = [], []
+    w_over = ov.get("load_weights")
+    if w_over is not None and len(w_over) != len(cases):
+        raise DeckError(f"--load-weights supplied {len(w_over)} weight(s) but the deck defines {len(cases)} "
+                        "load case(s)")
+    for li, lc in enumerate(cases):
+        where = f"load_cases[{li}]"
+        for key in ("tractions", "pressures", "body_forces", "centrifugal", "temperature", "springs",
+                    "prescribed_displacements"):
+            if lc.get(key):
+                _unsupported(f"{where}.{key}", key.replace("_", " "))
+        g = lc.get("gravity")
+        if g is not None and any(float(v) != 0.0 for v in g):
+            _unsupported(f"{where}.gravity", "self-weight")
+        cname = lc.get("name", f"case{li}")
+        w = float(w_over[li]) if w_over is not None else float(lc.get("weight", 1.0))
+        if not w >= 0.0:
+            raise DeckError(f"'{where}.weight' must be non-negative, got {w}")
+        case = m.add_load_case(w)
+        for pi, pl in enumerate(lc.get("point_loads") or []):
+            pw = f"{where}.point_loads[{pi}]"
+            if "moment" in pl:
+                _unsupported(f"{pw}.moment", "a point moment")
+            force = _vec3(pl.get("force"), dim, f"{pw}.force")[:dim]
+            dist = pl.get("distribution", "total")
+            if dist not in ("total", "per_node"):
+                raise DeckError(f"'{pw}.distribution' must be 'total' or 'per_node', got '{dist}'")
+            lname = pl.get("name", f"{cname}_point{pi}")
+            nodes = geo.nodes_of(parse_region(pl.get("region"), dim, f"{pw}.region"), lname)
+            if len(nodes) == 0:
+                raise DeckError(f"point load '{lname}' selected no nodes")
+            m.add_point_load(case, nodes, force, total=(dist == "total"))
+        names.append(cname)
+        weights.append(w)
+    if sum(weights) <= 0.0:
+        raise DeckError("the load-case weights sum to zero")
+
+    # the optimisation problem (Config.cpp:1542-1730)
+    for key, what in (("projection", "Heaviside projection"), ("stress", "a stress constraint"),
+                      ("overhang", "an overhang filter"), ("buckling_constraint", "a buckling constraint"),
+                      ("length_scale", "a length-scale check")):
+        node = topo.get(key)
+        if isinstance(node, dict) and node.get("enabled", False):
+            _unsupported(f"topology.{key}", what)
+    opt = topo.get("optimizer", {})
+    method = opt.get("method", "oc")
+    if method != "oc":
+        _unsupported("topology.optimizer.method", f"the {method} update")
+    filt = topo.get("filter", {})
+    ftype = str(ov.get("filter_type") or filt.get("type", "density")).lower()
+    if ftype == "off":
+        ftype = "none"
+    if ftype == "sensitivity":
+        _unsupported("topology.filter.type", "the sensitivity filter")
+    if ftype not in ("none", "density"):
+        raise DeckError(f"unknown filter type '{ftype}' (expected none|density|sensitivity)")
+    radius = float(filt.get("radius", 0.0))
+    radius_elements = float(filt.get("radius_elements", 1.5))
+    if "filter_radius" in ov:
+        radius, radius_elements = float(ov["filter_radius"]), 0.0
+    if "filter_radius_elements" in ov:
+        radius, radius_elements = 0.0, float(ov["filter_radius_elements"])
+    if ftype == "none":
+        radius = 0.0
+    elif not radius > 0.0:
+        # Configuration::resolved_filter_radius (Config.cpp:360-370)
+        if not radius_elements > 0.0:
+            raise DeckError("topology.filter needs either 'radius' (metres) or a positive 'radius_elements'")
+        radius = radius_elements * m.mean_element_size()
+    m.filter_radius = radius
+
+    # passive regions (Config.cpp:1710-1727, DesignDomain.cpp:40-67)
+    tag = np.zeros(m.nelem, dtype=np.int8)  # 0 free, 1 solid, 2 void
+    for pi, pr in enumerate(topo.get("passive_regions") or []):
+        pname = pr.get("name", f"passive{pi}")
+        kind = pr.get("type", "solid")
+        if kind not in ("solid", "void"):
+            raise DeckError(f"'topology.passive_regions[{pi}].type' must be 'solid' or 'void', got '{kind}'")
+        solid = kind == "solid"
+        density = float(pr.get("density", 0.0))
+        if not solid and not (0.0 <= density < 1.0):
+            raise DeckError(f"passive void region '{pname}' needs a void density in [0, 1) (got {density})")
+        els = geo.elements_of(parse_region(pr.get("region"), dim, f"topology.passive_regions[{pi}].region"), pname)
+        if len(els) == 0:
+            raise DeckError(f"passive region '{pname}' selected no elements; check its geometry against the design "
+                            "domain extents")
+        t = 1 if solid else 2
+        clash = (tag[els] != 0) & (tag[els] != t)
+        if clash.any():
+            raise DeckError(f"element {int(els[clash][0])} is claimed by both a passive solid and a passive void "
+                            f"region (last one: '{pname}'); make the regions disjoint")
+        tag[els] = t
+        m.set_passive(els, solid, void_density=density)
+    if not (tag == 0).any():
+        raise DeckError("every element is passive; there is nothing left to optimise")
+    vol = (lx * ly * thickness) if dim == 2 else lx * ly * lz
+    ev = vol / m.nelem  # every cell of a structured mesh has the same measure
+    target = vf * vol
+    solid_volume = float((tag == 1).sum()) * ev
+    if solid_volume > target:
+        raise DeckError(f"the passive solid regions already occupy {solid_volume} m^3, more than the volume target "
+                        f"of {target} m^3")
+    if solid_volume + float((tag == 0).sum()) * ev < target:
+        raise DeckError(f"the volume target {target} m^3 is unreachable with the passive void regions")
+
+    init = float(topo.get("initial_density", -1.0))
+    start = init if init >= 0.0 else vf
+    if not (0.0 <= start <= 1.0):
+        raise DeckError(f"initial density must lie in [0, 1] (got {start})")
+
+    # TopologyOptimizerOptions (Config.cpp:1561-1576)
+    run = dict(
+        max_iterations=int(ov.get("max_iterations") or opt.get("max_iterations", 200)),
+        change_tol=float(opt.get("change_tolerance", 1e-2)),
+        objective_tol=float(opt.get("objective_tolerance", 0.0)),
+        window=int(opt.get("objective_window", 5)),
+        cont_steps=int(opt.get("continuation_steps", 1)),
+        penalty_start=float(opt.get("penalty_start", 1.0)),
+        cont_iterations=int(opt.get("continuation_iterations", 25)),
+        move=float(opt.get("move_limit", 0.2)),
+        damping=float(opt.get("damping", 0.5)),
+    )
+    lin = doc.get("solver", {}).get("linear", {})
+    tol = float(lin.get("residual_tolerance", 1e-8))
+    return Deck(name=name, description=doc.get("description", ""), source=source, model=m, origin=origin,
+                case_names=names, case_weights=weights, tolerance=tol, filter_type=ftype, filter_radius=radius,
+                initial_density=start, passive_solid=int((tag == 1).sum()), passive_void=int((tag == 2).sum()),
+                run=run, history_stride=int(opt.get("history_stride", 1)))
+
+
+def load_deck(path: str, overrides: dict | None = None) -> Deck:
+    return parse_deck(read_deck(path), source=os.path.abspath(path), overrides=overrides)

@@ -96,6 +96,27 @@ class RunOptions(C.Structure):
                 ("continuation_iterations", C.c_int), ("oc", OcOptions), ("eval", EvalOptions)]
 
 
+class RunResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("stop_reason", C.c_int),
+                ("compliance", C.c_double), ("volume", C.c_double), ("volume_fraction", C.c_double),
+                ("grey_level", C.c_double), ("final_design_change", C.c_double),
+                ("final_objective_change", C.c_double), ("linear_solves", C.c_longlong),
+                ("linear_iterations", C.c_longlong), ("total_seconds", C.c_double)]
+
+
+# slx_stop_reason -> the reference's stop_reason strings (TopologyOptimizer.hpp:206,
+# TopologyOptimizer.cpp:692)
+STOP_REASONS = ("iteration_cap", "design_change", "objective_stall")
+
+# columns of a run_oc history row, the order of the reference's history.csv
+# (src/io/ResultWriter.cpp:1428-1470)
+HISTORY_COLUMNS = ("iteration", "penalty[-]", "compliance[J]", "volume[m3]", "volume_fraction[-]",
+                   "max_design_change[-]", "lagrange_multiplier[-]", "grey_level[-]", "oc_bisections[-]",
+                   "volume_converged[-]", "seconds[s]", "linear_iterations[-]")
+H_ITER, H_PENALTY, H_COMPLIANCE, H_VOLUME, H_VF, H_CHANGE, H_LAMBDA, H_GREY, H_BISECT, H_VCONV, H_SECONDS, H_LINEAR = \
+    range(len(HISTORY_COLUMNS))
+
+
 class WarmStartOptions(C.Structure):
     _fields_ = [("enable", C.c_int), ("train_solves", C.c_int), ("min_samples", C.c_int),
                 ("steps_per_solve", C.c_int), ("buffer", C.c_int), ("learning_rate", C.c_double),
@@ -153,7 +174,8 @@ _SIGNATURES = {
     "slx_filter_apply": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "slx_evaluate": (C.c_int, [_VP, _VP, C.POINTER(EvalOptions), C.POINTER(Evaluation), _VP, _VP, _VP]),
     "slx_oc_update": (C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(OcOptions), _VP, C.POINTER(OcStep)]),
-    "slx_run_oc": (C.c_int, [_VP, C.POINTER(RunOptions), _VP, _VP, _VP, _VP]),
+    "slx_run_oc": (C.c_int, [_VP, C.POINTER(RunOptions), _VP, _VP, C.POINTER(RunResult)]),
+    "slx_load_case_compliance_get": (C.c_int, [_VP, _VP]),
     "slx_warmstart_default_options": (None, [C.POINTER(WarmStartOptions)]),
     "slx_warmstart_configure": (C.c_int, [_VP, C.POINTER(WarmStartOptions)]),
     "slx_warmstart_stats_get": (C.c_int, [_VP, C.POINTER(WarmStartStats)]),
@@ -581,6 +603,8 @@ class HipProblem:
 
     def run_oc(self, x0, max_iterations, change_tol=1e-3, objective_tol=0.0, window=5, cont_steps=1,
                penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5, tol=1e-8):
+        """TopologyOptimizer::run_oc: returns (x, history, iterations, final compliance);
+        the full result of the last call is in ``last_run``."""
         self._own()
         o = RunOptions()
         o.max_iterations = max_iterations
@@ -593,10 +617,16 @@ class HipProblem:
         o.oc = oc_options(move_limit=move, damping=damping)
         o.eval = eval_options(tolerance=tol, penalty=self.model.penalty, emin_ratio=self.model.emin_ratio)
         x = _f64(x0).copy()
-        hist = np.zeros(max_iterations * 8)
-        n = C.c_int()
-        fc = C.c_double()
-        self.ctx.check(load().slx_run_oc(self.ctx.h, C.byref(o), _p(x), _p(hist), C.byref(n), C.byref(fc)))
-        h = hist.reshape(max_iterations, 8)
-        rows = int((h[:, 0] > 0).sum())
-        return x, h[:rows], n.value, fc.value
+        hist = np.zeros(max(max_iterations, 1) * len(HISTORY_COLUMNS))
+        r = RunResult()
+        self.ctx.check(load().slx_run_oc(self.ctx.h, C.byref(o), _p(x), _p(hist), C.byref(r)))
+        h = hist.reshape(-1, len(HISTORY_COLUMNS))[: r.iterations]
+        self.last_run = {name: getattr(r, name) for name, _ in RunResult._fields_}
+        self.last_run["stop_reason"] = STOP_REASONS[r.stop_reason]
+        return x, h, r.iterations, r.compliance
+
+    def load_case_compliance(self):
+        """f_l . u_l of each load case at the last evaluation."""
+        out = np.zeros(self.model.loads.shape[0])
+        self.ctx.check(load().slx_load_case_compliance_get(self.ctx.h, _p(out)))
+        return out

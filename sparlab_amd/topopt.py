@@ -1,0 +1,240 @@
+"""Command-line driver for SIMP topology optimisation on one GPU.
+
+    python -m sparlab_amd.topopt --config deck.json [--output dir] [overrides]
+
+The counterpart of the reference's ``sparlab_topopt`` (apps/sparlab_topopt.cpp)
+for the decks this library runs (structured Q4 / Hex8, point loads, density
+filter, OC with penalty continuation; see :mod:`sparlab_amd.deck`). The deck is
+read unchanged; the linear solves run on the GPU's AMG-preconditioned CG
+whatever ``solver.linear.type`` the deck names, to the deck's
+``residual_tolerance`` unless ``--tolerance`` says otherwise.
+
+Outputs in the output directory (src/io/ResultWriter.cpp):
+
+* ``history.csv``: one row per iteration, the reference's columns
+  (ResultWriter.cpp:1428-1470);
+* ``density_final.csv``: per element centroid, volume, design, physical
+  density, stiffness factor, passive tag and strain energy
+  (ResultWriter.cpp:1474-1515);
+* ``summary.json``: provenance, mesh, material, ``optimization_setup``,
+  ``optimization_result`` (ResultWriter.cpp:2560-2810) and
+  ``mass_stiffness_comparison`` (apps/sparlab_topopt.cpp:438-466,873-930).
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import deck as D
+
+_OVERRIDES = (("volume-fraction", float), ("penalty", float), ("filter-radius", float),
+              ("filter-radius-elements", float), ("filter-type", str), ("max-iterations", int), ("nx", int),
+              ("ny", int), ("nz", int), ("youngs-modulus", float))
+_PRECISION = {"fp64": 0, "mixed": 1, "bf16": 2}
+
+
+def _fmt(v) -> str:
+    """CsvWriter's 17 significant digits (include/sparlab/io/CsvWriter.hpp:21-22)."""
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    return format(float(v), ".17g")
+
+
+def write_csv(path, header, rows):
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f, lineterminator="\n")
+        w.writerow(header)
+        for r in rows:
+            w.writerow([_fmt(v) for v in r])
+
+
+def _num(v):
+    """JSON has no infinity: the reference's writer emits null for it."""
+    v = float(v)
+    return v if math.isfinite(v) else None
+
+
+def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str = "fp64",
+        tolerance: float | None = None, tag: str = "", write_csv_files: bool = True, log=print) -> dict:
+    """Optimise one deck on the GPU and write its outputs; returns the summary."""
+    from . import native as N
+
+    m = dk.model
+    tol = dk.tolerance if tolerance is None else tolerance
+    t_all = time.perf_counter()
+    g = N.HipProblem(m, device=device)
+    x0 = dk.initial_design()
+    if precision != "fp64":
+        # the options take effect at a setup: run it on the first iteration's
+        # operator, where the loop's first setup would aggregate anyway
+        p0 = dk.run["penalty_start"] if dk.run["cont_steps"] > 1 else m.penalty
+        g.assemble(m.emin_ratio + (1.0 - m.emin_ratio) * np.clip(g.filter(x0), 0.0, 1.0) ** p0)
+        g.amg_setup(precision=_PRECISION[precision])
+    log(f"{dk.name}: {m.nelem} elements, {g.nfree} free DOFs, {len(dk.case_names)} load case(s), "
+        f"filter radius {dk.filter_radius:.6g} m, precision {precision}")
+    x, hist, iters, comp = g.run_oc(x0, dk.run["max_iterations"], change_tol=dk.run["change_tol"],
+                                    objective_tol=dk.run["objective_tol"], window=dk.run["window"],
+                                    cont_steps=dk.run["cont_steps"], penalty_start=dk.run["penalty_start"],
+                                    cont_iterations=dk.run["cont_iterations"], move=dk.run["move"],
+                                    damping=dk.run["damping"], tol=tol)
+    res = dict(g.last_run)
+    case_c = g.load_case_compliance()
+    st = g.amg_stats()
+    log(f"{dk.name}: {iters} iterations ({res['stop_reason']}), compliance {comp:.10g} J, "
+        f"volume fraction {res['volume_fraction']:.10g}, {res['total_seconds']:.2f} s")
+
+    # the final design's fields for density_final.csv (the run's last evaluation, repeated)
+    fin = g.evaluate(x, tol=tol, need_gradients=False)
+    phys = g.filter(x)
+    factor = m.emin_ratio + phys ** m.penalty * (1.0 - m.emin_ratio)
+    # the full solid domain (apps/sparlab_topopt.cpp:438-466)
+    solid = g.evaluate(np.ones(m.nelem), tol=tol, need_gradients=False, warm_start=False)["compliance"]
+    vf = res["volume_fraction"]
+
+    wsum = sum(dk.case_weights)
+    vol = dk.domain_volume
+    target = m.volume_fraction * vol
+    summary = {
+        "provenance": {
+            "code": "sparlab_amd", "config_source": dk.source, "case": dk.name + (("_" + tag) if tag else ""),
+            "description": dk.description, "units": "SI: m, N, Pa, kg, kg/m^3, Hz, J",
+            "timestamp_utc": time.strftime("%Y-%m-%dT%H:%M:%SZ", time.gmtime()), "device": f"cuda:{device}",
+        },
+        "mesh": {
+            "source": "structured_quad" if m.dim == 2 else "structured_hex",
+            "element_type": "Quad4" if m.dim == 2 else "Hex8", "dim": m.dim, "num_nodes": m.nnodes,
+            "num_elements": m.nelem, "num_dofs": m.ndofs, "num_free_dofs": g.nfree,
+            "num_prescribed_dofs": int(m.fixed.sum()), "thickness_m": m.thickness if m.dim == 2 else 1.0,
+            "domain_volume_m3": vol,
+            "structured_grid": {"nx": m.nx, "ny": m.ny, **({"nz": m.nz} if m.dim == 3 else {}), "lx_m": m.lx,
+                                "ly_m": m.ly, **({"lz_m": m.lz} if m.dim == 3 else {})},
+        },
+        "material": {"youngs_modulus_Pa": m.E, "poisson_ratio": m.nu},
+        "optimization_setup": {
+            "load_case_names": dk.case_names, "load_case_weights": dk.case_weights,
+            "load_case_weights_normalised": [w / wsum for w in dk.case_weights],
+            "volume_fraction_target": m.volume_fraction, "volume_target_m3": target,
+            "initial_density": dk.initial_density, "domain_volume_m3": vol, "num_design_variables": m.nelem,
+            "num_free_variables": m.nelem - dk.passive_solid - dk.passive_void,
+            "num_passive_solid": dk.passive_solid, "num_passive_void": dk.passive_void,
+            "filter_type": dk.filter_type, "filter_radius_m": dk.filter_radius, "simp_penalty": m.penalty,
+            "simp_emin_ratio": m.emin_ratio, "move_limit": dk.run["move"], "damping": dk.run["damping"],
+            "continuation_steps": dk.run["cont_steps"], "penalty_start": dk.run["penalty_start"],
+            "continuation_iterations": dk.run["cont_iterations"], "max_iterations": dk.run["max_iterations"],
+            "change_tolerance": dk.run["change_tol"], "objective_tolerance": dk.run["objective_tol"],
+            "objective_window": dk.run["window"], "method": "oc",
+        },
+        "optimization_result": {
+            "converged": bool(res["converged"]), "stop_reason": res["stop_reason"],
+            "final_design_change": _num(res["final_design_change"]),
+            "final_relative_objective_change": _num(res["final_objective_change"]),
+            "iterations": iters, "linear_solves": res["linear_solves"],
+            "linear_solver": {
+                "solver": "amg_cg (HIP, gfx950)", "precision": precision, "residual_tolerance": tol,
+                "iterative_iterations_total": res["linear_iterations"],
+                "iterative_iterations_per_solve": res["linear_iterations"] / max(res["linear_solves"], 1),
+                "multigrid": {
+                    "levels": st.levels,
+                    "level_sizes": [{"unknowns": int(st.unknowns[i]), "nonzeros": int(st.nonzeros[i]),
+                                     "lambda_max_Dinv_A": st.lambda_max[i]} for i in range(st.levels)],
+                    "operator_complexity": st.operator_complexity, "grid_complexity": st.grid_complexity,
+                    "near_null_space_dimension": st.near_null_space_dimension,
+                    "last_setup_seconds": st.setup_seconds,
+                    "aggregates_reused_in_last_setup": bool(st.reused_aggregates),
+                },
+            },
+            "compliance_J": comp, "load_case_compliance_J": [float(c) for c in case_c],
+            "volume_m3": res["volume"], "volume_fraction": vf,
+            "volume_constraint_relative_violation": (res["volume"] - target) / target,
+            "volume_constraint_satisfied": (res["volume"] - target) / target <= 1.0e-6,
+            "grey_level": res["grey_level"], "total_seconds": res["total_seconds"],
+            "seconds_per_iteration": res["total_seconds"] / iters if iters else 0.0, "warnings": [],
+        },
+        "mass_stiffness_comparison": {
+            "full_solid_compliance_J": solid, "optimised_compliance_J": comp,
+            "compliance_penalty_vs_full_solid": comp / solid if solid > 0 else 0.0,
+        },
+    }
+    if len(hist):
+        summary["optimization_result"]["initial_compliance_J"] = float(hist[0, N.H_COMPLIANCE])
+        summary["optimization_result"]["compliance_reduction_factor"] = (
+            float(hist[0, N.H_COMPLIANCE]) / comp if comp > 0 else 0.0)
+    if m.dim == 2:
+        plate = solid / vf if vf > 0 else 0.0
+        summary["mass_stiffness_comparison"]["equal_mass_uniform_plate_compliance_J"] = plate
+        summary["mass_stiffness_comparison"]["stiffness_gain_over_equal_mass_plate"] = plate / comp if comp > 0 else 0.0
+
+    if output:
+        os.makedirs(output, exist_ok=True)
+        if write_csv_files:
+            write_csv(os.path.join(output, "history.csv"), list(N.HISTORY_COLUMNS),
+                      [[int(r[0])] + list(r[1:N.H_BISECT]) + [int(r[N.H_BISECT]), int(r[N.H_VCONV])]
+                       + [r[N.H_SECONDS], int(r[N.H_LINEAR])] for r in hist])
+            cents = D.Geometry(m, dk.origin).cents[:, : m.dim]
+            ev = vol / m.nelem
+            tags = np.zeros(m.nelem, dtype=int)
+            pas = m.passive.astype(bool)
+            tags[pas & (m.lower >= 1.0)] = 1
+            tags[pas & (m.lower < 1.0)] = 2
+            header = ["element"] + ["cx[m]", "cy[m]", "cz[m]"][: m.dim] + (["area[m2]"] if m.dim == 2 else []) + [
+                "volume[m3]", "design_x[-]", "physical_density[-]", "stiffness_factor[-]", "passive_tag[-]",
+                "strain_energy[J]"]
+            area = m.lx * m.ly / m.nelem
+            rows = []
+            for e in range(m.nelem):
+                rows.append([e] + list(cents[e]) + ([area] if m.dim == 2 else []) +
+                            [ev, x[e], phys[e], factor[e], int(tags[e]), fin["energy"][e]])
+            write_csv(os.path.join(output, "density_final.csv"), header, rows)
+        with open(os.path.join(output, "summary.json"), "w") as f:
+            json.dump(summary, f, indent=2)
+        log(f"{dk.name}: wrote {output}")
+    summary["_seconds_wall"] = time.perf_counter() - t_all
+    return summary
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m sparlab_amd.topopt",
+                                 description="SIMP topology optimisation of a SparLab deck on one GPU")
+    ap.add_argument("--config", required=True, help="input deck with a topology section")
+    ap.add_argument("--output", help="output directory (default results/<case>)")
+    for name, typ in _OVERRIDES:
+        ap.add_argument("--" + name, type=typ)
+    ap.add_argument("--load-weights", help="override the per-load-case weights, w1,w2,...")
+    ap.add_argument("--tag", default="", help="suffix appended to the case name in summary.json")
+    ap.add_argument("--no-csv", action="store_true", help="skip the CSV outputs")
+    ap.add_argument("--no-vtk", action="store_true", help="accepted for compatibility; no VTK is written")
+    ap.add_argument("--solver", help="accepted for compatibility; the GPU solver is always amg_cg")
+    ap.add_argument("--precision", choices=sorted(_PRECISION), default="fp64",
+                    help="multigrid hierarchy precision (fp64: the drop-in path)")
+    ap.add_argument("--tolerance", type=float, help="relative residual of the CG solves (default: the deck's)")
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    ov = {}
+    for name, _ in _OVERRIDES:
+        v = getattr(a, name.replace("-", "_"))
+        if v is not None:
+            ov[name.replace("-", "_")] = v
+    if a.load_weights:
+        ov["load_weights"] = [float(w) for w in a.load_weights.split(",")]
+    if a.solver and a.solver != "amg_cg":
+        print(f"note: solver.linear.type {a.solver} is replaced by the GPU's amg_cg", file=sys.stderr)
+    try:
+        dk = D.load_deck(a.config, overrides=ov)
+    except D.DeckError as e:
+        print(f"configuration error: {e}", file=sys.stderr)
+        return 2
+    out = a.output or os.path.join("results", dk.name + (("_" + a.tag) if a.tag else ""))
+    run(dk, out, device=a.device, precision=a.precision, tolerance=a.tolerance, tag=a.tag,
+        write_csv_files=not a.no_csv, log=lambda s: print(s, file=sys.stderr, flush=True))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -154,7 +154,7 @@ def test_short_oc_run_against_oracle(ctx):
     xo, ho, no, fo = o.run_oc(x0, 15, change_tol=0.0)
     assert ng == no == 15
     np.testing.assert_allclose(hg[:, 2], ho[:, 2], rtol=1e-8)
-    np.testing.assert_allclose(hg[:, 3], ho[:, 3], rtol=1e-8)
+    np.testing.assert_allclose(hg[:, 4], ho[:, 4], rtol=1e-8)
     assert fg == pytest.approx(fo, rel=1e-7)
 
 
@@ -169,7 +169,7 @@ def test_c1_mbb_120x40_hundred_oc_iterations(ctx):
     xo, ho, no, fo = o.run_oc(x0, 100, change_tol=0.0)
     assert ng == no == 100
     np.testing.assert_allclose(hg[:, 2], ho[:, 2], rtol=1e-6)
-    np.testing.assert_allclose(hg[:, 3], ho[:, 3], rtol=1e-8)
+    np.testing.assert_allclose(hg[:, 4], ho[:, 4], rtol=1e-8)
     assert fg == pytest.approx(fo, rel=1e-6)
     assert np.abs(xg - xo).max() < 1e-4
 

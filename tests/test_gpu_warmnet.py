@@ -113,8 +113,8 @@ def test_learned_warm_start_in_an_oc_run(ctx):
         ctx2.close()
     k = st["predictions"]
     ratios = {name: st[f"{name}_ratio_sum"] / k for name in ("energy", "net", "secant")}
-    print("plain iterations", hp[:, 6].astype(int).tolist(), flush=True)
-    print("learned iterations", hl[:, 6].astype(int).tolist(), flush=True)
+    print("plain iterations", hp[:, 11].astype(int).tolist(), flush=True)
+    print("learned iterations", hl[:, 11].astype(int).tolist(), flush=True)
     print("warm-start stats", st, "mean K-norm ratios", ratios, flush=True)
     assert st["samples"] == train and st["train_steps"] == train * 24
     assert st["last_loss"] < st["first_loss"]
@@ -122,6 +122,6 @@ def test_learned_warm_start_in_an_oc_run(ctx):
     assert ratios["net"] < 1.0                             # the trained direction alone beats u_prev
     assert ratios["energy"] <= ratios["secant"] + 1e-9     # and never hurts the combination
     assert ratios["energy"] < 0.8
-    assert hl[train + 1:, 6].sum() <= hp[train + 1:, 6].sum()
+    assert hl[train + 1:, 11].sum() <= hp[train + 1:, 11].sum()
     np.testing.assert_allclose(hl[:, 2], hp[:, 2], rtol=1e-4)
     assert fl == pytest.approx(fp, rel=1e-4)

@@ -451,4 +451,4 @@ def test_short_oc_run_improves_design():
     o = Oracle(m)
     x, hist, n, fc = o.run_oc(m.initial_design(), 40, change_tol=0.01)
     assert hist[0, 2] / fc > 1.5
-    assert abs(hist[-1, 3] - 0.4) < 1e-6
+    assert abs(hist[-1, 4] - 0.4) < 1e-6
