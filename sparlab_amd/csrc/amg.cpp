@@ -22,7 +22,6 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
-#include <future>
 #include <sstream>
 
 #include "slx_kernels.hpp"
@@ -408,49 +407,6 @@ double tridiag_max(const std::vector<double>& d, const std::vector<double>& e) {
     if (below(mid) >= (int)k) hi = mid; else lo = mid;
   }
   return hi;
-}
-
-// Dense LDL^T of the coarsest operator, its pivot ratio (Multigrid.cpp:766-781)
-// and its inverse, row-major.
-void coarse_inverse(int n, const std::vector<double>& dense, std::vector<double>& inv, double& ratio) {
-  std::vector<double> L(dense);
-  std::vector<double> D((size_t)n);
-  for (int k = 0; k < n; ++k) {
-    double dk = L[(size_t)k * n + k];
-    for (int j = 0; j < k; ++j) dk -= L[(size_t)k * n + j] * L[(size_t)k * n + j] * D[(size_t)j];
-    D[(size_t)k] = dk;
-    if (dk == 0.0) continue;
-#pragma omp parallel for schedule(static) if (n - k > 256)
-    for (int i = k + 1; i < n; ++i) {
-      double s = L[(size_t)i * n + k];
-      for (int j = 0; j < k; ++j) s -= L[(size_t)i * n + j] * L[(size_t)k * n + j] * D[(size_t)j];
-      L[(size_t)i * n + k] = s / dk;
-    }
-  }
-  double mx = 0.0, mn = std::numeric_limits<double>::max();
-  for (double v : D) {
-    mx = std::max(mx, std::abs(v));
-    mn = std::min(mn, v);
-  }
-  ratio = mx > 0.0 ? mn / mx : 0.0;
-  inv.assign((size_t)n * n, 0.0);
-#pragma omp parallel for schedule(dynamic, 8)
-  for (int c = 0; c < n; ++c) {
-    std::vector<double> y((size_t)n, 0.0);
-    y[(size_t)c] = 1.0;
-    for (int i = c + 1; i < n; ++i) {  // L y = e_c
-      double s = 0.0;
-      for (int j = c; j < i; ++j) s += L[(size_t)i * n + j] * y[(size_t)j];
-      y[(size_t)i] = -s;
-    }
-    for (int i = 0; i < n; ++i) y[(size_t)i] /= D[(size_t)i];
-    for (int i = n - 1; i >= 0; --i) {  // L^T z = y
-      double s = y[(size_t)i];
-      for (int j = i + 1; j < n; ++j) s -= L[(size_t)j * n + i] * y[(size_t)j];
-      y[(size_t)i] = s;
-    }
-    for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = y[(size_t)i];
-  }
 }
 
 void build_finest_nodes(Context& c, Level& lvl) {
@@ -1149,32 +1105,31 @@ double device_dot(Context& c, long long n, const double* a, const double* b) {
   return hb[15];
 }
 
-struct Context::CoarseJob {
-  int n = 0;
-  std::vector<double> dense, inv;
-  double ratio = 0.0;
-  std::future<void> done;
-};
-
+// The pivot check of the coarsest level's inverse (Multigrid.cpp:766-781): the
+// device inverse records the pivots of the unpivoted LDL^T; an unresisted
+// rigid-body mode shows as a round-off sized (or non-positive) pivot.
 void finish_coarse(Context& c) {
-  if (!c.coarse_job) return;
+  if (!c.coarse_pending) return;
   if (c.capturing) fail(SLX_SOLVER_ERROR, "multigrid: internal error (coarse inverse pending during graph capture)");
-  std::shared_ptr<Context::CoarseJob> job = std::move(c.coarse_job);
-  c.coarse_job.reset();
-  if (job->done.valid()) job->done.get();
-  c.coarse_pivot_ratio = job->ratio;
-  if (!(job->ratio > c.amg_opt.coarse_pivot_tolerance)) {
+  c.coarse_pending = false;
+  std::vector<double> d((size_t)c.coarse_n);
+  c.coarse_piv.download(d.data(), d.size(), c.stream);
+  SLX_HIP(hipStreamSynchronize(c.stream));
+  double mx = 0.0, mn = std::numeric_limits<double>::max();
+  bool finite = true;
+  for (double v : d) {
+    finite = finite && std::isfinite(v);
+    mx = std::max(mx, std::abs(v));
+    mn = std::min(mn, v);
+  }
+  c.coarse_pivot_ratio = finite && mx > 0.0 ? mn / mx : 0.0;
+  if (!(c.coarse_pivot_ratio > c.amg_opt.coarse_pivot_tolerance)) {
     c.amg_ready = false;
     std::ostringstream os;
-    os << "multigrid: the coarsest operator (" << job->n << " unknowns) is singular or indefinite (smallest / largest "
-       << "pivot " << job->ratio << "). A rigid-body mode survived to the coarse level, so the model is "
-       << "under-constrained: too few supports, or a part connected to nothing";
+    os << "multigrid: the coarsest operator (" << c.coarse_n << " unknowns) is singular or indefinite (smallest / "
+       << "largest pivot " << c.coarse_pivot_ratio << "). A rigid-body mode survived to the coarse level, so the model "
+       << "is under-constrained: too few supports, or a part connected to nothing";
     fail(SLX_SOLVER_ERROR, os.str());
-  }
-  c.coarse_inv.upload(job->inv, c.stream);
-  if (c.amg_opt.precision >= 1) {
-    c.coarse_inv32.alloc(c.coarse_inv.n);
-    k::to_float((long long)c.coarse_inv.n, c.coarse_inv.p, c.coarse_inv32.p, c.stream);
   }
 }
 
@@ -1182,10 +1137,7 @@ void amg_setup(Context& c, bool defer_coarse) {
   const auto t0 = Clock::now();
   hipStream_t s = c.stream;
   const int n = c.K.rows;
-  if (c.coarse_job) {  // a previous setup's inverse nobody used
-    if (c.coarse_job->done.valid()) c.coarse_job->done.wait();
-    c.coarse_job.reset();
-  }
+  c.coarse_pending = false;  // a previous setup's inverse nobody used
   invalidate_graphs(c);
   for (double& ph : c.phase_seconds) ph = 0.0;
   auto tick = Clock::now();
@@ -1353,24 +1305,22 @@ void amg_setup(Context& c, bool defer_coarse) {
   }
   c.levels.resize(l + 1);
   c.levels.back().has_next = false;
-  // coarsest level: dense inverse
+  // coarsest level: dense inverse on the device (its pivots are checked by
+  // finish_coarse before the first V-cycle)
   {
-    Level& cl = c.levels.back();
-    const int nc = cl.n;
+    const int nc = c.levels.back().n;
     if (nc > 8000) fail(SLX_SOLVER_ERROR, "multigrid: the coarsest level exceeds the dense solver's 8000 unknowns");
-    HCsr ac = download_csr(level_op(c, l), s);
-    auto job = std::make_shared<Context::CoarseJob>();
-    job->n = nc;
-    job->dense.assign((size_t)nc * nc, 0.0);
-    for (int i = 0; i < nc; ++i)
-      for (int k2 = ac.ptr[(size_t)i]; k2 < ac.ptr[(size_t)i + 1]; ++k2)
-        job->dense[(size_t)i * nc + ac.col[(size_t)k2]] = ac.val[(size_t)k2];
-    Context::CoarseJob* jp = job.get();
-    if (defer_coarse)
-      job->done = std::async(std::launch::async, [jp] { coarse_inverse(jp->n, jp->dense, jp->inv, jp->ratio); });
-    else
-      coarse_inverse(nc, job->dense, job->inv, job->ratio);
-    c.coarse_job = std::move(job);
+    const DCsr& ac = level_op(c, l);
+    c.coarse_inv.alloc((size_t)nc * nc);
+    c.coarse_piv.alloc((size_t)nc);
+    c.coarse_work.alloc((size_t)k::dense_inverse_work(nc));
+    k::dense_inverse(nc, ac.ptr.p, ac.col.p, ac.val.p, c.coarse_inv.p, c.coarse_piv.p, c.coarse_work.p, s);
+    if (o.precision >= 1) {
+      c.coarse_inv32.alloc(c.coarse_inv.n);
+      k::to_float((long long)c.coarse_inv.n, c.coarse_inv.p, c.coarse_inv32.p, s);
+    }
+    c.coarse_n = nc;
+    c.coarse_pending = true;
   }
   lap(5);
   for (auto& lv : c.levels) {

@@ -450,6 +450,144 @@ __global__ void k_copy(long long n, const double* __restrict__ in, double* __res
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
+// ---- dense inverse of the coarsest operator --------------------------------
+// Blocked in-place Gauss-Jordan on the row-major n x n matrix, one pivot block
+// K of kGjB rows at a time (the sweep of K):
+//   P = A_KK^-1,  A_KJ <- P A_KJ,  A_IJ <- A_IJ - A_IK A_KJ,  A_IK <- -A_IK P,  A_KK <- P
+// for I, J outside K. After the last block A holds A^-1. The scalar pivots met
+// inside the diagonal blocks are the D of the unpivoted LDL^T, which the caller
+// checks for a singular coarse operator (Multigrid.cpp:766-781). The operator
+// is symmetric positive definite, so no pivoting is needed.
+constexpr int kGjB = 32;   // pivot block
+constexpr int kGjT = 64;   // update tile
+
+// row i of the dense matrix: zero it, then scatter the CSR row
+__global__ void __launch_bounds__(256) k_dense_from_csr(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                       const double* __restrict__ val, double* __restrict__ a) {
+  const int i = blockIdx.x;
+  double* row = a + (size_t)i * n;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) row[j] = 0.0;
+  __syncthreads();
+  for (int k = ptr[i] + threadIdx.x; k < ptr[i + 1]; k += blockDim.x) row[col[k]] = val[k];
+}
+
+// Every block inverts A_KK in LDS (redundantly), then updates the row panel
+// A_KJ for its kGjT columns and saves the old column panel A_IK of its kGjT
+// rows into cbuf[n][kGjB]. Block 0 also writes P and the pivots.
+__global__ void __launch_bounds__(256) k_gj_panel(int n, int k0, double* __restrict__ a, double* __restrict__ pbuf,
+                                                 double* __restrict__ cbuf, double* __restrict__ piv) {
+  __shared__ double P[kGjB][kGjB + 1];
+  __shared__ double R[kGjB][kGjT + 1];
+  const int m = min(kGjB, n - k0);
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kGjB * kGjB; e += 256) {
+    const int r = e / kGjB, q = e % kGjB;
+    P[r][q] = (r < m && q < m) ? a[(size_t)(k0 + r) * n + k0 + q] : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {
+    const double pk = P[k][k];
+    const double p = 1.0 / pk;
+    double nv[(kGjB * kGjB) / 256];
+#pragma unroll
+    for (int t = 0; t < (kGjB * kGjB) / 256; ++t) {
+      const int e = tid + 256 * t, r = e / kGjB, q = e % kGjB;
+      double v = P[r][q];
+      if (r < m && q < m) {
+        if (r == k) v = q == k ? p : v * p;
+        else if (q == k) v = -P[r][k] * p;
+        else v = v - P[r][k] * (P[k][q] * p);
+      }
+      nv[t] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < (kGjB * kGjB) / 256; ++t) {
+      const int e = tid + 256 * t;
+      P[e / kGjB][e % kGjB] = nv[t];
+    }
+    if (blockIdx.x == 0 && tid == 0) piv[k0 + k] = pk;
+    __syncthreads();
+  }
+  if (blockIdx.x == 0)
+    for (int e = tid; e < kGjB * kGjB; e += 256) pbuf[e] = P[e / kGjB][e % kGjB];
+  // row panel: A_KJ <- P A_KJ over this block's columns outside K
+  const int j0 = blockIdx.x * kGjT;
+  for (int e = tid; e < kGjB * kGjT; e += 256) {
+    const int t = e / kGjT, q = e % kGjT, j = j0 + q;
+    R[t][q] = (t < m && j < n) ? a[(size_t)(k0 + t) * n + j] : 0.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < kGjB * kGjT; e += 256) {
+    const int r = e / kGjT, q = e % kGjT, j = j0 + q;
+    if (r >= m || j >= n || (j >= k0 && j < k0 + m)) continue;
+    double s = 0.0;
+    for (int t = 0; t < m; ++t) s = fma(P[r][t], R[t][q], s);
+    a[(size_t)(k0 + r) * n + j] = s;
+  }
+  // old column panel of this block's rows (zero inside K)
+  for (int e = tid; e < kGjT * kGjB; e += 256) {
+    const int r = e / kGjB, t = e % kGjB, i = j0 + r;
+    if (i >= n) continue;
+    const bool inK = i >= k0 && i < k0 + m;
+    cbuf[(size_t)i * kGjB + t] = (!inK && t < m) ? a[(size_t)i * n + k0 + t] : 0.0;
+  }
+}
+
+// A_IJ <- A_IJ - C_I R_J with C the saved column panel and R the new row panel
+// (P in the columns of K, which gives A_IK <- -A_IK P); A_KK <- P.
+__global__ void __launch_bounds__(256) k_gj_update(int n, int k0, double* __restrict__ a,
+                                                  const double* __restrict__ pbuf, const double* __restrict__ cbuf) {
+  __shared__ double Cs[kGjT][kGjB + 1];
+  __shared__ double Rs[kGjB][kGjT + 1];
+  const int m = min(kGjB, n - k0);
+  const int i0 = blockIdx.y * kGjT, j0 = blockIdx.x * kGjT;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kGjT * kGjB; e += 256) {
+    const int r = e / kGjB, t = e % kGjB, i = i0 + r;
+    Cs[r][t] = (i < n && t < m) ? cbuf[(size_t)i * kGjB + t] : 0.0;
+  }
+  for (int e = tid; e < kGjB * kGjT; e += 256) {
+    const int t = e / kGjT, q = e % kGjT, j = j0 + q;
+    double v = 0.0;
+    if (t < m && j < n) v = (j >= k0 && j < k0 + m) ? pbuf[t * kGjB + (j - k0)] : a[(size_t)(k0 + t) * n + j];
+    Rs[t][q] = v;
+  }
+  __syncthreads();
+  const int tx = tid % 16, ty = tid / 16;
+  double acc[4][4] = {};
+  for (int t = 0; t < m; ++t) {
+    double cv[4], rv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cv[u] = Cs[ty + 16 * u][t];
+      rv[u] = Rs[t][tx + 16 * u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = fma(cv[u], rv[v], acc[u][v]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int i = i0 + ty + 16 * u;
+    if (i >= n) continue;
+    const bool iK = i >= k0 && i < k0 + m;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int j = j0 + tx + 16 * v;
+      if (j >= n) continue;
+      const bool jK = j >= k0 && j < k0 + m;
+      double* x = a + (size_t)i * n + j;
+      if (iK) {
+        if (jK) *x = pbuf[(i - k0) * kGjB + (j - k0)];
+      } else {
+        *x = jK ? -acc[u][v] : *x - acc[u][v];
+      }
+    }
+  }
+}
+
 }  // namespace
 
 void block_product(const BlockProductArgs& a, bool smooth, hipStream_t s) {
@@ -556,6 +694,21 @@ void fill(long long n, double v, double* out, hipStream_t s) {
 
 void copy(long long n, const double* in, double* out, hipStream_t s) {
   if (n) k_copy<<<nblk(n, 256), 256, 0, s>>>(n, in, out);
+}
+
+int dense_inverse_work(int n) { return kGjB * kGjB + n * kGjB; }
+
+void dense_inverse(int n, const int* ptr, const int* col, const double* val, double* a, double* piv, double* work,
+                   hipStream_t s) {
+  if (n <= 0) return;
+  k_dense_from_csr<<<n, 256, 0, s>>>(n, ptr, col, val, a);
+  double* pbuf = work;
+  double* cbuf = work + kGjB * kGjB;
+  const int tiles = (n + kGjT - 1) / kGjT;
+  for (int k0 = 0; k0 < n; k0 += kGjB) {
+    k_gj_panel<<<tiles, 256, 0, s>>>(n, k0, a, pbuf, cbuf, piv);
+    k_gj_update<<<dim3(tiles, tiles), 256, 0, s>>>(n, k0, a, pbuf, cbuf);
+  }
 }
 
 }  // namespace k

@@ -206,10 +206,11 @@ struct Context {
   std::vector<int> layout_unknowns;
   DBuf<double> coarse_inv;  // dense inverse of the coarsest operator
   DBuf<float> coarse_inv32;
-  // the coarsest level's inverse while a host thread computes it (amg.cpp);
-  // finish_coarse() installs it before the first V-cycle
-  struct CoarseJob;
-  std::shared_ptr<CoarseJob> coarse_job;
+  DBuf<double> coarse_piv, coarse_work;  // its pivots, the inversion's panels
+  int coarse_n = 0;
+  // the inverse is queued by a setup; finish_coarse() checks its pivots before
+  // the first V-cycle
+  bool coarse_pending = false;
   double coarse_pivot_ratio = 0.0;
   double setup_seconds = 0.0;
   int tuned_precision = -1;  // precision the CSR kernels were last autotuned for

@@ -30,6 +30,12 @@ void cheb_first(int n, const double* dinv, const double* r, double theta, double
 void cheb_first_zero32(int n, const float* dinv, const float* b, double theta, float* x, float* d, hipStream_t s);
 void cheb_first32(int n, const float* dinv, const float* r, double theta, float* x, float* d, hipStream_t s);
 void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t s);
+// a[n*n] (row-major) = the inverse of the symmetric positive definite CSR matrix
+// (ptr, col, val), by blocked in-place Gauss-Jordan on the device; piv[n] = the
+// scalar pivots (the D of its unpivoted LDL^T). work: dense_inverse_work(n) doubles.
+int dense_inverse_work(int n);
+void dense_inverse(int n, const int* ptr, const int* col, const double* val, double* a, double* piv, double* work,
+                   hipStream_t s);
 void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s);
 
 // deterministic reductions: partial buffer of at least 2048 doubles

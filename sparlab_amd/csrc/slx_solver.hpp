@@ -8,9 +8,9 @@
 namespace slx {
 
 // SA-AMG setup of the context's current K_ff (multigrid hierarchy on the device).
-// defer_coarse: the coarsest level's dense inverse is computed on a host
-// thread, overlapping the device work that follows; finish_coarse() (called by
-// every V-cycle entry) waits for it, checks its pivots and uploads it
+// The coarsest level's dense inverse is queued on the device. defer_coarse:
+// its pivot check is left to finish_coarse() (called by every V-cycle entry),
+// so the setup returns without reading the pivots back
 void amg_setup(Context& c, bool defer_coarse = false);
 void finish_coarse(Context& c);
 // average milliseconds of one Chebyshev step at a middle level, over reps
