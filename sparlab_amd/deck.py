@@ -2,10 +2,11 @@
 
 Reads the decks ``sparlab_topopt`` takes (src/io/Config.cpp:538-1730) and turns
 the part this library runs -- a structured Q4 / Hex8 mesh, supports, point
-loads, load-case weights, the SIMP law, the density filter, passive regions and
-the optimality-criteria loop with its penalty continuation -- into a
+loads, tractions and pressures on the boundary, load-case weights, the SIMP
+law, the density filter, passive regions and the optimality-criteria loop with
+its penalty continuation -- into a
 :class:`~sparlab_amd.model.StructuredModel` and the ``run_oc`` options. Settings
-outside that path (other meshes, tractions, body loads, projection, MMA,
+outside that path (other meshes, body loads, projection, MMA,
 stress or buckling constraints, ...) raise :class:`DeckError` naming the key,
 rather than being ignored.
 
@@ -226,6 +227,67 @@ class Geometry:
         return np.nonzero(hit != region["invert"])[0]
 
 
+def boundary_faces(m: StructuredModel):
+    """The faces on the boundary of a structured grid: (nodes[F, k], outward
+    normals[F, 3], measures[F]), k = 2 (edges) or 4 (faces). A face's measure
+    includes the thickness in 2-D (the area the traction acts on)."""
+    nx, ny = m.nx, m.ny
+    if m.dim == 2:
+        hx, hy = m.lx / nx, m.ly / ny
+        node = lambda i, j: j * (nx + 1) + i  # noqa: E731
+        faces, normals, meas = [], [], []
+        for i in range(nx):
+            faces += [(node(i, 0), node(i + 1, 0)), (node(i, ny), node(i + 1, ny))]
+            normals += [(0.0, -1.0, 0.0), (0.0, 1.0, 0.0)]
+            meas += [hx * m.thickness] * 2
+        for j in range(ny):
+            faces += [(node(0, j), node(0, j + 1)), (node(nx, j), node(nx, j + 1))]
+            normals += [(-1.0, 0.0, 0.0), (1.0, 0.0, 0.0)]
+            meas += [hy * m.thickness] * 2
+        return np.array(faces, dtype=np.int64), np.array(normals), np.array(meas)
+    nz = m.nz
+    hx, hy, hz = m.lx / nx, m.ly / ny, m.lz / nz
+    node = lambda i, j, k: (k * (ny + 1) + j) * (nx + 1) + i  # noqa: E731
+    faces, normals, meas = [], [], []
+    for a in range(nx):
+        for b in range(ny):
+            for k, sgn in ((0, -1.0), (nz, 1.0)):
+                faces.append((node(a, b, k), node(a + 1, b, k), node(a + 1, b + 1, k), node(a, b + 1, k)))
+                normals.append((0.0, 0.0, sgn))
+                meas.append(hx * hy)
+    for a in range(nx):
+        for c in range(nz):
+            for j, sgn in ((0, -1.0), (ny, 1.0)):
+                faces.append((node(a, j, c), node(a + 1, j, c), node(a + 1, j, c + 1), node(a, j, c + 1)))
+                normals.append((0.0, sgn, 0.0))
+                meas.append(hx * hz)
+    for b in range(ny):
+        for c in range(nz):
+            for i, sgn in ((0, -1.0), (nx, 1.0)):
+                faces.append((node(i, b, c), node(i, b + 1, c), node(i, b + 1, c + 1), node(i, b, c + 1)))
+                normals.append((sgn, 0.0, 0.0))
+                meas.append(hy * hz)
+    return np.array(faces, dtype=np.int64), np.array(normals), np.array(meas)
+
+
+def add_face_load(m: StructuredModel, case: int, nodes_in: np.ndarray, traction=None, pressure=None) -> int:
+    """Consistent nodal loads of a uniform traction [Pa] or pressure [Pa] on the
+    boundary faces whose nodes all lie in the region (BoundaryConditions.cpp:63-77,
+    159-232): on a straight edge or a flat rectangular face the linear / bilinear
+    shape functions give each node measure / (nodes per face); a pressure acts
+    against the outward normal (FaceGeometry.cpp:174-191). Returns the faces matched."""
+    faces, normals, meas = boundary_faces(m)
+    mask = np.zeros(m.nnodes, dtype=bool)
+    mask[nodes_in] = True
+    sel = np.all(mask[faces], axis=1)
+    k = faces.shape[1]
+    for f, nrm, a in zip(faces[sel], normals[sel], meas[sel]):
+        force = np.asarray(traction, dtype=float)[: m.dim] * a if traction is not None else -pressure * a * nrm[: m.dim]
+        for n in f:
+            m.loads[case, n * m.dim: (n + 1) * m.dim] += force / k
+    return int(sel.sum())
+
+
 # ---------------------------------------------------------------------------
 # The deck
 # ---------------------------------------------------------------------------
@@ -375,8 +437,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
                         "load case(s)")
     for li, lc in enumerate(cases):
         where = f"load_cases[{li}]"
-        for key in ("tractions", "pressures", "body_forces", "centrifugal", "temperature", "springs",
-                    "prescribed_displacements"):
+        for key in ("body_forces", "centrifugal", "temperature", "springs", "prescribed_displacements"):
             if lc.get(key):
                 _unsupported(f"{where}.{key}", key.replace("_", " "))
         g = lc.get("gravity")
@@ -400,6 +461,24 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
             if len(nodes) == 0:
                 raise DeckError(f"point load '{lname}' selected no nodes")
             m.add_point_load(case, nodes, force, total=(dist == "total"))
+        for ti, tr in enumerate(lc.get("tractions") or []):
+            tw = f"{where}.tractions[{ti}]"
+            t = _vec3(tr.get("traction"), dim, f"{tw}.traction")
+            tname = tr.get("name", f"{cname}_traction{ti}")
+            nodes = geo.nodes_of(parse_region(tr.get("region"), dim, f"{tw}.region"), tname)
+            if add_face_load(m, case, nodes, traction=t) == 0:
+                raise DeckError(f"traction region '{tname}' in load case '{cname}' matched no boundary "
+                                + ("edge" if dim == 2 else "face"))
+        for pi, pr in enumerate(lc.get("pressures") or []):
+            pw = f"{where}.pressures[{pi}]"
+            pval = pr.get("pressure")
+            if isinstance(pval, bool) or not isinstance(pval, (int, float)):
+                raise DeckError(f"'{pw}.pressure' must be a number")
+            pname = pr.get("name", f"{cname}_pressure{pi}")
+            nodes = geo.nodes_of(parse_region(pr.get("region"), dim, f"{pw}.region"), pname)
+            if add_face_load(m, case, nodes, pressure=float(pval)) == 0:
+                raise DeckError(f"pressure region '{pname}' in load case '{cname}' matched no boundary "
+                                + ("edge" if dim == 2 else "face"))
         names.append(cname)
         weights.append(w)
     if sum(weights) <= 0.0:

@@ -1,6 +1,6 @@
 """Writes the deck fixtures the deck-driver tests read.
 
-Reads four of the reference's benchmark decks (configs/benchmarks/*.json) and
+Reads five of the reference's benchmark decks (configs/benchmarks/*.json) and
 its published results table (docs/results/benchmarks.csv) as data, and writes
 
 * decks/<case>.json: each deck as plain JSON (comments removed, nothing else
@@ -19,7 +19,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from sparlab_amd.deck import read_deck  # noqa: E402
 
 REF = "/root/reference"
-CASES = ("cantilever_beam", "mbb_beam", "aerospace_bracket", "bracket_3d")
+CASES = ("cantilever_beam", "mbb_beam", "aerospace_bracket", "bracket_3d", "wing_rib")
 FIELDS = {"dim": int, "elements": int, "dofs": int, "volume_fraction_target": float,
           "volume_fraction_achieved": float, "iterations": int, "stop_reason": str, "compliance_J": float,
           "initial_compliance_J": float, "full_solid_compliance_J": float,

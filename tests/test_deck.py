@@ -1,6 +1,6 @@
 """Deck reader: JSON with comments, regions, and the reference's benchmark decks.
 
-The four benchmark decks (tests/golden/decks, the reference's
+The five benchmark decks (tests/golden/decks, the reference's
 configs/benchmarks/*.json with comments removed) are checked against the
 published table of the reference (docs/results/benchmarks.csv, in
 tests/golden/benchmarks_published.json): the oracle's compliance of the initial
@@ -20,7 +20,7 @@ from sparlab_amd.model import StructuredModel
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
-CASES = ("cantilever_beam", "mbb_beam", "aerospace_bracket", "bracket_3d")
+CASES = ("cantilever_beam", "mbb_beam", "aerospace_bracket", "bracket_3d", "wing_rib")
 PUBLISHED = json.load(open(os.path.join(GOLD, "benchmarks_published.json")))["cases"]
 REF_DECKS = "/root/reference/configs/benchmarks"
 
@@ -149,7 +149,10 @@ def test_overrides():
 
 @pytest.mark.parametrize("edit, match", [
     (lambda d: d["mesh"].update(type="file"), "file mesh"),
-    (lambda d: d["load_cases"][0].update(tractions=[{"region": {"all": {}}, "traction": [1, 0]}]), "tractions"),
+    (lambda d: d["load_cases"][0].update(body_forces=[{"region": {"all": {}}, "force": [1, 0]}]), "body forces"),
+    (lambda d: d["load_cases"][0].update(tractions=[{"region": {"box": {"xmin": 0.1, "xmax": 0.2, "ymin": 0.1,
+                                                                        "ymax": 0.15}}, "traction": [1, 0]}]),
+     "matched no boundary edge"),
     (lambda d: d["load_cases"][0].update(gravity=[0, -9.81]), "self-weight"),
     (lambda d: d["topology"].update(projection={"enabled": True}), "projection"),
     (lambda d: d["topology"]["optimizer"].update(method="mma"), "mma"),
@@ -167,6 +170,32 @@ def test_unsupported_and_invalid_decks_are_refused(edit, match):
     edit(doc)
     with pytest.raises(D.DeckError, match=match):
         D.parse_deck(doc)
+
+
+def test_tractions_and_pressures_on_the_boundary():
+    """A uniform traction puts t * measure on the selected faces, split evenly
+    over their nodes; a pressure p equals the traction -p n (FaceGeometry.cpp:174-191)."""
+    m = StructuredModel(dim=2, nx=8, ny=4, lx=2.0, ly=1.0, thickness=0.1)
+    g = D.Geometry(m)
+    top = g.nodes_of(D.parse_region({"box": {"ymin": 1.0, "xmax": 1.0}}, 2, "r"), "r")  # 5 nodes, 4 edges
+    c = m.add_load_case()
+    assert D.add_face_load(m, c, top, traction=np.array([3.0, -5.0, 0.0])) == 4
+    f = m.loads[c].reshape(-1, 2)
+    np.testing.assert_allclose(f.sum(axis=0), [3.0 * 1.0 * 0.1, -5.0 * 1.0 * 0.1], rtol=1e-14)
+    ends = [top[0], top[-1]]
+    np.testing.assert_allclose(f[ends, 1], -5.0 * 0.25 * 0.1 / 2, rtol=1e-14)  # one edge each
+    np.testing.assert_allclose(f[top[1:-1], 1], -5.0 * 0.25 * 0.1, rtol=1e-14)  # two edges each
+    c2 = m.add_load_case()
+    D.add_face_load(m, c2, top, pressure=5.0)  # pushes down on the top edge (outward normal +y)
+    np.testing.assert_allclose(m.loads[c2].reshape(-1, 2)[:, 1], f[:, 1], rtol=1e-14)
+    # 3-D: a pressure on the x = lx face of a block, every face node
+    h = StructuredModel(dim=3, nx=3, ny=2, nz=2, lx=3.0, ly=2.0, lz=1.0)
+    right = D.Geometry(h).nodes_of(D.parse_region({"box": {"xmin": 3.0}}, 3, "r"), "r")
+    c3 = h.add_load_case()
+    assert D.add_face_load(h, c3, right, pressure=2.0) == 2 * 2
+    fh = h.loads[c3].reshape(-1, 3)
+    np.testing.assert_allclose(fh.sum(axis=0), [-2.0 * 2.0 * 1.0, 0.0, 0.0], rtol=1e-14)
+    assert np.count_nonzero(fh[:, 0]) == len(right) == 9
 
 
 def test_history_csv_format(tmp_path):
