@@ -85,39 +85,60 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
     fi[1] = f[1];
     fi[2] = f[2];
   }
-  // stage the input (scaled for the Lanczos mode) over the tile and its halo
-  for (int h = threadIdx.x; h < HZ * HY * HX; h += kThreads) {
+  // Stage the input (scaled for the Lanczos mode) over the tile and its halo,
+  // and the factors of the tile's elements. Every load is issued before any
+  // is used (indices, then values): entries outside the mesh load node 0 and
+  // are masked, so no branch splits the loads.
+  constexpr int kHalo = HZ * HY * HX, kHaloIt = (kHalo + kThreads - 1) / kThreads;
+  constexpr int kElem = EZ * EY * EX, kElemIt = (kElem + kThreads - 1) / kThreads;
+  const int nex = a.X - 1, ney = a.Y - 1, nez = a.Z - 1;  // elements per direction
+  int hf[kHaloIt][3];
+#pragma unroll
+  for (int it = 0; it < kHaloIt; ++it) {
+    const int h = threadIdx.x + it * kThreads;
     const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
     const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-    T u0 = 0, u1 = 0, u2 = 0;
-    if (gx >= 0 && gx < a.X && gy >= 0 && gy < a.Y && gz >= 0 && gz < a.Z) {
-      const int* f = a.reduced + 3 * (((long long)gz * a.Y + gy) * a.X + gx);
-      const int f0 = f[0], f1 = f[1], f2 = f[2];
-      if (MODE == 4 || MODE == 5) {
-        if (f0 >= 0) u0 = a.x[f0] * a.in_scale[f0];
-        if (f1 >= 0) u1 = a.x[f1] * a.in_scale[f1];
-        if (f2 >= 0) u2 = a.x[f2] * a.in_scale[f2];
-      } else {
-        if (f0 >= 0) u0 = a.x[f0];
-        if (f1 >= 0) u1 = a.x[f1];
-        if (f2 >= 0) u2 = a.x[f2];
-      }
-    }
-    V4 v;
-    v.x = u0;
-    v.y = u1;
-    v.z = u2;
-    v.w = 0;
-    s_u[h] = v;
+    const bool ok = h < kHalo && gx >= 0 && gx < a.X && gy >= 0 && gy < a.Y && gz >= 0 && gz < a.Z;
+    const int* f = a.reduced + 3 * (ok ? (((long long)gz * a.Y + gy) * a.X + gx) : 0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hf[it][c] = ok ? f[c] : -1;
   }
-  const int nex = a.X - 1, ney = a.Y - 1, nez = a.Z - 1;  // elements per direction
-  for (int h = threadIdx.x; h < EZ * EY * EX; h += kThreads) {
+  T sv[kElemIt];
+#pragma unroll
+  for (int it = 0; it < kElemIt; ++it) {
+    const int h = threadIdx.x + it * kThreads;
     const int ex = h % EX, ey = (h / EX) % EY, ez = h / (EX * EY);
     const int gx = x0 - 1 + ex, gy = y0 - 1 + ey, gz = z0 - 1 + ez;
-    T sv = 0;
-    if (gx >= 0 && gx < nex && gy >= 0 && gy < ney && gz >= 0 && gz < nez)
-      sv = a.scale[((long long)gz * ney + gy) * nex + gx];
-    s_s[h] = sv;
+    const bool ok = h < kElem && gx >= 0 && gx < nex && gy >= 0 && gy < ney && gz >= 0 && gz < nez;
+    const T v = a.scale[ok ? ((long long)gz * ney + gy) * nex + gx : 0];
+    sv[it] = ok ? v : T(0);
+  }
+  T hu[kHaloIt][3];
+#pragma unroll
+  for (int it = 0; it < kHaloIt; ++it)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int f = hf[it][c] < 0 ? 0 : hf[it][c];
+      T v = a.x[f];
+      if (MODE == 4 || MODE == 5) v *= a.in_scale[f];
+      hu[it][c] = hf[it][c] < 0 ? T(0) : v;
+    }
+#pragma unroll
+  for (int it = 0; it < kHaloIt; ++it) {
+    const int h = threadIdx.x + it * kThreads;
+    if (h < kHalo) {
+      V4 v;
+      v.x = hu[it][0];
+      v.y = hu[it][1];
+      v.z = hu[it][2];
+      v.w = 0;
+      s_u[h] = v;
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < kElemIt; ++it) {
+    const int h = threadIdx.x + it * kThreads;
+    if (h < kElem) s_s[h] = sv[it];
   }
   // (prescribed DOFs load entry 0 and discard it: no branches around the loads)
   T px[3] = {0, 0, 0}, pb[3] = {0, 0, 0}, pd[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
@@ -142,24 +163,36 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
     // time, and the 8 element products take their corners from registers
     // (27 reads instead of 64). The products run in the same order as a
     // corner-by-corner loop, so the sums are the same.
-    T nx[3][3][3], ny[3][3][3], nz[3][3][3];
-    auto load_layer = [&](int dz) {
+    // two z layers live at a time: [0] the lower, [1] the upper of the
+    // element layer being summed
+    T nx[2][3][3], ny[2][3][3], nz[2][3][3];
+    auto load_layer = [&](int slot, int dz) {
 #pragma unroll
       for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
         for (int dx = 0; dx < 3; ++dx) {
           const V4 u = s_u[((tz + dz) * HY + (ty + dy)) * HX + (tx + dx)];
-          nx[dz][dy][dx] = u.x;
-          ny[dz][dy][dx] = u.y;
-          nz[dz][dy][dx] = u.z;
+          nx[slot][dy][dx] = u.x;
+          ny[slot][dy][dx] = u.y;
+          nz[slot][dy][dx] = u.z;
         }
     };
-    load_layer(0);
-    load_layer(1);
+    load_layer(0, 0);
+    load_layer(1, 1);
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
       const int ox = (o & 1) - 1, oy = ((o >> 1) & 1) - 1, oz = (o >> 2) - 1;
-      if (o == 4) load_layer(2);
+      if (o == 4) {
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+          for (int dx = 0; dx < 3; ++dx) {
+            nx[0][dy][dx] = nx[1][dy][dx];
+            ny[0][dy][dx] = ny[1][dy][dx];
+            nz[0][dy][dx] = nz[1][dy][dx];
+          }
+        load_layer(1, 2);
+      }
       // element at (node + offset); tile-local element and halo coordinates coincide
       const int ex = tx + ox + 1, ey = ty + oy + 1, ez = tz + oz + 1;
       const T se = s_s[(ez * EY + ey) * EX + ex];
@@ -172,7 +205,7 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
         // fused multiply-adds: the element product differs from the assembled
         // operator's sum in rounding only
 #pragma clang fp contract(fast)
-        const int dz = oz + 1 + corner_z(b), dy = oy + 1 + corner_y(b), dx = ox + 1 + corner_x(b);
+        const int dz = corner_z(b), dy = oy + 1 + corner_y(b), dx = ox + 1 + corner_x(b);
         const T ux = nx[dz][dy][dx], uy = ny[dz][dy][dx], uz = nz[dz][dy][dx];
         const int cb = 3 * b;
         t0 += kr[cb] * ux;
