@@ -71,6 +71,18 @@ void set_layout_from_model(Context& c) {
 void upload_filter(Context& c) {
   c.dH.upload(c.H, c.stream);
   c.dHt.upload(c.Ht, c.stream);
+  c.htvol_valid = false;
+}
+
+// dV/dx = H^T vol (DensityFilter::pull_back of the element volumes), computed
+// once per filter
+const double* htvol(Context& c) {
+  if (!c.htvol_valid) {
+    c.d_htvol.alloc((size_t)c.nelem);
+    k::spmv_serial(c.dHt, c.d_vol.p, c.d_htvol.p, c.stream);
+    c.htvol_valid = true;
+  }
+  return c.d_htvol.p;
 }
 
 void ensure_topopt_buffers(Context& c) {
@@ -83,10 +95,16 @@ void ensure_topopt_buffers(Context& c) {
     c.lower.assign(ne, 0.0);
     c.upper.assign(ne, 1.0);
     c.passive.assign(ne, 0);
+    c.design_dirty = true;
   }
-  c.d_lower.upload(c.lower, c.stream);
-  c.d_upper.upload(c.upper, c.stream);
-  c.d_passive.upload(c.passive, c.stream);
+  if (c.design_dirty || c.d_lower.n != ne) {  // the bounds change only through slx_design_set
+    c.d_lower.upload(c.lower, c.stream);
+    c.d_upper.upload(c.upper, c.stream);
+    c.d_passive.upload(c.passive, c.stream);
+    c.unit_bounds = true;
+    for (size_t e = 0; e < ne && c.unit_bounds; ++e) c.unit_bounds = c.lower[e] >= 0.0 && c.upper[e] <= 1.0;
+    c.design_dirty = false;
+  }
   if (c.H.rows != (int)ne) {
     identity_filter(c);
     upload_filter(c);
@@ -168,12 +186,10 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   k::sensitivity(c.dim, (int)ne, c.nx, c.ny, c.d_ke0.p, c.d_reduced.p, c.d_u.p, (long long)nf, c.ncases, c.d_weights.p, c.d_fac.p,
                  c.d_dfac.p, c.d_energy.p, c.d_dc.p, s);
   out.volume = device_dot(c, ne, c.d_phys.p, c.d_vol.p);
-  double dv = 0.0;
-  for (double v : c.volumes) dv += v;
-  out.volume_fraction = out.volume / dv;
+  out.volume_fraction = out.volume / c.total_volume;
   if (need_gradients) {
     k::spmv_serial(c.dHt, c.d_dc.p, c.d_dcx.p, s);
-    k::spmv_serial(c.dHt, c.d_vol.p, c.d_dvx.p, s);
+    k::copy(ne, htvol(c), c.d_dvx.p, s);
   }
   warm_after_evaluate(c);
   sync(c);
@@ -189,25 +205,27 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
   hipStream_t s = c.stream;
   if (!(o.move_limit > 0.0 && o.move_limit <= 1.0)) fail(SLX_CONFIG_ERROR, "optimizer.move_limit must lie in (0, 1]");
   if (!(o.damping > 0.0 && o.damping <= 1.0)) fail(SLX_CONFIG_ERROR, "optimizer.damping must lie in (0, 1]");
-  if (!device_all_finite(c, ne, c.d_dcx.p) || !device_all_finite(c, ne, c.d_dvx.p))
-    fail(SLX_SOLVER_ERROR, "optimality criteria received a non-finite gradient");
-  double dv = 0.0;
-  for (double v : c.volumes) dv += v;
-  const double target = c.volume_fraction * dv;
-  // designs within [0, 1] filter into [0, 1] (rows of H are convex weights), so the
-  // clamp of the filtered volume is inactive and V(x) = (H^T vol).x: one filter
-  // product per update instead of one per bisection trial
-  bool bounded = true;
-  for (size_t e = 0; e < c.lower.size() && bounded; ++e) bounded = c.lower[e] >= 0.0 && c.upper[e] <= 1.0;
-  if (bounded) {
-    c.d_ocw.alloc((size_t)ne);
-    c.oc_state.alloc(8);
-    k::spmv_serial(c.dHt, c.d_vol.p, c.d_ocw.p, s);
+  {
+    // both gradients finite: their sums of squares, read together
+    k::dot(ne, c.d_dcx.p, c.d_dcx.p, c.partials.p, c.scal.p + 11, s);
+    k::dot(ne, c.d_dvx.p, c.d_dvx.p, c.partials.p, c.scal.p + 12, s);
+    double* hb = c.host_buffer(16);
+    SLX_HIP(hipMemcpyAsync(hb + 11, c.scal.p + 11, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    if (!std::isfinite(hb[11]) || !std::isfinite(hb[12]))
+      fail(SLX_SOLVER_ERROR, "optimality criteria received a non-finite gradient");
   }
+  const double target = c.volume_fraction * c.total_volume;
+  // designs within [0, 1] filter into [0, 1] (rows of H are convex weights), so the
+  // clamp of the filtered volume is inactive and V(x) = (H^T vol).x: no filter
+  // product per bisection trial
+  const bool bounded = c.unit_bounds;
+  const double* ocw = bounded ? htvol(c) : nullptr;
+  if (bounded) c.oc_state.alloc(8);
   auto volume_at = [&](double lambda) {
     if (bounded) {
       k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit,
-                  o.damping, c.d_ocw.p, c.oc_state.p, 1, lambda, target, o.volume_tolerance, o.max_bisections,
+                  o.damping, ocw, c.oc_state.p, 1, lambda, target, o.volume_tolerance, o.max_bisections,
                   c.d_tmp2.p, c.partials.p, s);
       return read_scalar(c, c.oc_state.p + 2);
     }
@@ -238,17 +256,19 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
   slx_oc_step step{};
   if (bounded) {
     // the bisection on the device, checked from the host every few trials
-    const double init[8] = {lo, hi, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    c.oc_state.upload(init, 8, s);
+    // state {lo, hi, 0, ...} set on the device (no host copy to wait for)
+    k::fill(8, 0.0, c.oc_state.p, s);
+    k::fill(1, lo, c.oc_state.p, s);
+    k::fill(1, hi, c.oc_state.p + 1, s);
     k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit, o.damping,
-                c.d_ocw.p, c.oc_state.p, 1, lo, target, o.volume_tolerance, o.max_bisections, c.d_tmp2.p,
+                ocw, c.oc_state.p, 1, lo, target, o.volume_tolerance, o.max_bisections, c.d_tmp2.p,
                 c.partials.p, s);
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (o.max_bisections > 0) {
       do {
         for (int k = 0; k < 8; ++k)
           k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit,
-                      o.damping, c.d_ocw.p, c.oc_state.p, 0, 0.0, target, o.volume_tolerance, o.max_bisections,
+                      o.damping, ocw, c.oc_state.p, 0, 0.0, target, o.volume_tolerance, o.max_bisections,
                       c.d_tmp2.p, c.partials.p, s);
         double* hb = c.host_buffer(16);
         SLX_HIP(hipMemcpyAsync(hb, c.oc_state.p, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -448,6 +468,9 @@ slx_status slx_model_finalize(slx_context* ctx) {
     const std::vector<double> xe = element0_coords(c);
     c.ke0 = c.dim == 3 ? hex8_ke0(xe.data(), c.E, c.nu) : quad4_ke0(xe.data(), c.E, c.nu, c.thickness);
     c.volumes.assign((size_t)c.nelem, element_volume(c));
+    c.total_volume = 0.0;
+    for (double v : c.volumes) c.total_volume += v;
+    c.htvol_valid = false;
     const HCsr pat = structured_kff_pattern(c);
     c.K.upload_pattern(pat, c.stream);
     c.d_free.upload(c.free_dofs, c.stream);
@@ -760,6 +783,7 @@ slx_status slx_design_set(slx_context* ctx, double volume_fraction, const double
     c.lower.assign(lower, lower + ne);
     c.upper.assign(upper, upper + ne);
     c.passive.assign(passive, passive + ne);
+    c.design_dirty = true;
     ensure_topopt_buffers(c);
     sync(c);
   });
@@ -840,13 +864,10 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
     c.d_x.upload(x, (size_t)ne, c.stream);
     k::clamp(ne, c.d_x.p, c.d_lower.p, c.d_upper.p, c.stream);
     std::vector<double> chist;
-    std::vector<double> phys((size_t)ne);
+    // gray_level (src/topopt/TopologyOptimizer.cpp:106-113): 4/n sum rho (1 - rho)
     auto grey = [&]() {
-      c.d_phys.download(phys.data(), (size_t)ne, c.stream);
-      sync(c);
-      double g = 0.0;
-      for (double r : phys) g += r * (1.0 - r);
-      return 4.0 * g / double(std::max(ne, 1));
+      k::grey_sum(ne, c.d_phys.p, c.partials.p, c.scal.p + 10, c.stream);
+      return 4.0 * read_scalar(c, c.scal.p + 10) / double(std::max(ne, 1));
     };
     slx_run_result res{};
     res.stop_reason = SLX_STOP_ITERATION_CAP;

@@ -226,6 +226,16 @@ __global__ void __launch_bounds__(kBlock) k_dot_partials(int n, const double* __
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// partials of sum rho (1 - rho), the grey level's sum
+__global__ void __launch_bounds__(kBlock) k_grey_partials(int n, const double* __restrict__ rho,
+                                                           double* __restrict__ partials) {
+  __shared__ double sh[kBlock / 64];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += rho[i] * (1.0 - rho[i]);
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
 // Sums m partials in a fixed order into out[0]; optional sqrt.
 __global__ void __launch_bounds__(1024) k_reduce_partials(int m, const double* __restrict__ partials,
                                                           double* __restrict__ out, int take_sqrt) {
@@ -770,6 +780,11 @@ int reduce_blocks(long long n) { return (int)std::max<long long>(1, std::min<lon
 void dot(long long n, const double* a, const double* b, double* partials, double* out, hipStream_t s) {
   const int nb = reduce_blocks(n);
   k_dot_partials<<<nb, kBlock, 0, s>>>((int)n, a, b, partials);
+  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+}
+void grey_sum(long long n, const double* rho, double* partials, double* out, hipStream_t s) {
+  const int nb = reduce_blocks(n);
+  k_grey_partials<<<nb, kBlock, 0, s>>>((int)n, rho, partials);
   k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
 }
 void reduce(int m, const double* partials, double* out, int take_sqrt, hipStream_t s) {

@@ -175,6 +175,11 @@ struct Context {
   std::vector<double> ke0;              // edofs x edofs
   std::vector<double> coords;           // nnodes x dim
   std::vector<double> volumes;          // element volumes
+  double total_volume = 0.0;            // their sum in element order (the domain volume)
+  bool unit_bounds = true;              // every design bound lies in [0, 1]
+  bool design_dirty = true;             // host bounds not yet on the device
+  DBuf<double> d_htvol;                 // H^T vol = dV/dx, fixed by the filter and the volumes
+  bool htvol_valid = false;
   bool finalized = false;
   int ncases = 0;
   std::vector<double> loads;    // ncases x ndofs
@@ -239,7 +244,7 @@ struct Context {
   HCsr H, Ht;  // filter and its transpose
   DCsr dH, dHt;
   DBuf<double> d_x, d_phys, d_fac, d_dfac, d_dc, d_energy, d_vol, d_tmp, d_tmp2, d_lower, d_upper, d_dcx, d_dvx;
-  DBuf<double> d_ocw, oc_state;  // H^T vol and the device bisection state
+  DBuf<double> oc_state;  // the device bisection state
   DBuf<unsigned char> d_passive;
 
   // ---- learned warm start -------------------------------------------------------

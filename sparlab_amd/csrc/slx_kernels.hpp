@@ -41,6 +41,8 @@ void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s
 // deterministic reductions: partial buffer of at least 2048 doubles
 int reduce_blocks(long long n);
 void dot(long long n, const double* a, const double* b, double* partials, double* out, hipStream_t s);
+// out[0] = sum_i rho_i (1 - rho_i) (the grey level times n / 4)
+void grey_sum(long long n, const double* rho, double* partials, double* out, hipStream_t s);
 void reduce(int m, const double* partials, double* out, int take_sqrt, hipStream_t s);
 void pcg_update(long long n, double* x, double* r, const double* p, const double* q, const double* scal, int i_rz,
                 int i_pq, double* partials, double* out, hipStream_t s);
