@@ -938,19 +938,12 @@ HCsr download_csr(const DCsr& a, hipStream_t s) {
   return h;
 }
 
-int read_flag(Context& c, DBuf<int>& flag) {
-  int v = 0;
-  flag.download(&v, 1, c.stream);
-  SLX_HIP(hipStreamSynchronize(c.stream));
-  return v;
-}
-
 // Lanczos estimate of lambda_max(D^-1 A) on the device (Multigrid.cpp:288-327).
-// All steps are queued with their scalars in device memory; one read at the
-// end brings alpha and beta back, and the recurrence is cut on the host where
-// the reference breaks (beta <= 1e-12 |alpha|). Steps queued after such a
-// breakdown compute values nobody reads.
-double lanczos(Context& c, size_t l) {
+// All steps are queued with their scalars in device memory (lanczos_queue
+// returns the step count); one read brings alpha and beta back, and
+// lanczos_lambda cuts the recurrence where the reference breaks. Steps queued
+// after such a breakdown compute values nobody reads.
+int lanczos_queue(Context& c, size_t l) {
   Level& lv = c.levels[l];
   DCsr& A = level_op(c, l);
   const int n = lv.n;
@@ -988,9 +981,12 @@ double lanczos(Context& c, size_t l) {
     k::reduce(k::axmy_blocks(n), c.partials.p, beta + j, 1, s);
     if (j + 1 < m) k::lanczos_shift(n, beta + j, lv.lz_w.p, lv.lz_v.p, lv.lz_vprev.p, s);
   }
-  std::vector<double> h((size_t)(2 * m + 1));
-  sc.download(h.data(), h.size(), s);
-  SLX_HIP(hipStreamSynchronize(s));
+  return m;
+}
+
+// lambda_max from the Lanczos scalars read back (h = lz_scal): the recurrence
+// is cut where the reference breaks (beta <= 1e-12 |alpha|)
+double lanczos_lambda(const std::vector<double>& h, int m) {
   std::vector<double> a, b;
   for (int j = 0; j < m; ++j) {
     const double aj = h[(size_t)(1 + j)], bj = h[(size_t)(1 + m + j)];
@@ -1002,35 +998,43 @@ double lanczos(Context& c, size_t l) {
   return tridiag_max(a, sub);
 }
 
+// D^-1 and lambda_max(D^-1 A) of level l with one read from the device: the
+// Lanczos scalars, the level's diagonal flag and, at level 0, the finiteness
+// sum of the matrix queued by amg_setup.
 void numeric_level(Context& c, size_t l) {
   Level& lv = c.levels[l];
   DCsr& A = level_op(c, l);
+  hipStream_t s = c.stream;
   lv.dinv.alloc((size_t)lv.n);
-  DBuf<int> bad;
-  bad.alloc(1);
-  const int init = std::numeric_limits<int>::max();
-  bad.upload(&init, 1, c.stream);
-  k::inv_diag(A, lv.dinv.p, bad.p, c.stream);
-  const int b = read_flag(c, bad);
-  if (b != init) {
+  int* bad = c.setup_flags.p + 1 + l;
+  constexpr int kNoBadRow = 0x7F7F7F7F;  // above any row index
+  SLX_HIP(hipMemsetAsync(bad, 0x7F, sizeof(int), s));
+  k::inv_diag(A, lv.dinv.p, bad, s);
+  const int m = lanczos_queue(c, l);
+  std::vector<double> h((size_t)(2 * m + 1));
+  lv.lz_scal.download(h.data(), h.size(), s);
+  int b = kNoBadRow;
+  SLX_HIP(hipMemcpyAsync(&b, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  double sumsq = 0.0;
+  if (l == 0) SLX_HIP(hipMemcpyAsync(&sumsq, c.scal.p + 9, sizeof(double), hipMemcpyDeviceToHost, s));
+  SLX_HIP(hipStreamSynchronize(s));
+  // finiteness of every entry, each call (Multigrid.cpp:992-996): the sum of
+  // squares is finite exactly when every entry is
+  if (l == 0 && !std::isfinite(sumsq)) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix has a non-finite entry");
+  if (b != kNoBadRow) {
     std::ostringstream os;
     os << "multigrid level " << l << ": diagonal entry " << b
        << " is not positive; a stiffness matrix must have a positive diagonal. The model has an unconnected degree "
           "of freedom or a non-positive stiffness";
     fail(SLX_SOLVER_ERROR, os.str());
   }
-  lv.lambda_max = lanczos(c, l);
+  lv.lambda_max = lanczos_lambda(h, m);
   if (!(lv.lambda_max > 0.0) || !std::isfinite(lv.lambda_max)) {
     std::ostringstream os;
     os << "multigrid level " << l << ": the spectral estimate of D^-1 A is " << lv.lambda_max
        << "; the matrix is not positive definite";
     fail(SLX_SOLVER_ERROR, os.str());
   }
-}
-
-void check_flag(Context& c, DBuf<int>& flag, const char* what) {
-  const int v = read_flag(c, flag);
-  if (v != 0) fail(SLX_SOLVER_ERROR, std::string("multigrid: internal error in ") + what + " (pattern mismatch)");
 }
 
 // The CSR kernels' lanes per row, timed on each operator of the hierarchy (the
@@ -1140,12 +1144,15 @@ void amg_setup(Context& c, bool defer_coarse) {
   c.coarse_pending = false;  // a previous setup's inverse nobody used
   invalidate_graphs(c);
   for (double& ph : c.phase_seconds) ph = 0.0;
-  auto tick = Clock::now();
+  // phase times from device events (no waits between the phases): event k
+  // closes phase lap_phase[k]
+  std::vector<std::pair<int, hipEvent_t>> laps;
+  hipEvent_t lap_start = c.take_event();
+  SLX_HIP(hipEventRecord(lap_start, s));
   auto lap = [&](int phase) {
-    SLX_HIP(hipStreamSynchronize(s));
-    const auto now = Clock::now();
-    c.phase_seconds[phase] += std::chrono::duration<double>(now - tick).count();
-    tick = now;
+    hipEvent_t e = c.take_event();
+    SLX_HIP(hipEventRecord(e, s));
+    laps.push_back({phase, e});
   };
   if (n == 0) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix must be square and non-empty");
   if (c.layout_dim != 2 && c.layout_dim != 3) fail(SLX_SOLVER_ERROR, "multigrid: the DOF layout must be two- or three-dimensional");
@@ -1155,11 +1162,16 @@ void amg_setup(Context& c, bool defer_coarse) {
   const slx_amg_options& o = c.amg_opt;
   if (o.smoother != 0) fail(SLX_CONFIG_ERROR, "multigrid: only the Chebyshev smoother runs on the GPU");
   ensure_workspace(c);
-  const bool same = c.amg_ready && c.fp_rows == n && c.fp_nnz == c.K.nnz && c.fp_unknowns == c.layout_unknowns;
+  const bool same = c.amg_ready && c.fp_rows == n && c.fp_nnz == c.K.nnz && c.fp_layout_version == c.layout_version;
   const bool reuse = o.reuse_aggregates && same;
   c.amg_ready = false;  // until this setup completes
-  // finiteness of every entry, each call (Multigrid.cpp:992-996)
-  if (!device_all_finite(c, c.K.nnz, c.K.val.p)) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix has a non-finite entry");
+  // device flags: [0] a Galerkin pattern mismatch, [1 + l] level l's first
+  // non-positive diagonal row; read with the level's Lanczos scalars and at the end
+  c.setup_flags.alloc((size_t)o.max_levels + 2);
+  SLX_HIP(hipMemsetAsync(c.setup_flags.p, 0, sizeof(int), s));
+  // finiteness of every entry (Multigrid.cpp:992-996): the sum of squares,
+  // checked with level 0's read
+  k::dot(c.K.nnz, c.K.val.p, c.K.val.p, c.partials.p, c.scal.p + 9, s);
   HCsr host_a;  // values of the level being aggregated (first setup only)
   if (!reuse) {
     host_a = download_csr(c.K, s);
@@ -1197,9 +1209,7 @@ void amg_setup(Context& c, bool defer_coarse) {
     c.amg_ready = false;
   }
   lap(0);
-  DBuf<int> flag;
-  flag.alloc(1);
-  const int zero = 0;
+  int* flag = c.setup_flags.p;
   size_t l = 0;
   const size_t reused_levels = c.levels.size();
   while (true) {
@@ -1271,7 +1281,6 @@ void amg_setup(Context& c, bool defer_coarse) {
     Level& cur = c.levels[l];
     Level& coarse = c.levels[l + 1];
     DCsr& A = level_op(c, l);
-    flag.upload(&zero, 1, s);
     const double omega = o.prolongator_damping / cur.lambda_max;
     cur.omega = omega;
     if (cur.node_ok) {
@@ -1293,13 +1302,12 @@ void amg_setup(Context& c, bool defer_coarse) {
         k::row_product(row_args(cur, cur.plan_raw, true, cur.R, cur.AP, cur.RAW), s);
       }
     } else {
-      k::smooth_prolongator(A, cur.Ptent, cur.P, cur.dinv.p, omega, flag.p, s);
+      k::smooth_prolongator(A, cur.Ptent, cur.P, cur.dinv.p, omega, flag, s);
       k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
-      k::spgemm_numeric(A, cur.P, cur.AP, flag.p, s);
-      k::spgemm_numeric(cur.R, cur.AP, cur.RAW, flag.p, s);
+      k::spgemm_numeric(A, cur.P, cur.AP, flag, s);
+      k::spgemm_numeric(cur.R, cur.AP, cur.RAW, flag, s);
     }
     k::symmetrize_values(coarse.A.nnz, cur.sym_a.p, cur.sym_b.p, cur.RAW.val.p, coarse.A.val.p, s);
-    check_flag(c, flag, "the Galerkin product");
     lap(4);
     ++l;
   }
@@ -1335,11 +1343,25 @@ void amg_setup(Context& c, bool defer_coarse) {
     autotune(c);
     c.tuned_precision = o.precision;
   }
+  int galerkin_flag = 0;
+  SLX_HIP(hipMemcpyAsync(&galerkin_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
   SLX_HIP(hipStreamSynchronize(s));
+  {
+    float ms = 0.0f;
+    hipEvent_t prev = lap_start;
+    for (auto& pe : laps) {
+      SLX_HIP(hipEventElapsedTime(&ms, prev, pe.second));
+      c.phase_seconds[pe.first] += 1.0e-3 * ms;
+      c.event_pool.push_back(prev);
+      prev = pe.second;
+    }
+    c.event_pool.push_back(prev);
+  }
+  if (galerkin_flag != 0) fail(SLX_SOLVER_ERROR, "multigrid: internal error in the Galerkin product (pattern mismatch)");
   ++c.setup_id;
   c.fp_rows = n;
   c.fp_nnz = c.K.nnz;
-  c.fp_unknowns = c.layout_unknowns;
+  c.fp_layout_version = c.layout_version;
   c.amg_ready = true;
   c.reused = reuse;
   if (!defer_coarse) finish_coarse(c);

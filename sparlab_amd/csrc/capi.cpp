@@ -66,6 +66,7 @@ void set_layout_from_model(Context& c) {
   c.layout_nnodes = (int)c.nnodes;
   c.layout_coords = c.coords;
   c.layout_unknowns = c.free_dofs;
+  ++c.layout_version;
 }
 
 void upload_filter(Context& c) {
@@ -603,15 +604,19 @@ slx_status slx_factorize_csr(slx_context* ctx, int n, const int* ptr, const int*
     for (int i = 0; i < n; ++i)
       for (int k2 = h.ptr[(size_t)i] + 1; k2 < h.ptr[(size_t)i + 1]; ++k2)
         if (h.col[(size_t)k2] <= h.col[(size_t)k2 - 1]) fail(SLX_SOLVER_ERROR, "CSR rows must be sorted by column without duplicates");
-    const bool same_pattern = c.K.rows == n && c.K.nnz == h.nnz() && c.layout_unknowns.size() == (size_t)n &&
-                              std::equal(unknowns, unknowns + n, c.layout_unknowns.begin());
+    const bool same_unknowns =
+        c.layout_unknowns.size() == (size_t)n && std::equal(unknowns, unknowns + n, c.layout_unknowns.begin());
+    const bool same_pattern = c.K.rows == n && c.K.nnz == h.nnz() && same_unknowns;
     if (same_pattern) c.K.upload_values(h.val.data(), c.stream); else c.K.upload(h, c.stream);
     c.mf = false;  // a caller's matrix: products use it as given
     c.layout_dim = dim;
     c.layout_ndpn = ndpn > 0 ? ndpn : dim;
     c.layout_nnodes = nnodes;
     c.layout_coords.assign(coords, coords + (size_t)nnodes * dim);
-    c.layout_unknowns.assign(unknowns, unknowns + n);
+    if (!same_unknowns) {
+      c.layout_unknowns.assign(unknowns, unknowns + n);
+      ++c.layout_version;
+    }
     slx_amg_options o;
     slx_amg_default_options(&o);
     if (opt) o = *opt;

@@ -205,10 +205,12 @@ struct Context {
   bool amg_ready = false;
   bool reused = false;
   long long fp_rows = -1, fp_nnz = -1;
-  std::vector<int> fp_unknowns;
+  long long fp_layout_version = -1;  // layout_version at the last setup
   std::vector<double> layout_coords;  // coords of the DOF layout (nnodes x dim)
   int layout_dim = 0, layout_ndpn = 0, layout_nnodes = 0;
   std::vector<int> layout_unknowns;
+  long long layout_version = 0;  // changes whenever layout_unknowns does
+  DBuf<int> setup_flags;         // device error flags of a setup (amg.cpp)
   DBuf<double> coarse_inv;  // dense inverse of the coarsest operator
   DBuf<float> coarse_inv32;
   DBuf<double> coarse_piv, coarse_work;  // its pivots, the inversion's panels
