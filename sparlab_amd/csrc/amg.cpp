@@ -958,6 +958,7 @@ int lanczos_queue(Context& c, size_t l) {
   lv.lz_v.alloc((size_t)n);
   lv.lz_vprev.alloc((size_t)n);
   lv.lz_w.alloc((size_t)n);
+  lv.lz_part.alloc((size_t)k::axmy_blocks(n));
   DBuf<double>& sc = lv.lz_scal;  // [0] norm, [1..m] alpha, [m+1..2m] beta
   sc.alloc((size_t)(2 * m + 1));
   k::vsqrt(n, lv.dinv.p, lv.sq.p, s);
@@ -976,10 +977,8 @@ int lanczos_queue(Context& c, size_t l) {
                               : (k::lanczos_apply(A, lv.sq.p, lv.lz_v.p, lv.lz_vprev.p, bprev, lv.lz_w.p,
                                                   c.partials.p, s),
                                  k::lanczos_blocks(n));
-    k::reduce(nb, c.partials.p, alpha + j, 0, s);
-    k::axmy_dot(n, alpha + j, lv.lz_v.p, lv.lz_w.p, c.partials.p, s);
-    k::reduce(k::axmy_blocks(n), c.partials.p, beta + j, 1, s);
-    if (j + 1 < m) k::lanczos_shift(n, beta + j, lv.lz_w.p, lv.lz_v.p, lv.lz_vprev.p, s);
+    k::axmy_dot_sum(n, nb, c.partials.p, alpha + j, lv.lz_v.p, lv.lz_w.p, lv.lz_part.p, s);
+    k::lanczos_shift_sum(n, k::axmy_blocks(n), lv.lz_part.p, beta + j, lv.lz_w.p, lv.lz_v.p, lv.lz_vprev.p, s);
   }
   return m;
 }

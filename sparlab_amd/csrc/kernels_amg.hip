@@ -406,10 +406,26 @@ __global__ void __launch_bounds__(256) k_lanczos_apply(int n, const int* __restr
 }
 
 // w -= a v with a = *a_ptr, and the block partials of w.w (grid-stride)
-__global__ void __launch_bounds__(256) k_axmy_dot(int n, const double* __restrict__ a_ptr, const double* __restrict__ v,
-                                                  double* __restrict__ w, double* __restrict__ partials) {
+// the sum of m partials in a fixed order, the same in every block of 256
+__device__ __forceinline__ double sum_partials_256(int m, const double* __restrict__ p, double* sh) {
+  double t = 0.0;
+  for (int i = threadIdx.x; i < m; i += 256) t += p[i];
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+  __syncthreads();
+  const double r = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return r;
+}
+
+// alpha = the sum of the product's partials (block 0 stores it); w -= alpha v
+// with the partials of w.w
+__global__ void __launch_bounds__(256) k_axmy_dot_sum(int n, int m, const double* __restrict__ part_in,
+                                                      double* __restrict__ alpha_out, const double* __restrict__ v,
+                                                      double* __restrict__ w, double* __restrict__ part_out) {
   __shared__ double sh[4];
-  const double a = *a_ptr;
+  const double a = sum_partials_256(m, part_in, sh);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *alpha_out = a;
   double dot = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double wi = w[i] - a * v[i];
@@ -419,7 +435,21 @@ __global__ void __launch_bounds__(256) k_axmy_dot(int n, const double* __restric
   for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = dot;
   __syncthreads();
-  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+  if (threadIdx.x == 0) part_out[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// beta = sqrt(the sum of the partials) (block 0 stores it); vprev = v, v = w / beta
+__global__ void __launch_bounds__(256) k_lanczos_shift_sum(int n, int m, const double* __restrict__ part_in,
+                                                           double* __restrict__ beta_out,
+                                                           const double* __restrict__ w, double* __restrict__ v,
+                                                           double* __restrict__ vprev) {
+  __shared__ double sh[4];
+  const double b = sqrt(sum_partials_256(m, part_in, sh));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *beta_out = b;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    vprev[i] = v[i];
+    v[i] = w[i] / b;
+  }
 }
 
 __global__ void k_sqrt(int n, const double* __restrict__ in, double* __restrict__ out) {
@@ -677,10 +707,16 @@ void lanczos_apply(const DCsr& A, const double* sq, const double* v, const doubl
 
 int axmy_blocks(int n) { return std::max(1, std::min(1024, nblk(n, 256))); }
 
-void axmy_dot(int n, const double* a, const double* v, double* w, double* partials, hipStream_t s) {
-  if (n) k_axmy_dot<<<axmy_blocks(n), 256, 0, s>>>(n, a, v, w, partials);
+
+void axmy_dot_sum(int n, int m, const double* part_in, double* alpha, const double* v, double* w, double* part_out,
+                  hipStream_t s) {
+  if (n) k_axmy_dot_sum<<<axmy_blocks(n), 256, 0, s>>>(n, m, part_in, alpha, v, w, part_out);
 }
 
+void lanczos_shift_sum(int n, int m, const double* part_in, double* beta, const double* w, double* v, double* vprev,
+                       hipStream_t s) {
+  if (n) k_lanczos_shift_sum<<<axmy_blocks(n), 256, 0, s>>>(n, m, part_in, beta, w, v, vprev);
+}
 
 void lanczos_shift(int n, const double* b, const double* w, double* v, double* vprev, hipStream_t s) {
   if (n) k_lanczos_shift<<<nblk(n, 256), 256, 0, s>>>(n, b, w, v, vprev);

@@ -146,7 +146,7 @@ struct Level {
   DBuf<int> f_node_ptr, c_node_ptr, row_agg;
   ProductPlan plan_p;          // P = (I - w D^-1 A) P-hat
   RowPlan plan_ap, plan_raw;   // A P, R (A P)
-  DBuf<double> dinv, sq, lz_v, lz_vprev, lz_w, lz_start, lz_scal;
+  DBuf<double> dinv, sq, lz_v, lz_vprev, lz_w, lz_start, lz_scal, lz_part;
   DBuf<double> x, b, res, d, d2;
   DBuf<float> dinv32, x32, b32, res32, d32, d232;
 };

@@ -153,9 +153,16 @@ void vsqrt(int n, const double* in, double* out, hipStream_t s);
 int lanczos_blocks(int n);
 void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, const double* beta,
                    double* w, double* partials, hipStream_t s);
-// w -= a v with the partials of w.w: axmy_blocks(n) of them
+// blocks (and partials) of the Lanczos vector steps below
 int axmy_blocks(int n);
-void axmy_dot(int n, const double* a, const double* v, double* w, double* partials, hipStream_t s);
+// Lanczos vector steps with the scalar reductions folded in (every block sums
+// the m partials of the previous kernel itself; block 0 stores the scalar):
+// alpha = sum part_in, w -= alpha v, part_out = partials of w.w;
+void axmy_dot_sum(int n, int m, const double* part_in, double* alpha, const double* v, double* w, double* part_out,
+                  hipStream_t s);
+// beta = sqrt(sum part_in), vprev = v, v = w / beta (axmy_blocks(n) blocks)
+void lanczos_shift_sum(int n, int m, const double* part_in, double* beta, const double* w, double* v, double* vprev,
+                       hipStream_t s);
 void lanczos_shift(int n, const double* b, const double* w, double* v, double* vprev, hipStream_t s);
 void sqrt_scalar(double* p, hipStream_t s);
 void fill(long long n, double v, double* out, hipStream_t s);
