@@ -409,6 +409,63 @@ double tridiag_max(const std::vector<double>& d, const std::vector<double>& e) {
   return hi;
 }
 
+// P-hat's aggregate blocks for the transfer kernels (k::TentBlocks): every row
+// holds one run of at most 6 consecutive columns, the runs of an aggregate's
+// rows are the same, and the aggregates' runs tile the coarse columns. Leaves
+// tent_ok false (the transfers then use the CSR products) when they do not.
+void build_tent_blocks(Level& lv, const HCsr& pt, hipStream_t s) {
+  lv.tent_ok = false;
+  const int n = pt.rows;
+  std::vector<int> c0len((size_t)n, 0);
+  std::vector<double> ell((size_t)6 * n, 0.0);
+  std::vector<int> start_len((size_t)pt.cols, -1);  // run length at an aggregate's first column
+  for (int r = 0; r < n; ++r) {
+    const int k0 = pt.ptr[(size_t)r], len = pt.ptr[(size_t)r + 1] - k0;
+    if (len == 0) continue;
+    if (len > 6) return;
+    const int c0 = pt.col[(size_t)k0];
+    for (int k = 0; k < len; ++k) {
+      if (pt.col[(size_t)k0 + k] != c0 + k) return;
+      ell[(size_t)k * n + r] = pt.val[(size_t)k0 + k];
+    }
+    if (start_len[(size_t)c0] >= 0 && start_len[(size_t)c0] != len) return;
+    start_len[(size_t)c0] = len;
+    c0len[(size_t)r] = c0 * 8 + len;
+  }
+  std::vector<int> agg_of((size_t)pt.cols, -1), agg_c0len;
+  for (int col = 0; col < pt.cols;) {  // the runs must tile [0, cols)
+    const int len = start_len[(size_t)col];
+    if (len <= 0) return;
+    agg_of[(size_t)col] = (int)agg_c0len.size();
+    agg_c0len.push_back(col * 8 + len);
+    col += len;
+  }
+  const int na = (int)agg_c0len.size();
+  std::vector<int> aptr((size_t)na + 1, 0), rows;
+  for (int r = 0; r < n; ++r)
+    if (c0len[(size_t)r]) ++aptr[(size_t)agg_of[(size_t)(c0len[(size_t)r] >> 3)] + 1];
+  for (int a = 0; a < na; ++a) aptr[(size_t)a + 1] += aptr[(size_t)a];
+  rows.resize((size_t)aptr[(size_t)na]);
+  {
+    std::vector<int> fill(aptr.begin(), aptr.end() - 1);
+    for (int r = 0; r < n; ++r)
+      if (c0len[(size_t)r]) rows[(size_t)fill[(size_t)agg_of[(size_t)(c0len[(size_t)r] >> 3)]]++] = r;
+  }
+  std::vector<double> ella((size_t)6 * n, 0.0);
+  for (size_t i = 0; i < rows.size(); ++i)
+    for (int k = 0; k < 6; ++k) ella[(size_t)k * n + i] = ell[(size_t)k * n + (size_t)rows[i]];
+  lv.tent_c0len.upload(c0len, s);
+  lv.tent_ell.upload(ell, s);
+  lv.tent_ell32.upload(std::vector<float>(ell.begin(), ell.end()), s);
+  lv.tent_ella.upload(ella, s);
+  lv.tent_ella32.upload(std::vector<float>(ella.begin(), ella.end()), s);
+  lv.tent_agg_ptr.upload(aptr, s);
+  lv.tent_agg_rows.upload(rows, s);
+  lv.tent_agg_c0len.upload(agg_c0len, s);
+  lv.tent_naggs = na;
+  lv.tent_ok = true;
+}
+
 void build_finest_nodes(Context& c, Level& lvl) {
   const int n = lvl.n, dim = c.layout_dim, ndpn = c.layout_ndpn;
   const std::vector<int>& u = c.layout_unknowns;
@@ -1239,6 +1296,7 @@ void amg_setup(Context& c, bool defer_coarse) {
       std::vector<int> ia, ib;
       symmetrize_pattern(rawpat, spat, ia, ib);
       lv.Ptent.upload(pt, s);
+      if (l == 0 && c.mf) build_tent_blocks(lv, pt, s);
       {
         HCsr rt;
         std::vector<int> rtsrc;

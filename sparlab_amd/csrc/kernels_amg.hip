@@ -480,6 +480,58 @@ __global__ void k_copy(long long n, const double* __restrict__ in, double* __res
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
+// ---- tentative-prolongator transfers ----------------------------------------
+// P-hat is block diagonal over the aggregates: fine row r holds the len_r <= 6
+// consecutive columns c0_r.. of its aggregate. Rows store c0 * 8 + len and the
+// values column-major (ell[k * n + r]), so a thread per row reads coalesced.
+template <typename T>
+__global__ void __launch_bounds__(256) k_tent_prolong(int n, const int* __restrict__ c0len, const T* __restrict__ ell,
+                                                      const T* __restrict__ xc, T* __restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int e = c0len[r], c0 = e >> 3, len = e & 7;
+  T acc = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    if (k < len) acc += ell[(size_t)k * n + r] * xc[c0 + k];
+  y[r] = acc;
+}
+
+// xc = P-hat^T y: 32 lanes per aggregate over its fine rows, then a reduction
+// across the lanes for each of its columns. ella holds the values in the
+// aggregates' row order (ella[k * n + i] for member i), so the lanes read them
+// contiguously and only y is gathered.
+template <typename T>
+__global__ void __launch_bounds__(256) k_tent_restrict(int naggs, const int* __restrict__ agg_ptr,
+                                                       const int* __restrict__ agg_rows,
+                                                       const int* __restrict__ agg_c0len, int n,
+                                                       const T* __restrict__ ella, const T* __restrict__ y,
+                                                       T* __restrict__ xc) {
+  const int a = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+  const int lane = threadIdx.x & 31;
+  if (a >= naggs) return;  // whole 32-lane groups leave together
+  const int e = agg_c0len[a], c0 = e >> 3, len = e & 7;
+  T acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = agg_ptr[a] + lane; i < agg_ptr[a + 1]; i += 32) {
+    const T yr = y[agg_rows[i]];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      if (k < len) acc[k] += ella[(size_t)k * n + i] * yr;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, 32);
+  }
+  if (lane < len) {
+    T v = acc[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k)
+      if (lane == k) v = acc[k];
+    xc[c0 + lane] = v;
+  }
+}
+
 // ---- dense inverse of the coarsest operator --------------------------------
 // Blocked in-place Gauss-Jordan on the row-major n x n matrix, one pivot block
 // K of kGjB rows at a time (the sweep of K):
@@ -746,6 +798,20 @@ void dense_inverse(int n, const int* ptr, const int* col, const double* val, dou
     k_gj_update<<<dim3(tiles, tiles), 256, 0, s>>>(n, k0, a, pbuf, cbuf);
   }
 }
+
+template <typename T>
+void tent_prolong(const TentBlocks& t, const T* ell, const T* xc, T* y, hipStream_t s) {
+  if (t.n) k_tent_prolong<T><<<nblk(t.n, 256), 256, 0, s>>>(t.n, t.c0len, ell, xc, y);
+}
+template <typename T>
+void tent_restrict(const TentBlocks& t, const T* ella, const T* y, T* xc, hipStream_t s) {
+  if (t.naggs) k_tent_restrict<T><<<nblk((long long)t.naggs * 32, 256), 256, 0, s>>>(t.naggs, t.agg_ptr, t.agg_rows,
+                                                                                    t.agg_c0len, t.n, ella, y, xc);
+}
+template void tent_prolong<double>(const TentBlocks&, const double*, const double*, double*, hipStream_t);
+template void tent_prolong<float>(const TentBlocks&, const float*, const float*, float*, hipStream_t);
+template void tent_restrict<double>(const TentBlocks&, const double*, const double*, double*, hipStream_t);
+template void tent_restrict<float>(const TentBlocks&, const float*, const float*, float*, hipStream_t);
 
 }  // namespace k
 }  // namespace slx

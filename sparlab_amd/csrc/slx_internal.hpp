@@ -147,6 +147,12 @@ struct Level {
   ProductPlan plan_p;          // P = (I - w D^-1 A) P-hat
   RowPlan plan_ap, plan_raw;   // A P, R (A P)
   DBuf<double> dinv, sq, lz_v, lz_vprev, lz_w, lz_start, lz_scal, lz_part;
+  // P-hat's aggregate blocks for the level-0 transfers (amg.cpp build_tent_blocks)
+  bool tent_ok = false;
+  int tent_naggs = 0;
+  DBuf<int> tent_c0len, tent_agg_ptr, tent_agg_rows, tent_agg_c0len;
+  DBuf<double> tent_ell, tent_ella;  // by fine row; in the aggregates' member order
+  DBuf<float> tent_ell32, tent_ella32;
   DBuf<double> x, b, res, d, d2;
   DBuf<float> dinv32, x32, b32, res32, d32, d232;
 };

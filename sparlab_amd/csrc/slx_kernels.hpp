@@ -7,6 +7,24 @@
 namespace slx {
 namespace k {
 
+// The tentative prolongator's aggregate blocks (built in amg.cpp): per fine row
+// c0 * 8 + len (its len <= 6 consecutive columns from c0), values column-major
+// (ell[k * n + r]); per aggregate its fine rows (agg_ptr, agg_rows) and c0 * 8 + len.
+struct TentBlocks {
+  int n = 0, naggs = 0;
+  const int* c0len = nullptr;
+  const int* agg_ptr = nullptr;
+  const int* agg_rows = nullptr;
+  const int* agg_c0len = nullptr;
+};
+// y = P-hat xc (ell: values by fine row) and xc = P-hat^T y (ella: the same
+// values in the aggregates' member order, ella[k * n + i] for agg_rows[i])
+template <typename T>
+void tent_prolong(const TentBlocks& t, const T* ell, const T* xc, T* y, hipStream_t s);
+template <typename T>
+void tent_restrict(const TentBlocks& t, const T* ella, const T* y, T* xc, hipStream_t s);
+
+
 // operator products: y = A x, y += A x, r = b - A x (FP64 and FP32 storage)
 void spmv(const DCsr& A, const double* x, double* y, hipStream_t s);
 void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s);

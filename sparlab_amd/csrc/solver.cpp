@@ -73,8 +73,20 @@ void level_residual(Context& c, size_t l, const T* x, const T* b, T* r) {
 // Level-0 transfers with the matrix-free operator: the smoothed prolongator is
 // P = (I - omega D^-1 K) P-hat (Multigrid.cpp:686-701) and R = P^T =
 // P-hat^T (I - omega K D^-1), so both apply as one operator pass and one
-// product with the tentative prolongator (one block per node) instead of
-// reading the assembled P and R. scratch is a free level-0 vector.
+// product with the tentative prolongator (through its aggregate blocks when
+// build_tent_blocks found them) instead of reading the assembled P and R.
+// scratch is a free level-0 vector.
+k::TentBlocks tent_blocks(const Level& lv) {
+  k::TentBlocks t;
+  t.n = lv.n;
+  t.naggs = lv.tent_naggs;
+  t.c0len = lv.tent_c0len.p;
+  t.agg_ptr = lv.tent_agg_ptr.p;
+  t.agg_rows = lv.tent_agg_rows.p;
+  t.agg_c0len = lv.tent_agg_c0len.p;
+  return t;
+}
+
 template <typename T>
 void restrict_mf(Context& c, const T* r, T* scratch, T* bc);
 template <typename T>
@@ -88,7 +100,8 @@ void restrict_mf<double>(Context& c, const double* r, double* scratch, double* b
   a.c1 = lv.omega;
   a.y = scratch;
   kmf::apply(a, 5, c.stream);
-  k::spmv(lv.Rt, scratch, bc, c.stream);
+  if (lv.tent_ok) k::tent_restrict(tent_blocks(lv), lv.tent_ella.p, static_cast<const double*>(scratch), bc, c.stream);
+  else k::spmv(lv.Rt, scratch, bc, c.stream);
 }
 template <>
 void restrict_mf<float>(Context& c, const float* r, float* scratch, float* bc) {
@@ -99,12 +112,14 @@ void restrict_mf<float>(Context& c, const float* r, float* scratch, float* bc) {
   a.c1 = lv.omega;
   a.y = scratch;
   kmf::apply(a, 5, c.stream);
-  k::spmv32(lv.Rt, scratch, bc, c.stream);
+  if (lv.tent_ok) k::tent_restrict(tent_blocks(lv), lv.tent_ella32.p, static_cast<const float*>(scratch), bc, c.stream);
+  else k::spmv32(lv.Rt, scratch, bc, c.stream);
 }
 template <>
 void prolong_mf<double>(Context& c, const double* xc, double* scratch, double* x) {
   Level& lv = c.levels[0];
-  k::spmv(lv.Ptent, xc, scratch, c.stream);
+  if (lv.tent_ok) k::tent_prolong(tent_blocks(lv), lv.tent_ell.p, xc, scratch, c.stream);
+  else k::spmv(lv.Ptent, xc, scratch, c.stream);
   kmf::MfArgs<double> a = mf_base<double>(c);
   a.x = scratch;
   a.dinv = lv.dinv.p;
@@ -115,7 +130,8 @@ void prolong_mf<double>(Context& c, const double* xc, double* scratch, double* x
 template <>
 void prolong_mf<float>(Context& c, const float* xc, float* scratch, float* x) {
   Level& lv = c.levels[0];
-  k::spmv32(lv.Ptent, xc, scratch, c.stream);
+  if (lv.tent_ok) k::tent_prolong(tent_blocks(lv), lv.tent_ell32.p, xc, scratch, c.stream);
+  else k::spmv32(lv.Ptent, xc, scratch, c.stream);
   kmf::MfArgs<float> a = mf_base<float>(c);
   a.x = scratch;
   a.dinv = lv.dinv32.p;
@@ -389,7 +405,9 @@ void prepare_mixed(Context& c) {
     } else {
       A.val16.release();
     }
-    if (lv.has_next && use_mf(c, l)) {
+    if (lv.has_next && use_mf(c, l) && lv.tent_ok) {
+      // the transfers read P-hat's aggregate blocks (FP32 copy made with them)
+    } else if (lv.has_next && use_mf(c, l)) {
       lv.Ptent.val32.alloc((size_t)lv.Ptent.nnz);
       k::to_float(lv.Ptent.nnz, lv.Ptent.val.p, lv.Ptent.val32.p, s);
       lv.Rt.val32.alloc((size_t)lv.Rt.nnz);
