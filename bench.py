@@ -132,8 +132,9 @@ def level_info(g, level, which):
 
 
 def measured_traffic(name, sizes):
-    """HBM bytes per launch from the committed rocprofv3 counter summary, when its sizes match the run."""
-    path = os.path.join(ROOT, "profiles", "traffic_row_products_cantilever_128x64x64.txt")
+    """HBM bytes per launch from the committed rocprofv3 counter summary, when its sizes (three
+    integers: nnz of A, B, C for a product; free DOFs, elements, nodes for the operator) match the run."""
+    path = os.path.join(ROOT, "profiles", "traffic_level0_kernels_cantilever_128x64x64.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -141,8 +142,7 @@ def measured_traffic(name, sizes):
             parts = line.split()
             if not parts or parts[0].startswith("#") or parts[0] != name or len(parts) != 8:
                 continue
-            na, nb, nc = (int(v) for v in parts[1:4])
-            if (na, nb, nc) != (sizes["nnz_a"], sizes["nnz_b"], sizes["nnz_c"]):
+            if tuple(int(v) for v in parts[1:4]) != tuple(sizes):
                 return None
             fk, wk, fs, ws = float(parts[4]), float(parts[5]), float(parts[6]), float(parts[7])
             return fk * 1024 * fs + wk * 1024 * ws
@@ -169,6 +169,27 @@ def smoother_roofline(g, fp32):
                       + f", {'FP32' if fp32 else 'FP64'} CSR with shared column lists)",
             "bytes_per_launch": nbytes, "avg_ms": avg, "launches": launches, "total_ms": avg * launches,
             "sizes": {"rows": rows, "nnz": nnz, "nodes": nodes, "shared_columns": ncol}}
+
+
+def mf_smoother_roofline(g, m, fp32, vcycles):
+    """The level-0 Chebyshev step through the matrix-free operator (k_mf_hex8 mode 3), the kernel
+    with the largest share of a SIMP step: timed with HIP events around 50 back-to-back launches.
+    Algorithmic HBM bytes per launch: 7 vector words per free DOF (d, r and x read; r, d and x
+    written; D^-1 read), one word per element factor and the 3 free indices of each node.
+    Launches in the timed region: 2 (degree - 1) = 4 per V-cycle."""
+    word = 4 if fp32 else 8
+    launches = 50
+    avg = g.time_smoother(0, launches)
+    g.amg_apply(np.ones(g.nfree))  # the level's vectors back to a V-cycle's state
+    nnodes = m.nnodes
+    nbytes = 7 * word * g.nfree + word * m.nelem + 12 * nnodes
+    ach = nbytes / (avg * 1e-3) / 1e9
+    in_run = 4 * vcycles
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": measured_traffic("mf_cheb32" if fp32 else "mf_cheb64", (g.nfree, m.nelem, nnodes)),
+            "kernel": f"k_mf_hex8<{'float' if fp32 else 'double'},3> (level-0 Chebyshev step, matrix-free)",
+            "bytes_per_launch": nbytes, "avg_ms": avg, "launches": in_run, "total_ms": avg * in_run,
+            "launches_timed": launches, "sizes": {"free_dofs": g.nfree, "elements": m.nelem, "nodes": nnodes}}
 
 
 def rooflines(g, m, timers):
@@ -209,7 +230,8 @@ def rooflines(g, m, timers):
         ach = nbytes / (avg * 1e-3) / 1e9
         sz = {"nnz_a": a_nnz, "nnz_b": b_nnz, "nnz_c": c_nnz, "nodes": nodes, "runs": runs, "entries": entries}
         out.append({"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                    "traffic": measured_traffic(name, sz), "kernel": label, "bytes_per_launch": nbytes,
+                    "traffic": measured_traffic(name, (a_nnz, b_nnz, c_nnz)), "kernel": label,
+                    "bytes_per_launch": nbytes,
                     "avg_ms": avg, "launches": launches, "total_ms": ms, "sizes": sz})
     out.sort(key=lambda r: -r["total_ms"])
     return out
@@ -359,9 +381,13 @@ def main():
     solves = head["steps"] + 1  # run_oc re-evaluates the final design
     value = solves / head["elapsed"]
     roofs = rooflines(g, m, head["timers"])
+    # V-cycles of the timed region: one per CG iteration and one for each solve's first residual
+    vcycles = int(np.sum(head["iterations"])) + len(head["iterations"])
+    roofs.append(mf_smoother_roofline(g, m, args.precision == "mixed", vcycles))
     sm = smoother_roofline(g, args.precision == "mixed")
     if sm is not None:
         roofs.append(sm)
+    roofs.sort(key=lambda r: -r["total_ms"])
     config = {"workload": f"cantilever Hex8 {nx}x{ny}x{nz} SIMP/OC from x0 = 0.2, {n} free DOFs, rtol {tol}; "
                           f"{'FP32 V-cycle under FP64 flexible CG' if args.precision == 'mixed' else 'FP64'}"
                           f"{' + learned warm start' if not args.no_learned else ''}",

@@ -274,9 +274,10 @@ slx_status slx_solve_segmented(slx_context* ctx, int nseg, const int* seg_ptr, c
                                int max_iterations, slx_pcg_result* results);
 slx_status slx_amg_apply(slx_context* ctx, const double* r, double* z);
 /* Measurement (no counterpart in the reference): the average time of one
- * Chebyshev step of a level strictly between the finest and the coarsest, over
- * reps back-to-back launches between two device events. It overwrites that
- * level's work vectors (the next V-cycle sets them again). */
+ * Chebyshev step of a level above the coarsest (level 0: the matrix-free
+ * operator's step when the level-0 products are matrix-free), over reps
+ * back-to-back launches between two device events. It overwrites that level's
+ * work vectors (the next V-cycle sets them again). */
 slx_status slx_amg_time_smoother(slx_context* ctx, int level, int reps, double* ms_per_step);
 slx_status slx_amg_stats_get(const slx_context* ctx, slx_amg_stats* out);
 /* Test access to the hierarchy. which: 0 A_l, 1 P, 2 P-hat, 3 R, 4 A P, 5 R (A P):

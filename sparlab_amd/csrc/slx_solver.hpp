@@ -13,8 +13,8 @@ namespace slx {
 // so the setup returns without reading the pivots back
 void amg_setup(Context& c, bool defer_coarse = false);
 void finish_coarse(Context& c);
-// average milliseconds of one Chebyshev step at a middle level, over reps
-// back-to-back launches between two events (measurement only)
+// average milliseconds of one Chebyshev step at a level above the coarsest,
+// over reps back-to-back launches between two events (measurement only)
 double time_smoother(Context& c, int level, int reps);
 // FP32 copies of the hierarchy for the mixed-precision V-cycle.
 void prepare_mixed(Context& c);

@@ -458,8 +458,8 @@ void vcycle(Context& c, const double* r, double* z) {
 
 double time_smoother(Context& c, int level, int reps) {
   if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: no hierarchy to time");
-  if (level < 1 || level + 1 >= (int)c.levels.size() || reps < 1)
-    fail(SLX_CONFIG_ERROR, "smoother timing: the level must lie strictly between the finest and the coarsest");
+  if (level < 0 || level + 1 >= (int)c.levels.size() || reps < 1)
+    fail(SLX_CONFIG_ERROR, "smoother timing: the level must lie above the coarsest");
   Level& lv = c.levels[(size_t)level];
   DCsr& A = op(c, (size_t)level);
   const bool f32 = c.amg_opt.precision >= 1;
@@ -472,11 +472,21 @@ double time_smoother(Context& c, int level, int reps) {
     k::copy(lv.n, lv.dinv.p, lv.d.p, s);
     k::copy(lv.n, lv.dinv.p, lv.res.p, s);
   }
+  const bool mf = use_mf(c, (size_t)level);
   auto step = [&] {
-    if (f32)
+    if (mf && f32) {
+      kmf::MfArgs<float> a = mf_base<float>(c);
+      a.x = lv.d32.p, a.b = lv.res32.p, a.y = lv.b32.p, a.dinv = lv.dinv32.p, a.d_out = lv.d232.p, a.xs = lv.x32.p;
+      kmf::apply(a, 3, s);
+    } else if (mf) {
+      kmf::MfArgs<double> a = mf_base<double>(c);
+      a.x = lv.d.p, a.b = lv.res.p, a.y = lv.b.p, a.dinv = lv.dinv.p, a.d_out = lv.d2.p, a.xs = lv.x.p;
+      kmf::apply(a, 3, s);
+    } else if (f32) {
       k::cheb_step32(A, lv.dinv32.p, lv.res32.p, lv.b32.p, lv.d32.p, lv.d232.p, lv.x32.p, 0.0, 0.0, s);
-    else
+    } else {
       k::cheb_step(A, lv.dinv.p, lv.res.p, lv.b.p, lv.d.p, lv.d2.p, lv.x.p, 0.0, 0.0, s);
+    }
   };
   step();  // warm
   hipEvent_t a = c.take_event(), b = c.take_event();
