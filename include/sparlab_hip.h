@@ -32,6 +32,8 @@
  *   slx_evaluate         ComplianceObjective::evaluate (src/topopt/Sensitivity.cpp:110-237)
  *   slx_oc_update        optimality_criteria_update (src/topopt/OptimalityCriteria.cpp:40-149)
  *   slx_run_oc           TopologyOptimizer::run_oc (src/topopt/TopologyOptimizer.cpp:556-724)
+ *   slx_modal_solve      solve_modal's shift-invert subspace iteration with warm-started
+ *                        AMG-CG solves (src/fem/ModalAnalysis.cpp:120-418)
  *
  * Additions with no counterpart in the reference:
  *   slx_warmstart_*      a learned initial guess for the compliance solves, which
@@ -310,6 +312,47 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
 /* compliance[ncases]: f_l . u_l of each load case at the last evaluation
  * (ObjectiveEvaluation::load_case_compliance, include/sparlab/topopt/Sensitivity.hpp:78) */
 slx_status slx_load_case_compliance_get(const slx_context* ctx, double* compliance);
+
+/* ---- free vibration ----------------------------------------------------------
+ * ModalAnalysisOptions and ModalResult (include/sparlab/fem/ModalAnalysis.hpp:44-90).
+ * The shifted systems are solved with the context's AMG-PCG (the options of the
+ * last slx_amg_setup, or the defaults), each warm started from the converging
+ * subspace. Only shift_factor = 0 (the reference's default) is supported. */
+typedef struct {
+  int num_modes;              /* lowest modes requested (6) */
+  int lumped;                 /* 0: consistent mass (default), 1: row-sum lumped */
+  int max_iterations;         /* subspace iterations (200) */
+  double tolerance;           /* relative eigenvalue change (1e-10) */
+  double residual_tolerance;  /* eigenpair residual |K phi - lambda M phi| / |lambda M phi| (1e-6) */
+  double rigid_body_ratio;    /* eigenvalues below this times max K_ii / max M_ii are flagged (1e-10) */
+  double shift_factor;        /* must be 0 */
+  double density;             /* kg/m^3, > 0 */
+  double linear_tolerance;    /* relative residual of each AMG-PCG solve (1e-8) */
+  int linear_max_iterations;  /* per solve (1000) */
+} slx_modal_options;
+
+typedef struct {
+  int num_modes;              /* returned modes: min(num_modes, free DOFs) */
+  int subspace_size;          /* q = min(n, max(2 m, m + 8)) */
+  int iterations;
+  int converged;
+  int rigid_body_modes;       /* modes flagged as rigid-body / mechanism (warnings in the reference) */
+  int residual_warnings;      /* modes whose final residual exceeds residual_tolerance */
+  long long linear_iterations;
+  double final_change;
+  double total_mass;          /* kg: sum over elements of scale * density * volume */
+  double seconds;
+} slx_modal_result;
+
+void slx_modal_default_options(slx_modal_options* o);
+/* Lowest eigenpairs of (K_ff, M_ff) for the context's model. stiffness_scale,
+ * mass_scale [nelem]: per-element factors (NULL: 1). eigenvalues[num_modes]
+ * (1/s^2, ascending), residuals[num_modes], shapes[ndofs * num_modes] (may be
+ * NULL; column-major, M-orthonormal, zero on the fixed DOFs). Leaves the
+ * context's K_ff assembled and factorized for stiffness_scale. */
+slx_status slx_modal_solve(slx_context* ctx, const slx_modal_options* opt, const double* stiffness_scale,
+                           const double* mass_scale, double* eigenvalues, double* residuals, double* shapes,
+                           slx_modal_result* result);
 
 /* ---- learned warm start ------------------------------------------------------ */
 void slx_warmstart_default_options(slx_warmstart_options* o);

@@ -9,6 +9,7 @@
 #include <sstream>
 
 #include "slx_kernels.hpp"
+#include "slx_modal.hpp"
 #include "slx_model.hpp"
 #include "slx_net.hpp"
 #include "slx_solver.hpp"
@@ -950,6 +951,65 @@ slx_status slx_load_case_compliance_get(const slx_context* ctx, double* complian
   return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
     if (c.case_compliance.empty()) fail(SLX_CONFIG_ERROR, "no evaluation has run yet");
     std::copy(c.case_compliance.begin(), c.case_compliance.end(), compliance);
+  });
+}
+
+void slx_modal_default_options(slx_modal_options* o) {
+  // ModalAnalysisOptions (include/sparlab/fem/ModalAnalysis.hpp:44-68), Config.cpp:1120-1136
+  o->num_modes = 6;
+  o->lumped = 0;
+  o->max_iterations = 200;
+  o->tolerance = 1.0e-10;
+  o->residual_tolerance = 1.0e-6;
+  o->rigid_body_ratio = 1.0e-10;
+  o->shift_factor = 0.0;
+  o->density = 0.0;
+  o->linear_tolerance = 1.0e-8;
+  o->linear_max_iterations = 1000;
+}
+
+slx_status slx_modal_solve(slx_context* ctx, const slx_modal_options* opt, const double* stiffness_scale,
+                           const double* mass_scale, double* eigenvalues, double* residuals, double* shapes,
+                           slx_modal_result* result) {
+  return guarded(ctx, [&](Context& c) {
+    require_model(c);
+    slx_modal_options o;
+    slx_modal_default_options(&o);
+    if (opt) o = *opt;
+    if (o.num_modes < 1) fail(SLX_CONFIG_ERROR, "modal analysis requires num_modes >= 1");
+    if (!(o.density > 0.0))
+      fail(SLX_MODEL_ERROR,
+           "mass assembly requires a positive density of every material. Set its 'density' to run a modal or "
+           "dynamic analysis");
+    if (o.shift_factor != 0.0)
+      fail(SLX_CONFIG_ERROR, "modal analysis on the device supports shift_factor = 0 only");
+    if (o.max_iterations < 1) fail(SLX_CONFIG_ERROR, "modal.max_iterations must be at least 1");
+    const size_t ne = (size_t)c.nelem;
+    auto check_scale = [&](const double* h, const char* what) {
+      if (!h) return;
+      for (size_t e = 0; e < ne; ++e)
+        if (!std::isfinite(h[e]) || h[e] < 0.0)
+          fail(SLX_MODEL_ERROR, std::string(what) + " scale vector has a negative or non-finite entry");
+    };
+    check_scale(stiffness_scale, "stiffness");
+    check_scale(mass_scale, "mass");
+    double total_mass = 0.0;
+    for (size_t e = 0; e < ne; ++e) total_mass += (mass_scale ? mass_scale[e] : 1.0) * o.density * c.volumes[e];
+    std::vector<double> ones;
+    const double* h = stiffness_scale;
+    if (!h) {
+      ones.assign(ne, 1.0);
+      h = ones.data();
+    }
+    c.d_scale.upload(h, ne, c.stream);
+    assemble(c, c.d_scale.p);
+    amg_setup(c);
+    DBuf<double> dm;
+    if (mass_scale) dm.upload(mass_scale, ne, c.stream);
+    slx_modal_result r{};
+    modal_solve(c, o, mass_scale ? dm.p : nullptr, total_mass, eigenvalues, residuals, shapes, r);
+    c.have_u.assign(c.have_u.size(), 0);  // the stored solutions belong to another operator
+    if (result) *result = r;
   });
 }
 

@@ -103,6 +103,22 @@ int filter_volume_blocks(const DCsr& H);
 void filter_volume(const DCsr& H, const double* x, const double* vol, double* partials, double* out, hipStream_t s);
 void max_abs_diff(int n, const double* a, const double* b, double* partials, double* out, hipStream_t s);
 void clamp(int n, double* x, const double* lo, const double* hi, hipStream_t s);
+// modal analysis (kernels_modal.hip). y = M x on the free DOFs of a structured
+// grid (rho_v = density * element volume; scale: per-element factors or null),
+// or its diagonal; lumped: row-sum lumping
+void mass_apply(int dim, int nx, int ny, int nz, double rho_v, bool lumped, bool diagonal, const double* scale,
+                const int* reduced, const double* x, double* y, hipStream_t s);
+void csr_diagonal(const DCsr& A, double* d, hipStream_t s);
+// doubles of partials gram() and eig_residuals() need for n rows and q columns
+int modal_partials(int n, int q);
+// out[i * q + j] = A_i . B_j for column-major n x q blocks A, B
+void gram(int n, int q, const double* A, const double* B, double* partials, double* out, hipStream_t s);
+// Y = X R: X n x q, R q x m column-major, Y n x m (q, m <= 64)
+void block_combine(int n, int q, int m, const double* X, const double* R, double* Y, hipStream_t s);
+// out[2 i] = |KX_i - mu_i MX_i|^2, out[2 i + 1] = |mu_i MX_i|^2 (mu on the device)
+void eig_residuals(int n, int m, const double* KX, const double* MX, const double* mu, double* partials, double* out,
+                   hipStream_t s);
+void scale_copy(int n, double a, const double* x, double* y, hipStream_t s);
 
 // SA-AMG numeric setup (kernels_amg.hip)
 struct BlockProductArgs {

@@ -308,6 +308,9 @@ class Deck:
     passive_void: int
     run: dict                   # keyword arguments of HipProblem.run_oc
     history_stride: int = 1
+    density: float = 0.0        # kg/m^3 (material.density)
+    modal: dict | None = None   # keyword arguments of HipProblem.modal when modal.enabled, else None
+    modal_flags: dict | None = None  # analyse_optimised_topology, compare_mass_matched_baseline
 
     @property
     def domain_volume(self) -> float:
@@ -571,10 +574,33 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     )
     lin = doc.get("solver", {}).get("linear", {})
     tol = float(lin.get("residual_tolerance", 1e-8))
+    density = float(doc.get("material", {}).get("density", 0.0))
+    # modal block (Config.cpp:1118-1143)
+    mnode = doc.get("modal", {})
+    modal = flags = None
+    if mnode.get("enabled", False):
+        if not density > 0.0:
+            raise DeckError("modal analysis is enabled but material.density is zero; set a positive density [kg/m^3]")
+        mass_type = str(mnode.get("mass_type", "consistent")).lower()
+        if mass_type not in ("consistent", "lumped"):
+            raise DeckError(f"unknown modal.mass_type '{mass_type}' (consistent or lumped)")
+        if float(mnode.get("shift_factor", 0.0)) != 0.0:
+            _unsupported("modal.shift_factor", "a shifted eigenproblem")
+        modal = dict(num_modes=int(mnode.get("num_modes", 6)), mass_type=mass_type,
+                     max_iterations=int(mnode.get("max_iterations", 200)),
+                     tolerance=float(mnode.get("tolerance", 1e-10)),
+                     residual_tolerance=float(mnode.get("residual_tolerance", 1e-6)),
+                     rigid_body_ratio=float(mnode.get("rigid_body_ratio", 1e-10)),
+                     linear_tolerance=tol)
+        if modal["num_modes"] < 1:
+            raise DeckError("modal analysis requires num_modes >= 1")
+        flags = dict(analyse_optimised_topology=bool(mnode.get("analyse_optimised_topology", True)),
+                     compare_mass_matched_baseline=bool(mnode.get("compare_mass_matched_baseline", True)))
     return Deck(name=name, description=doc.get("description", ""), source=source, model=m, origin=origin,
                 case_names=names, case_weights=weights, tolerance=tol, filter_type=ftype, filter_radius=radius,
                 initial_density=start, passive_solid=int((tag == 1).sum()), passive_void=int((tag == 2).sum()),
-                run=run, history_stride=int(opt.get("history_stride", 1)))
+                run=run, history_stride=int(opt.get("history_stride", 1)), density=density, modal=modal,
+                modal_flags=flags)
 
 
 def load_deck(path: str, overrides: dict | None = None) -> Deck:

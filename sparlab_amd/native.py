@@ -117,6 +117,18 @@ H_ITER, H_PENALTY, H_COMPLIANCE, H_VOLUME, H_VF, H_CHANGE, H_LAMBDA, H_GREY, H_B
     range(len(HISTORY_COLUMNS))
 
 
+class ModalOptions(C.Structure):
+    _fields_ = [("num_modes", C.c_int), ("lumped", C.c_int), ("max_iterations", C.c_int), ("tolerance", C.c_double),
+                ("residual_tolerance", C.c_double), ("rigid_body_ratio", C.c_double), ("shift_factor", C.c_double),
+                ("density", C.c_double), ("linear_tolerance", C.c_double), ("linear_max_iterations", C.c_int)]
+
+
+class ModalResult(C.Structure):
+    _fields_ = [("num_modes", C.c_int), ("subspace_size", C.c_int), ("iterations", C.c_int), ("converged", C.c_int),
+                ("rigid_body_modes", C.c_int), ("residual_warnings", C.c_int), ("linear_iterations", C.c_longlong),
+                ("final_change", C.c_double), ("total_mass", C.c_double), ("seconds", C.c_double)]
+
+
 class WarmStartOptions(C.Structure):
     _fields_ = [("enable", C.c_int), ("train_solves", C.c_int), ("min_samples", C.c_int),
                 ("steps_per_solve", C.c_int), ("buffer", C.c_int), ("learning_rate", C.c_double),
@@ -176,6 +188,8 @@ _SIGNATURES = {
     "slx_oc_update": (C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(OcOptions), _VP, C.POINTER(OcStep)]),
     "slx_run_oc": (C.c_int, [_VP, C.POINTER(RunOptions), _VP, _VP, C.POINTER(RunResult)]),
     "slx_load_case_compliance_get": (C.c_int, [_VP, _VP]),
+    "slx_modal_default_options": (None, [C.POINTER(ModalOptions)]),
+    "slx_modal_solve": (C.c_int, [_VP, C.POINTER(ModalOptions), _VP, _VP, _VP, _VP, _VP, C.POINTER(ModalResult)]),
     "slx_warmstart_default_options": (None, [C.POINTER(WarmStartOptions)]),
     "slx_warmstart_configure": (C.c_int, [_VP, C.POINTER(WarmStartOptions)]),
     "slx_warmstart_stats_get": (C.c_int, [_VP, C.POINTER(WarmStartStats)]),
@@ -624,6 +638,67 @@ class HipProblem:
         self.last_run = {name: getattr(r, name) for name, _ in RunResult._fields_}
         self.last_run["stop_reason"] = STOP_REASONS[r.stop_reason]
         return x, h, r.iterations, r.compliance
+
+    def modal(self, density, num_modes=6, mass_type="consistent", max_iterations=200, tolerance=1e-10,
+              residual_tolerance=1e-6, rigid_body_ratio=1e-10, shift_factor=0.0, linear_tolerance=1e-8,
+              linear_max_iterations=1000, stiffness_scale=None, mass_scale=None, shapes=False):
+        """solve_modal (src/fem/ModalAnalysis.cpp:120-418) on the device: the lowest eigenpairs of
+        (K_ff, M_ff) by subspace iteration with warm-started AMG-PCG solves. Returns the fields of the
+        reference's modal_json (src/io/ResultWriter.cpp:114-136) plus ``mode_shapes`` (ndofs x m,
+        M-orthonormal) when ``shapes``. Re-assembles and re-factorizes the context's K_ff."""
+        self._own()
+        if mass_type not in ("consistent", "lumped"):
+            raise ConfigError(f"unknown modal.mass_type '{mass_type}' (consistent or lumped)")
+        o = ModalOptions()
+        load().slx_modal_default_options(C.byref(o))
+        o.num_modes = int(num_modes)
+        o.lumped = 1 if mass_type == "lumped" else 0
+        o.max_iterations = int(max_iterations)
+        o.tolerance = tolerance
+        o.residual_tolerance = residual_tolerance
+        o.rigid_body_ratio = rigid_body_ratio
+        o.shift_factor = shift_factor
+        o.density = float(density)
+        o.linear_tolerance = linear_tolerance
+        o.linear_max_iterations = int(linear_max_iterations)
+        m = max(1, min(int(num_modes), self.nfree))
+        lam = np.zeros(m)
+        res = np.zeros(m)
+        phi = np.zeros((m, self.ndofs)) if shapes else None
+        ks = None if stiffness_scale is None else _f64(stiffness_scale)
+        ms = None if mass_scale is None else _f64(mass_scale)
+        for name, v in (("stiffness", ks), ("mass", ms)):
+            if v is not None and v.shape != (self.ne,):
+                raise ModelError(f"{name} scale vector has {v.size} entries for {self.ne} elements")
+        r = ModalResult()
+        self.ctx.check(load().slx_modal_solve(self.ctx.h, C.byref(o), _p(ks), _p(ms), _p(lam), _p(res), _p(phi),
+                                              C.byref(r)))
+        omega = np.sqrt(np.maximum(lam, 0.0))
+        out = {
+            "num_modes": r.num_modes,
+            "eigenvalues_1_per_s2": lam.tolist(),
+            "angular_frequencies_rad_per_s": omega.tolist(),
+            "frequencies_hz": (omega / (2.0 * np.pi)).tolist(),
+            "eigenpair_residuals": res.tolist(),
+            "total_mass_kg": r.total_mass,
+            "subspace_size": r.subspace_size,
+            "iterations": r.iterations,
+            "converged": bool(r.converged),
+            "final_relative_change": r.final_change,
+            "linear_solver": "amg_cg",
+            "linear_iterations": int(r.linear_iterations),
+            "warnings": [],
+            "seconds": r.seconds,
+        }
+        if r.rigid_body_modes:
+            out["warnings"].append(f"{r.rigid_body_modes} mode(s) indistinguishable from zero at this "
+                                   "stiffness/mass scale: the model retains a rigid-body or mechanism mode")
+        if r.residual_warnings:
+            out["warnings"].append(f"{r.residual_warnings} mode(s) above the eigenpair residual tolerance "
+                                   f"{residual_tolerance}")
+        if shapes:
+            out["mode_shapes"] = phi.T.copy()
+        return out
 
     def load_case_compliance(self):
         """f_l . u_l of each load case at the last evaluation."""

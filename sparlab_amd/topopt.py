@@ -170,6 +170,19 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
         plate = solid / vf if vf > 0 else 0.0
         summary["mass_stiffness_comparison"]["equal_mass_uniform_plate_compliance_J"] = plate
         summary["mass_stiffness_comparison"]["stiffness_gain_over_equal_mass_plate"] = plate / comp if comp > 0 else 0.0
+    if dk.modal is not None:
+        # the full solid domain's lowest modes (apps/sparlab_topopt.cpp:612-620)
+        solid_modes = g.modal(dk.density, **dk.modal)
+        summary["modal_initial_solid"] = {k: v for k, v in solid_modes.items() if k != "seconds"}
+        log(f"{dk.name}: f1 solid {solid_modes['frequencies_hz'][0]:.10g} Hz (mass {solid_modes['total_mass_kg']:.10g} "
+            f"kg), {solid_modes['iterations']} subspace iterations, {solid_modes['linear_iterations']} CG iterations, "
+            f"{solid_modes['seconds']:.2f} s")
+        if m.dim == 2 and dk.modal_flags["compare_mass_matched_baseline"]:
+            # the equal-mass uniform plate (thickness times the achieved volume fraction,
+            # apps/sparlab_topopt.cpp:622-633): K and M both scale with the thickness
+            thin = g.modal(dk.density, stiffness_scale=np.full(m.nelem, vf), mass_scale=np.full(m.nelem, vf),
+                           **dk.modal)
+            summary["modal_mass_matched_baseline"] = {k: v for k, v in thin.items() if k != "seconds"}
 
     if output:
         os.makedirs(output, exist_ok=True)

@@ -23,7 +23,8 @@ CASES = ("cantilever_beam", "mbb_beam", "aerospace_bracket", "bracket_3d", "wing
 FIELDS = {"dim": int, "elements": int, "dofs": int, "volume_fraction_target": float,
           "volume_fraction_achieved": float, "iterations": int, "stop_reason": str, "compliance_J": float,
           "initial_compliance_J": float, "full_solid_compliance_J": float,
-          "equal_mass_plate_compliance_J": float, "grey_level": float, "linear_solver": str}
+          "equal_mass_plate_compliance_J": float, "grey_level": float, "linear_solver": str,
+          "f1_solid_Hz": float, "mass_solid_kg": float, "f1_topology_Hz": float, "mass_topology_kg": float}
 
 
 def main():

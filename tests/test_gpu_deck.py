@@ -80,3 +80,14 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
     assert v == pytest.approx(r["volume_m3"], rel=1e-9)
     saved = json.load(open(tmp_path / "summary.json"))
     assert saved["optimization_result"]["stop_reason"] == r["stop_reason"]
+    # the full solid domain's modes (published f1_solid_Hz, mass_solid_kg)
+    if d.modal is not None:
+        modes = saved["modal_initial_solid"]
+        assert modes["frequencies_hz"][0] == pytest.approx(pub["f1_solid_Hz"], rel=1e-9)
+        assert modes["total_mass_kg"] == pytest.approx(pub["mass_solid_kg"], rel=1e-11)
+        if d.model.dim == 2 and d.modal_flags["compare_mass_matched_baseline"]:
+            thin = saved["modal_mass_matched_baseline"]
+            assert thin["frequencies_hz"] == pytest.approx(modes["frequencies_hz"], rel=1e-8)
+            assert thin["total_mass_kg"] == pytest.approx(modes["total_mass_kg"] * r["volume_fraction"], rel=1e-12)
+    else:
+        assert "modal_initial_solid" not in saved
