@@ -155,3 +155,64 @@ def kff_csr(oracle, scale=None):
     ptr, col, val = oracle.assemble_kff(scale)
     n = len(ptr) - 1
     return sp.csr_matrix((val, col, ptr), shape=(n, n))
+
+
+def components_by_face(model, subset):
+    """components_by_face (src/mesh/SubMesh.cpp:38-90): union-find over the
+    elements of `subset` (ascending) that share a face (an edge in 2-D), keyed by
+    the face's sorted corner nodes; components in order of first member."""
+    nodes = element_nodes(model)
+    faces = ([(0, 1), (1, 2), (2, 3), (3, 0)] if model.dim == 2 else
+             [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)])
+    parent = list(range(len(subset)))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    owner = {}
+    for s, e in enumerate(subset):
+        for f in faces:
+            key = tuple(sorted(int(nodes[e, a]) for a in f))
+            if key in owner:
+                ra, rb = find(owner[key]), find(s)
+                if ra != rb:
+                    parent[rb] = ra
+            else:
+                owner[key] = s
+    comps, index = [], {}
+    for s, e in enumerate(subset):
+        r = find(s)
+        if r not in index:
+            index[r] = len(comps)
+            comps.append([])
+        comps[index[r]].append(int(e))
+    return comps
+
+
+def submesh_operators(model, ke0, density, elements, fixed):
+    """K_ff and M_ff of the mesh made of `elements` alone, nodes renumbered by
+    first appearance (extract_element_subset, src/mesh/SubMesh.cpp:96-140), with
+    the parent's fixed DOF mask restricted to the kept nodes."""
+    dim = model.dim
+    nodes = element_nodes(model)[np.asarray(elements)]
+    old_to_new = -np.ones(model.nnodes, dtype=np.int64)
+    order = []
+    for n in nodes.ravel():
+        if old_to_new[n] < 0:
+            old_to_new[n] = len(order)
+            order.append(n)
+    order = np.array(order)
+    conn = old_to_new[nodes]
+    nn = len(order)
+    dofs = (conn[:, :, None] * dim + np.arange(dim)[None, None, :]).reshape(len(elements), -1)
+    rows = np.repeat(dofs, dofs.shape[1], axis=1).ravel()
+    cols = np.tile(dofs, (1, dofs.shape[1])).ravel()
+    me = element_mass(model, density)
+    K = sp.csr_matrix((np.tile(np.asarray(ke0).ravel(), len(elements)), (rows, cols)), shape=(nn * dim, nn * dim))
+    M = sp.csr_matrix((np.tile(me.ravel(), len(elements)), (rows, cols)), shape=(nn * dim, nn * dim))
+    fx = np.asarray(fixed, dtype=bool).reshape(model.nnodes, dim)[order].ravel()
+    free = np.flatnonzero(~fx)
+    return reduce_free(K, free), reduce_free(M, free)

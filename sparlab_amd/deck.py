@@ -311,6 +311,7 @@ class Deck:
     density: float = 0.0        # kg/m^3 (material.density)
     modal: dict | None = None   # keyword arguments of HipProblem.modal when modal.enabled, else None
     modal_flags: dict | None = None  # analyse_optimised_topology, compare_mass_matched_baseline
+    interpretation_threshold: float = 0.5  # optimizer.interpretation_threshold (Config.cpp:1571)
 
     @property
     def domain_volume(self) -> float:
@@ -575,6 +576,9 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     lin = doc.get("solver", {}).get("linear", {})
     tol = float(lin.get("residual_tolerance", 1e-8))
     density = float(doc.get("material", {}).get("density", 0.0))
+    thr = float(opt.get("interpretation_threshold", 0.5))
+    if not (0.0 < thr < 1.0):
+        raise DeckError("optimizer.interpretation_threshold must lie in (0, 1)")
     # modal block (Config.cpp:1118-1143)
     mnode = doc.get("modal", {})
     modal = flags = None
@@ -600,7 +604,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
                 case_names=names, case_weights=weights, tolerance=tol, filter_type=ftype, filter_radius=radius,
                 initial_density=start, passive_solid=int((tag == 1).sum()), passive_void=int((tag == 2).sum()),
                 run=run, history_stride=int(opt.get("history_stride", 1)), density=density, modal=modal,
-                modal_flags=flags)
+                modal_flags=flags, interpretation_threshold=thr)
 
 
 def load_deck(path: str, overrides: dict | None = None) -> Deck:

@@ -1,0 +1,116 @@
+"""The optimised density field read as a solid body, and its free vibration.
+
+interpret_density_as_solid follows src/mesh/SubMesh.cpp:172-260: elements
+with a physical density at or above the threshold are kept, the largest
+face-connected group of them (edge-connected in 2-D; components_by_face,
+SubMesh.cpp:38-90) is retained, ties going to the group whose lowest element
+index comes first, and the volumes above the threshold, retained and
+discarded as islands are reported.
+
+The reference then builds a new mesh of the retained elements, renumbers its
+nodes and re-applies the deck's supports to it (apps/sparlab_topopt.cpp:92-130).
+On a structured grid the same operators come out of the parent grid directly:
+per-element stiffness and mass factors of 1 on retained elements and 0
+elsewhere, and every DOF of a node that no retained element touches fixed.
+K_ff and M_ff are then the sub-model's matrices with the same unknowns in the
+same relative order, so the eigenpairs are the same.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+from scipy import ndimage
+
+from .model import StructuredModel
+
+
+class InterpretationError(RuntimeError):
+    """No element reaches the threshold (the reference's MeshError)."""
+
+
+@dataclasses.dataclass
+class Interpretation:
+    threshold: float
+    elements_above_threshold: int
+    elements_retained: int
+    components_above_threshold: int
+    volume_above_threshold: float
+    volume_retained: float
+    volume_discarded_as_islands: float
+    retained: np.ndarray  # bool per element
+
+    def as_dict(self) -> dict:
+        d = dataclasses.asdict(self)
+        d.pop("retained")
+        return d
+
+
+def element_volume(model: StructuredModel) -> float:
+    return float(np.prod(model.cell)) * (model.thickness if model.dim == 2 else 1.0)
+
+
+def _grid_shape(model: StructuredModel) -> tuple:
+    return (model.ny, model.nx) if model.dim == 2 else (model.nz, model.ny, model.nx)
+
+
+def interpret_density_as_solid(model: StructuredModel, density, threshold: float = 0.5,
+                               largest_component_only: bool = True) -> Interpretation:
+    density = np.asarray(density, dtype=float)
+    if density.shape != (model.nelem,):
+        raise ValueError(f"density vector has length {density.size} but the mesh has {model.nelem} elements")
+    if not (0.0 < threshold < 1.0):
+        raise ValueError(f"density threshold must lie strictly between 0 and 1 (got {threshold})")
+    kept = density >= threshold
+    nk = int(kept.sum())
+    if nk == 0:
+        raise InterpretationError(f"no element reaches the density threshold {threshold} (max density is "
+                                  f"{density.max()}); lower the threshold or increase the volume fraction")
+    # face neighbours only (the default cross-shaped structure); labels follow
+    # the first element of each group in index order
+    labels, ncomp = ndimage.label(kept.reshape(_grid_shape(model)))
+    labels = labels.ravel()
+    ve = element_volume(model)
+    retained = kept
+    if largest_component_only and ncomp > 1:
+        counts = np.bincount(labels, minlength=ncomp + 1)[1:]
+        best = int(np.argmax(counts)) + 1  # first of the largest (uniform element volumes)
+        retained = labels == best
+    nr = int(retained.sum())
+    return Interpretation(threshold=threshold, elements_above_threshold=nk, elements_retained=nr,
+                          components_above_threshold=int(ncomp), volume_above_threshold=nk * ve,
+                          volume_retained=nr * ve, volume_discarded_as_islands=(nk - nr) * ve, retained=retained)
+
+
+def element_nodes(model: StructuredModel) -> np.ndarray:
+    """(nelem, npe) node indices in the local order of src/mesh/StructuredMesh.hpp."""
+    nnx, nny = model.nx + 1, model.ny + 1
+    e = np.arange(model.nelem)
+    ex = e % model.nx
+    if model.dim == 2:
+        n0 = (e // model.nx) * nnx + ex
+        return np.stack([n0, n0 + 1, n0 + nnx + 1, n0 + nnx], axis=1)
+    ey = (e // model.nx) % model.ny
+    ez = e // (model.nx * model.ny)
+    lay = nnx * nny
+    n0 = (ez * nny + ey) * nnx + ex
+    return np.stack([n0, n0 + 1, n0 + nnx + 1, n0 + nnx, n0 + lay, n0 + lay + 1, n0 + lay + nnx + 1, n0 + lay + nnx],
+                    axis=1)
+
+
+def solid_submodel(model: StructuredModel, retained) -> tuple[StructuredModel, np.ndarray]:
+    """The parent grid with every DOF of the nodes outside the retained elements
+    fixed; returns (model, retained-node mask). Supports keep their parent selection."""
+    retained = np.asarray(retained, dtype=bool)
+    used = np.zeros(model.nnodes, dtype=bool)
+    used[element_nodes(model)[retained].ravel()] = True
+    fixed = np.asarray(model.fixed, dtype=np.uint8).reshape(model.nnodes, model.dim).copy()
+    fixed[~used] = 1
+    sub = dataclasses.replace(model, fixed=fixed.ravel(), loads=None, weights=None)
+    return sub, used
+
+
+def supported(model: StructuredModel, used) -> bool:
+    """Whether any support of the parent acts on a retained node."""
+    parent = np.asarray(model.fixed, dtype=bool).reshape(model.nnodes, model.dim)
+    return bool(parent[np.asarray(used, dtype=bool)].any())

@@ -18,7 +18,10 @@ Outputs in the output directory (src/io/ResultWriter.cpp):
   (ResultWriter.cpp:1474-1515);
 * ``summary.json``: provenance, mesh, material, ``optimization_setup``,
   ``optimization_result`` (ResultWriter.cpp:2560-2810) and
-  ``mass_stiffness_comparison`` (apps/sparlab_topopt.cpp:438-466,873-930).
+  ``mass_stiffness_comparison`` (apps/sparlab_topopt.cpp:438-466,873-930)
+  and, when the deck enables ``modal``, ``modal_initial_solid``,
+  ``modal_mass_matched_baseline`` (2-D) and ``modal_optimised_topology`` with
+  the interpretation of the final density field (apps/sparlab_topopt.cpp:490-640).
 """
 from __future__ import annotations
 
@@ -33,6 +36,7 @@ import time
 import numpy as np
 
 from . import deck as D
+from . import interpret as I
 
 _OVERRIDES = (("volume-fraction", float), ("penalty", float), ("filter-radius", float),
               ("filter-radius-elements", float), ("filter-type", str), ("max-iterations", int), ("nx", int),
@@ -59,6 +63,43 @@ def _num(v):
     """JSON has no infinity: the reference's writer emits null for it."""
     v = float(v)
     return v if math.isfinite(v) else None
+
+
+def optimised_topology_modes(dk: D.Deck, phys, device, log) -> dict:
+    """The final density thresholded and reduced to its largest face-connected
+    group (src/mesh/SubMesh.cpp:172-260), and that solid's lowest modes
+    (apps/sparlab_topopt.cpp:490-560,635-643). A structure the threshold cannot
+    produce, or one no support holds, is reported instead of analysed."""
+    from . import native as N
+    m = dk.model
+    out = {}
+    try:
+        interp = I.interpret_density_as_solid(m, phys, dk.interpretation_threshold)
+    except I.InterpretationError as e:
+        log(f"{dk.name}: the density field cannot be interpreted as a solid: {e}")
+        out["interpreted_solid_analysis"] = {"analysis_failed": True, "reason": f"no interpreted structure: {e}"}
+        return out
+    out["topology_interpretation"] = interp.as_dict()
+    sub, used = I.solid_submodel(m, interp.retained)
+    block = {"num_elements": interp.elements_retained, "num_nodes": int(used.sum()),
+             "mass_kg": interp.volume_retained * dk.density, "loads_applied": False}
+    out["interpreted_solid_analysis"] = block
+    if not I.supported(m, used):
+        block.update(analysis_failed=True, reason="the retained structure touches no support")
+        log(f"{dk.name}: skipping the modal analysis of the optimised topology: no support acts on it")
+        return out
+    ind = interp.retained.astype(float)
+    try:
+        gs = N.HipProblem(sub, device=device)
+        modes = gs.modal(dk.density, stiffness_scale=ind, mass_scale=ind, **dk.modal)
+    except N.SparlabError as e:
+        block.update(analysis_failed=True, reason=str(e))
+        log(f"{dk.name}: the interpreted structure could not be analysed: {e}")
+        return out
+    out["modal_optimised_topology"] = {k: v for k, v in modes.items() if k != "seconds"}
+    log(f"{dk.name}: f1 topology {modes['frequencies_hz'][0]:.10g} Hz (mass {modes['total_mass_kg']:.10g} kg, "
+        f"{interp.elements_retained} elements), {modes['seconds']:.2f} s")
+    return out
 
 
 def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str = "fp64",
@@ -183,6 +224,9 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
             thin = g.modal(dk.density, stiffness_scale=np.full(m.nelem, vf), mass_scale=np.full(m.nelem, vf),
                            **dk.modal)
             summary["modal_mass_matched_baseline"] = {k: v for k, v in thin.items() if k != "seconds"}
+        if dk.modal_flags["analyse_optimised_topology"]:
+            topo = optimised_topology_modes(dk, phys, device, log)
+            summary.update(topo)
 
     if output:
         os.makedirs(output, exist_ok=True)

@@ -89,5 +89,13 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
             thin = saved["modal_mass_matched_baseline"]
             assert thin["frequencies_hz"] == pytest.approx(modes["frequencies_hz"], rel=1e-8)
             assert thin["total_mass_kg"] == pytest.approx(modes["total_mass_kg"] * r["volume_fraction"], rel=1e-12)
+        if d.modal_flags["analyse_optimised_topology"]:
+            # the thresholded, largest face-connected structure (published f1_topology_Hz, mass_topology_kg)
+            topo = saved["modal_optimised_topology"]
+            print(f"deck {case}: f1 topology {topo['frequencies_hz'][0]!r} Hz (published {pub['f1_topology_Hz']!r}), "
+                  f"mass {topo['total_mass_kg']!r} kg (published {pub['mass_topology_kg']!r})", flush=True)
+            assert topo["total_mass_kg"] == pytest.approx(pub["mass_topology_kg"], rel=1e-11)
+            assert saved["interpreted_solid_analysis"]["mass_kg"] == pytest.approx(pub["mass_topology_kg"], rel=1e-11)
+            assert topo["frequencies_hz"][0] == pytest.approx(pub["f1_topology_Hz"], rel=1e-9)
     else:
         assert "modal_initial_solid" not in saved
