@@ -115,6 +115,12 @@ typedef struct {
   double tolerance;       /* relative residual of PCG */
   int max_iterations;     /* 0 -> 1000 */
   int warm_start;         /* 1 */
+  /* Heaviside projection of the filtered density (ComplianceObjective::set_projection,
+   * src/topopt/Sensitivity.cpp:42-56; src/topopt/Projection.cpp:58-94): physical =
+   * (tanh(beta eta) + tanh(beta (rho - eta))) / (tanh(beta eta) + tanh(beta (1 - eta))).
+   * beta = 0: off (default). */
+  double beta;
+  double eta;             /* 0.5 */
 } slx_eval_options;
 
 typedef struct {
@@ -144,6 +150,22 @@ typedef struct {
   int volume_converged;
 } slx_oc_step;
 
+/* Beta continuation of the projection (ProjectionOptions,
+ * include/sparlab/topopt/Projection.hpp:68-97, without the robust formulation;
+ * the schedule of src/topopt/TopologyOptimizer.cpp:17-46). Stage k runs at
+ * min(beta_start beta_factor^k, beta_max); a stage ends after beta_interval
+ * iterations or, with advance_on_convergence, when the design change reaches
+ * change_tolerance. Only the last stage may stop the loop. */
+typedef struct {
+  int enabled;                 /* 0 */
+  double eta;                  /* 0.5 */
+  double beta_start;           /* 1 */
+  double beta_max;             /* 32 */
+  double beta_factor;          /* 2 */
+  int beta_interval;           /* 50 */
+  int advance_on_convergence;  /* 1 */
+} slx_projection_options;
+
 /* include/sparlab/topopt/TopologyOptimizer.hpp:81-121 (OC subset) */
 typedef struct {
   int max_iterations;
@@ -154,7 +176,8 @@ typedef struct {
   double penalty_start;
   int continuation_iterations;
   slx_oc_options oc;
-  slx_eval_options eval;
+  slx_eval_options eval;       /* its beta is ignored: the projection schedule sets it */
+  slx_projection_options projection;
 } slx_run_options;
 
 /* TopologyOptimizationResult (include/sparlab/topopt/TopologyOptimizer.hpp:193-258,
@@ -180,11 +203,13 @@ typedef struct {
   long long linear_solves;       /* one per load case and evaluation, the final one included */
   long long linear_iterations;
   double total_seconds;
+  double final_beta;             /* beta of the last stage (0 without projection) */
 } slx_run_result;
 
 /* Columns of one optimisation history row, in the order of the reference's
- * history.csv (src/io/ResultWriter.cpp:1428-1470, OC without projection). */
-#define SLX_HISTORY_COLUMNS 12
+ * history.csv (src/io/ResultWriter.cpp:1428-1470, OC): the 12 columns of a run
+ * without projection, then beta (0 without projection). */
+#define SLX_HISTORY_COLUMNS 13
 
 /* Learned warm start: a 3-level 3-D convolutional network on the node grid maps
  * per-node (density, density change, previous displacement, residual of the
@@ -306,7 +331,7 @@ slx_status slx_oc_update(slx_context* ctx, const double* x, const double* dc_dx,
  * SLX_HISTORY_COLUMNS] (may be NULL), one row per performed iteration:
  * iteration, penalty, compliance [J], volume [m^3], volume fraction, max design
  * change, Lagrange multiplier, grey level, OC bisections, volume converged (0/1),
- * seconds, linear iterations. */
+ * seconds, linear iterations, beta. */
 slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history,
                       slx_run_result* result);
 /* compliance[ncases]: f_l . u_l of each load case at the last evaluation

@@ -1546,18 +1546,67 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
   for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
   return s;
 }
-std::vector<double> physical_density(const Problem& p, const std::vector<double>& x) {
+// the clamped filtered density (ComplianceObjective::evaluate, Sensitivity.cpp:120-123)
+std::vector<double> filtered_density(const Problem& p, const std::vector<double>& x) {
   std::vector<double> rho(x.size());
   if (p.filter_on) p.filter.to_physical(x.data(), rho.data()); else rho = x;
   for (auto& r : rho) r = std::min(std::max(r, 0.0), 1.0);
   return rho;
 }
+
+// ComplianceObjective::physical_density (Sensitivity.cpp:68-72)
+std::vector<double> physical_density(const Problem& p, const std::vector<double>& x) {
+  std::vector<double> rho = filtered_density(p, x);
+  if (p.beta > 0.0)
+    for (auto& r : rho) r = heaviside_project(r, p.beta, p.eta);
+  return rho;
+}
 }  // namespace
+
+double heaviside_project(double r, double beta, double eta) {
+  const double a = std::tanh(beta * eta);
+  const double denom = a + std::tanh(beta * (1.0 - eta));
+  return (a + std::tanh(beta * (r - eta))) / denom;
+}
+
+double heaviside_derivative(double r, double beta, double eta) {
+  const double denom = std::tanh(beta * eta) + std::tanh(beta * (1.0 - eta));
+  const double t = std::tanh(beta * (r - eta));
+  return beta * (1.0 - t * t) / denom;
+}
+
+void Projection::validate() const {
+  if (!enabled) return;
+  if (!(eta > 0.0 && eta < 1.0)) fail(1, "topology.projection.eta must lie in (0, 1)");
+  if (!(beta_start > 0.0) || !std::isfinite(beta_start)) fail(1, "topology.projection.beta_start must be positive");
+  if (!(beta_max >= beta_start) || !std::isfinite(beta_max))
+    fail(1, "topology.projection.beta_max must be at least beta_start");
+  if (beta_max > 1024.0) fail(1, "topology.projection.beta_max is beyond any useful sharpness");
+  if (!(beta_factor > 1.0)) fail(1, "topology.projection.beta_factor must exceed 1");
+  if (beta_interval < 1) fail(1, "topology.projection.beta_interval must be at least 1");
+}
+
+int Projection::num_stages() const {
+  int stages = 1;
+  double b = beta_start;
+  while (b < beta_max && stages < 64) {
+    b *= beta_factor;
+    ++stages;
+  }
+  return stages;
+}
+
+double Projection::beta_of_stage(int k) const {
+  double b = beta_start;
+  for (int i = 0; i < k && b < beta_max; ++i) b *= beta_factor;
+  return std::min(b, beta_max);
+}
 
 Evaluation evaluate(Problem& p, const std::vector<double>& x, bool need_gradients) {
   const Index ne = p.mesh.ne;  // Sensitivity.cpp:110-237
   if ((Index)x.size() != ne) fail(1, "design vector has the wrong length");
   Evaluation out;
+  const std::vector<double> filtered = filtered_density(p, x);
   out.physical = physical_density(p, x);
   out.factors.resize((size_t)ne);
   for (Index e = 0; e < ne; ++e) out.factors[(size_t)e] = simp_factor(out.physical[(size_t)e], p.simp);
@@ -1618,12 +1667,20 @@ Evaluation evaluate(Problem& p, const std::vector<double>& x, bool need_gradient
   if (need_gradients) {
     out.dc_dx.resize((size_t)ne);
     out.dv_dx.resize((size_t)ne);
+    // through the projection, then the (linear) density filter (Sensitivity.cpp:210-219)
+    std::vector<double> dvol = p.volumes;
+    if (p.beta > 0.0)
+      for (Index e = 0; e < ne; ++e) {
+        const double d = heaviside_derivative(filtered[(size_t)e], p.beta, p.eta);
+        dc[(size_t)e] *= d;
+        dvol[(size_t)e] *= d;
+      }
     if (p.filter_on) {
       p.filter.pull_back(dc.data(), out.dc_dx.data());
-      p.filter.pull_back(p.volumes.data(), out.dv_dx.data());
+      p.filter.pull_back(dvol.data(), out.dv_dx.data());
     } else {
       out.dc_dx = dc;
-      out.dv_dx = p.volumes;
+      out.dv_dx = dvol;
     }
   }
   if (!std::isfinite(out.compliance)) fail(2, "the compliance objective evaluated to a non-finite value");
@@ -1704,6 +1761,16 @@ RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0)
   std::vector<double> x = x0;
   for (size_t e = 0; e < x.size(); ++e) x[e] = std::clamp(x[e], p.lower[e], p.upper[e]);
   std::vector<double> history;
+  // BetaSchedule (TopologyOptimizer.cpp:17-46): stage k runs at beta_of_stage(k)
+  const Projection& pr = p.projection;
+  pr.validate();
+  const double saved_beta = p.beta, saved_eta = p.eta;
+  const int stages = pr.enabled ? pr.num_stages() : 1;
+  int stage = 0, stage_start = 1;
+  if (pr.enabled) {
+    p.beta = pr.beta_of_stage(0);
+    p.eta = pr.eta;
+  }
   int iteration;
   for (iteration = 1; iteration <= o.max_iterations; ++iteration) {
     double penalty = p.simp.penalty;
@@ -1722,7 +1789,8 @@ RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0)
     for (double r : ev.physical) gray += r * (1.0 - r);
     gray = 4.0 * gray / double(ev.physical.size());
     res.history.push_back({iteration, penalty, ev.compliance, ev.volume, ev.volume_fraction, step.max_change,
-                           step.lambda, gray, step.bisections, step.volume_converged, ev.linear_iterations});
+                           step.lambda, gray, step.bisections, step.volume_converged, ev.linear_iterations,
+                           pr.enabled ? p.beta : 0.0});
     history.push_back(ev.compliance);
     x = step.x;
     for (size_t e = 0; e < x.size(); ++e) x[e] = std::clamp(x[e], p.lower[e], p.upper[e]);
@@ -1735,10 +1803,18 @@ RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0)
       spread = (*mm.second - *mm.first) / std::max(std::abs(history.back()), 1.0e-30);
     }
     const bool objective_ok = o.objective_tolerance > 0.0 && spread <= o.objective_tolerance;
-    if (at_final && (change_ok || objective_ok)) {
+    const bool final_stage = stage + 1 >= stages;
+    if (at_final && final_stage && (change_ok || objective_ok)) {
       res.converged = true;
       res.stop_reason = change_ok ? 1 : 2;  // TopologyOptimizer.cpp:692
       break;
+    }
+    if (!final_stage && (iteration - stage_start + 1 >= pr.beta_interval ||
+                         (pr.advance_on_convergence && step.max_change <= o.change_tolerance))) {
+      ++stage;  // TopologyOptimizer.cpp:706-711: the objective window restarts
+      stage_start = iteration + 1;
+      p.beta = pr.beta_of_stage(stage);
+      history.clear();
     }
   }
   res.iterations = std::min(iteration, o.max_iterations);
@@ -1751,6 +1827,9 @@ RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0)
   for (double r : fin.physical) g += r * (1.0 - r);
   res.gray = 4.0 * g / double(std::max<size_t>(fin.physical.size(), 1));
   res.x = x;
+  res.final_beta = pr.enabled ? p.beta : 0.0;
+  p.beta = saved_beta;
+  p.eta = saved_eta;
   return res;
 }
 

@@ -232,6 +232,19 @@ struct OcStep {
 
 // A structured SIMP problem: mesh, material, supports, load cases, design
 // domain and the solver state that persists over a run (Sensitivity.cpp:110-237).
+// Heaviside projection (src/topopt/Projection.cpp:58-94, include/sparlab/topopt/Projection.hpp:68-97)
+double heaviside_project(double r, double beta, double eta);
+double heaviside_derivative(double r, double beta, double eta);
+struct Projection {  // ProjectionOptions without the robust formulation
+  bool enabled = false;
+  double eta = 0.5, beta_start = 1.0, beta_max = 32.0, beta_factor = 2.0;
+  int beta_interval = 50;
+  bool advance_on_convergence = true;
+  void validate() const;  // Projection.cpp:10-56
+  int num_stages() const;
+  double beta_of_stage(int k) const;
+};
+
 struct Problem {
   Mesh mesh;
   int ndpn = 0;
@@ -253,6 +266,8 @@ struct Problem {
   double tolerance = 1.0e-8;
   int max_iterations = 1000;
   bool warm_start = true;
+  double beta = 0.0, eta = 0.5;  // projection of the physical density (beta 0: off)
+  Projection projection;         // run_oc's beta continuation
   // state
   Amg amg;
   Csr kff;
@@ -290,6 +305,7 @@ struct IterationRecord {  // one history.csv row (ResultWriter.cpp:1428-1470)
   int bisections;
   bool volume_converged;
   int linear_iterations;
+  double beta;
 };
 struct RunResult {
   std::vector<IterationRecord> history;
@@ -299,6 +315,7 @@ struct RunResult {
   int iterations = 0;
   bool converged = false;
   int stop_reason = 0;  // 0 iteration_cap, 1 design_change, 2 objective_stall
+  double final_beta = 0.0;
 };
 RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0);  // TopologyOptimizer.cpp:556-724
 

@@ -19,6 +19,21 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
+def heaviside_project(r, beta, eta):
+    return lib().ref_heaviside_project(float(r), float(beta), float(eta))
+
+
+def heaviside_derivative(r, beta, eta):
+    return lib().ref_heaviside_derivative(float(r), float(beta), float(eta))
+
+
+def projection_stages(beta_start, beta_max, beta_factor, k):
+    """(num_stages, beta_of_stage(k)) of the beta continuation."""
+    b = C.c_double()
+    n = lib().ref_projection_stages(beta_start, beta_max, beta_factor, k, C.byref(b))
+    return n, b.value
+
+
 def build(force: bool = False) -> None:
     if force or not os.path.exists(LIB):
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "-s"], check=True)
@@ -85,7 +100,13 @@ def lib():
         "ref_oc": (C.c_int, [v, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_int)]),
         "ref_run_oc": (C.c_int, [v, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_double),
-                                 C.POINTER(C.c_int)]),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "ref_problem_set_projection": (None, [v, C.c_double, C.c_double]),
+        "ref_problem_set_projection_schedule": (None, [v, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                                       C.c_int, C.c_int]),
+        "ref_heaviside_project": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+        "ref_heaviside_derivative": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+        "ref_projection_stages": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]),
         "ref_chunked_dot": (C.c_double, [_dp, _dp, C.c_int]),
         "ref_cpqr": (C.c_int, [_dp, C.c_int, C.c_int, _dp, _dp]),
         "ref_tridiag_max": (C.c_double, [_dp, _dp, C.c_int]),
@@ -299,17 +320,32 @@ class Oracle:
                             np.ascontiguousarray(dv), move, damping, xn, sc, C.byref(nb)))
         return xn, dict(lambda_=sc[0], volume=sc[1], max_change=sc[2], converged=bool(sc[3]), bisections=nb.value)
 
+    def set_projection(self, beta=0.0, eta=0.5):
+        """Project the physical density in evaluate() (beta 0: off)."""
+        lib().ref_problem_set_projection(self.h, beta, eta)
+
     def run_oc(self, x0, max_iterations, change_tol=1e-3, objective_tol=0.0, window=5, cont_steps=1,
-               penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5):
+               penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5, projection=None):
+        """History rows have 13 columns: the 12 of history.csv and beta (0 without projection).
+        projection: dict(eta, beta_start, beta_max, beta_factor, beta_interval, advance_on_convergence)."""
+        pr = dict(eta=0.5, beta_start=1.0, beta_max=32.0, beta_factor=2.0, beta_interval=50,
+                  advance_on_convergence=True)
+        pr.update(projection or {})
+        lib().ref_problem_set_projection_schedule(self.h, 1 if projection else 0, pr["eta"], pr["beta_start"],
+                                                  pr["beta_max"], pr["beta_factor"], int(pr["beta_interval"]),
+                                                  1 if pr["advance_on_convergence"] else 0)
         x = np.array(x0, dtype=np.float64)
-        hist = np.zeros(max(max_iterations, 1) * 12)
+        hist = np.zeros(max(max_iterations, 1) * 13)
         n = C.c_int()
         fc = C.c_double()
         stop = C.c_int()
+        fb = C.c_double()
         _check(lib().ref_run_oc(self.h, max_iterations, change_tol, objective_tol, window, cont_steps, penalty_start,
-                                cont_iterations, move, damping, x, hist, C.byref(n), C.byref(fc), C.byref(stop)))
+                                cont_iterations, move, damping, x, hist, C.byref(n), C.byref(fc), C.byref(stop),
+                                C.byref(fb)))
         self.last_stop_reason = ("iteration_cap", "design_change", "objective_stall")[stop.value]
-        return x, hist.reshape(-1, 12)[: n.value], n.value, fc.value
+        self.last_final_beta = fb.value
+        return x, hist.reshape(-1, 13)[: n.value], n.value, fc.value
 
 
 def chunked_dot(a, b):

@@ -271,11 +271,41 @@ int ref_oc(void* h, const double* x, const double* dc, const double* dv, double 
   });
 }
 
-// history: max_iterations x 12, the columns of history.csv (src/io/ResultWriter.cpp:1428-1470);
+// projection of the physical density for ref_evaluate (beta 0: off)
+void ref_problem_set_projection(void* h, double beta, double eta) {
+  P(h)->beta = beta;
+  P(h)->eta = eta;
+}
+
+// run_oc's beta continuation (ProjectionOptions, non-robust)
+void ref_problem_set_projection_schedule(void* h, int enabled, double eta, double beta_start, double beta_max,
+                                         double beta_factor, int beta_interval, int advance_on_convergence) {
+  Projection& pr = P(h)->projection;
+  pr.enabled = enabled != 0;
+  pr.eta = eta;
+  pr.beta_start = beta_start;
+  pr.beta_max = beta_max;
+  pr.beta_factor = beta_factor;
+  pr.beta_interval = beta_interval;
+  pr.advance_on_convergence = advance_on_convergence != 0;
+}
+
+double ref_heaviside_project(double r, double beta, double eta) { return heaviside_project(r, beta, eta); }
+double ref_heaviside_derivative(double r, double beta, double eta) { return heaviside_derivative(r, beta, eta); }
+int ref_projection_stages(double beta_start, double beta_max, double beta_factor, int k, double* beta) {
+  Projection pr;
+  pr.beta_start = beta_start;
+  pr.beta_max = beta_max;
+  pr.beta_factor = beta_factor;
+  *beta = pr.beta_of_stage(k);
+  return pr.num_stages();
+}
+
+// history: max_iterations x 13, the columns of history.csv (src/io/ResultWriter.cpp:1428-1470) with beta last;
 // seconds are not timed here
 int ref_run_oc(void* h, int max_iterations, double change_tol, double objective_tol, int window, int cont_steps,
                double penalty_start, int cont_iterations, double move, double damping, double* x_inout,
-               double* history, int* performed, double* final_compliance, int* stop_reason) {
+               double* history, int* performed, double* final_compliance, int* stop_reason, double* final_beta) {
   return guarded([&] {
     Problem* p = P(h);
     RunOptions o;
@@ -292,7 +322,7 @@ int ref_run_oc(void* h, int max_iterations, double change_tol, double objective_
     std::memcpy(x_inout, r.x.data(), r.x.size() * sizeof(double));
     for (size_t i = 0; i < r.history.size(); ++i) {
       const IterationRecord& it = r.history[i];
-      double* row = history + i * 12;  // the columns of history.csv
+      double* row = history + i * 13;  // the columns of history.csv
       row[0] = it.iteration;
       row[1] = it.penalty;
       row[2] = it.compliance;
@@ -305,8 +335,10 @@ int ref_run_oc(void* h, int max_iterations, double change_tol, double objective_
       row[9] = it.volume_converged ? 1.0 : 0.0;
       row[10] = 0.0;
       row[11] = it.linear_iterations;
+      row[12] = it.beta;
     }
     *performed = r.iterations;
+    *final_beta = r.final_beta;
     *final_compliance = r.compliance;
     *stop_reason = r.stop_reason;
   });

@@ -90,7 +90,7 @@ const double* htvol(Context& c) {
 void ensure_topopt_buffers(Context& c) {
   const size_t ne = (size_t)c.nelem;
   for (DBuf<double>* b : {&c.d_x, &c.d_phys, &c.d_fac, &c.d_dfac, &c.d_dc, &c.d_energy, &c.d_tmp, &c.d_tmp2,
-                          &c.d_dcx, &c.d_dvx})
+                          &c.d_dcx, &c.d_dvx, &c.d_dproj})
     b->alloc(ne);
   if (c.d_vol.n != ne) c.d_vol.upload(c.volumes, c.stream);
   if (c.lower.size() != ne) {
@@ -168,6 +168,15 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   slx_evaluation out{};
   const auto t0 = Clock::now();
   k::spmv_serial(c.dH, c.d_x.p, c.d_tmp.p, s);
+  // the projection of the clamped filtered density (Sensitivity.cpp:120-129)
+  const bool proj = o.beta > 0.0;
+  if (proj) {
+    if (!(o.eta > 0.0 && o.eta < 1.0) || !std::isfinite(o.beta))
+      fail(SLX_CONFIG_ERROR, "projection needs beta > 0 and eta in (0, 1)");
+    k::project((int)ne, c.d_tmp.p, o.beta, o.eta, c.d_tmp.p, c.d_dproj.p, s);
+  }
+  c.proj_beta = proj ? o.beta : 0.0;
+  c.proj_eta = o.eta;
   k::simp((int)ne, c.d_tmp.p, o.penalty, o.emin_ratio, c.d_phys.p, c.d_fac.p, c.d_dfac.p, s);
   assemble(c, c.d_fac.p);
   amg_setup(c, true);
@@ -190,8 +199,16 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   out.volume = device_dot(c, ne, c.d_phys.p, c.d_vol.p);
   out.volume_fraction = out.volume / c.total_volume;
   if (need_gradients) {
-    k::spmv_serial(c.dHt, c.d_dc.p, c.d_dcx.p, s);
-    k::copy(ne, htvol(c), c.d_dvx.p, s);
+    if (proj) {
+      // through the projection, then the density filter (Sensitivity.cpp:210-216)
+      k::mul((int)ne, c.d_dc.p, c.d_dproj.p, c.d_tmp.p, s);
+      k::spmv_serial(c.dHt, c.d_tmp.p, c.d_dcx.p, s);
+      k::mul((int)ne, c.d_vol.p, c.d_dproj.p, c.d_tmp.p, s);
+      k::spmv_serial(c.dHt, c.d_tmp.p, c.d_dvx.p, s);
+    } else {
+      k::spmv_serial(c.dHt, c.d_dc.p, c.d_dcx.p, s);
+      k::copy(ne, htvol(c), c.d_dvx.p, s);
+    }
   }
   warm_after_evaluate(c);
   sync(c);
@@ -221,7 +238,8 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
   // designs within [0, 1] filter into [0, 1] (rows of H are convex weights), so the
   // clamp of the filtered volume is inactive and V(x) = (H^T vol).x: no filter
   // product per bisection trial
-  const bool bounded = c.unit_bounds;
+  // (the projection makes V nonlinear in x: then each trial filters and projects)
+  const bool bounded = c.unit_bounds && c.proj_beta == 0.0;
   const double* ocw = bounded ? htvol(c) : nullptr;
   if (bounded) c.oc_state.alloc(8);
   auto volume_at = [&](double lambda) {
@@ -233,7 +251,14 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
     }
     k::oc_candidate(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, lambda,
                     o.move_limit, o.damping, c.d_tmp2.p, s);
-    k::filter_volume(c.dH, c.d_tmp2.p, c.d_vol.p, c.partials.p, c.scal.p + 14, s);
+    if (c.proj_beta > 0.0) {
+      // the volume of the physical density the candidate would produce (TopologyOptimizer.cpp:573-580)
+      k::spmv_serial(c.dH, c.d_tmp2.p, c.d_tmp.p, s);
+      k::project(ne, c.d_tmp.p, c.proj_beta, c.proj_eta, c.d_tmp.p, nullptr, s);
+      k::dot(ne, c.d_tmp.p, c.d_vol.p, c.partials.p, c.scal.p + 14, s);
+    } else {
+      k::filter_volume(c.dH, c.d_tmp2.p, c.d_vol.p, c.partials.p, c.scal.p + 14, s);
+    }
     return read_scalar(c, c.scal.p + 14);
   };
   double lo = o.lambda_lower, hi = o.lambda_upper;
@@ -380,6 +405,8 @@ void slx_eval_default_options(slx_eval_options* o) {
   o->tolerance = 1.0e-8;
   o->max_iterations = 1000;
   o->warm_start = 1;
+  o->beta = 0.0;
+  o->eta = 0.5;
 }
 
 void slx_oc_default_options(slx_oc_options* o) {
@@ -870,6 +897,40 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
     c.d_x.upload(x, (size_t)ne, c.stream);
     k::clamp(ne, c.d_x.p, c.d_lower.p, c.d_upper.p, c.stream);
     std::vector<double> chist;
+    // beta continuation (TopologyOptimizer.cpp:17-46; ProjectionOptions::validate, Projection.cpp:10-56)
+    const slx_projection_options& pr = o.projection;
+    if (pr.enabled) {
+      std::ostringstream os;
+      if (!(pr.eta > 0.0 && pr.eta < 1.0))
+        os << "topology.projection.eta must lie in (0, 1), got " << pr.eta;
+      else if (!(pr.beta_start > 0.0) || !std::isfinite(pr.beta_start))
+        os << "topology.projection.beta_start must be positive, got " << pr.beta_start;
+      else if (!(pr.beta_max >= pr.beta_start) || !std::isfinite(pr.beta_max))
+        os << "topology.projection.beta_max (" << pr.beta_max << ") must be at least beta_start (" << pr.beta_start
+           << ")";
+      else if (pr.beta_max > 1024.0)
+        os << "topology.projection.beta_max = " << pr.beta_max << " is beyond any useful sharpness";
+      else if (!(pr.beta_factor > 1.0))
+        os << "topology.projection.beta_factor must exceed 1, got " << pr.beta_factor;
+      else if (pr.beta_interval < 1)
+        os << "topology.projection.beta_interval must be at least 1, got " << pr.beta_interval;
+      if (!os.str().empty()) fail(SLX_CONFIG_ERROR, os.str());
+    }
+    auto beta_of_stage = [&](int k) {
+      double b = pr.beta_start;
+      for (int i = 0; i < k && b < pr.beta_max; ++i) b *= pr.beta_factor;
+      return std::min(b, pr.beta_max);
+    };
+    int stages = 1;
+    if (pr.enabled) {
+      double b = pr.beta_start;
+      while (b < pr.beta_max && stages < 64) {
+        b *= pr.beta_factor;
+        ++stages;
+      }
+    }
+    int stage = 0, stage_start = 1;
+    double beta = pr.enabled ? beta_of_stage(0) : 0.0;
     // gray_level (src/topopt/TopologyOptimizer.cpp:106-113): 4/n sum rho (1 - rho)
     auto grey = [&]() {
       k::grey_sum(ne, c.d_phys.p, c.partials.p, c.scal.p + 10, c.stream);
@@ -888,6 +949,8 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
         const double t = double(stage) / double(o.continuation_steps - 1);
         eo.penalty = o.penalty_start + t * (o.eval.penalty - o.penalty_start);
       }
+      eo.beta = beta;
+      eo.eta = pr.eta;
       const slx_evaluation ev = evaluate_device(c, eo, true);
       res.linear_solves += c.ncases;
       res.linear_iterations += ev.linear_iterations;
@@ -907,6 +970,7 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
         row[9] = st.volume_converged ? 1.0 : 0.0;
         row[10] = std::chrono::duration<double>(Clock::now() - t_iter).count();
         row[11] = ev.linear_iterations;
+        row[12] = beta;
       }
       chist.push_back(ev.compliance);
       std::swap(c.d_x, c.d_tmp2);
@@ -924,16 +988,27 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
       const bool objective_ok = o.objective_tolerance > 0.0 && spread <= o.objective_tolerance;
       res.final_design_change = st.max_change;
       res.final_objective_change = spread;
-      if (at_final && (change_ok || objective_ok)) {
+      const bool final_stage = stage + 1 >= stages;
+      if (at_final && final_stage && (change_ok || objective_ok)) {
         res.converged = 1;
         res.stop_reason = change_ok ? SLX_STOP_DESIGN_CHANGE : SLX_STOP_OBJECTIVE_STALL;
         break;
+      }
+      if (!final_stage && (iteration - stage_start + 1 >= pr.beta_interval ||
+                           (pr.advance_on_convergence && st.max_change <= o.change_tolerance))) {
+        ++stage;  // the objective changes with beta: its window restarts
+        stage_start = iteration + 1;
+        beta = beta_of_stage(stage);
+        chist.clear();
       }
     }
     res.iterations = std::min(iteration, o.max_iterations);
     // the final evaluation at the returned design (src/topopt/TopologyOptimizer.cpp:263-283)
     slx_eval_options eo = o.eval;
+    eo.beta = beta;
+    eo.eta = pr.eta;
     const slx_evaluation fin = evaluate_device(c, eo, true);
+    res.final_beta = beta;
     res.linear_solves += c.ncases;
     res.linear_iterations += fin.linear_iterations;
     res.compliance = fin.compliance;

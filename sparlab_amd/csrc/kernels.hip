@@ -516,6 +516,24 @@ __global__ void k_simp(int n, const double* __restrict__ filtered, double penalt
   dfac[e] = penalty * (1.0 - emin) * pow(r, penalty - 1.0);
 }
 
+// Heaviside projection of the clamped filtered density (src/topopt/Projection.cpp:58-70);
+// in place allowed. dproj (may be null) = its derivative with respect to the clamped value.
+__global__ void k_project(int n, const double* in, double beta, double eta, double* out, double* dproj) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double r = fmin(fmax(in[e], 0.0), 1.0);
+  const double a = tanh(beta * eta);
+  const double denom = a + tanh(beta * (1.0 - eta));
+  const double t = tanh(beta * (r - eta));
+  out[e] = (a + t) / denom;
+  if (dproj) dproj[e] = beta * (1.0 - t * t) / denom;
+}
+
+__global__ void k_mul(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) c[e] = a[e] * b[e];
+}
+
 // OC candidate (src/topopt/OptimalityCriteria.cpp:14-36)
 __global__ void k_oc_candidate(int n, const double* __restrict__ x, const double* __restrict__ dc,
                                const double* __restrict__ dv, const double* __restrict__ lower,
@@ -851,6 +869,12 @@ void sensitivity(int dim, int nelem, int nx, int ny, const double* ke0, const in
 void simp(int n, const double* filtered, double penalty, double emin, double* phys, double* fac, double* dfac,
           hipStream_t s) {
   if (n) k_simp<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, filtered, penalty, emin, phys, fac, dfac);
+}
+void project(int n, const double* in, double beta, double eta, double* out, double* dproj, hipStream_t s) {
+  if (n) k_project<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, in, beta, eta, out, dproj);
+}
+void mul(int n, const double* a, const double* b, double* c, hipStream_t s) {
+  if (n) k_mul<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, a, b, c);
 }
 void oc_candidate(int n, const double* x, const double* dc, const double* dv, const double* lower, const double* upper,
                   const unsigned char* passive, double lambda, double move, double damping, double* xn,

@@ -252,6 +252,8 @@ struct Context {
   HCsr H, Ht;  // filter and its transpose
   DCsr dH, dHt;
   DBuf<double> d_x, d_phys, d_fac, d_dfac, d_dc, d_energy, d_vol, d_tmp, d_tmp2, d_lower, d_upper, d_dcx, d_dvx;
+  DBuf<double> d_dproj;                   // d physical / d filtered under the projection
+  double proj_beta = 0.0, proj_eta = 0.5;  // projection of the last evaluation (beta 0: off)
   DBuf<double> oc_state;  // the device bisection state
   DBuf<unsigned char> d_passive;
 

@@ -89,6 +89,9 @@ void sensitivity(int dim, int nelem, int nx, int ny, const double* ke0, const in
                  double* dc, hipStream_t s);
 void simp(int n, const double* filtered, double penalty, double emin, double* phys, double* fac, double* dfac,
           hipStream_t s);
+// out = Heaviside projection of clamp(in) (in == out allowed), dproj its derivative (may be null)
+void project(int n, const double* in, double beta, double eta, double* out, double* dproj, hipStream_t s);
+void mul(int n, const double* a, const double* b, double* c, hipStream_t s);
 void oc_candidate(int n, const double* x, const double* dc, const double* dv, const double* lower, const double* upper,
                   const unsigned char* passive, double lambda, double move, double damping, double* xn,
                   hipStream_t s);

@@ -6,7 +6,7 @@ loads, tractions and pressures on the boundary, load-case weights, the SIMP
 law, the density filter, passive regions and the optimality-criteria loop with
 its penalty continuation -- into a
 :class:`~sparlab_amd.model.StructuredModel` and the ``run_oc`` options. Settings
-outside that path (other meshes, body loads, projection, MMA,
+outside that path (other meshes, body loads, robust projection, MMA,
 stress or buckling constraints, ...) raise :class:`DeckError` naming the key,
 rather than being ignored.
 
@@ -489,7 +489,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
         raise DeckError("the load-case weights sum to zero")
 
     # the optimisation problem (Config.cpp:1542-1730)
-    for key, what in (("projection", "Heaviside projection"), ("stress", "a stress constraint"),
+    for key, what in (("stress", "a stress constraint"),
                       ("overhang", "an overhang filter"), ("buckling_constraint", "a buckling constraint"),
                       ("length_scale", "a length-scale check")):
         node = topo.get(key)
@@ -499,6 +499,32 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     method = opt.get("method", "oc")
     if method != "oc":
         _unsupported("topology.optimizer.method", f"the {method} update")
+    # Heaviside projection with beta continuation (Config.cpp:1594-1617, Projection.cpp:10-56)
+    pnode = topo.get("projection", {}) or {}
+    projection = None
+    if pnode.get("enabled", False):
+        for key, what in (("robust", "the robust (eroded / dilated) formulation"),
+                          ("erosion_check", "the erosion check")):
+            if pnode.get(key, False):
+                _unsupported(f"topology.projection.{key}", what)
+        projection = dict(eta=float(pnode.get("eta", 0.5)), beta_start=float(pnode.get("beta_start", 1.0)),
+                          beta_max=float(pnode.get("beta_max", 32.0)), beta_factor=float(pnode.get("beta_factor", 2.0)),
+                          beta_interval=int(pnode.get("beta_interval", 50)),
+                          advance_on_convergence=bool(pnode.get("advance_on_convergence", True)))
+        pj = projection
+        if not (0.0 < pj["eta"] < 1.0):
+            raise DeckError(f"topology.projection.eta must lie in (0, 1), got {pj['eta']}")
+        if not (pj["beta_start"] > 0.0 and math.isfinite(pj["beta_start"])):
+            raise DeckError(f"topology.projection.beta_start must be positive, got {pj['beta_start']}")
+        if not (pj["beta_max"] >= pj["beta_start"] and math.isfinite(pj["beta_max"])):
+            raise DeckError(f"topology.projection.beta_max ({pj['beta_max']}) must be at least beta_start "
+                            f"({pj['beta_start']})")
+        if pj["beta_max"] > 1024.0:
+            raise DeckError(f"topology.projection.beta_max = {pj['beta_max']} is beyond any useful sharpness")
+        if not pj["beta_factor"] > 1.0:
+            raise DeckError(f"topology.projection.beta_factor must exceed 1, got {pj['beta_factor']}")
+        if pj["beta_interval"] < 1:
+            raise DeckError(f"topology.projection.beta_interval must be at least 1, got {pj['beta_interval']}")
     filt = topo.get("filter", {})
     ftype = str(ov.get("filter_type") or filt.get("type", "density")).lower()
     if ftype == "off":
@@ -572,6 +598,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
         cont_iterations=int(opt.get("continuation_iterations", 25)),
         move=float(opt.get("move_limit", 0.2)),
         damping=float(opt.get("damping", 0.5)),
+        projection=projection,
     )
     lin = doc.get("solver", {}).get("linear", {})
     tol = float(lin.get("residual_tolerance", 1e-8))

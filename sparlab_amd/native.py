@@ -72,7 +72,7 @@ class AmgStats(C.Structure):
 
 class EvalOptions(C.Structure):
     _fields_ = [("penalty", C.c_double), ("emin_ratio", C.c_double), ("tolerance", C.c_double),
-                ("max_iterations", C.c_int), ("warm_start", C.c_int)]
+                ("max_iterations", C.c_int), ("warm_start", C.c_int), ("beta", C.c_double), ("eta", C.c_double)]
 
 
 class Evaluation(C.Structure):
@@ -90,10 +90,16 @@ class OcStep(C.Structure):
                 ("bisections", C.c_int), ("volume_converged", C.c_int)]
 
 
+class ProjectionOptions(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("eta", C.c_double), ("beta_start", C.c_double), ("beta_max", C.c_double),
+                ("beta_factor", C.c_double), ("beta_interval", C.c_int), ("advance_on_convergence", C.c_int)]
+
+
 class RunOptions(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("change_tolerance", C.c_double), ("objective_tolerance", C.c_double),
                 ("objective_window", C.c_int), ("continuation_steps", C.c_int), ("penalty_start", C.c_double),
-                ("continuation_iterations", C.c_int), ("oc", OcOptions), ("eval", EvalOptions)]
+                ("continuation_iterations", C.c_int), ("oc", OcOptions), ("eval", EvalOptions),
+                ("projection", ProjectionOptions)]
 
 
 class RunResult(C.Structure):
@@ -101,7 +107,7 @@ class RunResult(C.Structure):
                 ("compliance", C.c_double), ("volume", C.c_double), ("volume_fraction", C.c_double),
                 ("grey_level", C.c_double), ("final_design_change", C.c_double),
                 ("final_objective_change", C.c_double), ("linear_solves", C.c_longlong),
-                ("linear_iterations", C.c_longlong), ("total_seconds", C.c_double)]
+                ("linear_iterations", C.c_longlong), ("total_seconds", C.c_double), ("final_beta", C.c_double)]
 
 
 # slx_stop_reason -> the reference's stop_reason strings (TopologyOptimizer.hpp:206,
@@ -109,12 +115,12 @@ class RunResult(C.Structure):
 STOP_REASONS = ("iteration_cap", "design_change", "objective_stall")
 
 # columns of a run_oc history row, the order of the reference's history.csv
-# (src/io/ResultWriter.cpp:1428-1470)
+# (src/io/ResultWriter.cpp:1428-1470); the reference writes beta only for a projected run
 HISTORY_COLUMNS = ("iteration", "penalty[-]", "compliance[J]", "volume[m3]", "volume_fraction[-]",
                    "max_design_change[-]", "lagrange_multiplier[-]", "grey_level[-]", "oc_bisections[-]",
-                   "volume_converged[-]", "seconds[s]", "linear_iterations[-]")
-H_ITER, H_PENALTY, H_COMPLIANCE, H_VOLUME, H_VF, H_CHANGE, H_LAMBDA, H_GREY, H_BISECT, H_VCONV, H_SECONDS, H_LINEAR = \
-    range(len(HISTORY_COLUMNS))
+                   "volume_converged[-]", "seconds[s]", "linear_iterations[-]", "beta[-]")
+H_ITER, H_PENALTY, H_COMPLIANCE, H_VOLUME, H_VF, H_CHANGE, H_LAMBDA, H_GREY, H_BISECT, H_VCONV, H_SECONDS, H_LINEAR, \
+    H_BETA = range(len(HISTORY_COLUMNS))
 
 
 class ModalOptions(C.Structure):
@@ -590,11 +596,13 @@ class HipProblem:
         self.ctx.check(load().slx_warmstart_stats_get(self.ctx.h, C.byref(st)))
         return st.as_dict()
 
-    def evaluate(self, x, tol=1e-8, need_gradients=True, warm_start=True, penalty=None, max_iterations=1000):
+    def evaluate(self, x, tol=1e-8, need_gradients=True, warm_start=True, penalty=None, max_iterations=1000,
+                 beta=0.0, eta=0.5):
+        """ComplianceObjective::evaluate; beta > 0 projects the filtered density at eta."""
         self._own()
         o = eval_options(tolerance=tol, warm_start=1 if warm_start else 0, max_iterations=max_iterations,
                          penalty=self.model.penalty if penalty is None else penalty,
-                         emin_ratio=self.model.emin_ratio)
+                         emin_ratio=self.model.emin_ratio, beta=beta, eta=eta)
         ev = Evaluation()
         dc = np.empty(self.ne) if need_gradients else None
         dv = np.empty(self.ne) if need_gradients else None
@@ -616,9 +624,11 @@ class HipProblem:
                         converged=bool(st.volume_converged), bisections=st.bisections)
 
     def run_oc(self, x0, max_iterations, change_tol=1e-3, objective_tol=0.0, window=5, cont_steps=1,
-               penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5, tol=1e-8):
+               penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5, tol=1e-8, projection=None):
         """TopologyOptimizer::run_oc: returns (x, history, iterations, final compliance);
-        the full result of the last call is in ``last_run``."""
+        the full result of the last call is in ``last_run``. projection: None, or a dict of
+        ProjectionOptions (eta, beta_start, beta_max, beta_factor, beta_interval,
+        advance_on_convergence) for the Heaviside projection with beta continuation."""
         self._own()
         o = RunOptions()
         o.max_iterations = max_iterations
@@ -630,6 +640,12 @@ class HipProblem:
         o.continuation_iterations = cont_iterations
         o.oc = oc_options(move_limit=move, damping=damping)
         o.eval = eval_options(tolerance=tol, penalty=self.model.penalty, emin_ratio=self.model.emin_ratio)
+        pr = dict(eta=0.5, beta_start=1.0, beta_max=32.0, beta_factor=2.0, beta_interval=50,
+                  advance_on_convergence=True)
+        pr.update(projection or {})
+        o.projection = ProjectionOptions(1 if projection else 0, pr["eta"], pr["beta_start"], pr["beta_max"],
+                                         pr["beta_factor"], int(pr["beta_interval"]),
+                                         1 if pr["advance_on_convergence"] else 0)
         x = _f64(x0).copy()
         hist = np.zeros(max(max_iterations, 1) * len(HISTORY_COLUMNS))
         r = RunResult()

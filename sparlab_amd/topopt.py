@@ -124,7 +124,7 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
                                     objective_tol=dk.run["objective_tol"], window=dk.run["window"],
                                     cont_steps=dk.run["cont_steps"], penalty_start=dk.run["penalty_start"],
                                     cont_iterations=dk.run["cont_iterations"], move=dk.run["move"],
-                                    damping=dk.run["damping"], tol=tol)
+                                    damping=dk.run["damping"], tol=tol, projection=dk.run["projection"])
     res = dict(g.last_run)
     case_c = g.load_case_compliance()
     st = g.amg_stats()
@@ -132,8 +132,16 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
         f"volume fraction {res['volume_fraction']:.10g}, {res['total_seconds']:.2f} s")
 
     # the final design's fields for density_final.csv (the run's last evaluation, repeated)
-    fin = g.evaluate(x, tol=tol, need_gradients=False)
-    phys = g.filter(x)
+    pj = dk.run["projection"]
+    beta = res["final_beta"] if pj else 0.0
+    eta = pj["eta"] if pj else 0.5
+    fin = g.evaluate(x, tol=tol, need_gradients=False, beta=beta, eta=eta)
+    filtered = np.clip(g.filter(x), 0.0, 1.0)
+    phys = filtered
+    if pj:
+        # the projected physical density (src/topopt/Projection.cpp:58-63)
+        a = math.tanh(beta * eta)
+        phys = (a + np.tanh(beta * (filtered - eta))) / (a + math.tanh(beta * (1.0 - eta)))
     factor = m.emin_ratio + phys ** m.penalty * (1.0 - m.emin_ratio)
     # the full solid domain (apps/sparlab_topopt.cpp:438-466)
     solid = g.evaluate(np.ones(m.nelem), tol=tol, need_gradients=False, warm_start=False)["compliance"]
@@ -203,6 +211,14 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
             "compliance_penalty_vs_full_solid": comp / solid if solid > 0 else 0.0,
         },
     }
+    if pj:
+        # ResultWriter.cpp:2640-2673 (setup) and 2753-2764 (result)
+        summary["optimization_setup"]["projection"] = dict(pj, enabled=True)
+        summary["optimization_result"]["projection"] = {
+            "final_beta": beta, "eta": eta,
+            "filtered_grey_level": float(4.0 * np.sum(filtered * (1.0 - filtered)) / m.nelem),
+            "note": "grey_level is that of the projected (physical) density; filtered_grey_level that of the "
+                    "density before projection"}
     if len(hist):
         summary["optimization_result"]["initial_compliance_J"] = float(hist[0, N.H_COMPLIANCE])
         summary["optimization_result"]["compliance_reduction_factor"] = (
@@ -231,9 +247,10 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
     if output:
         os.makedirs(output, exist_ok=True)
         if write_csv_files:
-            write_csv(os.path.join(output, "history.csv"), list(N.HISTORY_COLUMNS),
-                      [[int(r[0])] + list(r[1:N.H_BISECT]) + [int(r[N.H_BISECT]), int(r[N.H_VCONV])]
-                       + [r[N.H_SECONDS], int(r[N.H_LINEAR])] for r in hist])
+            ncol = N.H_BETA + 1 if pj else N.H_BETA  # beta only for a projected run
+            write_csv(os.path.join(output, "history.csv"), list(N.HISTORY_COLUMNS[:ncol]),
+                      [([int(r[0])] + list(r[1:N.H_BISECT]) + [int(r[N.H_BISECT]), int(r[N.H_VCONV])]
+                        + [r[N.H_SECONDS], int(r[N.H_LINEAR])] + ([r[N.H_BETA]] if pj else [])) for r in hist])
             cents = D.Geometry(m, dk.origin).cents[:, : m.dim]
             ev = vol / m.nelem
             tags = np.zeros(m.nelem, dtype=int)

@@ -19,12 +19,13 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from sparlab_amd.deck import read_deck  # noqa: E402
 
 REF = "/root/reference"
-CASES = ("cantilever_beam", "mbb_beam", "aerospace_bracket", "bracket_3d", "wing_rib")
+CASES = ("cantilever_beam", "mbb_beam", "aerospace_bracket", "bracket_3d", "wing_rib", "mbb_beam_projected",
+         "bracket_3d_projected")
 FIELDS = {"dim": int, "elements": int, "dofs": int, "volume_fraction_target": float,
           "volume_fraction_achieved": float, "iterations": int, "stop_reason": str, "compliance_J": float,
           "initial_compliance_J": float, "full_solid_compliance_J": float,
           "equal_mass_plate_compliance_J": float, "grey_level": float, "linear_solver": str,
-          "f1_solid_Hz": float, "mass_solid_kg": float, "f1_topology_Hz": float, "mass_topology_kg": float}
+          "f1_solid_Hz": float, "mass_solid_kg": float, "f1_topology_Hz": float, "mass_topology_kg": float, "final_beta": float}
 
 
 def main():

@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
-CASES = ("mbb_beam", "cantilever_beam", "bracket_3d", "aerospace_bracket", "wing_rib")
+CASES = ("mbb_beam", "cantilever_beam", "bracket_3d", "aerospace_bracket", "wing_rib", "mbb_beam_projected",
+         "bracket_3d_projected")
 PUBLISHED = json.load(open(os.path.join(GOLD, "benchmarks_published.json")))["cases"]
 
 
@@ -39,7 +40,9 @@ def test_initial_and_solid_compliance_on_gpu(case):
     solid = g.evaluate(np.ones(m.nelem), tol=1e-11, need_gradients=False, warm_start=False)["compliance"]
     assert solid == pytest.approx(pub["full_solid_compliance_J"], rel=1e-8)
     p0 = d.run["penalty_start"] if d.run["cont_steps"] > 1 else m.penalty
-    c0 = g.evaluate(d.initial_design(), tol=1e-11, need_gradients=False, warm_start=False, penalty=p0)["compliance"]
+    pj = d.run["projection"]
+    c0 = g.evaluate(d.initial_design(), tol=1e-11, need_gradients=False, warm_start=False, penalty=p0,
+                    beta=pj["beta_start"] if pj else 0.0, eta=pj["eta"] if pj else 0.5)["compliance"]
     assert c0 == pytest.approx(pub["initial_compliance_J"], rel=1e-8)
 
 
@@ -68,7 +71,11 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
 
     # outputs
     rows = list(csv.reader(open(tmp_path / "history.csv")))
-    assert rows[0] == list(N.HISTORY_COLUMNS)
+    pj = d.run["projection"]
+    assert rows[0] == list(N.HISTORY_COLUMNS[: N.H_BETA + (1 if pj else 0)])
+    if pj:
+        assert r["projection"]["final_beta"] == pub["final_beta"]
+        assert float(rows[-1][-1]) == pub["final_beta"]
     assert len(rows) == 1 + r["iterations"]
     assert [int(x[0]) for x in rows[1:]] == list(range(1, r["iterations"] + 1))
     assert float(rows[1][2]) == r["initial_compliance_J"]

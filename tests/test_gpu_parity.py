@@ -158,6 +158,33 @@ def test_short_oc_run_against_oracle(ctx):
     assert fg == pytest.approx(fo, rel=1e-7)
 
 
+def test_projected_evaluate_and_oc_run_against_oracle(ctx):
+    """Heaviside projection: the projected objective and its gradients, then a
+    short run through three beta stages, against the oracle."""
+    print("test_projected_evaluate_and_oc_run_against_oracle", flush=True)
+    m = M.mbb_beam(30, 15)
+    g, o = _pair(m, ctx, tol=1e-10)
+    x = np.random.default_rng(3).uniform(0.2, 0.8, m.nelem)
+    o.set_projection(4.0, 0.5)
+    eg = g.evaluate(x, tol=1e-10, beta=4.0, eta=0.5)
+    eo = o.evaluate(x)
+    o.set_projection(0.0)
+    assert eg["compliance"] == pytest.approx(eo["compliance"], rel=1e-9)
+    assert eg["volume"] == pytest.approx(eo["volume"], rel=1e-13)
+    np.testing.assert_allclose(eg["dc_dx"], eo["dc_dx"], rtol=1e-7, atol=1e-9 * np.abs(eo["dc_dx"]).max())
+    np.testing.assert_allclose(eg["dv_dx"], eo["dv_dx"], rtol=1e-12)
+    x0 = m.initial_design()
+    pr = dict(beta_start=1.0, beta_max=8.0, beta_interval=5)
+    xg, hg, ng, fg = g.run_oc(x0, 20, change_tol=0.0, tol=1e-10, projection=pr)
+    xo, ho, no, fo = o.run_oc(x0, 20, change_tol=0.0, projection=pr)
+    assert ng == no == 20
+    np.testing.assert_array_equal(hg[:, 12], ho[:, 12])
+    assert hg[-1, 12] == 8.0 and g.last_run["final_beta"] == 8.0
+    np.testing.assert_allclose(hg[:, 2], ho[:, 2], rtol=1e-7)
+    np.testing.assert_allclose(hg[:, 4], ho[:, 4], rtol=1e-8)
+    assert fg == pytest.approx(fo, rel=1e-7)
+
+
 def test_c1_mbb_120x40_hundred_oc_iterations(ctx):
     """The MBB half-beam at 120 x 40 (r 1.5, vf 0.5, p 3, rtol 1e-8), exactly 100
     OC iterations with no stopping rule: GPU loop against the oracle's loop."""
