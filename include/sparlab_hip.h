@@ -151,8 +151,8 @@ typedef struct {
 } slx_oc_step;
 
 /* Beta continuation of the projection (ProjectionOptions,
- * include/sparlab/topopt/Projection.hpp:68-97, without the robust formulation;
- * the schedule of src/topopt/TopologyOptimizer.cpp:17-46). Stage k runs at
+ * include/sparlab/topopt/Projection.hpp:68-97; the schedule of
+ * src/topopt/TopologyOptimizer.cpp:17-46). Stage k runs at
  * min(beta_start beta_factor^k, beta_max); a stage ends after beta_interval
  * iterations or, with advance_on_convergence, when the design change reaches
  * change_tolerance. Only the last stage may stop the loop. */
@@ -164,6 +164,13 @@ typedef struct {
   double beta_factor;          /* 2 */
   int beta_interval;           /* 50 */
   int advance_on_convergence;  /* 1 */
+  /* robust formulation (TopologyOptimizer.cpp:60-80,566-640): the objective on the
+   * eroded design (eta + delta), the volume constraint on the dilated design
+   * (eta - delta) against V* V_dilated / V_blueprint, rescaled every
+   * robust_volume_interval iterations; the blueprint (eta) is the result */
+  int robust;                  /* 0 */
+  double robust_delta;         /* 0.1 */
+  int robust_volume_interval;  /* 1 */
 } slx_projection_options;
 
 /* include/sparlab/topopt/TopologyOptimizer.hpp:81-121 (OC subset) */
@@ -204,12 +211,21 @@ typedef struct {
   long long linear_iterations;
   double total_seconds;
   double final_beta;             /* beta of the last stage (0 without projection) */
+  /* robust runs (RobustRecord, TopologyOptimizer.hpp): the designs projected at
+   * eta +- delta next to the blueprint the fields above describe */
+  double compliance_eroded;
+  double compliance_dilated;
+  double volume_fraction_eroded;
+  double volume_fraction_dilated;
+  double dilated_target_fraction; /* of the last rescaling */
 } slx_run_result;
 
 /* Columns of one optimisation history row, in the order of the reference's
  * history.csv (src/io/ResultWriter.cpp:1428-1470, OC): the 12 columns of a run
- * without projection, then beta (0 without projection). */
-#define SLX_HISTORY_COLUMNS 13
+ * without projection, then beta (0 without projection), then the eroded and
+ * dilated volume fractions and the dilated target fraction of a robust run (0
+ * otherwise; the volume columns then hold the blueprint's). */
+#define SLX_HISTORY_COLUMNS 16
 
 /* Learned warm start: a 3-level 3-D convolutional network on the node grid maps
  * per-node (density, density change, previous displacement, residual of the
@@ -331,7 +347,8 @@ slx_status slx_oc_update(slx_context* ctx, const double* x, const double* dc_dx,
  * SLX_HISTORY_COLUMNS] (may be NULL), one row per performed iteration:
  * iteration, penalty, compliance [J], volume [m^3], volume fraction, max design
  * change, Lagrange multiplier, grey level, OC bisections, volume converged (0/1),
- * seconds, linear iterations, beta. */
+ * seconds, linear iterations, beta, eroded / dilated volume fraction, dilated
+ * target fraction. */
 slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history,
                       slx_run_result* result);
 /* compliance[ncases]: f_l . u_l of each load case at the last evaluation

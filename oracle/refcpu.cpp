@@ -1584,6 +1584,9 @@ void Projection::validate() const {
   if (beta_max > 1024.0) fail(1, "topology.projection.beta_max is beyond any useful sharpness");
   if (!(beta_factor > 1.0)) fail(1, "topology.projection.beta_factor must exceed 1");
   if (beta_interval < 1) fail(1, "topology.projection.beta_interval must be at least 1");
+  if (robust && !(robust_delta > 0.0 && eta - robust_delta > 0.0 && eta + robust_delta < 1.0))
+    fail(1, "topology.projection.robust_delta must be positive and keep eta +- delta inside (0, 1)");
+  if (robust && robust_volume_interval < 1) fail(1, "topology.projection.robust_volume_interval must be at least 1");
 }
 
 int Projection::num_stages() const {
@@ -1710,10 +1713,18 @@ std::vector<double> oc_candidate(const Problem& p, const std::vector<double>& x,
 }  // namespace
 
 OcStep oc_update(const Problem& p, const std::vector<double>& x, const std::vector<double>& dc,
-                 const std::vector<double>& dv, const OcOptions& o) {
+                 const std::vector<double>& dv, const OcOptions& o, double volume_target, double vol_eta,
+                 bool step_toward) {
   // OptimalityCriteria.cpp:40-149
-  const double target = p.volume_fraction * p.domain_volume();
-  auto vol = [&](const std::vector<double>& xi) { return dot(physical_density(p, xi), p.volumes); };
+  const double target = volume_target > 0.0 ? volume_target : p.volume_fraction * p.domain_volume();
+  Problem& q = const_cast<Problem&>(p);
+  auto vol = [&](const std::vector<double>& xi) {
+    const double saved = q.eta;
+    if (vol_eta >= 0.0) q.eta = vol_eta;
+    const double v = dot(physical_density(q, xi), q.volumes);
+    q.eta = saved;
+    return v;
+  };
   double lo = o.lambda_lower, hi = o.lambda_upper;
   double vol_lo = vol(oc_candidate(p, x, dc, dv, lo, o));
   double vol_hi = vol(oc_candidate(p, x, dc, dv, hi, o));
@@ -1727,6 +1738,18 @@ OcStep oc_update(const Problem& p, const std::vector<double>& x, const std::vect
     hi *= 1.0e3;
     vol_hi = vol(oc_candidate(p, x, dc, dv, hi, o));
     ++widen;
+  }
+  if ((vol_lo < target || vol_hi > target) && step_toward) {
+    const bool too_light = vol_lo < target;  // OptimalityCriteria.cpp:82-97
+    OcStep step;
+    step.lambda = too_light ? lo : hi;
+    step.x = oc_candidate(p, x, dc, dv, step.lambda, o);
+    step.achieved_volume = too_light ? vol_lo : vol_hi;
+    step.volume_converged = false;
+    double mc = 0.0;
+    for (size_t e = 0; e < x.size(); ++e) mc = std::max(mc, std::abs(step.x[e] - x[e]));
+    step.max_change = mc;
+    return step;
   }
   if (vol_lo < target || vol_hi > target) fail(3, "the optimality-criteria bisection cannot bracket the volume target");
   OcStep step;
@@ -1767,10 +1790,23 @@ RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0)
   const double saved_beta = p.beta, saved_eta = p.eta;
   const int stages = pr.enabled ? pr.num_stages() : 1;
   int stage = 0, stage_start = 1;
+  const bool robust = pr.enabled && pr.robust;
+  // the objective acts on the eroded design in a robust run (TopologyOptimizer.cpp:566-571)
+  const double objective_eta = robust ? pr.eta + pr.robust_delta : pr.eta;
+  const double dilated_eta = pr.eta - pr.robust_delta;
+  double dilated_target = p.volume_fraction * p.domain_volume();
   if (pr.enabled) {
     p.beta = pr.beta_of_stage(0);
-    p.eta = pr.eta;
+    p.eta = objective_eta;
   }
+  // the volume of x projected at eta (its filtered density at the current beta)
+  auto volume_at_eta = [&](const std::vector<double>& xi, double eta) {
+    const double saved = p.eta;
+    p.eta = eta;
+    const double v = dot(physical_density(p, xi), p.volumes);
+    p.eta = saved;
+    return v;
+  };
   int iteration;
   for (iteration = 1; iteration <= o.max_iterations; ++iteration) {
     double penalty = p.simp.penalty;
@@ -1783,14 +1819,38 @@ RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0)
     const double saved = q.simp.penalty;
     q.simp.penalty = penalty;
     Evaluation ev = evaluate(q, x, true);
-    OcStep step = oc_update(q, x, ev.dc_dx, ev.dv_dx, o.oc);
+    std::vector<double> dv = ev.dv_dx;
+    double v_dil = 0.0, v_int = 0.0;
+    if (robust) {
+      // robust_volumes (TopologyOptimizer.cpp:70-80): the dilated design's volume and
+      // gradient, the blueprint's volume; the dilated target follows their ratio
+      v_dil = volume_at_eta(x, dilated_eta);
+      v_int = volume_at_eta(x, pr.eta);
+      const std::vector<double> filt = filtered_density(p, x);
+      std::vector<double> w(filt.size());
+      for (size_t e = 0; e < filt.size(); ++e) w[e] = p.volumes[e] * heaviside_derivative(filt[e], p.beta, dilated_eta);
+      if (p.filter_on) p.filter.pull_back(w.data(), dv.data()); else dv = w;
+      if ((iteration - 1) % pr.robust_volume_interval == 0 && v_int > 0.0)
+        dilated_target = p.volume_fraction * p.domain_volume() * v_dil / v_int;
+    }
+    OcStep step = robust ? oc_update(q, x, ev.dc_dx, dv, o.oc, dilated_target, dilated_eta, true)
+                         : oc_update(q, x, ev.dc_dx, dv, o.oc);
     q.simp.penalty = saved;
     double gray = 0.0;
     for (double r : ev.physical) gray += r * (1.0 - r);
     gray = 4.0 * gray / double(ev.physical.size());
-    res.history.push_back({iteration, penalty, ev.compliance, ev.volume, ev.volume_fraction, step.max_change,
-                           step.lambda, gray, step.bisections, step.volume_converged, ev.linear_iterations,
-                           pr.enabled ? p.beta : 0.0});
+    const double dom = p.domain_volume();
+    IterationRecord rec{iteration, penalty, ev.compliance, ev.volume, ev.volume_fraction, step.max_change,
+                        step.lambda, gray, step.bisections, step.volume_converged, ev.linear_iterations,
+                        pr.enabled ? p.beta : 0.0};
+    if (robust) {
+      rec.volume = v_int;
+      rec.volume_fraction = v_int / dom;
+      rec.eroded_vf = ev.volume_fraction;
+      rec.dilated_vf = v_dil / dom;
+      rec.dilated_target_fraction = dilated_target / dom;
+    }
+    res.history.push_back(rec);
     history.push_back(ev.compliance);
     x = step.x;
     for (size_t e = 0; e < x.size(); ++e) x[e] = std::clamp(x[e], p.lower[e], p.upper[e]);
@@ -1819,6 +1879,18 @@ RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0)
   }
   res.iterations = std::min(iteration, o.max_iterations);
   Evaluation fin = evaluate(p, x, true);  // TopologyOptimizer.cpp:263-283
+  if (robust) {
+    // the blueprint is the result; the eroded and dilated designs are recorded (:347-390)
+    res.compliance_eroded = fin.compliance;
+    res.vf_eroded = fin.volume_fraction;
+    p.eta = dilated_eta;
+    const Evaluation dil = evaluate(p, x, false);
+    res.compliance_dilated = dil.compliance;
+    res.vf_dilated = dil.volume_fraction;
+    p.eta = pr.eta;
+    fin = evaluate(p, x, false);
+    res.dilated_target_fraction = res.history.empty() ? 0.0 : res.history.back().dilated_target_fraction;
+  }
   res.compliance = fin.compliance;
   res.volume = fin.volume;
   res.volume_fraction = fin.volume_fraction;

@@ -240,6 +240,9 @@ struct Projection {  // ProjectionOptions without the robust formulation
   double eta = 0.5, beta_start = 1.0, beta_max = 32.0, beta_factor = 2.0;
   int beta_interval = 50;
   bool advance_on_convergence = true;
+  bool robust = false;             // objective on eta + delta, volume on eta - delta
+  double robust_delta = 0.1;
+  int robust_volume_interval = 1;  // iterations between dilated-target updates
   void validate() const;  // Projection.cpp:10-56
   int num_stages() const;
   double beta_of_stage(int k) const;
@@ -287,8 +290,12 @@ struct Evaluation {
   std::vector<std::vector<double>> u;  // full displacement vectors
 };
 Evaluation evaluate(Problem& p, const std::vector<double>& x, bool need_gradients);
+// volume_target > 0 replaces the domain's target; vol_eta >= 0 measures the volume
+// projected at that threshold; step_toward: step to the nearest reachable volume
+// when the move-limited box cannot reach the target (OptimalityCriteria.cpp:40-110)
 OcStep oc_update(const Problem& p, const std::vector<double>& x, const std::vector<double>& dc,
-                 const std::vector<double>& dv, const OcOptions& o);
+                 const std::vector<double>& dv, const OcOptions& o, double volume_target = 0.0,
+                 double vol_eta = -1.0, bool step_toward = false);
 struct RunOptions {
   int max_iterations = 200;
   double change_tolerance = 1.0e-3;
@@ -306,6 +313,7 @@ struct IterationRecord {  // one history.csv row (ResultWriter.cpp:1428-1470)
   bool volume_converged;
   int linear_iterations;
   double beta;
+  double eroded_vf = 0, dilated_vf = 0, dilated_target_fraction = 0;  // robust runs
 };
 struct RunResult {
   std::vector<IterationRecord> history;
@@ -316,6 +324,8 @@ struct RunResult {
   bool converged = false;
   int stop_reason = 0;  // 0 iteration_cap, 1 design_change, 2 objective_stall
   double final_beta = 0.0;
+  // robust runs (RobustRecord): the blueprint (eta) is the result above
+  double compliance_eroded = 0, compliance_dilated = 0, vf_eroded = 0, vf_dilated = 0, dilated_target_fraction = 0;
 };
 RunResult run_oc(Problem& p, const RunOptions& o, const std::vector<double>& x0);  // TopologyOptimizer.cpp:556-724
 

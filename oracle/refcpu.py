@@ -100,7 +100,8 @@ def lib():
         "ref_oc": (C.c_int, [v, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_int)]),
         "ref_run_oc": (C.c_int, [v, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_double),
-                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double), _dp]),
+        "ref_problem_set_robust": (None, [v, C.c_int, C.c_double, C.c_int]),
         "ref_problem_set_projection": (None, [v, C.c_double, C.c_double]),
         "ref_problem_set_projection_schedule": (None, [v, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                                        C.c_int, C.c_int]),
@@ -326,26 +327,33 @@ class Oracle:
 
     def run_oc(self, x0, max_iterations, change_tol=1e-3, objective_tol=0.0, window=5, cont_steps=1,
                penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5, projection=None):
-        """History rows have 13 columns: the 12 of history.csv and beta (0 without projection).
-        projection: dict(eta, beta_start, beta_max, beta_factor, beta_interval, advance_on_convergence)."""
+        """History rows have 16 columns: the 12 of history.csv, beta (0 without projection), and the
+        eroded / dilated volume fractions and dilated target fraction of a robust run.
+        projection: dict(eta, beta_start, beta_max, beta_factor, beta_interval, advance_on_convergence,
+        robust, robust_delta, robust_volume_interval)."""
         pr = dict(eta=0.5, beta_start=1.0, beta_max=32.0, beta_factor=2.0, beta_interval=50,
-                  advance_on_convergence=True)
+                  advance_on_convergence=True, robust=False, robust_delta=0.1, robust_volume_interval=1)
         pr.update(projection or {})
         lib().ref_problem_set_projection_schedule(self.h, 1 if projection else 0, pr["eta"], pr["beta_start"],
                                                   pr["beta_max"], pr["beta_factor"], int(pr["beta_interval"]),
                                                   1 if pr["advance_on_convergence"] else 0)
+        lib().ref_problem_set_robust(self.h, 1 if pr["robust"] else 0, pr["robust_delta"],
+                                     int(pr["robust_volume_interval"]))
         x = np.array(x0, dtype=np.float64)
-        hist = np.zeros(max(max_iterations, 1) * 13)
+        hist = np.zeros(max(max_iterations, 1) * 16)
+        rob = np.zeros(5)
         n = C.c_int()
         fc = C.c_double()
         stop = C.c_int()
         fb = C.c_double()
         _check(lib().ref_run_oc(self.h, max_iterations, change_tol, objective_tol, window, cont_steps, penalty_start,
                                 cont_iterations, move, damping, x, hist, C.byref(n), C.byref(fc), C.byref(stop),
-                                C.byref(fb)))
+                                C.byref(fb), rob))
         self.last_stop_reason = ("iteration_cap", "design_change", "objective_stall")[stop.value]
         self.last_final_beta = fb.value
-        return x, hist.reshape(-1, 13)[: n.value], n.value, fc.value
+        self.last_robust = dict(compliance_eroded=rob[0], compliance_dilated=rob[1], volume_fraction_eroded=rob[2],
+                                volume_fraction_dilated=rob[3], dilated_target_fraction=rob[4])
+        return x, hist.reshape(-1, 16)[: n.value], n.value, fc.value
 
 
 def chunked_dot(a, b):

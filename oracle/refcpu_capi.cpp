@@ -290,6 +290,14 @@ void ref_problem_set_projection_schedule(void* h, int enabled, double eta, doubl
   pr.advance_on_convergence = advance_on_convergence != 0;
 }
 
+// the robust formulation of the projection schedule
+void ref_problem_set_robust(void* h, int robust, double delta, int volume_interval) {
+  Projection& pr = P(h)->projection;
+  pr.robust = robust != 0;
+  pr.robust_delta = delta;
+  pr.robust_volume_interval = volume_interval;
+}
+
 double ref_heaviside_project(double r, double beta, double eta) { return heaviside_project(r, beta, eta); }
 double ref_heaviside_derivative(double r, double beta, double eta) { return heaviside_derivative(r, beta, eta); }
 int ref_projection_stages(double beta_start, double beta_max, double beta_factor, int k, double* beta) {
@@ -301,11 +309,14 @@ int ref_projection_stages(double beta_start, double beta_max, double beta_factor
   return pr.num_stages();
 }
 
-// history: max_iterations x 13, the columns of history.csv (src/io/ResultWriter.cpp:1428-1470) with beta last;
+// history: max_iterations x 16, the columns of history.csv (src/io/ResultWriter.cpp:1428-1470), then beta,
+// then the robust columns (eroded and dilated volume fractions, dilated target fraction);
+// robust[5]: eroded and dilated compliance and volume fraction, the last dilated target fraction;
 // seconds are not timed here
 int ref_run_oc(void* h, int max_iterations, double change_tol, double objective_tol, int window, int cont_steps,
                double penalty_start, int cont_iterations, double move, double damping, double* x_inout,
-               double* history, int* performed, double* final_compliance, int* stop_reason, double* final_beta) {
+               double* history, int* performed, double* final_compliance, int* stop_reason, double* final_beta,
+               double* robust) {
   return guarded([&] {
     Problem* p = P(h);
     RunOptions o;
@@ -322,7 +333,7 @@ int ref_run_oc(void* h, int max_iterations, double change_tol, double objective_
     std::memcpy(x_inout, r.x.data(), r.x.size() * sizeof(double));
     for (size_t i = 0; i < r.history.size(); ++i) {
       const IterationRecord& it = r.history[i];
-      double* row = history + i * 13;  // the columns of history.csv
+      double* row = history + i * 16;  // the columns of history.csv
       row[0] = it.iteration;
       row[1] = it.penalty;
       row[2] = it.compliance;
@@ -336,7 +347,15 @@ int ref_run_oc(void* h, int max_iterations, double change_tol, double objective_
       row[10] = 0.0;
       row[11] = it.linear_iterations;
       row[12] = it.beta;
+      row[13] = it.eroded_vf;
+      row[14] = it.dilated_vf;
+      row[15] = it.dilated_target_fraction;
     }
+    robust[0] = r.compliance_eroded;
+    robust[1] = r.compliance_dilated;
+    robust[2] = r.vf_eroded;
+    robust[3] = r.vf_dilated;
+    robust[4] = r.dilated_target_fraction;
     *performed = r.iterations;
     *final_beta = r.final_beta;
     *final_compliance = r.compliance;

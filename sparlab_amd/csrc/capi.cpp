@@ -219,7 +219,11 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
 
 // optimality_criteria_update on device vectors d_x, d_dcx, d_dvx; the new
 // design lands in d_tmp2 (src/topopt/OptimalityCriteria.cpp:40-149).
-slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
+// target_override > 0 replaces the volume target; vol_eta >= 0 projects the
+// trial volumes at that threshold; step_toward: when the move-limited box cannot
+// reach the target, step to its nearest corner (OptimalityCriteria.cpp:82-97)
+slx_oc_step oc_device(Context& c, const slx_oc_options& o, double target_override = 0.0, double vol_eta = -1.0,
+                      bool step_toward = false) {
   const int ne = (int)c.nelem;
   hipStream_t s = c.stream;
   if (!(o.move_limit > 0.0 && o.move_limit <= 1.0)) fail(SLX_CONFIG_ERROR, "optimizer.move_limit must lie in (0, 1]");
@@ -234,7 +238,8 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
     if (!std::isfinite(hb[11]) || !std::isfinite(hb[12]))
       fail(SLX_SOLVER_ERROR, "optimality criteria received a non-finite gradient");
   }
-  const double target = c.volume_fraction * c.total_volume;
+  const double target = target_override > 0.0 ? target_override : c.volume_fraction * c.total_volume;
+  const double trial_eta = vol_eta >= 0.0 ? vol_eta : c.proj_eta;
   // designs within [0, 1] filter into [0, 1] (rows of H are convex weights), so the
   // clamp of the filtered volume is inactive and V(x) = (H^T vol).x: no filter
   // product per bisection trial
@@ -254,7 +259,7 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
     if (c.proj_beta > 0.0) {
       // the volume of the physical density the candidate would produce (TopologyOptimizer.cpp:573-580)
       k::spmv_serial(c.dH, c.d_tmp2.p, c.d_tmp.p, s);
-      k::project(ne, c.d_tmp.p, c.proj_beta, c.proj_eta, c.d_tmp.p, nullptr, s);
+      k::project(ne, c.d_tmp.p, c.proj_beta, trial_eta, c.d_tmp.p, nullptr, s);
       k::dot(ne, c.d_tmp.p, c.d_vol.p, c.partials.p, c.scal.p + 14, s);
     } else {
       k::filter_volume(c.dH, c.d_tmp2.p, c.d_vol.p, c.partials.p, c.scal.p + 14, s);
@@ -273,6 +278,16 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o) {
     hi *= 1.0e3;
     vhi = volume_at(hi);
     ++widen;
+  }
+  if ((vlo < target || vhi > target) && step_toward) {
+    const bool too_light = vlo < target;
+    slx_oc_step step{};
+    step.lambda = too_light ? lo : hi;
+    step.achieved_volume = volume_at(step.lambda);  // leaves the candidate in d_tmp2
+    step.volume_converged = 0;
+    k::max_abs_diff(ne, c.d_tmp2.p, c.d_x.p, c.partials.p, c.scal.p + 13, s);
+    step.max_change = read_scalar(c, c.scal.p + 13);
+    return step;
   }
   if (vlo < target || vhi > target) {
     std::ostringstream os;
@@ -914,6 +929,11 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
         os << "topology.projection.beta_factor must exceed 1, got " << pr.beta_factor;
       else if (pr.beta_interval < 1)
         os << "topology.projection.beta_interval must be at least 1, got " << pr.beta_interval;
+      else if (pr.robust && !(pr.robust_delta > 0.0 && pr.eta - pr.robust_delta > 0.0 && pr.eta + pr.robust_delta < 1.0))
+        os << "topology.projection.robust_delta = " << pr.robust_delta
+           << " must be positive and keep eta +- delta inside (0, 1) (eta = " << pr.eta << ")";
+      else if (pr.robust && pr.robust_volume_interval < 1)
+        os << "topology.projection.robust_volume_interval must be at least 1, got " << pr.robust_volume_interval;
       if (!os.str().empty()) fail(SLX_CONFIG_ERROR, os.str());
     }
     auto beta_of_stage = [&](int k) {
@@ -931,6 +951,20 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
     }
     int stage = 0, stage_start = 1;
     double beta = pr.enabled ? beta_of_stage(0) : 0.0;
+    // robust formulation (TopologyOptimizer.cpp:60-80,566-640): objective on the
+    // eroded design, volume on the dilated one against a rescaled target
+    const bool robust = pr.enabled && pr.robust;
+    const double objective_eta = robust ? pr.eta + pr.robust_delta : pr.eta;
+    const double dilated_eta = pr.eta - pr.robust_delta;
+    const double domain = c.total_volume;
+    double dilated_target = c.volume_fraction * domain;
+    DBuf<double> rfilt, rtmp, rdp, rs;
+    if (robust) {
+      rfilt.alloc((size_t)ne);
+      rtmp.alloc((size_t)ne);
+      rdp.alloc((size_t)ne);
+      rs.alloc(2);
+    }
     // gray_level (src/topopt/TopologyOptimizer.cpp:106-113): 4/n sum rho (1 - rho)
     auto grey = [&]() {
       k::grey_sum(ne, c.d_phys.p, c.partials.p, c.scal.p + 10, c.stream);
@@ -950,11 +984,30 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
         eo.penalty = o.penalty_start + t * (o.eval.penalty - o.penalty_start);
       }
       eo.beta = beta;
-      eo.eta = pr.eta;
+      eo.eta = objective_eta;
       const slx_evaluation ev = evaluate_device(c, eo, true);
       res.linear_solves += c.ncases;
       res.linear_iterations += ev.linear_iterations;
-      const slx_oc_step st = oc_device(c, o.oc);
+      double v_dil = 0.0, v_int = 0.0;
+      if (robust) {
+        // robust_volumes: the dilated volume and its gradient (replacing dv/dx), the blueprint volume
+        hipStream_t s = c.stream;
+        k::spmv_serial(c.dH, c.d_x.p, rfilt.p, s);
+        k::project(ne, rfilt.p, beta, dilated_eta, rtmp.p, rdp.p, s);
+        k::dot(ne, rtmp.p, c.d_vol.p, c.partials.p, rs.p, s);
+        k::mul(ne, c.d_vol.p, rdp.p, rtmp.p, s);
+        k::spmv_serial(c.dHt, rtmp.p, c.d_dvx.p, s);
+        k::project(ne, rfilt.p, beta, pr.eta, rtmp.p, nullptr, s);
+        k::dot(ne, rtmp.p, c.d_vol.p, c.partials.p, rs.p + 1, s);
+        double* hb = c.host_buffer(16);
+        SLX_HIP(hipMemcpyAsync(hb, rs.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        sync(c);
+        v_dil = hb[0];
+        v_int = hb[1];
+        if ((iteration - 1) % pr.robust_volume_interval == 0 && v_int > 0.0)
+          dilated_target = c.volume_fraction * domain * v_dil / v_int;
+      }
+      const slx_oc_step st = robust ? oc_device(c, o.oc, dilated_target, dilated_eta, true) : oc_device(c, o.oc);
       const double gray = grey();
       if (history) {
         double* row = history + (size_t)(iteration - 1) * SLX_HISTORY_COLUMNS;
@@ -971,6 +1024,14 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
         row[10] = std::chrono::duration<double>(Clock::now() - t_iter).count();
         row[11] = ev.linear_iterations;
         row[12] = beta;
+        row[13] = row[14] = row[15] = 0.0;
+        if (robust) {
+          row[3] = v_int;
+          row[4] = v_int / domain;
+          row[13] = ev.volume_fraction;
+          row[14] = v_dil / domain;
+          row[15] = dilated_target / domain;
+        }
       }
       chist.push_back(ev.compliance);
       std::swap(c.d_x, c.d_tmp2);
@@ -1006,9 +1067,26 @@ slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, d
     // the final evaluation at the returned design (src/topopt/TopologyOptimizer.cpp:263-283)
     slx_eval_options eo = o.eval;
     eo.beta = beta;
-    eo.eta = pr.eta;
-    const slx_evaluation fin = evaluate_device(c, eo, true);
+    eo.eta = objective_eta;
+    slx_evaluation fin = evaluate_device(c, eo, true);
     res.final_beta = beta;
+    if (robust) {
+      // the blueprint is the result, the eroded and dilated designs are recorded
+      // (TopologyOptimizer.cpp:347-390)
+      res.compliance_eroded = fin.compliance;
+      res.volume_fraction_eroded = fin.volume_fraction;
+      res.linear_solves += c.ncases;
+      res.linear_iterations += fin.linear_iterations;
+      eo.eta = dilated_eta;
+      const slx_evaluation dil = evaluate_device(c, eo, false);
+      res.compliance_dilated = dil.compliance;
+      res.volume_fraction_dilated = dil.volume_fraction;
+      res.linear_solves += c.ncases;
+      res.linear_iterations += dil.linear_iterations;
+      eo.eta = pr.eta;
+      fin = evaluate_device(c, eo, false);
+      res.dilated_target_fraction = dilated_target / domain;
+    }
     res.linear_solves += c.ncases;
     res.linear_iterations += fin.linear_iterations;
     res.compliance = fin.compliance;

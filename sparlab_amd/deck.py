@@ -312,6 +312,7 @@ class Deck:
     modal: dict | None = None   # keyword arguments of HipProblem.modal when modal.enabled, else None
     modal_flags: dict | None = None  # analyse_optimised_topology, compare_mass_matched_baseline
     interpretation_threshold: float = 0.5  # optimizer.interpretation_threshold (Config.cpp:1571)
+    skipped_reports: list = dataclasses.field(default_factory=list)  # report-only settings not computed
 
     @property
     def domain_volume(self) -> float:
@@ -491,7 +492,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     # the optimisation problem (Config.cpp:1542-1730)
     for key, what in (("stress", "a stress constraint"),
                       ("overhang", "an overhang filter"), ("buckling_constraint", "a buckling constraint"),
-                      ("length_scale", "a length-scale check")):
+                      ("length_scale", "a length-scale filter")):
         node = topo.get(key)
         if isinstance(node, dict) and node.get("enabled", False):
             _unsupported(f"topology.{key}", what)
@@ -503,14 +504,16 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     pnode = topo.get("projection", {}) or {}
     projection = None
     if pnode.get("enabled", False):
-        for key, what in (("robust", "the robust (eroded / dilated) formulation"),
-                          ("erosion_check", "the erosion check")):
-            if pnode.get(key, False):
-                _unsupported(f"topology.projection.{key}", what)
+        robust = bool(pnode.get("robust", False))
+        if pnode.get("erosion_check", False) and not robust:
+            # a report on a non-robust design (TopologyOptimizer.cpp:396-420); a robust run ignores the flag
+            _unsupported("topology.projection.erosion_check", "the erosion check")
         projection = dict(eta=float(pnode.get("eta", 0.5)), beta_start=float(pnode.get("beta_start", 1.0)),
                           beta_max=float(pnode.get("beta_max", 32.0)), beta_factor=float(pnode.get("beta_factor", 2.0)),
                           beta_interval=int(pnode.get("beta_interval", 50)),
-                          advance_on_convergence=bool(pnode.get("advance_on_convergence", True)))
+                          advance_on_convergence=bool(pnode.get("advance_on_convergence", True)), robust=robust,
+                          robust_delta=float(pnode.get("robust_delta", 0.1)),
+                          robust_volume_interval=int(pnode.get("robust_volume_interval", 1)))
         pj = projection
         if not (0.0 < pj["eta"] < 1.0):
             raise DeckError(f"topology.projection.eta must lie in (0, 1), got {pj['eta']}")
@@ -525,6 +528,12 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
             raise DeckError(f"topology.projection.beta_factor must exceed 1, got {pj['beta_factor']}")
         if pj["beta_interval"] < 1:
             raise DeckError(f"topology.projection.beta_interval must be at least 1, got {pj['beta_interval']}")
+        d = pj["robust_delta"]
+        if robust and not (d > 0.0 and pj["eta"] - d > 0.0 and pj["eta"] + d < 1.0):
+            raise DeckError(f"topology.projection.robust_delta = {d} must be positive and keep eta +- delta inside "
+                            f"(0, 1) (eta = {pj['eta']})")
+        if robust and pj["robust_volume_interval"] < 1:
+            raise DeckError("topology.projection.robust_volume_interval must be at least 1")
     filt = topo.get("filter", {})
     ftype = str(ov.get("filter_type") or filt.get("type", "density")).lower()
     if ftype == "off":
@@ -604,6 +613,9 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     tol = float(lin.get("residual_tolerance", 1e-8))
     density = float(doc.get("material", {}).get("density", 0.0))
     thr = float(opt.get("interpretation_threshold", 0.5))
+    # checks that only report on the final design and do not change the optimisation
+    skipped = [f"topology.{k}" for k in ("length_scale_check", "overhang_check")
+               if isinstance(topo.get(k), dict) and topo[k].get("enabled", False)]
     if not (0.0 < thr < 1.0):
         raise DeckError("optimizer.interpretation_threshold must lie in (0, 1)")
     # modal block (Config.cpp:1118-1143)
@@ -631,7 +643,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
                 case_names=names, case_weights=weights, tolerance=tol, filter_type=ftype, filter_radius=radius,
                 initial_density=start, passive_solid=int((tag == 1).sum()), passive_void=int((tag == 2).sum()),
                 run=run, history_stride=int(opt.get("history_stride", 1)), density=density, modal=modal,
-                modal_flags=flags, interpretation_threshold=thr)
+                modal_flags=flags, interpretation_threshold=thr, skipped_reports=skipped)
 
 
 def load_deck(path: str, overrides: dict | None = None) -> Deck:

@@ -92,7 +92,8 @@ class OcStep(C.Structure):
 
 class ProjectionOptions(C.Structure):
     _fields_ = [("enabled", C.c_int), ("eta", C.c_double), ("beta_start", C.c_double), ("beta_max", C.c_double),
-                ("beta_factor", C.c_double), ("beta_interval", C.c_int), ("advance_on_convergence", C.c_int)]
+                ("beta_factor", C.c_double), ("beta_interval", C.c_int), ("advance_on_convergence", C.c_int),
+                ("robust", C.c_int), ("robust_delta", C.c_double), ("robust_volume_interval", C.c_int)]
 
 
 class RunOptions(C.Structure):
@@ -107,7 +108,10 @@ class RunResult(C.Structure):
                 ("compliance", C.c_double), ("volume", C.c_double), ("volume_fraction", C.c_double),
                 ("grey_level", C.c_double), ("final_design_change", C.c_double),
                 ("final_objective_change", C.c_double), ("linear_solves", C.c_longlong),
-                ("linear_iterations", C.c_longlong), ("total_seconds", C.c_double), ("final_beta", C.c_double)]
+                ("linear_iterations", C.c_longlong), ("total_seconds", C.c_double), ("final_beta", C.c_double),
+                ("compliance_eroded", C.c_double), ("compliance_dilated", C.c_double),
+                ("volume_fraction_eroded", C.c_double), ("volume_fraction_dilated", C.c_double),
+                ("dilated_target_fraction", C.c_double)]
 
 
 # slx_stop_reason -> the reference's stop_reason strings (TopologyOptimizer.hpp:206,
@@ -118,9 +122,10 @@ STOP_REASONS = ("iteration_cap", "design_change", "objective_stall")
 # (src/io/ResultWriter.cpp:1428-1470); the reference writes beta only for a projected run
 HISTORY_COLUMNS = ("iteration", "penalty[-]", "compliance[J]", "volume[m3]", "volume_fraction[-]",
                    "max_design_change[-]", "lagrange_multiplier[-]", "grey_level[-]", "oc_bisections[-]",
-                   "volume_converged[-]", "seconds[s]", "linear_iterations[-]", "beta[-]")
+                   "volume_converged[-]", "seconds[s]", "linear_iterations[-]", "beta[-]",
+                   "eroded_volume_fraction[-]", "dilated_volume_fraction[-]", "dilated_target_fraction[-]")
 H_ITER, H_PENALTY, H_COMPLIANCE, H_VOLUME, H_VF, H_CHANGE, H_LAMBDA, H_GREY, H_BISECT, H_VCONV, H_SECONDS, H_LINEAR, \
-    H_BETA = range(len(HISTORY_COLUMNS))
+    H_BETA, H_ERODED_VF, H_DILATED_VF, H_DILATED_TARGET = range(len(HISTORY_COLUMNS))
 
 
 class ModalOptions(C.Structure):
@@ -628,7 +633,8 @@ class HipProblem:
         """TopologyOptimizer::run_oc: returns (x, history, iterations, final compliance);
         the full result of the last call is in ``last_run``. projection: None, or a dict of
         ProjectionOptions (eta, beta_start, beta_max, beta_factor, beta_interval,
-        advance_on_convergence) for the Heaviside projection with beta continuation."""
+        advance_on_convergence, robust, robust_delta, robust_volume_interval) for the Heaviside
+        projection with beta continuation."""
         self._own()
         o = RunOptions()
         o.max_iterations = max_iterations
@@ -641,11 +647,12 @@ class HipProblem:
         o.oc = oc_options(move_limit=move, damping=damping)
         o.eval = eval_options(tolerance=tol, penalty=self.model.penalty, emin_ratio=self.model.emin_ratio)
         pr = dict(eta=0.5, beta_start=1.0, beta_max=32.0, beta_factor=2.0, beta_interval=50,
-                  advance_on_convergence=True)
+                  advance_on_convergence=True, robust=False, robust_delta=0.1, robust_volume_interval=1)
         pr.update(projection or {})
         o.projection = ProjectionOptions(1 if projection else 0, pr["eta"], pr["beta_start"], pr["beta_max"],
                                          pr["beta_factor"], int(pr["beta_interval"]),
-                                         1 if pr["advance_on_convergence"] else 0)
+                                         1 if pr["advance_on_convergence"] else 0, 1 if pr["robust"] else 0,
+                                         pr["robust_delta"], int(pr["robust_volume_interval"]))
         x = _f64(x0).copy()
         hist = np.zeros(max(max_iterations, 1) * len(HISTORY_COLUMNS))
         r = RunResult()

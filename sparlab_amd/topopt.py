@@ -133,6 +133,7 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
 
     # the final design's fields for density_final.csv (the run's last evaluation, repeated)
     pj = dk.run["projection"]
+    robust = bool(pj and pj["robust"])
     beta = res["final_beta"] if pj else 0.0
     eta = pj["eta"] if pj else 0.5
     fin = g.evaluate(x, tol=tol, need_gradients=False, beta=beta, eta=eta)
@@ -219,6 +220,18 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
             "filtered_grey_level": float(4.0 * np.sum(filtered * (1.0 - filtered)) / m.nelem),
             "note": "grey_level is that of the projected (physical) density; filtered_grey_level that of the "
                     "density before projection"}
+        if robust:
+            # RobustRecord (ResultWriter.cpp:2766-2790): the blueprint is the reported design
+            summary["optimization_result"]["robust"] = {
+                "eta_eroded": eta + pj["robust_delta"], "eta_intermediate": eta, "eta_dilated": eta - pj["robust_delta"],
+                "compliance_eroded_J": res["compliance_eroded"], "compliance_intermediate_J": comp,
+                "compliance_dilated_J": res["compliance_dilated"],
+                "volume_fraction_eroded": res["volume_fraction_eroded"], "volume_fraction_intermediate": vf,
+                "volume_fraction_dilated": res["volume_fraction_dilated"],
+                "dilated_target_fraction": res["dilated_target_fraction"]}
+    for key in dk.skipped_reports:
+        summary["optimization_result"]["warnings"].append(f"{key} is a report on the final design that this driver "
+                                                          "does not compute")
     if len(hist):
         summary["optimization_result"]["initial_compliance_J"] = float(hist[0, N.H_COMPLIANCE])
         summary["optimization_result"]["compliance_reduction_factor"] = (
@@ -247,10 +260,12 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
     if output:
         os.makedirs(output, exist_ok=True)
         if write_csv_files:
-            ncol = N.H_BETA + 1 if pj else N.H_BETA  # beta only for a projected run
-            write_csv(os.path.join(output, "history.csv"), list(N.HISTORY_COLUMNS[:ncol]),
+            # beta only for a projected run, the robust columns only for a robust one
+            extra = ([N.H_BETA] if pj else []) + ([N.H_ERODED_VF, N.H_DILATED_VF, N.H_DILATED_TARGET] if robust else [])
+            write_csv(os.path.join(output, "history.csv"), list(N.HISTORY_COLUMNS[:N.H_BETA]) +
+                      [N.HISTORY_COLUMNS[i] for i in extra],
                       [([int(r[0])] + list(r[1:N.H_BISECT]) + [int(r[N.H_BISECT]), int(r[N.H_VCONV])]
-                        + [r[N.H_SECONDS], int(r[N.H_LINEAR])] + ([r[N.H_BETA]] if pj else [])) for r in hist])
+                        + [r[N.H_SECONDS], int(r[N.H_LINEAR])] + [r[i] for i in extra]) for r in hist])
             cents = D.Geometry(m, dk.origin).cents[:, : m.dim]
             ev = vol / m.nelem
             tags = np.zeros(m.nelem, dtype=int)

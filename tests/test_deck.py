@@ -154,7 +154,9 @@ def test_overrides():
                                                                         "ymax": 0.15}}, "traction": [1, 0]}]),
      "matched no boundary edge"),
     (lambda d: d["load_cases"][0].update(gravity=[0, -9.81]), "self-weight"),
-    (lambda d: d["topology"].update(projection={"enabled": True, "robust": True}), "robust"),
+    (lambda d: d["topology"].update(projection={"enabled": True, "erosion_check": True}), "erosion_check"),
+    (lambda d: d["topology"].update(projection={"enabled": True, "robust": True, "robust_delta": 0.6}),
+     "robust_delta"),
     (lambda d: d["topology"].update(projection={"enabled": True, "beta_factor": 1.0}), "beta_factor"),
     (lambda d: d["topology"]["optimizer"].update(method="mma"), "mma"),
     (lambda d: d["topology"]["filter"].update(type="sensitivity"), "sensitivity"),

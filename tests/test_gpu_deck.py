@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
 CASES = ("mbb_beam", "cantilever_beam", "bracket_3d", "aerospace_bracket", "wing_rib", "mbb_beam_projected",
-         "bracket_3d_projected")
+         "bracket_3d_projected", "mbb_beam_robust")
 PUBLISHED = json.load(open(os.path.join(GOLD, "benchmarks_published.json")))["cases"]
 
 
@@ -41,8 +41,10 @@ def test_initial_and_solid_compliance_on_gpu(case):
     assert solid == pytest.approx(pub["full_solid_compliance_J"], rel=1e-8)
     p0 = d.run["penalty_start"] if d.run["cont_steps"] > 1 else m.penalty
     pj = d.run["projection"]
+    # the first history row: the eroded design (eta + delta) in a robust run
+    eta0 = (pj["eta"] + (pj["robust_delta"] if pj["robust"] else 0.0)) if pj else 0.5
     c0 = g.evaluate(d.initial_design(), tol=1e-11, need_gradients=False, warm_start=False, penalty=p0,
-                    beta=pj["beta_start"] if pj else 0.0, eta=pj["eta"] if pj else 0.5)["compliance"]
+                    beta=pj["beta_start"] if pj else 0.0, eta=eta0)["compliance"]
     assert c0 == pytest.approx(pub["initial_compliance_J"], rel=1e-8)
 
 
@@ -72,10 +74,16 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
     # outputs
     rows = list(csv.reader(open(tmp_path / "history.csv")))
     pj = d.run["projection"]
-    assert rows[0] == list(N.HISTORY_COLUMNS[: N.H_BETA + (1 if pj else 0)])
+    robust = bool(pj and pj["robust"])
+    assert rows[0] == list(N.HISTORY_COLUMNS[: N.H_BETA + (1 if pj else 0)]) + (
+        list(N.HISTORY_COLUMNS[N.H_ERODED_VF:]) if robust else [])
+    if robust:
+        rb = r["robust"]
+        assert rb["compliance_eroded_J"] > rb["compliance_intermediate_J"] > rb["compliance_dilated_J"]
+        assert rb["volume_fraction_eroded"] < rb["volume_fraction_intermediate"] < rb["volume_fraction_dilated"]
     if pj:
         assert r["projection"]["final_beta"] == pub["final_beta"]
-        assert float(rows[-1][-1]) == pub["final_beta"]
+        assert float(rows[-1][rows[0].index("beta[-]")]) == pub["final_beta"]
     assert len(rows) == 1 + r["iterations"]
     assert [int(x[0]) for x in rows[1:]] == list(range(1, r["iterations"] + 1))
     assert float(rows[1][2]) == r["initial_compliance_J"]

@@ -185,6 +185,28 @@ def test_projected_evaluate_and_oc_run_against_oracle(ctx):
     assert fg == pytest.approx(fo, rel=1e-7)
 
 
+def test_robust_projected_run_against_oracle(ctx):
+    """The robust formulation: objective on the eroded design, volume on the dilated
+    design against its rescaled target, blueprint reported; GPU loop vs oracle loop."""
+    print("test_robust_projected_run_against_oracle", flush=True)
+    m = M.mbb_beam(30, 15)
+    g, o = _pair(m, ctx, tol=1e-10)
+    x0 = m.initial_design()
+    pr = dict(beta_start=1.0, beta_max=8.0, beta_interval=6, robust=True, robust_delta=0.1, robust_volume_interval=4)
+    xg, hg, ng, fg = g.run_oc(x0, 20, change_tol=0.0, tol=1e-10, projection=pr, move=0.1)
+    xo, ho, no, fo = o.run_oc(x0, 20, change_tol=0.0, projection=pr, move=0.1)
+    assert ng == no == 20
+    np.testing.assert_array_equal(hg[:, 12], ho[:, 12])
+    for col in (2, 3, 4, 13, 14, 15):
+        np.testing.assert_allclose(hg[:, col], ho[:, col], rtol=1e-7, err_msg=f"history column {col}")
+    assert fg == pytest.approx(fo, rel=1e-7)
+    lr, orr = g.last_run, o.last_robust
+    for k in ("compliance_eroded", "compliance_dilated", "volume_fraction_eroded", "volume_fraction_dilated",
+              "dilated_target_fraction"):
+        assert lr[k] == pytest.approx(orr[k], rel=1e-7), k
+    assert lr["compliance_eroded"] > fg > lr["compliance_dilated"]
+
+
 def test_c1_mbb_120x40_hundred_oc_iterations(ctx):
     """The MBB half-beam at 120 x 40 (r 1.5, vf 0.5, p 3, rtol 1e-8), exactly 100
     OC iterations with no stopping rule: GPU loop against the oracle's loop."""

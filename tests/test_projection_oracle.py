@@ -83,3 +83,23 @@ def test_projected_run_ends_crisper_at_beta_max():
     # a stage lasts at most beta_interval iterations
     changes = np.flatnonzero(np.diff(hs[:, 12])) + 1
     assert np.all(np.diff(np.concatenate([[0], changes])) <= 25)
+
+
+def test_robust_run_orders_the_three_designs():
+    """The robust formulation (TopologyOptimizer.cpp:60-80,566-640,347-390): the eroded
+    design is the least stiff and lightest, the dilated the stiffest and heaviest; the
+    dilated target is rescaled only every robust_volume_interval iterations."""
+    m = M.mbb_beam(60, 20)
+    o = R.Oracle(m)
+    x, h, n, c = o.run_oc(m.initial_design(), 120, change_tol=0.01,
+                          projection=dict(beta_start=1.0, beta_max=8.0, beta_interval=25, robust=True,
+                                          robust_delta=0.1, robust_volume_interval=5))
+    rb = o.last_robust
+    assert rb["compliance_eroded"] > c > rb["compliance_dilated"]
+    assert rb["volume_fraction_eroded"] < h[-1, 4] < rb["volume_fraction_dilated"]
+    assert np.all(h[:, 13] < h[:, 4]) and np.all(h[:, 4] < h[:, 14])
+    changes = np.flatnonzero(np.diff(h[:, 15]) != 0) + 1  # 0-based rows where the target moved
+    assert np.all(changes % 5 == 0)
+    assert rb["dilated_target_fraction"] == h[-1, 15]
+    # the dilated design meets its rescaled target
+    assert h[-1, 14] == pytest.approx(h[-1, 15], rel=2e-2)
