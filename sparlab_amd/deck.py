@@ -610,7 +610,12 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
         projection=projection,
     )
     lin = doc.get("solver", {}).get("linear", {})
-    tol = float(lin.get("residual_tolerance", 1e-8))
+    # an iterative deck's CG stops at iterative_tolerance (LinearSolver.cpp:265, Config.cpp:1080-1084); a direct
+    # one's residual_tolerance is the acceptance bound of its factorisation, used here as the CG tolerance
+    if str(lin.get("type", "simplicial_ldlt")).lower() in ("amg_cg", "conjugate_gradient", "cg", "auto"):
+        tol = float(lin.get("iterative_tolerance", 1e-12))
+    else:
+        tol = float(lin.get("residual_tolerance", 1e-8))
     density = float(doc.get("material", {}).get("density", 0.0))
     thr = float(opt.get("interpretation_threshold", 0.5))
     # checks that only report on the final design and do not change the optimisation

@@ -20,7 +20,7 @@ from sparlab_amd.deck import read_deck  # noqa: E402
 
 REF = "/root/reference"
 CASES = ("cantilever_beam", "mbb_beam", "aerospace_bracket", "bracket_3d", "wing_rib", "mbb_beam_projected",
-         "bracket_3d_projected", "mbb_beam_robust")
+         "bracket_3d_projected", "mbb_beam_robust", "bracket_3d_large")
 FIELDS = {"dim": int, "elements": int, "dofs": int, "volume_fraction_target": float,
           "volume_fraction_achieved": float, "iterations": int, "stop_reason": str, "compliance_J": float,
           "initial_compliance_J": float, "full_solid_compliance_J": float,

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
 CASES = ("mbb_beam", "cantilever_beam", "bracket_3d", "aerospace_bracket", "wing_rib", "mbb_beam_projected",
-         "bracket_3d_projected", "mbb_beam_robust")
+         "bracket_3d_projected", "mbb_beam_robust", "bracket_3d_large")
 PUBLISHED = json.load(open(os.path.join(GOLD, "benchmarks_published.json")))["cases"]
 
 
@@ -98,8 +98,11 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
     # the full solid domain's modes (published f1_solid_Hz, mass_solid_kg)
     if d.modal is not None:
         modes = saved["modal_initial_solid"]
-        assert modes["frequencies_hz"][0] == pytest.approx(pub["f1_solid_Hz"], rel=1e-9)
-        assert modes["total_mass_kg"] == pytest.approx(pub["mass_solid_kg"], rel=1e-11)
+        # the eigenvalue-change tolerance of the deck bounds the agreement (1e-10 by default, 1e-8 in some decks);
+        # the published masses carry the reference's summation rounding (1e-11 over 110 k elements)
+        ftol = max(1e-9, d.modal["tolerance"])
+        assert modes["frequencies_hz"][0] == pytest.approx(pub["f1_solid_Hz"], rel=ftol)
+        assert modes["total_mass_kg"] == pytest.approx(pub["mass_solid_kg"], rel=1e-10)
         if d.model.dim == 2 and d.modal_flags["compare_mass_matched_baseline"]:
             thin = saved["modal_mass_matched_baseline"]
             assert thin["frequencies_hz"] == pytest.approx(modes["frequencies_hz"], rel=1e-8)
@@ -109,8 +112,8 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
             topo = saved["modal_optimised_topology"]
             print(f"deck {case}: f1 topology {topo['frequencies_hz'][0]!r} Hz (published {pub['f1_topology_Hz']!r}), "
                   f"mass {topo['total_mass_kg']!r} kg (published {pub['mass_topology_kg']!r})", flush=True)
-            assert topo["total_mass_kg"] == pytest.approx(pub["mass_topology_kg"], rel=1e-11)
-            assert saved["interpreted_solid_analysis"]["mass_kg"] == pytest.approx(pub["mass_topology_kg"], rel=1e-11)
-            assert topo["frequencies_hz"][0] == pytest.approx(pub["f1_topology_Hz"], rel=1e-9)
+            assert topo["total_mass_kg"] == pytest.approx(pub["mass_topology_kg"], rel=1e-10)
+            assert saved["interpreted_solid_analysis"]["mass_kg"] == pytest.approx(pub["mass_topology_kg"], rel=1e-10)
+            assert topo["frequencies_hz"][0] == pytest.approx(pub["f1_topology_Hz"], rel=ftol)
     else:
         assert "modal_initial_solid" not in saved
