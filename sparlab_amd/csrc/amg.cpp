@@ -1093,8 +1093,10 @@ void numeric_level(Context& c, size_t l) {
   }
 }
 
-// The CSR kernels' lanes per row, timed on each operator of the hierarchy (the
-// precision the V-cycle reads) and kept at the fastest; once per structure.
+// The CSR kernels' lanes per row, once per structure. A row's sum order depends
+// on the lane count, so the FP64 path keeps the kernels' fixed rule from the mean
+// row length (lanes 0), and its results are the same on every run; the FP32
+// V-cycle of the mixed path times each operator and keeps the fastest.
 void autotune(Context& c) {
   const char* e = std::getenv("SPARLAB_NO_AUTOTUNE");
   if (e && e[0] == '1') return;
@@ -1110,6 +1112,10 @@ void autotune(Context& c) {
       mats.push_back(&c.levels[l].Ptent);
       mats.push_back(&c.levels[l].Rt);
     }
+  }
+  if (!f32) {
+    for (DCsr* m : mats) m->lanes = 0;
+    return;
   }
   size_t rows = 1, cols = 1;
   for (DCsr* m : mats) {

@@ -207,6 +207,23 @@ def test_robust_projected_run_against_oracle(ctx):
     assert lr["compliance_eroded"] > fg > lr["compliance_dilated"]
 
 
+def test_fp64_runs_are_bitwise_reproducible():
+    """Two FP64 runs in separate contexts give the same history bit for bit (the
+    CSR kernels' lane counts follow a fixed rule on the FP64 path)."""
+    print("test_fp64_runs_are_bitwise_reproducible", flush=True)
+    m = M.mbb_beam(60, 20)
+    runs = []
+    for _ in range(2):
+        c = N.Context(0)
+        g = N.HipProblem(m, ctx=c)
+        x, h, n, f = g.run_oc(m.initial_design(), 25, change_tol=0.0, tol=1e-8)
+        runs.append((x, h[:, [2, 4, 5, 6, 7, 11]], f))
+        c.close()
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2]
+
+
 def test_c1_mbb_120x40_hundred_oc_iterations(ctx):
     """The MBB half-beam at 120 x 40 (r 1.5, vf 0.5, p 3, rtol 1e-8), exactly 100
     OC iterations with no stopping rule: GPU loop against the oracle's loop."""
