@@ -32,6 +32,8 @@
  *   slx_evaluate         ComplianceObjective::evaluate (src/topopt/Sensitivity.cpp:110-237)
  *   slx_oc_update        optimality_criteria_update (src/topopt/OptimalityCriteria.cpp:40-149)
  *   slx_run_oc           TopologyOptimizer::run_oc (src/topopt/TopologyOptimizer.cpp:556-724)
+ *   slx_solve_adjoint    ComplianceObjective::solve_adjoint with warm slots
+ *                        (src/topopt/Sensitivity.cpp:239-257)
  *   slx_modal_solve      solve_modal's shift-invert subspace iteration with warm-started
  *                        AMG-CG solves (src/fem/ModalAnalysis.cpp:120-418)
  *
@@ -351,6 +353,13 @@ slx_status slx_oc_update(slx_context* ctx, const double* x, const double* dc_dx,
  * target fraction. */
 slx_status slx_run_oc(slx_context* ctx, const slx_run_options* opt, double* x, double* history,
                       slx_run_result* result);
+/* ComplianceObjective::solve_adjoint (src/topopt/Sensitivity.cpp:239-257): solves
+ * K_ff lambda = rhs on the operator of the last evaluation (rhs and lambda full
+ * length ndofs, lambda zero on the fixed DOFs) with the context's AMG-PCG to tol.
+ * slot >= 0 warm-starts from that slot's previous solution and stores the new
+ * one; slot < 0 starts from zero and stores nothing. */
+slx_status slx_solve_adjoint(slx_context* ctx, const double* rhs, int slot, double tol, int max_iterations,
+                             double* lambda, slx_pcg_result* result);
 /* compliance[ncases]: f_l . u_l of each load case at the last evaluation
  * (ObjectiveEvaluation::load_case_compliance, include/sparlab/topopt/Sensitivity.hpp:78) */
 slx_status slx_load_case_compliance_get(const slx_context* ctx, double* compliance);

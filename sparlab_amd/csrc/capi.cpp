@@ -1107,6 +1107,42 @@ slx_status slx_load_case_compliance_get(const slx_context* ctx, double* complian
   });
 }
 
+slx_status slx_solve_adjoint(slx_context* ctx, const double* rhs, int slot, double tol, int max_iterations,
+                             double* lambda, slx_pcg_result* result) {
+  return guarded(ctx, [&](Context& c) {
+    // ComplianceObjective::solve_adjoint (src/topopt/Sensitivity.cpp:239-257) through
+    // StaticAnalysis::solve_homogeneous: K_ff lambda_f = rhs_f, zero on the fixed DOFs
+    if (!c.amg_ready || c.K.rows == 0) fail(SLX_MODEL_ERROR, "solve_adjoint called before the objective was evaluated");
+    if (!(tol > 0.0)) fail(SLX_CONFIG_ERROR, "the adjoint tolerance must be positive");
+    const size_t nf = c.free_dofs.size();
+    std::vector<double> bf(nf);
+    for (size_t i = 0; i < nf; ++i) bf[i] = rhs[c.free_dofs[i]];
+    DBuf<double> b, x;
+    b.upload(bf, c.stream);
+    x.alloc(nf);
+    if (slot >= 0 && (size_t)slot >= c.adjoint_slots.size()) c.adjoint_slots.resize((size_t)slot + 1);
+    if (slot >= 0 && c.adjoint_slots[(size_t)slot].n == nf)
+      k::copy((long long)nf, c.adjoint_slots[(size_t)slot].p, x.p, c.stream);  // warm start from the slot
+    else
+      k::fill((long long)nf, 0.0, x.p, c.stream);
+    const slx_pcg_result r = pcg(c, b.p, x.p, tol, max_iterations > 0 ? max_iterations : 1000);
+    ++c.adjoint_solves;
+    c.adjoint_iterations += r.iterations;
+    require_converged(c, r, tol);
+    if (slot >= 0) {
+      DBuf<double>& keep = c.adjoint_slots[(size_t)slot];
+      keep.alloc(nf);
+      k::copy((long long)nf, x.p, keep.p, c.stream);
+    }
+    std::vector<double> xf(nf);
+    x.download(xf.data(), nf, c.stream);
+    sync(c);
+    std::fill(lambda, lambda + c.ndofs, 0.0);
+    for (size_t i = 0; i < nf; ++i) lambda[c.free_dofs[i]] = xf[i];
+    if (result) *result = r;
+  });
+}
+
 void slx_modal_default_options(slx_modal_options* o) {
   // ModalAnalysisOptions (include/sparlab/fem/ModalAnalysis.hpp:44-68), Config.cpp:1120-1136
   o->num_modes = 6;

@@ -253,6 +253,8 @@ struct Context {
   DCsr dH, dHt;
   DBuf<double> d_x, d_phys, d_fac, d_dfac, d_dc, d_energy, d_vol, d_tmp, d_tmp2, d_lower, d_upper, d_dcx, d_dvx;
   DBuf<double> d_dproj;                   // d physical / d filtered under the projection
+  std::vector<DBuf<double>> adjoint_slots;  // last adjoint solution per warm-start slot (free DOFs)
+  long long adjoint_solves = 0, adjoint_iterations = 0;
   double proj_beta = 0.0, proj_eta = 0.5;  // projection of the last evaluation (beta 0: off)
   DBuf<double> oc_state;  // the device bisection state
   DBuf<unsigned char> d_passive;

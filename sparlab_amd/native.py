@@ -199,6 +199,7 @@ _SIGNATURES = {
     "slx_oc_update": (C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(OcOptions), _VP, C.POINTER(OcStep)]),
     "slx_run_oc": (C.c_int, [_VP, C.POINTER(RunOptions), _VP, _VP, C.POINTER(RunResult)]),
     "slx_load_case_compliance_get": (C.c_int, [_VP, _VP]),
+    "slx_solve_adjoint": (C.c_int, [_VP, _VP, C.c_int, C.c_double, C.c_int, _VP, C.POINTER(PcgResult)]),
     "slx_modal_default_options": (None, [C.POINTER(ModalOptions)]),
     "slx_modal_solve": (C.c_int, [_VP, C.POINTER(ModalOptions), _VP, _VP, _VP, _VP, _VP, C.POINTER(ModalResult)]),
     "slx_warmstart_default_options": (None, [C.POINTER(WarmStartOptions)]),
@@ -722,6 +723,20 @@ class HipProblem:
         if shapes:
             out["mode_shapes"] = phi.T.copy()
         return out
+
+    def solve_adjoint(self, rhs, slot=-1, tol=1e-8, max_iterations=1000):
+        """ComplianceObjective::solve_adjoint: K_ff lambda = rhs (full-length vectors) on the
+        operator of the last evaluation, warm-started from ``slot``'s previous solution when
+        slot >= 0. Returns (lambda, PCG iterations)."""
+        self._own()
+        b = _f64(rhs)
+        if b.shape != (self.ndofs,):
+            raise ConfigError(f"the adjoint right-hand side has {b.size} entries for {self.ndofs} DOFs")
+        lam = np.empty(self.ndofs)
+        r = PcgResult()
+        self.ctx.check(load().slx_solve_adjoint(self.ctx.h, _p(b), int(slot), tol, int(max_iterations), _p(lam),
+                                                C.byref(r)))
+        return lam, r.iterations
 
     def load_case_compliance(self):
         """f_l . u_l of each load case at the last evaluation."""

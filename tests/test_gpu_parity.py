@@ -207,6 +207,29 @@ def test_robust_projected_run_against_oracle(ctx):
     assert lr["compliance_eroded"] > fg > lr["compliance_dilated"]
 
 
+def test_adjoint_solves_with_warm_slots(ctx):
+    """ComplianceObjective::solve_adjoint: on the compliance operator the adjoint of the
+    load is the displacement; a warm slot starts from its last solution."""
+    print("test_adjoint_solves_with_warm_slots", flush=True)
+    m = M.mbb_beam(60, 20)
+    g, o = _pair(m, ctx, tol=1e-11)
+    with pytest.raises(N.ModelError):
+        g.solve_adjoint(m.loads[0])
+    x = np.random.default_rng(11).uniform(0.2, 0.9, m.nelem)
+    g.evaluate(x, tol=1e-11)
+    u = o.evaluate(x, want_u=True)["u"][0]
+    f = m.loads[0]
+    lam, it0 = g.solve_adjoint(f, slot=0, tol=1e-10)
+    np.testing.assert_allclose(lam, u, rtol=1e-7, atol=1e-9 * np.abs(u).max())
+    assert np.all(lam[m.fixed.astype(bool)] == 0.0)
+    lam2, it1 = g.solve_adjoint(1.001 * f, slot=0, tol=1e-10)
+    np.testing.assert_allclose(lam2, 1.001 * lam, rtol=1e-7, atol=1e-9 * np.abs(u).max())
+    lam3, it3 = g.solve_adjoint(1.001 * f, slot=-1, tol=1e-10)
+    np.testing.assert_allclose(lam3, lam2, rtol=1e-7, atol=1e-9 * np.abs(u).max())
+    print(f"adjoint iterations: cold {it0}, warm slot {it1}, no slot {it3}", flush=True)
+    assert it1 < it3
+
+
 def test_fp64_runs_are_bitwise_reproducible():
     """Two FP64 runs in separate contexts give the same history bit for bit (the
     CSR kernels' lane counts follow a fixed rule on the FP64 path)."""
