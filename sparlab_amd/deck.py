@@ -501,19 +501,25 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     if method != "oc":
         _unsupported("topology.optimizer.method", f"the {method} update")
     # Heaviside projection with beta continuation (Config.cpp:1594-1617, Projection.cpp:10-56)
-    pnode = topo.get("projection", {}) or {}
+    pnode = dict(topo.get("projection", {}) or {})
+    for key in ("projection", "robust"):  # the CLI's switches (apps/sparlab_topopt.cpp:300-318)
+        if key in ov:
+            pnode["enabled" if key == "projection" else "robust"] = bool(ov[key])
+    if ov.get("robust"):
+        pnode["enabled"] = True
+    if "beta_max" in ov:
+        pnode["beta_max"] = float(ov["beta_max"])
     projection = None
     if pnode.get("enabled", False):
         robust = bool(pnode.get("robust", False))
-        if pnode.get("erosion_check", False) and not robust:
-            # a report on a non-robust design (TopologyOptimizer.cpp:396-420); a robust run ignores the flag
-            _unsupported("topology.projection.erosion_check", "the erosion check")
         projection = dict(eta=float(pnode.get("eta", 0.5)), beta_start=float(pnode.get("beta_start", 1.0)),
                           beta_max=float(pnode.get("beta_max", 32.0)), beta_factor=float(pnode.get("beta_factor", 2.0)),
                           beta_interval=int(pnode.get("beta_interval", 50)),
                           advance_on_convergence=bool(pnode.get("advance_on_convergence", True)), robust=robust,
                           robust_delta=float(pnode.get("robust_delta", 0.1)),
-                          robust_volume_interval=int(pnode.get("robust_volume_interval", 1)))
+                          robust_volume_interval=int(pnode.get("robust_volume_interval", 1)),
+                          # a report on a non-robust design (TopologyOptimizer.cpp:396-420); robust runs ignore it
+                          erosion_check=bool(pnode.get("erosion_check", False)) and not robust)
         pj = projection
         if not (0.0 < pj["eta"] < 1.0):
             raise DeckError(f"topology.projection.eta must lie in (0, 1), got {pj['eta']}")
@@ -529,7 +535,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
         if pj["beta_interval"] < 1:
             raise DeckError(f"topology.projection.beta_interval must be at least 1, got {pj['beta_interval']}")
         d = pj["robust_delta"]
-        if robust and not (d > 0.0 and pj["eta"] - d > 0.0 and pj["eta"] + d < 1.0):
+        if (robust or pj["erosion_check"]) and not (d > 0.0 and pj["eta"] - d > 0.0 and pj["eta"] + d < 1.0):
             raise DeckError(f"topology.projection.robust_delta = {d} must be positive and keep eta +- delta inside "
                             f"(0, 1) (eta = {pj['eta']})")
         if robust and pj["robust_volume_interval"] < 1:
@@ -624,7 +630,10 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     if not (0.0 < thr < 1.0):
         raise DeckError("optimizer.interpretation_threshold must lie in (0, 1)")
     # modal block (Config.cpp:1118-1143)
-    mnode = doc.get("modal", {})
+    mnode = dict(doc.get("modal", {}))
+    if "modes" in ov:  # --modes n (apps/sparlab_topopt.cpp:295-297)
+        mnode["enabled"] = True
+        mnode["num_modes"] = int(ov["modes"])
     modal = flags = None
     if mnode.get("enabled", False):
         if not density > 0.0:

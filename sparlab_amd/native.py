@@ -649,7 +649,7 @@ class HipProblem:
         o.eval = eval_options(tolerance=tol, penalty=self.model.penalty, emin_ratio=self.model.emin_ratio)
         pr = dict(eta=0.5, beta_start=1.0, beta_max=32.0, beta_factor=2.0, beta_interval=50,
                   advance_on_convergence=True, robust=False, robust_delta=0.1, robust_volume_interval=1)
-        pr.update(projection or {})
+        pr.update({k: v for k, v in (projection or {}).items() if k != "erosion_check"})
         o.projection = ProjectionOptions(1 if projection else 0, pr["eta"], pr["beta_start"], pr["beta_max"],
                                          pr["beta_factor"], int(pr["beta_interval"]),
                                          1 if pr["advance_on_convergence"] else 0, 1 if pr["robust"] else 0,

@@ -220,15 +220,31 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
             "filtered_grey_level": float(4.0 * np.sum(filtered * (1.0 - filtered)) / m.nelem),
             "note": "grey_level is that of the projected (physical) density; filtered_grey_level that of the "
                     "density before projection"}
-        if robust:
-            # RobustRecord (ResultWriter.cpp:2766-2790): the blueprint is the reported design
-            summary["optimization_result"]["robust"] = {
-                "eta_eroded": eta + pj["robust_delta"], "eta_intermediate": eta, "eta_dilated": eta - pj["robust_delta"],
-                "compliance_eroded_J": res["compliance_eroded"], "compliance_intermediate_J": comp,
-                "compliance_dilated_J": res["compliance_dilated"],
-                "volume_fraction_eroded": res["volume_fraction_eroded"], "volume_fraction_intermediate": vf,
-                "volume_fraction_dilated": res["volume_fraction_dilated"],
-                "dilated_target_fraction": res["dilated_target_fraction"]}
+        if robust or pj["erosion_check"]:
+            # RobustRecord (ResultWriter.cpp:2766-2800): the three designs at eta + delta, eta, eta - delta
+            d = pj["robust_delta"]
+            if robust:
+                ce, ve, cd, vd = (res["compliance_eroded"], res["volume_fraction_eroded"], res["compliance_dilated"],
+                                  res["volume_fraction_dilated"])
+            else:
+                # the erosion check of a non-robust design (TopologyOptimizer.cpp:396-420)
+                er = g.evaluate(x, tol=tol, need_gradients=False, beta=beta, eta=eta + d)
+                di = g.evaluate(x, tol=tol, need_gradients=False, beta=beta, eta=eta - d)
+                ce, ve, cd, vd = er["compliance"], er["volume_fraction"], di["compliance"], di["volume_fraction"]
+            block = {"designs": [
+                {"design": "eroded", "eta": eta + d, "compliance_J": ce, "volume_fraction": ve},
+                {"design": "intermediate (blueprint)", "eta": eta, "compliance_J": comp, "volume_fraction": vf},
+                {"design": "dilated", "eta": eta - d, "compliance_J": cd, "volume_fraction": vd}]}
+            if robust:
+                block["dilated_target_fraction"] = res["dilated_target_fraction"]
+                block["note"] = ("compliance_J and volume_fraction below are the blueprint's; the optimiser "
+                                 "minimised the eroded design's compliance")
+                summary["optimization_result"]["robust"] = block
+            else:
+                block["note"] = ("a design optimised without the robust formulation, evaluated once at the eroded and "
+                                 "dilated thresholds eta +- robust_delta of the final beta: its compliance if the part "
+                                 "came out uniformly thinner or thicker than drawn")
+                summary["optimization_result"]["erosion_check"] = block
     for key in dk.skipped_reports:
         summary["optimization_result"]["warnings"].append(f"{key} is a report on the final design that this driver "
                                                           "does not compute")
@@ -304,6 +320,13 @@ def main(argv=None) -> int:
                     help="multigrid hierarchy precision (fp64: the drop-in path)")
     ap.add_argument("--tolerance", type=float, help="relative residual of the CG solves (default: the deck's)")
     ap.add_argument("--device", type=int, default=0)
+    # apps/sparlab_topopt.cpp:165-180,290-330
+    ap.add_argument("--modes", type=int, help="enable modal analysis with n modes")
+    ap.add_argument("--projection", action="store_true", help="switch the Heaviside projection on")
+    ap.add_argument("--no-projection", action="store_true", help="switch the deck's projection off")
+    ap.add_argument("--beta-max", type=float, help="override topology.projection.beta_max")
+    ap.add_argument("--robust", action="store_true", help="robust (eroded / dilated) formulation, with the projection")
+    ap.add_argument("--no-robust", action="store_true", help="switch the deck's robust formulation off")
     a = ap.parse_args(argv)
     ov = {}
     for name, _ in _OVERRIDES:
@@ -312,6 +335,18 @@ def main(argv=None) -> int:
             ov[name.replace("-", "_")] = v
     if a.load_weights:
         ov["load_weights"] = [float(w) for w in a.load_weights.split(",")]
+    if a.modes is not None:
+        ov["modes"] = a.modes
+    if a.projection or a.robust:
+        ov["projection"] = True
+    if a.no_projection:
+        ov["projection"] = False
+    if a.beta_max is not None:
+        ov["beta_max"] = a.beta_max
+    if a.robust:
+        ov["robust"] = True
+    if a.no_robust:
+        ov["robust"] = False
     if a.solver and a.solver != "amg_cg":
         print(f"note: solver.linear.type {a.solver} is replaced by the GPU's amg_cg", file=sys.stderr)
     try:

@@ -154,7 +154,6 @@ def test_overrides():
                                                                         "ymax": 0.15}}, "traction": [1, 0]}]),
      "matched no boundary edge"),
     (lambda d: d["load_cases"][0].update(gravity=[0, -9.81]), "self-weight"),
-    (lambda d: d["topology"].update(projection={"enabled": True, "erosion_check": True}), "erosion_check"),
     (lambda d: d["topology"].update(projection={"enabled": True, "robust": True, "robust_delta": 0.6}),
      "robust_delta"),
     (lambda d: d["topology"].update(projection={"enabled": True, "beta_factor": 1.0}), "beta_factor"),
@@ -232,3 +231,22 @@ def test_initial_and_solid_compliance_match_the_published_table(case):
     o = Oracle(m, tol=1e-12, max_iterations=5000, warm_start=False)
     c0 = o.evaluate(d.initial_design(), need_gradients=False)["compliance"]
     assert c0 == pytest.approx(pub["initial_compliance_J"], rel=1e-9)
+
+
+def test_cli_switches_for_projection_and_modes():
+    """--projection / --no-projection / --beta-max / --robust / --modes (apps/sparlab_topopt.cpp:165-330)."""
+    d = D.load_deck(fixture("mbb_beam_projected"), overrides=dict(projection=False))
+    assert d.run["projection"] is None
+    d = D.load_deck(fixture("mbb_beam"), overrides=dict(projection=True, beta_max=8.0))
+    assert d.run["projection"]["beta_max"] == 8.0 and not d.run["projection"]["robust"]
+    d = D.load_deck(fixture("mbb_beam"), overrides=dict(robust=True))
+    assert d.run["projection"]["robust"]
+    d = D.load_deck(fixture("mbb_beam_robust"), overrides=dict(robust=False))
+    assert d.run["projection"] is not None and not d.run["projection"]["robust"]
+    assert d.run["projection"]["erosion_check"]  # the deck's check then reports on the non-robust design
+    d = D.load_deck(fixture("cantilever_beam"), overrides=dict(modes=3))
+    assert d.modal["num_modes"] == 3
+    doc = json.load(open(fixture("mbb_beam")))
+    doc["material"]["density"] = 0.0
+    with pytest.raises(D.DeckError, match="density"):
+        D.parse_deck(doc, overrides=dict(modes=2))

@@ -78,9 +78,10 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
     assert rows[0] == list(N.HISTORY_COLUMNS[: N.H_BETA + (1 if pj else 0)]) + (
         list(N.HISTORY_COLUMNS[N.H_ERODED_VF:]) if robust else [])
     if robust:
-        rb = r["robust"]
-        assert rb["compliance_eroded_J"] > rb["compliance_intermediate_J"] > rb["compliance_dilated_J"]
-        assert rb["volume_fraction_eroded"] < rb["volume_fraction_intermediate"] < rb["volume_fraction_dilated"]
+        des = r["robust"]["designs"]
+        assert des[0]["compliance_J"] > des[1]["compliance_J"] > des[2]["compliance_J"]
+        assert des[0]["volume_fraction"] < des[1]["volume_fraction"] < des[2]["volume_fraction"]
+        assert des[1]["compliance_J"] == r["compliance_J"]
     if pj:
         assert r["projection"]["final_beta"] == pub["final_beta"]
         assert float(rows[-1][rows[0].index("beta[-]")]) == pub["final_beta"]
@@ -117,3 +118,22 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
             assert topo["frequencies_hz"][0] == pytest.approx(pub["f1_topology_Hz"], rel=ftol)
     else:
         assert "modal_initial_solid" not in saved
+
+
+def test_cli_switches_erosion_check_and_modes(tmp_path):
+    """--no-robust leaves the deck's erosion check, which reports the design at eta +- delta
+    (TopologyOptimizer.cpp:396-420); --modes turns the modal analysis on."""
+    from sparlab_amd.topopt import run
+    d = D.load_deck(os.path.join(GOLD, "decks", "mbb_beam_robust.json"),
+                    overrides=dict(robust=False, max_iterations=30, modes=2))
+    print("deck mbb_beam_robust --no-robust --max-iterations 30 --modes 2", flush=True)
+    s = run(d, str(tmp_path), log=lambda t: print(t, flush=True))
+    r = s["optimization_result"]
+    des = r["erosion_check"]["designs"]
+    assert des[0]["eta"] == pytest.approx(0.6) and des[2]["eta"] == pytest.approx(0.4)
+    assert des[0]["compliance_J"] > des[1]["compliance_J"] > des[2]["compliance_J"]
+    assert des[0]["volume_fraction"] < des[1]["volume_fraction"] < des[2]["volume_fraction"]
+    assert des[1]["compliance_J"] == r["compliance_J"]
+    assert "robust" not in r
+    modes = s["modal_initial_solid"]
+    assert modes["num_modes"] == 2 and modes["converged"]
