@@ -15,7 +15,12 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, never by the product package.
   ModalAnalysis.cpp:27-50), the relative-change and eigenpair-residual
   stopping rule, M-normalisation and total mass.
 
-Pinned in tests/test_modal_oracle.py against: the reference's published
+* ``components_by_face`` / ``submesh_operators``: components_by_face and
+  extract_element_subset (src/mesh/SubMesh.cpp:38-140), pinned in
+  tests/test_interpret.py on hand-built contact cases (corner, edge, face) and
+  on the every-element sub-mesh, which must equal the permuted full operators.
+
+The modal functions are pinned in tests/test_modal_oracle.py against: the reference's published
 full-solid first frequencies and masses of the benchmark decks
 (docs/results/benchmarks.csv), shift-invert Lanczos (scipy eigsh) and dense
 generalised eigensolves, Euler-Bernoulli and fixed-free rod theory

@@ -82,3 +82,40 @@ def test_masked_grid_operators_equal_the_extracted_submesh(dim):
     b = sla.eigh(Ks.toarray(), Ms.toarray(), eigvals_only=True)[:6]
     assert np.allclose(a, b, rtol=1e-10, atol=0)
     assert OM.total_mass(sub, RHO, ind) == pytest.approx(it.volume_retained * RHO, rel=1e-12)
+
+
+def test_oracle_face_components_known_answers():
+    """The oracle's union-find on hand-built cases: corner or edge contact is no connection,
+    a shared face (an edge in 2-D) is one."""
+    q = StructuredModel(dim=2, nx=4, ny=4, lx=1.0, ly=1.0)
+    assert len(OM.components_by_face(q, [0, 5])) == 2        # diagonal neighbours: a corner
+    assert len(OM.components_by_face(q, [0, 1, 5])) == 1     # an L through a shared edge
+    assert len(OM.components_by_face(q, [15, 0, 10, 5])) == 4  # a diagonal chain touches only at corners
+    h = StructuredModel(dim=3, nx=3, ny=3, nz=3, lx=1.0, ly=1.0, lz=1.0)
+    e = lambda i, j, k: (k * 3 + j) * 3 + i
+    assert len(OM.components_by_face(h, [e(0, 0, 0), e(1, 1, 0)])) == 2   # a shared edge only
+    assert len(OM.components_by_face(h, [e(0, 0, 0), e(1, 1, 1)])) == 2   # a shared corner only
+    assert len(OM.components_by_face(h, [e(0, 0, 0), e(0, 0, 1)])) == 1   # a shared face
+    assert OM.components_by_face(h, [e(2, 2, 2), e(0, 0, 0), e(1, 0, 0)]) == [[e(2, 2, 2)], [e(0, 0, 0), e(1, 0, 0)]]
+
+
+def test_oracle_submesh_of_every_element_is_the_full_model():
+    """Extracting every element only renumbers the nodes (by first appearance): the sub-mesh
+    operators are the full K_ff and M_ff permuted accordingly."""
+    m = StructuredModel(dim=2, nx=6, ny=3, lx=0.6, ly=0.3, E=70e9, nu=0.3, thickness=0.01)
+    m.fix(m.select_box(xmax=0.0))
+    o = Oracle(m)
+    K = OM.kff_csr(o)
+    M = OM.reduce_free(OM.assemble_mass(m, RHO), o.free())
+    Ks, Ms = OM.submesh_operators(m, o.ke0(), RHO, np.arange(m.nelem), m.fixed)
+    flat = OM.element_nodes(m).ravel()
+    _, first = np.unique(flat, return_index=True)
+    order = flat[np.sort(first)]                      # old node of each new node
+    dofs = (order[:, None] * 2 + np.arange(2)).ravel()  # old DOF of each new DOF
+    fixed = np.asarray(m.fixed, dtype=bool)
+    new_free_old = dofs[~fixed[dofs]]                 # old DOF of each new free DOF
+    pos = {d: i for i, d in enumerate(o.free())}
+    p = np.array([pos[d] for d in new_free_old])
+    Kp, Mp = K[p][:, p], M[p][:, p]
+    assert abs(Kp - Ks).max() <= 1e-12 * abs(K).max()
+    assert abs(Mp - Ms).max() <= 1e-15 * abs(M).max()
