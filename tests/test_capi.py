@@ -45,12 +45,17 @@ def test_binding_struct_layout_matches_header(tmp_path):
                "slx_amg_stats": native.AmgStats, "slx_eval_options": native.EvalOptions,
                "slx_evaluation": native.Evaluation, "slx_oc_options": native.OcOptions,
                "slx_oc_step": native.OcStep, "slx_run_options": native.RunOptions,
-               "slx_warmstart_options": native.WarmStartOptions, "slx_warmstart_stats": native.WarmStartStats}
+               "slx_warmstart_options": native.WarmStartOptions, "slx_warmstart_stats": native.WarmStartStats,
+               "slx_run_result": native.RunResult, "slx_projection_options": native.ProjectionOptions,
+               "slx_modal_options": native.ModalOptions, "slx_modal_result": native.ModalResult}
     src = tmp_path / "layout.c"
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "sparlab_hip.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
     lines.append('printf("run.eval %zu\\n", offsetof(slx_run_options, eval));')
+    for cname, cls in structs.items():  # every field's offset (ctypes names carry a trailing _ for C keywords)
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname.rstrip("_")}));')
     lines.append("return 0; }")
     src.write_text("\n".join(lines))
     exe = tmp_path / "layout"
@@ -59,3 +64,6 @@ def test_binding_struct_layout_matches_header(tmp_path):
     for cname, cls in structs.items():
         assert int(out[cname]) == ctypes.sizeof(cls), cname
     assert int(out["run.eval"]) == native.RunOptions.eval.offset
+    for cname, cls in structs.items():
+        for fname, _ in cls._fields_:
+            assert int(out[f"{cname}.{fname}"]) == getattr(cls, fname).offset, f"{cname}.{fname}"
