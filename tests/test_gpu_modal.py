@@ -126,3 +126,18 @@ def test_modal_errors():
         g.modal(RHO, num_modes=33)  # a subspace of 66 > 64
     with pytest.raises(N.ModelError):
         g.modal(RHO, mass_scale=-np.ones(64))
+
+
+def test_modes_with_the_mixed_precision_hierarchy():
+    """The shifted solves take the context's AMG options: with the FP32 V-cycle under FP64
+    flexible CG the eigenpairs are the same to the solve tolerance."""
+    from sparlab_amd import native as N
+    m = hex8_cantilever()
+    g = N.HipProblem(m, device=0)
+    g.assemble(np.ones(m.nelem))
+    g.amg_setup(precision=1)
+    r = g.modal(RHO, num_modes=4, linear_tolerance=1e-10)
+    ref, _, _ = oracle_modes(m, 4)
+    print(f"mixed precision: {r['iterations']} subspace iterations, {r['linear_iterations']} CG iterations", flush=True)
+    assert r["converged"]
+    assert np.allclose(r["eigenvalues_1_per_s2"], ref["eigenvalues"], rtol=1e-9, atol=0)
