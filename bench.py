@@ -347,6 +347,9 @@ def main():
         sys.exit(2)
 
     if args.workload == "batch":
+        if os.environ.get("SPARLAB_BATCH_BENCH") != "1":
+            sys.exit("bench.py: the batch workload is disabled: a GPU run of it ended with no output and no cause "
+                     "has been found; set SPARLAB_BATCH_BENCH=1 to run it anyway")
         batch_main(args)
         return
 

@@ -107,6 +107,12 @@ typedef struct {
   double galerkin_seconds;
   double coarse_seconds;
   int node_product_levels; /* levels whose products ran on the node-row kernels */
+  /* AmgStats::storage_nonzeros (src/fem/Multigrid.cpp:1089-1100): the coarse
+   * operators plus P-hat, P, R, A P and R A P of every level, plus n_c^2 when the
+   * coarsest level has at most 4000 unknowns; what LinearSolver::storage_nonzeros()
+   * reports (src/fem/LinearSolver.cpp:258) */
+  long long storage_nonzeros;
+  double tentative_seconds; /* AmgStats::tentative_seconds: the tentative prolongators */
 } slx_amg_stats;
 
 /* Evaluation options: SIMP law (include/sparlab/topopt/SimpInterpolation.hpp:71-80)

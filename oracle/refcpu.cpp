@@ -624,34 +624,111 @@ Csr symmetrize(const Csr& a) {
   return s;
 }
 
-double tridiagonal_max_eigenvalue(const std::vector<double>& diag, const std::vector<double>& sub) {
-  // Sturm-sequence bisection (Wilkinson): the number of eigenvalues below x is
-  // the number of negative pivots of T - xI. Bisect on a Gershgorin interval.
-  const size_t k = diag.size();
-  if (k == 1) return diag[0];
-  double lo = std::numeric_limits<double>::max(), hi = -lo;
-  for (size_t i = 0; i < k; ++i) {
-    const double r = (i > 0 ? std::abs(sub[i - 1]) : 0.0) + (i + 1 < k ? std::abs(sub[i]) : 0.0);
-    lo = std::min(lo, diag[i] - r);
-    hi = std::max(hi, diag[i] + r);
+namespace {
+
+// Plane rotation [c s; -s c] with [c s; -s c]^T [p; q] = [r; 0], in the sign
+// convention of Eigen 3.4.0's JacobiRotation::makeGivens (real case).
+void givens_rotation(double p, double q, double& c, double& s) {
+  if (q == 0.0) {
+    c = p < 0.0 ? -1.0 : 1.0;
+    s = 0.0;
+  } else if (p == 0.0) {
+    c = 0.0;
+    s = q < 0.0 ? 1.0 : -1.0;
+  } else if (std::abs(p) > std::abs(q)) {
+    const double t = q / p;
+    double u = std::sqrt(1.0 + t * t);
+    if (p < 0.0) u = -u;
+    c = 1.0 / u;
+    s = -t * c;
+  } else {
+    const double t = p / q;
+    double u = std::sqrt(1.0 + t * t);
+    if (q < 0.0) u = -u;
+    s = -1.0 / u;
+    c = -t * s;
   }
-  auto count_below = [&](double x) {
-    int c = 0;
-    double q = diag[0] - x;
-    if (q < 0) ++c;
-    for (size_t i = 1; i < k; ++i) {
-      if (q == 0.0) q = std::numeric_limits<double>::epsilon() * (std::abs(hi) + std::abs(lo) + 1e-300);
-      q = diag[i] - x - sub[i - 1] * sub[i - 1] / q;
-      if (q < 0) ++c;
+}
+
+// sqrt(x^2 + y^2) for x, y >= 0 as max * sqrt(1 + (min / max)^2)
+double scaled_hypot(double x, double y) {
+  const double p = std::max(x, y);
+  if (p == 0.0) return 0.0;
+  const double r = std::min(y, x) / p;
+  return p * std::sqrt(1.0 + r * r);
+}
+
+// One implicit symmetric QR step with the Wilkinson shift on the unreduced
+// block [start, end] (Golub & Van Loan, Algorithm 8.3.2), chasing the bulge
+// with plane rotations.
+void tridiagonal_qr_step(double* d, double* e, int start, int end) {
+  const double td = (d[end - 1] - d[end]) * 0.5;
+  const double f = e[end - 1];
+  double mu = d[end];
+  if (td == 0.0) {
+    mu -= std::abs(f);
+  } else if (f != 0.0) {
+    const double f2 = f * f;
+    const double h = scaled_hypot(std::abs(td), std::abs(f));
+    if (f2 == 0.0)
+      mu -= f / ((td + (td > 0.0 ? h : -h)) / f);
+    else
+      mu -= f2 / (td + (td > 0.0 ? h : -h));
+  }
+  double x = d[start] - mu, z = e[start];
+  for (int k = start; k < end && z != 0.0; ++k) {
+    double c, s;
+    givens_rotation(x, z, c, s);
+    const double sdk = s * d[k] + c * e[k];
+    const double dkp1 = s * e[k] + c * d[k + 1];
+    d[k] = c * (c * d[k] - s * e[k]) - s * (c * e[k] - s * d[k + 1]);
+    d[k + 1] = s * sdk + c * dkp1;
+    e[k] = c * sdk - s * dkp1;
+    if (k > start) e[k - 1] = c * e[k - 1] - s * z;
+    x = e[k];
+    if (k < end - 1) {
+      z = -s * e[k + 1];
+      e[k + 1] = c * e[k + 1];
     }
-    return c;
-  };
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;
-    if (count_below(mid) >= (int)k) hi = mid; else lo = mid;
   }
-  return hi;
+}
+
+}  // namespace
+
+double tridiagonal_max_eigenvalue(const std::vector<double>& diag, const std::vector<double>& sub) {
+  // Multigrid.cpp:324-326 takes the largest eigenvalue from Eigen 3.4.0's
+  // SelfAdjointEigenSolver::computeFromTridiagonal (eigenvalues only, no
+  // scaling, at most 30 n sweeps). That routine is the implicit symmetric QR
+  // iteration with Wilkinson shifts; restated here with its deflation rule: an
+  // off-diagonal entry is set to zero when it is below the smallest normal
+  // number or when (e_i / eps)^2 <= |d_i| + |d_(i+1)|; the trailing unreduced
+  // block is then stepped.
+  const int k = (int)diag.size();
+  if (k == 1) return diag[0];
+  std::vector<double> d(diag), e(sub.begin(), sub.begin() + (k - 1));
+  const double tiny = std::numeric_limits<double>::min();
+  const double inv_eps = 1.0 / std::numeric_limits<double>::epsilon();
+  int end = k - 1, start = 0;
+  long long iter = 0;
+  while (end > 0) {
+    for (int i = start; i < end; ++i) {
+      if (std::abs(e[(size_t)i]) < tiny) {
+        e[(size_t)i] = 0.0;
+      } else {
+        const double se = inv_eps * e[(size_t)i];
+        if (se * se <= std::abs(d[(size_t)i]) + std::abs(d[(size_t)i + 1])) e[(size_t)i] = 0.0;
+      }
+    }
+    while (end > 0 && e[(size_t)end - 1] == 0.0) --end;
+    if (end <= 0) break;
+    if (++iter > 30LL * k) break;
+    start = end - 1;
+    while (start > 0 && e[(size_t)start - 1] != 0.0) --start;
+    tridiagonal_qr_step(d.data(), e.data(), start, end);
+  }
+  double best = d[0];
+  for (int i = 1; i < k; ++i) best = std::max(best, d[(size_t)i]);
+  return best;
 }
 
 double lanczos_lambda_max(const Csr& a, const std::vector<double>& inv_diag, int steps) {
@@ -685,67 +762,157 @@ double lanczos_lambda_max(const Csr& a, const std::vector<double>& inv_diag, int
   return tridiagonal_max_eigenvalue(alpha, sub);
 }
 
+namespace {
+
+// Reduction orders of Eigen 3.4.0 built for x86-64 with SSE2 and no -march
+// flag (the reference's build), where a double packet holds two lanes.
+//
+// sum / squaredNorm / dot of a dynamic vector (redux, linear vectorised
+// traversal, no unrolling): lanes {0,1} and {2,3} start two packet
+// accumulators, each later step of four adds {i,i+1} to the first and
+// {i+2,i+3} to the second; the two packets are added, then one more packet if
+// two entries of the even part remain, then the packet's two lanes, then the
+// odd last entry.
+template <class F>
+double packet_redux(Index n, F&& x) {
+  if (n <= 0) return 0.0;
+  const Index n2 = (n / 2) * 2;
+  if (n2 == 0) return x(0);
+  double p0a = x(0), p0b = x(1);
+  if (n2 > 2) {
+    double p1a = x(2), p1b = x(3);
+    const Index n4 = (n / 4) * 4;
+    for (Index i = 4; i < n4; i += 4) {
+      p0a = p0a + x(i);
+      p0b = p0b + x(i + 1);
+      p1a = p1a + x(i + 2);
+      p1b = p1b + x(i + 3);
+    }
+    p0a = p0a + p1a;
+    p0b = p0b + p1b;
+    if (n2 > n4) {
+      p0a = p0a + x(n4);
+      p0b = p0b + x(n4 + 1);
+    }
+  }
+  double res = p0a + p0b;
+  for (Index i = n2; i < n; ++i) res = res + x(i);
+  return res;
+}
+
+// One output of the row-major matrix-vector kernel (row vector times a
+// column-major block): even and odd entries accumulate in the two lanes of one
+// packet (product, then add), the lanes are added, then the odd last product.
+double packet_gemv_dot(const double* a, const double* b, Index n) {
+  double c0 = 0.0, c1 = 0.0;
+  Index j = 0;
+  for (; j + 2 <= n; j += 2) {
+    c0 = a[j] * b[j] + c0;
+    c1 = a[j + 1] * b[j + 1] + c1;
+  }
+  double cc = c0 + c1;
+  for (; j < n; ++j) cc = cc + a[j] * b[j];
+  return 0.0 + cc;
+}
+
+// Applies H = I - tau [1; v] [1; v]^T to the rows x cols column-major block at
+// blk (leading dimension ld) as MatrixBase::applyHouseholderOnTheLeft does:
+// tmp = v^T bottom (the matrix-vector kernel, or a dot for one column),
+// tmp += row 0, row 0 -= tau tmp, bottom -= (tau v) tmp.
+void apply_householder_left(double* blk, Index ld, Index rows, Index cols, const double* v, double tau,
+                            std::vector<double>& tmp, std::vector<double>& tv) {
+  if (rows == 1) {
+    const double f = 1.0 - tau;
+    for (Index j = 0; j < cols; ++j) blk[(size_t)j * ld] *= f;
+    return;
+  }
+  if (tau == 0.0 || cols == 0) return;
+  const Index nb = rows - 1;
+  tmp.assign((size_t)cols, 0.0);
+  if (cols == 1) {
+    const double* b = blk + 1;
+    tmp[0] = 0.0 + packet_redux(nb, [&](Index i) { return v[i] * b[i]; });
+  } else {
+    for (Index j = 0; j < cols; ++j) tmp[(size_t)j] = packet_gemv_dot(blk + (size_t)j * ld + 1, v, nb);
+  }
+  for (Index j = 0; j < cols; ++j) tmp[(size_t)j] = tmp[(size_t)j] + blk[(size_t)j * ld];
+  for (Index j = 0; j < cols; ++j) blk[(size_t)j * ld] = blk[(size_t)j * ld] - tau * tmp[(size_t)j];
+  tv.resize((size_t)nb);
+  for (Index i = 0; i < nb; ++i) tv[(size_t)i] = tau * v[i];
+  for (Index j = 0; j < cols; ++j) {
+    double* col = blk + (size_t)j * ld + 1;
+    const double t = tmp[(size_t)j];
+    for (Index i = 0; i < nb; ++i) col[i] = col[i] - t * tv[(size_t)i];
+  }
+}
+
+}  // namespace
+
 Index cpqr_block(const std::vector<double>& block, Index rows, Index m, std::vector<double>& q,
                  std::vector<double>& r) {
-  // Column-pivoted Householder QR (Golub & Van Loan 5.4.1; norm downdating as
-  // in LAPACK xGEQP3, LAWN 176); Householder convention beta = -sign(x0)|x|
-  // (LAPACK xLARFG). Multigrid.cpp:606-617 takes rank at 1e-10 |R00|.
-  std::vector<double> a(block);  // rows x m row-major, overwritten by R and v
+  // Multigrid.cpp:606-617: Eigen::ColPivHouseholderQR of the aggregate's
+  // near-null-space rows, rank at 1e-10 |R00|, Q = householderQ() * I(rows,
+  // rank), R = R(0:rank, :) P^T. Restated from Eigen 3.4.0's
+  // ColPivHouseholderQR::computeInPlace (Businger-Golub column pivoting on the
+  // updated norms, first largest wins; LAPACK xGEQP3 norm downdating with the
+  // direct recomputation below sqrt(eps); Householder reflector beta =
+  // -sign(c0) sqrt(c0^2 + |tail|^2), v = tail / (c0 - beta), tau = (beta - c0)
+  // / beta, skipped when |tail|^2 <= DBL_MIN) and HouseholderSequence's
+  // unblocked application (reflectors last to first on the identity), with
+  // every sum in the order Eigen's SSE2 kernels use (packet_redux,
+  // packet_gemv_dot above).
+  std::vector<double> a((size_t)rows * m);  // column-major, like Eigen's storage
+  for (Index i = 0; i < rows; ++i)
+    for (Index j = 0; j < m; ++j) a[(size_t)j * rows + i] = block[(size_t)i * m + j];
+  auto at = [&](Index i, Index j) -> double& { return a[(size_t)j * rows + i]; };
+  auto norm_from = [&](Index j, Index from) {
+    const double* c = a.data() + (size_t)j * rows + from;
+    return std::sqrt(packet_redux(rows - from, [&](Index i) { return c[i] * c[i]; }));
+  };
   std::vector<Index> perm((size_t)m);
   std::iota(perm.begin(), perm.end(), 0);
   const Index size = std::min(rows, m);
-  std::vector<double> tau((size_t)size, 0.0), norm_upd((size_t)m), norm_dir((size_t)m);
-  auto col_norm = [&](Index j, Index from) {
-    double s = 0.0;
-    for (Index i = from; i < rows; ++i) s += a[(size_t)i * m + j] * a[(size_t)i * m + j];
-    return std::sqrt(s);
-  };
-  for (Index j = 0; j < m; ++j) norm_dir[(size_t)j] = norm_upd[(size_t)j] = col_norm(j, 0);
+  std::vector<double> tau((size_t)size, 0.0), norm_upd((size_t)m), norm_dir((size_t)m), tmp, tv;
+  for (Index j = 0; j < m; ++j) norm_dir[(size_t)j] = norm_upd[(size_t)j] = norm_from(j, 0);
   const double downdate_threshold = std::sqrt(std::numeric_limits<double>::epsilon());
   for (Index k = 0; k < size; ++k) {
     Index best = k;
     for (Index j = k + 1; j < m; ++j)
       if (norm_upd[(size_t)j] > norm_upd[(size_t)best]) best = j;
     if (best != k) {
-      for (Index i = 0; i < rows; ++i) std::swap(a[(size_t)i * m + k], a[(size_t)i * m + best]);
+      for (Index i = 0; i < rows; ++i) std::swap(at(i, k), at(i, best));
       std::swap(norm_upd[(size_t)k], norm_upd[(size_t)best]);
       std::swap(norm_dir[(size_t)k], norm_dir[(size_t)best]);
       std::swap(perm[(size_t)k], perm[(size_t)best]);
     }
-    // Householder vector for a(k:rows, k)
-    const double c0 = a[(size_t)k * m + k];
-    double tail = 0.0;
-    for (Index i = k + 1; i < rows; ++i) tail += a[(size_t)i * m + k] * a[(size_t)i * m + k];
+    double* col = a.data() + (size_t)k * rows;
+    const double c0 = col[k];
+    const Index nt = rows - k - 1;
+    const double tail_sq = nt > 0 ? packet_redux(nt, [&](Index i) { return col[k + 1 + i] * col[k + 1 + i]; }) : 0.0;
     double beta, t;
-    if (tail <= std::numeric_limits<double>::min()) {
+    if (tail_sq <= std::numeric_limits<double>::min()) {
       t = 0.0;
       beta = c0;
-      for (Index i = k + 1; i < rows; ++i) a[(size_t)i * m + k] = 0.0;
+      for (Index i = k + 1; i < rows; ++i) col[i] = 0.0;
     } else {
-      beta = std::sqrt(c0 * c0 + tail);
+      beta = std::sqrt(c0 * c0 + tail_sq);
       if (c0 >= 0.0) beta = -beta;
-      for (Index i = k + 1; i < rows; ++i) a[(size_t)i * m + k] /= (c0 - beta);
+      const double den = c0 - beta;
+      for (Index i = k + 1; i < rows; ++i) col[i] = col[i] / den;
       t = (beta - c0) / beta;
     }
     tau[(size_t)k] = t;
-    a[(size_t)k * m + k] = beta;
-    // apply H = I - t v v^T (v = [1; a(k+1:,k)]) to the trailing columns
-    for (Index j = k + 1; j < m; ++j) {
-      double dot = a[(size_t)k * m + j];
-      for (Index i = k + 1; i < rows; ++i) dot += a[(size_t)i * m + k] * a[(size_t)i * m + j];
-      dot *= t;
-      a[(size_t)k * m + j] -= dot;
-      for (Index i = k + 1; i < rows; ++i) a[(size_t)i * m + j] -= dot * a[(size_t)i * m + k];
-    }
+    col[k] = beta;
+    apply_householder_left(a.data() + (size_t)(k + 1) * rows + k, rows, rows - k, m - k - 1, col + k + 1, t, tmp, tv);
     for (Index j = k + 1; j < m; ++j) {
       if (norm_upd[(size_t)j] == 0.0) continue;
-      double temp = std::abs(a[(size_t)k * m + j]) / norm_upd[(size_t)j];
+      double temp = std::abs(at(k, j)) / norm_upd[(size_t)j];
       temp = (1.0 + temp) * (1.0 - temp);
       temp = temp < 0.0 ? 0.0 : temp;
       const double ratio = norm_upd[(size_t)j] / norm_dir[(size_t)j];
-      const double temp2 = temp * ratio * ratio;
+      const double temp2 = temp * (ratio * ratio);
       if (temp2 <= downdate_threshold) {
-        norm_dir[(size_t)j] = col_norm(j, k + 1);
+        norm_dir[(size_t)j] = norm_from(j, k + 1);
         norm_upd[(size_t)j] = norm_dir[(size_t)j];
       } else {
         norm_upd[(size_t)j] *= std::sqrt(temp);
@@ -753,29 +920,24 @@ Index cpqr_block(const std::vector<double>& block, Index rows, Index m, std::vec
     }
   }
   const Index kmax = std::min(rows, m);
-  const double lead = kmax > 0 ? std::abs(a[0]) : 0.0;
+  const double lead = kmax > 0 ? std::abs(at(0, 0)) : 0.0;
   Index rank = 0;
   for (Index i = 0; i < kmax; ++i)
-    if (std::abs(a[(size_t)i * m + i]) > 1.0e-10 * lead) ++rank;
+    if (std::abs(at(i, i)) > 1.0e-10 * lead) ++rank;
   rank = std::max<Index>(rank, 1);
-  // Q(:, 0:rank) = H_0 ... H_{rank-1} applied to the identity columns
+  // Q(:, 0:rank): the reflectors from the last to the first, each on rows k..rows-1
+  std::vector<double> qc((size_t)rows * rank, 0.0);  // column-major
+  for (Index c = 0; c < rank; ++c) qc[(size_t)c * rows + c] = 1.0;
+  for (Index k = size - 1; k >= 0; --k)
+    apply_householder_left(qc.data() + k, rows, rows - k, rank, a.data() + (size_t)k * rows + k + 1, tau[(size_t)k], tmp,
+                           tv);
   q.assign((size_t)rows * rank, 0.0);
-  for (Index c = 0; c < rank; ++c) {
-    std::vector<double> e((size_t)rows, 0.0);
-    e[(size_t)c] = 1.0;
-    for (Index k = std::min(c, size - 1); k >= 0; --k) {
-      double dot = e[(size_t)k];
-      for (Index i = k + 1; i < rows; ++i) dot += a[(size_t)i * m + k] * e[(size_t)i];
-      dot *= tau[(size_t)k];
-      e[(size_t)k] -= dot;
-      for (Index i = k + 1; i < rows; ++i) e[(size_t)i] -= dot * a[(size_t)i * m + k];
-    }
-    for (Index i = 0; i < rows; ++i) q[(size_t)i * rank + c] = e[(size_t)i];
-  }
+  for (Index i = 0; i < rows; ++i)
+    for (Index c = 0; c < rank; ++c) q[(size_t)i * rank + c] = qc[(size_t)c * rows + i];
   // R(0:rank, :) un-pivoted: column perm[j] of the result holds column j of R.
   r.assign((size_t)rank * m, 0.0);
   for (Index i = 0; i < rank; ++i)
-    for (Index j = i; j < m; ++j) r[(size_t)i * m + perm[(size_t)j]] = a[(size_t)i * m + j];
+    for (Index j = i; j < m; ++j) r[(size_t)i * m + perm[(size_t)j]] = at(i, j);
   return rank;
 }
 
@@ -1316,6 +1478,19 @@ void Amg::setup(const Csr& a, const Layout& layout) {
   stats_.operator_complexity = nnz_sum / double(stats_.levels.front().nonzeros);
   stats_.grid_complexity = rows_sum / double(stats_.levels.front().unknowns);
   stats_.reused_aggregates = reuse;
+  {
+    // Multigrid.cpp:1089-1100: coarse operators, P, R, A P, R A P and P-hat of
+    // every level, plus the dense coarsest matrix when it is factored densely
+    double stored = 0.0;
+    for (size_t k = 0; k < levels_.size(); ++k) {
+      const Level& lv = levels_[k];
+      if (k > 0) stored += double(lv.a.nnz());
+      stored += double(lv.p.nnz() + lv.r.nnz() + lv.ap.nnz() + lv.coarse_raw.nnz() + lv.p_tent.nnz());
+    }
+    const double nc = double(levels_.back().a.rows);
+    stored += levels_.back().a.rows <= 4000 ? nc * nc : 0.0;
+    stats_.storage_nonzeros = (long long)std::min(stored, 2.0e9);
+  }
 }
 
 void Amg::apply(const double* r, double* z) const {

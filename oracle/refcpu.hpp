@@ -128,6 +128,7 @@ struct AmgStats {  // Multigrid.hpp:126-150
   double operator_complexity = 0, grid_complexity = 0, coarse_pivot_ratio = 0;
   bool reused_aggregates = false;
   int near_null_space_dimension = 0;
+  long long storage_nonzeros = 0;  // Multigrid.cpp:1089-1100
 };
 
 double chunked_dot(const double* a, const double* b, Index n);  // Multigrid.cpp:74-95

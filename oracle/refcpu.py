@@ -81,6 +81,7 @@ def lib():
         "ref_amg_reused": (C.c_int, [v]),
         "ref_amg_operator_complexity": (C.c_double, [v]),
         "ref_amg_coarse_pivot_ratio": (C.c_double, [v]),
+        "ref_amg_storage_nonzeros": (C.c_longlong, [v]),
         "ref_amg_level_info": (None, [v, C.c_int, C.c_int, _ip]),
         "ref_amg_level_get": (None, [v, C.c_int, C.c_int, _ip, _ip, _dp]),
         "ref_amg_lambda": (C.c_double, [v, C.c_int]),
@@ -234,6 +235,9 @@ class Oracle:
 
     def amg_operator_complexity(self):
         return lib().ref_amg_operator_complexity(self.h)
+
+    def amg_storage_nonzeros(self):
+        return lib().ref_amg_storage_nonzeros(self.h)
 
     def amg_matrix(self, level, which):
         """which: 'A', 'P', 'Ptent', 'R' -> (rows, cols, ptr, col, val)."""

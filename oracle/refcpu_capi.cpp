@@ -176,6 +176,7 @@ int ref_amg_nlevels(void* h) { return (int)P(h)->amg.levels().size(); }
 int ref_amg_reused(void* h) { return P(h)->amg.stats().reused_aggregates ? 1 : 0; }
 double ref_amg_operator_complexity(void* h) { return P(h)->amg.stats().operator_complexity; }
 double ref_amg_coarse_pivot_ratio(void* h) { return P(h)->amg.stats().coarse_pivot_ratio; }
+long long ref_amg_storage_nonzeros(void* h) { return P(h)->amg.stats().storage_nonzeros; }
 // which: 0 = A_l, 1 = P, 2 = P-hat, 3 = R. Returns rows, cols, nnz through info.
 void ref_amg_level_info(void* h, int l, int which, int* info) {
   const Level& lv = P(h)->amg.levels()[(size_t)l];

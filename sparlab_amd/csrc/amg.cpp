@@ -225,22 +225,103 @@ void aggregate_nodes(const HCsr& a, const std::vector<int>& node_ptr, double the
   }
 }
 
-// Householder QR with column pivoting (Businger-Golub; LAPACK xGEQP3 norm
-// downdating; reflector beta = -sign(x0)|x|). a: rows x m row-major, in place.
-// Returns the rank at 1e-10 |R00| (Multigrid.cpp:606-617), Q's first rank
-// columns (rows x rank) and R (rank x m, columns in the original order).
+// Sums in the order of the SSE2 kernels (two-lane packets) of Eigen 3.4.0,
+// the library the reference's QR runs in: a vector reduction starts packet
+// accumulators on lanes {0,1} and {2,3}, adds {i,i+1} and {i+2,i+3} per step of
+// four, combines them, adds one more packet if two even entries remain, then
+// the two lanes, then the odd last entry.
+template <class F>
+double lane_sum(int n, F&& x) {
+  if (n <= 0) return 0.0;
+  const int n2 = n & ~1;
+  if (n2 == 0) return x(0);
+  double a0 = x(0), a1 = x(1);
+  if (n2 > 2) {
+    double b0 = x(2), b1 = x(3);
+    const int n4 = n & ~3;
+    for (int i = 4; i < n4; i += 4) {
+      a0 = a0 + x(i);
+      a1 = a1 + x(i + 1);
+      b0 = b0 + x(i + 2);
+      b1 = b1 + x(i + 3);
+    }
+    a0 = a0 + b0;
+    a1 = a1 + b1;
+    if (n2 > n4) {
+      a0 = a0 + x(n4);
+      a1 = a1 + x(n4 + 1);
+    }
+  }
+  double s = a0 + a1;
+  for (int i = n2; i < n; ++i) s = s + x(i);
+  return s;
+}
+
+// One output of Eigen's row-major matrix-vector kernel: even and odd products
+// accumulate in the two lanes, then the lanes are added, then the odd last
+// product.
+double lane_gemv(const double* a, const double* b, int n) {
+  double e = 0.0, o = 0.0;
+  int j = 0;
+  for (; j + 2 <= n; j += 2) {
+    e = a[j] * b[j] + e;
+    o = a[j + 1] * b[j + 1] + o;
+  }
+  double s = e + o;
+  for (; j < n; ++j) s = s + a[j] * b[j];
+  return 0.0 + s;
+}
+
+// H = I - tau [1; v][1; v]^T on a rows x cols column-major block (leading
+// dimension ld), as MatrixBase::applyHouseholderOnTheLeft forms it.
+void reflect_left(double* blk, int ld, int rows, int cols, const double* v, double tau, std::vector<double>& w,
+                  std::vector<double>& tv) {
+  if (rows == 1) {
+    const double f = 1.0 - tau;
+    for (int j = 0; j < cols; ++j) blk[(size_t)j * ld] *= f;
+    return;
+  }
+  if (tau == 0.0 || cols == 0) return;
+  const int nb = rows - 1;
+  w.assign((size_t)cols, 0.0);
+  if (cols == 1) {
+    const double* b = blk + 1;
+    w[0] = 0.0 + lane_sum(nb, [&](int i) { return v[i] * b[i]; });
+  } else {
+    for (int j = 0; j < cols; ++j) w[(size_t)j] = lane_gemv(blk + (size_t)j * ld + 1, v, nb);
+  }
+  for (int j = 0; j < cols; ++j) w[(size_t)j] = w[(size_t)j] + blk[(size_t)j * ld];
+  for (int j = 0; j < cols; ++j) blk[(size_t)j * ld] = blk[(size_t)j * ld] - tau * w[(size_t)j];
+  tv.resize((size_t)nb);
+  for (int i = 0; i < nb; ++i) tv[(size_t)i] = tau * v[i];
+  for (int j = 0; j < cols; ++j) {
+    double* c = blk + (size_t)j * ld + 1;
+    for (int i = 0; i < nb; ++i) c[i] = c[i] - w[(size_t)j] * tv[(size_t)i];
+  }
+}
+
+// Column-pivoted Householder QR of one aggregate's near-null-space rows (a:
+// rows x m, row-major), as Eigen::ColPivHouseholderQR (Multigrid.cpp:606-617):
+// Businger-Golub pivoting on the updated column norms (first largest wins),
+// LAPACK xGEQP3 norm downdating with direct recomputation below sqrt(eps),
+// reflectors beta = -sign(x0) sqrt(x0^2 + |tail|^2), v = tail / (x0 - beta),
+// tau = (beta - x0) / beta (none when |tail|^2 <= DBL_MIN), Q = the reflectors
+// applied last to first to I(rows, rank); rank at 1e-10 |R00|. Every sum runs
+// in the packet order above, so P-hat carries the reference's bits.
 int qr_colpiv(std::vector<double>& a, int rows, int m, std::vector<double>& q, std::vector<double>& r) {
   const int size = std::min(rows, m);
+  std::vector<double> c((size_t)rows * m);  // column-major working copy
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < m; ++j) c[(size_t)j * rows + i] = a[(size_t)i * m + j];
+  auto at = [&](int i, int j) -> double& { return c[(size_t)j * rows + i]; };
+  auto norm_from = [&](int j, int from) {
+    const double* col = c.data() + (size_t)j * rows + from;
+    return std::sqrt(lane_sum(rows - from, [&](int i) { return col[i] * col[i]; }));
+  };
   std::vector<int> perm((size_t)m);
   std::iota(perm.begin(), perm.end(), 0);
-  std::vector<double> tau((size_t)size, 0.0), nu((size_t)m), nd((size_t)m);
-  auto at = [&](int i, int j) -> double& { return a[(size_t)i * m + j]; };
-  auto tail_norm = [&](int j, int from) {
-    double s = 0.0;
-    for (int i = from; i < rows; ++i) s += at(i, j) * at(i, j);
-    return std::sqrt(s);
-  };
-  for (int j = 0; j < m; ++j) nu[(size_t)j] = nd[(size_t)j] = tail_norm(j, 0);
+  std::vector<double> tau((size_t)size, 0.0), nu((size_t)m), nd((size_t)m), w, tv;
+  for (int j = 0; j < m; ++j) nu[(size_t)j] = nd[(size_t)j] = norm_from(j, 0);
   const double thr = std::sqrt(std::numeric_limits<double>::epsilon());
   for (int k = 0; k < size; ++k) {
     int piv = k;
@@ -252,38 +333,33 @@ int qr_colpiv(std::vector<double>& a, int rows, int m, std::vector<double>& q, s
       std::swap(nd[(size_t)k], nd[(size_t)piv]);
       std::swap(perm[(size_t)k], perm[(size_t)piv]);
     }
-    const double x0 = at(k, k);
-    double tail = 0.0;
-    for (int i = k + 1; i < rows; ++i) tail += at(i, k) * at(i, k);
+    double* col = c.data() + (size_t)k * rows;
+    const double x0 = col[k];
+    const int nt = rows - k - 1;
+    const double tail = nt > 0 ? lane_sum(nt, [&](int i) { return col[k + 1 + i] * col[k + 1 + i]; }) : 0.0;
     double beta, t;
     if (tail <= std::numeric_limits<double>::min()) {
       t = 0.0;
       beta = x0;
-      for (int i = k + 1; i < rows; ++i) at(i, k) = 0.0;
+      for (int i = k + 1; i < rows; ++i) col[i] = 0.0;
     } else {
       beta = std::sqrt(x0 * x0 + tail);
       if (x0 >= 0.0) beta = -beta;
       const double den = x0 - beta;
-      for (int i = k + 1; i < rows; ++i) at(i, k) /= den;
+      for (int i = k + 1; i < rows; ++i) col[i] = col[i] / den;
       t = (beta - x0) / beta;
     }
     tau[(size_t)k] = t;
-    at(k, k) = beta;
-    for (int j = k + 1; j < m; ++j) {
-      double w = at(k, j);
-      for (int i = k + 1; i < rows; ++i) w += at(i, k) * at(i, j);
-      w *= t;
-      at(k, j) -= w;
-      for (int i = k + 1; i < rows; ++i) at(i, j) -= w * at(i, k);
-    }
+    col[k] = beta;
+    reflect_left(c.data() + (size_t)(k + 1) * rows + k, rows, rows - k, m - k - 1, col + k + 1, t, w, tv);
     for (int j = k + 1; j < m; ++j) {
       if (nu[(size_t)j] == 0.0) continue;
       double g = std::abs(at(k, j)) / nu[(size_t)j];
       g = (1.0 + g) * (1.0 - g);
       if (g < 0.0) g = 0.0;
       const double ratio = nu[(size_t)j] / nd[(size_t)j];
-      if (g * ratio * ratio <= thr) {
-        nd[(size_t)j] = tail_norm(j, k + 1);
+      if (g * (ratio * ratio) <= thr) {
+        nd[(size_t)j] = norm_from(j, k + 1);
         nu[(size_t)j] = nd[(size_t)j];
       } else {
         nu[(size_t)j] *= std::sqrt(g);
@@ -295,20 +371,13 @@ int qr_colpiv(std::vector<double>& a, int rows, int m, std::vector<double>& q, s
   for (int i = 0; i < size; ++i)
     if (std::abs(at(i, i)) > 1.0e-10 * lead) ++rank;
   rank = std::max(rank, 1);
+  std::vector<double> qc((size_t)rows * rank, 0.0);
+  for (int j = 0; j < rank; ++j) qc[(size_t)j * rows + j] = 1.0;
+  for (int k = size - 1; k >= 0; --k)
+    reflect_left(qc.data() + k, rows, rows - k, rank, c.data() + (size_t)k * rows + k + 1, tau[(size_t)k], w, tv);
   q.assign((size_t)rows * rank, 0.0);
-  std::vector<double> e((size_t)rows);
-  for (int c = 0; c < rank; ++c) {
-    std::fill(e.begin(), e.end(), 0.0);
-    e[(size_t)c] = 1.0;
-    for (int k = std::min(c, size - 1); k >= 0; --k) {
-      double w = e[(size_t)k];
-      for (int i = k + 1; i < rows; ++i) w += at(i, k) * e[(size_t)i];
-      w *= tau[(size_t)k];
-      e[(size_t)k] -= w;
-      for (int i = k + 1; i < rows; ++i) e[(size_t)i] -= w * at(i, k);
-    }
-    for (int i = 0; i < rows; ++i) q[(size_t)i * rank + c] = e[(size_t)i];
-  }
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < rank; ++j) q[(size_t)i * rank + j] = qc[(size_t)j * rows + i];
   r.assign((size_t)rank * m, 0.0);
   for (int i = 0; i < rank; ++i)
     for (int j = i; j < m; ++j) r[(size_t)i * m + perm[(size_t)j]] = at(i, j);
@@ -379,34 +448,89 @@ void tentative(Level& lvl, Level& next, int m, HCsr& pt) {
   }
 }
 
-double tridiag_max(const std::vector<double>& d, const std::vector<double>& e) {
-  // largest eigenvalue of a symmetric tridiagonal matrix by Sturm bisection
-  const size_t k = d.size();
-  if (k == 1) return d[0];
-  double lo = std::numeric_limits<double>::max(), hi = -lo;
-  for (size_t i = 0; i < k; ++i) {
-    const double rad = (i ? std::abs(e[i - 1]) : 0.0) + (i + 1 < k ? std::abs(e[i]) : 0.0);
-    lo = std::min(lo, d[i] - rad);
-    hi = std::max(hi, d[i] + rad);
-  }
-  const double tiny = std::numeric_limits<double>::epsilon() * (std::abs(lo) + std::abs(hi) + 1e-300);
-  auto below = [&](double x) {
-    int c = 0;
-    double piv = d[0] - x;
-    if (piv < 0) ++c;
-    for (size_t i = 1; i < k; ++i) {
-      if (piv == 0.0) piv = tiny;
-      piv = d[i] - x - e[i - 1] * e[i - 1] / piv;
-      if (piv < 0) ++c;
+// Largest eigenvalue of the symmetric tridiagonal Lanczos matrix, as the
+// reference takes it from Eigen 3.4.0's SelfAdjointEigenSolver::
+// computeFromTridiagonal (Multigrid.cpp:324-326): implicit symmetric QR steps
+// with the Wilkinson shift (Golub & Van Loan, Algorithm 8.3.2) on the trailing
+// unreduced block, an off-diagonal entry deflated when it is below DBL_MIN or
+// when (e_i / eps)^2 <= |d_i| + |d_(i+1)|, at most 30 k sweeps.
+double tridiag_max(const std::vector<double>& diag, const std::vector<double>& sub) {
+  const int k = (int)diag.size();
+  if (k == 1) return diag[0];
+  std::vector<double> d(diag), e(sub.begin(), sub.begin() + (k - 1));
+  auto rotation = [](double p, double q, double& c, double& s) {  // [c s; -s c]^T [p; q] = [r; 0]
+    if (q == 0.0) {
+      c = p < 0.0 ? -1.0 : 1.0;
+      s = 0.0;
+    } else if (p == 0.0) {
+      c = 0.0;
+      s = q < 0.0 ? 1.0 : -1.0;
+    } else if (std::abs(p) > std::abs(q)) {
+      const double t = q / p;
+      double u = std::sqrt(1.0 + t * t);
+      if (p < 0.0) u = -u;
+      c = 1.0 / u;
+      s = -t * c;
+    } else {
+      const double t = p / q;
+      double u = std::sqrt(1.0 + t * t);
+      if (q < 0.0) u = -u;
+      s = -1.0 / u;
+      c = -t * s;
     }
-    return c;
   };
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;
-    if (below(mid) >= (int)k) hi = mid; else lo = mid;
+  auto qr_step = [&](int start, int end) {
+    const double td = (d[(size_t)end - 1] - d[(size_t)end]) * 0.5;
+    const double f = e[(size_t)end - 1];
+    double mu = d[(size_t)end];
+    if (td == 0.0) {
+      mu -= std::abs(f);
+    } else if (f != 0.0) {
+      const double hi = std::max(std::abs(td), std::abs(f)), lo = std::min(std::abs(f), std::abs(td));
+      const double h = hi == 0.0 ? 0.0 : hi * std::sqrt(1.0 + (lo / hi) * (lo / hi));
+      if (f * f == 0.0)
+        mu -= f / ((td + (td > 0.0 ? h : -h)) / f);
+      else
+        mu -= (f * f) / (td + (td > 0.0 ? h : -h));
+    }
+    double x = d[(size_t)start] - mu, z = e[(size_t)start];
+    for (int i = start; i < end && z != 0.0; ++i) {
+      double c, s;
+      rotation(x, z, c, s);
+      const double sdk = s * d[(size_t)i] + c * e[(size_t)i];
+      const double dkp1 = s * e[(size_t)i] + c * d[(size_t)i + 1];
+      d[(size_t)i] = c * (c * d[(size_t)i] - s * e[(size_t)i]) - s * (c * e[(size_t)i] - s * d[(size_t)i + 1]);
+      d[(size_t)i + 1] = s * sdk + c * dkp1;
+      e[(size_t)i] = c * sdk - s * dkp1;
+      if (i > start) e[(size_t)i - 1] = c * e[(size_t)i - 1] - s * z;
+      x = e[(size_t)i];
+      if (i < end - 1) {
+        z = -s * e[(size_t)i + 1];
+        e[(size_t)i + 1] = c * e[(size_t)i + 1];
+      }
+    }
+  };
+  const double tiny = std::numeric_limits<double>::min();
+  const double inv_eps = 1.0 / std::numeric_limits<double>::epsilon();
+  int end = k - 1, start = 0;
+  long long sweeps = 0;
+  while (end > 0) {
+    for (int i = start; i < end; ++i) {
+      if (std::abs(e[(size_t)i]) < tiny) {
+        e[(size_t)i] = 0.0;
+      } else {
+        const double se = inv_eps * e[(size_t)i];
+        if (se * se <= std::abs(d[(size_t)i]) + std::abs(d[(size_t)i + 1])) e[(size_t)i] = 0.0;
+      }
+    }
+    while (end > 0 && e[(size_t)end - 1] == 0.0) --end;
+    if (end <= 0) break;
+    if (++sweeps > 30LL * k) break;
+    start = end - 1;
+    while (start > 0 && e[(size_t)start - 1] != 0.0) --start;
+    qr_step(start, end);
   }
-  return hi;
+  return *std::max_element(d.begin(), d.end());
 }
 
 // P-hat's aggregate blocks for the transfer kernels (k::TentBlocks): every row
@@ -1284,9 +1408,11 @@ void amg_setup(Context& c, bool defer_coarse) {
     if (!reuse) {
       if (l > 0) host_a = download_csr(lv.A, s);
       aggregate_nodes(host_a, lv.node_ptr, o.strength_threshold, lv.aggregate);
+      lap(2);
       Level next;
       HCsr pt;
       tentative(lv, next, c.nullspace_dim, pt);
+      lap(6);
       if (next.n >= (int)(0.85 * double(lv.n)) || next.n == 0) {
         lv.aggregate.clear();
         break;
@@ -1336,7 +1462,7 @@ void amg_setup(Context& c, bool defer_coarse) {
         }
       }
       prepare_node_products(lv, next, host_a, pt, ppat, rpat, appat, rawpat, xyz.empty() ? nullptr : &xyz, s);
-      lap(2);
+      lap(4);  // the symbolic products and their plans count as Galerkin time
       lv.has_next = true;
       SLX_HIP(hipStreamSynchronize(s));  // host pattern arrays go out of scope
       c.levels.push_back(std::move(next));

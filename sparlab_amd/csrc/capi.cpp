@@ -49,7 +49,15 @@ slx_status guarded(slx_context* ctx, F&& f) {
 }
 
 void require_model(const Context& c) {
+  if (c.k_source == Context::KSource::caller)
+    fail(SLX_MODEL_ERROR,
+         "the context holds a caller-assembled matrix (slx_factorize_csr); call slx_model_finalize to use the "
+         "structured model again");
   if (!c.finalized) fail(SLX_MODEL_ERROR, "the model is not finalised; call slx_model_finalize first");
+}
+
+void require_amg(const Context& c) {
+  if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "AMG-CG: solve called before factorize");
 }
 
 void sync(Context& c) { SLX_HIP(hipStreamSynchronize(c.stream)); }
@@ -179,6 +187,7 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   c.proj_eta = o.eta;
   k::simp((int)ne, c.d_tmp.p, o.penalty, o.emin_ratio, c.d_phys.p, c.d_fac.p, c.d_dfac.p, s);
   assemble(c, c.d_fac.p);
+  c.k_source = Context::KSource::evaluation;
   amg_setup(c, true);
   out.assemble_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
   const auto t1 = Clock::now();
@@ -525,6 +534,7 @@ slx_status slx_model_finalize(slx_context* ctx) {
     ensure_workspace(c);
     c.finalized = true;
     c.amg_ready = false;
+    c.k_source = Context::KSource::none;
     if (c.ncases > 0) {
       const size_t nf = c.free_dofs.size();
       std::vector<double> lf((size_t)c.ncases * nf);
@@ -582,6 +592,7 @@ slx_status slx_assemble(slx_context* ctx, const double* scale) {
     if (mn < 0.0) fail(SLX_MODEL_ERROR, "stiffness scale vector has a negative entry");
     c.d_scale.upload(h, ne, c.stream);
     assemble(c, c.d_scale.p);
+    c.k_source = Context::KSource::assembled;
     c.flush_timers();
     sync(c);
   });
@@ -652,6 +663,12 @@ slx_status slx_factorize_csr(slx_context* ctx, int n, const int* ptr, const int*
     const bool same_pattern = c.K.rows == n && c.K.nnz == h.nnz() && same_unknowns;
     if (same_pattern) c.K.upload_values(h.val.data(), c.stream); else c.K.upload(h, c.stream);
     c.mf = false;  // a caller's matrix: products use it as given
+    // the structured model's K_ff pattern, solutions and adjoint slots no longer
+    // describe K: the structured entry points refuse until slx_model_finalize
+    c.k_source = Context::KSource::caller;
+    c.finalized = false;
+    c.have_u.assign(c.have_u.size(), 0);
+    c.adjoint_slots.clear();
     c.layout_dim = dim;
     c.layout_ndpn = ndpn > 0 ? ndpn : dim;
     c.layout_nnodes = nnodes;
@@ -716,6 +733,9 @@ slx_status slx_solve_case(slx_context* ctx, int load_case, int warm_start, doubl
 
 slx_status slx_amg_time_smoother(slx_context* ctx, int level, int reps, double* ms_per_step) {
   return guarded(ctx, [&](Context& c) {
+    require_amg(c);
+    if (level < 0 || level >= (int)c.levels.size()) fail(SLX_CONFIG_ERROR, "no such multigrid level");
+    if (reps < 1) fail(SLX_CONFIG_ERROR, "the smoother timing needs at least one repetition");
     finish_coarse(c);
     *ms_per_step = time_smoother(c, level, reps);
   });
@@ -723,6 +743,7 @@ slx_status slx_amg_time_smoother(slx_context* ctx, int level, int reps, double* 
 
 slx_status slx_amg_apply(slx_context* ctx, const double* r, double* z) {
   return guarded(ctx, [&](Context& c) {
+    require_amg(c);
     const size_t n = (size_t)c.K.rows;
     c.bw.upload(r, n, c.stream);
     vcycle(c, c.bw.p, c.xw.p);
@@ -746,7 +767,21 @@ slx_status slx_amg_stats_get(const slx_context* ctx, slx_amg_stats* out) {
   out->smoothing_seconds = c->phase_seconds[3];
   out->galerkin_seconds = c->phase_seconds[4];
   out->coarse_seconds = c->phase_seconds[5];
+  out->tentative_seconds = c->phase_seconds[6];
   for (const Level& lv : c->levels) out->node_product_levels += lv.node_ok ? 1 : 0;
+  {
+    // Multigrid.cpp:1089-1100
+    double stored = 0.0;
+    for (size_t l = 0; l < c->levels.size(); ++l) {
+      const Level& lv = c->levels[l];
+      if (l > 0) stored += (double)lv.A.nnz;
+      if (!lv.has_next) continue;
+      stored += (double)(lv.P.nnz + lv.R.nnz + lv.AP.nnz + lv.RAW.nnz + lv.Ptent.nnz);
+    }
+    const double nc = (double)c->levels.back().n;
+    if (c->levels.back().n <= 4000) stored += nc * nc;
+    out->storage_nonzeros = (long long)std::min(stored, 2.0e9);
+  }
   double nnz_sum = 0, rows_sum = 0;
   for (size_t l = 0; l < c->levels.size() && l < 16; ++l) {
     const Level& lv = c->levels[l];
@@ -1112,7 +1147,11 @@ slx_status slx_solve_adjoint(slx_context* ctx, const double* rhs, int slot, doub
   return guarded(ctx, [&](Context& c) {
     // ComplianceObjective::solve_adjoint (src/topopt/Sensitivity.cpp:239-257) through
     // StaticAnalysis::solve_homogeneous: K_ff lambda_f = rhs_f, zero on the fixed DOFs
-    if (!c.amg_ready || c.K.rows == 0) fail(SLX_MODEL_ERROR, "solve_adjoint called before the objective was evaluated");
+    require_model(c);
+    if (!c.amg_ready || c.K.rows == 0 || c.k_source != Context::KSource::evaluation)
+      fail(SLX_MODEL_ERROR,
+           "solve_adjoint needs the operator of the last objective evaluation; evaluate the design again (the "
+           "context's K_ff was replaced by slx_assemble, a modal analysis or a caller matrix)");
     if (!(tol > 0.0)) fail(SLX_CONFIG_ERROR, "the adjoint tolerance must be positive");
     const size_t nf = c.free_dofs.size();
     std::vector<double> bf(nf);
@@ -1192,6 +1231,7 @@ slx_status slx_modal_solve(slx_context* ctx, const slx_modal_options* opt, const
     }
     c.d_scale.upload(h, ne, c.stream);
     assemble(c, c.d_scale.p);
+    c.k_source = Context::KSource::modal;
     amg_setup(c);
     DBuf<double> dm;
     if (mass_scale) dm.upload(mass_scale, ne, c.stream);

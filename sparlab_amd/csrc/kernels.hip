@@ -461,9 +461,8 @@ __global__ void __launch_bounds__(kBlock) k_sensitivity(int nelem, int nx, int n
                                                          double* __restrict__ energy, double* __restrict__ dc) {
   constexpr int NPE = DIM == 3 ? 8 : 4;
   constexpr int ND = NPE * DIM;
-  __shared__ double ke[ND * ND];
-  for (int t = threadIdx.x; t < ND * ND; t += blockDim.x) ke[t] = ke0[t];
-  __syncthreads();
+  // Ke0 is read with wave-uniform addresses (scalar loads); both loops below are
+  // unrolled so that the element vector stays in registers.
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
   const int nnx = nx + 1, nny = ny + 1;
@@ -491,10 +490,11 @@ __global__ void __launch_bounds__(kBlock) k_sensitivity(int nelem, int nx, int n
 #pragma unroll
     for (int i = 0; i < ND; ++i) ue[i] = idx[i] >= 0 ? ul[idx[i]] : 0.0;
     double quad = 0.0;
+#pragma unroll
     for (int i = 0; i < ND; ++i) {
       double s = 0.0;
 #pragma unroll
-      for (int j = 0; j < ND; ++j) s += ke[i * ND + j] * ue[j];
+      for (int j = 0; j < ND; ++j) s += ke0[i * ND + j] * ue[j];
       quad += ue[i] * s;
     }
     en += w[l] * 0.5 * fe * quad;

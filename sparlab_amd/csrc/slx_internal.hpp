@@ -197,6 +197,13 @@ struct Context {
   DBuf<double> d_u;                           // last solutions (ncases x nfree)
   std::vector<char> have_u;
   DCsr K;  // K_ff on the device
+  // what K holds: the structured pattern only (after slx_model_finalize), an
+  // assembly of caller factors (slx_assemble), the last objective evaluation,
+  // the modal analysis's stiffness, or a caller-assembled matrix
+  // (slx_factorize_csr, which leaves the structured model unusable until it is
+  // finalised again)
+  enum class KSource { none, assembled, evaluation, modal, caller };
+  KSource k_source = KSource::none;
   bool k_on_host = false;
   // matrix-free K_ff of a structured Hex8 model: the element factors of the last
   // assembly (FP64 and FP32) and Ke0 in FP32; used for level-0 products when mf
@@ -227,7 +234,7 @@ struct Context {
   double coarse_pivot_ratio = 0.0;
   double setup_seconds = 0.0;
   int tuned_precision = -1;  // precision the CSR kernels were last autotuned for
-  double phase_seconds[6] = {0, 0, 0, 0, 0, 0};  // check, spectral, aggregation, smoothing, Galerkin, coarse
+  double phase_seconds[7] = {0, 0, 0, 0, 0, 0, 0};  // check, spectral, aggregation, smoothing, Galerkin, coarse, tentative
 
   // ---- PCG workspace ------------------------------------------------------------
   DBuf<double> r, z, p, q, xw, bw, xpcg, zold;

@@ -67,7 +67,8 @@ class AmgStats(C.Structure):
                 ("nonzeros", C.c_longlong * 16), ("aggregates", C.c_longlong * 16), ("lambda_max", C.c_double * 16),
                 ("setup_seconds", C.c_double), ("check_seconds", C.c_double), ("spectral_seconds", C.c_double),
                 ("aggregation_seconds", C.c_double), ("smoothing_seconds", C.c_double),
-                ("galerkin_seconds", C.c_double), ("coarse_seconds", C.c_double), ("node_product_levels", C.c_int)]
+                ("galerkin_seconds", C.c_double), ("coarse_seconds", C.c_double), ("node_product_levels", C.c_int),
+                ("storage_nonzeros", C.c_longlong), ("tentative_seconds", C.c_double)]
 
 
 class EvalOptions(C.Structure):
@@ -447,9 +448,16 @@ class HipAmgCgSolver:
         return self._last.relative_residual
 
     def amg_stats(self):
+        """LinearSolver::amg_stats() (include/sparlab/fem/LinearSolver.hpp:100-104)."""
+        self._require_matrix(self._n if getattr(self, "_n", None) is not None else -1)
         s = AmgStats()
         self.ctx.check(load().slx_amg_stats_get(self.ctx.h, C.byref(s)))
         return s
+
+    def storage_nonzeros(self):
+        """LinearSolver::storage_nonzeros() of AmgCgSolver: AmgStats::storage_nonzeros
+        (src/fem/LinearSolver.cpp:258, src/fem/Multigrid.cpp:1089-1100)."""
+        return int(self.amg_stats().storage_nonzeros)
 
 
 class HipProblem:

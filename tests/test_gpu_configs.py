@@ -1,0 +1,173 @@
+"""Parity at the full sizes of the benchmark configurations, through the C ABI.
+
+- 3-D cantilever Hex8 128 x 64 x 64 (1 622 400 free DOFs), densities seeded
+  uniform in [0.001, 1], SIMP p 3, filter r 2 cells: the FP64 path at rtol 1e-12
+  against the oracle's evaluation (compliance, displacements, sensitivities),
+  then the mixed-precision hierarchies (FP32 V-cycle; FP32 with bfloat16 coarse
+  operators) at rtol 1e-8 against the same oracle evaluation, with the error of
+  each field stated.
+- The LinearSolver seam (slx_factorize_csr + slx_solve) on the oracle's
+  assembled K_ff of the sparlab_bench box, against the oracle's PCG and the
+  published compliance, iterations and stored non-zeros.
+- Every published sparlab_bench AMG row (Hex8 16..128 cells, Q4 40..1000 cells)
+  on the device: compliance, iterations, levels and operator complexity.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.refcpu import Oracle
+from sparlab_amd import model as M
+from sparlab_amd import native as N
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "published.json")))
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def c3():
+    print("c3: oracle evaluation of the seeded 128 x 64 x 64 cantilever at rtol 1e-12", flush=True)
+    m = M.cantilever_3d(128, 64, 64, volume_fraction=0.2, radius_cells=2.0)
+    x = M.seeded_density(m.nelem)
+    o = Oracle(m, tol=1e-12)
+    eo = o.evaluate(x, want_u=True)
+    del o
+    print(f"c3: oracle compliance {eo['compliance']!r}", flush=True)
+    return m, x, eo
+
+
+def test_c3_fp64_against_oracle(c3):
+    m, x, eo = c3
+    print("test_c3_fp64_against_oracle", flush=True)
+    ctx = N.Context(0)
+    g = N.HipProblem(m, ctx=ctx)
+    eg = g.evaluate(x, tol=1e-12, warm_start=False)
+    uo = eo["u"][0][g.free()]
+    ug = g.solution(0)
+    ec = abs(eg["compliance"] - eo["compliance"]) / eo["compliance"]
+    eu = _rel(ug, uo)
+    edc = float(np.abs(eg["dc_dx"] - eo["dc_dx"]).max() / np.abs(eo["dc_dx"]).max())
+    print(f"c3 fp64: c rel {ec:.2e}, u rel {eu:.2e}, dc max rel {edc:.2e}, {eg['iterations']} CG iterations",
+          flush=True)
+    # tolerances: the oracle's and the device's sums differ in order (Lanczos dots, matrix-free products)
+    assert ec <= 1e-9 and eu <= 1e-9 and edc <= 1e-8
+    assert eg["volume"] == pytest.approx(eo["volume"], rel=1e-12)
+    st = g.amg_stats()
+    assert st.levels >= 3 and st.unknowns[0] == 1622400 and st.nonzeros[0] == 128062062
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["fp32_cycle", "bf16_coarse"])
+def test_c3_mixed_precision_against_oracle(c3, precision):
+    """The mixed hierarchies under the FP64 outer (flexible) CG at rtol 1e-8: the
+    compliance within 1e-8 of the FP64 oracle; displacement and sensitivity errors
+    stated (bounded by the solve tolerance)."""
+    m, x, eo = c3
+    print(f"test_c3_mixed_precision_against_oracle[{precision}]", flush=True)
+    ctx = N.Context(0)
+    g = N.HipProblem(m, ctx=ctx)
+    g.assemble(1e-9 + (1 - 1e-9) * np.clip(g.filter(x), 0, 1) ** 3)
+    g.amg_setup(precision=precision)
+    eg = g.evaluate(x, tol=1e-8, warm_start=False)
+    uo = eo["u"][0][g.free()]
+    ec = abs(eg["compliance"] - eo["compliance"]) / eo["compliance"]
+    eu = _rel(g.solution(0), uo)
+    edc = float(np.abs(eg["dc_dx"] - eo["dc_dx"]).max() / np.abs(eo["dc_dx"]).max())
+    print(f"c3 precision {precision}: c rel {ec:.2e}, u rel {eu:.2e}, dc max rel {edc:.2e}, "
+          f"{eg['iterations']} CG iterations", flush=True)
+    assert ec <= 1e-8
+    assert eu <= 1e-5 and edc <= 1e-5
+    ctx.close()
+
+
+@pytest.mark.parametrize("nx", [16, 48])
+def test_caller_csr_seam_against_oracle_and_published(nx):
+    """AmgCgSolver::factorize + solve_from (src/fem/LinearSolver.cpp:229-281) on a
+    caller-assembled K_ff with its DOF layout."""
+    print(f"test_caller_csr_seam_against_oracle_and_published[{nx}]", flush=True)
+    row = next(r for r in GOLDEN["hex8_bench_amg"] if r["nx"] == nx)
+    m = M.bench_box(nx)
+    o = Oracle(m, tol=1e-10)
+    ptr, col, val = o.assemble_kff()
+    o.amg_setup()
+    b = m.free_loads()[0]
+    xo, ito, _, cvo = o.pcg(b, tol=1e-10)
+    s = N.HipAmgCgSolver(tolerance=1e-10, ctx=N.Context(0))
+    s.set_layout(3, m.node_coords(), m.free_dofs(), dofs_per_node=3)
+    s.factorize(ptr, col, val)
+    xg = s.solve(b)
+    c = float(b @ xg)
+    print(f"seam nx {nx}: c {c!r} (published {row['compliance']!r}), iterations {s.last_iterations()} "
+          f"(oracle {ito}, published {row['iterations']}), storage {s.storage_nonzeros()} "
+          f"(oracle {o.amg_storage_nonzeros()}, published {row['solver_nonzeros']})", flush=True)
+    assert cvo and s.last_error() <= 1e-10
+    assert abs(c - row["compliance"]) <= 1e-11 * row["compliance"]
+    assert _rel(xg, xo) <= 1e-9
+    assert abs(s.last_iterations() - ito) <= 1 and abs(s.last_iterations() - row["iterations"]) <= 1
+    st = s.amg_stats()
+    assert st.levels == row["levels"] == o.amg_levels()
+    assert st.operator_complexity == pytest.approx(o.amg_operator_complexity(), rel=1e-15)
+    assert s.storage_nonzeros() == o.amg_storage_nonzeros()
+    s.ctx.close()
+
+
+def _published_rows():
+    out = [("Hex8", r) for r in GOLDEN["hex8_bench_amg"] if r["nx"] >= 16]
+    out += [("Q4", r) for r in GOLDEN["q4_bench_amg"] if r["nx"] >= 40]
+    return out
+
+
+@pytest.mark.parametrize("kind,row", _published_rows(), ids=lambda v: v if isinstance(v, str) else f"nx{v['nx']}")
+def test_bench_box_rows_on_the_device(kind, row):
+    """sparlab_bench --solver amg_cg rows on the device (tolerance 1e-10). The
+    level-0 aggregates come from the bitwise K_ff, so they are the reference's;
+    coarser levels see the device's rounding of the coarse operators."""
+    print(f"test_bench_box_rows_on_the_device[{kind} {row['nx']}]", flush=True)
+    m = M.bench_box(row["nx"], dim=3 if kind == "Hex8" else 2)
+    ctx = N.Context(0)
+    g = N.HipProblem(m, ctx=ctx)
+    g.assemble()
+    g.amg_setup()
+    st = g.amg_stats()
+    b = m.free_loads()[0]
+    x, it, rr, cv = g.solve(b, tol=1e-10)
+    c = float(b @ x)
+    print(f"{kind} {row['nx']}: levels {st.levels} ({row['levels']}), complexity {st.operator_complexity!r} "
+          f"({row['operator_complexity']!r}), storage {st.storage_nonzeros} ({row['solver_nonzeros']}), "
+          f"iterations {it} ({row['iterations']}), c rel {abs(c - row['compliance']) / row['compliance']:.2e}",
+          flush=True)
+    assert cv and rr <= 1e-10
+    assert abs(c - row["compliance"]) <= 1e-11 * row["compliance"]
+    assert st.levels == row["levels"]
+    assert abs(it - row["iterations"]) <= 1
+    assert st.operator_complexity == pytest.approx(row["operator_complexity"], rel=1e-4)
+    assert st.storage_nonzeros == pytest.approx(row["solver_nonzeros"], rel=1e-3)
+    ctx.close()
+
+
+def test_level0_aggregates_and_tentative_prolongator_bitwise_at_128_cells():
+    """At the largest published Hex8 box (830 115 DOFs) the device's level-0
+    aggregates and tentative prolongator equal the oracle's bit for bit."""
+    print("test_level0_aggregates_and_tentative_prolongator_bitwise_at_128_cells", flush=True)
+    m = M.bench_box(128)
+    o = Oracle(m, tol=1e-10)
+    o.assemble_kff()
+    o.amg_setup()
+    ctx = N.Context(0)
+    g = N.HipProblem(m, ctx=ctx)
+    g.assemble()
+    g.amg_setup()
+    ao = o.amg_aggregates(0)
+    assert np.array_equal(g.amg_aggregates(0, len(ao)), ao)
+    gr, gc, gp, gcol, gv = g.amg_matrix(0, "Ptent")
+    orr, oc, op_, ocol, ov = o.amg_matrix(0, "Ptent")
+    assert (gr, gc) == (orr, oc)
+    assert np.array_equal(gp, op_) and np.array_equal(gcol, ocol) and np.array_equal(gv, ov)
+    assert g.amg_stats().lambda_max[0] == pytest.approx(o.amg_lambda(0), rel=1e-12)
+    ctx.close()

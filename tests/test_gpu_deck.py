@@ -60,7 +60,7 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
     print(f"deck {case}: {r['iterations']} iterations, {r['stop_reason']}, compliance {r['compliance_J']!r} "
           f"(published {pub['compliance_J']!r}), {r['total_seconds']:.2f} s", flush=True)
     assert r["stop_reason"] == pub["stop_reason"]
-    assert abs(r["iterations"] - pub["iterations"]) <= max(2, pub["iterations"] // 50)
+    assert r["iterations"] == pub["iterations"]
     assert r["compliance_J"] == pytest.approx(pub["compliance_J"], rel=1e-8)
     assert r["volume_fraction"] == pytest.approx(pub["volume_fraction_achieved"], abs=1e-8)
     assert r["grey_level"] == pytest.approx(pub["grey_level"], rel=1e-5)
