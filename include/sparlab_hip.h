@@ -303,6 +303,14 @@ slx_status slx_kff_get(const slx_context* ctx, int* ptr, int* col, double* val);
  * (optional) reports whether it ran matrix-free (structured Hex8 after slx_assemble,
  * unless SPARLAB_NO_MATRIX_FREE=1 was set at assembly) or on the assembled CSR. */
 slx_status slx_kff_apply(slx_context* ctx, const double* x, double* y, int* matrix_free);
+/* slx_kff_apply without any device allocation, and x.K_ff x into *xkx (optional).
+ * No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, hosts that forbid
+ * hipMalloc). x and y are staged through one file-scope device array per GPU
+ * (35.7 MB, held until the process ends): at most 2 097 152 unknowns; larger
+ * systems return SLX_CONFIG_ERROR. Every call on a GPU shares that array, so calls
+ * that overlap on one GPU (from several threads or contexts) race. */
+slx_status slx_kff_apply_noalloc(slx_context* ctx, const double* x, double* y, double* xkx, int* matrix_free);
 
 /* ---- linear solver ---------------------------------------------------------- */
 slx_status slx_amg_setup(slx_context* ctx, const slx_amg_options* opt);

@@ -374,6 +374,7 @@ slx_status slx_create(int device, slx_context** out) {
   c->device = device;
   try {
     SLX_HIP(hipSetDevice(device));
+    (void)noalloc_scratch(device);  // the scratch array's address on this device, once per device
     SLX_HIP(hipStreamCreate(&c->stream));
     slx_amg_default_options(&c->amg_opt);  // evaluate and run_oc use these until slx_amg_setup
     c->scal.alloc(16);

@@ -22,6 +22,9 @@ struct Error : std::exception {
   const char* what() const noexcept override { return msg.c_str(); }
 };
 [[noreturn]] void fail(int code, const std::string& msg);
+// device address of the no-allocation scratch array on `device` (noalloc.hip),
+// taken once per device with that device current
+double* noalloc_scratch(int device);
 void hip_check(hipError_t e, const char* what);
 #define SLX_HIP(expr) ::slx::hip_check((expr), #expr)
 

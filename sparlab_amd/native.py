@@ -180,6 +180,7 @@ _SIGNATURES = {
     "slx_assemble": (C.c_int, [_VP, _VP]),
     "slx_kff_get": (C.c_int, [_VP, _VP, _VP, _VP]),
     "slx_kff_apply": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_int)]),
+    "slx_kff_apply_noalloc": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "slx_amg_setup": (C.c_int, [_VP, C.POINTER(AmgOptions)]),
     "slx_factorize_csr": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP, _VP,
                                     C.POINTER(AmgOptions)]),
@@ -527,6 +528,16 @@ class HipProblem:
         mf = C.c_int()
         self.ctx.check(load().slx_kff_apply(self.ctx.h, _p(_f64(x)), _p(y), C.byref(mf)))
         return y, bool(mf.value)
+
+    def kff_apply_noalloc(self, x):
+        """y = K_ff x and x.K_ff x through the entry point that allocates no device
+        memory; returns (y, xKx, matrix_free)."""
+        self._own()
+        y = np.empty(self.nfree)
+        q = C.c_double()
+        mf = C.c_int()
+        self.ctx.check(load().slx_kff_apply_noalloc(self.ctx.h, _p(_f64(x)), _p(y), C.byref(q), C.byref(mf)))
+        return y, q.value, bool(mf.value)
 
     def amg_setup(self, **amg):
         self._own()
