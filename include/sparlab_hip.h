@@ -79,6 +79,15 @@ typedef struct {
   double coarse_pivot_tolerance; /* 1e-13 */
   int reuse_aggregates;          /* 1 */
   int precision;                 /* 0 */
+  /* Reuse of the coarse operators across objective evaluations (no counterpart in
+   * the reference, which recomputes every level per evaluation; default 0 = that
+   * behaviour). k > 0, structured Hex8 with the matrix-free operator only: after a
+   * full numeric setup, up to k evaluations refresh level 0's diagonal and spectral
+   * bound from the new element factors and keep A_1 and below, skipping the K_ff
+   * assembly and the Galerkin products; a full setup follows as soon as an
+   * evaluation's PCG iterations exceed those after the last full setup by 20 %
+   * (and at least 2). PCG still runs to the same tolerance. */
+  int coarse_reuse;              /* 0 */
 } slx_amg_options;
 
 typedef struct {

@@ -4,6 +4,7 @@
 #include "refcpu.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -1783,12 +1784,18 @@ double Projection::beta_of_stage(int k) const {
 Evaluation evaluate(Problem& p, const std::vector<double>& x, bool need_gradients) {
   const Index ne = p.mesh.ne;  // Sensitivity.cpp:110-237
   if ((Index)x.size() != ne) fail(1, "design vector has the wrong length");
+  using clock = std::chrono::steady_clock;
+  auto secs = [](clock::time_point a, clock::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+  const auto t_start = clock::now();
+  p.seconds_pcg = 0.0;
   Evaluation out;
   const std::vector<double> filtered = filtered_density(p, x);
   out.physical = physical_density(p, x);
   out.factors.resize((size_t)ne);
   for (Index e = 0; e < ne; ++e) out.factors[(size_t)e] = simp_factor(out.physical[(size_t)e], p.simp);
+  const auto t_asm = clock::now();
   p.kff = p.assemble_kff(out.factors.data());
+  p.seconds_assemble = secs(t_asm, clock::now());
   Layout layout;
   layout.dim = p.mesh.dim;
   layout.ndpn = p.ndpn;
@@ -1796,7 +1803,9 @@ Evaluation evaluate(Problem& p, const std::vector<double>& x, bool need_gradient
   layout.nn = p.mesh.nn;
   layout.unknowns = p.dofs.free.data();
   layout.n = (Index)p.dofs.free.size();
+  const auto t_setup = clock::now();
   p.amg.setup(p.kff, layout);
+  p.seconds_setup = secs(t_setup, clock::now());
   p.previous.resize(p.loads.size());
   const int npe = p.mesh.npe, nd = npe * p.ndpn;
   out.energy.assign((size_t)ne, 0.0);
@@ -1809,7 +1818,9 @@ Evaluation evaluate(Problem& p, const std::vector<double>& x, bool need_gradient
     std::vector<double>& prev = p.previous[l];
     if (p.warm_start && prev.size() == p.loads[l].size())
       for (Index i = 0; i < nf; ++i) xf[(size_t)i] = prev[(size_t)p.dofs.free[(size_t)i]];
+    const auto t_pcg = clock::now();
     const PcgResult res = pcg_solve(p.kff, b.data(), xf.data(), p.amg, p.tolerance, p.max_iterations);
+    p.seconds_pcg += secs(t_pcg, clock::now());
     if (!res.converged) {
       std::ostringstream os;
       os << "AMG-CG did not converge: " << res.iterations << " iterations left a relative residual of "
@@ -1862,6 +1873,7 @@ Evaluation evaluate(Problem& p, const std::vector<double>& x, bool need_gradient
     }
   }
   if (!std::isfinite(out.compliance)) fail(2, "the compliance objective evaluated to a non-finite value");
+  p.seconds_total = secs(t_start, clock::now());
   return out;
 }
 

@@ -277,6 +277,8 @@ struct Problem {
   Csr kff;
   std::vector<std::vector<double>> previous;
   long total_iterations = 0;
+  // wall seconds of the last evaluate(): K_ff assembly, AMG setup, PCG, and the whole call
+  double seconds_assemble = 0, seconds_setup = 0, seconds_pcg = 0, seconds_total = 0;
 
   void finalize();  // DOFs, Ke0, pattern, reduction, volumes
   Csr assemble_kff(const double* scale) const;

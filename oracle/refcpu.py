@@ -82,6 +82,8 @@ def lib():
         "ref_amg_operator_complexity": (C.c_double, [v]),
         "ref_amg_coarse_pivot_ratio": (C.c_double, [v]),
         "ref_amg_storage_nonzeros": (C.c_longlong, [v]),
+        "ref_last_evaluate_seconds": (None, [v, _dp]),
+        "ref_set_num_threads": (None, [C.c_int]),
         "ref_amg_level_info": (None, [v, C.c_int, C.c_int, _ip]),
         "ref_amg_level_get": (None, [v, C.c_int, C.c_int, _ip, _ip, _dp]),
         "ref_amg_lambda": (C.c_double, [v, C.c_int]),
@@ -317,6 +319,12 @@ class Oracle:
             out["u"] = u.reshape(ncases, self.ndofs)
         return out
 
+    def last_evaluate_seconds(self):
+        """Wall seconds of the last evaluate(): assembly, AMG setup, PCG, the whole call."""
+        t = np.empty(4)
+        lib().ref_last_evaluate_seconds(self.h, t)
+        return dict(assemble=t[0], setup=t[1], pcg=t[2], total=t[3])
+
     def oc(self, x, dc, dv, move=0.2, damping=0.5):
         xn = np.empty(self.ne)
         sc = np.empty(4)
@@ -377,6 +385,10 @@ def cpqr(block):
 def tridiag_max(diag, sub):
     return lib().ref_tridiag_max(np.ascontiguousarray(diag, dtype=np.float64),
                                  np.ascontiguousarray(sub if len(sub) else [0.0], dtype=np.float64), len(diag))
+
+
+def set_num_threads(n):
+    lib().ref_set_num_threads(int(n))
 
 
 def num_threads():

@@ -376,6 +376,20 @@ int ref_cpqr(const double* block, int rows, int m, double* q, double* r) {
 double ref_tridiag_max(const double* diag, const double* sub, int k) {
   return tridiagonal_max_eigenvalue(std::vector<double>(diag, diag + k), std::vector<double>(sub, sub + (k > 0 ? k - 1 : 0)));
 }
+// wall seconds of the last ref_evaluate: assembly, AMG setup, PCG, whole call
+void ref_last_evaluate_seconds(void* h, double* out) {
+  out[0] = P(h)->seconds_assemble;
+  out[1] = P(h)->seconds_setup;
+  out[2] = P(h)->seconds_pcg;
+  out[3] = P(h)->seconds_total;
+}
+void ref_set_num_threads(int n) {
+#if defined(_OPENMP)
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
 int ref_num_threads() {
 #if defined(_OPENMP)
   return omp_get_max_threads();

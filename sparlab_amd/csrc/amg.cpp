@@ -25,6 +25,7 @@
 #include <sstream>
 
 #include "slx_kernels.hpp"
+#include "slx_mf.hpp"
 #include "slx_solver.hpp"
 
 namespace slx {
@@ -1323,10 +1324,60 @@ void finish_coarse(Context& c) {
   }
 }
 
+// Level 0's diagonal and spectral bound from the current element factors, the
+// coarse levels of the last full numeric setup kept (coarse_reuse of the mixed
+// path: A_1 and below are those of an earlier design). The level-0 smoother and
+// transfers use the current operator: P = (I - omega D^-1 K) P-hat is applied
+// through it, with D and omega refreshed here. Matrix-free Hex8 models only.
+void amg_refresh_level0(Context& c) {
+  const auto t0 = Clock::now();
+  if (!c.amg_ready || !c.mf || c.levels.empty()) fail(SLX_SOLVER_ERROR, "multigrid: no hierarchy to refresh");
+  invalidate_graphs(c);  // the Chebyshev and transfer coefficients are kernel arguments
+  hipStream_t s = c.stream;
+  Level& lv = c.levels[0];
+  int* bad = c.setup_flags.p + 1;
+  constexpr int kNoBadRow = 0x7F7F7F7F;
+  SLX_HIP(hipMemsetAsync(bad, 0x7F, sizeof(int), s));
+  kmf::inv_diag(c.nx + 1, c.ny + 1, c.nz + 1, c.d_reduced.p, c.mf_scale.p, c.d_ke0.p, lv.dinv.p, bad, s);
+  k::dot(c.nelem, c.mf_scale.p, c.mf_scale.p, c.partials.p, c.scal.p + 9, s);
+  const int m = lanczos_queue(c, 0);
+  std::vector<double> h((size_t)(2 * m + 1));
+  lv.lz_scal.download(h.data(), h.size(), s);
+  int b = kNoBadRow;
+  double sumsq = 0.0;
+  SLX_HIP(hipMemcpyAsync(&b, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  SLX_HIP(hipMemcpyAsync(&sumsq, c.scal.p + 9, sizeof(double), hipMemcpyDeviceToHost, s));
+  SLX_HIP(hipStreamSynchronize(s));
+  if (!std::isfinite(sumsq)) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix has a non-finite entry");
+  if (b != kNoBadRow) {
+    std::ostringstream os;
+    os << "multigrid level 0: diagonal entry " << b
+       << " is not positive; a stiffness matrix must have a positive diagonal. The model has an unconnected degree "
+          "of freedom or a non-positive stiffness";
+    fail(SLX_SOLVER_ERROR, os.str());
+  }
+  lv.lambda_max = lanczos_lambda(h, m);
+  if (!(lv.lambda_max > 0.0) || !std::isfinite(lv.lambda_max))
+    fail(SLX_SOLVER_ERROR, "multigrid level 0: the spectral estimate of D^-1 A is not positive and finite");
+  lv.omega = c.amg_opt.prolongator_damping / lv.lambda_max;
+  if (c.amg_opt.precision >= 1) k::to_float(lv.n, lv.dinv.p, lv.dinv32.p, s);
+  for (double& ph : c.phase_seconds) ph = 0.0;
+  c.setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+  c.phase_seconds[1] = c.setup_seconds;
+  c.reused = true;
+}
+
 void amg_setup(Context& c, bool defer_coarse) {
   const auto t0 = Clock::now();
   hipStream_t s = c.stream;
   const int n = c.K.rows;
+  if (c.k_stale) {  // the last evaluation refreshed level 0 only: assemble its factors now
+    k::assemble_structured(c.dim, c.K, c.d_free.p, c.mf_scale.p, c.d_ke0.p, c.nx, c.ny, c.nz, s);
+    c.k_stale = false;
+  }
+  // coarse_reuse restarts from this setup: the next evaluation sets its base count
+  c.coarse_lag = 0;
+  c.lag_base_its = 0;
   c.coarse_pending = false;  // a previous setup's inverse nobody used
   invalidate_graphs(c);
   for (double& ph : c.phase_seconds) ph = 0.0;

@@ -124,10 +124,13 @@ void ensure_topopt_buffers(Context& c) {
   if (!c.scal.p) c.scal.alloc(16);
 }
 
-void assemble(Context& c, const double* d_scale) {
-  {
+// values = false keeps K_ff's values (they then lag the factors: k_stale) and
+// refreshes only the matrix-free operator's factors
+void assemble(Context& c, const double* d_scale, bool values = true) {
+  if (values) {
     Timed t(c, "assemble");
     k::assemble_structured(c.dim, c.K, c.d_free.p, d_scale, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream);
+    c.k_stale = false;
   }
   // the same factors drive the matrix-free products of Hex8 models
   c.mf = c.dim == 3 && matrix_free_enabled();
@@ -186,19 +189,40 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   c.proj_beta = proj ? o.beta : 0.0;
   c.proj_eta = o.eta;
   k::simp((int)ne, c.d_tmp.p, o.penalty, o.emin_ratio, c.d_phys.p, c.d_fac.p, c.d_dfac.p, s);
-  assemble(c, c.d_fac.p);
+  // coarse_reuse: keep the coarse operators of the last full setup while the
+  // iteration count stays within 20 % (and 2) of the count right after it
+  const int reuse = c.amg_opt.coarse_reuse;
+  const bool lag = reuse > 0 && c.dim == 3 && matrix_free_enabled() && c.mf && c.amg_ready &&
+                   c.k_source == Context::KSource::evaluation && c.levels.size() > 1 && c.coarse_lag < reuse &&
+                   c.lag_base_its > 0 && c.lag_last_its <= c.lag_base_its + std::max(2, c.lag_base_its / 5);
+  if (lag) {
+    assemble(c, c.d_fac.p, false);
+    c.k_stale = true;
+    amg_refresh_level0(c);
+  } else {
+    assemble(c, c.d_fac.p);
+    amg_setup(c, true);
+  }
   c.k_source = Context::KSource::evaluation;
-  amg_setup(c, true);
   out.assemble_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
   const auto t1 = Clock::now();
   std::vector<double> comp((size_t)c.ncases);
+  int most_its = 0;
   for (int l = 0; l < c.ncases; ++l) {
     if (o.warm_start) warm_before_solve(c, l);
     const slx_pcg_result r = solve_case(c, l, o.warm_start != 0, o.tolerance, o.max_iterations);
     require_converged(c, r, o.tolerance);
     if (o.warm_start) warm_after_solve(c, l);
     out.linear_iterations += r.iterations;
+    most_its = std::max(most_its, r.iterations);
     comp[(size_t)l] = device_dot(c, (long long)nf, c.d_loads_free.p + (size_t)l * nf, c.d_u.p + (size_t)l * nf);
+  }
+  if (lag) {
+    ++c.coarse_lag;
+    c.lag_last_its = most_its;
+  } else {
+    c.coarse_lag = 0;
+    c.lag_base_its = c.lag_last_its = most_its;
   }
   for (int l = 0; l < c.ncases; ++l) out.compliance += c.weights[(size_t)l] * comp[(size_t)l];
   c.case_compliance = comp;
@@ -422,6 +446,7 @@ void slx_amg_default_options(slx_amg_options* o) {
   o->coarse_pivot_tolerance = 1.0e-13;
   o->reuse_aggregates = 1;
   o->precision = 0;
+  o->coarse_reuse = 0;
 }
 
 void slx_eval_default_options(slx_eval_options* o) {
@@ -616,6 +641,10 @@ slx_status slx_kff_apply(slx_context* ctx, const double* x, double* y, int* matr
 slx_status slx_kff_get(const slx_context* ctx, int* ptr, int* col, double* val) {
   return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
     require_model(c);
+    if (c.k_stale) {  // values of the factors the operator uses
+      k::assemble_structured(c.dim, c.K, c.d_free.p, c.mf_scale.p, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream);
+      c.k_stale = false;
+    }
     if (ptr) c.K.ptr.download(ptr, (size_t)c.K.rows + 1, c.stream);
     if (col) c.K.col.download(col, (size_t)c.K.nnz, c.stream);
     if (val) c.K.val.download(val, (size_t)c.K.nnz, c.stream);
@@ -636,6 +665,7 @@ slx_status slx_amg_setup(slx_context* ctx, const slx_amg_options* opt) {
     if (!(o.chebyshev_ratio > 1.0)) fail(SLX_CONFIG_ERROR, "multigrid chebyshev_ratio must exceed 1");
     if (!(o.prolongator_damping > 0.0 && o.prolongator_damping < 2.0)) fail(SLX_CONFIG_ERROR, "multigrid prolongator_damping must lie in (0, 2)");
     if (o.lanczos_steps < 2) fail(SLX_CONFIG_ERROR, "multigrid lanczos_steps must be at least 2");
+    if (o.coarse_reuse < 0) fail(SLX_CONFIG_ERROR, "multigrid coarse_reuse must be 0 or positive");
     if (o.precision < 0 || o.precision > 2)
       fail(SLX_CONFIG_ERROR, "multigrid precision must be 0 (FP64), 1 (FP32 cycle) or 2 (FP32 cycle, bfloat16 coarse operators)");
     if (c.amg_ready && (o.precision != c.amg_opt.precision || o.max_levels != c.amg_opt.max_levels ||

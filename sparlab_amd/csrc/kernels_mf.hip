@@ -14,6 +14,8 @@
 // read as zero and are never written.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "slx_mf.hpp"
 
 namespace slx {
@@ -60,12 +62,27 @@ struct Vec4<double> {
   using type = double4;
 };
 
-template <typename T, int MODE>
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kElemPad = 416;  // the tile's 405 elements in 26 column tiles of 16
+
+// MX (FP32 only): the element products Y_e = Ke0 u_e of the tile's elements run on
+// the matrix cores as Y = Ke0 U (v_mfma_f32_16x16x32_bf16, FP32 accumulation),
+// each operand split into bfloat16 high and low parts (Ke0_h U_h + Ke0_h U_l +
+// Ke0_l U_h: about 16 significant bits, the error of the dropped low x low term);
+// the scaled products go to LDS by (element row, element) and each node sums its
+// 8 elements' rows. Lane layout of the instruction: the first operand's lane
+// (m, g) = (lane % 16, lane / 16) holds row m, columns 8g..8g+7; the second
+// operand's holds rows 8g..8g+7 of column m; the result's lane holds rows
+// 4g..4g+3 of column m.
+template <typename T, int MODE, bool MX>
 __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
+  static_assert(!MX || sizeof(T) == 4, "the matrix-core path is FP32");
   using V4 = typename Vec4<T>::type;
   __shared__ V4 s_u[HZ * HY * HX];
   __shared__ T s_s[EZ * EY * EX];
   __shared__ double s_red[kThreads / 64];
+  __shared__ float s_y[MX ? 24 * kElemPad : 1];
   const int ntx = (a.X + TX - 1) / TX, nty = (a.Y + TY - 1) / TY, ntz = (a.Z + TZ - 1) / TZ;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   if (tile >= ntx * nty * ntz) {
@@ -158,7 +175,68 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
   }
   __syncthreads();
   T acc0 = 0, acc1 = 0, acc2 = 0;
-  if (inside) {
+  if constexpr (MX) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    // Ke0 rows mt * 16 + m, columns 8g..8g+7 (zero past 24), high and low parts
+    bf16x8 ah[2], al[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const int row = mt * 16 + m;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = 8 * g + q;
+        const bool in = row < 24 && k < 24;
+        const float v = in ? (float)a.ke[in ? row * 24 + k : 0] : 0.f;
+        const __bf16 h = (__bf16)v;
+        ah[mt][q] = h;
+        al[mt][q] = (__bf16)(v - (float)h);
+      }
+    }
+    const float* su = reinterpret_cast<const float*>(s_u);
+    for (int nt = wave; nt < kElemPad / 16; nt += kThreads / 64) {
+      const int e = nt * 16 + m;  // the tile's element (column of U)
+      bf16x8 bh, bl;
+      const bool live = e < kElem && g < 3;
+      const int ex = e % EX, ey = (e / EX) % EY, ez = e / (EX * EY);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        // element DOF k = 8g + q: corner k / 3, component k % 3
+        const int k = 8 * g + q, b = k / 3, c = k - 3 * (k / 3);
+        // padded columns (e >= 405) and rows (k >= 24) read node 0 and are masked
+        const int node = live ? ((ez + corner_z(b)) * HY + (ey + corner_y(b))) * HX + (ex + corner_x(b)) : 0;
+        const float v = live ? su[4 * node + c] : 0.f;
+        const __bf16 h = (__bf16)v;
+        bh[q] = h;
+        bl[q] = (__bf16)(v - (float)h);
+      }
+      const float se = e < kElem ? (float)s_s[e] : 0.f;
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mt], bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt], bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt], bh, acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = mt * 16 + 4 * g + v;
+          if (row < 24) s_y[row * kElemPad + e] = acc[v] * se;
+        }
+      }
+    }
+    __syncthreads();
+    if (inside) {
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const int ox = (o & 1) - 1, oy = ((o >> 1) & 1) - 1, oz = (o >> 2) - 1;
+        const int e = ((tz + oz + 1) * EY + (ty + oy + 1)) * EX + (tx + ox + 1);
+        const int ca = corner(-ox, -oy, -oz);
+        acc0 += s_y[(3 * ca) * kElemPad + e];
+        acc1 += s_y[(3 * ca + 1) * kElemPad + e];
+        acc2 += s_y[(3 * ca + 2) * kElemPad + e];
+      }
+    }
+  } else if (inside) {
     // The node's 3 x 3 x 3 neighbourhood is read from LDS once, a z layer at a
     // time, and the 8 element products take their corners from registers
     // (27 reads instead of 64). The products run in the same order as a
@@ -266,20 +344,82 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
   }
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool MX = false>
 void launch(const MfArgs<T>& a, hipStream_t s) {
-  k_mf_hex8<T, MODE><<<blocks(a.X, a.Y, a.Z), kThreads, 0, s>>>(a);
+  k_mf_hex8<T, MODE, MX><<<blocks(a.X, a.Y, a.Z), kThreads, 0, s>>>(a);
+}
+
+// inv_diag of K_ff from the element factors: for each free DOF (node, k), 1 /
+// sum over the node's elements, in ascending element order, of s_e Ke0(a,k; a,k)
+// (a the node's corner in e) -- the assembled diagonal entry bit for bit, since
+// the assembly sums the same products in the same order. One thread per node.
+__global__ void __launch_bounds__(256) k_mf_inv_diag(int X, int Y, int Z, const int* __restrict__ reduced,
+                                                     const double* __restrict__ scale, const double* __restrict__ ke,
+                                                     double* __restrict__ dinv, int* __restrict__ bad_row) {
+  const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nn = (long long)X * Y * Z;
+  if (node >= nn) return;
+  const int i = (int)(node % X), j = (int)((node / X) % Y), k = (int)(node / ((long long)X * Y));
+  const int ex = X - 1, ey = Y - 1, ez = Z - 1;  // elements per direction
+  // VTK corner of offset (dx, dy, dz) from the element's first node
+  constexpr int corner[2][2][2] = {{{0, 1}, {3, 2}}, {{4, 5}, {7, 6}}};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int row = reduced[node * 3 + c];
+    if (row < 0) continue;
+    double d = 0.0;
+    for (int zz = k - 1; zz <= k; ++zz) {
+      if (zz < 0 || zz >= ez) continue;
+      for (int yy = j - 1; yy <= j; ++yy) {
+        if (yy < 0 || yy >= ey) continue;
+        for (int xx = i - 1; xx <= i; ++xx) {
+          if (xx < 0 || xx >= ex) continue;
+          const long long e = ((long long)zz * ey + yy) * ex + xx;
+          const int a = corner[k - zz][j - yy][i - xx];
+          d += scale[e] * ke[(a * 3 + c) * 24 + a * 3 + c];
+        }
+      }
+    }
+    if (!(d > 0.0)) atomicMin(bad_row, row);
+    dinv[row] = 1.0 / d;
+  }
 }
 
 }  // namespace
+
+void inv_diag(int X, int Y, int Z, const int* reduced, const double* scale, const double* ke, double* dinv,
+              int* bad_row, hipStream_t s) {
+  const long long nn = (long long)X * Y * Z;
+  if (nn) k_mf_inv_diag<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(X, Y, Z, reduced, scale, ke, dinv, bad_row);
+}
 
 int blocks(int X, int Y, int Z) {
   const int n = ((X + TX - 1) / TX) * ((Y + TY - 1) / TY) * ((Z + TZ - 1) / TZ);
   return (n + 7) / 8 * 8;
 }
 
+// opt-in (SPARLAB_MF_MFMA=1): at 128 x 64 x 64 the matrix-core variant's LDS (51 KB
+// per block) leaves 3 waves per SIMD, and the step, bound by its staging latency,
+// ran slower than the vector-FMA kernel
+bool matrix_cores() {
+  const char* e = std::getenv("SPARLAB_MF_MFMA");
+  return e && e[0] == '1';
+}
+
 template <typename T>
 void apply(const MfArgs<T>& a, int mode, hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    // the FP32 V-cycle's modes take the matrix-core element products
+    if (matrix_cores() && (mode == 2 || mode == 3 || mode == 5 || mode == 6)) {
+      switch (mode) {
+        case 2: launch<T, 2, true>(a, s); break;
+        case 3: launch<T, 3, true>(a, s); break;
+        case 5: launch<T, 5, true>(a, s); break;
+        default: launch<T, 6, true>(a, s); break;
+      }
+      return;
+    }
+  }
   switch (mode) {
     case 0: launch<T, 0>(a, s); break;
     case 1: launch<T, 1>(a, s); break;

@@ -208,6 +208,12 @@ struct Context {
   enum class KSource { none, assembled, evaluation, modal, caller };
   KSource k_source = KSource::none;
   bool k_on_host = false;
+  // K_ff's values lag the element factors: an evaluation that reused the coarse
+  // operators refreshed only level 0 and skipped the assembly (coarse_reuse)
+  bool k_stale = false;
+  int coarse_lag = 0;      // evaluations since the last full numeric setup
+  int lag_base_its = 0;    // PCG iterations of the evaluation right after it
+  int lag_last_its = 0;    // and of the latest evaluation
   // matrix-free K_ff of a structured Hex8 model: the element factors of the last
   // assembly (FP64 and FP32) and Ke0 in FP32; used for level-0 products when mf
   bool mf = false;

@@ -39,6 +39,10 @@ struct MfArgs {
 };
 
 int blocks(int X, int Y, int Z);
+// 1 / diag(K_ff) from the element factors (the assembled diagonal bit for bit);
+// a non-positive entry lowers *bad_row to its row
+void inv_diag(int X, int Y, int Z, const int* reduced, const double* scale, const double* ke, double* dinv,
+              int* bad_row, hipStream_t s);
 template <typename T>
 void apply(const MfArgs<T>& a, int mode, hipStream_t s);
 

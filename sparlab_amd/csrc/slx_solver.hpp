@@ -13,6 +13,9 @@ namespace slx {
 // so the setup returns without reading the pivots back
 void amg_setup(Context& c, bool defer_coarse = false);
 void finish_coarse(Context& c);
+// level 0's diagonal and spectral bound from the current element factors, the
+// coarse levels kept (matrix-free Hex8; the coarse_reuse option)
+void amg_refresh_level0(Context& c);
 // average milliseconds of one Chebyshev step at a level above the coarsest,
 // over reps back-to-back launches between two events (measurement only)
 double time_smoother(Context& c, int level, int reps);

@@ -53,7 +53,8 @@ class AmgOptions(C.Structure):
     _fields_ = [("strength_threshold", C.c_double), ("max_levels", C.c_int), ("coarse_size", C.c_int),
                 ("smoother", C.c_int), ("smoother_degree", C.c_int), ("chebyshev_ratio", C.c_double),
                 ("prolongator_damping", C.c_double), ("lanczos_steps", C.c_int),
-                ("coarse_pivot_tolerance", C.c_double), ("reuse_aggregates", C.c_int), ("precision", C.c_int)]
+                ("coarse_pivot_tolerance", C.c_double), ("reuse_aggregates", C.c_int), ("precision", C.c_int),
+                ("coarse_reuse", C.c_int)]
 
 
 class PcgResult(C.Structure):

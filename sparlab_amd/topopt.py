@@ -110,6 +110,8 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
     m = dk.model
     tol = dk.tolerance if tolerance is None else tolerance
     t_all = time.perf_counter()
+    timings = {}  # the reference's TimingLedger keys (apps/sparlab_topopt.cpp:404-860)
+    t0 = time.perf_counter()
     g = N.HipProblem(m, device=device)
     x0 = dk.initial_design()
     if precision != "fp64":
@@ -120,11 +122,14 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
         g.amg_setup(precision=_PRECISION[precision])
     log(f"{dk.name}: {m.nelem} elements, {g.nfree} free DOFs, {len(dk.case_names)} load case(s), "
         f"filter radius {dk.filter_radius:.6g} m, precision {precision}")
+    timings["model_build"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
     x, hist, iters, comp = g.run_oc(x0, dk.run["max_iterations"], change_tol=dk.run["change_tol"],
                                     objective_tol=dk.run["objective_tol"], window=dk.run["window"],
                                     cont_steps=dk.run["cont_steps"], penalty_start=dk.run["penalty_start"],
                                     cont_iterations=dk.run["cont_iterations"], move=dk.run["move"],
                                     damping=dk.run["damping"], tol=tol, projection=dk.run["projection"])
+    timings["optimization"] = time.perf_counter() - t0
     res = dict(g.last_run)
     case_c = g.load_case_compliance()
     st = g.amg_stats()
@@ -145,7 +150,9 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
         phys = (a + np.tanh(beta * (filtered - eta))) / (a + math.tanh(beta * (1.0 - eta)))
     factor = m.emin_ratio + phys ** m.penalty * (1.0 - m.emin_ratio)
     # the full solid domain (apps/sparlab_topopt.cpp:438-466)
+    t0 = time.perf_counter()
     solid = g.evaluate(np.ones(m.nelem), tol=tol, need_gradients=False, warm_start=False)["compliance"]
+    timings["solid_reference"] = time.perf_counter() - t0
     vf = res["volume_fraction"]
 
     wsum = sum(dk.case_weights)
@@ -155,7 +162,8 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
         "provenance": {
             "code": "sparlab_amd", "config_source": dk.source, "case": dk.name + (("_" + tag) if tag else ""),
             "description": dk.description, "units": "SI: m, N, Pa, kg, kg/m^3, Hz, J",
-            "timestamp_utc": time.strftime("%Y-%m-%dT%H:%M:%SZ", time.gmtime()), "device": f"cuda:{device}",
+            "timestamp_utc": time.strftime("%Y-%m-%dT%H:%M:%SZ", time.gmtime()),
+            "device": f"HIP device {device} (gfx950)",
         },
         "mesh": {
             "source": "structured_quad" if m.dim == 2 else "structured_hex",
@@ -196,6 +204,7 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
                                      "lambda_max_Dinv_A": st.lambda_max[i]} for i in range(st.levels)],
                     "operator_complexity": st.operator_complexity, "grid_complexity": st.grid_complexity,
                     "near_null_space_dimension": st.near_null_space_dimension,
+                    "coarse_pivot_ratio": st.coarse_pivot_ratio,
                     "last_setup_seconds": st.setup_seconds,
                     "aggregates_reused_in_last_setup": bool(st.reused_aggregates),
                 },
@@ -257,6 +266,7 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
         summary["mass_stiffness_comparison"]["equal_mass_uniform_plate_compliance_J"] = plate
         summary["mass_stiffness_comparison"]["stiffness_gain_over_equal_mass_plate"] = plate / comp if comp > 0 else 0.0
     if dk.modal is not None:
+        t0 = time.perf_counter()
         # the full solid domain's lowest modes (apps/sparlab_topopt.cpp:612-620)
         solid_modes = g.modal(dk.density, **dk.modal)
         summary["modal_initial_solid"] = {k: v for k, v in solid_modes.items() if k != "seconds"}
@@ -272,7 +282,9 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
         if dk.modal_flags["analyse_optimised_topology"]:
             topo = optimised_topology_modes(dk, phys, device, log)
             summary.update(topo)
+        timings["modal_analysis"] = time.perf_counter() - t0
 
+    t_out = time.perf_counter()
     if output:
         os.makedirs(output, exist_ok=True)
         if write_csv_files:
@@ -297,10 +309,15 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
                 rows.append([e] + list(cents[e]) + ([area] if m.dim == 2 else []) +
                             [ev, x[e], phys[e], factor[e], int(tags[e]), fin["energy"][e]])
             write_csv(os.path.join(output, "density_final.csv"), header, rows)
+        timings["output"] = time.perf_counter() - t_out
+        timings["total"] = time.perf_counter() - t_all
+        summary["timings_s"] = dict(timings)
         with open(os.path.join(output, "summary.json"), "w") as f:
             json.dump(summary, f, indent=2)
         log(f"{dk.name}: wrote {output}")
-    summary["_seconds_wall"] = time.perf_counter() - t_all
+    timings["total"] = time.perf_counter() - t_all
+    summary["timings_s"] = dict(timings)
+    summary["_seconds_wall"] = timings["total"]
     return summary
 
 

@@ -96,6 +96,13 @@ def test_deck_run_matches_the_published_benchmark(case, tmp_path):
     assert v == pytest.approx(r["volume_m3"], rel=1e-9)
     saved = json.load(open(tmp_path / "summary.json"))
     assert saved["optimization_result"]["stop_reason"] == r["stop_reason"]
+    # the reference's timing ledger keys and the multigrid block's coarse pivot ratio
+    # (ResultWriter.cpp:1664-1683,2917)
+    t = saved["timings_s"]
+    assert {"model_build", "optimization", "solid_reference", "output", "total"} <= set(t)
+    assert 0.0 < t["optimization"] <= t["total"]
+    mg = saved["optimization_result"]["linear_solver"]["multigrid"]
+    assert 0.0 < mg["coarse_pivot_ratio"] <= 1.0
     # the full solid domain's modes (published f1_solid_Hz, mass_solid_kg)
     if d.modal is not None:
         modes = saved["modal_initial_solid"]

@@ -426,3 +426,80 @@ def test_smoother_timing_leaves_the_solver_usable(ctx):
     np.testing.assert_array_equal(x0, x1)
     with pytest.raises(N.ConfigError):
         g.time_smoother(g.amg_stats().levels - 1, 5)  # the coarsest level has no smoother
+
+
+def test_coarse_reuse_keeps_the_answers_and_refreshes_level0():
+    """coarse_reuse (mixed path): evaluations that keep A_1 and below refresh level 0
+    only (no Galerkin time), an OC run follows the fresh-hierarchy run to the solve
+    tolerance, and K_ff read back after a lagged evaluation is the current operator's."""
+    print("test_coarse_reuse_keeps_the_answers_and_refreshes_level0", flush=True)
+    m = M.cantilever_3d(24, 12, 12)
+    x0 = m.initial_design()
+    runs = {}
+    for reuse in (0, 3):
+        c = N.Context(0)
+        g = N.HipProblem(m, ctx=c)
+        g.assemble(1e-9 + (1 - 1e-9) * np.clip(g.filter(x0), 0, 1) ** 3)
+        g.amg_setup(precision=1, coarse_reuse=reuse)
+        x, h, n, f = g.run_oc(x0, 12, change_tol=0.0, tol=1e-10)
+        runs[reuse] = (x, h, f)
+        if reuse:
+            # a full setup restarts the reuse: one full evaluation, then a lagged one (level 0 only)
+            g.amg_setup(precision=1, coarse_reuse=reuse)
+            x1 = np.clip(x * 1.01, 0.0, 1.0)
+            e1 = g.evaluate(x1, tol=1e-10)
+            st = g.amg_stats()
+            x2 = np.clip(x1 * 0.99, 0.0, 1.0)
+            e2 = g.evaluate(x2, tol=1e-10)
+            st2 = g.amg_stats()
+            print(f"coarse reuse: setups {st.setup_seconds * 1e3:.2f} / {st2.setup_seconds * 1e3:.2f} ms, "
+                  f"galerkin {st.galerkin_seconds * 1e3:.2f} / {st2.galerkin_seconds * 1e3:.2f} ms, "
+                  f"its {e1['iterations']} / {e2['iterations']}", flush=True)
+            assert st.galerkin_seconds > 0.0
+            assert st2.galerkin_seconds == 0.0 and st2.reused_aggregates
+            # the lagged solution solves the current operator: K u = f to the tolerance
+            u = g.solution(0)
+            y, _ = g.kff_apply(u)
+            b = m.free_loads()[0]
+            assert np.linalg.norm(y - b) <= 1e-9 * np.linalg.norm(b)
+            # K_ff read back after the lagged evaluation (assembled on demand) is the one a
+            # fresh context assembles for x2
+            pg, cg, vg = g.kff()
+            cf = N.Context(0)
+            gf = N.HipProblem(m, ctx=cf)
+            gf.evaluate(x2, tol=1e-10)
+            assert np.array_equal(vg, gf.kff()[2])
+            cf.close()
+        c.close()
+    (xa, ha, fa), (xb, hb, fb) = runs[0], runs[3]
+    np.testing.assert_allclose(hb[:, 2], ha[:, 2], rtol=1e-7)
+    assert fb == pytest.approx(fa, rel=1e-7)
+    print(f"coarse reuse: CG its fresh {ha[:, 11].astype(int).tolist()} reused {hb[:, 11].astype(int).tolist()}",
+          flush=True)
+
+
+def test_matrix_core_level0_products_match_the_vector_path(ctx, monkeypatch):
+    """The FP32 V-cycle's level-0 products on the matrix cores (split-bfloat16
+    MFMA, FP32 accumulation) against the FP32 vector-FMA path: one V-cycle to
+    1e-4, a solve to rtol 1e-10 in the same iterations (+-1) and to the same
+    compliance; the Chebyshev step's time for both."""
+    print("test_matrix_core_level0_products_match_the_vector_path", flush=True)
+    m = M.cantilever_3d(24, 12, 12)
+    g = N.HipProblem(m, ctx=ctx)
+    g.assemble(1e-9 + (1 - 1e-9) * M.seeded_density(m.nelem, seed=91) ** 3)
+    b = m.free_loads()[0]
+    r = np.random.default_rng(92).standard_normal(g.nfree)
+    out = {}
+    for mode in ("vector", "matrix"):
+        monkeypatch.setenv("SPARLAB_MF_MFMA", "0" if mode == "vector" else "1")
+        g.amg_setup(precision=1, coarse_size=300)
+        z = g.amg_apply(r)
+        x, it, rr, cv = g.solve(b, tol=1e-10)
+        ms = g.time_smoother(0, 20)
+        out[mode] = (z, x, it, cv, ms)
+    (zv, xv, itv, cvv, msv), (zm, xm, itm, cvm, msm) = out["vector"], out["matrix"]
+    dz = np.linalg.norm(zm - zv) / np.linalg.norm(zv)
+    print(f"matrix-core level 0: V-cycle rel diff {dz:.2e}, iterations {itv} / {itm}, "
+          f"Chebyshev step {1e3 * msv:.1f} / {1e3 * msm:.1f} us", flush=True)
+    assert cvv and cvm and dz <= 1e-4 and abs(itm - itv) <= 1
+    assert b @ xm == pytest.approx(b @ xv, rel=1e-10)
