@@ -208,11 +208,31 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   const auto t1 = Clock::now();
   std::vector<double> comp((size_t)c.ncases);
   int most_its = 0;
+  // several load cases (Sensitivity.cpp:169-176 solves them one by one): one
+  // block PCG drives them through each operator and V-cycle pass together,
+  // each case's result equal to its own solve; the learned warm start and the
+  // FP32 V-cycle keep the case-by-case loop
+  const bool block = c.ncases > 1 && block_cases_enabled() && c.amg_opt.precision == 0 &&
+                     !(c.ws && c.ws->opt.enable);
+  std::vector<slx_pcg_result> block_res;
+  if (block) {
+    if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "AMG-CG: solve called before factorize");
+    for (int l = 0; l < c.ncases; ++l)
+      if (!o.warm_start || !c.have_u[(size_t)l]) k::fill((long long)nf, 0.0, c.d_u.p + (size_t)l * nf, s);
+    pcg_block(c, c.ncases, c.d_loads_free.p, c.d_u.p, o.tolerance, o.max_iterations > 0 ? o.max_iterations : 1000,
+              block_res);
+    for (int l = 0; l < c.ncases; ++l) c.have_u[(size_t)l] = 1;
+  }
   for (int l = 0; l < c.ncases; ++l) {
-    if (o.warm_start) warm_before_solve(c, l);
-    const slx_pcg_result r = solve_case(c, l, o.warm_start != 0, o.tolerance, o.max_iterations);
+    slx_pcg_result r;
+    if (block) {
+      r = block_res[(size_t)l];
+    } else {
+      if (o.warm_start) warm_before_solve(c, l);
+      r = solve_case(c, l, o.warm_start != 0, o.tolerance, o.max_iterations);
+    }
     require_converged(c, r, o.tolerance);
-    if (o.warm_start) warm_after_solve(c, l);
+    if (!block && o.warm_start) warm_after_solve(c, l);
     out.linear_iterations += r.iterations;
     most_its = std::max(most_its, r.iterations);
     comp[(size_t)l] = device_dot(c, (long long)nf, c.d_loads_free.p + (size_t)l * nf, c.d_u.p + (size_t)l * nf);

@@ -87,8 +87,14 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ 
                                                   const int* __restrict__ cbase,
                                                   const VAL* __restrict__ val, const VT* __restrict__ x,
                                                   const VT* __restrict__ b, VT* __restrict__ y,
-                                                  double* __restrict__ partials) {
+                                                  double* __restrict__ partials, long long xs, long long ys) {
   __shared__ double sh[kBlock / 64];
+  // load case blockIdx.y of a block of right-hand sides (case-major vectors, strides xs and ys)
+  const long long vc = blockIdx.y;
+  x += vc * xs;
+  y += vc * ys;
+  if (MODE == 1) b += vc * ys;
+  if (MODE == 3) partials += vc * gridDim.x;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int rows_per_block = kBlock / LANES;
   const int row = blk * rows_per_block + threadIdx.x / LANES;
@@ -165,7 +171,13 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, const int* __restri
                                                        const VAL* __restrict__ val, const VT* __restrict__ dinv,
                                                        const VT* __restrict__ r_in, VT* __restrict__ r_out,
                                                        const VT* __restrict__ d_in, VT* __restrict__ d_out,
-                                                       VT* __restrict__ x, double c1, double c2) {
+                                                       VT* __restrict__ x, double c1, double c2, long long vs) {
+  const long long vc = blockIdx.y;  // load case of a block of right-hand sides
+  r_in += vc * vs;
+  r_out += vc * vs;
+  d_in += vc * vs;
+  d_out += vc * vs;
+  x += vc * vs;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int row = blk * (kBlock / LANES) + threadIdx.x / LANES;
   const int lane = threadIdx.x % LANES;
@@ -184,9 +196,13 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(int n, const int* __restri
 // Zero-guess start of the smoother: d = dinv b / theta, x = d.
 template <typename VT>
 __global__ void k_cheb_first_zero(int n, const VT* __restrict__ dinv, const VT* __restrict__ b, double theta,
-                                  VT* __restrict__ x, VT* __restrict__ d) {
+                                  VT* __restrict__ x, VT* __restrict__ d, long long vs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const long long o = blockIdx.y * vs;  // load case blockIdx.y
+  b += o;
+  x += o;
+  d += o;
   const double di = (double)dinv[i] * (double)b[i] / theta;
   d[i] = (VT)di;
   x[i] = (VT)(0.0 + di);
@@ -195,9 +211,13 @@ __global__ void k_cheb_first_zero(int n, const VT* __restrict__ dinv, const VT* 
 // Non-zero-guess start: d = dinv r / theta, x += d (r = b - A x computed before).
 template <typename VT>
 __global__ void k_cheb_first(int n, const VT* __restrict__ dinv, const VT* __restrict__ r, double theta,
-                             VT* __restrict__ x, VT* __restrict__ d) {
+                             VT* __restrict__ x, VT* __restrict__ d, long long vs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const long long o = blockIdx.y * vs;  // load case blockIdx.y
+  r += o;
+  x += o;
+  d += o;
   const double di = (double)dinv[i] * (double)r[i] / theta;
   d[i] = (VT)di;
   x[i] = (VT)((double)x[i] + di);
@@ -206,10 +226,12 @@ __global__ void k_cheb_first(int n, const VT* __restrict__ dinv, const VT* __res
 // y = M b for a dense row-major n x n M: one wave per row.
 template <typename VT>
 __global__ void __launch_bounds__(kBlock) k_dense_gemv(int n, const VT* __restrict__ M, const VT* __restrict__ b,
-                                                        VT* __restrict__ y) {
+                                                        VT* __restrict__ y, long long vs) {
   const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
+  b += blockIdx.y * vs;  // load case blockIdx.y
+  y += blockIdx.y * vs;
   double acc = 0.0;
   const VT* m = M + (size_t)row * n;
   for (int j = lane; j < n; j += 64) acc += (double)m[j] * (double)b[j];
@@ -218,8 +240,12 @@ __global__ void __launch_bounds__(kBlock) k_dense_gemv(int n, const VT* __restri
 }
 
 __global__ void __launch_bounds__(kBlock) k_dot_partials(int n, const double* __restrict__ a,
-                                                          const double* __restrict__ b, double* __restrict__ partials) {
+                                                          const double* __restrict__ b, double* __restrict__ partials,
+                                                          long long vs) {
   __shared__ double sh[kBlock / 64];
+  a += blockIdx.y * vs;  // load case blockIdx.y: its partials follow the previous case's
+  b += blockIdx.y * vs;
+  partials += (long long)blockIdx.y * gridDim.x;
   double s = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += a[i] * b[i];
   const double t = block_sum(s, sh);
@@ -238,8 +264,10 @@ __global__ void __launch_bounds__(kBlock) k_grey_partials(int n, const double* _
 
 // Sums m partials in a fixed order into out[0]; optional sqrt.
 __global__ void __launch_bounds__(1024) k_reduce_partials(int m, const double* __restrict__ partials,
-                                                          double* __restrict__ out, int take_sqrt) {
+                                                          double* __restrict__ out, int take_sqrt, int out_stride) {
   __shared__ double sh[16];
+  partials += (long long)blockIdx.x * m;  // one block per load case
+  out += (long long)blockIdx.x * out_stride;
   double s = 0.0;
   for (int i = threadIdx.x; i < m; i += blockDim.x) s += partials[i];
   const double t = block_sum(s, sh);
@@ -251,15 +279,30 @@ __global__ void __launch_bounds__(1024) k_reduce_partials(int m, const double* _
 __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, double* __restrict__ x, double* __restrict__ r,
                                                         const double* __restrict__ p, const double* __restrict__ q,
                                                         const double* __restrict__ scal, int i_rz, int i_pq,
-                                                        double* __restrict__ partials) {
+                                                        double* __restrict__ partials, long long vs,
+                                                        const int* __restrict__ active) {
   __shared__ double sh[kBlock / 64];
-  const double alpha = scal[i_rz] / scal[i_pq];
+  // load case blockIdx.y: vectors at stride vs, 16 scalars per case, partials per case;
+  // a case whose active flag is 0 keeps its iterate and residual
+  const long long o = blockIdx.y * vs;
+  x += o;
+  r += o;
+  p += o;
+  q += o;
+  scal += 16 * blockIdx.y;
+  partials += (long long)blockIdx.y * gridDim.x;
+  const bool live = !active || active[blockIdx.y];
+  const double alpha = live ? scal[i_rz] / scal[i_pq] : 0.0;
   double s = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    s += ri * ri;
+    if (live) {
+      x[i] += alpha * p[i];
+      const double ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      s += ri * ri;
+    } else {
+      s += r[i] * r[i];
+    }
   }
   const double t = block_sum(s, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
@@ -271,8 +314,14 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, double* __restrict
 template <typename VT>
 __global__ void __launch_bounds__(kBlock) k_precond_partials(int n, const VT* __restrict__ zc,
                                                               const double* __restrict__ r, double* __restrict__ z,
-                                                              int flexible, double* __restrict__ partials) {
+                                                              int flexible, double* __restrict__ partials,
+                                                              long long vs) {
   __shared__ double sh[kBlock / 64];
+  const long long o = blockIdx.y * vs;  // load case blockIdx.y
+  if (zc) zc += o;
+  r += o;
+  z += o;
+  partials += 2LL * blockIdx.y * gridDim.x;
   double s1 = 0.0, s2 = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double zo = z[i];
@@ -297,6 +346,8 @@ __global__ void __launch_bounds__(kBlock) k_precond_partials(int n, const VT* __
 __global__ void __launch_bounds__(1024) k_precond_finish(int m, const double* __restrict__ partials, double* scal,
                                                          int i_rz, int i_rzn, int i_beta, int flexible) {
   __shared__ double sh[16];
+  partials += 2LL * blockIdx.x * m;  // one block per load case
+  scal += 16 * blockIdx.x;
   double v = 0.0;
   for (int i = threadIdx.x; i < m; i += blockDim.x) v += partials[i];
   const double rzn = block_sum(v, sh);
@@ -329,7 +380,10 @@ __global__ void k_cheb_first_zero_from64(int n, const float* __restrict__ dinv, 
 
 // p = z + beta p with beta = rz_new / rz_old (src/fem/Multigrid.cpp:1209-1215)
 __global__ void k_pcg_direction(int n, double* __restrict__ p, const double* __restrict__ z,
-                                const double* __restrict__ scal, int i_new, int i_old) {
+                                const double* __restrict__ scal, int i_new, int i_old, long long vs) {
+  p += blockIdx.y * vs;  // load case blockIdx.y
+  z += blockIdx.y * vs;
+  scal += 16 * blockIdx.y;
   const double beta = i_old >= 0 ? scal[i_new] / scal[i_old] : scal[i_new];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
 }
@@ -689,18 +743,21 @@ int lanes_for(const DCsr& A) {
   return avg > 256 ? 64 : avg > 48 ? 16 : avg > 12 ? 8 : 4;
 }
 
+// nv load cases (gridDim.y), case-major vectors: x at stride A.cols, b and y at stride A.rows
 template <int MODE, typename VT, typename VAL = VT>
-void launch_spmv(const DCsr& A, const VAL* val, const VT* x, const VT* b, VT* y, double* partials, hipStream_t s) {
+void launch_spmv(const DCsr& A, const VAL* val, const VT* x, const VT* b, VT* y, double* partials, hipStream_t s,
+                 int nv = 1) {
   const int lanes = lanes_for(A);
-  const int nb = grid8(A.rows, kBlock / lanes);
+  const dim3 nb(grid8(A.rows, kBlock / lanes), nv);
   if (A.rows == 0) return;
   const int* cols = A.cbase.p ? A.ncol.p : A.col.p;
+  const long long xs = A.cols, ys = A.rows;
   switch (lanes) {
-    case 64: k_spmv<64, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
-    case 16: k_spmv<16, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
-    case 8: k_spmv<8, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
-    case 1: k_spmv<1, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
-    default: k_spmv<4, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials); break;
+    case 64: k_spmv<64, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
+    case 16: k_spmv<16, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
+    case 8: k_spmv<8, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
+    case 1: k_spmv<1, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
+    default: k_spmv<4, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
   }
 }
 
@@ -716,14 +773,14 @@ void launch_spmv32(const DCsr& A, const float* x, const float* b, float* y, hipS
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-void spmv(const DCsr& A, const double* x, double* y, hipStream_t s) {
-  launch_spmv<0, double>(A, A.val.p, x, nullptr, y, nullptr, s);
+void spmv(const DCsr& A, const double* x, double* y, hipStream_t s, int nv) {
+  launch_spmv<0, double>(A, A.val.p, x, nullptr, y, nullptr, s, nv);
 }
-void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s) {
-  launch_spmv<2, double>(A, A.val.p, x, nullptr, y, nullptr, s);
+void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s, int nv) {
+  launch_spmv<2, double>(A, A.val.p, x, nullptr, y, nullptr, s, nv);
 }
-void residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s) {
-  launch_spmv<1, double>(A, A.val.p, x, b, r, nullptr, s);
+void residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s, int nv) {
+  launch_spmv<1, double>(A, A.val.p, x, b, r, nullptr, s, nv);
 }
 void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s) { launch_spmv32<0>(A, x, nullptr, y, s); }
 void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s) { launch_spmv32<2>(A, x, nullptr, y, s); }
@@ -731,8 +788,8 @@ void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStre
   launch_spmv32<1>(A, x, b, r, s);
 }
 int spmv_dot_blocks(const DCsr& A) { return grid8(A.rows, kBlock / lanes_for(A)); }
-void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s) {
-  launch_spmv<3, double>(A, A.val.p, x, nullptr, y, partials, s);
+void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s, int nv) {
+  launch_spmv<3, double>(A, A.val.p, x, nullptr, y, partials, s, nv);
 }
 int staged_rows(const DCsr& A) {
   // rows per block such that every block's entries fit the stage
@@ -750,22 +807,23 @@ void spmv_serial(const DCsr& A, const double* x, double* y, hipStream_t s) {
 
 template <typename VT, typename VAL = VT>
 static void cheb_step_t(const DCsr& A, const VAL* val, const VT* dinv, const VT* r_in, VT* r_out, const VT* d_in,
-                        VT* d_out, VT* x, double c1, double c2, hipStream_t s) {
+                        VT* d_out, VT* x, double c1, double c2, hipStream_t s, int nv = 1) {
   const int lanes = lanes_for(A);
-  const int nb = grid8(A.rows, kBlock / lanes);
+  const dim3 nb(grid8(A.rows, kBlock / lanes), nv);
+  const long long vs = A.rows;
   if (A.rows == 0) return;
   const int* cols = A.cbase.p ? A.ncol.p : A.col.p;
   switch (lanes) {
-    case 64: k_cheb_step<64, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
-    case 16: k_cheb_step<16, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
-    case 8: k_cheb_step<8, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
-    case 1: k_cheb_step<1, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
-    default: k_cheb_step<4, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2); break;
+    case 64: k_cheb_step<64, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2, vs); break;
+    case 16: k_cheb_step<16, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2, vs); break;
+    case 8: k_cheb_step<8, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2, vs); break;
+    case 1: k_cheb_step<1, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2, vs); break;
+    default: k_cheb_step<4, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, dinv, r_in, r_out, d_in, d_out, x, c1, c2, vs); break;
   }
 }
 void cheb_step(const DCsr& A, const double* dinv, const double* r_in, double* r_out, const double* d_in, double* d_out,
-               double* x, double c1, double c2, hipStream_t s) {
-  cheb_step_t<double>(A, A.val.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s);
+               double* x, double c1, double c2, hipStream_t s, int nv) {
+  cheb_step_t<double>(A, A.val.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s, nv);
 }
 void cheb_step32(const DCsr& A, const float* dinv, const float* r_in, float* r_out, const float* d_in, float* d_out,
                  float* x, double c1, double c2, hipStream_t s) {
@@ -774,58 +832,62 @@ void cheb_step32(const DCsr& A, const float* dinv, const float* r_in, float* r_o
   else
     cheb_step_t<float>(A, A.val32.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s);
 }
-void cheb_first_zero(int n, const double* dinv, const double* b, double theta, double* x, double* d, hipStream_t s) {
-  if (n) k_cheb_first_zero<double><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, b, theta, x, d);
+void cheb_first_zero(int n, const double* dinv, const double* b, double theta, double* x, double* d, hipStream_t s,
+                     int nv) {
+  if (n) k_cheb_first_zero<double><<<dim3(blocks_for(n, kBlock), nv), kBlock, 0, s>>>(n, dinv, b, theta, x, d, n);
 }
-void cheb_first(int n, const double* dinv, const double* r, double theta, double* x, double* d, hipStream_t s) {
-  if (n) k_cheb_first<double><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, r, theta, x, d);
+void cheb_first(int n, const double* dinv, const double* r, double theta, double* x, double* d, hipStream_t s,
+                int nv) {
+  if (n) k_cheb_first<double><<<dim3(blocks_for(n, kBlock), nv), kBlock, 0, s>>>(n, dinv, r, theta, x, d, n);
 }
 void cheb_first_zero32(int n, const float* dinv, const float* b, double theta, float* x, float* d, hipStream_t s) {
-  if (n) k_cheb_first_zero<float><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, b, theta, x, d);
+  if (n) k_cheb_first_zero<float><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, b, theta, x, d, 0);
 }
 void cheb_first32(int n, const float* dinv, const float* r, double theta, float* x, float* d, hipStream_t s) {
-  if (n) k_cheb_first<float><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, r, theta, x, d);
+  if (n) k_cheb_first<float><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, r, theta, x, d, 0);
 }
-void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t s) {
-  if (n) k_dense_gemv<double><<<blocks_for((long long)n * 64, kBlock), kBlock, 0, s>>>(n, M, b, y);
+void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t s, int nv) {
+  if (n) k_dense_gemv<double><<<dim3(blocks_for((long long)n * 64, kBlock), nv), kBlock, 0, s>>>(n, M, b, y, n);
 }
 void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s) {
-  if (n) k_dense_gemv<float><<<blocks_for((long long)n * 64, kBlock), kBlock, 0, s>>>(n, M, b, y);
+  if (n) k_dense_gemv<float><<<blocks_for((long long)n * 64, kBlock), kBlock, 0, s>>>(n, M, b, y, 0);
 }
 
 int reduce_blocks(long long n) { return (int)std::max<long long>(1, std::min<long long>(kReduceBlocks, (n + kBlock - 1) / kBlock)); }
 
-void dot(long long n, const double* a, const double* b, double* partials, double* out, hipStream_t s) {
+void dot(long long n, const double* a, const double* b, double* partials, double* out, hipStream_t s, int nv,
+         int out_stride) {
   const int nb = reduce_blocks(n);
-  k_dot_partials<<<nb, kBlock, 0, s>>>((int)n, a, b, partials);
-  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+  k_dot_partials<<<dim3(nb, nv), kBlock, 0, s>>>((int)n, a, b, partials, n);
+  k_reduce_partials<<<nv, 1024, 0, s>>>(nb, partials, out, 0, out_stride);
 }
 void grey_sum(long long n, const double* rho, double* partials, double* out, hipStream_t s) {
   const int nb = reduce_blocks(n);
   k_grey_partials<<<nb, kBlock, 0, s>>>((int)n, rho, partials);
-  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0, 0);
 }
-void reduce(int m, const double* partials, double* out, int take_sqrt, hipStream_t s) {
-  k_reduce_partials<<<1, 1024, 0, s>>>(m, partials, out, take_sqrt);
+void reduce(int m, const double* partials, double* out, int take_sqrt, hipStream_t s, int nv, int out_stride) {
+  k_reduce_partials<<<nv, 1024, 0, s>>>(m, partials, out, take_sqrt, out_stride);
 }
 void pcg_update(long long n, double* x, double* r, const double* p, const double* q, const double* scal, int i_rz,
-                int i_pq, double* partials, double* out, hipStream_t s) {
+                int i_pq, double* partials, double* out, hipStream_t s, int nv, const int* active) {
   const int nb = reduce_blocks(n);
-  k_pcg_update<<<nb, kBlock, 0, s>>>((int)n, x, r, p, q, scal, i_rz, i_pq, partials);
-  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+  k_pcg_update<<<dim3(nb, nv), kBlock, 0, s>>>((int)n, x, r, p, q, scal, i_rz, i_pq, partials, n, active);
+  k_reduce_partials<<<nv, 1024, 0, s>>>(nb, partials, out, 0, 16);
 }
 void precond_tail(long long n, const float* zc, const double* r, double* z, double* scal, int i_rz, int i_rzn,
-                  int i_beta, bool flexible, double* partials, hipStream_t s) {
+                  int i_beta, bool flexible, double* partials, hipStream_t s, int nv) {
   const int nb = reduce_blocks(n);
-  k_precond_partials<float><<<nb, kBlock, 0, s>>>((int)n, zc, r, z, flexible ? 1 : 0, partials);
-  k_precond_finish<<<1, 1024, 0, s>>>(nb, partials, scal, i_rz, i_rzn, i_beta, flexible ? 1 : 0);
+  k_precond_partials<float><<<dim3(nb, nv), kBlock, 0, s>>>((int)n, zc, r, z, flexible ? 1 : 0, partials, n);
+  k_precond_finish<<<nv, 1024, 0, s>>>(nb, partials, scal, i_rz, i_rzn, i_beta, flexible ? 1 : 0);
 }
 void cheb_first_zero_from64(int n, const float* dinv, const double* r, double theta, float* b, float* x, float* d,
                             hipStream_t s) {
   if (n) k_cheb_first_zero_from64<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, r, theta, b, x, d);
 }
-void pcg_direction(long long n, double* p, const double* z, const double* scal, int i_new, int i_old, hipStream_t s) {
-  k_pcg_direction<<<reduce_blocks(n), kBlock, 0, s>>>((int)n, p, z, scal, i_new, i_old);
+void pcg_direction(long long n, double* p, const double* z, const double* scal, int i_new, int i_old, hipStream_t s,
+                   int nv) {
+  k_pcg_direction<<<dim3(reduce_blocks(n), nv), kBlock, 0, s>>>((int)n, p, z, scal, i_new, i_old, n);
 }
 void copy_scalar(double* scal, int from, int to, hipStream_t s) { k_copy_scalar<<<1, 1, 0, s>>>(scal, from, to); }
 void flexible_beta(double* scal, int i_new, int i_cross, int i_old, int i_out, hipStream_t s) {
@@ -899,12 +961,12 @@ void filter_volume(const DCsr& H, const double* x, const double* vol, double* pa
     // 128 rows per block keeps the partial count within the reduction buffer for any n up to 2^19 blocks
     const int nb = blocks_for(H.rows, 128);
     k_spmv_staged<128, 1><<<nb, kBlock, 0, s>>>(H.rows, H.ptr.p, H.col.p, H.val.p, x, nullptr, vol, partials);
-    k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+    k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0, 0);
     return;
   }
   const int nb = reduce_blocks(H.rows);
   k_filter_volume<<<nb, kBlock, 0, s>>>(H.rows, H.ptr.p, H.col.p, H.val.p, x, vol, partials);
-  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0);
+  k_reduce_partials<<<1, 1024, 0, s>>>(nb, partials, out, 0, 0);
 }
 int filter_volume_blocks(const DCsr& H) { return blocks_for(H.rows, 128); }
 void max_abs_diff(int n, const double* a, const double* b, double* partials, double* out, hipStream_t s) {

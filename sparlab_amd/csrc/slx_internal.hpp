@@ -158,6 +158,7 @@ struct Level {
   DBuf<float> tent_ell32, tent_ella32;
   DBuf<double> x, b, res, d, d2;
   DBuf<float> dinv32, x32, b32, res32, d32, d232;
+  DBuf<double> vx, vb, vres, vd, vd2;  // a block of load cases side by side (case-major)
 };
 
 struct TimerSlot {
@@ -254,6 +255,12 @@ struct Context {
   DBuf<double> partials;  // block partial sums
   DBuf<double> scal;      // device scalars
   std::vector<double> h_scal;
+  // several load cases through one operator and hierarchy pass (pcg_block):
+  // case-major vectors, 16 scalars and one active flag per case
+  DBuf<double> vx, vr, vz, vp, vq, vpart, vscal;
+  DBuf<int> vactive;
+  int block_nv = 0;  // cases the block workspace and graphs were made for
+  hipGraphExec_t g_vupdate = nullptr, g_vprecond = nullptr;
   // pinned host memory for small device -> host reads (scalars, per-segment sums)
   double* pinned = nullptr;
   size_t pinned_n = 0;

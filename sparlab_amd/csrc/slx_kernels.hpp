@@ -25,29 +25,33 @@ template <typename T>
 void tent_restrict(const TentBlocks& t, const T* ella, const T* y, T* xc, hipStream_t s);
 
 
-// operator products: y = A x, y += A x, r = b - A x (FP64 and FP32 storage)
-void spmv(const DCsr& A, const double* x, double* y, hipStream_t s);
-void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s);
-void residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s);
+// operator products: y = A x, y += A x, r = b - A x (FP64 and FP32 storage).
+// nv > 1: nv load cases side by side (case-major vectors, x at stride A.cols,
+// b, r and y at stride A.rows), each case's sums in the single-case order.
+void spmv(const DCsr& A, const double* x, double* y, hipStream_t s, int nv = 1);
+void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s, int nv = 1);
+void residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s, int nv = 1);
 void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s);
 void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s);
 void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s);
 // y = A x and per-block partial sums of x.y (spmv_dot_blocks(A) partials)
 int spmv_dot_blocks(const DCsr& A);
-void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s);
+void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s, int nv = 1);
 // one thread per row, column-order sums (bitwise equal to the CPU restatement)
 void spmv_serial(const DCsr& A, const double* x, double* y, hipStream_t s);
 
 // Chebyshev smoother pieces
 void cheb_step(const DCsr& A, const double* dinv, const double* r_in, double* r_out, const double* d_in, double* d_out,
-               double* x, double c1, double c2, hipStream_t s);
+               double* x, double c1, double c2, hipStream_t s, int nv = 1);
 void cheb_step32(const DCsr& A, const float* dinv, const float* r_in, float* r_out, const float* d_in, float* d_out,
                  float* x, double c1, double c2, hipStream_t s);
-void cheb_first_zero(int n, const double* dinv, const double* b, double theta, double* x, double* d, hipStream_t s);
-void cheb_first(int n, const double* dinv, const double* r, double theta, double* x, double* d, hipStream_t s);
+void cheb_first_zero(int n, const double* dinv, const double* b, double theta, double* x, double* d, hipStream_t s,
+                     int nv = 1);
+void cheb_first(int n, const double* dinv, const double* r, double theta, double* x, double* d, hipStream_t s,
+                int nv = 1);
 void cheb_first_zero32(int n, const float* dinv, const float* b, double theta, float* x, float* d, hipStream_t s);
 void cheb_first32(int n, const float* dinv, const float* r, double theta, float* x, float* d, hipStream_t s);
-void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t s);
+void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t s, int nv = 1);
 // a[n*n] (row-major) = the inverse of the symmetric positive definite CSR matrix
 // (ptr, col, val), by blocked in-place Gauss-Jordan on the device; piv[n] = the
 // scalar pivots (the D of its unpivoted LDL^T). work: dense_inverse_work(n) doubles.
@@ -58,19 +62,27 @@ void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s
 
 // deterministic reductions: partial buffer of at least 2048 doubles
 int reduce_blocks(long long n);
-void dot(long long n, const double* a, const double* b, double* partials, double* out, hipStream_t s);
+// nv > 1: case v's dot (vectors at stride n) into out[v * out_stride]; partials
+// then hold nv * reduce_blocks(n) doubles
+void dot(long long n, const double* a, const double* b, double* partials, double* out, hipStream_t s, int nv = 1,
+         int out_stride = 0);
 // out[0] = sum_i rho_i (1 - rho_i) (the grey level times n / 4)
 void grey_sum(long long n, const double* rho, double* partials, double* out, hipStream_t s);
-void reduce(int m, const double* partials, double* out, int take_sqrt, hipStream_t s);
+void reduce(int m, const double* partials, double* out, int take_sqrt, hipStream_t s, int nv = 1,
+            int out_stride = 0);
+// x += alpha p, r -= alpha q, out = r.r with alpha = scal[i_rz] / scal[i_pq].
+// nv > 1: case v at vector stride n, scalars at scal + 16 v and out + 16 v; a case
+// with active[v] == 0 (active may be null) keeps x and r and only recomputes r.r
 void pcg_update(long long n, double* x, double* r, const double* p, const double* q, const double* scal, int i_rz,
-                int i_pq, double* partials, double* out, hipStream_t s);
+                int i_pq, double* partials, double* out, hipStream_t s, int nv = 1, const int* active = nullptr);
 // p = z + beta p, beta = scal[i_new] / scal[i_old] (or scal[i_new] when i_old < 0)
-void pcg_direction(long long n, double* p, const double* z, const double* scal, int i_new, int i_old, hipStream_t s);
+void pcg_direction(long long n, double* p, const double* z, const double* scal, int i_new, int i_old, hipStream_t s,
+                   int nv = 1);
 // z = zc (null: z is already the cycle's result), r.z (and r.z_old when
 // flexible), beta into scal[i_beta], scal[i_rzn] = scal[i_rz] = r.z;
 // partials: 2 per block (reduce_blocks(n))
 void precond_tail(long long n, const float* zc, const double* r, double* z, double* scal, int i_rz, int i_rzn,
-                  int i_beta, bool flexible, double* partials, hipStream_t s);
+                  int i_beta, bool flexible, double* partials, hipStream_t s, int nv = 1);
 void cheb_first_zero_from64(int n, const float* dinv, const double* r, double theta, float* b, float* x, float* d,
                             hipStream_t s);
 void copy_scalar(double* scal, int from, int to, hipStream_t s);

@@ -25,6 +25,11 @@ void prepare_mixed(Context& c);
 void vcycle(Context& c, const double* r, double* z);
 // Preconditioned CG on the device (src/fem/Multigrid.cpp:1152-1237); b, x device pointers.
 slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_iterations);
+// The same for nv load cases at once (case-major b and x, nv x n): one operator
+// product and one V-cycle pass per iteration for every case; each case's
+// result equals pcg() on that case alone. FP64 V-cycle only.
+void pcg_block(Context& c, int nv, const double* b, double* x, double tol, int max_iterations,
+               std::vector<slx_pcg_result>& out);
 
 // Independent PCG solves on the diagonal blocks [seg_ptr[s], seg_ptr[s+1]) of the
 // factorized matrix, all segments per kernel launch; b, x device pointers.
@@ -47,6 +52,8 @@ bool matrix_free_enabled();
 
 // launches recorded into an executable graph (SPARLAB_NO_GRAPHS=1: callers launch eagerly)
 bool graphs_enabled();
+// several load cases of an evaluation through pcg_block (SPARLAB_NO_BLOCK_CASES=1: one by one)
+bool block_cases_enabled();
 hipGraphExec_t capture_graph(Context& c, const std::function<void()>& body);
 
 // drop the captured CG graphs (after a setup or a reallocation)

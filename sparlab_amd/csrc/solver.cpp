@@ -254,6 +254,80 @@ void cycle32(Context& c, size_t l, const float* b, float* x, bool started = fals
 }
 
 
+// ---- a block of nv load cases ----------------------------------------------
+// The same V-cycle for nv right-hand sides at once: every assembled-operator
+// pass (smoother, residual, transfers, coarse solve) reads its matrix once for
+// all cases, each case summed in the single-case order, so each case's result
+// equals its own cycle64 bit for bit. Level vectors are the block buffers
+// (vb, vx, vres, vd, vd2: nv x n_l, case-major); the matrix-free level-0
+// operator, which reads no matrix, runs once per case.
+void block_residual(Context& c, size_t l, const double* x, const double* b, double* r, int nv) {
+  if (use_mf(c, l)) {
+    const long long n = c.levels[l].n;
+    for (int v = 0; v < nv; ++v) level_residual<double>(c, l, x + v * n, b + v * n, r + v * n);
+  } else {
+    k::residual(op(c, l), x, b, r, c.stream, nv);
+  }
+}
+
+void block_cheb_steps(Context& c, size_t l, const double* r_first, double* x, int nv) {
+  Level& lv = c.levels[l];
+  DCsr& A = op(c, l);
+  const Cheb cp = cheb_params(c, lv);
+  const long long n = lv.n;
+  double rho = 1.0 / cp.sigma;
+  hipStream_t s = c.stream;
+  for (int k = 1; k < c.amg_opt.smoother_degree; ++k) {
+    const double rho_new = 1.0 / (2.0 * cp.sigma - rho);
+    const double c1 = rho_new * rho;
+    const double c2 = 2.0 * rho_new / cp.delta;
+    const double* rin = k == 1 ? r_first : lv.vres.p;
+    if (use_mf(c, l)) {
+      for (int v = 0; v < nv; ++v) {
+        kmf::MfArgs<double> a = mf_base<double>(c);
+        a.x = lv.vd.p + v * n, a.b = rin + v * n, a.y = lv.vres.p + v * n, a.dinv = lv.dinv.p;
+        a.d_out = lv.vd2.p + v * n, a.xs = x + v * n, a.c1 = c1, a.c2 = c2;
+        kmf::apply(a, 3, s);
+      }
+    } else {
+      k::cheb_step(A, lv.dinv.p, rin, lv.vres.p, lv.vd.p, lv.vd2.p, x, c1, c2, s, nv);
+    }
+    std::swap(lv.vd, lv.vd2);
+    rho = rho_new;
+  }
+}
+
+void block_cycle64(Context& c, size_t l, const double* b, double* x, int nv) {
+  Level& lv = c.levels[l];
+  hipStream_t s = c.stream;
+  if (l + 1 == c.levels.size()) {
+    k::dense_gemv(lv.n, c.coarse_inv.p, b, x, s, nv);
+    return;
+  }
+  const Cheb cp = cheb_params(c, lv);
+  const long long n = lv.n;
+  k::cheb_first_zero(lv.n, lv.dinv.p, b, cp.theta, x, lv.vd.p, s, nv);
+  block_cheb_steps(c, l, b, x, nv);
+  block_residual(c, l, x, b, lv.vres.p, nv);
+  Level& nx = c.levels[l + 1];
+  const long long nc = nx.n;
+  if (use_mf(c, l)) {
+    for (int v = 0; v < nv; ++v) restrict_mf<double>(c, lv.vres.p + v * n, lv.vd.p + v * n, nx.vb.p + v * nc);
+  } else {
+    k::spmv(lv.R, lv.vres.p, nx.vb.p, s, nv);
+  }
+  block_cycle64(c, l + 1, nx.vb.p, nx.vx.p, nv);
+  if (use_mf(c, l)) {
+    for (int v = 0; v < nv; ++v) prolong_mf<double>(c, nx.vx.p + v * nc, lv.vd.p + v * n, x + v * n);
+  } else {
+    k::spmv_add(lv.P, nx.vx.p, x, s, nv);
+  }
+  block_residual(c, l, x, b, lv.vres.p, nv);
+  k::cheb_first(lv.n, lv.dinv.p, lv.vres.p, cp.theta, x, lv.vd.p, s, nv);
+  block_cheb_steps(c, l, lv.vres.p, x, nv);
+}
+
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -358,7 +432,7 @@ int op_lanczos(Context& c, const double* sq, const double* v, const double* vpre
 }
 
 void invalidate_graphs(Context& c) {
-  for (hipGraphExec_t* g : {&c.g_update, &c.g_precond}) {
+  for (hipGraphExec_t* g : {&c.g_update, &c.g_precond, &c.g_vupdate, &c.g_vprecond}) {
     if (*g) (void)hipGraphExecDestroy(*g);
     *g = nullptr;
   }
@@ -501,6 +575,11 @@ double time_smoother(Context& c, int level, int reps) {
   return ms / reps;
 }
 
+bool block_cases_enabled() {
+  const char* e = std::getenv("SPARLAB_NO_BLOCK_CASES");
+  return !(e && e[0] == '1');
+}
+
 bool graphs_enabled() {
   const char* e = std::getenv("SPARLAB_NO_GRAPHS");
   return !(e && e[0] == '1');
@@ -638,6 +717,204 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
   k::copy(n, x, x_out, s);
   c.flush_timers();
   return res;
+}
+
+
+void ensure_block_workspace(Context& c, int nv) {
+  const size_t n = (size_t)c.K.rows;
+  if (c.block_nv != nv || c.vr.n != n * (size_t)nv) invalidate_graphs(c);
+  c.block_nv = nv;
+  for (DBuf<double>* v : {&c.vx, &c.vr, &c.vz, &c.vp, &c.vq}) v->alloc(n * (size_t)nv);
+  for (Level& lv : c.levels)
+    for (DBuf<double>* v : {&lv.vx, &lv.vb, &lv.vres, &lv.vd, &lv.vd2}) v->alloc((size_t)lv.n * (size_t)nv);
+  const size_t per = (size_t)std::max({op_dot_blocks(c), k::spmv_dot_blocks(c.K), 2 * k::reduce_blocks((long long)n)});
+  if (c.vpart.n < per * (size_t)nv + 8) {
+    invalidate_graphs(c);
+    c.vpart.alloc(per * (size_t)nv + 8);
+  }
+  if (c.vscal.n < 16 * (size_t)nv) {
+    invalidate_graphs(c);
+    c.vscal.alloc(16 * (size_t)nv);
+  }
+  if (c.vactive.n < (size_t)nv) {
+    invalidate_graphs(c);
+    c.vactive.alloc((size_t)nv);
+  }
+}
+
+// PCG for nv load cases at once (Multigrid.cpp:1152-1237 per case): one
+// operator product, one V-cycle pass and one set of reductions per iteration
+// for all cases still iterating. Each case follows the single-case pcg()
+// step for step: the same restarts, tests and iteration budget, and the same
+// arithmetic, so its iterate, residual and iteration count equal a pcg() call
+// on that case alone. A case that has met its tolerance is frozen (its x and
+// r stay) while the others continue. b and x are case-major (nv x n).
+void pcg_block(Context& c, int nv, const double* b, double* x_io, double tol, int max_iterations,
+               std::vector<slx_pcg_result>& out) {
+  if (nv < 1) fail(SLX_CONFIG_ERROR, "pcg_block: at least one load case");
+  if (c.amg_opt.precision != 0) fail(SLX_CONFIG_ERROR, "pcg_block: the block path runs the FP64 V-cycle only");
+  if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: apply() called before setup()");
+  finish_coarse(c);
+  ensure_block_workspace(c, nv);
+  const long long n = c.K.rows;
+  hipStream_t s = c.stream;
+  const int nb = op_dot_blocks(c);
+  double* scal = c.vscal.p;
+  double* part = c.vpart.p;
+  double* x = c.vx.p;  // the iterates (the graphs bind this buffer, not the caller's)
+  k::copy(n * nv, x_io, x, s);
+  // pinned host memory: 16 scalars per case, then the active flags
+  double* hb = c.host_buffer(16 * (size_t)nv + (size_t)nv);
+  int* hflags = reinterpret_cast<int*>(hb + 16 * (size_t)nv);
+  auto read_scalars = [&](int slot) {  // scal[16 v + slot] of every case to hb
+    SLX_HIP(hipMemcpyAsync(hb, scal, 16 * (size_t)nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    SLX_HIP(hipStreamSynchronize(s));
+    std::vector<double> v((size_t)nv);
+    for (int i = 0; i < nv; ++i) v[(size_t)i] = hb[16 * i + slot];
+    return v;
+  };
+  auto apply_dot = [&] {  // q = K p and the partials of p.q, per case
+    Timed t(c, "block_spmv");
+    if (c.mf) {
+      for (int v = 0; v < nv; ++v) op_apply_dot(c, c.vp.p + v * n, c.vq.p + v * n, part + (size_t)v * nb);
+    } else {
+      k::spmv_dot(c.K, c.vp.p, c.vq.p, part, s, nv);
+    }
+  };
+  auto body_update = [&] {
+    k::reduce(nb, part, scal + S_PQ, 0, s, nv, 16);
+    k::pcg_update(n, x, c.vr.p, c.vp.p, c.vq.p, scal, S_RZ, S_PQ, part, scal + S_RR, s, nv, c.vactive.p);
+  };
+  auto body_precond = [&] {
+    block_cycle64(c, 0, c.vr.p, c.vz.p, nv);
+    k::precond_tail(n, nullptr, c.vr.p, c.vz.p, scal, S_RZ, S_RZN, S_TMP, false, part, s, nv);
+    k::pcg_direction(n, c.vp.p, c.vz.p, scal, S_TMP, -1, s, nv);
+  };
+  const bool graphs = graphs_enabled();
+  if (graphs && !c.g_vupdate) {
+    c.g_vupdate = capture_graph(c, body_update);
+    c.g_vprecond = capture_graph(c, body_precond);
+  }
+  out.assign((size_t)nv, slx_pcg_result{0, 0.0, 0});
+  // ||b|| per case
+  k::dot(n, b, b, part, scal + S_TMP, s, nv, 16);
+  std::vector<double> bnorm = read_scalars(S_TMP), target((size_t)nv), best((size_t)nv,
+                                                                                std::numeric_limits<double>::max());
+  std::vector<int> total((size_t)nv, 0);
+  std::vector<char> done((size_t)nv, 0), rec_conv((size_t)nv, 0), loop_conv((size_t)nv, 0), live((size_t)nv, 0);
+  for (int v = 0; v < nv; ++v) {
+    bnorm[(size_t)v] = std::sqrt(bnorm[(size_t)v]);
+    target[(size_t)v] = tol * bnorm[(size_t)v];
+    if (bnorm[(size_t)v] == 0.0) {
+      k::fill(n, 0.0, x + v * n, s);  // as pcg(): a zero load gives x = 0
+      out[(size_t)v].converged = 1;
+      done[(size_t)v] = 1;
+    }
+  }
+  auto upload_flags = [&] {
+    for (int v = 0; v < nv; ++v) hflags[v] = live[(size_t)v] ? 1 : 0;
+    SLX_HIP(hipMemcpyAsync(c.vactive.p, hflags, sizeof(int) * (size_t)nv, hipMemcpyHostToDevice, s));
+  };
+  auto true_residuals = [&] {  // r = b - K x and r.r per case
+    block_residual(c, 0, x, b, c.vr.p, nv);
+    k::dot(n, c.vr.p, c.vr.p, part, scal + S_TMP, s, nv, 16);
+    return read_scalars(S_TMP);
+  };
+  hipEvent_t ev = c.take_event();
+  struct EventBack {
+    Context& c;
+    hipEvent_t e;
+    ~EventBack() { c.event_pool.push_back(e); }
+  } event_back{c, ev};
+  for (int restart = 0; restart < 3; ++restart) {
+    bool any = false;
+    for (int v = 0; v < nv; ++v) any = any || !done[(size_t)v];
+    if (!any) break;
+    const std::vector<double> rr0 = true_residuals();
+    bool any_live = false;
+    for (int v = 0; v < nv; ++v) {
+      live[(size_t)v] = 0;
+      loop_conv[(size_t)v] = 0;
+      if (done[(size_t)v]) continue;
+      const double rn = std::sqrt(rr0[(size_t)v]);
+      out[(size_t)v].relative_residual = rn / bnorm[(size_t)v];
+      if (rn <= target[(size_t)v]) {
+        out[(size_t)v].converged = 1;
+        done[(size_t)v] = 1;
+      } else {
+        live[(size_t)v] = 1;
+        any_live = true;
+      }
+    }
+    if (!any_live) break;
+    std::vector<char> in_round = live;
+    block_cycle64(c, 0, c.vr.p, c.vz.p, nv);
+    k::copy(n * nv, c.vz.p, c.vp.p, s);
+    k::dot(n, c.vr.p, c.vz.p, part, scal + S_RZ, s, nv, 16);
+    std::vector<char> sent((size_t)nv, 2);
+    for (;;) {
+      bool running = false;
+      for (int v = 0; v < nv; ++v) {
+        if (live[(size_t)v] && total[(size_t)v] >= max_iterations) live[(size_t)v] = 0;  // budget spent
+        running = running || live[(size_t)v];
+      }
+      if (!running) break;
+      if (live != sent) {
+        upload_flags();
+        sent = live;
+      }
+      apply_dot();
+      if (graphs) SLX_HIP(hipGraphLaunch(c.g_vupdate, s)); else body_update();
+      for (int v = 0; v < nv; ++v) total[(size_t)v] += live[(size_t)v] ? 1 : 0;
+      SLX_HIP(hipMemcpyAsync(hb, scal, 16 * (size_t)nv * sizeof(double), hipMemcpyDeviceToHost, s));
+      SLX_HIP(hipEventRecord(ev, s));
+      if (graphs) SLX_HIP(hipGraphLaunch(c.g_vprecond, s)); else body_precond();
+      {
+        hipError_t q;
+        while ((q = hipEventQuery(ev)) == hipErrorNotReady) {
+        }
+        SLX_HIP(q);
+      }
+      for (int v = 0; v < nv; ++v) {
+        if (!live[(size_t)v]) continue;
+        const double pq = hb[16 * v + S_PQ];
+        if (!(pq > 0.0) || !std::isfinite(pq)) {
+          std::ostringstream os;
+          os << "preconditioned CG broke down at iteration " << total[(size_t)v] << " of load case " << v
+             << " (p^T A p = " << pq
+             << "): the stiffness matrix or the multigrid preconditioner is not positive definite. Check the "
+                "supports, or use a direct solver to confirm";
+          fail(SLX_SOLVER_ERROR, os.str());
+        }
+        if (std::sqrt(hb[16 * v + S_RR]) <= target[(size_t)v]) {
+          loop_conv[(size_t)v] = 1;
+          live[(size_t)v] = 0;
+        }
+      }
+    }
+    const std::vector<double> rr = true_residuals();
+    for (int v = 0; v < nv; ++v) {
+      if (!in_round[(size_t)v]) continue;
+      slx_pcg_result& r = out[(size_t)v];
+      r.relative_residual = std::sqrt(rr[(size_t)v]) / bnorm[(size_t)v];
+      if (r.relative_residual <= tol) {
+        r.converged = 1;
+        done[(size_t)v] = 1;
+      } else if (!loop_conv[(size_t)v]) {
+        done[(size_t)v] = 1;  // iteration budget spent
+      } else {
+        rec_conv[(size_t)v] = 1;
+        if (!(r.relative_residual < 0.5 * best[(size_t)v])) done[(size_t)v] = 1;
+        best[(size_t)v] = r.relative_residual;
+      }
+    }
+  }
+  for (int v = 0; v < nv; ++v) {
+    if (!out[(size_t)v].converged && rec_conv[(size_t)v]) out[(size_t)v].converged = 1;
+    out[(size_t)v].iterations = total[(size_t)v];
+  }
+  k::copy(n * nv, x, x_io, s);
+  c.flush_timers();
 }
 
 }  // namespace slx

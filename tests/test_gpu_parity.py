@@ -503,3 +503,77 @@ def test_matrix_core_level0_products_match_the_vector_path(ctx, monkeypatch):
           f"Chebyshev step {1e3 * msv:.1f} / {1e3 * msm:.1f} us", flush=True)
     assert cvv and cvm and dz <= 1e-4 and abs(itm - itv) <= 1
     assert b @ xm == pytest.approx(b @ xv, rel=1e-10)
+
+
+def _three_case_model(kind):
+    """Three load cases: the model's own, a second load elsewhere (another
+    iteration count) and an unloaded case (zero right-hand side)."""
+    if kind == "q4":
+        m = M.mbb_beam(120, 40)
+        c = m.add_load_case(0.5)
+        m.add_point_load(c, m.select_nearest([60.0, 40.0]), [0.3, -1.0])
+    else:
+        m = M.cantilever_3d(24, 12, 12)
+        c = m.add_load_case(0.5)
+        m.add_point_load(c, m.select_box(xmin=24.0, ymin=12.0), [0.0, 0.0, 1.0])
+    m.add_load_case(0.25)
+    return m
+
+
+@pytest.mark.parametrize("kind", ["hex8_matrix_free", "hex8_assembled", "q4"])
+def test_block_load_cases_equal_case_by_case_solves(ctx, kind, monkeypatch):
+    """The load cases of an evaluation solved as one block PCG (one operator
+    product and one V-cycle pass per iteration for every case still iterating)
+    give each case's case-by-case result bit for bit: displacements, iteration
+    counts, compliance and sensitivities, over three evaluations that warm-start
+    from the previous solutions, with one case unloaded."""
+    print(f"test_block_load_cases_equal_case_by_case_solves[{kind}]", flush=True)
+    if kind == "hex8_assembled":
+        monkeypatch.setenv("SPARLAB_NO_MATRIX_FREE", "1")
+    m = _three_case_model("q4" if kind == "q4" else "hex8")
+    designs = [M.seeded_density(m.nelem, seed=31 + k) for k in range(3)]
+    out = {}
+    for mode in ("block", "case_by_case"):
+        if mode == "case_by_case":
+            monkeypatch.setenv("SPARLAB_NO_BLOCK_CASES", "1")
+        c = N.Context(0)
+        c.timing(True)
+        g = N.HipProblem(m, ctx=c)
+        runs = []
+        for x in designs:
+            ev = g.evaluate(x, tol=1e-10)
+            runs.append((ev, [g.solution(l) for l in range(3)]))
+        out[mode] = runs
+        launches = c.timer("block_spmv")[1]
+        print(f"{kind} {mode}: iterations {[r[0]['iterations'] for r in runs]}, block products {launches}",
+              flush=True)
+        assert (launches > 0) == (mode == "block")
+        c.close()
+        monkeypatch.delenv("SPARLAB_NO_BLOCK_CASES", raising=False)
+    for (eb, ub), (es, us) in zip(out["block"], out["case_by_case"]):
+        assert eb["iterations"] == es["iterations"]
+        assert eb["compliance"] == es["compliance"]
+        assert np.array_equal(eb["dc_dx"], es["dc_dx"])
+        for a, b in zip(ub, us):
+            assert np.array_equal(a, b)
+        assert not ub[2].any()  # the unloaded case
+
+
+def test_block_load_cases_oc_run_equals_case_by_case(monkeypatch):
+    """A short multi-case OC run (the block PCG inside every evaluation) gives
+    the case-by-case run's history and design bit for bit."""
+    print("test_block_load_cases_oc_run_equals_case_by_case", flush=True)
+    m = _three_case_model("q4")
+    res = {}
+    for mode in ("block", "case_by_case"):
+        if mode == "case_by_case":
+            monkeypatch.setenv("SPARLAB_NO_BLOCK_CASES", "1")
+        c = N.Context(0)
+        g = N.HipProblem(m, ctx=c)
+        res[mode] = g.run_oc(m.initial_design(), 12, tol=1e-10)
+        c.close()
+        monkeypatch.delenv("SPARLAB_NO_BLOCK_CASES", raising=False)
+    (xb, hb, ib, cb), (xs, hs, is_, cs) = res["block"], res["case_by_case"]
+    assert ib == is_ and cb == cs
+    keep = [k for k in range(hb.shape[1]) if k != N.H_SECONDS]  # wall-clock seconds differ
+    assert np.array_equal(xb, xs) and np.array_equal(hb[:, keep], hs[:, keep])
