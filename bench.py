@@ -347,79 +347,113 @@ def rooflines(g, m, timers):
     return out
 
 
-def batch_main(args):
-    """The batched workload: a 240 x 160 two-bolt bracket's three load cases and 32
-    MBB designs of seeded sizes, one block-diagonal system, segmented CG."""
+def cases_leg(args, prec, tol, log):
+    """The bracket's three load cases (vertical, lateral, torsional) as one block
+    right-hand side against the same evaluation solving them one at a time: each
+    timed step is one evaluation from a zero start (filter, SIMP, assembly, AMG
+    numeric setup, the three solves, sensitivities); solve_seconds is the PCG part."""
+    from sparlab_amd import batch as B
+    from sparlab_amd import model as M
+    from sparlab_amd import native as N
+
+    br = B.bracket_2d()
+    x = M.seeded_density(br.nelem, seed=20261203)
+    out = {}
+    for mode in ("block", "one_at_a_time"):
+        if mode == "one_at_a_time":
+            os.environ["SPARLAB_NO_BLOCK_CASES"] = "1"
+        ctx = N.Context(0)
+        g = N.HipProblem(br, ctx=ctx)
+        g.assemble()
+        g.amg_setup(precision=prec)
+        for _ in range(max(2, args.warmup // 4)):
+            g.evaluate(x, tol=tol, warm_start=False)
+        ctx.synchronize()
+        solve, its, comp = 0.0, 0, None
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ev = g.evaluate(x, tol=tol, warm_start=False)
+            solve += ev["solve_seconds"]
+            its = ev["iterations"]
+            comp = ev["compliance"]
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        out[mode] = dict(eval_ms=1e3 * wall, solve_ms=1e3 * solve / args.steps, iterations=its, compliance=comp)
+        log(f"bench[cases]: {mode}: evaluation {1e3 * wall:.2f} ms, three solves {1e3 * solve / args.steps:.2f} ms, "
+            f"{its} PCG iterations in all, compliance {comp!r}")
+        ctx.close()
+        os.environ.pop("SPARLAB_NO_BLOCK_CASES", None)
+    assert out["block"]["compliance"] == out["one_at_a_time"]["compliance"], "block and case-by-case differ"
+    assert out["block"]["iterations"] == out["one_at_a_time"]["iterations"]
+    return br, out
+
+
+def designs_leg(args, prec, tol, log):
+    """32 MBB designs of seeded uneven sizes as one block-diagonal system and one
+    segmented CG, against the same designs solved one at a time."""
     from sparlab_amd import batch as B
     from sparlab_amd import native as N
 
-    tol = 1e-8
-    prec = 1 if args.precision == "mixed" else 0
     ctx = N.Context(0)
     t0 = time.perf_counter()
-    bt = B.assemble(B.c4_parts(), 2, ctx=ctx)
+    parts = B.c4_parts(bracket=False)  # the MBB designs (the bracket's cases run as a block)
+    bt = B.assemble(parts, 2, ctx=ctx)
     nseg = len(bt.seg_ptr) - 1
-    print(f"bench[batch]: {nseg} segments, {len(bt.b)} unknowns, {len(bt.val)} nonzeros, assembled in "
-          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    log(f"bench[designs]: {nseg} segments, {len(bt.b)} unknowns, {len(bt.val)} nonzeros, assembled in "
+        f"{time.perf_counter() - t0:.1f} s")
     s = B.solver_for(bt, tol=tol, ctx=ctx, precision=prec)  # first setup: aggregation, untimed
-    for _ in range(args.warmup):
+    log("bench[designs]: hierarchy built")
+    for _ in range(max(1, args.warmup // 4)):
         s.solve_segmented(bt.seg_ptr, bt.b)
     ctx.synchronize()
+    log("bench[designs]: warm")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s.factorize(bt.ptr, bt.col, bt.val)  # values upload + numeric setup (aggregates reused)
     ctx.synchronize()
     t_setup = (time.perf_counter() - t0) / args.steps
-    ctx.timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         x, res = s.solve_segmented(bt.seg_ptr, bt.b)
     ctx.synchronize()
     t_pcg = (time.perf_counter() - t0) / args.steps
-    spmv_ms, spmv_n = ctx.timer("spmv")
-    ctx.timing(False)
     its = [r[0] for r in res]
     assert all(r[2] for r in res), "a segment did not converge"
-    print(f"bench[batch]: setup {1e3 * t_setup:.1f} ms, PCG {1e3 * t_pcg:.1f} ms per batch; iterations {its}",
-          file=sys.stderr, flush=True)
-    # the same segments one at a time (each block factorized alone, untimed; solves timed)
-    import scipy.sparse as sp
+    log(f"bench[designs]: setup {1e3 * t_setup:.1f} ms, PCG {1e3 * t_pcg:.1f} ms per batch; iterations {its}")
+    return dict(segments=nseg, unknowns=int(len(bt.b)), nonzeros=int(len(bt.val)), pcg_ms=1e3 * t_pcg,
+                setup_ms=1e3 * t_setup, iterations=its)
 
-    K = sp.csr_matrix((bt.val, bt.col, bt.ptr))
-    singles = []
-    for g in range(nseg):
-        a, b = int(bt.seg_ptr[g]), int(bt.seg_ptr[g + 1])
-        Kg = K[a:b, a:b].tocsr()
-        nodes = bt.unknowns[a:b] // 2
-        n0 = int(nodes.min())
-        one = N.HipAmgCgSolver(tolerance=tol, ctx=N.Context(0), precision=prec)  # a context per matrix
-        one.set_layout(2, bt.coords[n0:int(nodes.max()) + 1], bt.unknowns[a:b] - 2 * n0, dofs_per_node=2)
-        one.factorize(Kg.indptr, Kg.indices, Kg.data)
-        singles.append((one, bt.b[a:b]))
-    for one, bg in singles:
-        one.solve(bg)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        for one, bg in singles:
-            one.solve(bg)
-    ctx.synchronize()
-    t_single = (time.perf_counter() - t0) / args.steps
-    print(f"bench[batch]: one at a time {1e3 * t_single:.1f} ms per batch", file=sys.stderr, flush=True)
+
+def batch_main(args):
+    """The batched workload: a 240 x 160 two-bolt bracket's three load cases as one
+    block right-hand side, and (with SPARLAB_BATCH_BENCH=1) 32 MBB designs of
+    seeded sizes as one segmented CG."""
+    tol = 1e-8
+    prec = 1 if args.precision == "mixed" else 0
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    br, cases = cases_leg(args, prec, tol, log)
+    blk, one = cases["block"], cases["one_at_a_time"]
+    designs = designs_leg(args, prec, tol, log) if os.environ.get("SPARLAB_BATCH_BENCH") == "1" else None
+    solves = 3 + (designs["segments"] if designs else 0)
+    ms = blk["solve_ms"] + (designs["pcg_ms"] if designs else 0.0)
     out = {
-        "metric": "batched Q4 AMG-PCG segment solves/s @rtol 1e-8 (bracket 3 load cases + 32 MBB designs)",
-        "value": nseg / t_pcg, "unit": "segment solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * t_pcg, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": "batched Q4 AMG-PCG solves/s @rtol 1e-8 (bracket 240x160, 3 load cases as a block RHS"
+                  + (" + 32 MBB designs)" if designs else ")"),
+        "value": solves / (1e-3 * ms), "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64+f32" if prec else "f64", "data": "synthetic",
-        "config": {"workload": f"block-diagonal batch: bracket 240x160 x 3 load cases + 32 MBB designs "
-                               f"(numpy PCG64(4242) sizes, nx 60..240), seeded densities, one segmented CG",
-                   "segments": nseg, "unknowns": int(len(bt.b)), "nonzeros": int(len(bt.val)),
-                   "pcg_ms_per_batch": 1e3 * t_pcg, "setup_ms_per_batch": 1e3 * t_setup,
-                   "with_setup_segment_solves_per_s": nseg / (t_pcg + t_setup),
-                   "one_at_a_time_ms_per_batch": 1e3 * t_single,
-                   "speedup_vs_one_at_a_time": t_single / t_pcg,
-                   "iterations": its,
-                   "spmv_avg_ms": spmv_ms / max(spmv_n, 1)},
+        "config": {"workload": f"two-bolt bracket Q4 240x160 ({br.ndofs} DOFs), vertical, lateral and torsional "
+                               f"unit loads, seeded densities, SIMP p 3; one block PCG for the three cases"
+                               + ("; 32 MBB designs (numpy PCG64(4242) sizes, nx 60..240) as one segmented CG"
+                                  if designs else ""),
+                   "cases_block_solve_ms": blk["solve_ms"], "cases_one_at_a_time_solve_ms": one["solve_ms"],
+                   "cases_block_speedup": one["solve_ms"] / blk["solve_ms"],
+                   "cases_block_evaluation_ms": blk["eval_ms"], "cases_one_at_a_time_evaluation_ms": one["eval_ms"],
+                   "cases_iterations": blk["iterations"], "cases_compliance": blk["compliance"],
+                   "designs": designs},
     }
     print(json.dumps(out))
 
@@ -462,9 +496,6 @@ def main():
         sys.exit(2)
 
     if args.workload == "batch":
-        if os.environ.get("SPARLAB_BATCH_BENCH") != "1":
-            sys.exit("bench.py: the batch workload is disabled: a GPU run of it ended with no output and no cause "
-                     "has been found; set SPARLAB_BATCH_BENCH=1 to run it anyway")
         batch_main(args)
         return
 

@@ -106,17 +106,25 @@ def assemble(parts, dim, ctx=None):
                  seg_ptr=np.array(seg, dtype=np.int32), b=np.concatenate(bs), dim=dim, labels=labels)
 
 
-def c4_parts(count=32, seed=4242, density_seed=20261203, penalty=3.0, emin=1e-9):
-    """The batched workload: the bracket's three load cases plus `count` MBB designs,
-    each with seeded densities in [0.001, 1] through the SIMP law."""
-    def fac(m, s):
-        x = M.seeded_density(m.nelem, seed=s)
-        return emin + (1.0 - emin) * x ** penalty
-    br = bracket_2d()
-    parts = [(br, fac(br, density_seed), [0, 1, 2])]
-    for k, (nx, ny) in enumerate(mbb_sizes(count, seed)):
-        m = M.mbb_beam(nx, ny)
-        parts.append((m, fac(m, density_seed + 1 + k), [0]))
+def _factors(m, seed, penalty, emin):
+    x = M.seeded_density(m.nelem, seed=seed)
+    return emin + (1.0 - emin) * x ** penalty
+
+
+def mbb_designs(count=32, seed=4242, density_seed=20261203):
+    """`count` MBB designs of seeded uneven sizes and the seed of each one's densities."""
+    return [(M.mbb_beam(nx, ny), density_seed + 1 + k) for k, (nx, ny) in enumerate(mbb_sizes(count, seed))]
+
+
+def c4_parts(count=32, seed=4242, density_seed=20261203, penalty=3.0, emin=1e-9, bracket=True):
+    """The batched workload: the bracket's three load cases (when `bracket`) plus
+    `count` MBB designs, each with seeded densities in [0.001, 1] through the SIMP law."""
+    parts = []
+    if bracket:
+        br = bracket_2d()
+        parts.append((br, _factors(br, density_seed, penalty, emin), [0, 1, 2]))
+    for m, s in mbb_designs(count, seed, density_seed):
+        parts.append((m, _factors(m, s, penalty, emin), [0]))
     return parts
 
 

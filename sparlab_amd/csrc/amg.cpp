@@ -1399,7 +1399,8 @@ void amg_setup(Context& c, bool defer_coarse) {
   const slx_amg_options& o = c.amg_opt;
   if (o.smoother != 0) fail(SLX_CONFIG_ERROR, "multigrid: only the Chebyshev smoother runs on the GPU");
   ensure_workspace(c);
-  const bool same = c.amg_ready && c.fp_rows == n && c.fp_nnz == c.K.nnz && c.fp_layout_version == c.layout_version;
+  const bool same = c.amg_ready && c.fp_rows == n && c.fp_nnz == c.K.nnz && c.fp_layout_version == c.layout_version &&
+                    c.fp_pattern_version == c.k_pattern_version;
   const bool reuse = o.reuse_aggregates && same;
   c.amg_ready = false;  // until this setup completes
   // device flags: [0] a Galerkin pattern mismatch, [1 + l] level l's first
@@ -1602,6 +1603,7 @@ void amg_setup(Context& c, bool defer_coarse) {
   c.fp_rows = n;
   c.fp_nnz = c.K.nnz;
   c.fp_layout_version = c.layout_version;
+  c.fp_pattern_version = c.k_pattern_version;
   c.amg_ready = true;
   c.reused = reuse;
   if (!defer_coarse) finish_coarse(c);

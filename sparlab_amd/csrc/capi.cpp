@@ -210,10 +210,9 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   int most_its = 0;
   // several load cases (Sensitivity.cpp:169-176 solves them one by one): one
   // block PCG drives them through each operator and V-cycle pass together,
-  // each case's result equal to its own solve; the learned warm start and the
-  // FP32 V-cycle keep the case-by-case loop
-  const bool block = c.ncases > 1 && block_cases_enabled() && c.amg_opt.precision == 0 &&
-                     !(c.ws && c.ws->opt.enable);
+  // each case's result equal to its own solve; the learned warm start, which
+  // predicts one case at a time, keeps the case-by-case loop
+  const bool block = c.ncases > 1 && block_cases_enabled() && !(c.ws && c.ws->opt.enable);
   std::vector<slx_pcg_result> block_res;
   if (block) {
     if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "AMG-CG: solve called before factorize");
@@ -572,6 +571,7 @@ slx_status slx_model_finalize(slx_context* ctx) {
     c.htvol_valid = false;
     const HCsr pat = structured_kff_pattern(c);
     c.K.upload_pattern(pat, c.stream);
+    ++c.k_pattern_version;
     c.d_free.upload(c.free_dofs, c.stream);
     c.d_reduced.upload(c.reduced, c.stream);
     c.d_ke0.upload(c.ke0, c.stream);
@@ -701,18 +701,47 @@ slx_status slx_factorize_csr(slx_context* ctx, int n, const int* ptr, const int*
                              const slx_amg_options* opt) {
   return guarded(ctx, [&](Context& c) {
     if (n <= 0) fail(SLX_SOLVER_ERROR, "multigrid: the system matrix must be square and non-empty");
+    if (!ptr || !col || !val || !unknowns) fail(SLX_CONFIG_ERROR, "factorize: null matrix or layout array");
+    if (dim != 2 && dim != 3) fail(SLX_CONFIG_ERROR, "factorize: the layout dimension must be 2 or 3");
+    if (nnodes < 1 || !coords) fail(SLX_CONFIG_ERROR, "factorize: the layout needs its node coordinates");
+    // the host checks every index the device and the aggregation will follow
+    if (ptr[0] != 0) fail(SLX_CONFIG_ERROR, "factorize: CSR row pointers must start at 0");
+    for (int i = 0; i < n; ++i)
+      if (ptr[i + 1] < ptr[i]) fail(SLX_CONFIG_ERROR, "factorize: CSR row pointers must not decrease");
     HCsr h;
     h.rows = h.cols = n;
     h.ptr.assign(ptr, ptr + n + 1);
     h.col.assign(col, col + ptr[n]);
     h.val.assign(val, val + ptr[n]);
+    unsigned long long hash = 1469598103934665603ULL;  // FNV-1a over the pattern
+    auto mix = [&](int v) { hash = (hash ^ (unsigned long long)(unsigned)v) * 1099511628211ULL; };
+    for (int i = 0; i < n; ++i) {
+      const int k0 = h.ptr[(size_t)i], k1 = h.ptr[(size_t)i + 1];
+      mix(k1);
+      if (k1 > k0 && (h.col[(size_t)k0] < 0 || h.col[(size_t)k1 - 1] >= n))
+        fail(SLX_CONFIG_ERROR, "factorize: a CSR column index lies outside the matrix");
+      for (int k2 = k0; k2 < k1; ++k2) {
+        mix(h.col[(size_t)k2]);
+        if (k2 > k0 && h.col[(size_t)k2] <= h.col[(size_t)k2 - 1])
+          fail(SLX_SOLVER_ERROR, "CSR rows must be sorted by column without duplicates");
+      }
+    }
+    const int per_node = ndpn > 0 ? ndpn : dim;
     for (int i = 0; i < n; ++i)
-      for (int k2 = h.ptr[(size_t)i] + 1; k2 < h.ptr[(size_t)i + 1]; ++k2)
-        if (h.col[(size_t)k2] <= h.col[(size_t)k2 - 1]) fail(SLX_SOLVER_ERROR, "CSR rows must be sorted by column without duplicates");
+      if (unknowns[i] < 0 || (long long)unknowns[i] >= (long long)nnodes * per_node)
+        fail(SLX_CONFIG_ERROR, "factorize: a layout unknown lies outside the layout's nodes");
     const bool same_unknowns =
         c.layout_unknowns.size() == (size_t)n && std::equal(unknowns, unknowns + n, c.layout_unknowns.begin());
-    const bool same_pattern = c.K.rows == n && c.K.nnz == h.nnz() && same_unknowns;
-    if (same_pattern) c.K.upload_values(h.val.data(), c.stream); else c.K.upload(h, c.stream);
+    // only the values change when the pattern is the one K already holds
+    const bool same_pattern = c.K.rows == n && c.K.nnz == h.nnz() && same_unknowns && c.k_pattern_hash == hash &&
+                              c.k_source == Context::KSource::caller;
+    if (same_pattern) {
+      c.K.upload_values(h.val.data(), c.stream);
+    } else {
+      c.K.upload(h, c.stream);
+      ++c.k_pattern_version;
+    }
+    c.k_pattern_hash = hash;
     c.mf = false;  // a caller's matrix: products use it as given
     // the structured model's K_ff pattern, solutions and adjoint slots no longer
     // describe K: the structured entry points refuse until slx_model_finalize

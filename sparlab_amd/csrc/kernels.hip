@@ -368,9 +368,14 @@ __global__ void __launch_bounds__(1024) k_precond_finish(int m, const double* __
 // x = d = dinv b / theta.
 __global__ void k_cheb_first_zero_from64(int n, const float* __restrict__ dinv, const double* __restrict__ r,
                                          double theta, float* __restrict__ b, float* __restrict__ x,
-                                         float* __restrict__ d) {
+                                         float* __restrict__ d, long long vs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const long long o = blockIdx.y * vs;  // load case blockIdx.y
+  r += o;
+  b += o;
+  x += o;
+  d += o;
   const float bi = (float)r[i];
   b[i] = bi;
   const double di = (double)dinv[i] * (double)bi / theta;
@@ -763,9 +768,9 @@ void launch_spmv(const DCsr& A, const VAL* val, const VT* x, const VT* b, VT* y,
 
 // FP32-vector products read the bfloat16 copy of the values when there is one
 template <int MODE>
-void launch_spmv32(const DCsr& A, const float* x, const float* b, float* y, hipStream_t s) {
-  if (A.val16.p) launch_spmv<MODE, float, uint16_t>(A, A.val16.p, x, b, y, nullptr, s);
-  else launch_spmv<MODE, float>(A, A.val32.p, x, b, y, nullptr, s);
+void launch_spmv32(const DCsr& A, const float* x, const float* b, float* y, hipStream_t s, int nv) {
+  if (A.val16.p) launch_spmv<MODE, float, uint16_t>(A, A.val16.p, x, b, y, nullptr, s, nv);
+  else launch_spmv<MODE, float>(A, A.val32.p, x, b, y, nullptr, s, nv);
 }
 
 }  // namespace
@@ -782,10 +787,14 @@ void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s, int nv) 
 void residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s, int nv) {
   launch_spmv<1, double>(A, A.val.p, x, b, r, nullptr, s, nv);
 }
-void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s) { launch_spmv32<0>(A, x, nullptr, y, s); }
-void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s) { launch_spmv32<2>(A, x, nullptr, y, s); }
-void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s) {
-  launch_spmv32<1>(A, x, b, r, s);
+void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s, int nv) {
+  launch_spmv32<0>(A, x, nullptr, y, s, nv);
+}
+void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s, int nv) {
+  launch_spmv32<2>(A, x, nullptr, y, s, nv);
+}
+void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s, int nv) {
+  launch_spmv32<1>(A, x, b, r, s, nv);
 }
 int spmv_dot_blocks(const DCsr& A) { return grid8(A.rows, kBlock / lanes_for(A)); }
 void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s, int nv) {
@@ -826,11 +835,11 @@ void cheb_step(const DCsr& A, const double* dinv, const double* r_in, double* r_
   cheb_step_t<double>(A, A.val.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s, nv);
 }
 void cheb_step32(const DCsr& A, const float* dinv, const float* r_in, float* r_out, const float* d_in, float* d_out,
-                 float* x, double c1, double c2, hipStream_t s) {
+                 float* x, double c1, double c2, hipStream_t s, int nv) {
   if (A.val16.p)
-    cheb_step_t<float, uint16_t>(A, A.val16.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s);
+    cheb_step_t<float, uint16_t>(A, A.val16.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s, nv);
   else
-    cheb_step_t<float>(A, A.val32.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s);
+    cheb_step_t<float>(A, A.val32.p, dinv, r_in, r_out, d_in, d_out, x, c1, c2, s, nv);
 }
 void cheb_first_zero(int n, const double* dinv, const double* b, double theta, double* x, double* d, hipStream_t s,
                      int nv) {
@@ -840,17 +849,19 @@ void cheb_first(int n, const double* dinv, const double* r, double theta, double
                 int nv) {
   if (n) k_cheb_first<double><<<dim3(blocks_for(n, kBlock), nv), kBlock, 0, s>>>(n, dinv, r, theta, x, d, n);
 }
-void cheb_first_zero32(int n, const float* dinv, const float* b, double theta, float* x, float* d, hipStream_t s) {
-  if (n) k_cheb_first_zero<float><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, b, theta, x, d, 0);
+void cheb_first_zero32(int n, const float* dinv, const float* b, double theta, float* x, float* d, hipStream_t s,
+                       int nv) {
+  if (n) k_cheb_first_zero<float><<<dim3(blocks_for(n, kBlock), nv), kBlock, 0, s>>>(n, dinv, b, theta, x, d, n);
 }
-void cheb_first32(int n, const float* dinv, const float* r, double theta, float* x, float* d, hipStream_t s) {
-  if (n) k_cheb_first<float><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, r, theta, x, d, 0);
+void cheb_first32(int n, const float* dinv, const float* r, double theta, float* x, float* d, hipStream_t s,
+                  int nv) {
+  if (n) k_cheb_first<float><<<dim3(blocks_for(n, kBlock), nv), kBlock, 0, s>>>(n, dinv, r, theta, x, d, n);
 }
 void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t s, int nv) {
   if (n) k_dense_gemv<double><<<dim3(blocks_for((long long)n * 64, kBlock), nv), kBlock, 0, s>>>(n, M, b, y, n);
 }
-void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s) {
-  if (n) k_dense_gemv<float><<<blocks_for((long long)n * 64, kBlock), kBlock, 0, s>>>(n, M, b, y, 0);
+void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s, int nv) {
+  if (n) k_dense_gemv<float><<<dim3(blocks_for((long long)n * 64, kBlock), nv), kBlock, 0, s>>>(n, M, b, y, n);
 }
 
 int reduce_blocks(long long n) { return (int)std::max<long long>(1, std::min<long long>(kReduceBlocks, (n + kBlock - 1) / kBlock)); }
@@ -882,8 +893,8 @@ void precond_tail(long long n, const float* zc, const double* r, double* z, doub
   k_precond_finish<<<nv, 1024, 0, s>>>(nb, partials, scal, i_rz, i_rzn, i_beta, flexible ? 1 : 0);
 }
 void cheb_first_zero_from64(int n, const float* dinv, const double* r, double theta, float* b, float* x, float* d,
-                            hipStream_t s) {
-  if (n) k_cheb_first_zero_from64<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, dinv, r, theta, b, x, d);
+                            hipStream_t s, int nv) {
+  if (n) k_cheb_first_zero_from64<<<dim3(blocks_for(n, kBlock), nv), kBlock, 0, s>>>(n, dinv, r, theta, b, x, d, n);
 }
 void pcg_direction(long long n, double* p, const double* z, const double* scal, int i_new, int i_old, hipStream_t s,
                    int nv) {

@@ -159,6 +159,7 @@ struct Level {
   DBuf<double> x, b, res, d, d2;
   DBuf<float> dinv32, x32, b32, res32, d32, d232;
   DBuf<double> vx, vb, vres, vd, vd2;  // a block of load cases side by side (case-major)
+  DBuf<float> vx32, vb32, vres32, vd32, vd232;
 };
 
 struct TimerSlot {
@@ -208,6 +209,8 @@ struct Context {
   // finalised again)
   enum class KSource { none, assembled, evaluation, modal, caller };
   KSource k_source = KSource::none;
+  unsigned long long k_pattern_hash = 0;  // of a caller matrix's pattern (ptr, col)
+  long long k_pattern_version = 0;        // changes whenever K's pattern is replaced
   bool k_on_host = false;
   // K_ff's values lag the element factors: an evaluation that reused the coarse
   // operators refreshed only level 0 and skipped the assembly (coarse_reuse)
@@ -229,6 +232,7 @@ struct Context {
   bool reused = false;
   long long fp_rows = -1, fp_nnz = -1;
   long long fp_layout_version = -1;  // layout_version at the last setup
+  long long fp_pattern_version = -1;  // k_pattern_version at the last setup
   std::vector<double> layout_coords;  // coords of the DOF layout (nnodes x dim)
   int layout_dim = 0, layout_ndpn = 0, layout_nnodes = 0;
   std::vector<int> layout_unknowns;
@@ -259,7 +263,8 @@ struct Context {
   // case-major vectors, 16 scalars and one active flag per case
   DBuf<double> vx, vr, vz, vp, vq, vpart, vscal;
   DBuf<int> vactive;
-  int block_nv = 0;  // cases the block workspace and graphs were made for
+  int block_nv = 0;          // cases the block workspace and graphs were made for
+  int block_precision = -1;  // and the V-cycle precision
   hipGraphExec_t g_vupdate = nullptr, g_vprecond = nullptr;
   // pinned host memory for small device -> host reads (scalars, per-segment sums)
   double* pinned = nullptr;

@@ -31,9 +31,9 @@ void tent_restrict(const TentBlocks& t, const T* ella, const T* y, T* xc, hipStr
 void spmv(const DCsr& A, const double* x, double* y, hipStream_t s, int nv = 1);
 void spmv_add(const DCsr& A, const double* x, double* y, hipStream_t s, int nv = 1);
 void residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s, int nv = 1);
-void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s);
-void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s);
-void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s);
+void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s, int nv = 1);
+void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s, int nv = 1);
+void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s, int nv = 1);
 // y = A x and per-block partial sums of x.y (spmv_dot_blocks(A) partials)
 int spmv_dot_blocks(const DCsr& A);
 void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s, int nv = 1);
@@ -44,13 +44,15 @@ void spmv_serial(const DCsr& A, const double* x, double* y, hipStream_t s);
 void cheb_step(const DCsr& A, const double* dinv, const double* r_in, double* r_out, const double* d_in, double* d_out,
                double* x, double c1, double c2, hipStream_t s, int nv = 1);
 void cheb_step32(const DCsr& A, const float* dinv, const float* r_in, float* r_out, const float* d_in, float* d_out,
-                 float* x, double c1, double c2, hipStream_t s);
+                 float* x, double c1, double c2, hipStream_t s, int nv = 1);
 void cheb_first_zero(int n, const double* dinv, const double* b, double theta, double* x, double* d, hipStream_t s,
                      int nv = 1);
 void cheb_first(int n, const double* dinv, const double* r, double theta, double* x, double* d, hipStream_t s,
                 int nv = 1);
-void cheb_first_zero32(int n, const float* dinv, const float* b, double theta, float* x, float* d, hipStream_t s);
-void cheb_first32(int n, const float* dinv, const float* r, double theta, float* x, float* d, hipStream_t s);
+void cheb_first_zero32(int n, const float* dinv, const float* b, double theta, float* x, float* d, hipStream_t s,
+                       int nv = 1);
+void cheb_first32(int n, const float* dinv, const float* r, double theta, float* x, float* d, hipStream_t s,
+                  int nv = 1);
 void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t s, int nv = 1);
 // a[n*n] (row-major) = the inverse of the symmetric positive definite CSR matrix
 // (ptr, col, val), by blocked in-place Gauss-Jordan on the device; piv[n] = the
@@ -58,7 +60,7 @@ void dense_gemv(int n, const double* M, const double* b, double* y, hipStream_t 
 int dense_inverse_work(int n);
 void dense_inverse(int n, const int* ptr, const int* col, const double* val, double* a, double* piv, double* work,
                    hipStream_t s);
-void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s);
+void dense_gemv32(int n, const float* M, const float* b, float* y, hipStream_t s, int nv = 1);
 
 // deterministic reductions: partial buffer of at least 2048 doubles
 int reduce_blocks(long long n);
@@ -84,7 +86,7 @@ void pcg_direction(long long n, double* p, const double* z, const double* scal, 
 void precond_tail(long long n, const float* zc, const double* r, double* z, double* scal, int i_rz, int i_rzn,
                   int i_beta, bool flexible, double* partials, hipStream_t s, int nv = 1);
 void cheb_first_zero_from64(int n, const float* dinv, const double* r, double theta, float* b, float* x, float* d,
-                            hipStream_t s);
+                            hipStream_t s, int nv = 1);
 void copy_scalar(double* scal, int from, int to, hipStream_t s);
 void flexible_beta(double* scal, int i_new, int i_cross, int i_old, int i_out, hipStream_t s);
 void to_float(long long n, const double* in, float* out, hipStream_t s);

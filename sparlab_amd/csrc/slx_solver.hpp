@@ -27,7 +27,7 @@ void vcycle(Context& c, const double* r, double* z);
 slx_pcg_result pcg(Context& c, const double* b, double* x, double tol, int max_iterations);
 // The same for nv load cases at once (case-major b and x, nv x n): one operator
 // product and one V-cycle pass per iteration for every case; each case's
-// result equals pcg() on that case alone. FP64 V-cycle only.
+// result equals pcg() on that case alone (FP64 or FP32 V-cycle).
 void pcg_block(Context& c, int nv, const double* b, double* x, double tol, int max_iterations,
                std::vector<slx_pcg_result>& out);
 

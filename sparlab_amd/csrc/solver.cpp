@@ -258,20 +258,76 @@ void cycle32(Context& c, size_t l, const float* b, float* x, bool started = fals
 // The same V-cycle for nv right-hand sides at once: every assembled-operator
 // pass (smoother, residual, transfers, coarse solve) reads its matrix once for
 // all cases, each case summed in the single-case order, so each case's result
-// equals its own cycle64 bit for bit. Level vectors are the block buffers
-// (vb, vx, vres, vd, vd2: nv x n_l, case-major); the matrix-free level-0
-// operator, which reads no matrix, runs once per case.
-void block_residual(Context& c, size_t l, const double* x, const double* b, double* r, int nv) {
+// equals its own cycle bit for bit. Level vectors are the block buffers
+// (nv x n_l, case-major); the matrix-free level-0 operator, which reads no
+// matrix, runs once per case.
+template <typename T>
+struct BlockVecs {
+  DBuf<T>&x, &b, &res, &d, &d2;
+  const T* dinv;
+};
+template <typename T>
+BlockVecs<T> block_vecs(Level& lv);
+template <>
+BlockVecs<double> block_vecs<double>(Level& lv) {
+  return {lv.vx, lv.vb, lv.vres, lv.vd, lv.vd2, lv.dinv.p};
+}
+template <>
+BlockVecs<float> block_vecs<float>(Level& lv) {
+  return {lv.vx32, lv.vb32, lv.vres32, lv.vd32, lv.vd232, lv.dinv32.p};
+}
+// the assembled-operator kernels of either storage, for nv cases
+void csr_residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s, int nv) {
+  k::residual(A, x, b, r, s, nv);
+}
+void csr_residual(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s, int nv) {
+  k::residual32(A, x, b, r, s, nv);
+}
+void csr_apply(const DCsr& A, const double* x, double* y, hipStream_t s, int nv) { k::spmv(A, x, y, s, nv); }
+void csr_apply(const DCsr& A, const float* x, float* y, hipStream_t s, int nv) { k::spmv32(A, x, y, s, nv); }
+void csr_apply_add(const DCsr& A, const double* x, double* y, hipStream_t s, int nv) { k::spmv_add(A, x, y, s, nv); }
+void csr_apply_add(const DCsr& A, const float* x, float* y, hipStream_t s, int nv) { k::spmv_add32(A, x, y, s, nv); }
+void csr_cheb(const DCsr& A, const double* dinv, const double* ri, double* ro, const double* di, double* dout,
+              double* x, double c1, double c2, hipStream_t s, int nv) {
+  k::cheb_step(A, dinv, ri, ro, di, dout, x, c1, c2, s, nv);
+}
+void csr_cheb(const DCsr& A, const float* dinv, const float* ri, float* ro, const float* di, float* dout, float* x,
+              double c1, double c2, hipStream_t s, int nv) {
+  k::cheb_step32(A, dinv, ri, ro, di, dout, x, c1, c2, s, nv);
+}
+void first_zero(int n, const double* dinv, const double* b, double th, double* x, double* d, hipStream_t s, int nv) {
+  k::cheb_first_zero(n, dinv, b, th, x, d, s, nv);
+}
+void first_zero(int n, const float* dinv, const float* b, double th, float* x, float* d, hipStream_t s, int nv) {
+  k::cheb_first_zero32(n, dinv, b, th, x, d, s, nv);
+}
+void first(int n, const double* dinv, const double* r, double th, double* x, double* d, hipStream_t s, int nv) {
+  k::cheb_first(n, dinv, r, th, x, d, s, nv);
+}
+void first(int n, const float* dinv, const float* r, double th, float* x, float* d, hipStream_t s, int nv) {
+  k::cheb_first32(n, dinv, r, th, x, d, s, nv);
+}
+void coarse_solve(Context& c, int n, const double* b, double* x, int nv) {
+  k::dense_gemv(n, c.coarse_inv.p, b, x, c.stream, nv);
+}
+void coarse_solve(Context& c, int n, const float* b, float* x, int nv) {
+  k::dense_gemv32(n, c.coarse_inv32.p, b, x, c.stream, nv);
+}
+
+template <typename T>
+void block_residual(Context& c, size_t l, const T* x, const T* b, T* r, int nv) {
   if (use_mf(c, l)) {
     const long long n = c.levels[l].n;
-    for (int v = 0; v < nv; ++v) level_residual<double>(c, l, x + v * n, b + v * n, r + v * n);
+    for (int v = 0; v < nv; ++v) level_residual<T>(c, l, x + v * n, b + v * n, r + v * n);
   } else {
-    k::residual(op(c, l), x, b, r, c.stream, nv);
+    csr_residual(op(c, l), x, b, r, c.stream, nv);
   }
 }
 
-void block_cheb_steps(Context& c, size_t l, const double* r_first, double* x, int nv) {
+template <typename T>
+void block_cheb_steps(Context& c, size_t l, const T* r_first, T* x, int nv) {
   Level& lv = c.levels[l];
+  BlockVecs<T> bv = block_vecs<T>(lv);
   DCsr& A = op(c, l);
   const Cheb cp = cheb_params(c, lv);
   const long long n = lv.n;
@@ -281,52 +337,75 @@ void block_cheb_steps(Context& c, size_t l, const double* r_first, double* x, in
     const double rho_new = 1.0 / (2.0 * cp.sigma - rho);
     const double c1 = rho_new * rho;
     const double c2 = 2.0 * rho_new / cp.delta;
-    const double* rin = k == 1 ? r_first : lv.vres.p;
+    const T* rin = k == 1 ? r_first : bv.res.p;
     if (use_mf(c, l)) {
       for (int v = 0; v < nv; ++v) {
-        kmf::MfArgs<double> a = mf_base<double>(c);
-        a.x = lv.vd.p + v * n, a.b = rin + v * n, a.y = lv.vres.p + v * n, a.dinv = lv.dinv.p;
-        a.d_out = lv.vd2.p + v * n, a.xs = x + v * n, a.c1 = c1, a.c2 = c2;
+        kmf::MfArgs<T> a = mf_base<T>(c);
+        a.x = bv.d.p + v * n, a.b = rin + v * n, a.y = bv.res.p + v * n, a.dinv = bv.dinv;
+        a.d_out = bv.d2.p + v * n, a.xs = x + v * n, a.c1 = c1, a.c2 = c2;
         kmf::apply(a, 3, s);
       }
     } else {
-      k::cheb_step(A, lv.dinv.p, rin, lv.vres.p, lv.vd.p, lv.vd2.p, x, c1, c2, s, nv);
+      csr_cheb(A, bv.dinv, rin, bv.res.p, bv.d.p, bv.d2.p, x, c1, c2, s, nv);
     }
-    std::swap(lv.vd, lv.vd2);
+    std::swap(bv.d, bv.d2);
     rho = rho_new;
   }
 }
 
-void block_cycle64(Context& c, size_t l, const double* b, double* x, int nv) {
+// started: x and d already hold the zero-guess smoother start
+template <typename T>
+void block_cycle(Context& c, size_t l, const T* b, T* x, int nv, bool started = false) {
   Level& lv = c.levels[l];
+  BlockVecs<T> bv = block_vecs<T>(lv);
   hipStream_t s = c.stream;
   if (l + 1 == c.levels.size()) {
-    k::dense_gemv(lv.n, c.coarse_inv.p, b, x, s, nv);
+    coarse_solve(c, lv.n, b, x, nv);
     return;
   }
   const Cheb cp = cheb_params(c, lv);
   const long long n = lv.n;
-  k::cheb_first_zero(lv.n, lv.dinv.p, b, cp.theta, x, lv.vd.p, s, nv);
-  block_cheb_steps(c, l, b, x, nv);
-  block_residual(c, l, x, b, lv.vres.p, nv);
+  if (!started) first_zero(lv.n, bv.dinv, b, cp.theta, x, bv.d.p, s, nv);
+  block_cheb_steps<T>(c, l, b, x, nv);
+  block_residual<T>(c, l, x, b, bv.res.p, nv);
   Level& nx = c.levels[l + 1];
+  BlockVecs<T> nv_ = block_vecs<T>(nx);
   const long long nc = nx.n;
   if (use_mf(c, l)) {
-    for (int v = 0; v < nv; ++v) restrict_mf<double>(c, lv.vres.p + v * n, lv.vd.p + v * n, nx.vb.p + v * nc);
+    for (int v = 0; v < nv; ++v) restrict_mf<T>(c, bv.res.p + v * n, bv.d.p + v * n, nv_.b.p + v * nc);
   } else {
-    k::spmv(lv.R, lv.vres.p, nx.vb.p, s, nv);
+    csr_apply(lv.R, bv.res.p, nv_.b.p, s, nv);
   }
-  block_cycle64(c, l + 1, nx.vb.p, nx.vx.p, nv);
+  block_cycle<T>(c, l + 1, nv_.b.p, nv_.x.p, nv);
   if (use_mf(c, l)) {
-    for (int v = 0; v < nv; ++v) prolong_mf<double>(c, nx.vx.p + v * nc, lv.vd.p + v * n, x + v * n);
+    for (int v = 0; v < nv; ++v) prolong_mf<T>(c, nv_.x.p + v * nc, bv.d.p + v * n, x + v * n);
   } else {
-    k::spmv_add(lv.P, nx.vx.p, x, s, nv);
+    csr_apply_add(lv.P, nv_.x.p, x, s, nv);
   }
-  block_residual(c, l, x, b, lv.vres.p, nv);
-  k::cheb_first(lv.n, lv.dinv.p, lv.vres.p, cp.theta, x, lv.vd.p, s, nv);
-  block_cheb_steps(c, l, lv.vres.p, x, nv);
+  block_residual<T>(c, l, x, b, bv.res.p, nv);
+  first(lv.n, bv.dinv, bv.res.p, cp.theta, x, bv.d.p, s, nv);
+  block_cheb_steps<T>(c, l, bv.res.p, x, nv);
 }
 
+// z = M^-1 r for nv cases: the FP64 cycle into z (returns null), or the FP32
+// cycle, whose result stays in level 0's FP32 block iterate (returned)
+const float* block_vcycle_raw(Context& c, const double* r, double* z, int nv) {
+  Level& top = c.levels[0];
+  const int n = top.n;
+  if (c.amg_opt.precision >= 1) {
+    if (c.levels.size() > 1) {
+      k::cheb_first_zero_from64(n, top.dinv32.p, r, cheb_params(c, top).theta, top.vb32.p, top.vx32.p,
+                                top.vd32.p, c.stream, nv);
+      block_cycle<float>(c, 0, top.vb32.p, top.vx32.p, nv, true);
+    } else {
+      k::to_float((long long)n * nv, r, top.vb32.p, c.stream);
+      block_cycle<float>(c, 0, top.vb32.p, top.vx32.p, nv);
+    }
+    return top.vx32.p;
+  }
+  block_cycle<double>(c, 0, r, z, nv);
+  return nullptr;
+}
 
 }  // namespace
 
@@ -722,11 +801,21 @@ slx_pcg_result pcg(Context& c, const double* b, double* x_out, double tol, int m
 
 void ensure_block_workspace(Context& c, int nv) {
   const size_t n = (size_t)c.K.rows;
-  if (c.block_nv != nv || c.vr.n != n * (size_t)nv) invalidate_graphs(c);
+  if (c.block_nv != nv || c.block_precision != c.amg_opt.precision || c.vr.n != n * (size_t)nv)
+    invalidate_graphs(c);
   c.block_nv = nv;
+  c.block_precision = c.amg_opt.precision;
   for (DBuf<double>* v : {&c.vx, &c.vr, &c.vz, &c.vp, &c.vq}) v->alloc(n * (size_t)nv);
-  for (Level& lv : c.levels)
-    for (DBuf<double>* v : {&lv.vx, &lv.vb, &lv.vres, &lv.vd, &lv.vd2}) v->alloc((size_t)lv.n * (size_t)nv);
+  const bool f32 = c.amg_opt.precision >= 1;
+  for (Level& lv : c.levels) {
+    const size_t m = (size_t)lv.n * (size_t)nv;
+    for (DBuf<double>* v : {&lv.vx, &lv.vb, &lv.vres, &lv.vd, &lv.vd2}) {
+      if (f32) v->release(); else v->alloc(m);
+    }
+    for (DBuf<float>* v : {&lv.vx32, &lv.vb32, &lv.vres32, &lv.vd32, &lv.vd232}) {
+      if (f32) v->alloc(m); else v->release();
+    }
+  }
   const size_t per = (size_t)std::max({op_dot_blocks(c), k::spmv_dot_blocks(c.K), 2 * k::reduce_blocks((long long)n)});
   if (c.vpart.n < per * (size_t)nv + 8) {
     invalidate_graphs(c);
@@ -752,7 +841,6 @@ void ensure_block_workspace(Context& c, int nv) {
 void pcg_block(Context& c, int nv, const double* b, double* x_io, double tol, int max_iterations,
                std::vector<slx_pcg_result>& out) {
   if (nv < 1) fail(SLX_CONFIG_ERROR, "pcg_block: at least one load case");
-  if (c.amg_opt.precision != 0) fail(SLX_CONFIG_ERROR, "pcg_block: the block path runs the FP64 V-cycle only");
   if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: apply() called before setup()");
   finish_coarse(c);
   ensure_block_workspace(c, nv);
@@ -785,9 +873,11 @@ void pcg_block(Context& c, int nv, const double* b, double* x_io, double tol, in
     k::reduce(nb, part, scal + S_PQ, 0, s, nv, 16);
     k::pcg_update(n, x, c.vr.p, c.vp.p, c.vq.p, scal, S_RZ, S_PQ, part, scal + S_RR, s, nv, c.vactive.p);
   };
+  const bool flexible = c.amg_opt.precision >= 1;
   auto body_precond = [&] {
-    block_cycle64(c, 0, c.vr.p, c.vz.p, nv);
-    k::precond_tail(n, nullptr, c.vr.p, c.vz.p, scal, S_RZ, S_RZN, S_TMP, false, part, s, nv);
+    // as pcg(): z = M r, r.z (flexible CG: Polak-Ribiere beta), p = z + beta p
+    const float* zc = block_vcycle_raw(c, c.vr.p, c.vz.p, nv);
+    k::precond_tail(n, zc, c.vr.p, c.vz.p, scal, S_RZ, S_RZN, S_TMP, flexible, part, s, nv);
     k::pcg_direction(n, c.vp.p, c.vz.p, scal, S_TMP, -1, s, nv);
   };
   const bool graphs = graphs_enabled();
@@ -848,7 +938,10 @@ void pcg_block(Context& c, int nv, const double* b, double* x_io, double tol, in
     }
     if (!any_live) break;
     std::vector<char> in_round = live;
-    block_cycle64(c, 0, c.vr.p, c.vz.p, nv);
+    {
+      const float* zc = block_vcycle_raw(c, c.vr.p, c.vz.p, nv);
+      if (zc) k::to_double(n * nv, zc, c.vz.p, s);
+    }
     k::copy(n * nv, c.vz.p, c.vp.p, s);
     k::dot(n, c.vr.p, c.vz.p, part, scal + S_RZ, s, nv, 16);
     std::vector<char> sent((size_t)nv, 2);

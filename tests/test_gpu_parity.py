@@ -520,17 +520,23 @@ def _three_case_model(kind):
     return m
 
 
-@pytest.mark.parametrize("kind", ["hex8_matrix_free", "hex8_assembled", "q4"])
+@pytest.mark.parametrize("kind", ["hex8_matrix_free", "hex8_assembled", "q4", "hex8_matrix_free_fp32_cycle",
+                                  "q4_bf16_coarse"])
 def test_block_load_cases_equal_case_by_case_solves(ctx, kind, monkeypatch):
     """The load cases of an evaluation solved as one block PCG (one operator
     product and one V-cycle pass per iteration for every case still iterating)
     give each case's case-by-case result bit for bit: displacements, iteration
     counts, compliance and sensitivities, over three evaluations that warm-start
-    from the previous solutions, with one case unloaded."""
+    from the previous solutions, with one case unloaded. FP64 V-cycle, and the
+    mixed hierarchies (FP32 V-cycle under flexible CG; bfloat16 coarse operators)."""
     print(f"test_block_load_cases_equal_case_by_case_solves[{kind}]", flush=True)
+    precision = 1 if "fp32" in kind else 2 if "bf16" in kind else 0
+    if precision:
+        # the mixed path times the CSR kernels' lane counts per context: fix them
+        monkeypatch.setenv("SPARLAB_NO_AUTOTUNE", "1")
     if kind == "hex8_assembled":
         monkeypatch.setenv("SPARLAB_NO_MATRIX_FREE", "1")
-    m = _three_case_model("q4" if kind == "q4" else "hex8")
+    m = _three_case_model("q4" if kind.startswith("q4") else "hex8")
     designs = [M.seeded_density(m.nelem, seed=31 + k) for k in range(3)]
     out = {}
     for mode in ("block", "case_by_case"):
@@ -539,6 +545,9 @@ def test_block_load_cases_equal_case_by_case_solves(ctx, kind, monkeypatch):
         c = N.Context(0)
         c.timing(True)
         g = N.HipProblem(m, ctx=c)
+        if precision:
+            g.assemble()
+            g.amg_setup(precision=precision)
         runs = []
         for x in designs:
             ev = g.evaluate(x, tol=1e-10)
@@ -577,3 +586,49 @@ def test_block_load_cases_oc_run_equals_case_by_case(monkeypatch):
     assert ib == is_ and cb == cs
     keep = [k for k in range(hb.shape[1]) if k != N.H_SECONDS]  # wall-clock seconds differ
     assert np.array_equal(xb, xs) and np.array_equal(hb[:, keep], hs[:, keep])
+
+
+def test_caller_csr_is_checked_on_the_host_and_a_new_pattern_gets_a_new_setup():
+    """slx_factorize_csr checks every index the device and the aggregation will
+    follow (row pointers, column range, layout unknowns) and refuses bad input
+    with a ConfigError; a second matrix of the same size, non-zero count and
+    layout but another pattern gets a setup of its own (its solution equals a
+    fresh context's bit for bit)."""
+    print("test_caller_csr_is_checked_on_the_host_and_a_new_pattern_gets_a_new_setup", flush=True)
+    m = M.bench_box(8)
+    ptr, col, val = Oracle(m, tol=1e-10).assemble_kff()
+    n = len(ptr) - 1
+    coords, free = m.node_coords(), m.free_dofs()
+    s = N.HipAmgCgSolver(tolerance=1e-10, ctx=N.Context(0))
+    s.set_layout(3, coords, free, dofs_per_node=3)
+    bad = col.copy()
+    bad[-1] = n + 5
+    with pytest.raises(N.ConfigError):
+        s.factorize(ptr, bad, val)
+    bad = ptr.copy()
+    bad[5] = bad[6] + 1
+    with pytest.raises(N.ConfigError):
+        s.factorize(bad, col, val)
+    s.set_layout(3, coords, np.where(np.arange(n) == 3, 3 * m.nnodes, free), dofs_per_node=3)
+    with pytest.raises(N.ConfigError):
+        s.factorize(ptr, col, val)
+    s.set_layout(3, coords, free, dofs_per_node=3)
+    s.factorize(ptr, col, val)
+    b = m.free_loads()[0]
+    x0 = s.solve(b)
+    # a permuted ordering: the same n, nnz and layout, another pattern
+    K = sp.csr_matrix((val, col, ptr), shape=(n, n))
+    rev = np.random.default_rng(5).permutation(n)
+    Kr = K[rev][:, rev].tocsr()
+    Kr.sort_indices()
+    assert Kr.nnz == K.nnz and not np.array_equal(Kr.indices, col)
+    s.factorize(Kr.indptr, Kr.indices, Kr.data)
+    x1 = s.solve(b[rev])
+    fresh = N.HipAmgCgSolver(tolerance=1e-10, ctx=N.Context(0))
+    fresh.set_layout(3, coords, free, dofs_per_node=3)
+    fresh.factorize(Kr.indptr, Kr.indices, Kr.data)
+    x2 = fresh.solve(b[rev])
+    assert np.array_equal(x1, x2) and s.last_iterations() == fresh.last_iterations()
+    assert np.linalg.norm(x1 - x0[rev]) <= 1e-8 * np.linalg.norm(x0)  # (K[rev][:, rev]) x[rev] = b[rev]
+    s.ctx.close()
+    fresh.ctx.close()
