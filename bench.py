@@ -420,8 +420,33 @@ def designs_leg(args, prec, tol, log):
     its = [r[0] for r in res]
     assert all(r[2] for r in res), "a segment did not converge"
     log(f"bench[designs]: setup {1e3 * t_setup:.1f} ms, PCG {1e3 * t_pcg:.1f} ms per batch; iterations {its}")
+    # the same designs one at a time: a context and a hierarchy each (untimed), the solves timed
+    import scipy.sparse as sp
+
+    K = sp.csr_matrix((bt.val, bt.col, bt.ptr))
+    singles = []
+    for k in range(nseg):
+        a, b = int(bt.seg_ptr[k]), int(bt.seg_ptr[k + 1])
+        Kg = K[a:b, a:b].tocsr()
+        nodes = bt.unknowns[a:b] // 2
+        n0 = int(nodes.min())
+        one = N.HipAmgCgSolver(tolerance=tol, ctx=N.Context(0), precision=prec)
+        one.set_layout(2, bt.coords[n0:int(nodes.max()) + 1], bt.unknowns[a:b] - 2 * n0, dofs_per_node=2)
+        one.factorize(Kg.indptr, Kg.indices, Kg.data)
+        singles.append((one, bt.b[a:b]))
+    for one, bg in singles:
+        one.solve(bg)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for one, bg in singles:
+            one.solve(bg)
+    t_single = (time.perf_counter() - t0) / args.steps
+    for one, _ in singles:
+        one.ctx.close()
+    log(f"bench[designs]: one at a time {1e3 * t_single:.1f} ms per batch")
     return dict(segments=nseg, unknowns=int(len(bt.b)), nonzeros=int(len(bt.val)), pcg_ms=1e3 * t_pcg,
-                setup_ms=1e3 * t_setup, iterations=its)
+                setup_ms=1e3 * t_setup, iterations=its, one_at_a_time_pcg_ms=1e3 * t_single,
+                speedup_vs_one_at_a_time=t_single / t_pcg)
 
 
 def batch_main(args):
@@ -458,6 +483,85 @@ def batch_main(args):
     print(json.dumps(out))
 
 
+def warmstart_main(args):
+    """The learned warm start's workload: Hex8 96 x 48 x 48 cantilever (unit cells,
+    volume fraction 0.3, filter r 2), 200 SIMP/OC iterations from the uniform
+    design at rtol 1e-8 on the mixed hierarchy, once from the plain warm start
+    (the previous solution, as the reference), once from the previous solution
+    plus the K-optimal multiple of the previous change (the learned start's
+    machinery without the network), and once with the network trained online on
+    the first 40 iterations' solves. The iterations after training are compared:
+    PCG iterations, wall time and the K-norm distance of each starting point from
+    the converged solution."""
+    from sparlab_amd import model as M
+    from sparlab_amd import native as N
+
+    nx, ny, nz = 96, 48, 48
+    total, train, tol = 200, 40, 1e-8
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    m = M.cantilever_3d(nx, ny, nz, volume_fraction=0.3, radius_cells=2.0)
+    runs = {}
+    for label in ("plain", "previous_change", "learned"):
+        ctx = N.Context(0)
+        g = N.HipProblem(m, ctx=ctx)
+        x0 = m.initial_design()
+        g.assemble(1e-9 + (1 - 1e-9) * np.clip(g.filter(x0), 0, 1) ** 3)
+        g.amg_setup(precision=1)
+        if label == "learned":
+            g.warm_start(True, train_solves=train, min_samples=4, steps_per_solve=16, buffer=8, learning_rate=1e-3)
+        elif label == "previous_change":  # no training and no prediction: the extrapolation alone
+            g.warm_start(True, train_solves=0, min_samples=1 << 30)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        x1, h1, _, _ = g.run_oc(x0, train, change_tol=0.0, tol=tol)
+        ctx.synchronize()
+        t_train = time.perf_counter() - t0
+        log(f"bench[warmstart/{label}]: first {train} iterations in {t_train:.2f} s, CG its "
+            f"{h1[:, 11].astype(int).tolist()}")
+        t0 = time.perf_counter()
+        x2, h2, done, c2 = g.run_oc(x1, total - train, change_tol=0.0, tol=tol)
+        ctx.synchronize()
+        t_rest = time.perf_counter() - t0
+        its = h2[:, 11].astype(int)
+        st = g.warm_start_stats() if label != "plain" else None
+        log(f"bench[warmstart/{label}]: iterations {train + 1}..{train + done} in {t_rest:.2f} s, "
+            f"{int(its.sum())} CG iterations, c = {c2:.10e}")
+        runs[label] = dict(seconds=t_rest, steps=done, cg_iterations=int(its.sum()), compliance=c2,
+                           first_seconds=t_train, stats=st)
+        ctx.close()
+    pl, pc, le = runs["plain"], runs["previous_change"], runs["learned"]
+    st = le["stats"]
+    k = max(st["predictions"], 1)
+    out = {
+        "metric": "learned warm start: SIMP iterations/s after training (Hex8 96x48x48, rtol 1e-8)",
+        "value": le["steps"] / le["seconds"], "unit": "SIMP iterations/s", "n_gpus": 1, "steps": le["steps"],
+        "warmup": train, "ms_per_step": 1e3 * le["seconds"] / le["steps"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"cantilever Hex8 {nx}x{ny}x{nz}, {total} SIMP/OC iterations from x0 = 0.3, "
+                               f"network trained on the first {train} solves, mixed hierarchy, rtol {tol}",
+                   "plain_simp_iterations_per_s": pl["steps"] / pl["seconds"],
+                   "previous_change_simp_iterations_per_s": pc["steps"] / pc["seconds"],
+                   "speedup_vs_plain": pl["seconds"] / le["seconds"],
+                   "speedup_vs_previous_change": pc["seconds"] / le["seconds"],
+                   "cg_iterations_after_training": {"learned": le["cg_iterations"],
+                                                    "previous_change": pc["cg_iterations"],
+                                                    "plain": pl["cg_iterations"]},
+                   "final_compliance": {"learned": le["compliance"], "previous_change": pc["compliance"],
+                                        "plain": pl["compliance"]},
+                   "first_iterations_seconds": {"learned": le["first_seconds"], "previous_change": pc["first_seconds"],
+                                                "plain": pl["first_seconds"]},
+                   "knorm_error_ratio_vs_previous_solution": {
+                       "combination": st["energy_ratio_sum"] / k, "net_alone": st["net_ratio_sum"] / k,
+                       "previous_change_alone": st["secant_ratio_sum"] / k},
+                   "train_seconds": st["train_seconds"], "predict_seconds": st["predict_seconds"],
+                   "predictions": st["predictions"], "first_loss": st["first_loss"], "last_loss": st["last_loss"]},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -471,8 +575,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-bf16", action="store_true", help="skip the bfloat16 coarse-operator variant run")
     ap.add_argument("--no-reuse", action="store_true", help="skip the coarse-operator reuse variant run")
-    ap.add_argument("--workload", choices=["simp", "batch"], default="simp",
-                    help="simp: the headline SIMP loop; batch: the batched load-case and design solves")
+    ap.add_argument("--workload", choices=["simp", "batch", "warmstart"], default="simp",
+                    help="simp: the headline SIMP loop; batch: the batched load-case and design solves; "
+                         "warmstart: the learned warm start's 200-iteration run against the plain warm start")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference reading for the HBM byte counters)")
     ap.add_argument("--cpu-side", type=str, default=None, help=argparse.SUPPRESS)
@@ -497,6 +602,9 @@ def main():
 
     if args.workload == "batch":
         batch_main(args)
+        return
+    if args.workload == "warmstart":
+        warmstart_main(args)
         return
 
     from sparlab_amd import model as M

@@ -90,3 +90,45 @@ def test_iteration_cap_and_coupled_segments(ctx, batch):
     bad[1] = bad[1] // 2
     with pytest.raises(N.ConfigError):
         s.solve_segmented(bad, batch.b)
+
+
+def test_batched_workload_at_full_size():
+    """The batched workload at its sizes. The 240 x 160 two-bolt bracket's three
+    load cases (vertical, lateral, torsional) solved by the block PCG inside one
+    evaluation: each case's displacements and the compliance against the
+    oracle's evaluation (rtol 1e-10 both). The 32 MBB designs of seeded sizes
+    (60 x 20 .. 240 x 80, 594 390 unknowns) as one segmented CG at rtol 1e-8:
+    each segment's own residual, and its compliance against a direct solve."""
+    from oracle.refcpu import Oracle
+
+    print("test_batched_workload_at_full_size: bracket 240x160, three cases", flush=True)
+    br = B.bracket_2d()
+    x = M.seeded_density(br.nelem, seed=20261203)
+    c = N.Context(0)
+    g = N.HipProblem(br, ctx=c)
+    ev = g.evaluate(x, tol=1e-10, warm_start=False)
+    eo = Oracle(br, tol=1e-10).evaluate(x, want_u=True)
+    free = g.free()
+    for l in range(3):
+        uo = eo["u"][l][free]
+        assert np.linalg.norm(g.solution(l) - uo) <= 1e-8 * np.linalg.norm(uo), l
+    assert ev["compliance"] == pytest.approx(eo["compliance"], rel=1e-9)
+    assert np.abs(ev["dc_dx"] - eo["dc_dx"]).max() <= 1e-8 * np.abs(eo["dc_dx"]).max()
+    c.close()
+    print("test_batched_workload_at_full_size: 32 MBB designs, one segmented CG", flush=True)
+    c = N.Context(0)
+    bt = B.assemble(B.c4_parts(bracket=False), 2, ctx=c)
+    tol = 1e-8
+    s = B.solver_for(bt, tol=tol, ctx=c, precision=1)
+    xs, res = s.solve_segmented(bt.seg_ptr, bt.b)
+    K = sp.csr_matrix((bt.val, bt.col, bt.ptr))
+    assert len(res) == 32 and len(bt.b) == 594390
+    for k, (its, rel, conv) in enumerate(res):
+        a, b = bt.seg_ptr[k], bt.seg_ptr[k + 1]
+        Kg, bg = K[a:b, a:b].tocsc(), bt.b[a:b]
+        assert conv and rel <= tol, (k, res[k])
+        assert np.linalg.norm(bg - Kg @ xs[a:b]) <= 1.01 * tol * np.linalg.norm(bg), k
+        ref = spla.spsolve(Kg, bg)
+        assert bg @ xs[a:b] == pytest.approx(bg @ ref, rel=1e-6), k
+    print(f"designs: iterations {[r[0] for r in res]}", flush=True)
+    c.close()

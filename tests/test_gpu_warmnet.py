@@ -125,3 +125,48 @@ def test_learned_warm_start_in_an_oc_run(ctx):
     assert hl[train + 1:, 11].sum() <= hp[train + 1:, 11].sum()
     np.testing.assert_allclose(hl[:, 2], hp[:, 2], rtol=1e-4)
     assert fl == pytest.approx(fp, rel=1e-4)
+
+
+def test_learned_warm_start_at_full_size():
+    """The learned warm start's workload at its size: Hex8 96 x 48 x 48, 200 OC
+    iterations at rtol 1e-8 on the mixed hierarchy, the network trained on the
+    first 40 solves. Three runs: the plain warm start (the previous solution, as
+    the reference), the previous solution plus the K-optimal multiple of the
+    previous change (the method without the network), and the learned start.
+    All three reach the same designs (CG runs to the same tolerance); the
+    starting point CG gets is never worse in the K-norm than the method without
+    the network; what the network adds over it is stated."""
+    print("test_learned_warm_start_at_full_size", flush=True)
+    m = M.cantilever_3d(96, 48, 48, volume_fraction=0.3, radius_cells=2.0)
+    runs = {}
+    for label in ("plain", "previous_change", "learned"):
+        c = N.Context(0)
+        g = N.HipProblem(m, ctx=c)
+        x0 = m.initial_design()
+        g.assemble(1e-9 + (1 - 1e-9) * np.clip(g.filter(x0), 0, 1) ** 3)
+        g.amg_setup(precision=1)
+        if label == "learned":
+            g.warm_start(True, train_solves=40, min_samples=4, steps_per_solve=16, buffer=8, learning_rate=1e-3)
+        elif label == "previous_change":
+            g.warm_start(True, train_solves=0, min_samples=1 << 30)
+        x, h, done, comp = g.run_oc(x0, 200, change_tol=0.0, tol=1e-8)
+        st = g.warm_start_stats() if label != "plain" else None
+        runs[label] = (x, h, comp, st)
+        print(f"{label}: {int(h[40:, 11].sum())} CG iterations after iteration 40, c = {comp!r}", flush=True)
+        c.close()
+    xp, hp, cp, _ = runs["plain"]
+    for label in ("previous_change", "learned"):
+        x, h, comp, st = runs[label]
+        assert comp == pytest.approx(cp, rel=1e-7)
+        np.testing.assert_allclose(h[:, 2], hp[:, 2], rtol=1e-6)
+        assert st["predictions"] >= 195 and st["rejected"] == 0
+    _, hs, _, ss = runs["previous_change"]
+    _, hl, _, sl = runs["learned"]
+    k = sl["predictions"]
+    comb, net, sec = (sl[f"{n}_ratio_sum"] / k for n in ("energy", "net", "secant"))
+    print(f"K-norm error of the start / the previous solution's: combination {comb:.4f}, network alone {net:.4f}, "
+          f"previous change alone {sec:.4f}", flush=True)
+    assert sl["samples"] == 40 and sl["train_steps"] == 40 * 16
+    assert comb <= sec + 1e-9 and net < 1.0 and sec < 0.6
+    assert hs[40:, 11].sum() < 0.95 * hp[40:, 11].sum()
+    assert hl[40:, 11].sum() <= 1.02 * hs[40:, 11].sum()
