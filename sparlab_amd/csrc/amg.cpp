@@ -18,6 +18,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -47,67 +48,109 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// SPARLAB_SETUP_TRACE=1: the host steps of a full setup, timed to stderr
+bool setup_trace_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("SPARLAB_SETUP_TRACE");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+Clock::time_point g_trace_clock;
+void setup_trace_start() { g_trace_clock = Clock::now(); }
+void setup_trace(const char* what, int level, hipStream_t s) {
+  if (!setup_trace_on()) return;
+  SLX_HIP(hipStreamSynchronize(s));
+  const auto now = Clock::now();
+  std::fprintf(stderr, "setup trace: level %d %-28s %9.3f ms\n", level, what,
+               1e3 * std::chrono::duration<double>(now - g_trace_clock).count());
+  g_trace_clock = now;
+}
+
 // ---------------------------------------------------------------------------
 // Symbolic sparse algebra (patterns only; rows sorted by column)
 // ---------------------------------------------------------------------------
 HCsr pattern_product(const HCsr& a, const HCsr& b) {
+  // two passes over the rows (count, then fill and sort in place), each row on
+  // one thread with its own marker array
   HCsr c;
   c.rows = a.rows;
   c.cols = b.cols;
   c.ptr.assign((size_t)a.rows + 1, 0);
-  const int nt = omp_get_max_threads();
-  std::vector<std::vector<int>> rows_cols((size_t)a.rows);
-#pragma omp parallel num_threads(nt)
-  {
-    std::vector<int> marker((size_t)b.cols, -1), cols;
+  auto walk = [&](bool fill) {
+#pragma omp parallel
+    {
+      std::vector<int> marker((size_t)b.cols, -1);
 #pragma omp for schedule(dynamic, 512)
-    for (int i = 0; i < a.rows; ++i) {
-      cols.clear();
-      for (int k = a.ptr[(size_t)i]; k < a.ptr[(size_t)i + 1]; ++k) {
-        const int j = a.col[(size_t)k];
-        for (int kk = b.ptr[(size_t)j]; kk < b.ptr[(size_t)j + 1]; ++kk) {
-          const int cc = b.col[(size_t)kk];
-          if (marker[(size_t)cc] != i) {
-            marker[(size_t)cc] = i;
-            cols.push_back(cc);
+      for (int i = 0; i < a.rows; ++i) {
+        int cnt = 0;
+        int* out = fill ? c.col.data() + c.ptr[(size_t)i] : nullptr;
+        for (int k = a.ptr[(size_t)i]; k < a.ptr[(size_t)i + 1]; ++k) {
+          const int j = a.col[(size_t)k];
+          for (int kk = b.ptr[(size_t)j]; kk < b.ptr[(size_t)j + 1]; ++kk) {
+            const int cc = b.col[(size_t)kk];
+            if (marker[(size_t)cc] != i) {
+              marker[(size_t)cc] = i;
+              if (fill) out[cnt] = cc;
+              ++cnt;
+            }
           }
         }
+        // (the marker holds row numbers: each row of one pass stamps its own)
+        if (fill) std::sort(out, out + cnt);
+        else c.ptr[(size_t)i + 1] = cnt;
       }
-      std::sort(cols.begin(), cols.end());
-      rows_cols[(size_t)i] = cols;
-      c.ptr[(size_t)i + 1] = (int)cols.size();
     }
-  }
+  };
+  walk(false);
   for (int i = 0; i < a.rows; ++i) {
     const long long next = (long long)c.ptr[(size_t)i + 1] + c.ptr[(size_t)i];
     if (next > std::numeric_limits<int>::max()) fail(SLX_SOLVER_ERROR, "multigrid: a Galerkin product exceeds 2^31 entries");
     c.ptr[(size_t)i + 1] = (int)next;
   }
   c.col.resize((size_t)c.nnz());
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < a.rows; ++i) {
-    std::copy(rows_cols[(size_t)i].begin(), rows_cols[(size_t)i].end(), c.col.begin() + c.ptr[(size_t)i]);
-    std::vector<int>().swap(rows_cols[(size_t)i]);
-  }
+  walk(true);
   return c;
 }
 
 // R = P^T with src[k] = index in P of R's entry k; columns of R ascend.
 void transpose_pattern(const HCsr& p, HCsr& r, std::vector<int>& src) {
+  // row blocks per thread: per-block column counts give every block its start
+  // in each column, so the entries of a column keep ascending rows
   r.rows = p.cols;
   r.cols = p.rows;
   r.ptr.assign((size_t)p.cols + 1, 0);
-  for (long long k = 0; k < p.nnz(); ++k) ++r.ptr[(size_t)p.col[(size_t)k] + 1];
-  for (int j = 0; j < p.cols; ++j) r.ptr[(size_t)j + 1] += r.ptr[(size_t)j];
   r.col.resize((size_t)p.nnz());
   src.resize((size_t)p.nnz());
-  std::vector<int> next(r.ptr.begin(), r.ptr.end() - 1);
-  for (int i = 0; i < p.rows; ++i)
-    for (int k = p.ptr[(size_t)i]; k < p.ptr[(size_t)i + 1]; ++k) {
-      const int q = next[(size_t)p.col[(size_t)k]]++;
-      r.col[(size_t)q] = i;
-      src[(size_t)q] = k;
+  const int nt = std::max(1, std::min(omp_get_max_threads(),
+                                      (int)std::min<long long>(64, (1LL << 26) / std::max(1, p.cols))));
+  std::vector<int> cnt((size_t)nt * (size_t)p.cols, 0);
+  auto block = [&](int t) { return (int)((long long)p.rows * t / nt); };
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+  for (int t = 0; t < nt; ++t) {
+    int* ct = cnt.data() + (size_t)t * p.cols;
+    for (int k = p.ptr[(size_t)block(t)]; k < p.ptr[(size_t)block(t + 1)]; ++k) ++ct[p.col[(size_t)k]];
+  }
+  // r.ptr, then each block's start in each column
+  for (int j = 0; j < p.cols; ++j) {
+    int run = r.ptr[(size_t)j];
+    for (int t = 0; t < nt; ++t) {
+      const int v = cnt[(size_t)t * p.cols + j];
+      cnt[(size_t)t * p.cols + j] = run;
+      run += v;
     }
+    r.ptr[(size_t)j + 1] = run;
+  }
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+  for (int t = 0; t < nt; ++t) {
+    int* next = cnt.data() + (size_t)t * p.cols;
+    for (int i = block(t); i < block(t + 1); ++i)
+      for (int k = p.ptr[(size_t)i]; k < p.ptr[(size_t)i + 1]; ++k) {
+        const int q = next[p.col[(size_t)k]]++;
+        r.col[(size_t)q] = i;
+        src[(size_t)q] = k;
+      }
+  }
 }
 
 // S = (C + C^T)/2 pattern; ia / ib: index in C of (i, j) / (j, i) or -1.
@@ -118,54 +161,65 @@ void symmetrize_pattern(const HCsr& c, HCsr& s, std::vector<int>& ia, std::vecto
   s.rows = c.rows;
   s.cols = c.cols;
   s.ptr.assign((size_t)c.rows + 1, 0);
-  s.col.clear();
-  ia.clear();
-  ib.clear();
   const int big = std::numeric_limits<int>::max();
-  for (int i = 0; i < c.rows; ++i) {
-    int p = c.ptr[(size_t)i], q = t.ptr[(size_t)i];
+  // the merge of row i of C and of C^T: counted, then written at the row's offset
+  auto merge = [&](int i, bool fill) {
+    int p = c.ptr[(size_t)i], q = t.ptr[(size_t)i], o = fill ? s.ptr[(size_t)i] : 0, n = 0;
     const int pe = c.ptr[(size_t)i + 1], qe = t.ptr[(size_t)i + 1];
     while (p < pe || q < qe) {
       const int cp = p < pe ? c.col[(size_t)p] : big;
       const int cq = q < qe ? t.col[(size_t)q] : big;
+      int col, a = -1, b = -1;
       if (cp == cq) {
-        s.col.push_back(cp);
-        ia.push_back(p++);
-        ib.push_back(tsrc[(size_t)q++]);
+        col = cp;
+        a = p++;
+        b = tsrc[(size_t)q++];
       } else if (cp < cq) {
-        s.col.push_back(cp);
-        ia.push_back(p++);
-        ib.push_back(-1);
+        col = cp;
+        a = p++;
       } else {
-        s.col.push_back(cq);
-        ia.push_back(-1);
-        ib.push_back(tsrc[(size_t)q++]);
+        col = cq;
+        b = tsrc[(size_t)q++];
       }
+      if (fill) {
+        s.col[(size_t)o + n] = col;
+        ia[(size_t)o + n] = a;
+        ib[(size_t)o + n] = b;
+      }
+      ++n;
     }
-    s.ptr[(size_t)i + 1] = (int)s.col.size();
-  }
+    return n;
+  };
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < c.rows; ++i) s.ptr[(size_t)i + 1] = merge(i, false);
+  for (int i = 0; i < c.rows; ++i) s.ptr[(size_t)i + 1] += s.ptr[(size_t)i];
+  s.col.resize((size_t)s.ptr[(size_t)c.rows]);
+  ia.resize(s.col.size());
+  ib.resize(s.col.size());
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < c.rows; ++i) merge(i, true);
 }
 
 // ---------------------------------------------------------------------------
 // Aggregation (Multigrid.cpp:457-565): block Frobenius strength, theta test,
 // greedy passes in node order.
 // ---------------------------------------------------------------------------
-void aggregate_nodes(const HCsr& a, const std::vector<int>& node_ptr, double theta, std::vector<int>& agg) {
+// The node blocks of a matrix for the strength test: per node, its neighbour
+// nodes in ascending order with the sum of squares of the block's entries
+// (rows in order, columns in order within a row), and the diagonal block's sum.
+struct NodeBlocks {
+  std::vector<int> ptr, node;
+  std::vector<double> sumsq, diag;
+};
+
+NodeBlocks host_node_blocks(const HCsr& a, const std::vector<int>& node_ptr) {
   const int nn = (int)node_ptr.size() - 1;
   std::vector<int> node_of((size_t)a.rows);
   for (int nd = 0; nd < nn; ++nd)
     for (int i = node_ptr[(size_t)nd]; i < node_ptr[(size_t)nd + 1]; ++i) node_of[(size_t)i] = nd;
-  std::vector<double> bdiag((size_t)nn);
-#pragma omp parallel for schedule(static)
-  for (int nd = 0; nd < nn; ++nd) {
-    double s = 0.0;
-    for (int i = node_ptr[(size_t)nd]; i < node_ptr[(size_t)nd + 1]; ++i)
-      for (int k = a.ptr[(size_t)i]; k < a.ptr[(size_t)i + 1]; ++k)
-        if (node_of[(size_t)a.col[(size_t)k]] == nd) s += a.val[(size_t)k] * a.val[(size_t)k];
-    bdiag[(size_t)nd] = std::sqrt(s);
-  }
-  // strong neighbours per node, ascending, with their relative strength
-  std::vector<std::vector<std::pair<int, double>>> strong((size_t)nn);
+  NodeBlocks b;
+  b.diag.assign((size_t)nn, 0.0);
+  std::vector<std::vector<std::pair<int, double>>> per((size_t)nn);
 #pragma omp parallel
   {
     std::vector<int> marker((size_t)nn, -1), touched;
@@ -173,25 +227,62 @@ void aggregate_nodes(const HCsr& a, const std::vector<int>& node_ptr, double the
 #pragma omp for schedule(dynamic, 1024)
     for (int nd = 0; nd < nn; ++nd) {
       touched.clear();
+      double sd = 0.0;
       for (int i = node_ptr[(size_t)nd]; i < node_ptr[(size_t)nd + 1]; ++i)
         for (int k = a.ptr[(size_t)i]; k < a.ptr[(size_t)i + 1]; ++k) {
           const int o = node_of[(size_t)a.col[(size_t)k]];
-          if (o == nd) continue;
+          const double v = a.val[(size_t)k];
+          if (o == nd) {
+            sd += v * v;
+            continue;
+          }
           if (marker[(size_t)o] != nd) {
             marker[(size_t)o] = nd;
             acc[(size_t)o] = 0.0;
             touched.push_back(o);
           }
-          acc[(size_t)o] += a.val[(size_t)k] * a.val[(size_t)k];
+          acc[(size_t)o] += v * v;
         }
       std::sort(touched.begin(), touched.end());
-      const double di = bdiag[(size_t)nd];
-      auto& out = strong[(size_t)nd];
-      for (int o : touched) {
-        const double s = std::sqrt(acc[(size_t)o]);
-        const double ref = std::sqrt(di * bdiag[(size_t)o]);
-        if (s > 0.0 && s >= theta * ref) out.emplace_back(o, ref > 0.0 ? s / ref : 0.0);
-      }
+      b.diag[(size_t)nd] = sd;
+      auto& out = per[(size_t)nd];
+      out.reserve(touched.size());
+      for (int o : touched) out.emplace_back(o, acc[(size_t)o]);
+    }
+  }
+  b.ptr.assign((size_t)nn + 1, 0);
+  for (int nd = 0; nd < nn; ++nd) b.ptr[(size_t)nd + 1] = b.ptr[(size_t)nd] + (int)per[(size_t)nd].size();
+  b.node.resize((size_t)b.ptr[(size_t)nn]);
+  b.sumsq.resize(b.node.size());
+#pragma omp parallel for schedule(static)
+  for (int nd = 0; nd < nn; ++nd) {
+    int q = b.ptr[(size_t)nd];
+    for (auto& pr : per[(size_t)nd]) {
+      b.node[(size_t)q] = pr.first;
+      b.sumsq[(size_t)q++] = pr.second;
+    }
+  }
+  return b;
+}
+
+// Aggregation (Multigrid.cpp:457-565) from the node blocks: block Frobenius
+// strength against the diagonal blocks' norms, theta test, greedy passes in
+// node order.
+void aggregate_blocks(const NodeBlocks& b, double theta, std::vector<int>& agg) {
+  const int nn = (int)b.diag.size();
+  std::vector<double> bdiag((size_t)nn);
+  for (int nd = 0; nd < nn; ++nd) bdiag[(size_t)nd] = std::sqrt(b.diag[(size_t)nd]);
+  // strong neighbours per node, ascending, with their relative strength
+  std::vector<std::vector<std::pair<int, double>>> strong((size_t)nn);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int nd = 0; nd < nn; ++nd) {
+    const double di = bdiag[(size_t)nd];
+    auto& out = strong[(size_t)nd];
+    for (int q = b.ptr[(size_t)nd]; q < b.ptr[(size_t)nd + 1]; ++q) {
+      const int o = b.node[(size_t)q];
+      const double s = std::sqrt(b.sumsq[(size_t)q]);
+      const double ref = std::sqrt(di * bdiag[(size_t)o]);
+      if (s > 0.0 && s >= theta * ref) out.emplace_back(o, ref > 0.0 ? s / ref : 0.0);
     }
   }
   agg.assign((size_t)nn, -1);
@@ -224,6 +315,10 @@ void aggregate_nodes(const HCsr& a, const std::vector<int>& node_ptr, double the
       if (agg[(size_t)pr.first] < 0) agg[(size_t)pr.first] = count;
     ++count;
   }
+}
+
+void aggregate_nodes(const HCsr& a, const std::vector<int>& node_ptr, double theta, std::vector<int>& agg) {
+  aggregate_blocks(host_node_blocks(a, node_ptr), theta, agg);
 }
 
 // Sums in the order of the SSE2 kernels (two-lane packets) of Eigen 3.4.0,
@@ -721,25 +816,40 @@ bool block_lists(const HCsr& m, const std::vector<int>& node_ptr, const std::vec
                  std::vector<int>& bpos) {
   const int nn = (int)node_ptr.size() - 1;
   bptr.assign((size_t)nn + 1, 0);
-  bagg.clear();
-  bpos.clear();
-  for (int nd = 0; nd < nn; ++nd) {
+  std::atomic<bool> ok{true};
+  // node nd's blocks: counted (and checked), then written at the node's offset
+  auto walk = [&](int nd, bool fill) {
     const int r0 = node_ptr[(size_t)nd];
-    int k = m.ptr[(size_t)r0];
+    int k = m.ptr[(size_t)r0], n = 0, o = fill ? bptr[(size_t)nd] : 0;
     const int k1 = m.ptr[(size_t)r0 + 1];
     while (k < k1) {
       const int c = m.col[(size_t)k];
       const int a = agg_of_col[(size_t)c];
       const int w = agg_off[(size_t)a + 1] - agg_off[(size_t)a];
-      if (c != agg_off[(size_t)a] || k + w > k1) return false;
+      if (c != agg_off[(size_t)a] || k + w > k1) return -1;
       for (int q = 0; q < w; ++q)
-        if (m.col[(size_t)k + q] != c + q) return false;
-      bagg.push_back(a);
-      bpos.push_back(k - m.ptr[(size_t)r0]);
+        if (m.col[(size_t)k + q] != c + q) return -1;
+      if (fill) {
+        bagg[(size_t)o + n] = a;
+        bpos[(size_t)o + n] = k - m.ptr[(size_t)r0];
+      }
+      ++n;
       k += w;
     }
-    bptr[(size_t)nd + 1] = (int)bagg.size();
+    return n;
+  };
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int nd = 0; nd < nn; ++nd) {
+    const int n = walk(nd, false);
+    if (n < 0) ok.store(false, std::memory_order_relaxed);
+    bptr[(size_t)nd + 1] = std::max(n, 0);
   }
+  if (!ok.load()) return false;
+  for (int nd = 0; nd < nn; ++nd) bptr[(size_t)nd + 1] += bptr[(size_t)nd];
+  bagg.resize((size_t)bptr[(size_t)nn]);
+  bpos.resize(bagg.size());
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int nd = 0; nd < nn; ++nd) walk(nd, true);
   return true;
 }
 
@@ -750,21 +860,40 @@ bool run_table(const HCsr& m, const std::vector<int>& node_ptr, const std::vecto
                const std::vector<int>& col_node_of, std::vector<int>& run_ptr, std::vector<int>& runs) {
   const int nn = (int)node_ptr.size() - 1;
   run_ptr.assign((size_t)nn + 1, 0);
-  runs.clear();
-  for (int nd = 0; nd < nn; ++nd) {
+  std::atomic<bool> ok{true};
+  // node nd's runs: counted (and checked), then written at the node's offset
+  auto walk = [&](int nd, bool fill) {
     const int r0 = node_ptr[(size_t)nd];
     const int k0 = m.ptr[(size_t)r0], k1 = m.ptr[(size_t)r0 + 1];
-    int k = k0;
+    int k = k0, n = 0;
+    int* out = fill ? runs.data() + (size_t)4 * run_ptr[(size_t)nd] : nullptr;
     while (k < k1) {
       const int j = m.col[(size_t)k];
       const int J = col_node_of[(size_t)j];
       const int dj = col_node_ptr[(size_t)J + 1] - col_node_ptr[(size_t)J];
-      if (dj > 6) return false;
-      runs.insert(runs.end(), {k - k0, j, dj, J});
+      if (dj > 6) return -1;
+      if (fill) {
+        out[4 * n] = k - k0;
+        out[4 * n + 1] = j;
+        out[4 * n + 2] = dj;
+        out[4 * n + 3] = J;
+      }
+      ++n;
       k += dj;
     }
-    run_ptr[(size_t)nd + 1] = (int)(runs.size() / 4);
+    return n;
+  };
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int nd = 0; nd < nn; ++nd) {
+    const int n = walk(nd, false);
+    if (n < 0) ok.store(false, std::memory_order_relaxed);
+    run_ptr[(size_t)nd + 1] = std::max(n, 0);
   }
+  if (!ok.load()) return false;
+  for (int nd = 0; nd < nn; ++nd) run_ptr[(size_t)nd + 1] += run_ptr[(size_t)nd];
+  runs.resize((size_t)4 * run_ptr[(size_t)nn]);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int nd = 0; nd < nn; ++nd) walk(nd, true);
   return true;
 }
 
@@ -820,6 +949,7 @@ bool build_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& cbp, 
   walk(true, &list, &pos);
   std::vector<int> node((size_t)nitems);
   int dmax = 1, wmax = 1;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(max : dmax, wmax)
   for (int I = 0; I < nn; ++I) {
     dmax = std::max(dmax, cnode_ptr[(size_t)I + 1] - cnode_ptr[(size_t)I]);
     for (int q = cbp[(size_t)I]; q < cbp[(size_t)I + 1]; ++q) {
@@ -830,12 +960,23 @@ bool build_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& cbp, 
   }
   if (dmax > 6 || wmax > 64) return false;
   if (order) {
-    // emit the items node by node in the given order (the lists move with them)
+    // emit the items node by node in the given order (the lists move with them):
+    // each node's first item and first list entry in the new order, then the moves
+    if ((int)order->size() != nn) return false;
+    std::vector<long long> item0((size_t)nn + 1, 0), entry0((size_t)nn + 1, 0);
+    for (int r = 0; r < nn; ++r) {
+      const int I = (*order)[(size_t)r];
+      item0[(size_t)r + 1] = item0[(size_t)r] + (cbp[(size_t)I + 1] - cbp[(size_t)I]);
+      entry0[(size_t)r + 1] = entry0[(size_t)r] + (begin[(size_t)cbp[(size_t)I + 1]] - begin[(size_t)cbp[(size_t)I]]);
+    }
+    if (item0[(size_t)nn] != nitems) return false;
     std::vector<int> n2((size_t)nitems), a2((size_t)nitems), p2((size_t)nitems);
     std::vector<long long> b2((size_t)nitems + 1, 0);
     std::vector<uint2> l2(list.size());
-    long long q2 = 0, e2 = 0;
-    for (int I : *order)
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int r = 0; r < nn; ++r) {
+      const int I = (*order)[(size_t)r];
+      long long q2 = item0[(size_t)r], e2 = entry0[(size_t)r];
       for (int q = cbp[(size_t)I]; q < cbp[(size_t)I + 1]; ++q, ++q2) {
         n2[(size_t)q2] = I;
         a2[(size_t)q2] = cba[(size_t)q];
@@ -843,7 +984,7 @@ bool build_plan(const std::vector<int>& cnode_ptr, const std::vector<int>& cbp, 
         for (long long e = begin[(size_t)q]; e < begin[(size_t)q + 1]; ++e) l2[(size_t)e2++] = list[(size_t)e];
         b2[(size_t)q2 + 1] = e2;
       }
-    if (q2 != nitems) return false;
+    }
     plan.nitems = nitems;
     plan.dmax = dmax;
     plan.wmax = wmax;
@@ -1019,14 +1160,17 @@ void prepare_node_products(Level& lv, const Level& next, const HCsr& a, const HC
       !node_uniform(rpat, next.node_ptr) || !node_uniform(rawpat, next.node_ptr))
     return;
   if (!runs_complete(a, lv.node_ptr, f_of) || !runs_complete(rpat, lv.node_ptr, f_of)) return;
+  setup_trace("  plans: pattern checks", -1, s);
   std::vector<int> tp, ta, tq, pp, pa, pq, qp, qa, qq, wp, wa, wq;
   if (!block_lists(pt, lv.node_ptr, c_of, off, tp, ta, tq) || !block_lists(ppat, lv.node_ptr, c_of, off, pp, pa, pq) ||
       !block_lists(appat, lv.node_ptr, c_of, off, qp, qa, qq) ||
       !block_lists(rawpat, next.node_ptr, c_of, off, wp, wa, wq))
     return;
+  setup_trace("  plans: block lists", -1, s);
   std::vector<int> arp, ar, rrp, rr;
   if (!run_table(a, lv.node_ptr, lv.node_ptr, f_of, arp, ar) || !run_table(rpat, next.node_ptr, lv.node_ptr, f_of, rrp, rr))
     return;
+  setup_trace("  plans: run tables", -1, s);
   for (size_t nd = 0; nd + 1 < tp.size(); ++nd)
     if (tp[nd + 1] - tp[nd] != 1) return;  // P-hat: one block per node
   // with node coordinates, fine nodes and aggregates (at their first node) in Morton order
@@ -1046,10 +1190,15 @@ void prepare_node_products(Level& lv, const Level& next, const HCsr& a, const HC
   }
   const std::vector<int>* fo = xyz ? &f_order : nullptr;
   const std::vector<int>* co = xyz ? &c_order : nullptr;
-  if (!build_plan(lv.node_ptr, pp, pa, pq, arp, ar, tp, ta, tq, pt.ptr, off, fo, lv.plan_p, s) ||
-      !build_row_plan(lv.node_ptr, appat.ptr, qp, qa, qq, arp, ar, pp, pa, pq, ppat.ptr, off, fo, lv.plan_ap, s) ||
-      !build_row_plan(next.node_ptr, rawpat.ptr, wp, wa, wq, rrp, rr, qp, qa, qq, appat.ptr, off, co, lv.plan_raw, s))
+  setup_trace("  plans: Morton orders", -1, s);
+  if (!build_plan(lv.node_ptr, pp, pa, pq, arp, ar, tp, ta, tq, pt.ptr, off, fo, lv.plan_p, s)) return;
+  setup_trace("  plans: P plan", -1, s);
+  if (!build_row_plan(lv.node_ptr, appat.ptr, qp, qa, qq, arp, ar, pp, pa, pq, ppat.ptr, off, fo, lv.plan_ap, s))
     return;
+  setup_trace("  plans: A P row plan", -1, s);
+  if (!build_row_plan(next.node_ptr, rawpat.ptr, wp, wa, wq, rrp, rr, qp, qa, qq, appat.ptr, off, co, lv.plan_raw, s))
+    return;
+  setup_trace("  plans: R A P row plan", -1, s);
   std::vector<int> row_agg((size_t)lv.n);
   for (int i = 0; i < lv.n; ++i) row_agg[(size_t)i] = c_of[(size_t)pt.col[(size_t)pt.ptr[(size_t)i]]];
   lv.f_node_ptr.upload(lv.node_ptr, s);
@@ -1118,6 +1267,68 @@ HCsr download_csr(const DCsr& a, hipStream_t s) {
   a.val.download(h.val.data(), h.val.size(), s);
   SLX_HIP(hipStreamSynchronize(s));
   return h;
+}
+
+// The level-0 node blocks from K's host pattern and its device values: the
+// runs of each node's first row (one neighbour node each) are counted on the
+// host, their sums of squares over the node's rows computed on the device in
+// the host's order. False when the rows of a node do not share one pattern of
+// complete node runs (the host path then reads K).
+bool device_node_blocks(Context& c, const std::vector<int>& node_ptr, NodeBlocks& b) {
+  const HCsr& a = c.k_host;
+  const std::vector<int> node_of = node_of_rows(node_ptr);
+  if (!node_uniform(a, node_ptr) || !runs_complete(a, node_ptr, node_of)) return false;
+  const int nn = (int)node_ptr.size() - 1;
+  std::vector<int> run_ptr((size_t)nn + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int nd = 0; nd < nn; ++nd) {
+    const int r0 = node_ptr[(size_t)nd];
+    int runs = 0;
+    for (int k = a.ptr[(size_t)r0]; k < a.ptr[(size_t)r0 + 1];) {
+      const int J = node_of[(size_t)a.col[(size_t)k]];
+      k += node_ptr[(size_t)J + 1] - node_ptr[(size_t)J];
+      ++runs;
+    }
+    run_ptr[(size_t)nd + 1] = runs;
+  }
+  for (int nd = 0; nd < nn; ++nd) run_ptr[(size_t)nd + 1] += run_ptr[(size_t)nd];
+  hipStream_t s = c.stream;
+  DBuf<int> d_node_ptr, d_node_of, d_run_ptr;
+  DBuf<double> d_sums;
+  d_node_ptr.upload(node_ptr, s);
+  d_node_of.upload(node_of, s);
+  d_run_ptr.upload(run_ptr, s);
+  d_sums.alloc((size_t)std::max(1, run_ptr[(size_t)nn]));
+  k::node_block_sums(nn, d_node_ptr.p, d_node_of.p, c.K.ptr.p, c.K.col.p, c.K.val.p, d_run_ptr.p, d_sums.p, s);
+  std::vector<double> sums((size_t)run_ptr[(size_t)nn]);
+  d_sums.download(sums.data(), sums.size(), s);
+  SLX_HIP(hipStreamSynchronize(s));
+  // the node's own run is its diagonal block; the others ascend with the columns
+  b.diag.assign((size_t)nn, 0.0);
+  b.ptr.assign((size_t)nn + 1, 0);
+  for (int nd = 0; nd < nn; ++nd) b.ptr[(size_t)nd + 1] = b.ptr[(size_t)nd] + (run_ptr[(size_t)nd + 1] - run_ptr[(size_t)nd]) - 1;
+  b.node.resize((size_t)b.ptr[(size_t)nn]);
+  b.sumsq.resize(b.node.size());
+  std::atomic<bool> ok{true};
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int nd = 0; nd < nn; ++nd) {
+    const int r0 = node_ptr[(size_t)nd];
+    int t = run_ptr[(size_t)nd], q = b.ptr[(size_t)nd];
+    bool own = false;
+    for (int k = a.ptr[(size_t)r0]; k < a.ptr[(size_t)r0 + 1]; ++t) {
+      const int J = node_of[(size_t)a.col[(size_t)k]];
+      k += node_ptr[(size_t)J + 1] - node_ptr[(size_t)J];
+      if (J == nd) {
+        b.diag[(size_t)nd] = sums[(size_t)t];
+        own = true;
+      } else if (q < b.ptr[(size_t)nd + 1]) {
+        b.node[(size_t)q] = J;
+        b.sumsq[(size_t)q++] = sums[(size_t)t];
+      }
+    }
+    if (!own) ok.store(false, std::memory_order_relaxed);
+  }
+  return ok.load();
 }
 
 // Lanczos estimate of lambda_max(D^-1 A) on the device (Multigrid.cpp:288-327).
@@ -1410,29 +1621,51 @@ void amg_setup(Context& c, bool defer_coarse) {
   // finiteness of every entry (Multigrid.cpp:992-996): the sum of squares,
   // checked with level 0's read
   k::dot(c.K.nnz, c.K.val.p, c.K.val.p, c.partials.p, c.scal.p + 9, s);
-  HCsr host_a;  // values of the level being aggregated (first setup only)
+  setup_trace_start();
+  auto trace = [&](const char* what, size_t lvl) { setup_trace(what, (int)lvl, s); };
+  HCsr host_a;  // the level being aggregated (first setup only): pattern and values
+  // Level 0: K's pattern is on the host where it was uploaded; the symmetry
+  // check and the strength sums read the values on the device, so K itself is
+  // not brought back.
+  const bool dev0 = !reuse && c.k_host_version == c.k_pattern_version && c.k_host.rows == n &&
+                    c.k_host.nnz() == c.K.nnz;
   if (!reuse) {
-    host_a = download_csr(c.K, s);
     // symmetry check once per structure (Multigrid.cpp:968-991)
-    double max_abs = 0.0;
-    for (double v : host_a.val) max_abs = std::max(max_abs, std::abs(v));
     int bad_i = -1, bad_j = -1;
+    if (dev0) {
+      DBuf<unsigned long long> bad;
+      bad.alloc(1);
+      SLX_HIP(hipMemsetAsync(bad.p, 0xff, sizeof(unsigned long long), s));
+      k::symmetry_check(c.K, 1.0e-10, c.partials.p, bad.p, s);
+      unsigned long long hb = ~0ULL;
+      bad.download(&hb, 1, s);
+      SLX_HIP(hipStreamSynchronize(s));
+      if (hb != ~0ULL) {
+        bad_i = (int)(hb >> 32);
+        bad_j = (int)(hb & 0xffffffffULL);
+      }
+    } else {
+      host_a = download_csr(c.K, s);
+      trace("download K", 0);
+      double max_abs = 0.0;
+      for (double v : host_a.val) max_abs = std::max(max_abs, std::abs(v));
 #pragma omp parallel for schedule(dynamic, 4096)
-    for (int i = 0; i < n; ++i)
-      for (int k = host_a.ptr[(size_t)i]; k < host_a.ptr[(size_t)i + 1]; ++k) {
-        const int j = host_a.col[(size_t)k];
-        const int* b0 = host_a.col.data() + host_a.ptr[(size_t)j];
-        const int* b1 = host_a.col.data() + host_a.ptr[(size_t)j + 1];
-        const int* hit = std::lower_bound(b0, b1, i);
-        const double other = (hit != b1 && *hit == i) ? host_a.val[(size_t)(hit - host_a.col.data())] : 0.0;
-        if (std::abs(other - host_a.val[(size_t)k]) > 1.0e-10 * max_abs) {
+      for (int i = 0; i < n; ++i)
+        for (int k = host_a.ptr[(size_t)i]; k < host_a.ptr[(size_t)i + 1]; ++k) {
+          const int j = host_a.col[(size_t)k];
+          const int* b0 = host_a.col.data() + host_a.ptr[(size_t)j];
+          const int* b1 = host_a.col.data() + host_a.ptr[(size_t)j + 1];
+          const int* hit = std::lower_bound(b0, b1, i);
+          const double other = (hit != b1 && *hit == i) ? host_a.val[(size_t)(hit - host_a.col.data())] : 0.0;
+          if (std::abs(other - host_a.val[(size_t)k]) > 1.0e-10 * max_abs) {
 #pragma omp critical
-          {
-            bad_i = i;
-            bad_j = j;
+            {
+              bad_i = i;
+              bad_j = j;
+            }
           }
         }
-      }
+    }
     if (bad_i >= 0) {
       std::ostringstream os;
       os << "multigrid: the system matrix is not symmetric (entry (" << bad_i << ", " << bad_j
@@ -1443,8 +1676,10 @@ void amg_setup(Context& c, bool defer_coarse) {
     c.levels.reserve((size_t)o.max_levels + 1);
     c.levels.emplace_back();
     c.levels[0].n = n;
+    trace("symmetry check", 0);
     build_finest_nodes(c, c.levels[0]);
     c.amg_ready = false;
+    trace("finest nodes", 0);
   }
   lap(0);
   int* flag = c.setup_flags.p;
@@ -1453,32 +1688,53 @@ void amg_setup(Context& c, bool defer_coarse) {
   while (true) {
     numeric_level(c, l);
     lap(1);
+    trace("numeric level (Lanczos)", l);
     Level& lv = c.levels[l];
     const bool stop = reuse ? (l + 1 == reused_levels)
                             : (lv.n <= o.coarse_size || (int)l + 1 >= o.max_levels);
     if (stop) break;
     if (!reuse) {
       if (l > 0) host_a = download_csr(lv.A, s);
-      aggregate_nodes(host_a, lv.node_ptr, o.strength_threshold, lv.aggregate);
+      const HCsr* hap = &host_a;  // the level's pattern (values only where downloaded)
+      if (l == 0 && dev0) {
+        NodeBlocks nb;
+        if (device_node_blocks(c, lv.node_ptr, nb)) {
+          aggregate_blocks(nb, o.strength_threshold, lv.aggregate);
+          hap = &c.k_host;
+        } else {  // a pattern the device sums do not cover: the host reads K
+          host_a = download_csr(c.K, s);
+          aggregate_nodes(host_a, lv.node_ptr, o.strength_threshold, lv.aggregate);
+        }
+      } else {
+        aggregate_nodes(host_a, lv.node_ptr, o.strength_threshold, lv.aggregate);
+      }
+      const HCsr& ha = *hap;
       lap(2);
+      trace("aggregation", l);
       Level next;
       HCsr pt;
       tentative(lv, next, c.nullspace_dim, pt);
       lap(6);
+      trace("tentative prolongator", l);
       if (next.n >= (int)(0.85 * double(lv.n)) || next.n == 0) {
         lv.aggregate.clear();
         break;
       }
       // symbolic patterns of P, R, A P, R (A P) and the coarse operator
-      HCsr ppat = pattern_product(host_a, pt);
+      HCsr ppat = pattern_product(ha, pt);
+      trace("pattern P", l);
       HCsr rpat;
       std::vector<int> rsrc;
       transpose_pattern(ppat, rpat, rsrc);
-      HCsr appat = pattern_product(host_a, ppat);
+      trace("pattern R", l);
+      HCsr appat = pattern_product(ha, ppat);
+      trace("pattern A P", l);
       HCsr rawpat = pattern_product(rpat, appat);
+      trace("pattern R A P", l);
       HCsr spat;
       std::vector<int> ia, ib;
       symmetrize_pattern(rawpat, spat, ia, ib);
+      trace("pattern symmetrised", l);
       lv.Ptent.upload(pt, s);
       if (l == 0 && c.mf) build_tent_blocks(lv, pt, s);
       {
@@ -1498,11 +1754,13 @@ void amg_setup(Context& c, bool defer_coarse) {
       lv.sym_a.upload(ia, s);
       lv.sym_b.upload(ib, s);
       next.A.upload_pattern(spat, s);
+      trace("uploads", l);
       share_columns(next.A, spat, next.node_ptr, s);
       share_columns(lv.P, ppat, lv.node_ptr, s);
       share_columns(lv.R, rpat, next.node_ptr, s);
       share_columns(lv.Ptent, pt, lv.node_ptr, s);
-      if (l == 0 && !c.mf) share_columns(c.K, host_a, lv.node_ptr, s);
+      if (l == 0 && !c.mf) share_columns(c.K, ha, lv.node_ptr, s);
+      trace("shared columns", l);
       std::vector<double> xyz;  // level-0 node coordinates (mesh node of each node block's first unknown)
       if (l == 0 && !c.layout_coords.empty() && c.layout_ndpn > 0) {
         const int nn = (int)lv.node_ptr.size() - 1;
@@ -1513,8 +1771,9 @@ void amg_setup(Context& c, bool defer_coarse) {
             xyz[(size_t)nd * 3 + d] = c.layout_coords[(size_t)mesh * c.layout_dim + d];
         }
       }
-      prepare_node_products(lv, next, host_a, pt, ppat, rpat, appat, rawpat, xyz.empty() ? nullptr : &xyz, s);
+      prepare_node_products(lv, next, ha, pt, ppat, rpat, appat, rawpat, xyz.empty() ? nullptr : &xyz, s);
       lap(4);  // the symbolic products and their plans count as Galerkin time
+      trace("product plans", l);
       lv.has_next = true;
       SLX_HIP(hipStreamSynchronize(s));  // host pattern arrays go out of scope
       c.levels.push_back(std::move(next));
@@ -1550,6 +1809,7 @@ void amg_setup(Context& c, bool defer_coarse) {
     }
     k::symmetrize_values(coarse.A.nnz, cur.sym_a.p, cur.sym_b.p, cur.RAW.val.p, coarse.A.val.p, s);
     lap(4);
+    trace("numeric products", l);
     ++l;
   }
   c.levels.resize(l + 1);

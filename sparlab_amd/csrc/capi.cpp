@@ -146,6 +146,14 @@ void assemble(Context& c, const double* d_scale, bool values = true) {
   }
 }
 
+// K's pattern stays on the host beside its device copy: the multigrid setup
+// reads its structure there, while the values stay on the device
+void keep_host_pattern(Context& c, HCsr&& pat) {
+  c.k_host = std::move(pat);
+  std::vector<double>().swap(c.k_host.val);
+  c.k_host_version = c.k_pattern_version;
+}
+
 // one load case from its last solution (or zero) to the tolerance
 slx_pcg_result solve_case(Context& c, int l, bool warm, double tol, int maxit) {
   if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "AMG-CG: solve called before factorize");
@@ -569,9 +577,10 @@ slx_status slx_model_finalize(slx_context* ctx) {
     c.total_volume = 0.0;
     for (double v : c.volumes) c.total_volume += v;
     c.htvol_valid = false;
-    const HCsr pat = structured_kff_pattern(c);
+    HCsr pat = structured_kff_pattern(c);
     c.K.upload_pattern(pat, c.stream);
     ++c.k_pattern_version;
+    keep_host_pattern(c, std::move(pat));
     c.d_free.upload(c.free_dofs, c.stream);
     c.d_reduced.upload(c.reduced, c.stream);
     c.d_ke0.upload(c.ke0, c.stream);
@@ -740,6 +749,7 @@ slx_status slx_factorize_csr(slx_context* ctx, int n, const int* ptr, const int*
     } else {
       c.K.upload(h, c.stream);
       ++c.k_pattern_version;
+      keep_host_pattern(c, std::move(h));
     }
     c.k_pattern_hash = hash;
     c.mf = false;  // a caller's matrix: products use it as given

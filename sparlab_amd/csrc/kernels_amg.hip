@@ -670,7 +670,87 @@ __global__ void __launch_bounds__(256) k_gj_update(int n, int k0, double* __rest
   }
 }
 
+// Node-block sums of squares for the strength of connection (Multigrid.cpp:
+// 457-490): for node I (rows [node_ptr[I], node_ptr[I+1]), one shared pattern)
+// and each run of its first row (the columns of one node J), the sum over the
+// node's rows in order, and over the run's entries in column order, of v^2 --
+// the order of the host's accumulation, so the sums are the same bits.
+__global__ void k_node_block_sums(int nn, const int* __restrict__ node_ptr, const int* __restrict__ node_of,
+                                  const int* __restrict__ ptr, const int* __restrict__ col,
+                                  const double* __restrict__ val, const int* __restrict__ run_ptr,
+                                  double* __restrict__ sums) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nn) return;
+  const int r0 = node_ptr[I], dn = node_ptr[I + 1] - r0;
+  const int k0 = ptr[r0], k1 = ptr[r0 + 1];
+  int t = run_ptr[I];
+  for (int k = k0; k < k1;) {
+    const int J = node_of[col[k]];
+    const int dj = node_ptr[J + 1] - node_ptr[J];
+    const int koff = k - k0;
+    double acc = 0.0;
+    for (int i = 0; i < dn; ++i) {
+      const double* v = val + ptr[r0 + i] + koff;
+      for (int q = 0; q < dj; ++q) acc += v[q] * v[q];
+    }
+    sums[t++] = acc;
+    k += dj;
+  }
+}
+
+// max |v| over the values (block maxima, then one block)
+__global__ void __launch_bounds__(256) k_absmax(long long n, const double* __restrict__ v, double* __restrict__ out) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    m = fmax(m, fabs(v[i]));
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+// Symmetry of a CSR matrix with sorted rows (Multigrid.cpp:968-991): entry
+// (i, j) against (j, i), found by bisection in row j (0 when absent); a
+// difference above tol * max|v| lowers *bad to (i << 32) | j.
+__global__ void k_symmetry(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                           const double* __restrict__ val, const double* __restrict__ maxabs, double tol,
+                           unsigned long long* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double lim = tol * maxabs[0];
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const int j = col[k];
+    int lo = ptr[j], hi = ptr[j + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (col[mid] < i) lo = mid + 1; else hi = mid;
+    }
+    const double other = (lo < ptr[j + 1] && col[lo] == i) ? val[lo] : 0.0;
+    if (fabs(other - val[k]) > lim) {
+      atomicMin(bad, ((unsigned long long)i << 32) | (unsigned)j);
+      return;
+    }
+  }
+}
+
 }  // namespace
+
+void node_block_sums(int nn, const int* node_ptr, const int* node_of, const int* ptr, const int* col,
+                     const double* val, const int* run_ptr, double* sums, hipStream_t s) {
+  if (nn) k_node_block_sums<<<nblk(nn, 256), 256, 0, s>>>(nn, node_ptr, node_of, ptr, col, val, run_ptr, sums);
+}
+
+void symmetry_check(const DCsr& A, double tol, double* scratch, unsigned long long* bad, hipStream_t s) {
+  // scratch: 1025 doubles (block maxima, then the maximum)
+  const int nb = std::max(1, std::min(1024, nblk(A.nnz, 256)));
+  k_absmax<<<nb, 256, 0, s>>>(A.nnz, A.val.p, scratch + 1);
+  k_absmax<<<1, 256, 0, s>>>(nb, scratch + 1, scratch);
+  if (A.rows) k_symmetry<<<nblk(A.rows, 256), 256, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, scratch, tol, bad);
+}
 
 void block_product(const BlockProductArgs& a, bool smooth, hipStream_t s) {
   if (a.nitems == 0) return;

@@ -36,7 +36,14 @@ struct HCsr {
   long long nnz() const { return ptr.empty() ? 0 : ptr.back(); }
 };
 
-template <class T>
+// Large host <-> device copies through a pinned staging buffer (two halves, the
+// host copy of one overlapping the DMA of the other); both complete before
+// returning. Pageable copies of hundreds of MB run several times slower.
+constexpr size_t kStageMinBytes = size_t(16) << 20;
+void stage_copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
+void stage_copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+template <typename T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
@@ -63,14 +70,18 @@ struct DBuf {
   // Host data may be pageable and short-lived: the copy completes before return.
   void upload(const T* h, size_t count, hipStream_t s) {
     alloc(count);
-    if (count) {
+    if (count * sizeof(T) >= kStageMinBytes) {
+      stage_copy_h2d(p, h, count * sizeof(T), s);
+    } else if (count) {
       SLX_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
       SLX_HIP(hipStreamSynchronize(s));
     }
   }
   void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+  // small copies are queued (the caller synchronises); large ones complete here
   void download(T* h, size_t count, hipStream_t s) const {
-    if (count) SLX_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    if (count * sizeof(T) >= kStageMinBytes) stage_copy_d2h(h, p, count * sizeof(T), s);
+    else if (count) SLX_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
   }
 };
 
@@ -92,7 +103,10 @@ struct DCsr {
   // up to 8 values from the start of a run)
   void upload_values(const double* h, hipStream_t s) {
     if (val.n < (size_t)nnz + 8) val.alloc((size_t)nnz + 8);
-    if (nnz) SLX_HIP(hipMemcpyAsync(val.p, h, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, s));
+    if (sizeof(double) * (size_t)nnz >= kStageMinBytes)
+      stage_copy_h2d(val.p, h, sizeof(double) * (size_t)nnz, s);
+    else if (nnz)
+      SLX_HIP(hipMemcpyAsync(val.p, h, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, s));
     SLX_HIP(hipStreamSynchronize(s));
   }
   void upload(const HCsr& h, hipStream_t s) {
@@ -211,6 +225,8 @@ struct Context {
   KSource k_source = KSource::none;
   unsigned long long k_pattern_hash = 0;  // of a caller matrix's pattern (ptr, col)
   long long k_pattern_version = 0;        // changes whenever K's pattern is replaced
+  HCsr k_host;                            // K's pattern (ptr, col) on the host
+  long long k_host_version = -1;          // k_pattern_version it belongs to
   bool k_on_host = false;
   // K_ff's values lag the element factors: an evaluation that reused the coarse
   // operators refreshed only level 0 and skipped the assembly (coarse_reuse)

@@ -137,6 +137,15 @@ void eig_residuals(int n, int m, const double* KX, const double* MX, const doubl
                    hipStream_t s);
 void scale_copy(int n, double a, const double* x, double* y, hipStream_t s);
 
+// Strength of connection at level 0 (kernels_amg.hip): per node and per run of
+// its first row's pattern (one neighbour node), the sum of squares over the
+// node's rows (run_ptr: the runs of each node, in row order)
+void node_block_sums(int nn, const int* node_ptr, const int* node_of, const int* ptr, const int* col,
+                     const double* val, const int* run_ptr, double* sums, hipStream_t s);
+// *bad = min over asymmetric entries of (i << 32) | j (|a_ij - a_ji| > tol max|a|);
+// set *bad to all ones before; scratch: 1025 doubles
+void symmetry_check(const DCsr& A, double tol, double* scratch, unsigned long long* bad, hipStream_t s);
+
 // SA-AMG numeric setup (kernels_amg.hip)
 struct BlockProductArgs {
   long long nitems;           // C blocks (one per aggregate block of each C node-row)

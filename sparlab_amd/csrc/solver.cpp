@@ -14,9 +14,11 @@
 // symmetric, so the outer method becomes flexible CG (Polak-Ribiere beta).
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <functional>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <sstream>
 
 #include "slx_kernels.hpp"
@@ -420,6 +422,64 @@ Timed::~Timed() {
   if (!a) return;
   (void)hipEventRecord(b, c.stream);
   c.pending.push_back({name, {a, b}});
+}
+
+namespace {
+constexpr size_t kStageChunk = size_t(32) << 20;
+std::mutex g_stage_mutex;
+char* g_stage[2] = {nullptr, nullptr};
+
+void stage_init() {
+  if (!g_stage[0])
+    for (char*& b : g_stage) SLX_HIP(hipHostMalloc((void**)&b, kStageChunk, hipHostMallocDefault));
+}
+
+// host memcpy of one chunk, split over the host threads
+void host_copy(char* dst, const char* src, size_t bytes) {
+  const size_t slice = size_t(1) << 20;
+  const long long nslices = (long long)((bytes + slice - 1) / slice);
+#pragma omp parallel for schedule(static) if (nslices > 1)
+  for (long long i = 0; i < nslices; ++i) {
+    const size_t off = (size_t)i * slice;
+    std::memcpy(dst + off, src + off, std::min(slice, bytes - off));
+  }
+}
+}  // namespace
+
+void stage_copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_stage_mutex);
+  stage_init();
+  char* d = static_cast<char*>(dst);
+  const char* h = static_cast<const char*>(src);
+  size_t i = 0;
+  for (size_t off = 0; off < bytes; off += kStageChunk, ++i) {
+    const size_t len = std::min(kStageChunk, bytes - off);
+    if (i >= 2) SLX_HIP(hipStreamSynchronize(s));  // the copy that last read this half is done
+    host_copy(g_stage[i % 2], h + off, len);
+    SLX_HIP(hipMemcpyAsync(d + off, g_stage[i % 2], len, hipMemcpyHostToDevice, s));
+  }
+  SLX_HIP(hipStreamSynchronize(s));
+}
+
+void stage_copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_stage_mutex);
+  stage_init();
+  char* h = static_cast<char*>(dst);
+  const char* d = static_cast<const char*>(src);
+  const size_t nchunks = (bytes + kStageChunk - 1) / kStageChunk;
+  if (nchunks == 0) return;
+  SLX_HIP(hipMemcpyAsync(g_stage[0], d, std::min(kStageChunk, bytes), hipMemcpyDeviceToHost, s));
+  for (size_t i = 0; i < nchunks; ++i) {
+    SLX_HIP(hipStreamSynchronize(s));  // chunk i has arrived
+    const size_t off = i * kStageChunk, len = std::min(kStageChunk, bytes - off);
+    if (i + 1 < nchunks) {
+      const size_t noff = off + kStageChunk;
+      SLX_HIP(hipMemcpyAsync(g_stage[(i + 1) % 2], d + noff, std::min(kStageChunk, bytes - noff),
+                             hipMemcpyDeviceToHost, s));
+    }
+    host_copy(h + off, g_stage[i % 2], len);
+  }
+  SLX_HIP(hipStreamSynchronize(s));
 }
 
 double* Context::host_buffer(size_t n) {
