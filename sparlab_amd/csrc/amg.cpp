@@ -1834,6 +1834,7 @@ void amg_setup(Context& c, bool defer_coarse) {
   lap(5);
   for (auto& lv : c.levels) {
     lv.x.alloc((size_t)lv.n);
+    lv.x2.alloc((size_t)lv.n);
     lv.b.alloc((size_t)lv.n);
     lv.res.alloc((size_t)lv.n);
     lv.d.alloc((size_t)lv.n);

@@ -79,7 +79,24 @@ __device__ __forceinline__ double row_dot(int s, int e, int lane, const int* __r
   return acc;
 }
 
-// MODE 0: y = A x; 1: y = b - A x; 2: y += A x; 3: y = A x and partial x.y
+// The smoother start fused into a product's epilogue (modes 4 and 5 of
+// k_spmv): the output rows' D^-1, the start's d and iterate, and theta.
+template <typename VT>
+struct StartArgs {
+  const VT* dinv = nullptr;
+  VT* d = nullptr;
+  VT* x2 = nullptr;
+  double theta = 1.0;
+};
+
+// MODE 0: y = A x; 1: y = b - A x; 2: y += A x; 3: y = A x and partial x.y;
+// 4: y = A x, then the zero-guess smoother start on y: d = dinv y / theta,
+//    x2 = d (a restriction that starts the coarse level's pre-smoothing);
+// 5: y = r = b - A x, then the smoother start from x: d = dinv r / theta,
+//    x2 = x + d (post-smoothing; x2 is another buffer than x, which other
+//    rows still read).
+// Modes 4 and 5 round r and d to VT before using them, as the separate
+// k_cheb_first_zero / k_cheb_first passes do, so the results are the same bits.
 // cbase (may be null): rows of one node share one column list, entry k of row
 // i reads its column at col[cbase[i] + k]
 template <int LANES, int MODE, typename VT, typename VAL = VT>
@@ -87,14 +104,19 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ 
                                                   const int* __restrict__ cbase,
                                                   const VAL* __restrict__ val, const VT* __restrict__ x,
                                                   const VT* __restrict__ b, VT* __restrict__ y,
-                                                  double* __restrict__ partials, long long xs, long long ys) {
+                                                  double* __restrict__ partials, long long xs, long long ys,
+                                                  StartArgs<VT> st) {
   __shared__ double sh[kBlock / 64];
   // load case blockIdx.y of a block of right-hand sides (case-major vectors, strides xs and ys)
   const long long vc = blockIdx.y;
   x += vc * xs;
   y += vc * ys;
-  if (MODE == 1) b += vc * ys;
+  if (MODE == 1 || MODE == 5) b += vc * ys;
   if (MODE == 3) partials += vc * gridDim.x;
+  if (MODE == 4 || MODE == 5) {
+    st.d += vc * ys;
+    st.x2 += vc * ys;
+  }
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int rows_per_block = kBlock / LANES;
   const int row = blk * rows_per_block + threadIdx.x / LANES;
@@ -111,6 +133,20 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* __restrict__ 
       if (MODE == 3) {
         y[row] = (VT)acc;
         prod = (double)x[row] * acc;
+      }
+      if (MODE == 4) {
+        const VT yr = (VT)acc;
+        y[row] = yr;
+        const double di = (double)st.dinv[row] * (double)yr / st.theta;
+        st.d[row] = (VT)di;
+        st.x2[row] = (VT)(0.0 + di);
+      }
+      if (MODE == 5) {
+        const VT rr = (VT)((double)b[row] - acc);
+        y[row] = rr;
+        const double di = (double)st.dinv[row] * (double)rr / st.theta;
+        st.d[row] = (VT)di;
+        st.x2[row] = (VT)((double)x[row] + di);
       }
     }
   }
@@ -751,26 +787,27 @@ int lanes_for(const DCsr& A) {
 // nv load cases (gridDim.y), case-major vectors: x at stride A.cols, b and y at stride A.rows
 template <int MODE, typename VT, typename VAL = VT>
 void launch_spmv(const DCsr& A, const VAL* val, const VT* x, const VT* b, VT* y, double* partials, hipStream_t s,
-                 int nv = 1) {
+                 int nv = 1, StartArgs<VT> st = StartArgs<VT>()) {
   const int lanes = lanes_for(A);
   const dim3 nb(grid8(A.rows, kBlock / lanes), nv);
   if (A.rows == 0) return;
   const int* cols = A.cbase.p ? A.ncol.p : A.col.p;
   const long long xs = A.cols, ys = A.rows;
   switch (lanes) {
-    case 64: k_spmv<64, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
-    case 16: k_spmv<16, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
-    case 8: k_spmv<8, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
-    case 1: k_spmv<1, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
-    default: k_spmv<4, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys); break;
+    case 64: k_spmv<64, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys, st); break;
+    case 16: k_spmv<16, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys, st); break;
+    case 8: k_spmv<8, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys, st); break;
+    case 1: k_spmv<1, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys, st); break;
+    default: k_spmv<4, MODE, VT, VAL><<<nb, kBlock, 0, s>>>(A.rows, A.ptr.p, cols, A.cbase.p, val, x, b, y, partials, xs, ys, st); break;
   }
 }
 
 // FP32-vector products read the bfloat16 copy of the values when there is one
 template <int MODE>
-void launch_spmv32(const DCsr& A, const float* x, const float* b, float* y, hipStream_t s, int nv) {
-  if (A.val16.p) launch_spmv<MODE, float, uint16_t>(A, A.val16.p, x, b, y, nullptr, s, nv);
-  else launch_spmv<MODE, float>(A, A.val32.p, x, b, y, nullptr, s, nv);
+void launch_spmv32(const DCsr& A, const float* x, const float* b, float* y, hipStream_t s, int nv,
+                   StartArgs<float> st = StartArgs<float>()) {
+  if (A.val16.p) launch_spmv<MODE, float, uint16_t>(A, A.val16.p, x, b, y, nullptr, s, nv, st);
+  else launch_spmv<MODE, float>(A, A.val32.p, x, b, y, nullptr, s, nv, st);
 }
 
 }  // namespace
@@ -795,6 +832,22 @@ void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s, int nv) 
 }
 void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s, int nv) {
   launch_spmv32<1>(A, x, b, r, s, nv);
+}
+void spmv_start(const DCsr& A, const double* x, double* y, const double* dinv, double theta, double* d, double* x2,
+                hipStream_t s, int nv) {
+  launch_spmv<4, double>(A, A.val.p, x, nullptr, y, nullptr, s, nv, StartArgs<double>{dinv, d, x2, theta});
+}
+void residual_first(const DCsr& A, const double* x, const double* b, double* r, const double* dinv, double theta,
+                    double* d, double* x2, hipStream_t s, int nv) {
+  launch_spmv<5, double>(A, A.val.p, x, b, r, nullptr, s, nv, StartArgs<double>{dinv, d, x2, theta});
+}
+void spmv_start32(const DCsr& A, const float* x, float* y, const float* dinv, double theta, float* d, float* x2,
+                  hipStream_t s, int nv) {
+  launch_spmv32<4>(A, x, nullptr, y, s, nv, StartArgs<float>{dinv, d, x2, theta});
+}
+void residual_first32(const DCsr& A, const float* x, const float* b, float* r, const float* dinv, double theta,
+                      float* d, float* x2, hipStream_t s, int nv) {
+  launch_spmv32<5>(A, x, b, r, s, nv, StartArgs<float>{dinv, d, x2, theta});
 }
 int spmv_dot_blocks(const DCsr& A) { return grid8(A.rows, kBlock / lanes_for(A)); }
 void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s, int nv) {

@@ -501,12 +501,21 @@ __global__ void __launch_bounds__(256) k_tent_prolong(int n, const int* __restri
 // across the lanes for each of its columns. ella holds the values in the
 // aggregates' row order (ella[k * n + i] for member i), so the lanes read them
 // contiguously and only y is gathered.
+// With start.dinv set, the coarse level's zero-guess smoother start follows
+// in the epilogue: d = dinv xc / theta, x2 = d (k_cheb_first_zero's rounding).
+template <typename T>
+struct TentStart {
+  const T* dinv = nullptr;
+  T* d = nullptr;
+  T* x2 = nullptr;
+  double theta = 1.0;
+};
 template <typename T>
 __global__ void __launch_bounds__(256) k_tent_restrict(int naggs, const int* __restrict__ agg_ptr,
                                                        const int* __restrict__ agg_rows,
                                                        const int* __restrict__ agg_c0len, int n,
                                                        const T* __restrict__ ella, const T* __restrict__ y,
-                                                       T* __restrict__ xc) {
+                                                       T* __restrict__ xc, TentStart<T> st) {
   const int a = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
   const int lane = threadIdx.x & 31;
   if (a >= naggs) return;  // whole 32-lane groups leave together
@@ -529,6 +538,11 @@ __global__ void __launch_bounds__(256) k_tent_restrict(int naggs, const int* __r
     for (int k = 1; k < 6; ++k)
       if (lane == k) v = acc[k];
     xc[c0 + lane] = v;
+    if (st.dinv) {
+      const double di = (double)st.dinv[c0 + lane] * (double)v / st.theta;
+      st.d[c0 + lane] = (T)di;
+      st.x2[c0 + lane] = (T)(0.0 + di);
+    }
   }
 }
 
@@ -884,14 +898,22 @@ void tent_prolong(const TentBlocks& t, const T* ell, const T* xc, T* y, hipStrea
   if (t.n) k_tent_prolong<T><<<nblk(t.n, 256), 256, 0, s>>>(t.n, t.c0len, ell, xc, y);
 }
 template <typename T>
-void tent_restrict(const TentBlocks& t, const T* ella, const T* y, T* xc, hipStream_t s) {
+void tent_restrict(const TentBlocks& t, const T* ella, const T* y, T* xc, hipStream_t s, const T* dinv, double theta,
+                   T* d, T* x2) {
+  TentStart<T> st;
+  st.dinv = dinv;
+  st.d = d;
+  st.x2 = x2;
+  st.theta = theta;
   if (t.naggs) k_tent_restrict<T><<<nblk((long long)t.naggs * 32, 256), 256, 0, s>>>(t.naggs, t.agg_ptr, t.agg_rows,
-                                                                                    t.agg_c0len, t.n, ella, y, xc);
+                                                                                    t.agg_c0len, t.n, ella, y, xc, st);
 }
 template void tent_prolong<double>(const TentBlocks&, const double*, const double*, double*, hipStream_t);
 template void tent_prolong<float>(const TentBlocks&, const float*, const float*, float*, hipStream_t);
-template void tent_restrict<double>(const TentBlocks&, const double*, const double*, double*, hipStream_t);
-template void tent_restrict<float>(const TentBlocks&, const float*, const float*, float*, hipStream_t);
+template void tent_restrict<double>(const TentBlocks&, const double*, const double*, double*, hipStream_t,
+                                    const double*, double, double*, double*);
+template void tent_restrict<float>(const TentBlocks&, const float*, const float*, float*, hipStream_t, const float*,
+                                   double, float*, float*);
 
 }  // namespace k
 }  // namespace slx

@@ -162,8 +162,9 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int g = fi[c] < 0 ? 0 : fi[c];
-    if (MODE == 1 || MODE == 3 || MODE == 4 || MODE == 5 || MODE == 6) px[c] = a.x[g];
-    if (MODE == 2 || MODE == 3) pb[c] = a.b[g];
+    if (MODE == 1 || MODE == 3 || MODE == 4 || MODE == 5 || MODE == 6 || MODE == 7) px[c] = a.x[g];
+    if (MODE == 2 || MODE == 3 || MODE == 7) pb[c] = a.b[g];
+    if (MODE == 7) pd[c] = a.dinv[g];
     if (MODE == 3 || MODE == 6) {
       pd[c] = a.dinv[g];
       ps[c] = a.xs[g];
@@ -329,6 +330,14 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
         const double wv = (double)y * (double)pd[c] - beta * (double)ps[c];
         a.y[f] = (T)wv;
         dot += wv * (double)px[c];
+      } else if (MODE == 7) {
+        // post-smoothing residual and the smoother's start from x:
+        // r = b - K x, d = D^-1 r / theta, x2 = x + d (k_cheb_first's rounding)
+        const T r = pb[c] - y;
+        const double di = (double)pd[c] * (double)r / a.c1;
+        a.y[f] = r;
+        a.d_out[f] = (T)di;
+        a.xs[f] = (T)((double)px[c] + di);
       } else if (MODE == 5) {
         // smoothed restriction, first half: y = (I - omega K D^-1) x
         a.y[f] = (T)((double)px[c] - a.c1 * (double)y);
@@ -427,7 +436,8 @@ void apply(const MfArgs<T>& a, int mode, hipStream_t s) {
     case 3: launch<T, 3>(a, s); break;
     case 4: launch<T, 4>(a, s); break;
     case 5: launch<T, 5>(a, s); break;
-    default: launch<T, 6>(a, s); break;
+    case 6: launch<T, 6>(a, s); break;
+    default: launch<T, 7>(a, s); break;
   }
 }
 

@@ -170,10 +170,11 @@ struct Level {
   DBuf<int> tent_c0len, tent_agg_ptr, tent_agg_rows, tent_agg_c0len;
   DBuf<double> tent_ell, tent_ella;  // by fine row; in the aggregates' member order
   DBuf<float> tent_ell32, tent_ella32;
-  DBuf<double> x, b, res, d, d2;
-  DBuf<float> dinv32, x32, b32, res32, d32, d232;
-  DBuf<double> vx, vb, vres, vd, vd2;  // a block of load cases side by side (case-major)
-  DBuf<float> vx32, vb32, vres32, vd32, vd232;
+  // V-cycle vectors: x the pre-smoothing iterate, x2 the post-smoothed result
+  DBuf<double> x, x2, b, res, d, d2;
+  DBuf<float> dinv32, x32, x232, b32, res32, d32, d232;
+  DBuf<double> vx, vx2, vb, vres, vd, vd2;  // a block of load cases side by side (case-major)
+  DBuf<float> vx32, vx232, vb32, vres32, vd32, vd232;
 };
 
 struct TimerSlot {

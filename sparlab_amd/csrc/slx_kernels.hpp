@@ -18,11 +18,14 @@ struct TentBlocks {
   const int* agg_c0len = nullptr;
 };
 // y = P-hat xc (ell: values by fine row) and xc = P-hat^T y (ella: the same
-// values in the aggregates' member order, ella[k * n + i] for agg_rows[i])
+// values in the aggregates' member order, ella[k * n + i] for agg_rows[i]);
+// with dinv set, tent_restrict also starts the coarse level's smoother from a
+// zero guess: d = dinv xc / theta, x2 = d
 template <typename T>
 void tent_prolong(const TentBlocks& t, const T* ell, const T* xc, T* y, hipStream_t s);
 template <typename T>
-void tent_restrict(const TentBlocks& t, const T* ella, const T* y, T* xc, hipStream_t s);
+void tent_restrict(const TentBlocks& t, const T* ella, const T* y, T* xc, hipStream_t s, const T* dinv = nullptr,
+                   double theta = 1.0, T* d = nullptr, T* x2 = nullptr);
 
 
 // operator products: y = A x, y += A x, r = b - A x (FP64 and FP32 storage).
@@ -34,6 +37,19 @@ void residual(const DCsr& A, const double* x, const double* b, double* r, hipStr
 void spmv32(const DCsr& A, const float* x, float* y, hipStream_t s, int nv = 1);
 void spmv_add32(const DCsr& A, const float* x, float* y, hipStream_t s, int nv = 1);
 void residual32(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s, int nv = 1);
+// The smoother start fused into a product: spmv_start forms y = A x and then
+// the zero-guess start of y's level, d = dinv y / theta and x2 = d (the result
+// of a restriction, cheb_first_zero in one pass); residual_first forms
+// r = b - A x and then d = dinv r / theta, x2 = x + d (cheb_first after the
+// post-smoothing residual; x2 must not alias x). Same bits as the two passes.
+void spmv_start(const DCsr& A, const double* x, double* y, const double* dinv, double theta, double* d, double* x2,
+                hipStream_t s, int nv = 1);
+void residual_first(const DCsr& A, const double* x, const double* b, double* r, const double* dinv, double theta,
+                    double* d, double* x2, hipStream_t s, int nv = 1);
+void spmv_start32(const DCsr& A, const float* x, float* y, const float* dinv, double theta, float* d, float* x2,
+                  hipStream_t s, int nv = 1);
+void residual_first32(const DCsr& A, const float* x, const float* b, float* r, const float* dinv, double theta,
+                      float* d, float* x2, hipStream_t s, int nv = 1);
 // y = A x and per-block partial sums of x.y (spmv_dot_blocks(A) partials)
 int spmv_dot_blocks(const DCsr& A);
 void spmv_dot(const DCsr& A, const double* x, double* y, double* partials, hipStream_t s, int nv = 1);

@@ -16,6 +16,8 @@ namespace kmf {
 //      and when partials is set partials[block] = sum of y.x over the block
 //   5  restriction smoothing: y = x - c1 K (in_scale x)   (in_scale = D^-1, c1 = omega)
 //   6  prolongation smoothing: xs += x - c1 dinv (K x)
+//   7  residual and the smoother's start: y = r = b - K x, d_out = dinv r / c1
+//      (c1 = theta), xs = x + d_out (xs another buffer than x)
 // X, Y, Z are the node counts per direction; reduced maps DOF = node * 3 + k to
 // the free index or -1; scale holds the element factors s_e and ke the 24 x 24
 // element matrix (node-major DOF order, VTK corner numbering).

@@ -89,12 +89,16 @@ k::TentBlocks tent_blocks(const Level& lv) {
   return t;
 }
 
+// Restriction into the next level's right-hand side bc; with cdinv set, the
+// next level's zero-guess smoother start follows (cd = cx = cdinv bc / theta).
 template <typename T>
-void restrict_mf(Context& c, const T* r, T* scratch, T* bc);
+void restrict_mf(Context& c, const T* r, T* scratch, T* bc, const T* cdinv = nullptr, double theta = 1.0,
+                 T* cd = nullptr, T* cx = nullptr);
 template <typename T>
 void prolong_mf(Context& c, const T* xc, T* scratch, T* x);
 template <>
-void restrict_mf<double>(Context& c, const double* r, double* scratch, double* bc) {
+void restrict_mf<double>(Context& c, const double* r, double* scratch, double* bc, const double* cdinv, double theta,
+                         double* cd, double* cx) {
   Level& lv = c.levels[0];
   kmf::MfArgs<double> a = mf_base<double>(c);
   a.x = r;
@@ -102,11 +106,14 @@ void restrict_mf<double>(Context& c, const double* r, double* scratch, double* b
   a.c1 = lv.omega;
   a.y = scratch;
   kmf::apply(a, 5, c.stream);
-  if (lv.tent_ok) k::tent_restrict(tent_blocks(lv), lv.tent_ella.p, static_cast<const double*>(scratch), bc, c.stream);
-  else k::spmv(lv.Rt, scratch, bc, c.stream);
+  const double* sc = scratch;
+  if (lv.tent_ok) k::tent_restrict(tent_blocks(lv), lv.tent_ella.p, sc, bc, c.stream, cdinv, theta, cd, cx);
+  else if (cdinv) k::spmv_start(lv.Rt, sc, bc, cdinv, theta, cd, cx, c.stream);
+  else k::spmv(lv.Rt, sc, bc, c.stream);
 }
 template <>
-void restrict_mf<float>(Context& c, const float* r, float* scratch, float* bc) {
+void restrict_mf<float>(Context& c, const float* r, float* scratch, float* bc, const float* cdinv, double theta,
+                        float* cd, float* cx) {
   Level& lv = c.levels[0];
   kmf::MfArgs<float> a = mf_base<float>(c);
   a.x = r;
@@ -114,8 +121,10 @@ void restrict_mf<float>(Context& c, const float* r, float* scratch, float* bc) {
   a.c1 = lv.omega;
   a.y = scratch;
   kmf::apply(a, 5, c.stream);
-  if (lv.tent_ok) k::tent_restrict(tent_blocks(lv), lv.tent_ella32.p, static_cast<const float*>(scratch), bc, c.stream);
-  else k::spmv32(lv.Rt, scratch, bc, c.stream);
+  const float* sc = scratch;
+  if (lv.tent_ok) k::tent_restrict(tent_blocks(lv), lv.tent_ella32.p, sc, bc, c.stream, cdinv, theta, cd, cx);
+  else if (cdinv) k::spmv_start32(lv.Rt, sc, bc, cdinv, theta, cd, cx, c.stream);
+  else k::spmv32(lv.Rt, sc, bc, c.stream);
 }
 template <>
 void prolong_mf<double>(Context& c, const double* xc, double* scratch, double* x) {
@@ -156,128 +165,38 @@ Cheb cheb_params(const Context& c, const Level& lv) {
   return p;
 }
 
-// Chebyshev steps 1..degree-1 given x, d and the residual source r_first.
-template <bool F32>
-void cheb_steps(Context& c, size_t l, const void* r_first, void* xv, bool timed) {
-  Level& lv = c.levels[l];
-  DCsr& A = op(c, l);
-  const Cheb cp = cheb_params(c, lv);
-  double rho = 1.0 / cp.sigma;
-  hipStream_t s = c.stream;
-  for (int k = 1; k < c.amg_opt.smoother_degree; ++k) {
-    const double rho_new = 1.0 / (2.0 * cp.sigma - rho);
-    const double c1 = rho_new * rho;
-    const double c2 = 2.0 * rho_new / cp.delta;
-    if (F32) {
-      const float* rin = k == 1 ? static_cast<const float*>(r_first) : lv.res32.p;
-      Timed t(c, timed ? "cheb" : l == 1 ? "cheb_l1" : nullptr);
-      if (use_mf(c, l)) {
-        kmf::MfArgs<float> a = mf_base<float>(c);
-        a.x = lv.d32.p, a.b = rin, a.y = lv.res32.p, a.dinv = lv.dinv32.p, a.d_out = lv.d232.p;
-        a.xs = static_cast<float*>(xv), a.c1 = c1, a.c2 = c2;
-        kmf::apply(a, 3, s);
-      } else {
-        k::cheb_step32(A, lv.dinv32.p, rin, lv.res32.p, lv.d32.p, lv.d232.p, static_cast<float*>(xv), c1, c2, s);
-      }
-      std::swap(lv.d32, lv.d232);
-    } else {
-      const double* rin = k == 1 ? static_cast<const double*>(r_first) : lv.res.p;
-      Timed t(c, timed ? "cheb" : l == 1 ? "cheb_l1" : nullptr);
-      if (use_mf(c, l)) {
-        kmf::MfArgs<double> a = mf_base<double>(c);
-        a.x = lv.d.p, a.b = rin, a.y = lv.res.p, a.dinv = lv.dinv.p, a.d_out = lv.d2.p;
-        a.xs = static_cast<double*>(xv), a.c1 = c1, a.c2 = c2;
-        kmf::apply(a, 3, s);
-      } else {
-        k::cheb_step(A, lv.dinv.p, rin, lv.res.p, lv.d.p, lv.d2.p, static_cast<double*>(xv), c1, c2, s);
-      }
-      std::swap(lv.d, lv.d2);
-    }
-    rho = rho_new;
-  }
-}
-
-// V-cycle from level l for right-hand side b into x (Multigrid.cpp:857-885)
-void cycle64(Context& c, size_t l, const double* b, double* x) {
-  Level& lv = c.levels[l];
-  hipStream_t s = c.stream;
-  if (l + 1 == c.levels.size()) {
-    k::dense_gemv(lv.n, c.coarse_inv.p, b, x, s);
-    return;
-  }
-  const bool top = l == 0;
-  const Cheb cp = cheb_params(c, lv);
-  // pre-smoothing from a zero guess: x = d = D^-1 b / theta, r starts as b
-  k::cheb_first_zero(lv.n, lv.dinv.p, b, cp.theta, x, lv.d.p, s);
-  cheb_steps<false>(c, l, b, x, top);
-  {
-    Timed t(c, top ? "residual" : nullptr);
-    level_residual<double>(c, l, x, b, lv.res.p);
-  }
-  Level& nx = c.levels[l + 1];
-  if (use_mf(c, l)) restrict_mf<double>(c, lv.res.p, lv.d.p, nx.b.p); else k::spmv(lv.R, lv.res.p, nx.b.p, s);
-  cycle64(c, l + 1, nx.b.p, nx.x.p);
-  if (use_mf(c, l)) prolong_mf<double>(c, nx.x.p, lv.d.p, x); else k::spmv_add(lv.P, nx.x.p, x, s);
-  // post-smoothing from the current x
-  {
-    Timed t(c, top ? "residual" : nullptr);
-    level_residual<double>(c, l, x, b, lv.res.p);
-  }
-  k::cheb_first(lv.n, lv.dinv.p, lv.res.p, cp.theta, x, lv.d.p, s);
-  cheb_steps<false>(c, l, lv.res.p, x, top);
-}
-
-// started: x and d already hold the zero-guess smoother start
-void cycle32(Context& c, size_t l, const float* b, float* x, bool started = false) {
-  Level& lv = c.levels[l];
-  hipStream_t s = c.stream;
-  if (l + 1 == c.levels.size()) {
-    k::dense_gemv32(lv.n, c.coarse_inv32.p, b, x, s);
-    return;
-  }
-  const bool top = l == 0;
-  const Cheb cp = cheb_params(c, lv);
-  if (!started) k::cheb_first_zero32(lv.n, lv.dinv32.p, b, cp.theta, x, lv.d32.p, s);
-  cheb_steps<true>(c, l, b, x, top);
-  {
-    Timed t(c, top ? "residual" : nullptr);
-    level_residual<float>(c, l, x, b, lv.res32.p);
-  }
-  Level& nx = c.levels[l + 1];
-  if (use_mf(c, l)) restrict_mf<float>(c, lv.res32.p, lv.d32.p, nx.b32.p); else k::spmv32(lv.R, lv.res32.p, nx.b32.p, s);
-  cycle32(c, l + 1, nx.b32.p, nx.x32.p);
-  if (use_mf(c, l)) prolong_mf<float>(c, nx.x32.p, lv.d32.p, x); else k::spmv_add32(lv.P, nx.x32.p, x, s);
-  {
-    Timed t(c, top ? "residual" : nullptr);
-    level_residual<float>(c, l, x, b, lv.res32.p);
-  }
-  k::cheb_first32(lv.n, lv.dinv32.p, lv.res32.p, cp.theta, x, lv.d32.p, s);
-  cheb_steps<true>(c, l, lv.res32.p, x, top);
-}
-
-
-// ---- a block of nv load cases ----------------------------------------------
-// The same V-cycle for nv right-hand sides at once: every assembled-operator
-// pass (smoother, residual, transfers, coarse solve) reads its matrix once for
-// all cases, each case summed in the single-case order, so each case's result
-// equals its own cycle bit for bit. Level vectors are the block buffers
-// (nv x n_l, case-major); the matrix-free level-0 operator, which reads no
-// matrix, runs once per case.
+// ---- the V-cycle ------------------------------------------------------------
+// One implementation serves one right-hand side and a block of nv load cases
+// (case-major level vectors, nv x n_l): every assembled-operator pass
+// (smoother, residual, transfers, coarse solve) reads its matrix once for all
+// cases, each case summed in the single-case order, so each case's result
+// equals its own cycle bit for bit. The matrix-free level-0 operator, which
+// reads no matrix, runs once per case.
+//
+// The smoother's first step is fused into the pass before it: the restriction
+// that forms a level's right-hand side also forms its zero-guess start
+// (d = x = D^-1 b / theta), and the post-smoothing residual also forms the
+// start from the prolongated iterate (d = D^-1 r / theta into a second
+// iterate x2 = x + d, since other rows still read x). The post-smoothed result
+// of a level therefore lands in its second iterate.
 template <typename T>
-struct BlockVecs {
-  DBuf<T>&x, &b, &res, &d, &d2;
+struct CycleVecs {
+  T *x, *x2, *b, *res, *d, *d2;
   const T* dinv;
 };
 template <typename T>
-BlockVecs<T> block_vecs(Level& lv);
+CycleVecs<T> level_vecs(Level& lv, bool block);
 template <>
-BlockVecs<double> block_vecs<double>(Level& lv) {
-  return {lv.vx, lv.vb, lv.vres, lv.vd, lv.vd2, lv.dinv.p};
+CycleVecs<double> level_vecs<double>(Level& lv, bool block) {
+  if (block) return {lv.vx.p, lv.vx2.p, lv.vb.p, lv.vres.p, lv.vd.p, lv.vd2.p, lv.dinv.p};
+  return {lv.x.p, lv.x2.p, lv.b.p, lv.res.p, lv.d.p, lv.d2.p, lv.dinv.p};
 }
 template <>
-BlockVecs<float> block_vecs<float>(Level& lv) {
-  return {lv.vx32, lv.vb32, lv.vres32, lv.vd32, lv.vd232, lv.dinv32.p};
+CycleVecs<float> level_vecs<float>(Level& lv, bool block) {
+  if (block) return {lv.vx32.p, lv.vx232.p, lv.vb32.p, lv.vres32.p, lv.vd32.p, lv.vd232.p, lv.dinv32.p};
+  return {lv.x32.p, lv.x232.p, lv.b32.p, lv.res32.p, lv.d32.p, lv.d232.p, lv.dinv32.p};
 }
+
 // the assembled-operator kernels of either storage, for nv cases
 void csr_residual(const DCsr& A, const double* x, const double* b, double* r, hipStream_t s, int nv) {
   k::residual(A, x, b, r, s, nv);
@@ -285,8 +204,24 @@ void csr_residual(const DCsr& A, const double* x, const double* b, double* r, hi
 void csr_residual(const DCsr& A, const float* x, const float* b, float* r, hipStream_t s, int nv) {
   k::residual32(A, x, b, r, s, nv);
 }
+void csr_residual_first(const DCsr& A, const double* x, const double* b, double* r, const double* dinv, double th,
+                        double* d, double* x2, hipStream_t s, int nv) {
+  k::residual_first(A, x, b, r, dinv, th, d, x2, s, nv);
+}
+void csr_residual_first(const DCsr& A, const float* x, const float* b, float* r, const float* dinv, double th,
+                        float* d, float* x2, hipStream_t s, int nv) {
+  k::residual_first32(A, x, b, r, dinv, th, d, x2, s, nv);
+}
 void csr_apply(const DCsr& A, const double* x, double* y, hipStream_t s, int nv) { k::spmv(A, x, y, s, nv); }
 void csr_apply(const DCsr& A, const float* x, float* y, hipStream_t s, int nv) { k::spmv32(A, x, y, s, nv); }
+void csr_apply_start(const DCsr& A, const double* x, double* y, const double* dinv, double th, double* d, double* x2,
+                     hipStream_t s, int nv) {
+  k::spmv_start(A, x, y, dinv, th, d, x2, s, nv);
+}
+void csr_apply_start(const DCsr& A, const float* x, float* y, const float* dinv, double th, float* d, float* x2,
+                     hipStream_t s, int nv) {
+  k::spmv_start32(A, x, y, dinv, th, d, x2, s, nv);
+}
 void csr_apply_add(const DCsr& A, const double* x, double* y, hipStream_t s, int nv) { k::spmv_add(A, x, y, s, nv); }
 void csr_apply_add(const DCsr& A, const float* x, float* y, hipStream_t s, int nv) { k::spmv_add32(A, x, y, s, nv); }
 void csr_cheb(const DCsr& A, const double* dinv, const double* ri, double* ro, const double* di, double* dout,
@@ -303,12 +238,6 @@ void first_zero(int n, const double* dinv, const double* b, double th, double* x
 void first_zero(int n, const float* dinv, const float* b, double th, float* x, float* d, hipStream_t s, int nv) {
   k::cheb_first_zero32(n, dinv, b, th, x, d, s, nv);
 }
-void first(int n, const double* dinv, const double* r, double th, double* x, double* d, hipStream_t s, int nv) {
-  k::cheb_first(n, dinv, r, th, x, d, s, nv);
-}
-void first(int n, const float* dinv, const float* r, double th, float* x, float* d, hipStream_t s, int nv) {
-  k::cheb_first32(n, dinv, r, th, x, d, s, nv);
-}
 void coarse_solve(Context& c, int n, const double* b, double* x, int nv) {
   k::dense_gemv(n, c.coarse_inv.p, b, x, c.stream, nv);
 }
@@ -316,6 +245,7 @@ void coarse_solve(Context& c, int n, const float* b, float* x, int nv) {
   k::dense_gemv32(n, c.coarse_inv32.p, b, x, c.stream, nv);
 }
 
+// r = b - A_l x for nv cases
 template <typename T>
 void block_residual(Context& c, size_t l, const T* x, const T* b, T* r, int nv) {
   if (use_mf(c, l)) {
@@ -326,10 +256,12 @@ void block_residual(Context& c, size_t l, const T* x, const T* b, T* r, int nv) 
   }
 }
 
+// Chebyshev steps 1..degree-1 from x and d (Multigrid.cpp:833-853); r_first is
+// the residual the first step starts from. d and d2 swap roles per step.
 template <typename T>
-void block_cheb_steps(Context& c, size_t l, const T* r_first, T* x, int nv) {
+void cheb_steps(Context& c, size_t l, const T* r_first, T* x, T* d, T* d2, T* res, const T* dinv, int nv,
+                const char* timer) {
   Level& lv = c.levels[l];
-  BlockVecs<T> bv = block_vecs<T>(lv);
   DCsr& A = op(c, l);
   const Cheb cp = cheb_params(c, lv);
   const long long n = lv.n;
@@ -339,74 +271,115 @@ void block_cheb_steps(Context& c, size_t l, const T* r_first, T* x, int nv) {
     const double rho_new = 1.0 / (2.0 * cp.sigma - rho);
     const double c1 = rho_new * rho;
     const double c2 = 2.0 * rho_new / cp.delta;
-    const T* rin = k == 1 ? r_first : bv.res.p;
+    const T* rin = k == 1 ? r_first : res;
+    Timed t(c, nv == 1 ? timer : nullptr);
     if (use_mf(c, l)) {
       for (int v = 0; v < nv; ++v) {
         kmf::MfArgs<T> a = mf_base<T>(c);
-        a.x = bv.d.p + v * n, a.b = rin + v * n, a.y = bv.res.p + v * n, a.dinv = bv.dinv;
-        a.d_out = bv.d2.p + v * n, a.xs = x + v * n, a.c1 = c1, a.c2 = c2;
+        a.x = d + v * n, a.b = rin + v * n, a.y = res + v * n, a.dinv = dinv;
+        a.d_out = d2 + v * n, a.xs = x + v * n, a.c1 = c1, a.c2 = c2;
         kmf::apply(a, 3, s);
       }
     } else {
-      csr_cheb(A, bv.dinv, rin, bv.res.p, bv.d.p, bv.d2.p, x, c1, c2, s, nv);
+      csr_cheb(A, dinv, rin, res, d, d2, x, c1, c2, s, nv);
     }
-    std::swap(bv.d, bv.d2);
+    std::swap(d, d2);
     rho = rho_new;
   }
 }
 
-// started: x and d already hold the zero-guess smoother start
+// V-cycle from level l for right-hand side b (Multigrid.cpp:857-885). x is the
+// pre-smoothing iterate and x_out receives the result (the second iterate of
+// the fused post-smoothing start); started: x and the level's d already hold
+// the zero-guess smoother start. Returns x_out.
 template <typename T>
-void block_cycle(Context& c, size_t l, const T* b, T* x, int nv, bool started = false) {
+T* cycle(Context& c, size_t l, const T* b, T* x, T* x_out, int nv, bool block, bool started) {
   Level& lv = c.levels[l];
-  BlockVecs<T> bv = block_vecs<T>(lv);
   hipStream_t s = c.stream;
   if (l + 1 == c.levels.size()) {
-    coarse_solve(c, lv.n, b, x, nv);
-    return;
+    coarse_solve(c, lv.n, b, x_out, nv);
+    return x_out;
   }
-  const Cheb cp = cheb_params(c, lv);
+  const CycleVecs<T> v = level_vecs<T>(lv, block);
   const long long n = lv.n;
-  if (!started) first_zero(lv.n, bv.dinv, b, cp.theta, x, bv.d.p, s, nv);
-  block_cheb_steps<T>(c, l, b, x, nv);
-  block_residual<T>(c, l, x, b, bv.res.p, nv);
-  Level& nx = c.levels[l + 1];
-  BlockVecs<T> nv_ = block_vecs<T>(nx);
-  const long long nc = nx.n;
-  if (use_mf(c, l)) {
-    for (int v = 0; v < nv; ++v) restrict_mf<T>(c, bv.res.p + v * n, bv.d.p + v * n, nv_.b.p + v * nc);
-  } else {
-    csr_apply(lv.R, bv.res.p, nv_.b.p, s, nv);
+  const bool mf = use_mf(c, l);
+  const char* cheb_timer = l == 0 ? "cheb" : l == 1 ? "cheb_l1" : nullptr;
+  const char* res_timer = l == 0 ? "residual" : nullptr;
+  const Cheb cp = cheb_params(c, lv);
+  // pre-smoothing from a zero guess: x = d = D^-1 b / theta, r starts as b
+  if (!started) first_zero(lv.n, v.dinv, b, cp.theta, x, v.d, s, nv);
+  cheb_steps<T>(c, l, b, x, v.d, v.d2, v.res, v.dinv, nv, cheb_timer);
+  {
+    Timed t(c, nv == 1 ? res_timer : nullptr);
+    block_residual<T>(c, l, x, b, v.res, nv);
   }
-  block_cycle<T>(c, l + 1, nv_.b.p, nv_.x.p, nv);
-  if (use_mf(c, l)) {
-    for (int v = 0; v < nv; ++v) prolong_mf<T>(c, nv_.x.p + v * nc, bv.d.p + v * n, x + v * n);
+  // the smoother's directions are free between pre- and post-smoothing: d2
+  // serves as the transfers' scratch
+  T* const scratch = v.d2;
+  Level& nl = c.levels[l + 1];
+  const CycleVecs<T> w = level_vecs<T>(nl, block);
+  const long long nc = nl.n;
+  // a level above the coarsest starts its smoother in the restriction's pass
+  const bool start_next = l + 2 < c.levels.size();
+  const double th_next = start_next ? cheb_params(c, nl).theta : 1.0;
+  if (mf) {
+    for (int q = 0; q < nv; ++q) {
+      if (start_next) restrict_mf<T>(c, v.res + q * n, scratch + q * n, w.b + q * nc, w.dinv, th_next, w.d + q * nc,
+                                     w.x + q * nc);
+      else restrict_mf<T>(c, v.res + q * n, scratch + q * n, w.b + q * nc);
+    }
+  } else if (start_next) {
+    csr_apply_start(lv.R, v.res, w.b, w.dinv, th_next, w.d, w.x, s, nv);
   } else {
-    csr_apply_add(lv.P, nv_.x.p, x, s, nv);
+    csr_apply(lv.R, v.res, w.b, s, nv);
   }
-  block_residual<T>(c, l, x, b, bv.res.p, nv);
-  first(lv.n, bv.dinv, bv.res.p, cp.theta, x, bv.d.p, s, nv);
-  block_cheb_steps<T>(c, l, bv.res.p, x, nv);
+  const T* xc = cycle<T>(c, l + 1, w.b, w.x, w.x2, nv, block, start_next);
+  if (mf) {
+    for (int q = 0; q < nv; ++q) prolong_mf<T>(c, xc + q * nc, scratch + q * n, x + q * n);
+  } else {
+    csr_apply_add(lv.P, xc, x, s, nv);
+  }
+  // post-smoothing from the current x: its residual and the smoother's start
+  // (d = D^-1 r / theta, x_out = x + d) in one pass
+  {
+    Timed t(c, nv == 1 ? res_timer : nullptr);
+    if (mf) {
+      for (int q = 0; q < nv; ++q) {
+        kmf::MfArgs<T> a = mf_base<T>(c);
+        a.x = x + q * n, a.b = b + q * n, a.y = v.res + q * n, a.dinv = v.dinv;
+        a.d_out = v.d + q * n, a.xs = x_out + q * n, a.c1 = cp.theta;
+        kmf::apply(a, 7, s);
+      }
+    } else {
+      csr_residual_first(op(c, l), x, b, v.res, v.dinv, cp.theta, v.d, x_out, s, nv);
+    }
+  }
+  cheb_steps<T>(c, l, v.res, x_out, v.d, v.d2, v.res, v.dinv, nv, cheb_timer);
+  return x_out;
 }
 
-// z = M^-1 r for nv cases: the FP64 cycle into z (returns null), or the FP32
-// cycle, whose result stays in level 0's FP32 block iterate (returned)
-const float* block_vcycle_raw(Context& c, const double* r, double* z, int nv) {
+// z = M^-1 r for nv cases (block: the level block vectors). The FP64 cycle
+// writes z (returns null); the FP32 cycle returns level 0's FP32 result.
+const float* vcycle_cases(Context& c, const double* r, double* z, int nv, bool block) {
   Level& top = c.levels[0];
   const int n = top.n;
   if (c.amg_opt.precision >= 1) {
+    const CycleVecs<float> v = level_vecs<float>(top, block);
     if (c.levels.size() > 1) {
-      k::cheb_first_zero_from64(n, top.dinv32.p, r, cheb_params(c, top).theta, top.vb32.p, top.vx32.p,
-                                top.vd32.p, c.stream, nv);
-      block_cycle<float>(c, 0, top.vb32.p, top.vx32.p, nv, true);
-    } else {
-      k::to_float((long long)n * nv, r, top.vb32.p, c.stream);
-      block_cycle<float>(c, 0, top.vb32.p, top.vx32.p, nv);
+      // b = (float) r and the smoother's zero-guess start in one pass
+      k::cheb_first_zero_from64(n, v.dinv, r, cheb_params(c, top).theta, v.b, v.x, v.d, c.stream, nv);
+      return cycle<float>(c, 0, v.b, v.x, v.x2, nv, block, true);
     }
-    return top.vx32.p;
+    k::to_float((long long)n * nv, r, v.b, c.stream);
+    return cycle<float>(c, 0, v.b, v.x, v.x2, nv, block, false);
   }
-  block_cycle<double>(c, 0, r, z, nv);
+  const CycleVecs<double> v = level_vecs<double>(top, block);
+  cycle<double>(c, 0, r, v.x, z, nv, block, false);
   return nullptr;
+}
+
+const float* block_vcycle_raw(Context& c, const double* r, double* z, int nv) {
+  return vcycle_cases(c, r, z, nv, true);
 }
 
 }  // namespace
@@ -634,6 +607,7 @@ void prepare_mixed(Context& c) {
     lv.dinv32.alloc((size_t)lv.n);
     k::to_float(lv.n, lv.dinv.p, lv.dinv32.p, s);
     lv.x32.alloc((size_t)lv.n);
+    lv.x232.alloc((size_t)lv.n);
     lv.b32.alloc((size_t)lv.n);
     lv.res32.alloc((size_t)lv.n);
     lv.d32.alloc((size_t)lv.n);
@@ -646,22 +620,7 @@ void prepare_mixed(Context& c) {
 const float* vcycle_raw(Context& c, const double* r, double* z) {
   if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: apply() called before setup()");
   finish_coarse(c);
-  Level& top = c.levels[0];
-  const int n = top.n;
-  if (c.amg_opt.precision >= 1) {
-    if (c.levels.size() > 1) {
-      // b = (float) r and the smoother's zero-guess start in one pass
-      k::cheb_first_zero_from64(n, top.dinv32.p, r, cheb_params(c, top).theta, top.b32.p, top.x32.p, top.d32.p,
-                                c.stream);
-      cycle32(c, 0, top.b32.p, top.x32.p, true);
-    } else {
-      k::to_float((long long)n, r, top.b32.p, c.stream);
-      cycle32(c, 0, top.b32.p, top.x32.p);
-    }
-    return top.x32.p;
-  }
-  cycle64(c, 0, r, z);
-  return nullptr;
+  return vcycle_cases(c, r, z, 1, false);
 }
 
 void vcycle(Context& c, const double* r, double* z) {
@@ -869,10 +828,10 @@ void ensure_block_workspace(Context& c, int nv) {
   const bool f32 = c.amg_opt.precision >= 1;
   for (Level& lv : c.levels) {
     const size_t m = (size_t)lv.n * (size_t)nv;
-    for (DBuf<double>* v : {&lv.vx, &lv.vb, &lv.vres, &lv.vd, &lv.vd2}) {
+    for (DBuf<double>* v : {&lv.vx, &lv.vx2, &lv.vb, &lv.vres, &lv.vd, &lv.vd2}) {
       if (f32) v->release(); else v->alloc(m);
     }
-    for (DBuf<float>* v : {&lv.vx32, &lv.vb32, &lv.vres32, &lv.vd32, &lv.vd232}) {
+    for (DBuf<float>* v : {&lv.vx32, &lv.vx232, &lv.vb32, &lv.vres32, &lv.vd32, &lv.vd232}) {
       if (f32) v->alloc(m); else v->release();
     }
   }
