@@ -9,6 +9,7 @@
 #include <sstream>
 
 #include "slx_kernels.hpp"
+#include "slx_mf.hpp"
 #include "slx_modal.hpp"
 #include "slx_model.hpp"
 #include "slx_net.hpp"
@@ -583,6 +584,11 @@ slx_status slx_model_finalize(slx_context* ctx) {
     keep_host_pattern(c, std::move(pat));
     c.d_free.upload(c.free_dofs, c.stream);
     c.d_reduced.upload(c.reduced, c.stream);
+    if (c.dim == 3) {
+      std::vector<int> map;
+      if (!kmf::node_map(c.reduced, map)) fail(SLX_MODEL_ERROR, "the model has more than 2^28 free DOFs");
+      c.d_nodemap.upload(map, c.stream);
+    }
     c.d_ke0.upload(c.ke0, c.stream);
     c.d_vol.upload(c.volumes, c.stream);
     set_layout_from_model(c);

@@ -75,8 +75,42 @@ constexpr int kElemPad = 416;  // the tile's 405 elements in 26 column tiles of 
 // (m, g) = (lane % 16, lane / 16) holds row m, columns 8g..8g+7; the second
 // operand's holds rows 8g..8g+7 of column m; the result's lane holds rows
 // 4g..4g+3 of column m.
+// A node's free DOFs are consecutive in the free numbering, so the packed map
+// nodemap[node] = (first free index << 3) | (mask of its free components) locates
+// all three with one load. Component c sits at base + popcount(mask below c); a
+// prescribed one reads entry base (or 0) and is zeroed, so no branch splits the loads.
+template <typename T>
+__device__ __forceinline__ void load3(const T* __restrict__ v, int m, T& a0, T& a1, T& a2) {
+  const int base = m >> 3, mask = m & 7;
+  const int i1 = base + (mask & 1), i2 = i1 + ((mask >> 1) & 1);
+  const T v0 = v[base], v1 = v[(mask & 2) ? i1 : base], v2 = v[(mask & 4) ? i2 : base];
+  a0 = (mask & 1) ? v0 : T(0);
+  a1 = (mask & 2) ? v1 : T(0);
+  a2 = (mask & 4) ? v2 : T(0);
+}
+// stores the free components; a node with all three free takes one 3-wide store
+template <typename T>
+__device__ __forceinline__ void store3(T* __restrict__ v, int m, const T (&o)[3]) {
+  const int base = m >> 3, mask = m & 7;
+  if (mask == 7) {
+    T* p = v + base;
+    p[0] = o[0];
+    p[1] = o[1];
+    p[2] = o[2];
+  } else {
+    int i = base;
+    if (mask & 1) v[i++] = o[0];
+    if (mask & 2) v[i++] = o[1];
+    if (mask & 4) v[i] = o[2];
+  }
+}
+
+// FP32 vector-FMA variants are held to 80 registers (6 waves per SIMD): the step is
+// bound by how many tiles' staging and products overlap, and the few registers
+// that spill cost less than the lost occupancy
 template <typename T, int MODE, bool MX>
-__global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !MX ? 6 : 1)))
+k_mf_hex8(MfArgs<T> a) {
   static_assert(!MX || sizeof(T) == 4, "the matrix-core path is FP32");
   using V4 = typename Vec4<T>::type;
   __shared__ V4 s_u[HZ * HY * HX];
@@ -93,15 +127,9 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
   const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
   const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
   const bool inside = gx < a.X && gy < a.Y && gz < a.Z;
-  // the node's free indices first: the epilogue's operands load before the
-  // element products, which hide their latency
-  int fi[3] = {-1, -1, -1};
-  if (inside) {
-    const int* f = a.reduced + 3 * (((long long)gz * a.Y + gy) * a.X + gx);
-    fi[0] = f[0];
-    fi[1] = f[1];
-    fi[2] = f[2];
-  }
+  // the node's map first: the epilogue's operands load before the element
+  // products, which hide their latency
+  const int me = inside ? a.nodemap[((long long)gz * a.Y + gy) * a.X + gx] : 0;
   // Stage the input (scaled for the Lanczos mode) over the tile and its halo,
   // and the factors of the tile's elements. Every load is issued before any
   // is used (indices, then values): entries outside the mesh load node 0 and
@@ -109,16 +137,15 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
   constexpr int kHalo = HZ * HY * HX, kHaloIt = (kHalo + kThreads - 1) / kThreads;
   constexpr int kElem = EZ * EY * EX, kElemIt = (kElem + kThreads - 1) / kThreads;
   const int nex = a.X - 1, ney = a.Y - 1, nez = a.Z - 1;  // elements per direction
-  int hf[kHaloIt][3];
+  int hm[kHaloIt];
 #pragma unroll
   for (int it = 0; it < kHaloIt; ++it) {
     const int h = threadIdx.x + it * kThreads;
     const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
     const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
     const bool ok = h < kHalo && gx >= 0 && gx < a.X && gy >= 0 && gy < a.Y && gz >= 0 && gz < a.Z;
-    const int* f = a.reduced + 3 * (ok ? (((long long)gz * a.Y + gy) * a.X + gx) : 0);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) hf[it][c] = ok ? f[c] : -1;
+    const int m = a.nodemap[ok ? (((long long)gz * a.Y + gy) * a.X + gx) : 0];
+    hm[it] = ok ? m : 0;
   }
   T sv[kElemIt];
 #pragma unroll
@@ -132,14 +159,16 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
   }
   T hu[kHaloIt][3];
 #pragma unroll
-  for (int it = 0; it < kHaloIt; ++it)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int f = hf[it][c] < 0 ? 0 : hf[it][c];
-      T v = a.x[f];
-      if (MODE == 4 || MODE == 5) v *= a.in_scale[f];
-      hu[it][c] = hf[it][c] < 0 ? T(0) : v;
+  for (int it = 0; it < kHaloIt; ++it) {
+    load3(a.x, hm[it], hu[it][0], hu[it][1], hu[it][2]);
+    if (MODE == 4 || MODE == 5) {
+      T s0, s1, s2;
+      load3(a.in_scale, hm[it], s0, s1, s2);
+      hu[it][0] *= s0;
+      hu[it][1] *= s1;
+      hu[it][2] *= s2;
     }
+  }
 #pragma unroll
   for (int it = 0; it < kHaloIt; ++it) {
     const int h = threadIdx.x + it * kThreads;
@@ -157,22 +186,15 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
     const int h = threadIdx.x + it * kThreads;
     if (h < kElem) s_s[h] = sv[it];
   }
-  // (prescribed DOFs load entry 0 and discard it: no branches around the loads)
+  // (prescribed components read as zero: no branches around the loads)
   T px[3] = {0, 0, 0}, pb[3] = {0, 0, 0}, pd[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const int g = fi[c] < 0 ? 0 : fi[c];
-    if (MODE == 1 || MODE == 3 || MODE == 4 || MODE == 5 || MODE == 6 || MODE == 7) px[c] = a.x[g];
-    if (MODE == 2 || MODE == 3 || MODE == 7) pb[c] = a.b[g];
-    if (MODE == 7) pd[c] = a.dinv[g];
-    if (MODE == 3 || MODE == 6) {
-      pd[c] = a.dinv[g];
-      ps[c] = a.xs[g];
-    }
-    if (MODE == 4) {
-      pd[c] = a.in_scale[g];
-      ps[c] = a.vprev[g];
-    }
+  if (MODE == 1 || MODE == 3 || MODE == 4 || MODE == 5 || MODE == 6 || MODE == 7) load3(a.x, me, px[0], px[1], px[2]);
+  if (MODE == 2 || MODE == 3 || MODE == 7) load3(a.b, me, pb[0], pb[1], pb[2]);
+  if (MODE == 3 || MODE == 6 || MODE == 7) load3(a.dinv, me, pd[0], pd[1], pd[2]);
+  if (MODE == 3 || MODE == 6) load3(a.xs, me, ps[0], ps[1], ps[2]);
+  if (MODE == 4) {
+    load3(a.in_scale, me, pd[0], pd[1], pd[2]);
+    load3(a.vprev, me, ps[0], ps[1], ps[2]);
   }
   __syncthreads();
   T acc0 = 0, acc1 = 0, acc2 = 0;
@@ -305,47 +327,51 @@ __global__ void __launch_bounds__(kThreads) k_mf_hex8(MfArgs<T> a) {
   double dot = 0.0;
   {
     const T acc[3] = {acc0, acc1, acc2};
+    const int mask = me & 7;
+    T o1[3], o2[3], o3[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const int f = fi[c];
-      if (f < 0) continue;
+      const bool free_c = (mask >> c) & 1;
       const T y = acc[c];
       if (MODE == 0) {
-        a.y[f] = y;
+        o1[c] = y;
       } else if (MODE == 1) {
-        a.y[f] = y;
-        dot += (double)px[c] * (double)y;
+        o1[c] = y;
+        if (free_c) dot += (double)px[c] * (double)y;
       } else if (MODE == 2) {
-        a.y[f] = pb[c] - y;
+        o1[c] = pb[c] - y;
       } else if (MODE == 3) {
         // r_out = r_in - K d_in, d_out = c1 d_in + c2 D^-1 r_out, x += d_out (x input = d_in)
         const double r = (double)pb[c] - (double)y;
         const double dn = a.c1 * (double)px[c] + a.c2 * (double)pd[c] * r;
-        a.y[f] = (T)r;
-        a.d_out[f] = (T)dn;
-        a.xs[f] = (T)((double)ps[c] + dn);
+        o1[c] = (T)r;
+        o2[c] = (T)dn;
+        o3[c] = (T)((double)ps[c] + dn);
       } else if (MODE == 4) {
         // Lanczos: w = s (K (s v)) - beta vprev, with the partials of w.v
         const double beta = a.beta ? *a.beta : 0.0;
         const double wv = (double)y * (double)pd[c] - beta * (double)ps[c];
-        a.y[f] = (T)wv;
-        dot += wv * (double)px[c];
+        o1[c] = (T)wv;
+        if (free_c) dot += wv * (double)px[c];
       } else if (MODE == 7) {
         // post-smoothing residual and the smoother's start from x:
         // r = b - K x, d = D^-1 r / theta, x2 = x + d (k_cheb_first's rounding)
         const T r = pb[c] - y;
         const double di = (double)pd[c] * (double)r / a.c1;
-        a.y[f] = r;
-        a.d_out[f] = (T)di;
-        a.xs[f] = (T)((double)px[c] + di);
+        o1[c] = r;
+        o2[c] = (T)di;
+        o3[c] = (T)((double)px[c] + di);
       } else if (MODE == 5) {
         // smoothed restriction, first half: y = (I - omega K D^-1) x
-        a.y[f] = (T)((double)px[c] - a.c1 * (double)y);
+        o1[c] = (T)((double)px[c] - a.c1 * (double)y);
       } else {
         // smoothed prolongation, second half: xs += (I - omega D^-1 K) x
-        a.xs[f] = (T)((double)ps[c] + (double)px[c] - a.c1 * (double)pd[c] * (double)y);
+        o3[c] = (T)((double)ps[c] + (double)px[c] - a.c1 * (double)pd[c] * (double)y);
       }
     }
+    if (MODE != 6) store3(a.y, me, o1);
+    if (MODE == 3 || MODE == 7) store3(a.d_out, me, o2);
+    if (MODE == 3 || MODE == 6 || MODE == 7) store3(a.xs, me, o3);
   }
   if (MODE == 1 || (MODE == 4 && a.partials)) {
     const double s = block_sum<T>(dot, s_red);
@@ -400,6 +426,24 @@ void inv_diag(int X, int Y, int Z, const int* reduced, const double* scale, cons
               int* bad_row, hipStream_t s) {
   const long long nn = (long long)X * Y * Z;
   if (nn) k_mf_inv_diag<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(X, Y, Z, reduced, scale, ke, dinv, bad_row);
+}
+
+bool node_map(const std::vector<int>& reduced, std::vector<int>& map) {
+  const size_t nn = reduced.size() / 3;
+  map.assign(nn, 0);
+  for (size_t n = 0; n < nn; ++n) {
+    int base = -1, mask = 0;
+    for (int k = 0; k < 3; ++k) {
+      const int r = reduced[3 * n + (size_t)k];
+      if (r < 0) continue;
+      if (base < 0) base = r;
+      mask |= 1 << k;
+    }
+    if (mask == 0) continue;
+    if (base >= (1 << 28)) return false;
+    map[n] = (base << 3) | mask;
+  }
+  return true;
 }
 
 int blocks(int X, int Y, int Z) {

@@ -213,6 +213,7 @@ struct Context {
   std::vector<double> case_compliance;  // f_l . u_l of the last evaluation
   // device model state
   DBuf<int> d_free, d_reduced;
+  DBuf<int> d_nodemap;  // Hex8: packed per-node free map of the matrix-free kernels
   DBuf<double> d_ke0, d_scale, d_loads_free, d_weights;  // loads restricted to the free DOFs (ncases x nfree)
   DBuf<double> d_u;                           // last solutions (ncases x nfree)
   std::vector<char> have_u;

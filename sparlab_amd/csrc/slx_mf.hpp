@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace slx {
 namespace kmf {
 
@@ -18,13 +20,15 @@ namespace kmf {
 //   6  prolongation smoothing: xs += x - c1 dinv (K x)
 //   7  residual and the smoother's start: y = r = b - K x, d_out = dinv r / c1
 //      (c1 = theta), xs = x + d_out (xs another buffer than x)
-// X, Y, Z are the node counts per direction; reduced maps DOF = node * 3 + k to
-// the free index or -1; scale holds the element factors s_e and ke the 24 x 24
-// element matrix (node-major DOF order, VTK corner numbering).
+// X, Y, Z are the node counts per direction; nodemap[node] = (free index of the
+// node's first free DOF << 3) | (bit k set when DOF node * 3 + k is free), 0 for a
+// node with every DOF prescribed (node_map() builds it); scale holds the element
+// factors s_e and ke the 24 x 24 element matrix (node-major DOF order, VTK corner
+// numbering).
 template <typename T>
 struct MfArgs {
   int X, Y, Z;
-  const int* reduced;
+  const int* nodemap;
   const T* scale;
   const T* ke;
   const T* x;
@@ -41,6 +45,9 @@ struct MfArgs {
 };
 
 int blocks(int X, int Y, int Z);
+// the packed node map from DOF -> free index (-1 prescribed), 3 DOFs per node;
+// false when a free index does not fit in 28 bits
+bool node_map(const std::vector<int>& reduced, std::vector<int>& map);
 // 1 / diag(K_ff) from the element factors (the assembled diagonal bit for bit);
 // a non-positive entry lowers *bad_row to its row
 void inv_diag(int X, int Y, int Z, const int* reduced, const double* scale, const double* ke, double* dinv,

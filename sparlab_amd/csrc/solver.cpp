@@ -41,7 +41,7 @@ template <>
 kmf::MfArgs<double> mf_base<double>(const Context& c) {
   kmf::MfArgs<double> a{};
   a.X = c.nx + 1, a.Y = c.ny + 1, a.Z = c.nz + 1;
-  a.reduced = c.d_reduced.p;
+  a.nodemap = c.d_nodemap.p;
   a.scale = c.mf_scale.p;
   a.ke = c.d_ke0.p;
   return a;
@@ -50,7 +50,7 @@ template <>
 kmf::MfArgs<float> mf_base<float>(const Context& c) {
   kmf::MfArgs<float> a{};
   a.X = c.nx + 1, a.Y = c.ny + 1, a.Z = c.nz + 1;
-  a.reduced = c.d_reduced.p;
+  a.nodemap = c.d_nodemap.p;
   a.scale = c.mf_scale32.p;
   a.ke = c.mf_ke32.p;
   return a;
