@@ -71,7 +71,7 @@ typedef struct {
   double strength_threshold;     /* 0.02 */
   int max_levels;                /* 12 */
   int coarse_size;               /* 1500 */
-  int smoother;                  /* 0 Chebyshev (only this one runs on the GPU) */
+  int smoother;                  /* 0 Chebyshev, 1 symmetric Gauss-Seidel (sweeps by dependency level) */
   int smoother_degree;           /* 3 */
   double chebyshev_ratio;        /* 30 */
   double prolongator_damping;    /* 4/3 */

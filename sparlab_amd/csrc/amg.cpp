@@ -1390,6 +1390,63 @@ double lanczos_lambda(const std::vector<double>& h, int m) {
   return tridiag_max(a, sub);
 }
 
+// Dependency levels of the symmetric Gauss-Seidel sweeps of one level
+// (Multigrid.cpp:788-806): in the forward sweep row i reads the new values of
+// the rows j < i in its pattern, so its level is one above the highest of
+// theirs; in the backward sweep likewise for the rows j > i. Rows of one level
+// never read each other's unknowns. Each level's rows are kept ascending.
+void gs_levels(const int* ptr, const int* col, int n, bool forward, std::vector<int>& lptr, std::vector<int>& rows) {
+  std::vector<int> lev((size_t)n, 0);
+  int nl = 0;
+  for (int q = 0; q < n; ++q) {
+    const int i = forward ? q : n - 1 - q;
+    int m = 0;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const int j = col[k];
+      if (forward ? j < i : j > i) m = std::max(m, lev[(size_t)j] + 1);
+    }
+    lev[(size_t)i] = m;
+    nl = std::max(nl, m + 1);
+  }
+  lptr.assign((size_t)nl + 1, 0);
+  for (int i = 0; i < n; ++i) ++lptr[(size_t)lev[(size_t)i] + 1];
+  for (int L = 0; L < nl; ++L) lptr[(size_t)L + 1] += lptr[(size_t)L];
+  rows.assign((size_t)n, 0);
+  std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+  for (int i = 0; i < n; ++i) rows[(size_t)fill[(size_t)lev[(size_t)i]]++] = i;
+}
+
+// the sweeps' schedules of every smoothed level, built once per structure from
+// the level's pattern (level 0: the host copy of K's pattern when it is current)
+void gs_schedules(Context& c) {
+  hipStream_t s = c.stream;
+  for (size_t l = 0; l + 1 < c.levels.size(); ++l) {
+    Level& lv = c.levels[l];
+    if (!lv.gs_fptr.empty() && lv.gs_frows.n == (size_t)lv.n) continue;
+    const DCsr& A = level_op(c, l);
+    std::vector<int> hptr, hcol;
+    const int *ptr, *col;
+    if (l == 0 && c.k_host_version == c.k_pattern_version && c.k_host.rows == A.rows) {
+      ptr = c.k_host.ptr.data();
+      col = c.k_host.col.data();
+    } else {
+      hptr.resize((size_t)A.rows + 1);
+      hcol.resize((size_t)A.nnz);
+      A.ptr.download(hptr.data(), hptr.size(), s);
+      A.col.download(hcol.data(), hcol.size(), s);
+      SLX_HIP(hipStreamSynchronize(s));
+      ptr = hptr.data();
+      col = hcol.data();
+    }
+    std::vector<int> rows;
+    gs_levels(ptr, col, A.rows, true, lv.gs_fptr, rows);
+    lv.gs_frows.upload(rows, s);
+    gs_levels(ptr, col, A.rows, false, lv.gs_bptr, rows);
+    lv.gs_brows.upload(rows, s);
+    SLX_HIP(hipStreamSynchronize(s));
+  }
+}
+
 // D^-1 and lambda_max(D^-1 A) of level l with one read from the device: the
 // Lanczos scalars, the level's diagonal flag and, at level 0, the finiteness
 // sum of the matrix queued by amg_setup.
@@ -1608,7 +1665,8 @@ void amg_setup(Context& c, bool defer_coarse) {
     fail(SLX_SOLVER_ERROR, "multigrid: unsupported DOFs per node (translations only, or 3 + 3 in 3-D)");
   if ((int)c.layout_unknowns.size() != n) fail(SLX_SOLVER_ERROR, "multigrid: the DOF layout does not match the matrix");
   const slx_amg_options& o = c.amg_opt;
-  if (o.smoother != 0) fail(SLX_CONFIG_ERROR, "multigrid: only the Chebyshev smoother runs on the GPU");
+  if (o.smoother != 0 && o.smoother != 1)
+    fail(SLX_CONFIG_ERROR, "multigrid: unknown smoother (0 Chebyshev, 1 symmetric Gauss-Seidel)");
   ensure_workspace(c);
   const bool same = c.amg_ready && c.fp_rows == n && c.fp_nnz == c.K.nnz && c.fp_layout_version == c.layout_version &&
                     c.fp_pattern_version == c.k_pattern_version;
@@ -1818,7 +1876,12 @@ void amg_setup(Context& c, bool defer_coarse) {
   // finish_coarse before the first V-cycle)
   {
     const int nc = c.levels.back().n;
-    if (nc > 8000) fail(SLX_SOLVER_ERROR, "multigrid: the coarsest level exceeds the dense solver's 8000 unknowns");
+    if (nc > kDenseCoarseMax) {
+      std::ostringstream os;
+      os << "multigrid: the coarsest level has " << nc << " unknowns, above the " << kDenseCoarseMax
+         << " of the device's dense coarse solver; lower amg.coarse_size or raise amg.max_levels";
+      fail(SLX_SOLVER_ERROR, os.str());
+    }
     const DCsr& ac = level_op(c, l);
     c.coarse_inv.alloc((size_t)nc * nc);
     c.coarse_piv.alloc((size_t)nc);
@@ -1840,6 +1903,7 @@ void amg_setup(Context& c, bool defer_coarse) {
     lv.d.alloc((size_t)lv.n);
     lv.d2.alloc((size_t)lv.n);
   }
+  if (o.smoother == 1) gs_schedules(c);
   if (o.precision >= 1) prepare_mixed(c);
   if (!reuse || c.tuned_precision != o.precision) {
     autotune(c);

@@ -169,7 +169,7 @@ slx_pcg_result solve_case(Context& c, int l, bool warm, double tol, int maxit) {
 void require_converged(Context& c, const slx_pcg_result& r, double tol) {
   if (r.converged) return;
   std::ostringstream os;
-  os << "AMG-CG(chebyshev) did not converge: " << r.iterations << " iterations left a relative residual of "
+  os << "AMG-CG(" << (c.amg_opt.smoother == 1 ? "gauss_seidel" : "chebyshev") << ") did not converge: " << r.iterations << " iterations left a relative residual of "
      << r.relative_residual << " against a tolerance of " << tol << " (" << c.levels.size()
      << " levels). Raise solver.linear.max_iterations, lower solver.linear.amg.strength_threshold, raise "
         "amg.smoother_degree, or use \"simplicial_ldlt\"";
@@ -696,7 +696,12 @@ slx_status slx_amg_setup(slx_context* ctx, const slx_amg_options* opt) {
     if (!(o.strength_threshold >= 0.0 && o.strength_threshold < 1.0)) fail(SLX_CONFIG_ERROR, "multigrid strength_threshold must lie in [0, 1)");
     if (o.max_levels < 1 || o.max_levels > 16) fail(SLX_CONFIG_ERROR, "multigrid max_levels must lie in [1, 16]");
     if (o.coarse_size < 1) fail(SLX_CONFIG_ERROR, "multigrid coarse_size must be positive");
+    if (o.smoother != 0 && o.smoother != 1)
+      fail(SLX_CONFIG_ERROR, "multigrid smoother must be 0 (chebyshev) or 1 (gauss_seidel)");
     if (o.smoother_degree < 1) fail(SLX_CONFIG_ERROR, "multigrid smoother_degree must be at least 1");
+    if (o.smoother == 1 && o.coarse_reuse > 0)
+      fail(SLX_CONFIG_ERROR, "multigrid coarse_reuse needs the Chebyshev smoother (the Gauss-Seidel sweeps read the "
+                             "assembled level-0 operator, which a reused evaluation does not assemble)");
     if (!(o.chebyshev_ratio > 1.0)) fail(SLX_CONFIG_ERROR, "multigrid chebyshev_ratio must exceed 1");
     if (!(o.prolongator_damping > 0.0 && o.prolongator_damping < 2.0)) fail(SLX_CONFIG_ERROR, "multigrid prolongator_damping must lie in (0, 2)");
     if (o.lanczos_steps < 2) fail(SLX_CONFIG_ERROR, "multigrid lanczos_steps must be at least 2");

@@ -85,6 +85,10 @@ struct DBuf {
   }
 };
 
+// Largest coarsest level the device's dense coarse solver takes: its FP64 inverse
+// (n^2 doubles, 2 GiB at this size) is formed by blocked Gauss-Jordan at every setup.
+constexpr int kDenseCoarseMax = 16384;
+
 // Device CSR matrix (int32 indices, FP64 values, optional FP32 and bfloat16 copies of the values).
 struct DCsr {
   int rows = 0, cols = 0;
@@ -170,6 +174,10 @@ struct Level {
   DBuf<int> tent_c0len, tent_agg_ptr, tent_agg_rows, tent_agg_c0len;
   DBuf<double> tent_ell, tent_ella;  // by fine row; in the aggregates' member order
   DBuf<float> tent_ell32, tent_ella32;
+  // symmetric Gauss-Seidel smoother: the rows grouped by dependency level for the
+  // forward and the backward sweep (amg.cpp gs_schedule), once per structure
+  DBuf<int> gs_frows, gs_brows;
+  std::vector<int> gs_fptr, gs_bptr;
   // V-cycle vectors: x the pre-smoothing iterate, x2 the post-smoothed result
   DBuf<double> x, x2, b, res, d, d2;
   DBuf<float> dinv32, x32, x232, b32, res32, d32, d232;

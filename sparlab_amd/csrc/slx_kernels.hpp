@@ -242,6 +242,14 @@ void lanczos_shift_sum(int n, int m, const double* part_in, double* beta, const 
 void lanczos_shift(int n, const double* b, const double* w, double* v, double* vprev, hipStream_t s);
 void sqrt_scalar(double* p, hipStream_t s);
 void fill(long long n, double v, double* out, hipStream_t s);
+void fill32(long long n, float v, float* out, hipStream_t s);
+// one symmetric Gauss-Seidel sweep of A (forward or backward by the schedule
+// given): rows[level_ptr[L] .. level_ptr[L + 1]) is dependency level L, run in
+// order, one launch each (kernels_gs.hip); nv cases side by side (stride rows)
+void gs_sweep(const DCsr& A, const int* rows, const std::vector<int>& level_ptr, const double* dinv, const double* b,
+              double* x, hipStream_t s, int nv = 1);
+void gs_sweep32(const DCsr& A, const int* rows, const std::vector<int>& level_ptr, const float* dinv, const float* b,
+                float* x, hipStream_t s, int nv = 1);
 void copy(long long n, const double* in, double* out, hipStream_t s);
 
 }  // namespace k

@@ -288,6 +288,56 @@ void cheb_steps(Context& c, size_t l, const T* r_first, T* x, T* d, T* d2, T* re
   }
 }
 
+// Symmetric Gauss-Seidel V-cycle from level l (Multigrid.cpp:788-806,857-885):
+// x = 0, forward sweeps, residual, restriction, the coarse correction added to
+// x, backward sweeps. The result stays in x, which is returned.
+void gs_sweeps(Context& c, size_t l, bool forward, const double* b, double* x, int nv) {
+  Level& lv = c.levels[l];
+  const DBuf<int>& rows = forward ? lv.gs_frows : lv.gs_brows;
+  const std::vector<int>& lp = forward ? lv.gs_fptr : lv.gs_bptr;
+  for (int k = 0; k < c.amg_opt.smoother_degree; ++k) k::gs_sweep(op(c, l), rows.p, lp, lv.dinv.p, b, x, c.stream, nv);
+}
+void gs_sweeps(Context& c, size_t l, bool forward, const float* b, float* x, int nv) {
+  Level& lv = c.levels[l];
+  const DBuf<int>& rows = forward ? lv.gs_frows : lv.gs_brows;
+  const std::vector<int>& lp = forward ? lv.gs_fptr : lv.gs_bptr;
+  for (int k = 0; k < c.amg_opt.smoother_degree; ++k)
+    k::gs_sweep32(op(c, l), rows.p, lp, lv.dinv32.p, b, x, c.stream, nv);
+}
+void zero(long long n, double* x, hipStream_t s) { k::fill(n, 0.0, x, s); }
+void zero(long long n, float* x, hipStream_t s) { k::fill32(n, 0.0f, x, s); }
+
+template <typename T>
+T* cycle_gs(Context& c, size_t l, const T* b, T* x, T* x_out, int nv, bool block) {
+  Level& lv = c.levels[l];
+  if (l + 1 == c.levels.size()) {
+    coarse_solve(c, lv.n, b, x_out, nv);
+    return x_out;
+  }
+  if (lv.gs_fptr.empty()) fail(SLX_SOLVER_ERROR, "multigrid: internal error (Gauss-Seidel schedule missing)");
+  const CycleVecs<T> v = level_vecs<T>(lv, block);
+  const long long n = lv.n;
+  zero(n * nv, x, c.stream);
+  gs_sweeps(c, l, true, b, x, nv);
+  block_residual<T>(c, l, x, b, v.res, nv);
+  Level& nl = c.levels[l + 1];
+  const CycleVecs<T> w = level_vecs<T>(nl, block);
+  const long long nc = nl.n;
+  if (use_mf(c, l)) {
+    for (int q = 0; q < nv; ++q) restrict_mf<T>(c, v.res + q * n, v.d2 + q * n, w.b + q * nc);
+  } else {
+    csr_apply(lv.R, v.res, w.b, c.stream, nv);
+  }
+  const T* xc = cycle_gs<T>(c, l + 1, w.b, w.x, w.x2, nv, block);
+  if (use_mf(c, l)) {
+    for (int q = 0; q < nv; ++q) prolong_mf<T>(c, xc + q * nc, v.d2 + q * n, x + q * n);
+  } else {
+    csr_apply_add(lv.P, xc, x, c.stream, nv);
+  }
+  gs_sweeps(c, l, false, b, x, nv);
+  return x;
+}
+
 // V-cycle from level l for right-hand side b (Multigrid.cpp:857-885). x is the
 // pre-smoothing iterate and x_out receives the result (the second iterate of
 // the fused post-smoothing start); started: x and the level's d already hold
@@ -363,6 +413,17 @@ T* cycle(Context& c, size_t l, const T* b, T* x, T* x_out, int nv, bool block, b
 const float* vcycle_cases(Context& c, const double* r, double* z, int nv, bool block) {
   Level& top = c.levels[0];
   const int n = top.n;
+  if (c.amg_opt.smoother == 1) {
+    if (c.amg_opt.precision >= 1) {
+      const CycleVecs<float> v = level_vecs<float>(top, block);
+      k::to_float((long long)n * nv, r, v.b, c.stream);
+      return cycle_gs<float>(c, 0, v.b, v.x, v.x2, nv, block);
+    }
+    const CycleVecs<double> v = level_vecs<double>(top, block);
+    const double* zr = cycle_gs<double>(c, 0, r, v.x, z, nv, block);
+    if (zr != z) k::copy((long long)n * nv, zr, z, c.stream);
+    return nullptr;
+  }
   if (c.amg_opt.precision >= 1) {
     const CycleVecs<float> v = level_vecs<float>(top, block);
     if (c.levels.size() > 1) {
@@ -581,9 +642,13 @@ void prepare_mixed(Context& c) {
   for (size_t l = 0; l < c.levels.size(); ++l) {
     Level& lv = c.levels[l];
     DCsr& A = op(c, l);
-    if (!use_mf(c, l)) {
+    // (the Gauss-Seidel sweeps read level 0's assembled values even where its
+    // other products are matrix-free)
+    if (!use_mf(c, l) || (c.amg_opt.smoother == 1 && lv.has_next)) {
       A.val32.alloc((size_t)A.nnz);
       k::to_float(A.nnz, A.val.p, A.val32.p, s);
+    } else {
+      A.val32.release();
     }
     if (c.amg_opt.precision == 2 && l > 0) {  // coarse operators stored in bfloat16
       A.val16.alloc((size_t)A.nnz);

@@ -313,6 +313,7 @@ class Deck:
     modal_flags: dict | None = None  # analyse_optimised_topology, compare_mass_matched_baseline
     interpretation_threshold: float = 0.5  # optimizer.interpretation_threshold (Config.cpp:1571)
     skipped_reports: list = dataclasses.field(default_factory=list)  # report-only settings not computed
+    amg: dict = dataclasses.field(default_factory=dict)  # solver.linear.amg keys set by the deck (library names)
 
     @property
     def domain_volume(self) -> float:
@@ -336,6 +337,55 @@ def _number(d: dict, key: str, default, where: str):
     if isinstance(v, bool) or not isinstance(v, (int, float)):
         raise DeckError(f"'{where}.{key}' must be a number")
     return v
+
+
+_SMOOTHERS = {"chebyshev": 0, "gauss_seidel": 1, "symmetric_gauss_seidel": 1}
+
+
+def _amg_options(lin: dict) -> dict:
+    """solver.linear.amg (Config.cpp:1097-1110) as keyword arguments of the
+    library's multigrid options, only the keys the deck sets; checked as
+    AmgPreconditioner's constructor checks them (Multigrid.cpp:899-925)."""
+    a = lin.get("amg", {})
+    if not isinstance(a, dict):
+        raise DeckError("'solver.linear.amg' must be an object")
+    where = "solver.linear.amg"
+    out = {}
+    for key in ("strength_threshold", "chebyshev_ratio", "prolongator_damping", "coarse_pivot_tolerance"):
+        if key in a:
+            out[key] = float(_number(a, key, 0.0, where))
+    for key in ("max_levels", "coarse_size", "smoother_degree", "lanczos_steps"):
+        if key in a:
+            v = _number(a, key, 0, where)
+            if v != int(v):
+                raise DeckError(f"'{where}.{key}' must be an integer")
+            out[key] = int(v)
+    if "smoother" in a:
+        text = str(a["smoother"])
+        if text not in _SMOOTHERS:
+            raise DeckError(f"unknown multigrid smoother '{text}' (expected chebyshev|gauss_seidel)")
+        out["smoother"] = _SMOOTHERS[text]
+    if "reuse_aggregates" in a:
+        if not isinstance(a["reuse_aggregates"], bool):
+            raise DeckError(f"'{where}.reuse_aggregates' must be true or false")
+        out["reuse_aggregates"] = int(a["reuse_aggregates"])
+    if "strength_threshold" in out and not 0.0 <= out["strength_threshold"] < 1.0:
+        raise DeckError("multigrid strength_threshold must lie in [0, 1)")
+    if out.get("max_levels", 1) < 1:
+        raise DeckError("multigrid max_levels must be at least 1")
+    if out.get("coarse_size", 1) < 1:
+        raise DeckError("multigrid coarse_size must be positive")
+    if out.get("smoother_degree", 1) < 1:
+        raise DeckError("multigrid smoother_degree must be at least 1")
+    if not out.get("chebyshev_ratio", 2.0) > 1.0:
+        raise DeckError("multigrid chebyshev_ratio must exceed 1")
+    if not 0.0 < out.get("prolongator_damping", 1.0) < 2.0:
+        raise DeckError("multigrid prolongator_damping must lie in (0, 2)")
+    if out.get("lanczos_steps", 2) < 2:
+        raise DeckError("multigrid lanczos_steps must be at least 2")
+    if out.get("max_levels", 1) > 16:
+        _unsupported(f"{where}.max_levels", "more than 16 multigrid levels")
+    return out
 
 
 def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> Deck:
@@ -622,6 +672,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
         tol = float(lin.get("iterative_tolerance", 1e-12))
     else:
         tol = float(lin.get("residual_tolerance", 1e-8))
+    amg = _amg_options(lin)
     density = float(doc.get("material", {}).get("density", 0.0))
     thr = float(opt.get("interpretation_threshold", 0.5))
     # checks that only report on the final design and do not change the optimisation
@@ -657,7 +708,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
                 case_names=names, case_weights=weights, tolerance=tol, filter_type=ftype, filter_radius=radius,
                 initial_density=start, passive_solid=int((tag == 1).sum()), passive_void=int((tag == 2).sum()),
                 run=run, history_stride=int(opt.get("history_stride", 1)), density=density, modal=modal,
-                modal_flags=flags, interpretation_threshold=thr, skipped_reports=skipped)
+                modal_flags=flags, interpretation_threshold=thr, skipped_reports=skipped, amg=amg)
 
 
 def load_deck(path: str, overrides: dict | None = None) -> Deck:

@@ -102,6 +102,19 @@ def optimised_topology_modes(dk: D.Deck, phys, device, log) -> dict:
     return out
 
 
+def _multigrid_options(amg: dict) -> dict:
+    """The multigrid options in effect, named as ResultWriter.cpp:1694-1707 names them."""
+    from . import native as N
+
+    o = N.amg_options(**amg)
+    out = {k: getattr(o, k) for k in ("strength_threshold", "max_levels", "coarse_size", "smoother_degree",
+                                      "chebyshev_ratio", "prolongator_damping", "lanczos_steps",
+                                      "coarse_pivot_tolerance")}
+    out["smoother"] = "gauss_seidel" if o.smoother == 1 else "chebyshev"
+    out["reuse_aggregates"] = bool(o.reuse_aggregates)
+    return out
+
+
 def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str = "fp64",
         tolerance: float | None = None, tag: str = "", write_csv_files: bool = True, log=print) -> dict:
     """Optimise one deck on the GPU and write its outputs; returns the summary."""
@@ -114,12 +127,13 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
     t0 = time.perf_counter()
     g = N.HipProblem(m, device=device)
     x0 = dk.initial_design()
-    if precision != "fp64":
-        # the options take effect at a setup: run it on the first iteration's
-        # operator, where the loop's first setup would aggregate anyway
+    if precision != "fp64" or dk.amg:
+        # the options (precision, the deck's solver.linear.amg) take effect at a
+        # setup: run it on the first iteration's operator, where the loop's first
+        # setup would aggregate anyway
         p0 = dk.run["penalty_start"] if dk.run["cont_steps"] > 1 else m.penalty
         g.assemble(m.emin_ratio + (1.0 - m.emin_ratio) * np.clip(g.filter(x0), 0.0, 1.0) ** p0)
-        g.amg_setup(precision=_PRECISION[precision])
+        g.amg_setup(precision=_PRECISION[precision], **dk.amg)
     log(f"{dk.name}: {m.nelem} elements, {g.nfree} free DOFs, {len(dk.case_names)} load case(s), "
         f"filter radius {dk.filter_radius:.6g} m, precision {precision}")
     timings["model_build"] = time.perf_counter() - t0
@@ -196,6 +210,7 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
             "iterations": iters, "linear_solves": res["linear_solves"],
             "linear_solver": {
                 "solver": "amg_cg (HIP, gfx950)", "precision": precision, "residual_tolerance": tol,
+                "multigrid_options": _multigrid_options(dk.amg),
                 "iterative_iterations_total": res["linear_iterations"],
                 "iterative_iterations_per_solve": res["linear_iterations"] / max(res["linear_solves"], 1),
                 "multigrid": {

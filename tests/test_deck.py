@@ -126,6 +126,41 @@ def test_deck_settings_follow_the_reference_defaults():
     assert np.all(x0[(b.model.passive == 1) & (b.model.lower == 0.0)] == 0.0)
 
 
+def test_multigrid_block_is_read():
+    """solver.linear.amg (Config.cpp:1097-1110): the keys a deck sets reach the
+    library's options under their names, the smoother by its name; the
+    constructor's checks (Multigrid.cpp:899-925) refuse bad values."""
+    doc = json.load(open(fixture("cantilever_beam")))
+    assert D.parse_deck(doc).amg == {}
+    doc.setdefault("solver", {}).setdefault("linear", {})["amg"] = {
+        "smoother": "gauss_seidel", "smoother_degree": 2, "coarse_size": 400, "max_levels": 6,
+        "strength_threshold": 0.05, "chebyshev_ratio": 20, "prolongator_damping": 1.2, "lanczos_steps": 10,
+        "reuse_aggregates": False, "coarse_pivot_tolerance": 1e-12}
+    a = D.parse_deck(doc).amg
+    assert a == dict(smoother=1, smoother_degree=2, coarse_size=400, max_levels=6, strength_threshold=0.05,
+                     chebyshev_ratio=20.0, prolongator_damping=1.2, lanczos_steps=10, reuse_aggregates=0,
+                     coarse_pivot_tolerance=1e-12)
+    doc["solver"]["linear"]["amg"] = {"smoother": "symmetric_gauss_seidel"}
+    assert D.parse_deck(doc).amg == dict(smoother=1)
+    # every key the library's options take is a field of its struct
+    from sparlab_amd.native import AmgOptions
+    fields = {f[0] for f in AmgOptions._fields_}
+    assert set(a) <= fields
+    for bad, match in [({"smoother": "jacobi"}, "unknown multigrid smoother 'jacobi'"),
+                       ({"strength_threshold": 1.0}, r"strength_threshold must lie in \[0, 1\)"),
+                       ({"max_levels": 0}, "max_levels must be at least 1"),
+                       ({"coarse_size": 0}, "coarse_size must be positive"),
+                       ({"smoother_degree": 0}, "smoother_degree must be at least 1"),
+                       ({"chebyshev_ratio": 1.0}, "chebyshev_ratio must exceed 1"),
+                       ({"prolongator_damping": 2.0}, r"prolongator_damping must lie in \(0, 2\)"),
+                       ({"lanczos_steps": 1}, "lanczos_steps must be at least 2"),
+                       ({"coarse_size": 2.5}, "must be an integer"),
+                       ({"reuse_aggregates": 1}, "must be true or false")]:
+        doc["solver"]["linear"]["amg"] = bad
+        with pytest.raises(D.DeckError, match=match):
+            D.parse_deck(doc)
+
+
 def test_mbb_supports_are_the_nearest_nodes():
     d = D.load_deck(fixture("mbb_beam"))
     m = d.model

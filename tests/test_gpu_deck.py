@@ -144,3 +144,26 @@ def test_cli_switches_erosion_check_and_modes(tmp_path):
     assert "robust" not in r
     modes = s["modal_initial_solid"]
     assert modes["num_modes"] == 2 and modes["converged"]
+
+
+def test_deck_with_the_gauss_seidel_smoother_against_the_oracle(tmp_path):
+    """A deck whose solver.linear.amg block selects the symmetric Gauss-Seidel
+    smoother (and a small coarse size, so that the sweeps run on two levels):
+    the driver's run against the oracle's OC loop with the same options."""
+    from oracle.refcpu import Oracle
+    from sparlab_amd.topopt import run
+    doc = json.load(open(os.path.join(GOLD, "decks", "mbb_beam.json")))
+    doc.setdefault("solver", {}).setdefault("linear", {})["amg"] = {"smoother": "gauss_seidel", "coarse_size": 100}
+    d = D.parse_deck(doc, overrides=dict(nx=60, ny=20, max_iterations=12))
+    print("deck mbb_beam 60 x 20, gauss_seidel, 12 iterations", flush=True)
+    s = run(d, str(tmp_path), log=lambda t: print(t, flush=True))
+    r = s["optimization_result"]
+    assert r["linear_solver"]["multigrid_options"]["smoother"] == "gauss_seidel"
+    assert r["linear_solver"]["multigrid"]["levels"] >= 3
+    o = Oracle(d.model, tol=d.tolerance, amg=dict(smoother=1, coarse_size=100))
+    kw = {k: v for k, v in d.run.items() if k != "max_iterations"}
+    xo, ho, no, fo = o.run_oc(d.initial_design(), d.run["max_iterations"], **kw)
+    rows = list(csv.reader(open(tmp_path / "history.csv")))[1:]
+    assert r["iterations"] == no
+    np.testing.assert_allclose([float(x[2]) for x in rows], ho[:no, 2], rtol=1e-8)
+    assert r["compliance_J"] == pytest.approx(fo, rel=1e-7)

@@ -521,16 +521,18 @@ def _three_case_model(kind):
 
 
 @pytest.mark.parametrize("kind", ["hex8_matrix_free", "hex8_assembled", "q4", "hex8_matrix_free_fp32_cycle",
-                                  "q4_bf16_coarse"])
+                                  "q4_bf16_coarse", "hex8_matrix_free_gauss_seidel", "q4_gauss_seidel_fp32_cycle"])
 def test_block_load_cases_equal_case_by_case_solves(ctx, kind, monkeypatch):
     """The load cases of an evaluation solved as one block PCG (one operator
     product and one V-cycle pass per iteration for every case still iterating)
     give each case's case-by-case result bit for bit: displacements, iteration
     counts, compliance and sensitivities, over three evaluations that warm-start
-    from the previous solutions, with one case unloaded. FP64 V-cycle, and the
-    mixed hierarchies (FP32 V-cycle under flexible CG; bfloat16 coarse operators)."""
+    from the previous solutions, with one case unloaded. FP64 V-cycle, the
+    mixed hierarchies (FP32 V-cycle under flexible CG; bfloat16 coarse operators)
+    and the Gauss-Seidel smoother."""
     print(f"test_block_load_cases_equal_case_by_case_solves[{kind}]", flush=True)
     precision = 1 if "fp32" in kind else 2 if "bf16" in kind else 0
+    smoother = 1 if "gauss_seidel" in kind else 0
     if precision:
         # the mixed path times the CSR kernels' lane counts per context: fix them
         monkeypatch.setenv("SPARLAB_NO_AUTOTUNE", "1")
@@ -545,9 +547,9 @@ def test_block_load_cases_equal_case_by_case_solves(ctx, kind, monkeypatch):
         c = N.Context(0)
         c.timing(True)
         g = N.HipProblem(m, ctx=c)
-        if precision:
+        if precision or smoother:
             g.assemble()
-            g.amg_setup(precision=precision)
+            g.amg_setup(precision=precision, smoother=smoother)
         runs = []
         for x in designs:
             ev = g.evaluate(x, tol=1e-10)
@@ -632,3 +634,32 @@ def test_caller_csr_is_checked_on_the_host_and_a_new_pattern_gets_a_new_setup():
     assert np.linalg.norm(x1 - x0[rev]) <= 1e-8 * np.linalg.norm(x0)  # (K[rev][:, rev]) x[rev] = b[rev]
     s.ctx.close()
     fresh.ctx.close()
+
+
+def test_coarsest_level_above_eight_thousand_unknowns(ctx):
+    """A one-level hierarchy (max_levels 1) makes the whole system the coarsest
+    level: 8 712 unknowns, above the reference's dense limit (Multigrid.cpp:749-764
+    takes its sparse LDL^T there) and the former 8 000 of this path. The dense
+    inverse then solves it directly: one or two CG iterations, the solution of a sparse
+    direct solve. Above 16 384 unknowns the setup refuses with the options to change."""
+    import scipy.sparse.linalg as sla
+    print("test_coarsest_level_above_eight_thousand_unknowns", flush=True)
+    m = M.cantilever_3d(24, 10, 10)
+    g = N.HipProblem(m, ctx=ctx)
+    scale = 0.2 + 0.8 * M.seeded_density(m.nelem, seed=13)
+    g.assemble(scale)
+    g.amg_setup(max_levels=1)
+    st = g.amg_stats()
+    assert st.levels == 1 and st.unknowns[0] == g.nfree == 8712
+    assert 0.0 < st.coarse_pivot_ratio <= 1.0
+    b = m.free_loads()[0]
+    x, it, rr, cv = g.solve(b, tol=1e-10)
+    K = _csr(g.nfree, g.nfree, *g.kff())
+    xd = sla.spsolve(K.tocsc(), b)
+    assert cv and it <= 2
+    assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+    big = M.cantilever_3d(32, 16, 16)
+    gb = N.HipProblem(big, ctx=ctx)
+    gb.assemble()
+    with pytest.raises(N.SolverError, match="lower amg.coarse_size or raise amg.max_levels"):
+        gb.amg_setup(max_levels=1)
