@@ -285,14 +285,15 @@ def mf_smoother_roofline(g, m, fp32, vcycles):
     """The level-0 Chebyshev step through the matrix-free operator (k_mf_hex8 mode 3), the kernel
     with the largest share of a SIMP step: timed with HIP events around 50 back-to-back launches.
     Algorithmic HBM bytes per launch: 7 vector words per free DOF (d, r and x read; r, d and x
-    written; D^-1 read), one word per element factor and the 3 free indices of each node.
+    written; D^-1 read), one word per element factor and one 4-byte map entry per node (the node's
+    first free index and free-component mask).
     Launches in the timed region: 2 (degree - 1) = 4 per V-cycle."""
     word = 4 if fp32 else 8
     launches = 50
     avg = g.time_smoother(0, launches)
     g.amg_apply(np.ones(g.nfree))  # the level's vectors back to a V-cycle's state
     nnodes = m.nnodes
-    nbytes = 7 * word * g.nfree + word * m.nelem + 12 * nnodes
+    nbytes = 7 * word * g.nfree + word * m.nelem + 4 * nnodes
     ach = nbytes / (avg * 1e-3) / 1e9
     in_run = 4 * vcycles
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
