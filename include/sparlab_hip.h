@@ -391,7 +391,10 @@ slx_status slx_load_case_compliance_get(const slx_context* ctx, double* complian
  * ModalAnalysisOptions and ModalResult (include/sparlab/fem/ModalAnalysis.hpp:44-90).
  * The shifted systems are solved with the context's AMG-PCG (the options of the
  * last slx_amg_setup, or the defaults), each warm started from the converging
- * subspace. Only shift_factor = 0 (the reference's default) is supported. */
+ * subspace when warm_start is set. A system of at most 400 free DOFs, or one
+ * the subspace covers, takes the dense generalised eigensolve instead
+ * (iterations 0), as the reference does. Only shift_factor = 0 (the
+ * reference's default) is supported. */
 typedef struct {
   int num_modes;              /* lowest modes requested (6) */
   int lumped;                 /* 0: consistent mass (default), 1: row-sum lumped */
@@ -403,6 +406,7 @@ typedef struct {
   double density;             /* kg/m^3, > 0 */
   double linear_tolerance;    /* relative residual of each AMG-PCG solve (1e-8) */
   int linear_max_iterations;  /* per solve (1000) */
+  int warm_start;             /* start each solve from the last Ritz pair (1; linear.warm_start) */
 } slx_modal_options;
 
 typedef struct {

@@ -197,11 +197,9 @@ def components_by_face(model, subset):
     return comps
 
 
-def submesh_operators(model, ke0, density, elements, fixed):
-    """K_ff and M_ff of the mesh made of `elements` alone, nodes renumbered by
-    first appearance (extract_element_subset, src/mesh/SubMesh.cpp:96-140), with
-    the parent's fixed DOF mask restricted to the kept nodes."""
-    dim = model.dim
+def submesh_node_order(model, elements):
+    """extract_element_subset's node numbering (src/mesh/SubMesh.cpp:110-122):
+    the parent node of each sub-mesh node, and the parent-to-sub map (-1: dropped)."""
     nodes = element_nodes(model)[np.asarray(elements)]
     old_to_new = -np.ones(model.nnodes, dtype=np.int64)
     order = []
@@ -209,7 +207,16 @@ def submesh_operators(model, ke0, density, elements, fixed):
         if old_to_new[n] < 0:
             old_to_new[n] = len(order)
             order.append(n)
-    order = np.array(order)
+    return np.array(order), old_to_new
+
+
+def submesh_operators(model, ke0, density, elements, fixed):
+    """K_ff and M_ff of the mesh made of `elements` alone, nodes renumbered by
+    first appearance (extract_element_subset, src/mesh/SubMesh.cpp:96-140), with
+    the given fixed DOF mask (over the parent's DOFs) restricted to the kept nodes."""
+    dim = model.dim
+    nodes = element_nodes(model)[np.asarray(elements)]
+    order, old_to_new = submesh_node_order(model, elements)
     conn = old_to_new[nodes]
     nn = len(order)
     dofs = (conn[:, :, None] * dim + np.arange(dim)[None, None, :]).reshape(len(elements), -1)

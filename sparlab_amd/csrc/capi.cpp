@@ -1295,6 +1295,7 @@ void slx_modal_default_options(slx_modal_options* o) {
   o->density = 0.0;
   o->linear_tolerance = 1.0e-8;
   o->linear_max_iterations = 1000;
+  o->warm_start = 1;
 }
 
 slx_status slx_modal_solve(slx_context* ctx, const slx_modal_options* opt, const double* stiffness_scale,

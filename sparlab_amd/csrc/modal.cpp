@@ -1,5 +1,7 @@
 // Free vibration of a structured model: the lowest eigenpairs of (K_ff, M_ff)
-// by shift-invert subspace iteration (src/fem/ModalAnalysis.cpp:120-418). The
+// by shift-invert subspace iteration (src/fem/ModalAnalysis.cpp:120-418), or,
+// for a system of at most 400 free DOFs or one the subspace covers, by the dense
+// generalised eigensolve of the reference's small-system branch. The
 // q shifted systems of each iteration are solved with the context's AMG-PCG,
 // each warm started from the Ritz vector divided by its Ritz value; the mass
 // products, the projections x^T K x and x^T M x, the rotation to the Ritz basis
@@ -159,7 +161,11 @@ void modal_solve(Context& c, const slx_modal_options& o, const double* d_mscale,
   if (n == 0) fail(SLX_MODEL_ERROR, "modal analysis has no free degrees of freedom; every DOF is constrained");
   const int m = std::min(o.num_modes, n);
   const int q = std::min(n, std::max(2 * m, m + 8));
-  if (q > 64) fail(SLX_CONFIG_ERROR, "modal analysis on the device supports at most 32 modes (a subspace of 64)");
+  // a subspace that covers the system, or a small system, takes the dense
+  // generalised eigensolve (ModalAnalysis.cpp:166-179)
+  const bool dense = q >= n || n <= 400;
+  if (!dense && q > 64)
+    fail(SLX_CONFIG_ERROR, "modal analysis on the device supports at most 32 modes (a subspace of 64)");
   R = slx_modal_result{};
   R.num_modes = m;
   R.subspace_size = q;
@@ -216,12 +222,76 @@ void modal_solve(Context& c, const slx_modal_options& o, const double* d_mscale,
   std::vector<double> prev((size_t)m, std::numeric_limits<double>::max()), mu_prev, mu, ritz, g((size_t)2 * q * q);
   std::vector<double> ritz_cm((size_t)q * q), resid((size_t)2 * m), last_res((size_t)m, 0.0);
   const double* phi_src = nullptr;
-  for (int iter = 1; iter <= o.max_iterations; ++iter) {
+  if (dense) {
+    // K_ff and M_ff as dense matrices (K from its CSR, M column by column from
+    // the device's mass product), the pencil solved as the projected one is
+    std::vector<double> kn((size_t)n * n, 0.0), mn((size_t)n * n);
+    {
+      std::vector<int> hp((size_t)n + 1), hc((size_t)c.K.nnz);
+      std::vector<double> hv((size_t)c.K.nnz);
+      c.K.ptr.download(hp.data(), hp.size(), s);
+      c.K.col.download(hc.data(), hc.size(), s);
+      c.K.val.download(hv.data(), hv.size(), s);
+      SLX_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i < n; ++i)
+        for (int k = hp[(size_t)i]; k < hp[(size_t)i + 1]; ++k) kn[(size_t)i * n + hc[(size_t)k]] = hv[(size_t)k];
+      DBuf<double> E, ME;
+      std::vector<double> eye((size_t)n * n, 0.0);
+      for (int i = 0; i < n; ++i) eye[(size_t)i * n + i] = 1.0;
+      E.upload(eye, s);
+      ME.alloc((size_t)n * n);
+      for (int col = 0; col < n; ++col) mass(E.p + (size_t)col * n, ME.p + (size_t)col * n);
+      ME.download(mn.data(), mn.size(), s);  // column j = M e_j; M is symmetric
+      SLX_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<double> lam, vec;
+    if (!lowest_pencil_pairs(kn, mn, n, m, lam, vec))
+      fail(SLX_SOLVER_ERROR,
+           "dense generalised eigensolve failed: the mass matrix is not positive definite and the stiffness matrix "
+           "is not either, or fewer DOFs than the modes requested carry mass (check material.density and the mass "
+           "interpolation floor)");
+    // phi^T M phi = 1 and the eigenpair residuals |K phi - lambda M phi| / |lambda M phi|
+    // (ModalAnalysis.cpp:311-322,366-368)
+    std::vector<double> phi((size_t)n * m), kp((size_t)n), mp((size_t)n);
+    for (int j = 0; j < m; ++j) {
+      for (int i = 0; i < n; ++i) {
+        double a = 0.0;
+        for (int k = 0; k < n; ++k) a += mn[(size_t)i * n + k] * vec[(size_t)k * m + j];
+        mp[(size_t)i] = a;
+      }
+      double mm = 0.0;
+      for (int i = 0; i < n; ++i) mm += vec[(size_t)i * m + j] * mp[(size_t)i];
+      const double mnorm = std::sqrt(std::max(mm, 0.0));
+      if (!(mnorm > 0.0))
+        fail(SLX_SOLVER_ERROR,
+             "a computed mode shape has zero modal mass; the mass matrix is singular on the free DOFs");
+      for (int i = 0; i < n; ++i) phi[(size_t)j * n + i] = vec[(size_t)i * m + j] / mnorm;
+      double rr = 0.0, ref = 0.0;
+      for (int i = 0; i < n; ++i) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < n; ++k) {
+          a += kn[(size_t)i * n + k] * phi[(size_t)j * n + k];
+          b += mn[(size_t)i * n + k] * phi[(size_t)j * n + k];
+        }
+        const double lm = lam[(size_t)j] * b;
+        rr += (a - lm) * (a - lm);
+        ref += lm * lm;
+      }
+      prev[(size_t)j] = lam[(size_t)j];
+      last_res[(size_t)j] = std::sqrt(rr) / std::max(std::sqrt(ref), 1.0e-300);
+    }
+    KX.upload(phi.data(), phi.size(), s);
+    R.converged = 1;
+    R.iterations = 0;
+    R.final_change = 0.0;
+  }
+  for (int iter = 1; !dense && iter <= o.max_iterations; ++iter) {
     for (int col = 0; col < q; ++col) mass(X.p + (size_t)col * n, Y.p + (size_t)col * n);
     for (int col = 0; col < q; ++col) {
       double* xb = Xb.p + (size_t)col * n;
       // near convergence K^-1 M x = x / lambda: the last Ritz pair is the guess
-      if (mu_prev.size() == (size_t)q && mu_prev[(size_t)col] > 0.0)
+      // (with warm_start set, as linear.warm_start in the reference)
+      if (o.warm_start && mu_prev.size() == (size_t)q && mu_prev[(size_t)col] > 0.0)
         k::scale_copy(n, 1.0 / mu_prev[(size_t)col], X.p + (size_t)col * n, xb, s);
       else
         k::fill(n, 0.0, xb, s);
@@ -306,16 +376,19 @@ void modal_solve(Context& c, const slx_modal_options& o, const double* d_mscale,
   }
 
   // phi^T M phi = 1 (ModalAnalysis.cpp:311-322), through M x = (M xbar) ritz
-  k::gram(n, m, phi_src, MX.p, parts.p, G.p, s);
-  std::vector<double> gm((size_t)m * m);
-  G.download(gm.data(), (size_t)m * m, s);
-  SLX_HIP(hipStreamSynchronize(s));
-  for (int i = 0; i < m; ++i) {
-    const double mnorm = std::sqrt(std::max(gm[(size_t)i * m + i], 0.0));
-    if (!(mnorm > 0.0))
-      fail(SLX_SOLVER_ERROR,
-           "a computed mode shape has zero modal mass; the mass matrix is singular on the free DOFs");
-    k::scale_copy(n, 1.0 / mnorm, phi_src + (size_t)i * n, KX.p + (size_t)i * n, s);
+  // (the dense branch normalised its vectors on the host)
+  if (!dense) {
+    k::gram(n, m, phi_src, MX.p, parts.p, G.p, s);
+    std::vector<double> gm((size_t)m * m);
+    G.download(gm.data(), (size_t)m * m, s);
+    SLX_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < m; ++i) {
+      const double mnorm = std::sqrt(std::max(gm[(size_t)i * m + i], 0.0));
+      if (!(mnorm > 0.0))
+        fail(SLX_SOLVER_ERROR,
+             "a computed mode shape has zero modal mass; the mass matrix is singular on the free DOFs");
+      k::scale_copy(n, 1.0 / mnorm, phi_src + (size_t)i * n, KX.p + (size_t)i * n, s);
+    }
   }
   // validity screening (ModalAnalysis.cpp:333-375)
   for (int i = 0; i < m; ++i) {

@@ -193,6 +193,19 @@ class Geometry:
         ext = [model.lx, model.ly] + ([model.lz] if model.dim == 3 else [])
         self.tol = 1.0e-9 * max(math.sqrt(sum(e * e for e in ext)), 1.0e-30)
 
+    @classmethod
+    def of_nodes(cls, dim: int, nodes: np.ndarray) -> "Geometry":
+        """An unstructured node set (the nodes of an extracted sub-mesh, in its
+        numbering), its tolerance from its own bounding box (Selector.cpp:12-18)."""
+        g = cls.__new__(cls)
+        g.dim = dim
+        g.nodes = np.zeros((len(nodes), 3))
+        g.nodes[:, :dim] = np.asarray(nodes, dtype=float)[:, :dim]
+        g.cents = np.zeros((0, 3))
+        ext = g.nodes.max(axis=0) - g.nodes.min(axis=0) if len(nodes) else np.zeros(3)
+        g.tol = 1.0e-9 * max(float(np.sqrt((ext[:dim] ** 2).sum())), 1.0e-30)
+        return g
+
     def nodes_of(self, region: dict, where: str) -> np.ndarray:
         """SelectorGroup::select_nodes (Selector.cpp:73-148)."""
         hit = np.zeros(len(self.nodes), dtype=bool)
@@ -225,6 +238,21 @@ class Geometry:
             else:
                 hit |= _contains(sel, self.cents, self.tol)
         return np.nonzero(hit != region["invert"])[0]
+
+
+def apply_constraints(geo: Geometry, constraints: list) -> np.ndarray:
+    """apply_constraints (BoundaryConditions.cpp:93-130): the fixed components,
+    [nodes, dim] bool, of the parsed boundary conditions on a node set; a
+    condition that selects no node is an error, as there."""
+    fixed = np.zeros((len(geo.nodes), geo.dim), dtype=bool)
+    for bc in constraints:
+        nodes = geo.nodes_of(bc["region"], bc["name"])
+        if len(nodes) == 0:
+            raise DeckError(f"boundary condition '{bc['name']}' selected no nodes; check its coordinates against "
+                            "the mesh extents (an unconstrained model yields a singular stiffness matrix)")
+        for c in bc["comps"]:
+            fixed[nodes, c] = True
+    return fixed
 
 
 def boundary_faces(m: StructuredModel):
@@ -314,6 +342,7 @@ class Deck:
     interpretation_threshold: float = 0.5  # optimizer.interpretation_threshold (Config.cpp:1571)
     skipped_reports: list = dataclasses.field(default_factory=list)  # report-only settings not computed
     amg: dict = dataclasses.field(default_factory=dict)  # solver.linear.amg keys set by the deck (library names)
+    constraints: list = dataclasses.field(default_factory=list)  # parsed boundary conditions (region, comps, name)
 
     @property
     def domain_volume(self) -> float:
@@ -460,6 +489,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     if not bcs:
         raise DeckError("'boundary_conditions' is missing or empty; an unconstrained model has a singular "
                         "stiffness matrix")
+    constraints = []
     for i, bc in enumerate(bcs):
         where = f"boundary_conditions[{i}]"
         fix = bc.get("fix") or []
@@ -476,10 +506,10 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
         value = bc.get("value")
         if value is not None and any(float(v) != 0.0 for v in value):
             _unsupported(f"{where}.value", "a prescribed non-zero displacement")
-        nodes = geo.nodes_of(parse_region(bc.get("region"), dim, f"{where}.region"), bc.get("name", f"bc{i}"))
-        if len(nodes) == 0:
-            raise DeckError(f"boundary condition '{bc.get('name', f'bc{i}')}' selected no nodes")
-        m.fix(nodes, comps)
+        spec = dict(region=parse_region(bc.get("region"), dim, f"{where}.region"), comps=comps,
+                    name=bc.get("name", f"bc{i}"))
+        constraints.append(spec)
+    m.fixed = apply_constraints(geo, constraints).astype(np.uint8).ravel()
 
     # load cases (Config.cpp:943-1070)
     cases = doc.get("load_cases") or []
@@ -673,6 +703,11 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
     else:
         tol = float(lin.get("residual_tolerance", 1e-8))
     amg = _amg_options(lin)
+    # linear.warm_start (Config.cpp:1086): solves start from the previous solution
+    warm = lin.get("warm_start", True)
+    if not isinstance(warm, bool):
+        raise DeckError("'solver.linear.warm_start' must be true or false")
+    run["warm_start"] = warm
     density = float(doc.get("material", {}).get("density", 0.0))
     thr = float(opt.get("interpretation_threshold", 0.5))
     # checks that only report on the final design and do not change the optimisation
@@ -699,7 +734,7 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
                      tolerance=float(mnode.get("tolerance", 1e-10)),
                      residual_tolerance=float(mnode.get("residual_tolerance", 1e-6)),
                      rigid_body_ratio=float(mnode.get("rigid_body_ratio", 1e-10)),
-                     linear_tolerance=tol)
+                     linear_tolerance=tol, warm_start=warm)
         if modal["num_modes"] < 1:
             raise DeckError("modal analysis requires num_modes >= 1")
         flags = dict(analyse_optimised_topology=bool(mnode.get("analyse_optimised_topology", True)),
@@ -708,7 +743,8 @@ def parse_deck(doc: dict, source: str = "", overrides: dict | None = None) -> De
                 case_names=names, case_weights=weights, tolerance=tol, filter_type=ftype, filter_radius=radius,
                 initial_density=start, passive_solid=int((tag == 1).sum()), passive_void=int((tag == 2).sum()),
                 run=run, history_stride=int(opt.get("history_stride", 1)), density=density, modal=modal,
-                modal_flags=flags, interpretation_threshold=thr, skipped_reports=skipped, amg=amg)
+                modal_flags=flags, interpretation_threshold=thr, skipped_reports=skipped, amg=amg,
+                constraints=constraints)
 
 
 def load_deck(path: str, overrides: dict | None = None) -> Deck:

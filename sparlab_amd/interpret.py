@@ -8,12 +8,16 @@ index comes first, and the volumes above the threshold, retained and
 discarded as islands are reported.
 
 The reference then builds a new mesh of the retained elements, renumbers its
-nodes and re-applies the deck's supports to it (apps/sparlab_topopt.cpp:92-130).
-On a structured grid the same operators come out of the parent grid directly:
+nodes in order of first appearance (SubMesh.cpp:98-130) and re-applies the
+deck's supports to it (apps/sparlab_topopt.cpp:92-130). The supports are
+re-applied here the same way: each boundary condition's region is evaluated on
+the retained nodes in that numbering, with the tolerance of their bounding box,
+so a nearest_node support picks the nearest retained node and node_ids name
+sub-mesh nodes. The operators then come out of the parent grid directly:
 per-element stiffness and mass factors of 1 on retained elements and 0
 elsewhere, and every DOF of a node that no retained element touches fixed.
-K_ff and M_ff are then the sub-model's matrices with the same unknowns in the
-same relative order, so the eigenpairs are the same.
+K_ff and M_ff are the sub-model's matrices with the same unknowns in the same
+relative order, so the eigenpairs are the same.
 """
 from __future__ import annotations
 
@@ -98,13 +102,35 @@ def element_nodes(model: StructuredModel) -> np.ndarray:
                     axis=1)
 
 
-def solid_submodel(model: StructuredModel, retained) -> tuple[StructuredModel, np.ndarray]:
+def submesh_nodes(model: StructuredModel, retained) -> np.ndarray:
+    """The parent node of each sub-mesh node, in the order extract_element_subset
+    numbers them (SubMesh.cpp:110-122): retained elements ascending, each
+    element's nodes in its local order, a node numbered where it first appears."""
+    conn = element_nodes(model)[np.asarray(retained, dtype=bool)].ravel()
+    _, first = np.unique(conn, return_index=True)
+    return conn[np.sort(first)]
+
+
+def solid_submodel(model: StructuredModel, retained, constraints=None,
+                   origin=(0.0, 0.0, 0.0)) -> tuple[StructuredModel, np.ndarray]:
     """The parent grid with every DOF of the nodes outside the retained elements
-    fixed; returns (model, retained-node mask). Supports keep their parent selection."""
+    fixed; returns (model, retained-node mask). With ``constraints`` (a deck's
+    parsed boundary conditions) the supports are re-applied on the sub-mesh as
+    the reference does; without, the parent's fixed DOFs on retained nodes are
+    kept. A condition that selects no retained node raises DeckError."""
+    from . import deck as D
+
     retained = np.asarray(retained, dtype=bool)
     used = np.zeros(model.nnodes, dtype=bool)
     used[element_nodes(model)[retained].ravel()] = True
-    fixed = np.asarray(model.fixed, dtype=np.uint8).reshape(model.nnodes, model.dim).copy()
+    if constraints is None:
+        fixed = np.asarray(model.fixed, dtype=np.uint8).reshape(model.nnodes, model.dim).copy()
+    else:
+        node_map = submesh_nodes(model, retained)
+        coords = model.node_coords()[node_map] + np.asarray(origin, dtype=float)[: model.dim]
+        sub_fixed = D.apply_constraints(D.Geometry.of_nodes(model.dim, coords), constraints)
+        fixed = np.zeros((model.nnodes, model.dim), dtype=np.uint8)
+        fixed[node_map] = sub_fixed
     fixed[~used] = 1
     sub = dataclasses.replace(model, fixed=fixed.ravel(), loads=None, weights=None)
     return sub, used

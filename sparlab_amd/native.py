@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import warnings
 
 import numpy as np
 
@@ -133,7 +134,8 @@ H_ITER, H_PENALTY, H_COMPLIANCE, H_VOLUME, H_VF, H_CHANGE, H_LAMBDA, H_GREY, H_B
 class ModalOptions(C.Structure):
     _fields_ = [("num_modes", C.c_int), ("lumped", C.c_int), ("max_iterations", C.c_int), ("tolerance", C.c_double),
                 ("residual_tolerance", C.c_double), ("rigid_body_ratio", C.c_double), ("shift_factor", C.c_double),
-                ("density", C.c_double), ("linear_tolerance", C.c_double), ("linear_max_iterations", C.c_int)]
+                ("density", C.c_double), ("linear_tolerance", C.c_double), ("linear_max_iterations", C.c_int),
+                ("warm_start", C.c_int)]
 
 
 class ModalResult(C.Structure):
@@ -650,12 +652,14 @@ class HipProblem:
                         converged=bool(st.volume_converged), bisections=st.bisections)
 
     def run_oc(self, x0, max_iterations, change_tol=1e-3, objective_tol=0.0, window=5, cont_steps=1,
-               penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5, tol=1e-8, projection=None):
+               penalty_start=1.0, cont_iterations=25, move=0.2, damping=0.5, tol=1e-8, projection=None,
+               warm_start=True):
         """TopologyOptimizer::run_oc: returns (x, history, iterations, final compliance);
         the full result of the last call is in ``last_run``. projection: None, or a dict of
         ProjectionOptions (eta, beta_start, beta_max, beta_factor, beta_interval,
         advance_on_convergence, robust, robust_delta, robust_volume_interval) for the Heaviside
-        projection with beta continuation."""
+        projection with beta continuation. warm_start: each solve starts from the previous
+        solution (linear.warm_start, Config.cpp:1086), else from zero."""
         self._own()
         o = RunOptions()
         o.max_iterations = max_iterations
@@ -666,7 +670,8 @@ class HipProblem:
         o.penalty_start = penalty_start
         o.continuation_iterations = cont_iterations
         o.oc = oc_options(move_limit=move, damping=damping)
-        o.eval = eval_options(tolerance=tol, penalty=self.model.penalty, emin_ratio=self.model.emin_ratio)
+        o.eval = eval_options(tolerance=tol, penalty=self.model.penalty, emin_ratio=self.model.emin_ratio,
+                              warm_start=1 if warm_start else 0)
         pr = dict(eta=0.5, beta_start=1.0, beta_max=32.0, beta_factor=2.0, beta_interval=50,
                   advance_on_convergence=True, robust=False, robust_delta=0.1, robust_volume_interval=1)
         pr.update({k: v for k, v in (projection or {}).items() if k != "erosion_check"})
@@ -685,7 +690,7 @@ class HipProblem:
 
     def modal(self, density, num_modes=6, mass_type="consistent", max_iterations=200, tolerance=1e-10,
               residual_tolerance=1e-6, rigid_body_ratio=1e-10, shift_factor=0.0, linear_tolerance=1e-8,
-              linear_max_iterations=1000, stiffness_scale=None, mass_scale=None, shapes=False):
+              linear_max_iterations=1000, stiffness_scale=None, mass_scale=None, shapes=False, warm_start=True):
         """solve_modal (src/fem/ModalAnalysis.cpp:120-418) on the device: the lowest eigenpairs of
         (K_ff, M_ff) by subspace iteration with warm-started AMG-PCG solves. Returns the fields of the
         reference's modal_json (src/io/ResultWriter.cpp:114-136) plus ``mode_shapes`` (ndofs x m,
@@ -705,7 +710,13 @@ class HipProblem:
         o.density = float(density)
         o.linear_tolerance = linear_tolerance
         o.linear_max_iterations = int(linear_max_iterations)
+        o.warm_start = 1 if warm_start else 0
         m = max(1, min(int(num_modes), self.nfree))
+        clamp = None
+        if m < int(num_modes):  # ModalAnalysis.cpp:153-160
+            clamp = (f"requested {int(num_modes)} modes but the model has only {self.nfree} free DOFs; "
+                     f"returning {m}")
+            warnings.warn(clamp, RuntimeWarning, stacklevel=2)
         lam = np.zeros(m)
         res = np.zeros(m)
         phi = np.zeros((m, self.ndofs)) if shapes else None
@@ -731,7 +742,7 @@ class HipProblem:
             "final_relative_change": r.final_change,
             "linear_solver": "amg_cg",
             "linear_iterations": int(r.linear_iterations),
-            "warnings": [],
+            "warnings": [clamp] if clamp else [],
             "seconds": r.seconds,
         }
         if r.rigid_body_modes:

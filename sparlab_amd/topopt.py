@@ -80,13 +80,15 @@ def optimised_topology_modes(dk: D.Deck, phys, device, log) -> dict:
         out["interpreted_solid_analysis"] = {"analysis_failed": True, "reason": f"no interpreted structure: {e}"}
         return out
     out["topology_interpretation"] = interp.as_dict()
-    sub, used = I.solid_submodel(m, interp.retained)
-    block = {"num_elements": interp.elements_retained, "num_nodes": int(used.sum()),
+    block = {"num_elements": interp.elements_retained,
+             "num_nodes": int(len(I.submesh_nodes(m, interp.retained))),
              "mass_kg": interp.volume_retained * dk.density, "loads_applied": False}
     out["interpreted_solid_analysis"] = block
-    if not I.supported(m, used):
-        block.update(analysis_failed=True, reason="the retained structure touches no support")
-        log(f"{dk.name}: skipping the modal analysis of the optimised topology: no support acts on it")
+    try:  # the deck's supports re-applied on the extracted structure (apps/sparlab_topopt.cpp:109)
+        sub, used = I.solid_submodel(m, interp.retained, dk.constraints, dk.origin)
+    except D.DeckError as e:
+        block.update(analysis_failed=True, reason=str(e))
+        log(f"{dk.name}: skipping the modal analysis of the optimised topology: {e}")
         return out
     ind = interp.retained.astype(float)
     try:
@@ -142,7 +144,8 @@ def run(dk: D.Deck, output: str | None = None, device: int = 0, precision: str =
                                     objective_tol=dk.run["objective_tol"], window=dk.run["window"],
                                     cont_steps=dk.run["cont_steps"], penalty_start=dk.run["penalty_start"],
                                     cont_iterations=dk.run["cont_iterations"], move=dk.run["move"],
-                                    damping=dk.run["damping"], tol=tol, projection=dk.run["projection"])
+                                    damping=dk.run["damping"], tol=tol, projection=dk.run["projection"],
+                                    warm_start=dk.run.get("warm_start", True))
     timings["optimization"] = time.perf_counter() - t0
     res = dict(g.last_run)
     case_c = g.load_case_compliance()

@@ -160,8 +160,8 @@ def test_deck_with_the_gauss_seidel_smoother_against_the_oracle(tmp_path):
     r = s["optimization_result"]
     assert r["linear_solver"]["multigrid_options"]["smoother"] == "gauss_seidel"
     assert r["linear_solver"]["multigrid"]["levels"] >= 3
-    o = Oracle(d.model, tol=d.tolerance, amg=dict(smoother=1, coarse_size=100))
-    kw = {k: v for k, v in d.run.items() if k != "max_iterations"}
+    o = Oracle(d.model, tol=d.tolerance, warm_start=d.run["warm_start"], amg=dict(smoother=1, coarse_size=100))
+    kw = {k: v for k, v in d.run.items() if k not in ("max_iterations", "warm_start")}
     xo, ho, no, fo = o.run_oc(d.initial_design(), d.run["max_iterations"], **kw)
     rows = list(csv.reader(open(tmp_path / "history.csv")))[1:]
     assert r["iterations"] == no

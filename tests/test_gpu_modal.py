@@ -76,6 +76,42 @@ def test_q4_modes_match_the_oracle():
     check_against_oracle(q4_cantilever(), 5)
 
 
+def test_small_systems_take_the_dense_eigensolve():
+    """At most 400 free DOFs (or a subspace that covers the system): the dense
+    generalised eigensolve of ModalAnalysis.cpp:166-179, no subspace iterations
+    and no linear solves; more modes than free DOFs are clamped with the
+    reference's warning (ModalAnalysis.cpp:153-160)."""
+    m = q4_cantilever(20, 4)  # 200 free DOFs
+    g, r = check_against_oracle(m, 6)
+    assert r["iterations"] == 0 and r["linear_iterations"] == 0 and r["final_relative_change"] == 0.0
+    assert r["warnings"] == []
+    tiny = q4_cantilever(2, 1)  # 8 free DOFs
+    from sparlab_amd import native as N
+    gt = N.HipProblem(tiny, device=0)
+    with pytest.warns(RuntimeWarning, match="requested 10 modes but the model has only 8 free DOFs; returning 8"):
+        rt = gt.modal(RHO, num_modes=10, shapes=True)
+    assert rt["num_modes"] == 8 and rt["iterations"] == 0
+    assert rt["warnings"] == ["requested 10 modes but the model has only 8 free DOFs; returning 8"]
+    ref, M, free = oracle_modes(tiny, 8)
+    assert np.allclose(rt["eigenvalues_1_per_s2"], ref["eigenvalues"], rtol=1e-9, atol=0)
+    pf = rt["mode_shapes"][free]
+    assert np.allclose(pf.T @ (M @ pf), np.eye(8), atol=1e-9)
+
+
+def test_subspace_solves_without_warm_start():
+    """modal warm_start off (linear.warm_start false): every shifted solve starts
+    from zero; the modes are the same, the solves take more CG iterations."""
+    from sparlab_amd import native as N
+    m = q4_cantilever()
+    g = N.HipProblem(m, device=0)
+    warm = g.modal(RHO, num_modes=3)
+    cold = g.modal(RHO, num_modes=3, warm_start=False)
+    print(f"linear iterations warm {warm['linear_iterations']}, cold {cold['linear_iterations']}", flush=True)
+    assert np.allclose(cold["eigenvalues_1_per_s2"], warm["eigenvalues_1_per_s2"], rtol=1e-9, atol=0)
+    assert abs(cold["iterations"] - warm["iterations"]) <= 1
+    assert cold["linear_iterations"] > warm["linear_iterations"]
+
+
 def test_q4_modes_with_element_factors_and_lumped_mass():
     m = q4_cantilever(30, 6)
     rng = np.random.default_rng(20261203)
@@ -122,8 +158,12 @@ def test_modal_errors():
         g.modal(RHO, shift_factor=-0.1)
     with pytest.raises(N.ConfigError):
         g.modal(RHO, num_modes=0)
+    # the subspace iteration holds at most 64 vectors; a small system takes the
+    # dense eigensolve, which has no such limit
+    assert g.modal(RHO, num_modes=33)["num_modes"] == 33
+    big = N.HipProblem(q4_cantilever(), device=0)  # 720 free DOFs
     with pytest.raises(N.ConfigError):
-        g.modal(RHO, num_modes=33)  # a subspace of 66 > 64
+        big.modal(RHO, num_modes=33)  # a subspace of 66 > 64
     with pytest.raises(N.ModelError):
         g.modal(RHO, mass_scale=-np.ones(64))
 

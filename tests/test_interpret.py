@@ -12,6 +12,7 @@ import scipy.linalg as sla
 
 from oracle import modal as OM
 from oracle.refcpu import Oracle
+from sparlab_amd import deck as D
 from sparlab_amd import interpret as I
 from sparlab_amd.model import StructuredModel
 
@@ -82,6 +83,55 @@ def test_masked_grid_operators_equal_the_extracted_submesh(dim):
     b = sla.eigh(Ks.toarray(), Ms.toarray(), eigvals_only=True)[:6]
     assert np.allclose(a, b, rtol=1e-10, atol=0)
     assert OM.total_mass(sub, RHO, ind) == pytest.approx(it.volume_retained * RHO, rel=1e-12)
+
+
+@pytest.mark.parametrize("dim,seed", [(2, 5), (3, 6)])
+def test_submesh_numbering_is_first_appearance(dim, seed):
+    m = (StructuredModel(dim=2, nx=9, ny=7, lx=0.9, ly=0.7) if dim == 2 else
+         StructuredModel(dim=3, nx=5, ny=4, nz=4, lx=0.5, ly=0.4, lz=0.4))
+    keep = field((m.nelem,), seed) > 0.4
+    order, _ = OM.submesh_node_order(m, np.flatnonzero(keep))
+    assert np.array_equal(I.submesh_nodes(m, keep), order)
+
+
+def test_supports_are_reapplied_on_the_extracted_structure():
+    """The deck's boundary conditions evaluated on the retained nodes in the
+    sub-mesh numbering (apps/sparlab_topopt.cpp:109): a box keeps its nodes, a
+    nearest_node support whose parent node was dropped moves to the nearest
+    retained node, node_ids name sub-mesh nodes, and a region that selects no
+    retained node is an error."""
+    m = StructuredModel(dim=2, nx=6, ny=4, lx=6.0, ly=4.0)
+    bcs = [dict(region=D.parse_region({"box": {"xmax": 0.0}}, 2, "a"), comps=[0, 1], name="root"),
+           dict(region=D.parse_region({"nearest_node": [6.0, 0.2]}, 2, "b"), comps=[1], name="roller"),
+           dict(region=D.parse_region({"node_ids": [5]}, 2, "c"), comps=[0], name="pin")]
+    m.fixed = D.apply_constraints(D.Geometry(m), bcs).astype(np.uint8).ravel()
+    keep = np.ones(m.nelem, dtype=bool)
+    keep[5] = False  # the bottom-right element: the only one touching node (6, 0)
+    node = lambda i, j: j * 7 + i  # noqa: E731
+    parent = m.fixed.reshape(-1, 2).astype(bool)
+    assert parent[node(6, 0), 1] and parent[5, 0]  # the parent's roller and pin
+    sub, used = I.solid_submodel(m, keep, bcs)
+    f = sub.fixed.reshape(-1, 2).astype(bool)
+    assert not used[node(6, 0)] and f[node(6, 0)].all()  # dropped node: every DOF fixed
+    # roller: (6, 1) at distance 0.8 beats (5, 0) at 1.02
+    assert f[node(6, 1), 1] and not f[node(6, 1), 0]
+    # pin: sub-mesh node 5 is parent node (2, 1) (element 1's third corner, first seen there)
+    assert I.submesh_nodes(m, keep)[5] == node(2, 1) and f[node(2, 1), 0] and not f[node(2, 1), 1]
+    assert not f[5, 0]  # the parent's node 5 keeps no pin
+    root = [node(0, j) for j in range(5)]
+    assert f[root].all()
+    expected = np.zeros_like(f)
+    expected[root] = True
+    expected[node(6, 1), 1] = expected[node(2, 1), 0] = True
+    expected[~used] = True
+    assert np.array_equal(f, expected)
+    # the parent's own selection is what the call without constraints keeps
+    sub0, _ = I.solid_submodel(m, keep)
+    f0 = sub0.fixed.reshape(-1, 2).astype(bool)
+    assert f0[5, 0] and not f0[node(6, 1), 1]
+    bad = bcs + [dict(region=D.parse_region({"box": {"xmin": 10.0}}, 2, "d"), comps=[0], name="far")]
+    with pytest.raises(D.DeckError, match="boundary condition 'far' selected no nodes"):
+        I.solid_submodel(m, keep, bad)
 
 
 def test_oracle_face_components_known_answers():
