@@ -1390,6 +1390,40 @@ double lanczos_lambda(const std::vector<double>& h, int m) {
   return tridiag_max(a, sub);
 }
 
+// whether R = P^T can be gathered by coarse node: every row of a coarse node
+// has the length of the node's first row, and its entry k reads the value after
+// the one the row before reads
+bool node_gather_ok(const HCsr& r, const std::vector<int>& src, const std::vector<int>& cnode) {
+  if (cnode.size() < 2 || cnode.back() != r.rows) return false;
+  const int nn = (int)cnode.size() - 1;
+  bool ok = true;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+  for (int a = 0; a < nn; ++a) {
+    const int j0 = cnode[(size_t)a], d = cnode[(size_t)a + 1] - j0;
+    const int b0 = r.ptr[(size_t)j0], len = r.ptr[(size_t)j0 + 1] - b0;
+    for (int q = 1; q < d && ok; ++q) {
+      const int bq = r.ptr[(size_t)j0 + q];
+      if (r.ptr[(size_t)j0 + q + 1] - bq != len) {
+        ok = false;
+        break;
+      }
+      for (int k = 0; k < len; ++k)
+        if (src[(size_t)bq + k] != src[(size_t)b0 + k] + q) {
+          ok = false;
+          break;
+        }
+    }
+  }
+  return ok;
+}
+
+void r_transpose(Level& lv, int ncoarse_nodes, hipStream_t s) {
+  if (lv.r_nodes)
+    k::gather_nodes(ncoarse_nodes, lv.r_cnode.p, lv.R.ptr.p, lv.r_src.p, lv.P.val.p, lv.R.val.p, s);
+  else
+    k::gather(lv.R.nnz, lv.r_src.p, lv.P.val.p, lv.R.val.p, s);
+}
+
 // Dependency levels of the symmetric Gauss-Seidel sweeps of one level
 // (Multigrid.cpp:788-806): in the forward sweep row i reads the new values of
 // the rows j < i in its pattern, so its level is one above the highest of
@@ -1640,7 +1674,7 @@ void amg_setup(Context& c, bool defer_coarse) {
   hipStream_t s = c.stream;
   const int n = c.K.rows;
   if (c.k_stale) {  // the last evaluation refreshed level 0 only: assemble its factors now
-    k::assemble_structured(c.dim, c.K, c.d_free.p, c.mf_scale.p, c.d_ke0.p, c.nx, c.ny, c.nz, s);
+    k::assemble_structured(c.dim, c.K, c.d_free.p, c.mf_scale.p, c.d_ke0.p, c.nx, c.ny, c.nz, s, c.d_dofxyz.p);
     c.k_stale = false;
   }
   // coarse_reuse restarts from this setup: the next evaluation sets its base count
@@ -1807,6 +1841,8 @@ void amg_setup(Context& c, bool defer_coarse) {
       lv.P.upload_pattern(ppat, s);
       lv.R.upload_pattern(rpat, s);
       lv.r_src.upload(rsrc, s);
+      lv.r_nodes = node_gather_ok(rpat, rsrc, next.node_ptr);
+      if (lv.r_nodes) lv.r_cnode.upload(next.node_ptr, s);
       lv.AP.upload_pattern(appat, s);
       lv.RAW.upload_pattern(rawpat, s);
       lv.sym_a.upload(ia, s);
@@ -1849,7 +1885,7 @@ void amg_setup(Context& c, bool defer_coarse) {
       g.tval = cur.Ptent.val.p;
       g.row_agg = cur.row_agg.p;
       k::block_product(g, true, s);
-      k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
+      r_transpose(cur, (int)coarse.node_ptr.size() - 1, s);
       lap(3);
       {
         Timed t(c, l == 0 ? "galerkin_ap" : nullptr);
@@ -1861,7 +1897,7 @@ void amg_setup(Context& c, bool defer_coarse) {
       }
     } else {
       k::smooth_prolongator(A, cur.Ptent, cur.P, cur.dinv.p, omega, flag, s);
-      k::gather(cur.R.nnz, cur.r_src.p, cur.P.val.p, cur.R.val.p, s);
+      r_transpose(cur, (int)coarse.node_ptr.size() - 1, s);
       k::spgemm_numeric(A, cur.P, cur.AP, flag, s);
       k::spgemm_numeric(cur.R, cur.AP, cur.RAW, flag, s);
     }

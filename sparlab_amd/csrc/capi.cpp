@@ -130,7 +130,7 @@ void ensure_topopt_buffers(Context& c) {
 void assemble(Context& c, const double* d_scale, bool values = true) {
   if (values) {
     Timed t(c, "assemble");
-    k::assemble_structured(c.dim, c.K, c.d_free.p, d_scale, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream);
+    k::assemble_structured(c.dim, c.K, c.d_free.p, d_scale, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream, c.d_dofxyz.p);
     c.k_stale = false;
   }
   // the same factors drive the matrix-free products of Hex8 models
@@ -589,6 +589,21 @@ slx_status slx_model_finalize(slx_context* ctx) {
       if (!kmf::node_map(c.reduced, map)) fail(SLX_MODEL_ERROR, "the model has more than 2^28 free DOFs");
       c.d_nodemap.upload(map, c.stream);
     }
+    {
+      const long long nnx = c.nx + 1, nny = c.ny + 1;
+      const bool fits = c.dim == 3 ? (nnx <= 1024 && nny <= 1024 && c.nz + 1 <= 1024) : (nnx <= 32768 && nny <= 32768);
+      c.d_dofxyz.release();
+      if (fits) {
+        std::vector<unsigned> xyz(c.free_dofs.size());
+        for (size_t f = 0; f < xyz.size(); ++f) {
+          const long long g = c.free_dofs[f], node = g / c.dim, comp = g % c.dim;
+          const unsigned x = (unsigned)(node % nnx), y = (unsigned)((node / nnx) % nny);
+          xyz[f] = c.dim == 3 ? (x | y << 10 | (unsigned)(node / (nnx * nny)) << 20 | (unsigned)comp << 30)
+                              : (x | y << 15 | (unsigned)comp << 30);
+        }
+        c.d_dofxyz.upload(xyz, c.stream);
+      }
+    }
     c.d_ke0.upload(c.ke0, c.stream);
     c.d_vol.upload(c.volumes, c.stream);
     set_layout_from_model(c);
@@ -677,7 +692,8 @@ slx_status slx_kff_get(const slx_context* ctx, int* ptr, int* col, double* val) 
   return guarded(const_cast<slx_context*>(ctx), [&](Context& c) {
     require_model(c);
     if (c.k_stale) {  // values of the factors the operator uses
-      k::assemble_structured(c.dim, c.K, c.d_free.p, c.mf_scale.p, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream);
+      k::assemble_structured(c.dim, c.K, c.d_free.p, c.mf_scale.p, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream,
+                             c.d_dofxyz.p);
       c.k_stale = false;
     }
     if (ptr) c.K.ptr.download(ptr, (size_t)c.K.rows + 1, c.stream);

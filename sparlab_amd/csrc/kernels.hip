@@ -470,10 +470,28 @@ __device__ __forceinline__ int hex_corner(int lx, int ly, int lz) {
   return lz * 4 + (ly ? (lx ? 2 : 3) : (lx ? 1 : 0));
 }
 
-template <int LANES>
+// A free DOF's grid node and component: from the packed table (x | y << 10 |
+// z << 20 | c << 30, built once per model when the grid fits) or by division.
+template <bool PACKED>
+__device__ __forceinline__ void dof_coords3(int f, const int* __restrict__ free_dofs,
+                                            const unsigned* __restrict__ xyzc, int nnx, int nny, int& x, int& y, int& z,
+                                            int& c) {
+  if (PACKED) {
+    const unsigned p = xyzc[f];
+    x = (int)(p & 1023u), y = (int)((p >> 10) & 1023u), z = (int)((p >> 20) & 1023u), c = (int)(p >> 30);
+  } else {
+    const int g = free_dofs[f];
+    const int I = g / 3;
+    c = g % 3;
+    x = I % nnx, y = (I / nnx) % nny, z = I / (nnx * nny);
+  }
+}
+
+template <int LANES, bool PACKED>
 __global__ void __launch_bounds__(kBlock) k_assemble_hex8(int nrows, const int* __restrict__ ptr,
                                                            const int* __restrict__ col, double* __restrict__ val,
                                                            const int* __restrict__ free_dofs,
+                                                           const unsigned* __restrict__ xyzc,
                                                            const double* __restrict__ scale,
                                                            const double* __restrict__ ke0, int nx, int ny, int nz) {
   __shared__ double ke[576];
@@ -484,13 +502,11 @@ __global__ void __launch_bounds__(kBlock) k_assemble_hex8(int nrows, const int* 
   const int lane = threadIdx.x % LANES;
   if (row >= nrows) return;
   const int nnx = nx + 1, nny = ny + 1;
-  const int g = free_dofs[row];
-  const int I = g / 3, ci = g % 3;
-  const int ix = I % nnx, iy = (I / nnx) % nny, iz = I / (nnx * nny);
+  int ix, iy, iz, ci;
+  dof_coords3<PACKED>(row, free_dofs, xyzc, nnx, nny, ix, iy, iz, ci);
   for (int k = ptr[row] + lane; k < ptr[row + 1]; k += LANES) {
-    const int gj = free_dofs[col[k]];
-    const int J = gj / 3, cj = gj % 3;
-    const int jx = J % nnx, jy = (J / nnx) % nny, jz = J / (nnx * nny);
+    int jx, jy, jz, cj;
+    dof_coords3<PACKED>(col[k], free_dofs, xyzc, nnx, nny, jx, jy, jz, cj);
     const int x0 = max(max(ix, jx) - 1, 0), x1 = min(min(ix, jx), nx - 1);
     const int y0 = max(max(iy, jy) - 1, 0), y1 = min(min(iy, jy), ny - 1);
     const int z0 = max(max(iz, jz) - 1, 0), z1 = min(min(iz, jz), nz - 1);
@@ -507,10 +523,26 @@ __global__ void __launch_bounds__(kBlock) k_assemble_hex8(int nrows, const int* 
   }
 }
 
-template <int LANES>
+// x | y << 15 | c << 30 for Q4
+template <bool PACKED>
+__device__ __forceinline__ void dof_coords2(int f, const int* __restrict__ free_dofs,
+                                            const unsigned* __restrict__ xyzc, int nnx, int& x, int& y, int& c) {
+  if (PACKED) {
+    const unsigned p = xyzc[f];
+    x = (int)(p & 32767u), y = (int)((p >> 15) & 32767u), c = (int)(p >> 30);
+  } else {
+    const int g = free_dofs[f];
+    const int I = g / 2;
+    c = g % 2;
+    x = I % nnx, y = I / nnx;
+  }
+}
+
+template <int LANES, bool PACKED>
 __global__ void __launch_bounds__(kBlock) k_assemble_quad4(int nrows, const int* __restrict__ ptr,
                                                             const int* __restrict__ col, double* __restrict__ val,
                                                             const int* __restrict__ free_dofs,
+                                                            const unsigned* __restrict__ xyzc,
                                                             const double* __restrict__ scale,
                                                             const double* __restrict__ ke0, int nx, int ny) {
   __shared__ double ke[64];
@@ -521,13 +553,11 @@ __global__ void __launch_bounds__(kBlock) k_assemble_quad4(int nrows, const int*
   const int lane = threadIdx.x % LANES;
   if (row >= nrows) return;
   const int nnx = nx + 1;
-  const int g = free_dofs[row];
-  const int I = g / 2, ci = g % 2;
-  const int ix = I % nnx, iy = I / nnx;
+  int ix, iy, ci;
+  dof_coords2<PACKED>(row, free_dofs, xyzc, nnx, ix, iy, ci);
   for (int k = ptr[row] + lane; k < ptr[row + 1]; k += LANES) {
-    const int gj = free_dofs[col[k]];
-    const int J = gj / 2, cj = gj % 2;
-    const int jx = J % nnx, jy = J / nnx;
+    int jx, jy, cj;
+    dof_coords2<PACKED>(col[k], free_dofs, xyzc, nnx, jx, jy, cj);
     const int x0 = max(max(ix, jx) - 1, 0), x1 = min(min(ix, jx), nx - 1);
     const int y0 = max(max(iy, jy) - 1, 0), y1 = min(min(iy, jy), ny - 1);
     double v = 0.0;
@@ -971,14 +1001,24 @@ void axpy(long long n, double a, const double* x, double* y, hipStream_t s) {
 }
 
 void assemble_structured(int dim, const DCsr& K, const int* free_dofs, const double* scale, const double* ke0, int nx,
-                         int ny, int nz, hipStream_t s) {
+                         int ny, int nz, hipStream_t s, const unsigned* xyzc) {
   constexpr int LANES = 16;
   const int nb = grid8(K.rows, kBlock / LANES);
   if (K.rows == 0) return;
-  if (dim == 3)
-    k_assemble_hex8<LANES><<<nb, kBlock, 0, s>>>(K.rows, K.ptr.p, K.col.p, K.val.p, free_dofs, scale, ke0, nx, ny, nz);
-  else
-    k_assemble_quad4<LANES><<<nb, kBlock, 0, s>>>(K.rows, K.ptr.p, K.col.p, K.val.p, free_dofs, scale, ke0, nx, ny);
+  if (dim == 3) {
+    if (xyzc)
+      k_assemble_hex8<LANES, true><<<nb, kBlock, 0, s>>>(K.rows, K.ptr.p, K.col.p, K.val.p, free_dofs, xyzc, scale, ke0,
+                                                         nx, ny, nz);
+    else
+      k_assemble_hex8<LANES, false><<<nb, kBlock, 0, s>>>(K.rows, K.ptr.p, K.col.p, K.val.p, free_dofs, xyzc, scale,
+                                                          ke0, nx, ny, nz);
+  } else if (xyzc) {
+    k_assemble_quad4<LANES, true><<<nb, kBlock, 0, s>>>(K.rows, K.ptr.p, K.col.p, K.val.p, free_dofs, xyzc, scale, ke0,
+                                                        nx, ny);
+  } else {
+    k_assemble_quad4<LANES, false><<<nb, kBlock, 0, s>>>(K.rows, K.ptr.p, K.col.p, K.val.p, free_dofs, xyzc, scale,
+                                                         ke0, nx, ny);
+  }
 }
 
 void sensitivity(int dim, int nelem, int nx, int ny, const double* ke0, const int* reduced, const double* u,

@@ -103,6 +103,22 @@ __global__ void k_gather(long long n, const int* __restrict__ src, const double*
   if (i < n) out[i] = in[src[i]];
 }
 
+// R = P^T by coarse node: the d unknowns of an aggregate are consecutive
+// columns of every P row, so their R rows share one pattern and row J0 + q,
+// entry k, is P's value src[J0's entry k] + q. One block per coarse node reads
+// each run of d values of P once and writes the d rows as coalesced streams.
+__global__ void __launch_bounds__(128) k_gather_nodes(const int* __restrict__ cnode, const int* __restrict__ rptr,
+                                                      const int* __restrict__ src, const double* __restrict__ in,
+                                                      double* __restrict__ out) {
+  const int A = blockIdx.x;
+  const int j0 = cnode[A], d = cnode[A + 1] - j0;
+  const int b0 = rptr[j0], len = rptr[j0 + 1] - b0;
+  for (int k = threadIdx.x; k < len; k += blockDim.x) {
+    const int s0 = src[b0 + k];
+    for (int q = 0; q < d; ++q) out[rptr[j0 + q] + k] = in[s0 + q];
+  }
+}
+
 // (C + C^T)/2 from the raw product (Multigrid.cpp:250-284): ia / ib index the
 // raw values of (i, j) and (j, i), or -1 where absent.
 __global__ void k_symmetrize(long long n, const int* __restrict__ ia, const int* __restrict__ ib,
@@ -828,6 +844,11 @@ void smooth_prolongator(const DCsr& A, const DCsr& Ptent, DCsr& P, const double*
 
 void gather(long long n, const int* src, const double* in, double* out, hipStream_t s) {
   if (n) k_gather<<<nblk(n, 256), 256, 0, s>>>(n, src, in, out);
+}
+
+void gather_nodes(int nnodes, const int* cnode, const int* rptr, const int* src, const double* in, double* out,
+                  hipStream_t s) {
+  if (nnodes > 0) k_gather_nodes<<<nnodes, 128, 0, s>>>(cnode, rptr, src, in, out);
 }
 
 void symmetrize_values(long long n, const int* ia, const int* ib, const double* raw, double* out, hipStream_t s) {

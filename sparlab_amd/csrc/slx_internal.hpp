@@ -160,6 +160,8 @@ struct Level {
                            // apply P = (I - omega D^-1 K) P-hat and R = P-hat^T (I - omega K D^-1)
   double omega = 0.0;      // prolongator damping / lambda_max
   DBuf<int> r_src;         // R.val[k] = P.val[r_src[k]]
+  bool r_nodes = false;    // R's rows come in coarse-node groups reading consecutive P values (r_cnode)
+  DBuf<int> r_cnode;       // the next level's node pointer, for that gather
   DBuf<int> sym_a, sym_b;  // next level's A from RAW (symmetrisation)
   // block-product plans (see kernels_amg.hip); node_ok when every operator of
   // this level has node-uniform rows in complete aggregate blocks
@@ -222,6 +224,9 @@ struct Context {
   // device model state
   DBuf<int> d_free, d_reduced;
   DBuf<int> d_nodemap;  // Hex8: packed per-node free map of the matrix-free kernels
+  // per free DOF its grid node and component, packed for the assembly (x | y << 10 |
+  // z << 20 | c << 30 in 3-D, x | y << 15 | c << 30 in 2-D); empty when the grid does not fit
+  DBuf<unsigned> d_dofxyz;
   DBuf<double> d_ke0, d_scale, d_loads_free, d_weights;  // loads restricted to the free DOFs (ncases x nfree)
   DBuf<double> d_u;                           // last solutions (ncases x nfree)
   std::vector<char> have_u;

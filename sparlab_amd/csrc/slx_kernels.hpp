@@ -112,7 +112,8 @@ void axpy(long long n, double a, const double* x, double* y, hipStream_t s);
 
 // structured-grid assembly of the K_ff values into K's pattern
 void assemble_structured(int dim, const DCsr& K, const int* free_dofs, const double* scale, const double* ke0, int nx,
-                         int ny, int nz, hipStream_t s);
+                         int ny, int nz, hipStream_t s,
+                         const unsigned* xyzc = nullptr);
 // element loop of the sensitivity
 void sensitivity(int dim, int nelem, int nx, int ny, const double* ke0, const int* reduced, const double* u,
                  long long nfree, int ncases, const double* w, const double* fac, const double* dfac, double* energy,
@@ -222,6 +223,10 @@ void spgemm_numeric(const DCsr& A, const DCsr& B, DCsr& C, int* error_flag, hipS
 void smooth_prolongator(const DCsr& A, const DCsr& Ptent, DCsr& P, const double* dinv, double omega, int* error_flag,
                         hipStream_t s);
 void gather(long long n, const int* src, const double* in, double* out, hipStream_t s);
+// R = P^T by coarse node (cnode: the coarse nodes' first rows, nnodes + 1; every
+// row of a node has the pattern of its first and reads src + q)
+void gather_nodes(int nnodes, const int* cnode, const int* rptr, const int* src, const double* in, double* out,
+                  hipStream_t s);
 void symmetrize_values(long long n, const int* ia, const int* ib, const double* raw, double* out, hipStream_t s);
 void inv_diag(const DCsr& A, double* dinv, int* bad_row, hipStream_t s);
 void vsqrt(int n, const double* in, double* out, hipStream_t s);
