@@ -352,7 +352,9 @@ slx_status slx_amg_stats_get(const slx_context* ctx, slx_amg_stats* out);
  * info = rows, cols, nnz (slx_amg_level_get: 0..3 only); 6, 7: the device plans of
  * the products A P and R (A P): info = output nodes, runs, entries (all 0 when the
  * level uses the generic product); 8: the level's operator: nodes, length of the
- * shared column lists (0: none), lanes per row of its CSR products. */
+ * shared column lists (0: none), lanes per row of its CSR products; 9 (level 0): the
+ * nodes per direction of the matrix-free operator's tile the setup timed fastest for
+ * the V-cycle's precision (all 0 when level 0 is assembled). */
 slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long long* info);
 slx_status slx_amg_level_get(const slx_context* ctx, int level, int which, int* ptr, int* col, double* val);
 slx_status slx_amg_aggregates_get(const slx_context* ctx, int level, int* aggregates);

@@ -1943,6 +1943,7 @@ void amg_setup(Context& c, bool defer_coarse) {
   if (o.precision >= 1) prepare_mixed(c);
   if (!reuse || c.tuned_precision != o.precision) {
     autotune(c);
+    tune_tile(c);
     c.tuned_precision = o.precision;
   }
   int galerkin_flag = 0;

@@ -925,6 +925,15 @@ slx_status slx_amg_level_info(const slx_context* ctx, int level, int which, long
       info[2] = a.lanes;
       return;
     }
+    if (which == 9) {  // level 0's matrix-free tile (nodes per direction) for the V-cycle's precision; 0s: assembled
+      info[0] = info[1] = info[2] = 0;
+      if (level == 0 && c.mf) {
+        int d[3];
+        kmf::tile_shape(c.mf_tile[c.amg_opt.precision >= 1 ? 1 : 0], d);
+        info[0] = d[0], info[1] = d[1], info[2] = d[2];
+      }
+      return;
+    }
     if (which == 6 || which == 7) {  // row plans of A P and R (A P): C nodes, runs, entries
       const RowPlan& pl = which == 6 ? lv.plan_ap : lv.plan_raw;
       info[0] = pl.nnodes;

@@ -2,10 +2,11 @@
 //   (K x)_i = sum over the elements e around node i of s_e (Ke0 x_e) restricted to i's rows,
 // the same operator the assembled K_ff holds (K_ff = sum_e s_e Ke0 scattered,
 // src/fem/Assembler.cpp:182-214), applied without reading it. One thread per
-// node; a block stages the input of an 8 x 8 x 4 node tile plus a one-node halo
-// and the factors of the tile's elements in LDS. For each of the 8 elements
-// around a node the node's local corner is fixed by the element's offset, so
-// the Ke0 coefficients a wave needs are uniform across it.
+// node; a block stages the input of a 256-node tile (8 x 8 x 4 by default, or
+// one of the other shapes of Tiles) plus a one-node halo and the factors of the
+// tile's elements in LDS. For each of the 8 elements around a node the node's
+// local corner is fixed by the element's offset, so the Ke0 coefficients a wave
+// needs are uniform across it. A node's sums do not depend on the tile shape.
 //
 // Epilogues fuse what the callers do with the product: y = Kx with the partial
 // sums of x.y (CG), r = b - Kx (residuals), the Chebyshev step, the Lanczos
@@ -22,9 +23,28 @@ namespace slx {
 namespace kmf {
 namespace {
 
-constexpr int TX = 8, TY = 8, TZ = 4, kThreads = TX * TY * TZ;
-constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;  // node halo
-constexpr int EX = TX + 1, EY = TY + 1, EZ = TZ + 1;  // elements touching the tile
+constexpr int kThreads = 256;
+
+// a tile of SX x SY x SZ nodes, one per thread
+template <int SX, int SY, int SZ>
+struct Tile {
+  static_assert(SX * SY * SZ == kThreads, "one node per thread");
+  static constexpr int TX = SX, TY = SY, TZ = SZ;
+  static constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;  // node halo
+  static constexpr int EX = TX + 1, EY = TY + 1, EZ = TZ + 1;  // elements touching the tile
+};
+// the shapes MfArgs::tile selects (kTileShapes of them)
+using Tile0 = Tile<8, 8, 4>;
+using Tile1 = Tile<16, 4, 4>;
+using Tile2 = Tile<32, 4, 2>;
+using Tile3 = Tile<16, 8, 2>;
+static_assert(kTileShapes == 4, "one Tile per shape");
+
+template <typename S>
+int tile_blocks(int X, int Y, int Z) {
+  const int n = ((X + S::TX - 1) / S::TX) * ((Y + S::TY - 1) / S::TY) * ((Z + S::TZ - 1) / S::TZ);
+  return (n + 7) / 8 * 8;
+}
 
 // VTK hexahedron corner of local offsets (StructuredMesh.cpp:224-234) and its inverse
 __device__ __forceinline__ constexpr int corner(int lx, int ly, int lz) {
@@ -108,10 +128,13 @@ __device__ __forceinline__ void store3(T* __restrict__ v, int m, const T (&o)[3]
 // FP32 vector-FMA variants are held to 80 registers (6 waves per SIMD): the step is
 // bound by how many tiles' staging and products overlap, and the few registers
 // that spill cost less than the lost occupancy
-template <typename T, int MODE, bool MX>
+template <typename T, int MODE, bool MX, typename S>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !MX ? 6 : 1)))
 k_mf_hex8(MfArgs<T> a) {
   static_assert(!MX || sizeof(T) == 4, "the matrix-core path is FP32");
+  constexpr int TX = S::TX, TY = S::TY, TZ = S::TZ, HX = S::HX, HY = S::HY, HZ = S::HZ;
+  constexpr int EX = S::EX, EY = S::EY, EZ = S::EZ;
+  static_assert(!MX || EX * EY * EZ <= kElemPad, "the matrix-core path pads the tile's elements to kElemPad");
   using V4 = typename Vec4<T>::type;
   __shared__ V4 s_u[HZ * HY * HX];
   __shared__ T s_s[EZ * EY * EX];
@@ -379,9 +402,20 @@ k_mf_hex8(MfArgs<T> a) {
   }
 }
 
-template <typename T, int MODE, bool MX = false>
+template <typename T, int MODE, bool MX = false, typename S = Tile0>
 void launch(const MfArgs<T>& a, hipStream_t s) {
-  k_mf_hex8<T, MODE, MX><<<blocks(a.X, a.Y, a.Z), kThreads, 0, s>>>(a);
+  k_mf_hex8<T, MODE, MX, S><<<tile_blocks<S>(a.X, a.Y, a.Z), kThreads, 0, s>>>(a);
+}
+
+// modes without block partials, in the shape a.tile names
+template <typename T, int MODE>
+void launch_tiled(const MfArgs<T>& a, hipStream_t s) {
+  switch (a.tile) {
+    case 1: launch<T, MODE, false, Tile1>(a, s); break;
+    case 2: launch<T, MODE, false, Tile2>(a, s); break;
+    case 3: launch<T, MODE, false, Tile3>(a, s); break;
+    default: launch<T, MODE, false, Tile0>(a, s); break;
+  }
 }
 
 // inv_diag of K_ff from the element factors: for each free DOF (node, k), 1 /
@@ -446,9 +480,15 @@ bool node_map(const std::vector<int>& reduced, std::vector<int>& map) {
   return true;
 }
 
-int blocks(int X, int Y, int Z) {
-  const int n = ((X + TX - 1) / TX) * ((Y + TY - 1) / TY) * ((Z + TZ - 1) / TZ);
-  return (n + 7) / 8 * 8;
+int blocks(int X, int Y, int Z) { return tile_blocks<Tile0>(X, Y, Z); }
+
+void tile_shape(int tile, int* dims) {
+  switch (tile) {
+    case 1: dims[0] = Tile1::TX, dims[1] = Tile1::TY, dims[2] = Tile1::TZ; break;
+    case 2: dims[0] = Tile2::TX, dims[1] = Tile2::TY, dims[2] = Tile2::TZ; break;
+    case 3: dims[0] = Tile3::TX, dims[1] = Tile3::TY, dims[2] = Tile3::TZ; break;
+    default: dims[0] = Tile0::TX, dims[1] = Tile0::TY, dims[2] = Tile0::TZ; break;
+  }
 }
 
 // opt-in (SPARLAB_MF_MFMA=1): at 128 x 64 x 64 the matrix-core variant's LDS (51 KB
@@ -476,12 +516,12 @@ void apply(const MfArgs<T>& a, int mode, hipStream_t s) {
   switch (mode) {
     case 0: launch<T, 0>(a, s); break;
     case 1: launch<T, 1>(a, s); break;
-    case 2: launch<T, 2>(a, s); break;
-    case 3: launch<T, 3>(a, s); break;
+    case 2: launch_tiled<T, 2>(a, s); break;
+    case 3: launch_tiled<T, 3>(a, s); break;
     case 4: launch<T, 4>(a, s); break;
-    case 5: launch<T, 5>(a, s); break;
-    case 6: launch<T, 6>(a, s); break;
-    default: launch<T, 7>(a, s); break;
+    case 5: launch_tiled<T, 5>(a, s); break;
+    case 6: launch_tiled<T, 6>(a, s); break;
+    default: launch_tiled<T, 7>(a, s); break;
   }
 }
 

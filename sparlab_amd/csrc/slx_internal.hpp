@@ -279,6 +279,7 @@ struct Context {
   double coarse_pivot_ratio = 0.0;
   double setup_seconds = 0.0;
   int tuned_precision = -1;  // precision the CSR kernels were last autotuned for
+  int mf_tile[2] = {0, 0};   // tile shapes of the FP64 and FP32 matrix-free kernels (kmf::tile_shape)
   double phase_seconds[7] = {0, 0, 0, 0, 0, 0, 0};  // check, spectral, aggregation, smoothing, Galerkin, coarse, tentative
 
   // ---- PCG workspace ------------------------------------------------------------

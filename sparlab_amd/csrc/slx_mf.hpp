@@ -24,10 +24,14 @@ namespace kmf {
 // node's first free DOF << 3) | (bit k set when DOF node * 3 + k is free), 0 for a
 // node with every DOF prescribed (node_map() builds it); scale holds the element
 // factors s_e and ke the 24 x 24 element matrix (node-major DOF order, VTK corner
-// numbering).
+// numbering). tile picks the launch's tile shape (tile_shape()) for modes 2, 3, 5,
+// 6 and 7; modes 0, 1 and 4 and the matrix-core variant take shape 0, so the
+// blocks() partials keep one grouping. Every shape gives the same values bit for bit.
+constexpr int kTileShapes = 4;
 template <typename T>
 struct MfArgs {
   int X, Y, Z;
+  int tile = 0;
   const int* nodemap;
   const T* scale;
   const T* ke;
@@ -45,6 +49,8 @@ struct MfArgs {
 };
 
 int blocks(int X, int Y, int Z);
+// nodes per direction of tile shape t (0: 8 x 8 x 4, 1: 16 x 4 x 4, 2: 32 x 4 x 2, 3: 16 x 8 x 2)
+void tile_shape(int t, int* dims);
 // the packed node map from DOF -> free index (-1 prescribed), 3 DOFs per node;
 // false when a free index does not fit in 28 bits
 bool node_map(const std::vector<int>& reduced, std::vector<int>& map);

@@ -19,6 +19,12 @@ void amg_refresh_level0(Context& c);
 // average milliseconds of one Chebyshev step at a level above the coarsest,
 // over reps back-to-back launches between two events (measurement only)
 double time_smoother(Context& c, int level, int reps);
+// The matrix-free operator's tile shape for the V-cycle's precision, once per
+// structure: level 0's Chebyshev step is timed in each shape (kmf::tile_shape)
+// and the fastest kept. A node's sums run in one order whatever the tile, so the
+// choice changes time only. SPARLAB_MF_TILE=t fixes shape t; SPARLAB_NO_AUTOTUNE=1
+// keeps shape 0.
+void tune_tile(Context& c);
 // FP32 copies of the hierarchy for the mixed-precision V-cycle.
 void prepare_mixed(Context& c);
 // z = M^-1 r: one V-cycle from a zero guess (device pointers, length n of level 0).

@@ -41,6 +41,7 @@ template <>
 kmf::MfArgs<double> mf_base<double>(const Context& c) {
   kmf::MfArgs<double> a{};
   a.X = c.nx + 1, a.Y = c.ny + 1, a.Z = c.nz + 1;
+  a.tile = c.mf_tile[0];
   a.nodemap = c.d_nodemap.p;
   a.scale = c.mf_scale.p;
   a.ke = c.d_ke0.p;
@@ -50,6 +51,7 @@ template <>
 kmf::MfArgs<float> mf_base<float>(const Context& c) {
   kmf::MfArgs<float> a{};
   a.X = c.nx + 1, a.Y = c.ny + 1, a.Z = c.nz + 1;
+  a.tile = c.mf_tile[1];
   a.nodemap = c.d_nodemap.p;
   a.scale = c.mf_scale32.p;
   a.ke = c.mf_ke32.p;
@@ -693,10 +695,9 @@ void vcycle(Context& c, const double* r, double* z) {
   if (zc) k::to_double((long long)c.levels[0].n, zc, z, c.stream);
 }
 
-double time_smoother(Context& c, int level, int reps) {
-  if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: no hierarchy to time");
-  if (level < 0 || level + 1 >= (int)c.levels.size() || reps < 1)
-    fail(SLX_CONFIG_ERROR, "smoother timing: the level must lie above the coarsest");
+namespace {
+
+double smoother_ms(Context& c, int level, int reps) {
   Level& lv = c.levels[(size_t)level];
   DCsr& A = op(c, (size_t)level);
   const bool f32 = c.amg_opt.precision >= 1;
@@ -736,6 +737,39 @@ double time_smoother(Context& c, int level, int reps) {
   c.event_pool.push_back(a);
   c.event_pool.push_back(b);
   return ms / reps;
+}
+
+}  // namespace
+
+double time_smoother(Context& c, int level, int reps) {
+  if (!c.amg_ready) fail(SLX_SOLVER_ERROR, "multigrid: no hierarchy to time");
+  if (level < 0 || level + 1 >= (int)c.levels.size() || reps < 1)
+    fail(SLX_CONFIG_ERROR, "smoother timing: the level must lie above the coarsest");
+  return smoother_ms(c, level, reps);
+}
+
+void tune_tile(Context& c) {
+  c.mf_tile[0] = c.mf_tile[1] = 0;
+  if (!c.mf || c.levels.size() < 2) return;
+  const int slot = c.amg_opt.precision >= 1 ? 1 : 0;
+  const char* f = std::getenv("SPARLAB_MF_TILE");
+  if (f && f[0] >= '0' && f[0] < '0' + kmf::kTileShapes && f[1] == 0) {
+    c.mf_tile[slot] = f[0] - '0';
+    return;
+  }
+  const char* e = std::getenv("SPARLAB_NO_AUTOTUNE");
+  if (e && e[0] == '1') return;
+  float best = std::numeric_limits<float>::max();
+  int pick = 0;
+  for (int t = 0; t < kmf::kTileShapes; ++t) {
+    c.mf_tile[slot] = t;
+    const float ms = (float)smoother_ms(c, 0, 5);
+    if (ms < best) {
+      best = ms;
+      pick = t;
+    }
+  }
+  c.mf_tile[slot] = pick;
 }
 
 bool block_cases_enabled() {

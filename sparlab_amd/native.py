@@ -565,6 +565,14 @@ class HipProblem:
         self.ctx.check(L.slx_amg_level_get(self.ctx.h, level, w, _p(ptr), _p(col), _p(val)))
         return rows, cols, ptr, col, val
 
+    def mf_tile(self):
+        """Nodes per direction of the matrix-free operator's tile that the setup
+        timed fastest, (0, 0, 0) when level 0 is assembled."""
+        self._own()
+        info = np.zeros(3, dtype=np.int64)
+        self.ctx.check(load().slx_amg_level_info(self.ctx.h, 0, 9, _p(info)))
+        return tuple(int(v) for v in info)
+
     def amg_aggregates(self, level, num_nodes):
         self._own()
         a = np.empty(num_nodes, dtype=np.int32)

@@ -505,6 +505,47 @@ def test_matrix_core_level0_products_match_the_vector_path(ctx, monkeypatch):
     assert b @ xm == pytest.approx(b @ xv, rel=1e-10)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_matrix_free_tile_shapes_give_the_same_bits(ctx, precision, monkeypatch):
+    """Every tile shape of the matrix-free kernels (fixed with SPARLAB_MF_TILE)
+    gives the V-cycle and the solve the same bits, on a grid that none of the
+    shapes divides (21 x 10 x 8 nodes): a node's sums run in one order whatever
+    the tile. Without the variable the setup times the shapes and keeps one."""
+    print(f"test_matrix_free_tile_shapes_give_the_same_bits[{precision}]", flush=True)
+    m = M.cantilever_3d(20, 9, 7)
+    scale = 1e-9 + (1 - 1e-9) * M.seeded_density(m.nelem, seed=31) ** 3
+    b = m.free_loads()[0]
+    r = np.random.default_rng(32).standard_normal(len(b))
+    shapes = {0: (8, 8, 4), 1: (16, 4, 4), 2: (32, 4, 2), 3: (16, 8, 2)}
+
+    def run():
+        # a new problem per shape: a setup that reuses the structure keeps its tile
+        g = N.HipProblem(m, ctx=ctx)
+        g.assemble(scale)
+        g.amg_setup(precision=precision, coarse_size=300)
+        assert g.amg_stats().levels >= 2
+        z = g.amg_apply(r)
+        x, it, rr, cv = g.solve(b, tol=1e-10)
+        assert cv
+        return g.mf_tile(), (z, x, it, rr)
+
+    ref = None
+    for t, dims in shapes.items():
+        monkeypatch.setenv("SPARLAB_MF_TILE", str(t))
+        tile, out = run()
+        assert tile == dims
+        if ref is None:
+            ref = out
+        else:
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), f"tile {dims}"
+            assert out[2:] == ref[2:]
+    monkeypatch.delenv("SPARLAB_MF_TILE")
+    tile, out = run()
+    print(f"precision {precision}: the setup picked tile {tile}", flush=True)
+    assert tile in shapes.values()
+    assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])
+
+
 def _three_case_model(kind):
     """Three load cases: the model's own, a second load elsewhere (another
     iteration count) and an unloaded case (zero right-hand side)."""
