@@ -1674,7 +1674,8 @@ void amg_setup(Context& c, bool defer_coarse) {
   hipStream_t s = c.stream;
   const int n = c.K.rows;
   if (c.k_stale) {  // the last evaluation refreshed level 0 only: assemble its factors now
-    k::assemble_structured(c.dim, c.K, c.d_free.p, c.mf_scale.p, c.d_ke0.p, c.nx, c.ny, c.nz, s, c.d_dofxyz.p);
+    k::assemble_structured(c.dim, c.K, c.d_free.p, c.mf_scale.p, c.d_ke0.p, c.nx, c.ny, c.nz, s, c.d_dofxyz.p,
+                           c.d_nodemap.p);
     c.k_stale = false;
   }
   // coarse_reuse restarts from this setup: the next evaluation sets its base count

@@ -130,7 +130,8 @@ void ensure_topopt_buffers(Context& c) {
 void assemble(Context& c, const double* d_scale, bool values = true) {
   if (values) {
     Timed t(c, "assemble");
-    k::assemble_structured(c.dim, c.K, c.d_free.p, d_scale, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream, c.d_dofxyz.p);
+    k::assemble_structured(c.dim, c.K, c.d_free.p, d_scale, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream, c.d_dofxyz.p,
+                           c.d_nodemap.p);
     c.k_stale = false;
   }
   // the same factors drive the matrix-free products of Hex8 models
@@ -693,7 +694,7 @@ slx_status slx_kff_get(const slx_context* ctx, int* ptr, int* col, double* val) 
     require_model(c);
     if (c.k_stale) {  // values of the factors the operator uses
       k::assemble_structured(c.dim, c.K, c.d_free.p, c.mf_scale.p, c.d_ke0.p, c.nx, c.ny, c.nz, c.stream,
-                             c.d_dofxyz.p);
+                             c.d_dofxyz.p, c.d_nodemap.p);
       c.k_stale = false;
     }
     if (ptr) c.K.ptr.download(ptr, (size_t)c.K.rows + 1, c.stream);

@@ -523,6 +523,69 @@ __global__ void __launch_bounds__(kBlock) k_assemble_hex8(int nrows, const int* 
   }
 }
 
+// Node-block assembly of a Hex8 K_ff: 32 lanes per node, lane q < 27 the
+// neighbour at offset (q % 3, q / 3 % 3, q / 9) - 1 (ascending node order). A
+// node's rows share one pattern, the free DOFs of its neighbours in ascending
+// node order, so the lane's block starts at the number of free components of the
+// earlier neighbours in every row of the node. The lane sums the 3 x 3 block over
+// the cells the two nodes share in ascending element order, each entry with the
+// products and sums of k_assemble_hex8 (the same bits), and writes its free
+// entries. It reads no column indices.
+__global__ void __launch_bounds__(kBlock) k_assemble_hex8_nodes(int nnx, int nny, int nnz,
+                                                                 const int* __restrict__ nodemap,
+                                                                 const int* __restrict__ ptr,
+                                                                 const double* __restrict__ scale,
+                                                                 const double* __restrict__ ke0,
+                                                                 double* __restrict__ val) {
+  __shared__ double ke[576];
+  __shared__ int cnt[kBlock];
+  for (int t = threadIdx.x; t < 576; t += blockDim.x) ke[t] = ke0[t];
+  const int nx = nnx - 1, ny = nny - 1, nz = nnz - 1;  // cells per direction
+  const long long nn = (long long)nnx * nny * nnz;
+  const int q = threadIdx.x & 31, g0 = threadIdx.x & ~31;
+  const long long node = (long long)xcd_block(blockIdx.x, gridDim.x) * (kBlock / 32) + (threadIdx.x >> 5);
+  const bool valid = node < nn && q < 27;
+  int ix = 0, iy = 0, iz = 0, jx = 0, jy = 0, jz = 0, mi = 0, mj = 0;
+  if (valid) {
+    ix = (int)(node % nnx), iy = (int)((node / nnx) % nny), iz = (int)(node / ((long long)nnx * nny));
+    jx = ix + q % 3 - 1, jy = iy + (q / 3) % 3 - 1, jz = iz + q / 9 - 1;
+    mi = nodemap[node];
+    if (jx >= 0 && jx < nnx && jy >= 0 && jy < nny && jz >= 0 && jz < nnz)
+      mj = nodemap[((long long)jz * nny + jy) * nnx + jx];
+  }
+  cnt[threadIdx.x] = __popc(mj & 7);
+  __syncthreads();
+  if ((mi & 7) == 0 || (mj & 7) == 0) return;
+  int off = 0;
+  for (int p = 0; p < q; ++p) off += cnt[g0 + p];
+  const int x0 = max(max(ix, jx) - 1, 0), x1 = min(min(ix, jx), nx - 1);
+  const int y0 = max(max(iy, jy) - 1, 0), y1 = min(min(iy, jy), ny - 1);
+  const int z0 = max(max(iz, jz) - 1, 0), z1 = min(min(iz, jz), nz - 1);
+  double v[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int cz = z0; cz <= z1; ++cz)
+    for (int cy = y0; cy <= y1; ++cy)
+      for (int cx = x0; cx <= x1; ++cx) {
+        const double se = scale[((long long)cz * ny + cy) * nx + cx];
+        const int a = hex_corner(jx - cx, jy - cy, jz - cz);
+        const int b = hex_corner(ix - cx, iy - cy, iz - cz);
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+          for (int cj = 0; cj < 3; ++cj) v[ci][cj] += se * ke[(a * 3 + cj) * 24 + b * 3 + ci];
+      }
+  int row = mi >> 3;
+#pragma unroll
+  for (int ci = 0; ci < 3; ++ci) {
+    if (!((mi >> ci) & 1)) continue;
+    double* out = val + ptr[row] + off;
+    int k = 0;
+#pragma unroll
+    for (int cj = 0; cj < 3; ++cj)
+      if ((mj >> cj) & 1) out[k++] = v[ci][cj];
+    ++row;
+  }
+}
+
 // x | y << 15 | c << 30 for Q4
 template <bool PACKED>
 __device__ __forceinline__ void dof_coords2(int f, const int* __restrict__ free_dofs,
@@ -1001,10 +1064,18 @@ void axpy(long long n, double a, const double* x, double* y, hipStream_t s) {
 }
 
 void assemble_structured(int dim, const DCsr& K, const int* free_dofs, const double* scale, const double* ke0, int nx,
-                         int ny, int nz, hipStream_t s, const unsigned* xyzc) {
+                         int ny, int nz, hipStream_t s, const unsigned* xyzc, const int* nodemap) {
   constexpr int LANES = 16;
   const int nb = grid8(K.rows, kBlock / LANES);
   if (K.rows == 0) return;
+  const char* rows_env = std::getenv("SPARLAB_ROW_ASSEMBLY");
+  const bool by_rows = rows_env && rows_env[0] == '1';
+  if (dim == 3 && nodemap && !by_rows) {
+    const long long nn = (long long)(nx + 1) * (ny + 1) * (nz + 1);
+    k_assemble_hex8_nodes<<<grid8(nn, kBlock / 32), kBlock, 0, s>>>(nx + 1, ny + 1, nz + 1, nodemap, K.ptr.p, scale,
+                                                                     ke0, K.val.p);
+    return;
+  }
   if (dim == 3) {
     if (xyzc)
       k_assemble_hex8<LANES, true><<<nb, kBlock, 0, s>>>(K.rows, K.ptr.p, K.col.p, K.val.p, free_dofs, xyzc, scale, ke0,

@@ -110,10 +110,12 @@ void to_double(long long n, const float* in, double* out, hipStream_t s);
 void to_bf16(long long n, const double* in, uint16_t* out, hipStream_t s);
 void axpy(long long n, double a, const double* x, double* y, hipStream_t s);
 
-// structured-grid assembly of the K_ff values into K's pattern
+// structured-grid assembly of the K_ff values into K's pattern; Hex8 with the
+// packed node map (kmf::node_map) goes node block by node block (the same values;
+// SPARLAB_ROW_ASSEMBLY=1 keeps the per-entry kernel)
 void assemble_structured(int dim, const DCsr& K, const int* free_dofs, const double* scale, const double* ke0, int nx,
                          int ny, int nz, hipStream_t s,
-                         const unsigned* xyzc = nullptr);
+                         const unsigned* xyzc = nullptr, const int* nodemap = nullptr);
 // element loop of the sensitivity
 void sensitivity(int dim, int nelem, int nx, int ny, const double* ke0, const int* reduced, const double* u,
                  long long nfree, int ncases, const double* w, const double* fac, const double* dfac, double* energy,

@@ -36,10 +36,24 @@ def _pair(m, ctx, tol=1e-8, **amg):
     return N.HipProblem(m, ctx=ctx), Oracle(m, tol=tol, amg=amg or None)
 
 
-@pytest.mark.parametrize("case", ["hex", "quad"])
-def test_ke0_dofs_pattern_and_assembly_bitwise(ctx, case):
+def _partly_fixed_box():
+    """Hex8 13 x 7 x 5 with nodes of every free-component mask: the clamped
+    face, rollers (z fixed) on the bottom face, y fixed on a side strip and x
+    and z on a few nodes inside."""
+    m = M.cantilever_3d(13, 7, 5)
+    m.fix(m.select_box(zmax=0.0), [2])
+    m.fix(m.select_box(ymax=0.0, xmin=4.0, xmax=9.0), [1])
+    m.fix(m.select_box(xmin=6.0, xmax=7.0, ymin=3.0, ymax=4.0, zmin=2.0, zmax=3.0), [0, 2])
+    return m
+
+
+@pytest.mark.parametrize("case", ["hex", "hex_partly_fixed", "quad"])
+def test_ke0_dofs_pattern_and_assembly_bitwise(ctx, case, monkeypatch):
+    """K_ff against the oracle bit for bit; Hex8 through both device kernels (node
+    blocks, and per entry with SPARLAB_ROW_ASSEMBLY=1)."""
     print(f"test_ke0_dofs_pattern_and_assembly_bitwise[{case}]", flush=True)
-    m = M.cantilever_3d(12, 6, 5) if case == "hex" else M.mbb_beam(30, 15)
+    m = {"hex": lambda: M.cantilever_3d(12, 6, 5), "hex_partly_fixed": _partly_fixed_box,
+         "quad": lambda: M.mbb_beam(30, 15)}[case]()
     g, o = _pair(m, ctx)
     assert np.array_equal(g.ke0(), o.ke0())
     assert np.array_equal(g.free(), o.free())
@@ -49,6 +63,10 @@ def test_ke0_dofs_pattern_and_assembly_bitwise(ctx, case):
     op_, oc, ov = o.assemble_kff(scale)
     assert np.array_equal(gp, op_) and np.array_equal(gc, oc)
     assert np.array_equal(gv, ov)
+    if case != "quad":
+        monkeypatch.setenv("SPARLAB_ROW_ASSEMBLY", "1")
+        g.assemble(scale)
+        assert np.array_equal(g.kff()[2], ov)
 
 
 def test_aggregates_and_hierarchy(ctx):
