@@ -391,7 +391,12 @@ __global__ void __launch_bounds__(256) k_row_product(
 // in device memory, so a level's steps run without host round trips:
 // w_i = s_i (A (s v))_i - beta vprev_i, with s = sqrt(dinv), beta = *beta_ptr (0 if null)
 // ... with the block partials of w.v (rows in a grid-stride loop; partials[block])
+// cbase (may be null): rows of one node share one column list, entry k of row i
+// reads its column at col[cbase[i] + k]. Each lane keeps four entries' loads in
+// flight and adds them in its own order (k, k + 16, ...), as a one-at-a-time loop
+// would.
 __global__ void __launch_bounds__(256) k_lanczos_apply(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                       const int* __restrict__ cbase,
                                                        const double* __restrict__ val, const double* __restrict__ s,
                                                        const double* __restrict__ v, const double* __restrict__ vprev,
                                                        const double* __restrict__ beta_ptr, double* __restrict__ w,
@@ -402,9 +407,21 @@ __global__ void __launch_bounds__(256) k_lanczos_apply(int n, const int* __restr
   const double beta = beta_ptr ? *beta_ptr : 0.0;
   double dot = 0.0;
   for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / LANES; row < n; row += gridDim.x * (blockDim.x / LANES)) {
+    const int* colr = cbase ? col + cbase[row] : col;
+    const int e = ptr[row + 1];
     double acc = 0.0;
-    for (int k = ptr[row] + lane; k < ptr[row + 1]; k += LANES) {
-      const int j = col[k];
+    int k = ptr[row] + lane;
+    for (; k + 3 * LANES < e; k += 4 * LANES) {
+      const int j0 = colr[k], j1 = colr[k + LANES], j2 = colr[k + 2 * LANES], j3 = colr[k + 3 * LANES];
+      const double a0 = val[k], a1 = val[k + LANES], a2 = val[k + 2 * LANES], a3 = val[k + 3 * LANES];
+      const double u0 = s[j0] * v[j0], u1 = s[j1] * v[j1], u2 = s[j2] * v[j2], u3 = s[j3] * v[j3];
+      acc += a0 * u0;
+      acc += a1 * u1;
+      acc += a2 * u2;
+      acc += a3 * u3;
+    }
+    for (; k < e; k += LANES) {
+      const int j = colr[k];
       acc += val[k] * (s[j] * v[j]);
     }
 #pragma unroll
@@ -868,8 +885,8 @@ int lanczos_blocks(int n) { return std::max(1, std::min(1024, nblk((long long)n 
 void lanczos_apply(const DCsr& A, const double* sq, const double* v, const double* vprev, const double* beta,
                    double* w, double* partials, hipStream_t s) {
   if (A.rows)
-    k_lanczos_apply<<<lanczos_blocks(A.rows), 256, 0, s>>>(A.rows, A.ptr.p, A.col.p, A.val.p, sq, v, vprev, beta, w,
-                                                           partials);
+    k_lanczos_apply<<<lanczos_blocks(A.rows), 256, 0, s>>>(A.rows, A.ptr.p, A.cbase.p ? A.ncol.p : A.col.p,
+                                                           A.cbase.p, A.val.p, sq, v, vprev, beta, w, partials);
 }
 
 int axmy_blocks(int n) { return std::max(1, std::min(1024, nblk(n, 256))); }
