@@ -1492,7 +1492,10 @@ void numeric_level(Context& c, size_t l) {
   int* bad = c.setup_flags.p + 1 + l;
   constexpr int kNoBadRow = 0x7F7F7F7F;  // above any row index
   SLX_HIP(hipMemsetAsync(bad, 0x7F, sizeof(int), s));
-  k::inv_diag(A, lv.dinv.p, bad, s);
+  if (l == 0 && c.mf)  // from the element factors: the assembled diagonal bit for bit, K not read
+    kmf::inv_diag(c.nx + 1, c.ny + 1, c.nz + 1, c.d_reduced.p, c.mf_scale.p, c.d_ke0.p, lv.dinv.p, bad, s);
+  else
+    k::inv_diag(A, lv.dinv.p, bad, s);
   const int m = lanczos_queue(c, l);
   std::vector<double> h((size_t)(2 * m + 1));
   lv.lz_scal.download(h.data(), h.size(), s);
@@ -1712,8 +1715,12 @@ void amg_setup(Context& c, bool defer_coarse) {
   c.setup_flags.alloc((size_t)o.max_levels + 2);
   SLX_HIP(hipMemsetAsync(c.setup_flags.p, 0, sizeof(int), s));
   // finiteness of every entry (Multigrid.cpp:992-996): the sum of squares,
-  // checked with level 0's read
-  k::dot(c.K.nnz, c.K.val.p, c.K.val.p, c.partials.p, c.scal.p + 9, s);
+  // checked with level 0's read; a matrix-free model's K = sum_e s_e Ke0 is
+  // finite with its element factors, whose squares are summed instead
+  if (c.mf)
+    k::dot(c.nelem, c.mf_scale.p, c.mf_scale.p, c.partials.p, c.scal.p + 9, s);
+  else
+    k::dot(c.K.nnz, c.K.val.p, c.K.val.p, c.partials.p, c.scal.p + 9, s);
   setup_trace_start();
   auto trace = [&](const char* what, size_t lvl) { setup_trace(what, (int)lvl, s); };
   HCsr host_a;  // the level being aggregated (first setup only): pattern and values
