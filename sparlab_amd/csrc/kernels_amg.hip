@@ -517,17 +517,34 @@ inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 // P-hat is block diagonal over the aggregates: fine row r holds the len_r <= 6
 // consecutive columns c0_r.. of its aggregate. Rows store c0 * 8 + len and the
 // values column-major (ell[k * n + r]), so a thread per row reads coalesced.
+// Two rows per thread, h apart (h = half the rows, rounded up), so that every
+// load instruction stays coalesced; both rows' loads issue before either sum.
 template <typename T>
-__global__ void __launch_bounds__(256) k_tent_prolong(int n, const int* __restrict__ c0len, const T* __restrict__ ell,
-                                                      const T* __restrict__ xc, T* __restrict__ y) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const int e = c0len[r], c0 = e >> 3, len = e & 7;
-  T acc = 0;
+__global__ void __launch_bounds__(256) k_tent_prolong(int n, int h, const int* __restrict__ c0len,
+                                                      const T* __restrict__ ell, const T* __restrict__ xc,
+                                                      T* __restrict__ y) {
+  const int r0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r0 >= h) return;
+  const int r1 = r0 + h;
+  const bool two = r1 < n;
+  const int e0 = c0len[r0], e1 = two ? c0len[r1] : 0;
+  const int c00 = e0 >> 3, len0 = e0 & 7, c01 = e1 >> 3, len1 = e1 & 7;
+  T a0[6], a1[6], x0[6], x1[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k)
-    if (k < len) acc += ell[(size_t)k * n + r] * xc[c0 + k];
-  y[r] = acc;
+  for (int k = 0; k < 6; ++k) {
+    a0[k] = k < len0 ? ell[(size_t)k * n + r0] : T(0);
+    x0[k] = k < len0 ? xc[c00 + k] : T(0);
+    a1[k] = k < len1 ? ell[(size_t)k * n + r1] : T(0);
+    x1[k] = k < len1 ? xc[c01 + k] : T(0);
+  }
+  T acc0 = 0, acc1 = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (k < len0) acc0 += a0[k] * x0[k];
+    if (k < len1) acc1 += a1[k] * x1[k];
+  }
+  y[r0] = acc0;
+  if (two) y[r1] = acc1;
 }
 
 // xc = P-hat^T y: 32 lanes per aggregate over its fine rows, then a reduction
@@ -554,11 +571,28 @@ __global__ void __launch_bounds__(256) k_tent_restrict(int naggs, const int* __r
   if (a >= naggs) return;  // whole 32-lane groups leave together
   const int e = agg_c0len[a], c0 = e >> 3, len = e & 7;
   T acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = agg_ptr[a] + lane; i < agg_ptr[a + 1]; i += 32) {
-    const T yr = y[agg_rows[i]];
+  // four of the lane's rows at a time: their indices, then y and the values
+  // load before any is added; the lane still adds its rows in ascending order
+  const int i1 = agg_ptr[a + 1];
+  for (int i0 = agg_ptr[a] + lane; i0 < i1; i0 += 4 * 32) {
+    int rows[4];
 #pragma unroll
-    for (int k = 0; k < 6; ++k)
-      if (k < len) acc[k] += ella[(size_t)k * n + i] * yr;
+    for (int u = 0; u < 4; ++u) rows[u] = i0 + 32 * u < i1 ? agg_rows[i0 + 32 * u] : 0;
+    T yr[4], ev[4][6];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool in = i0 + 32 * u < i1;
+      yr[u] = in ? y[rows[u]] : T(0);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ev[u][k] = (in && k < len) ? ella[(size_t)k * n + i0 + 32 * u] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (i0 + 32 * u >= i1) break;
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        if (k < len) acc[k] += ev[u][k] * yr[u];
+    }
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
@@ -933,7 +967,8 @@ void dense_inverse(int n, const int* ptr, const int* col, const double* val, dou
 
 template <typename T>
 void tent_prolong(const TentBlocks& t, const T* ell, const T* xc, T* y, hipStream_t s) {
-  if (t.n) k_tent_prolong<T><<<nblk(t.n, 256), 256, 0, s>>>(t.n, t.c0len, ell, xc, y);
+  const int h = (t.n + 1) / 2;
+  if (t.n) k_tent_prolong<T><<<nblk(h, 256), 256, 0, s>>>(t.n, h, t.c0len, ell, xc, y);
 }
 template <typename T>
 void tent_restrict(const TentBlocks& t, const T* ella, const T* y, T* xc, hipStream_t s, const T* dinv, double theta,
