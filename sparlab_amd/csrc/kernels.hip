@@ -359,13 +359,27 @@ __global__ void __launch_bounds__(kBlock) k_precond_partials(int n, const VT* __
   z += o;
   partials += 2LL * blockIdx.y * gridDim.x;
   double s1 = 0.0, s2 = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double zo = z[i];
-    const double zn = zc ? (double)zc[i] : zo;
-    if (zc) z[i] = zn;
-    const double ri = r[i];
-    s1 += ri * zn;
-    s2 += ri * zo;
+  // four of the thread's elements at a time: every load before any sum, the
+  // sums in the thread's ascending order
+  const int stride = gridDim.x * blockDim.x;
+  for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    double zo[4], zn[4], ri[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * stride;
+      const bool in = i < n;
+      zo[u] = in ? z[i] : 0.0;
+      zn[u] = in ? (zc ? (double)zc[i] : zo[u]) : 0.0;
+      ri[u] = in ? r[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * stride;
+      if (i >= n) break;
+      if (zc) z[i] = zn[u];
+      s1 += ri[u] * zn[u];
+      s2 += ri[u] * zo[u];
+    }
   }
   const double t1 = block_sum(s1, sh);
   __syncthreads();
