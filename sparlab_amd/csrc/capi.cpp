@@ -188,6 +188,7 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
   hipStream_t s = c.stream;
   slx_evaluation out{};
   const auto t0 = Clock::now();
+  if (c.eval_dots.n < (size_t)c.ncases + 1) c.eval_dots.alloc((size_t)c.ncases + 1);
   k::spmv_serial(c.dH, c.d_x.p, c.d_tmp.p, s);
   // the projection of the clamped filtered density (Sensitivity.cpp:120-129)
   const bool proj = o.beta > 0.0;
@@ -244,7 +245,9 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
     if (!block && o.warm_start) warm_after_solve(c, l);
     out.linear_iterations += r.iterations;
     most_its = std::max(most_its, r.iterations);
-    comp[(size_t)l] = device_dot(c, (long long)nf, c.d_loads_free.p + (size_t)l * nf, c.d_u.p + (size_t)l * nf);
+    // f_l . u_l, read with the volume at the end (no wait here)
+    k::dot((long long)nf, c.d_loads_free.p + (size_t)l * nf, c.d_u.p + (size_t)l * nf, c.partials.p,
+           c.eval_dots.p + l, s);
   }
   if (lag) {
     ++c.coarse_lag;
@@ -253,13 +256,10 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
     c.coarse_lag = 0;
     c.lag_base_its = c.lag_last_its = most_its;
   }
-  for (int l = 0; l < c.ncases; ++l) out.compliance += c.weights[(size_t)l] * comp[(size_t)l];
-  c.case_compliance = comp;
   if (c.d_weights.n != c.weights.size()) c.d_weights.upload(c.weights, s);
   k::sensitivity(c.dim, (int)ne, c.nx, c.ny, c.d_ke0.p, c.d_reduced.p, c.d_u.p, (long long)nf, c.ncases, c.d_weights.p, c.d_fac.p,
                  c.d_dfac.p, c.d_energy.p, c.d_dc.p, s);
-  out.volume = device_dot(c, ne, c.d_phys.p, c.d_vol.p);
-  out.volume_fraction = out.volume / c.total_volume;
+  k::dot(ne, c.d_phys.p, c.d_vol.p, c.partials.p, c.eval_dots.p + c.ncases, s);
   if (need_gradients) {
     if (proj) {
       // through the projection, then the density filter (Sensitivity.cpp:210-216)
@@ -273,7 +273,15 @@ slx_evaluation evaluate_device(Context& c, const slx_eval_options& o, bool need_
     }
   }
   warm_after_evaluate(c);
+  // the load cases' f.u and the volume: one read
+  double* hb = c.host_buffer((size_t)c.ncases + 1);
+  SLX_HIP(hipMemcpyAsync(hb, c.eval_dots.p, ((size_t)c.ncases + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
   sync(c);
+  for (int l = 0; l < c.ncases; ++l) comp[(size_t)l] = hb[l];
+  out.volume = hb[c.ncases];
+  for (int l = 0; l < c.ncases; ++l) out.compliance += c.weights[(size_t)l] * comp[(size_t)l];
+  c.case_compliance = comp;
+  out.volume_fraction = out.volume / c.total_volume;
   out.solve_seconds = std::chrono::duration<double>(Clock::now() - t1).count();
   if (!std::isfinite(out.compliance)) fail(SLX_SOLVER_ERROR, "the compliance objective evaluated to a non-finite value");
   return out;

@@ -430,6 +430,12 @@ __device__ __forceinline__ double node_mean(const double* ev, int i, int j, int 
 // channels: 0 node density, 1 its change, then dim each: u_prev / s0, r / s1 and the
 // previous displacement change / s2 (zero when dprev is null);
 // weight[v]: mean stiffness factor of the node's elements (the loss weight)
+// out[i] = sqrt(sumsq[i] / n): root-mean-square scales from sums of squares on the device
+__global__ void k_rms(int count, const double* __restrict__ sumsq, double n, double* __restrict__ out) {
+  const int i = threadIdx.x;
+  if (i < count) out[i] = sqrt(sumsq[i] / n);
+}
+
 __global__ void k_features(int dim, int nx, int ny, int nz, const double* __restrict__ phys,
                            const double* __restrict__ phys_prev, const double* __restrict__ fac,
                            const int* __restrict__ reduced, const double* __restrict__ u,
@@ -582,6 +588,10 @@ void features(int dim, int nx, int ny, int nz, const double* phys, const double*
   const long long V = (long long)(nx + 1) * (ny + 1) * (dim == 3 ? nz + 1 : 1);
   k_features<<<blocks_for(V, 256), 256, 0, s>>>(dim, nx, ny, nz, phys, phys_prev, fac, reduced, u, r, dprev, scales,
                                                 C, out, weight);
+}
+
+void rms(int count, const double* sumsq, double n, double* out, hipStream_t s) {
+  if (count > 0) k_rms<<<1, 64, 0, s>>>(count, sumsq, n, out);
 }
 
 void prediction(int dim, long long nodes, int C, const int* reduced, const float* y, double* d, hipStream_t s) {

@@ -289,6 +289,7 @@ struct Context {
   hipGraphExec_t g_update = nullptr, g_precond = nullptr;  // captured CG iteration parts
   bool capturing = false;
   DBuf<double> partials;  // block partial sums
+  DBuf<double> eval_dots;  // an evaluation's f.u per load case and its volume, read together
   DBuf<double> scal;      // device scalars
   std::vector<double> h_scal;
   // several load cases through one operator and hierarchy pass (pcg_block):

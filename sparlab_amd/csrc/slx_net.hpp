@@ -66,6 +66,8 @@ void features(int dim, int nx, int ny, int nz, const double* phys, const double*
               const int* reduced, const double* u, const double* r, const double* dprev, const double* scales, int C,
               uint16_t* out, float* weight, hipStream_t s);
 void prediction(int dim, long long nodes, int C, const int* reduced, const float* y, double* d, hipStream_t s);
+// out[i] = sqrt(sumsq[i] / n) for i < count (count <= 64)
+void rms(int count, const double* sumsq, double n, double* out, hipStream_t s);
 void target(int dim, long long nodes, const int* reduced, const double* delta, const double* scale, float* t,
             hipStream_t s);
 
@@ -135,6 +137,16 @@ struct WarmStart {
   DBuf<double> uold, r, dir, kdir, ksec, x0;
   double gain_net = 0.0, gain_sec = 0.0;  // energy gains of each direction alone (current solve)
   double beta_sec = 0.0;                  // K-optimal step along the previous change (current solve)
+  // the dot products of a solve's prediction and statistics, on the device:
+  // [0] u.u [1] r.r [2] s.s [3] s.r [4] s.Ks [5] d.r [6] d.Kd [7] d.Ks (s the
+  // previous change, d the network's direction), [8] r0.r0 after the start is
+  // formed, [9] [10] the K-norm errors of u_prev and of the start, [11] the sum
+  // of squares of the training target
+  DBuf<double> dots;
+  // statistics of the last solve whose values [8]-[10] are still on the device,
+  // folded in by the next read (the next prediction or a statistics query)
+  bool pending_r0 = false, pending_energy = false;
+  double pending_rr = 0.0, pending_gain_net = 0.0, pending_gain_sec = 0.0;
   DBuf<double> dprev;           // last displacement change per load case (ncases x nfree)
   std::vector<char> have_dprev;
   std::vector<DBuf<uint16_t>> xs;  // replay buffer: packed inputs
@@ -150,5 +162,7 @@ struct WarmStart {
 void warm_before_solve(Context& c, int load_case);
 void warm_after_solve(Context& c, int load_case);
 void warm_after_evaluate(Context& c);
+// folds the statistics still on the device into stats (one read)
+void warm_flush_stats(Context& c);
 
 }  // namespace slx

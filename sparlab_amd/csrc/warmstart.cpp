@@ -5,6 +5,7 @@
 // starts from the point of u_prev + span(d, previous change) closest to the
 // solution in the K-norm (never further than u_prev). CG then runs to the
 // unchanged tolerance, so the result is judged by the same test.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -41,6 +42,8 @@ void ensure_model_net(Context& c, WarmStart& w) {
   for (auto& b : w.ws) b.alloc(V);
   const size_t nf = c.free_dofs.size();
   w.scales.alloc(4);
+  w.dots.alloc(16);
+  w.pending_r0 = w.pending_energy = false;
   w.dprev.alloc(nf * (size_t)c.ncases);
   w.have_dprev.assign((size_t)c.ncases, 0);
   w.uold.alloc(nf);
@@ -79,6 +82,31 @@ void train(Context& c, WarmStart& w) {
   w.stats.train_seconds += seconds_since(t0);
 }
 
+// the solve's dot products [0, count) on the host, in one read
+const double* read_dots(Context& c, WarmStart& w, int count) {
+  double* hb = c.host_buffer(16);
+  SLX_HIP(hipMemcpyAsync(hb, w.dots.p, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SLX_HIP(hipStreamSynchronize(c.stream));
+  return hb;
+}
+
+// folds the statistics of the previous solve (dots [8]-[10]) into stats
+void fold_pending(WarmStart& w, const double* h) {
+  if (w.pending_r0) {
+    w.stats.residual_ratio_sum += w.pending_rr > 0.0 ? std::sqrt(h[8] / w.pending_rr) : 1.0;
+    w.pending_r0 = false;
+  }
+  if (w.pending_energy) {
+    const double e_plain = h[9], e_start = h[10];
+    auto ratio = [&](double e) { return e_plain > 0.0 ? std::sqrt(std::max(e, 0.0) / e_plain) : 1.0; };
+    w.stats.energy_ratio_sum += ratio(e_start);
+    w.stats.last_energy_ratio = ratio(e_start);
+    w.stats.net_ratio_sum += ratio(e_plain - w.pending_gain_net);
+    w.stats.secant_ratio_sum += ratio(e_plain - w.pending_gain_sec);
+    w.pending_energy = false;
+  }
+}
+
 }  // namespace
 
 void warm_before_solve(Context& c, int l) {
@@ -96,25 +124,26 @@ void warm_before_solve(Context& c, int l) {
   const bool predicting = w.count >= w.opt.min_samples;
   w.predicted = false;
   w.beta_sec = 0.0;
-  // the plain warm start's residual and the feature scales
+  double* D = w.dots.p;
+  // the plain warm start's residual and the feature scales (rms of u_prev, r and
+  // the previous change), formed on the device
   op_residual(c, u, b, w.r.p);
-  const double rr = device_dot(c, n, w.r.p, w.r.p);
-  const double uu = device_dot(c, n, u, u);
   const double* dprev = w.have_dprev[(size_t)l] ? w.dprev.p + (size_t)l * nf : nullptr;
-  const double dd = dprev ? device_dot(c, n, dprev, dprev) : 0.0;
-  const double sc[4] = {std::sqrt(uu / (double)n), std::sqrt(rr / (double)n), std::sqrt(dd / (double)n), 0.0};
-  w.scales.upload(sc, 4, s);  // the fourth (the target's scale) is set after the solve
+  k::dot(n, u, u, c.partials.p, D + 0, s);
+  k::dot(n, w.r.p, w.r.p, c.partials.p, D + 1, s);
+  if (dprev)
+    k::dot(n, dprev, dprev, c.partials.p, D + 2, s);
+  else
+    k::fill(1, 0.0, D + 2, s);
+  knet::rms(3, D, (double)n, w.scales.p, s);  // the fourth (the target's scale) is set after the solve
   k::copy(n, u, w.uold.p, s);
   w.uold_valid = true;
   // the previous change s and its K-optimal step
-  double sr = 0.0, sks = 0.0;
   if (dprev) {
     op_apply(c, dprev, w.ksec.p);
-    sr = device_dot(c, n, dprev, w.r.p);
-    sks = device_dot(c, n, dprev, w.ksec.p);
+    k::dot(n, dprev, w.r.p, c.partials.p, D + 3, s);
+    k::dot(n, dprev, w.ksec.p, c.partials.p, D + 4, s);
   }
-  const bool sec_ok = dprev && sks > 0.0 && std::isfinite(sks) && std::isfinite(sr);
-  if (sec_ok) w.beta_sec = sr / sks;
   const size_t V = (size_t)w.net.voxels();
   if (training || predicting) {
     const int slot = training ? w.next : w.opt.buffer;  // weights of a prediction-only solve go to the spare
@@ -127,17 +156,29 @@ void warm_before_solve(Context& c, int l) {
     SLX_HIP(hipMemcpyAsync(w.xs[(size_t)w.pending].p, w.net.input(), V * WarmNet::kIn * sizeof(uint16_t),
                            hipMemcpyDeviceToDevice, s));
   }
-  // x0 = u_prev + a d + b s, (a, b) from the Galerkin condition on span(d, s):
-  // the point of that plane closest to the solution in the K-norm
-  double dr = 0.0, dkd = 0.0, dks = 0.0;
   if (predicting) {
     w.net.forward(s);
     knet::prediction(c.dim, (long long)V, WarmNet::kIn, c.d_reduced.p, w.net.output(), w.dir.p, s);
     op_apply(c, w.dir.p, w.kdir.p);
-    dr = device_dot(c, n, w.dir.p, w.r.p);
-    dkd = device_dot(c, n, w.dir.p, w.kdir.p);
-    if (sec_ok) dks = device_dot(c, n, w.dir.p, w.ksec.p);
+    k::dot(n, w.dir.p, w.r.p, c.partials.p, D + 5, s);
+    k::dot(n, w.dir.p, w.kdir.p, c.partials.p, D + 6, s);
+    if (dprev) k::dot(n, w.dir.p, w.ksec.p, c.partials.p, D + 7, s);
   }
+  // one read: this solve's dots and the previous solve's statistics
+  double h[11];
+  {
+    const double* hb = read_dots(c, w, 11);
+    std::copy(hb, hb + 11, h);
+  }
+  fold_pending(w, h);
+  const double rr = h[1];
+  const double sr = dprev ? h[3] : 0.0, sks = dprev ? h[4] : 0.0;
+  const bool sec_ok = dprev && sks > 0.0 && std::isfinite(sks) && std::isfinite(sr);
+  if (sec_ok) w.beta_sec = sr / sks;
+  // x0 = u_prev + a d + b s, (a, b) from the Galerkin condition on span(d, s):
+  // the point of that plane closest to the solution in the K-norm
+  const double dr = predicting ? h[5] : 0.0, dkd = predicting ? h[6] : 0.0;
+  const double dks = predicting && sec_ok ? h[7] : 0.0;
   const bool net_ok = predicting && dkd > 0.0 && std::isfinite(dkd) && std::isfinite(dr);
   double a = 0.0, bs = 0.0;
   const double det = dkd * sks - dks * dks;
@@ -159,10 +200,12 @@ void warm_before_solve(Context& c, int l) {
       k::axpy(n, -bs, w.ksec.p, w.r.p, s);
     }
     k::copy(n, u, w.x0.p, s);
-    const double r0 = device_dot(c, n, w.r.p, w.r.p);
+    // the start's residual norm, read with the next prediction
+    k::dot(n, w.r.p, w.r.p, c.partials.p, D + 8, s);
+    w.pending_r0 = true;
+    w.pending_rr = rr;
     ++w.stats.predictions;
     w.stats.step_sum += a;
-    w.stats.residual_ratio_sum += rr > 0.0 ? std::sqrt(r0 / rr) : 1.0;
     // energy gains of each direction alone: E(u_prev + t v) = E(u_prev) - (v.r)^2 / v.K v
     w.gain_net = net_ok ? dr * dr / dkd : 0.0;
     w.gain_sec = sec_ok ? sr * sr / sks : 0.0;
@@ -170,7 +213,6 @@ void warm_before_solve(Context& c, int l) {
   } else if (predicting) {
     ++w.stats.rejected;
   }
-  SLX_HIP(hipStreamSynchronize(s));
   w.stats.predict_seconds += seconds_since(t0);
 }
 
@@ -178,44 +220,45 @@ void warm_after_solve(Context& c, int l) {
   WarmStart* wp = c.ws.get();
   if (!wp || !wp->opt.enable) return;
   WarmStart& w = *wp;
+  hipStream_t s = c.stream;
   const size_t nf = c.free_dofs.size();
   const long long n = (long long)nf;
   const double* un = c.d_u.p + (size_t)l * nf;
+  double* D = w.dots.p;
   if (w.predicted) {
-    // energy-norm distance of both starting points from the converged solution
-    auto energy = [&](const double* from) {
-      k::copy(n, un, w.dir.p, c.stream);
-      k::axpy(n, -1.0, from, w.dir.p, c.stream);
+    // energy-norm distance of both starting points from the converged solution,
+    // read with the next prediction
+    auto energy = [&](const double* from, double* out) {
+      k::copy(n, un, w.dir.p, s);
+      k::axpy(n, -1.0, from, w.dir.p, s);
       op_apply(c, w.dir.p, w.kdir.p);
-      return device_dot(c, n, w.dir.p, w.kdir.p);
+      k::dot(n, w.dir.p, w.kdir.p, c.partials.p, out, s);
     };
-    const double e_plain = energy(w.uold.p), e_start = energy(w.x0.p);
-    auto ratio = [&](double e) { return e_plain > 0.0 ? std::sqrt(std::max(e, 0.0) / e_plain) : 1.0; };
-    w.stats.energy_ratio_sum += ratio(e_start);
-    w.stats.last_energy_ratio = ratio(e_start);
-    w.stats.net_ratio_sum += ratio(e_plain - w.gain_net);
-    w.stats.secant_ratio_sum += ratio(e_plain - w.gain_sec);
+    energy(w.uold.p, D + 9);
+    energy(w.x0.p, D + 10);
+    w.pending_energy = true;
+    w.pending_gain_net = w.gain_net;
+    w.pending_gain_sec = w.gain_sec;
     w.predicted = false;
   }
   if (!w.uold_valid) return;
   w.uold_valid = false;
   double* dl = w.dprev.p + (size_t)l * nf;
   // this solve's displacement change (w.dir) ...
-  k::copy(n, un, w.dir.p, c.stream);
-  k::axpy(n, -1.0, w.uold.p, w.dir.p, c.stream);
+  k::copy(n, un, w.dir.p, s);
+  k::axpy(n, -1.0, w.uold.p, w.dir.p, s);
   if (w.pending >= 0) {
     // ... minus what the previous change explains is the target, scale-free (the
-    // Galerkin step fixes the magnitude)
-    k::copy(n, w.dir.p, w.kdir.p, c.stream);
-    if (w.beta_sec != 0.0 && w.have_dprev[(size_t)l]) k::axpy(n, -w.beta_sec, dl, w.kdir.p, c.stream);
-    const double sdu = std::sqrt(device_dot(c, n, w.kdir.p, w.kdir.p) / (double)n);
-    SLX_HIP(hipMemcpyAsync(w.scales.p + 3, &sdu, sizeof(double), hipMemcpyHostToDevice, c.stream));
+    // Galerkin step fixes the magnitude); its rms goes to scales[3] on the device
+    k::copy(n, w.dir.p, w.kdir.p, s);
+    if (w.beta_sec != 0.0 && w.have_dprev[(size_t)l]) k::axpy(n, -w.beta_sec, dl, w.kdir.p, s);
+    k::dot(n, w.kdir.p, w.kdir.p, c.partials.p, D + 11, s);
+    knet::rms(1, D + 11, (double)n, w.scales.p + 3, s);
     const size_t V = (size_t)w.net.voxels();
-    knet::target(c.dim, (long long)V, c.d_reduced.p, w.kdir.p, w.scales.p + 3, w.ts[(size_t)w.pending].p, c.stream);
-    SLX_HIP(hipStreamSynchronize(c.stream));  // sdu lives on the host stack
+    knet::target(c.dim, (long long)V, c.d_reduced.p, w.kdir.p, w.scales.p + 3, w.ts[(size_t)w.pending].p, s);
   }
   // ... is the next solve's previous change
-  k::copy(n, w.dir.p, dl, c.stream);
+  k::copy(n, w.dir.p, dl, s);
   w.have_dprev[(size_t)l] = 1;
   if (w.pending < 0) return;
   w.next = (w.pending + 1) % w.opt.buffer;
@@ -224,6 +267,15 @@ void warm_after_solve(Context& c, int l) {
   ++w.stats.samples;
   ++w.solves;
   train(c, w);
+}
+
+void warm_flush_stats(Context& c) {
+  WarmStart* wp = c.ws.get();
+  if (!wp || !(wp->pending_r0 || wp->pending_energy) || !wp->dots.p) return;
+  double h[11];
+  const double* hb = read_dots(c, *wp, 11);
+  std::copy(hb, hb + 11, h);
+  fold_pending(*wp, h);
 }
 
 void warm_after_evaluate(Context& c) {
@@ -304,10 +356,17 @@ slx_status slx_warmstart_configure(slx_context* ctx, const slx_warmstart_options
 }
 
 slx_status slx_warmstart_stats_get(const slx_context* ctx, slx_warmstart_stats* out) {
-  const Context* c = reinterpret_cast<const Context*>(ctx);
-  if (!c || !out) return SLX_CONFIG_ERROR;
-  *out = c->ws ? c->ws->stats : slx_warmstart_stats{};
-  return SLX_OK;
+  if (!ctx || !out) return SLX_CONFIG_ERROR;
+  Context* c = reinterpret_cast<Context*>(const_cast<slx_context*>(ctx));
+  if (!c->ws) {
+    *out = slx_warmstart_stats{};
+    return SLX_OK;
+  }
+  // the last solve's statistics may still be on the device
+  return net_guarded(const_cast<slx_context*>(ctx), [&](Context& cc, WarmStart& w) {
+    warm_flush_stats(cc);
+    *out = w.stats;
+  });
 }
 
 slx_status slx_net_configure(slx_context* ctx, int X, int Y, int Z, int nin, int nout, unsigned long long seed) {
