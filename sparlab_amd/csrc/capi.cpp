@@ -298,16 +298,13 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o, double target_overrid
   hipStream_t s = c.stream;
   if (!(o.move_limit > 0.0 && o.move_limit <= 1.0)) fail(SLX_CONFIG_ERROR, "optimizer.move_limit must lie in (0, 1]");
   if (!(o.damping > 0.0 && o.damping <= 1.0)) fail(SLX_CONFIG_ERROR, "optimizer.damping must lie in (0, 1]");
-  {
-    // both gradients finite: their sums of squares, read together
-    k::dot(ne, c.d_dcx.p, c.d_dcx.p, c.partials.p, c.scal.p + 11, s);
-    k::dot(ne, c.d_dvx.p, c.d_dvx.p, c.partials.p, c.scal.p + 12, s);
-    double* hb = c.host_buffer(16);
-    SLX_HIP(hipMemcpyAsync(hb + 11, c.scal.p + 11, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    sync(c);
-    if (!std::isfinite(hb[11]) || !std::isfinite(hb[12]))
+  // both gradients finite: their sums of squares (read with the bracket's volumes)
+  k::dot(ne, c.d_dcx.p, c.d_dcx.p, c.partials.p, c.scal.p + 11, s);
+  k::dot(ne, c.d_dvx.p, c.d_dvx.p, c.partials.p, c.scal.p + 12, s);
+  auto check_gradients = [&](const double* h) {
+    if (!std::isfinite(h[0]) || !std::isfinite(h[1]))
       fail(SLX_SOLVER_ERROR, "optimality criteria received a non-finite gradient");
-  }
+  };
   const double target = target_override > 0.0 ? target_override : c.volume_fraction * c.total_volume;
   const double trial_eta = vol_eta >= 0.0 ? vol_eta : c.proj_eta;
   // designs within [0, 1] filter into [0, 1] (rows of H are convex weights), so the
@@ -316,7 +313,7 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o, double target_overrid
   // (the projection makes V nonlinear in x: then each trial filters and projects)
   const bool bounded = c.unit_bounds && c.proj_beta == 0.0;
   const double* ocw = bounded ? htvol(c) : nullptr;
-  if (bounded) c.oc_state.alloc(8);
+  if (bounded) c.oc_state.alloc(16);
   auto volume_at = [&](double lambda) {
     if (bounded) {
       k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit,
@@ -337,7 +334,33 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o, double target_overrid
     return read_scalar(c, c.scal.p + 14);
   };
   double lo = o.lambda_lower, hi = o.lambda_upper;
-  double vlo = volume_at(lo), vhi = volume_at(hi);
+  double vlo, vhi;
+  if (bounded) {
+    // the volumes at both ends of the bracket and the gradient check in one read:
+    // V(lo) moves to state[8] before the second trial
+    k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit,
+                o.damping, ocw, c.oc_state.p, 1, lo, target, o.volume_tolerance, o.max_bisections, c.d_tmp2.p,
+                c.partials.p, s);
+    k::copy(1, c.oc_state.p + 2, c.oc_state.p + 8, s);
+    k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit,
+                o.damping, ocw, c.oc_state.p, 1, hi, target, o.volume_tolerance, o.max_bisections, c.d_tmp2.p,
+                c.partials.p, s);
+    double* hb = c.host_buffer(16);
+    SLX_HIP(hipMemcpyAsync(hb, c.scal.p + 11, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    SLX_HIP(hipMemcpyAsync(hb + 2, c.oc_state.p + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+    SLX_HIP(hipMemcpyAsync(hb + 3, c.oc_state.p + 8, sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    check_gradients(hb);
+    vhi = hb[2];
+    vlo = hb[3];
+  } else {
+    double* hb = c.host_buffer(16);
+    SLX_HIP(hipMemcpyAsync(hb, c.scal.p + 11, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    check_gradients(hb);
+    vlo = volume_at(lo);
+    vhi = volume_at(hi);
+  }
   int widen = 0;
   while (vlo < target && widen < 60) {
     lo *= 1.0e-3;
@@ -377,26 +400,38 @@ slx_oc_step oc_device(Context& c, const slx_oc_options& o, double target_overrid
                 c.partials.p, s);
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (o.max_bisections > 0) {
+      // the first batch as long as the last update's bisection (trials after the
+      // stop return at once), then 8 at a time; each batch ends with the largest
+      // design change of the candidate it leaves, which is the step's once the
+      // bisection has stopped
+      int batch = std::max(8, (c.oc_last_bisections + 7) / 8 * 8);
+      double change = 0.0;
       do {
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < batch; ++k)
           k::oc_trial(ne, c.d_x.p, c.d_dcx.p, c.d_dvx.p, c.d_lower.p, c.d_upper.p, c.d_passive.p, o.move_limit,
                       o.damping, ocw, c.oc_state.p, 0, 0.0, target, o.volume_tolerance, o.max_bisections,
                       c.d_tmp2.p, c.partials.p, s);
+        k::max_abs_diff(ne, c.d_tmp2.p, c.d_x.p, c.partials.p, c.scal.p + 13, s);
         double* hb = c.host_buffer(16);
         SLX_HIP(hipMemcpyAsync(hb, c.oc_state.p, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+        SLX_HIP(hipMemcpyAsync(hb + 8, c.scal.p + 13, sizeof(double), hipMemcpyDeviceToHost, s));
         sync(c);
         std::copy(hb, hb + 8, st);
+        change = hb[8];
+        batch = 8;
       } while (st[5] == 0.0);
+      c.oc_last_bisections = (int)st[4];
+      step.max_change = change;
     } else {
       c.oc_state.download(st, 8, s);
       sync(c);
+      k::max_abs_diff(ne, c.d_tmp2.p, c.d_x.p, c.partials.p, c.scal.p + 13, s);
+      step.max_change = read_scalar(c, c.scal.p + 13);
     }
     step.achieved_volume = st[2];
     step.lambda = st[3];
     step.bisections = (int)st[4];
     step.volume_converged = st[6] != 0.0;
-    k::max_abs_diff(ne, c.d_tmp2.p, c.d_x.p, c.partials.p, c.scal.p + 13, s);
-    step.max_change = read_scalar(c, c.scal.p + 13);
     return step;
   }
   double volume = volume_at(lo);

@@ -290,6 +290,7 @@ struct Context {
   bool capturing = false;
   DBuf<double> partials;  // block partial sums
   DBuf<double> eval_dots;  // an evaluation's f.u per load case and its volume, read together
+  int oc_last_bisections = 0;  // bisection steps of the last OC update (the next one's first batch)
   DBuf<double> scal;      // device scalars
   std::vector<double> h_scal;
   // several load cases through one operator and hierarchy pass (pcg_block):
