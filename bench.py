@@ -298,7 +298,7 @@ def mf_smoother_roofline(g, m, fp32, vcycles):
     in_run = 4 * vcycles
     tile = "x".join(str(v) for v in g.mf_tile())
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": measured_traffic("mf_cheb32" if fp32 else "mf_cheb64", (g.nfree, m.nelem, nnodes)),
+            "traffic": measured_traffic(f"mf_cheb{32 if fp32 else 64}_{tile}", (g.nfree, m.nelem, nnodes)),
             "kernel": f"k_mf_hex8<{'float' if fp32 else 'double'},3> (level-0 Chebyshev step, matrix-free, "
                       f"{tile}-node tiles)",
             "bytes_per_launch": nbytes, "avg_ms": avg, "launches": in_run, "total_ms": avg * in_run,
