@@ -56,7 +56,7 @@ int op_lanczos(Context& c, const double* sq, const double* v, const double* vpre
 // whether level-0 products run matrix-free (SPARLAB_NO_MATRIX_FREE=1 turns it off)
 bool matrix_free_enabled();
 
-// launches recorded into an executable graph (SPARLAB_NO_GRAPHS=1: callers launch eagerly)
+// launches recorded into an executable graph when SPARLAB_GRAPHS=1 (default: eager launches)
 bool graphs_enabled();
 // several load cases of an evaluation through pcg_block (SPARLAB_NO_BLOCK_CASES=1: one by one)
 bool block_cases_enabled();

@@ -70,9 +70,9 @@ def test_segments_match_separate_solves(ctx, batch):
 def test_graphs_and_eager_launches_agree(ctx, batch, monkeypatch):
     print("test_graphs_and_eager_launches_agree", flush=True)
     s = B.solver_for(batch, tol=1e-9, ctx=ctx, coarse_size=200, precision=1)
-    monkeypatch.setenv("SPARLAB_NO_GRAPHS", "1")
+    monkeypatch.delenv("SPARLAB_GRAPHS", raising=False)
     xe, re = s.solve_segmented(batch.seg_ptr, batch.b)
-    monkeypatch.delenv("SPARLAB_NO_GRAPHS")
+    monkeypatch.setenv("SPARLAB_GRAPHS", "1")
     for _ in range(2):
         xg, rg = s.solve_segmented(batch.seg_ptr, batch.b)
         assert [r[0] for r in rg] == [r[0] for r in re]

@@ -378,18 +378,19 @@ def test_matrix_free_and_assembled_solves_agree(ctx, monkeypatch):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_captured_cg_graphs_match_eager_launches(ctx, precision, monkeypatch):
-    """The CG iteration replayed from HIP graphs gives the eager launches' bits,
-    on every replay, with eager work (a V-cycle, a fresh setup) in between."""
+    """The CG iteration replayed from HIP graphs (SPARLAB_GRAPHS=1) gives the
+    eager launches' bits, on every replay, with eager work (a V-cycle, a fresh
+    setup) in between."""
     print(f"test_captured_cg_graphs_match_eager_launches[{precision}]", flush=True)
     m = M.cantilever_3d(16, 8, 8)
     g = N.HipProblem(m, ctx=ctx)
     g.assemble(1e-9 + (1 - 1e-9) * M.seeded_density(m.nelem, seed=11) ** 3)
     b = m.free_loads()[0]
     r = np.random.default_rng(6).standard_normal(g.nfree)
-    monkeypatch.setenv("SPARLAB_NO_GRAPHS", "1")
+    monkeypatch.delenv("SPARLAB_GRAPHS", raising=False)
     g.amg_setup(coarse_size=300, precision=precision)
     x_eager, it_eager, rr_eager, _ = g.solve(b, tol=1e-10)
-    monkeypatch.delenv("SPARLAB_NO_GRAPHS")
+    monkeypatch.setenv("SPARLAB_GRAPHS", "1")
     for replay in range(3):
         if replay == 2:
             g.amg_setup(coarse_size=300, precision=precision)
@@ -627,6 +628,36 @@ def test_block_load_cases_equal_case_by_case_solves(ctx, kind, monkeypatch):
         for a, b in zip(ub, us):
             assert np.array_equal(a, b)
         assert not ub[2].any()  # the unloaded case
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_block_cases_replayed_from_graphs_match_eager_launches(precision, monkeypatch):
+    """With SPARLAB_GRAPHS=1 the block PCG replays its captured iteration; the
+    evaluations give the eager launches' bits (iterations, compliance,
+    sensitivities, every case's displacements), across a new setup per design."""
+    print(f"test_block_cases_replayed_from_graphs_match_eager_launches[{precision}]", flush=True)
+    if precision:
+        monkeypatch.setenv("SPARLAB_NO_AUTOTUNE", "1")
+    m = _three_case_model("hex8")
+    designs = [M.seeded_density(m.nelem, seed=51 + k) for k in range(2)]
+    out = {}
+    for mode in ("eager", "graphs"):
+        if mode == "graphs":
+            monkeypatch.setenv("SPARLAB_GRAPHS", "1")
+        else:
+            monkeypatch.delenv("SPARLAB_GRAPHS", raising=False)
+        c = N.Context(0)
+        g = N.HipProblem(m, ctx=c)
+        if precision:
+            g.assemble()
+            g.amg_setup(precision=precision)
+        out[mode] = [(g.evaluate(x, tol=1e-10), [g.solution(l) for l in range(3)]) for x in designs]
+        c.close()
+    for (ee, ue), (eg, ug) in zip(out["eager"], out["graphs"]):
+        assert ee["iterations"] == eg["iterations"] and ee["compliance"] == eg["compliance"]
+        assert np.array_equal(ee["dc_dx"], eg["dc_dx"])
+        for a, b in zip(ue, ug):
+            assert np.array_equal(a, b)
 
 
 def test_block_load_cases_oc_run_equals_case_by_case(monkeypatch):
