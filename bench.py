@@ -386,8 +386,13 @@ def cases_leg(args, prec, tol, log):
             f"{its} PCG iterations in all, compliance {comp!r}")
         ctx.close()
         os.environ.pop("SPARLAB_NO_BLOCK_CASES", None)
-    assert out["block"]["compliance"] == out["one_at_a_time"]["compliance"], "block and case-by-case differ"
-    assert out["block"]["iterations"] == out["one_at_a_time"]["iterations"]
+    # The two runs use separate contexts, and the mixed path's FP32 V-cycle picks its
+    # CSR lane counts per context by timing, so their summation orders (and the last
+    # bits) can differ; both solve to the same rtol.
+    rel = abs(out["block"]["compliance"] - out["one_at_a_time"]["compliance"]) / abs(out["one_at_a_time"]["compliance"])
+    out["compliance_rel_diff"] = rel
+    assert rel <= 1e-9, f"block and case-by-case compliances differ by {rel:.3g} (relative)"
+    assert abs(out["block"]["iterations"] - out["one_at_a_time"]["iterations"]) <= 3, "iteration counts differ"
     return br, out
 
 
@@ -454,8 +459,8 @@ def designs_leg(args, prec, tol, log):
 
 def batch_main(args):
     """The batched workload: a 240 x 160 two-bolt bracket's three load cases as one
-    block right-hand side, and (with SPARLAB_BATCH_BENCH=1) 32 MBB designs of
-    seeded sizes as one segmented CG."""
+    block right-hand side. The leg of 32 MBB designs as one segmented CG
+    (designs_leg) is switched off."""
     tol = 1e-8
     prec = 1 if args.precision == "mixed" else 0
 
@@ -464,7 +469,13 @@ def batch_main(args):
 
     br, cases = cases_leg(args, prec, tol, log)
     blk, one = cases["block"], cases["one_at_a_time"]
-    designs = designs_leg(args, prec, tol, log) if os.environ.get("SPARLAB_BATCH_BENCH") == "1" else None
+    if os.environ.get("SPARLAB_BATCH_BENCH") == "1":
+        # switched off: a profiled run (rocprofv3 --kernel-trace) of the batch bench with this leg
+        # enabled ended in a GPU memory fault whose cause has not been found (the load-case leg had
+        # run under the profiler without one); the segmented solve stays covered by its tests
+        raise SystemExit("bench: the 32-design segmented leg is switched off (a profiled run of it faulted the "
+                         "GPU, cause not found); the batch bench runs the load-case leg only")
+    designs = None
     solves = 3 + (designs["segments"] if designs else 0)
     ms = blk["solve_ms"] + (designs["pcg_ms"] if designs else 0.0)
     out = {
@@ -481,6 +492,7 @@ def batch_main(args):
                    "cases_block_speedup": one["solve_ms"] / blk["solve_ms"],
                    "cases_block_evaluation_ms": blk["eval_ms"], "cases_one_at_a_time_evaluation_ms": one["eval_ms"],
                    "cases_iterations": blk["iterations"], "cases_compliance": blk["compliance"],
+                   "cases_compliance_rel_diff_one_at_a_time": cases["compliance_rel_diff"],
                    "designs": designs},
     }
     print(json.dumps(out))
